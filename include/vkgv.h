@@ -1,0 +1,262 @@
+/*
+ * vkgv.h - C ABI of libvkgv_hip.so, the MI355X (gfx950) implementation of the
+ * GPU-driven frame of vk_gltf_viewer: meshlet cull + compaction, Hi-Z depth
+ * pyramid, visibility-buffer rasterization and material resolve.
+ *
+ * Everything crossing this boundary is plain C: fixed-layout structs, pointers
+ * and sizes. No C++ types, no exceptions, no HIP types (a stream is a void*).
+ *
+ * The data contract mirrors the reference's dual-compiled GLSL/C++ structs
+ * byte for byte (shaders/mesh_common.h.glsl, shaders/visbuffer/visbuffer.h.glsl),
+ * so that a caller holding the reference's glsl:: arrays can hand them over
+ * unchanged.  Offsets are pinned with static assertions in the library.
+ *
+ * Status codes: 0 = ok, negative = error; vkgv_last_error() returns the text of
+ * the last error raised on a context.  A context and its views are externally
+ * synchronized: one thread per context per call.
+ */
+#ifndef VKGV_H
+#define VKGV_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ------------------------------------------------------------------------ */
+/* Limits (shaders/mesh_common.h.glsl:36-38, shaders/visbuffer/visbuffer.h.glsl:15-17) */
+#define VKGV_MAX_VERTICES            64u   /* maxVertices */
+#define VKGV_MAX_PRIMITIVES          126u  /* maxPrimitives */
+#define VKGV_MESHLET_TRIANGLES       124u  /* alignDown(maxPrimitives, 4): assets.cpp:324 */
+#define VKGV_MAX_MESHLETS_PER_TASK   102u  /* maxMeshlets */
+#define VKGV_TRIANGLE_BITS           7u    /* triangleBits */
+#define VKGV_DRAW_INDEX_BITS         25u   /* drawIndexBits */
+#define VKGV_MAX_DRAWS               (1u << VKGV_DRAW_INDEX_BITS)
+#define VKGV_VISBUFFER_CLEAR         0xFFFFFFFFu /* visbufferClearValue */
+
+/* ------------------------------------------------------------------------ */
+/* Data contract.  Column-major float[16] matrices (glm::mat4 layout).       */
+
+/* glsl::Meshlet, mesh_common.h.glsl:49-58 (36 bytes) */
+typedef struct vkgv_meshlet {
+    uint32_t vertex_offset;    /* into the primitive's vertex-index buffer */
+    uint32_t triangle_offset;  /* byte offset into the primitive's u8 triangle buffer */
+    uint8_t  vertex_count;     /* <= 64 */
+    uint8_t  triangle_count;   /* <= 124 */
+    uint8_t  pad[2];
+    float    aabb_extents[3];  /* @12 */
+    float    aabb_center[3];   /* @24 */
+} vkgv_meshlet;
+
+/* glsl::Vertex, mesh_common.h.glsl:60-71 (24 bytes) */
+typedef struct vkgv_vertex {
+    float    position[3];      /* @0 */
+    uint8_t  color[4];         /* @12, unorm8 RGBA */
+    uint8_t  normal[3];        /* @16, (n + 1) * 127 */
+    uint8_t  pad;
+    uint16_t uv[2];            /* @20, float16 bits */
+} vkgv_vertex;
+
+/* glsl::MeshletDraw, mesh_common.h.glsl:96-100 (12 bytes) */
+typedef struct vkgv_meshlet_draw {
+    uint32_t primitive_index;
+    uint32_t meshlet_index;
+    uint32_t transform_index;
+} vkgv_meshlet_draw;
+
+/* glsl::Primitive, mesh_common.h.glsl:102-113 (64 bytes).  The four buffer
+ * fields are device addresses, exactly as the reference's buffer-device
+ * addresses; the library fills them in vkgv_upload_scene. */
+typedef struct vkgv_primitive {
+    uint64_t vertex_index_buffer;    /* uint32_t[]            */
+    uint64_t primitive_index_buffer; /* uint8_t[] (3 per tri) */
+    uint64_t vertex_buffer;          /* vkgv_vertex[]         */
+    uint64_t meshlet_buffer;         /* vkgv_meshlet[]        */
+    float    aabb_extents[3];        /* @32 */
+    float    aabb_center[3];         /* @44 */
+    uint32_t meshlet_count;          /* @56 */
+    uint32_t material_index;         /* @60, glTF index + 1; 0 = default material */
+} vkgv_primitive;
+
+/* glsl::Material, mesh_common.h.glsl:115-126 (48 bytes) */
+typedef struct vkgv_material {
+    float    albedo_factor[4];       /* linear RGBA */
+    uint32_t albedo_index;           /* @16, texture handle (unused on this path) */
+    float    uv_offset[2];           /* @20 */
+    float    uv_scale[2];            /* @28 */
+    float    uv_rotation;            /* @36 */
+    float    alpha_cutoff;           /* @40 */
+    uint32_t double_sided;           /* @44, GLSL bool (4 bytes) */
+} vkgv_material;
+
+/* glsl::Camera, mesh_common.h.glsl:20-27 (352 bytes) */
+typedef struct vkgv_camera {
+    float prev_view_projection[16];
+    float prev_occlusion_view_projection[16];
+    float view_projection[16];
+    float occlusion_view_projection[16];
+    float frustum[6][4];             /* normalized planes, w negated (camera.cpp:70-84) */
+} vkgv_camera;
+
+/* Normal cone of a meshlet (extension: the reference leaves cone culling off,
+ * assets.cpp:323).  Layout of meshopt_Bounds' cone part: the meshlet is
+ * back-facing for every eye position p with
+ *     dot(normalize(apex - p), axis) >= cutoff      (cutoff >= 1: never). */
+typedef struct vkgv_meshlet_cone {
+    float apex[3];
+    float axis[3];
+    float cutoff;
+    float pad;
+} vkgv_meshlet_cone;                 /* 32 bytes */
+
+/* ------------------------------------------------------------------------ */
+/* Scene upload (assets.cpp:375-429 + world.cpp:10-185 equivalent).          */
+
+/* One set of per-primitive buffers (PrimitiveBuffers, assets.hpp:16-23).  A
+ * mesh may be shared by several primitives (e.g. one icosphere drawn with 256
+ * materials); each primitive's four device addresses then point at it. */
+typedef struct vkgv_mesh_desc {
+    const uint32_t*          vertex_indices;     /* meshlet-local -> vertex */
+    uint64_t                 vertex_index_count;
+    const uint8_t*           triangle_indices;   /* 3 u8 per triangle, 4-byte padded per meshlet */
+    uint64_t                 triangle_index_bytes;
+    const vkgv_vertex*       vertices;
+    uint64_t                 vertex_count;
+    const vkgv_meshlet*      meshlets;
+    uint32_t                 meshlet_count;
+    uint32_t                 pad;
+    const vkgv_meshlet_cone* cones;              /* optional, meshlet_count entries */
+} vkgv_mesh_desc;
+
+typedef struct vkgv_primitive_desc {
+    uint32_t mesh_index;                 /* into the vkgv_mesh_desc array */
+    uint32_t material_index;             /* glTF index + 1; 0 = default */
+    float    aabb_extents[3];
+    float    aabb_center[3];
+} vkgv_primitive_desc;
+
+/* ------------------------------------------------------------------------ */
+/* Frame flags                                                               */
+#define VKGV_CULL_FRUSTUM        (1u << 0)  /* isAabbInFrustum (culling.h.glsl:8-19) */
+#define VKGV_CULL_OCCLUSION_PREV (1u << 1)  /* single pass vs. the previous frame's pyramid (visbuffer.task.glsl:54-65) */
+#define VKGV_CULL_TWO_PASS       (1u << 2)  /* early pass = visible last frame, pyramid, late pass vs. it */
+#define VKGV_CULL_CONE           (1u << 3)  /* normal-cone back-face cull per meshlet */
+#define VKGV_RESOLVE_BARY        (1u << 4)  /* resolve recomputes barycentrics and interpolates vertex colour */
+#define VKGV_BUILD_PYRAMID       (1u << 5)  /* build the pyramid after the resolve, for the next frame */
+#define VKGV_TIMING              (1u << 6)  /* record per-pass HIP events (vkgv_view_read_stats) */
+#define VKGV_NO_BACKFACE         (1u << 7)  /* disable per-triangle back-face culling */
+#define VKGV_SKIP_RESOLVE        (1u << 8)  /* leave the colour image untouched */
+
+/* The reference's own frame: frustum + previous-frame Hi-Z occlusion, pyramid
+ * rebuilt after the resolve (application.cpp:843-1003). */
+#define VKGV_MODE_REFERENCE (VKGV_CULL_FRUSTUM | VKGV_CULL_OCCLUSION_PREV | VKGV_BUILD_PYRAMID)
+
+typedef struct vkgv_frame_desc {
+    vkgv_camera camera;              /* the per-frame camera buffer (camera.cpp:180-193) */
+    float       eye[3];              /* world-space eye position: cone culling only */
+    uint32_t    flags;
+    void*       stream;              /* hipStream_t; NULL = the context's own stream */
+} vkgv_frame_desc;
+
+typedef struct vkgv_frame_stats {
+    uint32_t draws;                  /* meshlet draws submitted */
+    uint32_t visible_early;          /* draws rasterized in the first (or only) pass */
+    uint32_t visible_late;           /* draws rasterized in the late pass (two-pass) */
+    uint32_t big_triangles;          /* triangles sent to the tiled path */
+    uint32_t big_tiles;              /* 16x16 tiles they cover */
+    uint32_t clipped_triangles;      /* triangles that needed near/guard-band clipping */
+    uint32_t overflow;               /* non-zero if a queue overflowed (frame incomplete) */
+    uint32_t pad;
+    /* per-pass GPU milliseconds, filled when the frame ran with VKGV_TIMING */
+    float    ms_clear, ms_cull, ms_raster, ms_hiz, ms_cull_late, ms_raster_late,
+             ms_resolve, ms_pyramid, ms_total;
+    float    pad2;
+} vkgv_frame_stats;
+
+typedef struct vkgv_create_info {
+    int32_t  device;                 /* HIP device ordinal */
+    uint32_t big_triangle_capacity;  /* 0 = default (1M entries) */
+} vkgv_create_info;
+
+typedef struct vkgv_ctx  vkgv_ctx;
+typedef struct vkgv_view vkgv_view;
+
+/* Status codes */
+#define VKGV_OK                       0
+#define VKGV_ERROR_INVALID_ARGUMENT (-1)
+#define VKGV_ERROR_OUT_OF_MEMORY    (-2)
+#define VKGV_ERROR_HIP              (-3)
+#define VKGV_ERROR_LIMIT            (-4)
+#define VKGV_ERROR_STATE            (-5)
+
+/* ------------------------------------------------------------------------ */
+/* Context.  Replaces Device + World's GPU buffers (device.cpp:99-290,
+ * world.cpp:10-185).                                                        */
+int         vkgv_create(const vkgv_create_info* info, vkgv_ctx** out);
+void        vkgv_destroy(vkgv_ctx* ctx);
+const char* vkgv_last_error(const vkgv_ctx* ctx);
+int         vkgv_sync(vkgv_ctx* ctx);      /* waits for the context's stream */
+
+/* Copies meshes, primitives and materials to device memory owned by the
+ * context (assets.cpp:375-429 staging upload, world.cpp:10-185 primitive and
+ * material buffers).  Replaces any previous scene. */
+int vkgv_upload_scene(vkgv_ctx* ctx,
+                      const vkgv_mesh_desc* meshes, uint32_t mesh_count,
+                      const vkgv_primitive_desc* primitives, uint32_t primitive_count,
+                      const vkgv_material* materials, uint32_t material_count);
+
+/* The draw list (World::rebuildDrawBuffer, world.cpp:230-293): at most 2^25
+ * draws, each index checked against the uploaded scene. */
+int vkgv_set_draws(vkgv_ctx* ctx, const vkgv_meshlet_draw* draws, uint32_t count);
+
+/* Per-node mat4 transforms (World::updateTransformBuffer, world.cpp:295-345). */
+int vkgv_set_transforms(vkgv_ctx* ctx, const float* mat4s, uint32_t count);
+
+/* ------------------------------------------------------------------------ */
+/* Views: the per-resolution images of the frame (visbuffer, depth, Hi-Z
+ * pyramid, colour), application.cpp:181-576.                                */
+int  vkgv_view_create(vkgv_ctx* ctx, uint32_t width, uint32_t height, vkgv_view** out);
+void vkgv_view_destroy(vkgv_view* view);
+
+/* Records one frame on the stream: cull + compact, rasterize to the 64-bit
+ * visibility buffer, resolve, Hi-Z (application.cpp:843-1003). */
+int vkgv_render(vkgv_ctx* ctx, vkgv_view* view, const vkgv_frame_desc* frame);
+
+/* Builds the Hi-Z pyramid from the view's current depth
+ * (application.cpp:951-1003 + hiz_reduce.comp.glsl). */
+int vkgv_view_build_pyramid(vkgv_ctx* ctx, vkgv_view* view, void* stream);
+
+/* Forgets the two-pass visibility history (every draw "not visible"). */
+int vkgv_view_reset_history(vkgv_ctx* ctx, vkgv_view* view);
+
+/* Pyramid geometry: levels = floor(log2(max(W,H))) (application.cpp:472-473);
+ * level k is max(1, W >> (k+1)) x max(1, H >> (k+1)).  sizes gets 2*levels
+ * values (w0,h0,w1,h1,...); texels is the total float count. */
+int vkgv_view_pyramid_info(const vkgv_view* view, uint32_t* levels, uint32_t* sizes, uint64_t* texels);
+
+/* Readbacks / uploads (synchronous on the context's stream). */
+int vkgv_view_read_visbuffer(vkgv_ctx* ctx, vkgv_view* view, uint64_t* keys);      /* W*H */
+int vkgv_view_read_ids_depth(vkgv_ctx* ctx, vkgv_view* view, uint32_t* ids, float* depth);
+int vkgv_view_read_color(vkgv_ctx* ctx, vkgv_view* view, uint32_t* rgba8);         /* W*H */
+int vkgv_view_read_pyramid(vkgv_ctx* ctx, vkgv_view* view, float* texels);
+int vkgv_view_write_pyramid(vkgv_ctx* ctx, vkgv_view* view, const float* texels);
+int vkgv_view_write_depth(vkgv_ctx* ctx, vkgv_view* view, const float* depth);     /* ids cleared */
+/* pass 0: first/only pass list, pass 1: late pass list.  Order within a list
+ * is unspecified (compare as sets); *count receives the full length. */
+int vkgv_view_read_visible(vkgv_ctx* ctx, vkgv_view* view, uint32_t pass,
+                           uint32_t* out, uint32_t capacity, uint32_t* count);
+int vkgv_view_read_stats(vkgv_ctx* ctx, vkgv_view* view, vkgv_frame_stats* stats);
+
+/* Visibility-buffer key layout of this implementation: one 64-bit word per
+ * pixel, (~float_bits(depth) << 32) | (0xFFFFFFFE - id), min-reduced with
+ * atomicMin.  Reversed-Z GREATER_OR_EQUAL (larger depth wins, ties go to
+ * the later primitive) becomes a plain unsigned minimum; the clear value is
+ * all ones (depth 0.0, id 0xFFFFFFFF). */
+#define VKGV_KEY_CLEAR 0xFFFFFFFFFFFFFFFFull
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* VKGV_H */

@@ -1,0 +1,183 @@
+"""The oracle: ctypes bindings of oracle/build/libvkgv_oracle.so (cpuref.c).
+
+Test infrastructure only.  Imported by tests/, by the smoke check in
+__graft_entry__.py and by bench.py's CPU-baseline leg, always as the checker
+or the CPU baseline, never as the product path.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB = os.path.join(HERE, "build", "libvkgv_oracle.so")
+
+_lib = None
+
+
+def build():
+    subprocess.run(["make", "-C", HERE], check=True)
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB):
+            build()
+        _lib = C.CDLL(LIB)
+        _setup(_lib)
+    return _lib
+
+
+class RefScene(C.Structure):
+    _fields_ = [("meshes", C.c_void_p), ("mesh_count", C.c_uint32),
+                ("primitives", C.c_void_p), ("primitive_count", C.c_uint32),
+                ("materials", C.c_void_p), ("material_count", C.c_uint32),
+                ("draws", C.c_void_p), ("draw_count", C.c_uint32),
+                ("transforms", C.c_void_p), ("transform_count", C.c_uint32)]
+
+
+def _setup(L):
+    p = C.c_void_p
+    u32 = C.c_uint32
+    f32 = C.c_float
+    sig = {
+        "ref_mat4_mul": (None, [p, p, p]),
+        "ref_perspective_rh_zo": (None, [f32, f32, f32, f32, p]),
+        "ref_ortho_rh_zo": (None, [f32, f32, f32, f32, f32, f32, p]),
+        "ref_look_at_rh": (None, [p, p, p, p]),
+        "ref_reverse_depth": (None, [p, p]),
+        "ref_frustum_planes": (None, [p, p]),
+        "ref_make_camera": (None, [p, p, p]),
+        "ref_aabb_in_frustum": (C.c_int, [p, p, p]),
+        "ref_world_aabb": (None, [p, p, p, p, p]),
+        "ref_project_aabb": (None, [p, p, p, p, p]),
+        "ref_pyramid_levels": (u32, [u32, u32, p, p]),
+        "ref_pyramid_probe": (f32, [p, u32, u32, p, f32]),
+        "ref_floor_log2": (f32, [f32]),
+        "ref_occlusion_visible": (C.c_int, [p, p, p, p, u32, u32]),
+        "ref_cone_culled": (C.c_int, [p, p, p]),
+        "ref_cull": (None, [p, p, p, p, u32, u32, u32, p]),
+        "ref_hiz_build": (None, [p, u32, u32, p]),
+        "ref_keys_clear": (None, [p, C.c_uint64]),
+        "ref_raster": (None, [p, p, p, u32, u32, u32, u32, p]),
+        "ref_raster_triangle": (None, [p, p, p, u32, u32, u32, p]),
+        "ref_keys_depth": (None, [p, C.c_uint64, p]),
+        "ref_keys_ids": (None, [p, C.c_uint64, p]),
+        "ref_from_linear": (f32, [f32]),
+        "ref_resolve": (None, [p, p, p, u32, u32, u32, p]),
+        "ref_frame_reference": (u32, [p, p, p, u32, u32, u32, p, p, p, p]),
+        "ref_frame_two_pass": (None, [p, p, p, u32, u32, u32, p, p, p, p, p, p]),
+    }
+    for name, (res, args) in sig.items():
+        fn = getattr(L, name)
+        fn.restype = res
+        fn.argtypes = args
+
+
+def _ptr(a):
+    return a.ctypes.data if a is not None else None
+
+
+class Scene:
+    """Oracle view of a SceneArrays (keeps the arrays alive)."""
+
+    def __init__(self, sc):
+        self.sc = sc
+        self._mesh_arr, self._keep = sc.mesh_descs()
+        self._prims = np.ascontiguousarray(sc.primitives)
+        self._mats = np.ascontiguousarray(sc.materials)
+        self._draws = np.ascontiguousarray(sc.draws)
+        self._tr = np.ascontiguousarray(sc.transforms, dtype=np.float32)
+        self.s = RefScene(C.cast(self._mesh_arr, C.c_void_p), len(sc.meshes),
+                          self._prims.ctypes.data, len(self._prims), self._mats.ctypes.data, len(self._mats),
+                          self._draws.ctypes.data, len(self._draws), self._tr.ctypes.data, self._tr.shape[0])
+
+    @property
+    def ref(self):
+        return C.byref(self.s)
+
+    @property
+    def draw_count(self):
+        return len(self._draws)
+
+
+def pyramid_info(w, h):
+    sizes = np.zeros(64, dtype=np.uint32)
+    offs = np.zeros(33, dtype=np.uint64)
+    n = lib().ref_pyramid_levels(w, h, sizes.ctypes.data, offs.ctypes.data)
+    return n, [(int(sizes[2 * k]), int(sizes[2 * k + 1])) for k in range(n)], int(offs[n])
+
+
+def hiz_build(depth: np.ndarray) -> np.ndarray:
+    h, w = depth.shape
+    _, _, total = pyramid_info(w, h)
+    out = np.zeros(total, dtype=np.float32)
+    d = np.ascontiguousarray(depth, dtype=np.float32)
+    lib().ref_hiz_build(d.ctypes.data, w, h, out.ctypes.data)
+    return out
+
+
+def cull(scene: Scene, camera, pyramid, w, h, flags, eye=None) -> np.ndarray:
+    vis = np.zeros(scene.draw_count, dtype=np.uint8)
+    pyr = np.ascontiguousarray(pyramid, dtype=np.float32)
+    e = np.asarray(eye if eye is not None else (0, 0, 0), dtype=np.float32)
+    lib().ref_cull(scene.ref, C.byref(camera), e.ctypes.data, pyr.ctypes.data, w, h, flags, vis.ctypes.data)
+    return np.nonzero(vis)[0].astype(np.uint32)
+
+
+def raster(scene: Scene, vp, draw_list, w, h, flags=0, keys=None) -> np.ndarray:
+    if keys is None:
+        keys = np.full(w * h, 0xFFFFFFFFFFFFFFFF, dtype=np.uint64)
+    lst = np.ascontiguousarray(draw_list, dtype=np.uint32)
+    vpa = np.ascontiguousarray(np.asarray(vp, dtype=np.float32).ravel())
+    lib().ref_raster(scene.ref, vpa.ctypes.data, lst.ctypes.data, lst.size, w, h, flags, keys.ctypes.data)
+    return keys.reshape(h, w)
+
+
+def resolve(scene: Scene, vp, keys, flags=0) -> np.ndarray:
+    h, w = keys.shape
+    out = np.zeros(w * h, dtype=np.uint32)
+    vpa = np.ascontiguousarray(np.asarray(vp, dtype=np.float32).ravel())
+    k = np.ascontiguousarray(keys, dtype=np.uint64)
+    lib().ref_resolve(scene.ref, vpa.ctypes.data, k.ctypes.data, w, h, flags, out.ctypes.data)
+    return out.reshape(h, w)
+
+
+def keys_depth(keys):
+    k = np.ascontiguousarray(keys, dtype=np.uint64)
+    return (~(k >> np.uint64(32)).astype(np.uint32)).view(np.float32)
+
+
+def keys_ids(keys):
+    k = np.ascontiguousarray(keys, dtype=np.uint64)
+    return (np.uint32(0xFFFFFFFE) - (k & np.uint64(0xFFFFFFFF)).astype(np.uint32)).astype(np.uint32)
+
+
+def frame_reference(scene: Scene, camera, w, h, flags, pyramid, eye=None):
+    """Reference frame; `pyramid` (float array) is read then overwritten."""
+    keys = np.empty(w * h, dtype=np.uint64)
+    rgba = np.zeros(w * h, dtype=np.uint32)
+    vis = np.zeros(scene.draw_count, dtype=np.uint8)
+    e = np.asarray(eye if eye is not None else (0, 0, 0), dtype=np.float32)
+    lib().ref_frame_reference(scene.ref, C.byref(camera), e.ctypes.data, w, h, flags, pyramid.ctypes.data,
+                              keys.ctypes.data, rgba.ctypes.data, vis.ctypes.data)
+    return keys.reshape(h, w), rgba.reshape(h, w), np.nonzero(vis)[0].astype(np.uint32)
+
+
+def frame_two_pass(scene: Scene, camera, w, h, flags, history, eye=None):
+    """Two-pass frame; `history` (u8 per draw) is read and updated."""
+    _, _, total = pyramid_info(w, h)
+    pyr = np.zeros(total, dtype=np.float32)
+    keys = np.empty(w * h, dtype=np.uint64)
+    rgba = np.zeros(w * h, dtype=np.uint32)
+    ve = np.zeros(scene.draw_count, dtype=np.uint8)
+    vl = np.zeros(scene.draw_count, dtype=np.uint8)
+    e = np.asarray(eye if eye is not None else (0, 0, 0), dtype=np.float32)
+    lib().ref_frame_two_pass(scene.ref, C.byref(camera), e.ctypes.data, w, h, flags, history.ctypes.data,
+                             pyr.ctypes.data, keys.ctypes.data, rgba.ctypes.data, ve.ctypes.data, vl.ctypes.data)
+    return (keys.reshape(h, w), rgba.reshape(h, w), pyr,
+            np.nonzero(ve)[0].astype(np.uint32), np.nonzero(vl)[0].astype(np.uint32))

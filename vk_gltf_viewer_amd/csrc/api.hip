@@ -1,0 +1,658 @@
+// C ABI of libvkgv_hip.so (include/vkgv.h): context, scene upload, views,
+// frame orchestration and readbacks.
+//
+// The frame (application.cpp:843-1003) becomes a sequence of stream-ordered
+// launches with no host synchronization:
+//   single pass:  reset, clear, cull (vs. previous pyramid), raster, tiled
+//                 raster, resolve, [pyramid for the next frame]
+//   two pass:     reset, clear, cull early (visible last frame), raster,
+//                 pyramid, cull late (vs. that pyramid), raster, resolve
+// Every index that a kernel dereferences is validated here on upload, so the
+// kernels need no bounds checks on scene data.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "internal.hpp"
+
+using namespace vkgv;
+
+namespace vkgv {
+uint32_t hiz_tile_levels(const PyramidDesc& pd);
+}
+
+namespace {
+
+struct DevBuf {
+    void* p = nullptr;
+    size_t bytes = 0;
+};
+
+enum Ev { EV_START, EV_CLEAR, EV_CULL, EV_RASTER, EV_HIZ, EV_CULL_LATE, EV_RASTER_LATE, EV_RESOLVE, EV_PYRAMID, EV_COUNT };
+
+} // namespace
+
+struct vkgv_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    std::string err;
+    int cus = 256;
+    // scene
+    std::vector<DevBuf> mesh_bufs;
+    std::vector<uint32_t> mesh_meshlets;
+    DevBuf prims, materials, cone_base, cones;
+    uint32_t prim_count = 0, material_count = 0;
+    std::vector<uint32_t> prim_meshlets;     // meshlet count per primitive (host copy for validation)
+    bool scene = false;
+    // frame inputs
+    DevBuf draws, cull, transforms;
+    uint32_t draw_count = 0, transform_count = 0;
+    uint32_t max_transform = 0;
+    bool draws_set = false;
+    RasterTuning tuning{ 32u, 3u, 2048u };
+    uint32_t big_capacity = 1u << 20;
+};
+
+struct vkgv_view {
+    vkgv_ctx* ctx = nullptr;
+    ViewDev dev{};
+    DevBuf keys, color, pyramid, visible0, visible1, history, counters, big, big_start;
+    uint32_t list_capacity = 0;
+    hipEvent_t ev[EV_COUNT] = {};
+    bool timed = false;
+    uint32_t last_flags = 0;
+    hipStream_t last_stream = nullptr;
+};
+
+namespace {
+
+int fail(vkgv_ctx* c, int code, const std::string& msg)
+{
+    if (c) c->err = msg;
+    return code;
+}
+
+#define HIP_TRY(ctx, expr)                                                                         \
+    do {                                                                                           \
+        hipError_t e_ = (expr);                                                                    \
+        if (e_ != hipSuccess)                                                                      \
+            return fail((ctx), VKGV_ERROR_HIP, std::string(#expr ": ") + hipGetErrorString(e_));   \
+    } while (0)
+
+void dfree(DevBuf& b)
+{
+    if (b.p) (void)hipFree(b.p);
+    b.p = nullptr;
+    b.bytes = 0;
+}
+
+int dalloc(vkgv_ctx* c, DevBuf& b, size_t bytes)
+{
+    dfree(b);
+    if (bytes == 0) bytes = 16;
+    hipError_t e = hipMalloc(&b.p, bytes);
+    if (e != hipSuccess) {
+        b.p = nullptr;
+        return fail(c, e == hipErrorOutOfMemory ? VKGV_ERROR_OUT_OF_MEMORY : VKGV_ERROR_HIP,
+                    std::string("hipMalloc: ") + hipGetErrorString(e));
+    }
+    b.bytes = bytes;
+    return VKGV_OK;
+}
+
+int upload(vkgv_ctx* c, DevBuf& b, const void* src, size_t bytes)
+{
+    int rc = dalloc(c, b, bytes);
+    if (rc) return rc;
+    if (bytes) HIP_TRY(c, hipMemcpyAsync(b.p, src, bytes, hipMemcpyHostToDevice, c->stream));
+    return VKGV_OK;
+}
+
+SceneDev scene_dev(const vkgv_ctx* c)
+{
+    SceneDev s{};
+    s.draws = static_cast<const vkgv_meshlet_draw*>(c->draws.p);
+    s.draw_count = c->draw_count;
+    s.transforms = static_cast<const float*>(c->transforms.p);
+    s.transform_count = c->transform_count;
+    s.prims = static_cast<const vkgv_primitive*>(c->prims.p);
+    s.prim_count = c->prim_count;
+    s.materials = static_cast<const vkgv_material*>(c->materials.p);
+    s.material_count = c->material_count;
+    s.prim_cone_base = static_cast<const uint32_t*>(c->cone_base.p);
+    s.cones = static_cast<const vkgv_meshlet_cone*>(c->cones.p);
+    s.cull = static_cast<const CullRec*>(c->cull.p);
+    return s;
+}
+
+Mat4 to_mat4(const float* m)
+{
+    Mat4 r;
+    std::memcpy(r.m, m, sizeof r.m);
+    return r;
+}
+
+int ensure_lists(vkgv_ctx* c, vkgv_view* v)
+{
+    if (v->list_capacity >= c->draw_count && v->visible0.p) return VKGV_OK;
+    const uint32_t cap = std::max<uint32_t>(c->draw_count, 64u);
+    int rc;
+    if ((rc = dalloc(c, v->visible0, sizeof(uint32_t) * cap))) return rc;
+    if ((rc = dalloc(c, v->visible1, sizeof(uint32_t) * cap))) return rc;
+    const size_t hwords = (cap + 63u) / 64u;
+    if ((rc = dalloc(c, v->history, sizeof(uint64_t) * hwords))) return rc;
+    HIP_TRY(c, hipMemsetAsync(v->history.p, 0, v->history.bytes, c->stream));
+    v->list_capacity = cap;
+    v->dev.visible[0] = static_cast<uint32_t*>(v->visible0.p);
+    v->dev.visible[1] = static_cast<uint32_t*>(v->visible1.p);
+    v->dev.history = static_cast<uint32_t*>(v->history.p);
+    return VKGV_OK;
+}
+
+int check_view(vkgv_ctx* c, vkgv_view* v)
+{
+    if (!c) return VKGV_ERROR_INVALID_ARGUMENT;
+    if (!v || v->ctx != c) return fail(c, VKGV_ERROR_INVALID_ARGUMENT, "view does not belong to this context");
+    return VKGV_OK;
+}
+
+int sync_view(vkgv_ctx* c, vkgv_view* v)
+{
+    if (v->last_stream && v->last_stream != c->stream) HIP_TRY(c, hipStreamSynchronize(v->last_stream));
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    return VKGV_OK;
+}
+
+} // namespace
+
+// ---------------------------------------------------------------------------
+extern "C" {
+
+int vkgv_create(const vkgv_create_info* info, vkgv_ctx** out)
+{
+    if (!out) return VKGV_ERROR_INVALID_ARGUMENT;
+    *out = nullptr;
+    auto* c = new vkgv_ctx();
+    c->device = info ? info->device : 0;
+    if (info && info->big_triangle_capacity) c->big_capacity = info->big_triangle_capacity;
+    if (c->big_capacity > (1u << 23)) { delete c; return VKGV_ERROR_LIMIT; }
+    int count = 0;
+    if (hipGetDeviceCount(&count) != hipSuccess || count <= c->device || c->device < 0) {
+        delete c;
+        return VKGV_ERROR_HIP;
+    }
+    if (hipSetDevice(c->device) != hipSuccess) { delete c; return VKGV_ERROR_HIP; }
+    hipDeviceProp_t prop;
+    if (hipGetDeviceProperties(&prop, c->device) == hipSuccess && prop.multiProcessorCount > 0)
+        c->cus = prop.multiProcessorCount;
+    if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) { delete c; return VKGV_ERROR_HIP; }
+    *out = c;
+    return VKGV_OK;
+}
+
+void vkgv_destroy(vkgv_ctx* c)
+{
+    if (!c) return;
+    (void)hipSetDevice(c->device);
+    (void)hipStreamSynchronize(c->stream);
+    for (auto& b : c->mesh_bufs) dfree(b);
+    dfree(c->prims); dfree(c->materials); dfree(c->cone_base); dfree(c->cones);
+    dfree(c->draws); dfree(c->cull); dfree(c->transforms);
+    (void)hipStreamDestroy(c->stream);
+    delete c;
+}
+
+const char* vkgv_last_error(const vkgv_ctx* c) { return c ? c->err.c_str() : "null context"; }
+
+int vkgv_sync(vkgv_ctx* c)
+{
+    if (!c) return VKGV_ERROR_INVALID_ARGUMENT;
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    return VKGV_OK;
+}
+
+int vkgv_upload_scene(vkgv_ctx* c, const vkgv_mesh_desc* meshes, uint32_t mesh_count,
+                      const vkgv_primitive_desc* prims, uint32_t prim_count, const vkgv_material* materials,
+                      uint32_t material_count)
+{
+    if (!c) return VKGV_ERROR_INVALID_ARGUMENT;
+    if ((mesh_count && !meshes) || (prim_count && !prims) || !materials || material_count == 0)
+        return fail(c, VKGV_ERROR_INVALID_ARGUMENT, "upload_scene: null array or no materials");
+    HIP_TRY(c, hipSetDevice(c->device));
+    // ---- validation: everything a kernel dereferences stays in bounds
+    for (uint32_t mi = 0; mi < mesh_count; ++mi) {
+        const vkgv_mesh_desc& m = meshes[mi];
+        if ((m.meshlet_count && !m.meshlets) || (m.vertex_index_count && !m.vertex_indices)
+            || (m.triangle_index_bytes && !m.triangle_indices) || (m.vertex_count && !m.vertices))
+            return fail(c, VKGV_ERROR_INVALID_ARGUMENT, "upload_scene: mesh " + std::to_string(mi) + " has a null array");
+        for (uint64_t k = 0; k < m.vertex_index_count; ++k)
+            if (m.vertex_indices[k] >= m.vertex_count)
+                return fail(c, VKGV_ERROR_INVALID_ARGUMENT, "upload_scene: vertex index out of range in mesh " + std::to_string(mi));
+        for (uint32_t k = 0; k < m.meshlet_count; ++k) {
+            const vkgv_meshlet& ml = m.meshlets[k];
+            const uint64_t tbytes = ((uint64_t)ml.triangle_count * 3u + 3u) & ~3ull;
+            if (ml.vertex_count == 0 || ml.vertex_count > VKGV_MAX_VERTICES || ml.triangle_count == 0
+                || ml.triangle_count > VKGV_MAX_PRIMITIVES || (ml.triangle_offset & 3u)
+                || (uint64_t)ml.vertex_offset + ml.vertex_count > m.vertex_index_count
+                || (uint64_t)ml.triangle_offset + tbytes > m.triangle_index_bytes)
+                return fail(c, VKGV_ERROR_INVALID_ARGUMENT,
+                            "upload_scene: meshlet " + std::to_string(k) + " of mesh " + std::to_string(mi) + " is malformed");
+            for (uint32_t t = 0; t < 3u * ml.triangle_count; ++t)
+                if (m.triangle_indices[ml.triangle_offset + t] >= ml.vertex_count)
+                    return fail(c, VKGV_ERROR_INVALID_ARGUMENT,
+                                "upload_scene: triangle index out of range in meshlet " + std::to_string(k) + " of mesh "
+                                    + std::to_string(mi));
+        }
+    }
+    for (uint32_t p = 0; p < prim_count; ++p)
+        if (prims[p].mesh_index >= mesh_count || prims[p].material_index >= material_count)
+            return fail(c, VKGV_ERROR_INVALID_ARGUMENT, "upload_scene: primitive " + std::to_string(p) + " out of range");
+
+    // ---- device copies (assets.cpp:375-429 staging upload)
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    for (auto& b : c->mesh_bufs) dfree(b);
+    c->mesh_bufs.assign((size_t)mesh_count * 4, DevBuf());
+    c->mesh_meshlets.assign(mesh_count, 0);
+    std::vector<uint32_t> cone_first(mesh_count, 0xFFFFFFFFu);
+    std::vector<vkgv_meshlet_cone> all_cones;
+    int rc;
+    for (uint32_t mi = 0; mi < mesh_count; ++mi) {
+        const vkgv_mesh_desc& m = meshes[mi];
+        if ((rc = upload(c, c->mesh_bufs[4 * mi + 0], m.vertex_indices, 4 * m.vertex_index_count))) return rc;
+        if ((rc = upload(c, c->mesh_bufs[4 * mi + 1], m.triangle_indices, m.triangle_index_bytes))) return rc;
+        if ((rc = upload(c, c->mesh_bufs[4 * mi + 2], m.vertices, sizeof(vkgv_vertex) * m.vertex_count))) return rc;
+        if ((rc = upload(c, c->mesh_bufs[4 * mi + 3], m.meshlets, sizeof(vkgv_meshlet) * m.meshlet_count))) return rc;
+        c->mesh_meshlets[mi] = m.meshlet_count;
+        if (m.cones) {
+            cone_first[mi] = (uint32_t)all_cones.size();
+            all_cones.insert(all_cones.end(), m.cones, m.cones + m.meshlet_count);
+        }
+    }
+    std::vector<vkgv_primitive> dp(prim_count);
+    std::vector<uint32_t> cb(std::max<uint32_t>(prim_count, 1u), 0xFFFFFFFFu);
+    c->prim_meshlets.assign(prim_count, 0);
+    for (uint32_t p = 0; p < prim_count; ++p) {
+        const uint32_t mi = prims[p].mesh_index;
+        vkgv_primitive& q = dp[p];
+        q.vertex_index_buffer = reinterpret_cast<uint64_t>(c->mesh_bufs[4 * mi + 0].p);
+        q.primitive_index_buffer = reinterpret_cast<uint64_t>(c->mesh_bufs[4 * mi + 1].p);
+        q.vertex_buffer = reinterpret_cast<uint64_t>(c->mesh_bufs[4 * mi + 2].p);
+        q.meshlet_buffer = reinterpret_cast<uint64_t>(c->mesh_bufs[4 * mi + 3].p);
+        std::memcpy(q.aabb_extents, prims[p].aabb_extents, 12);
+        std::memcpy(q.aabb_center, prims[p].aabb_center, 12);
+        q.meshlet_count = meshes[mi].meshlet_count;
+        q.material_index = prims[p].material_index;
+        cb[p] = cone_first[mi];
+        c->prim_meshlets[p] = meshes[mi].meshlet_count;
+    }
+    if ((rc = upload(c, c->prims, dp.data(), sizeof(vkgv_primitive) * prim_count))) return rc;
+    if ((rc = upload(c, c->materials, materials, sizeof(vkgv_material) * material_count))) return rc;
+    if ((rc = upload(c, c->cone_base, cb.data(), sizeof(uint32_t) * cb.size()))) return rc;
+    if ((rc = upload(c, c->cones, all_cones.data(), sizeof(vkgv_meshlet_cone) * all_cones.size()))) return rc;
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    c->prim_count = prim_count;
+    c->material_count = material_count;
+    c->scene = true;
+    // a new scene invalidates the draw list (world.cpp:181-182)
+    c->draws_set = false;
+    c->draw_count = 0;
+    return VKGV_OK;
+}
+
+int vkgv_set_draws(vkgv_ctx* c, const vkgv_meshlet_draw* draws, uint32_t count)
+{
+    if (!c) return VKGV_ERROR_INVALID_ARGUMENT;
+    if (!c->scene) return fail(c, VKGV_ERROR_STATE, "set_draws: no scene uploaded");
+    if (count && !draws) return fail(c, VKGV_ERROR_INVALID_ARGUMENT, "set_draws: null draws");
+    if (count > VKGV_MAX_DRAWS) return fail(c, VKGV_ERROR_LIMIT, "set_draws: more than 2^25 draws (25-bit draw index)");
+    uint32_t maxt = 0;
+    for (uint32_t d = 0; d < count; ++d) {
+        const vkgv_meshlet_draw& dr = draws[d];
+        if (dr.primitive_index >= c->prim_count || dr.meshlet_index >= c->prim_meshlets[dr.primitive_index])
+            return fail(c, VKGV_ERROR_INVALID_ARGUMENT, "set_draws: draw " + std::to_string(d) + " out of range");
+        maxt = std::max(maxt, dr.transform_index);
+    }
+    HIP_TRY(c, hipSetDevice(c->device));
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    int rc;
+    if ((rc = upload(c, c->draws, draws, sizeof(vkgv_meshlet_draw) * count))) return rc;
+    if ((rc = dalloc(c, c->cull, sizeof(CullRec) * std::max<uint32_t>(count, 1u)))) return rc;
+    c->draw_count = count;
+    c->max_transform = maxt;
+    c->draws_set = true;
+    launch_build_cull_records(scene_dev(c), static_cast<CullRec*>(c->cull.p), c->stream);
+    HIP_TRY(c, hipGetLastError());
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    return VKGV_OK;
+}
+
+int vkgv_set_transforms(vkgv_ctx* c, const float* mat4s, uint32_t count)
+{
+    if (!c) return VKGV_ERROR_INVALID_ARGUMENT;
+    if (count && !mat4s) return fail(c, VKGV_ERROR_INVALID_ARGUMENT, "set_transforms: null array");
+    HIP_TRY(c, hipSetDevice(c->device));
+    const size_t bytes = sizeof(float) * 16 * (size_t)count;
+    if (c->transforms.bytes < bytes || !c->transforms.p) {
+        HIP_TRY(c, hipStreamSynchronize(c->stream));
+        int rc = dalloc(c, c->transforms, bytes);
+        if (rc) return rc;
+    }
+    if (bytes) HIP_TRY(c, hipMemcpyAsync(c->transforms.p, mat4s, bytes, hipMemcpyHostToDevice, c->stream));
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    c->transform_count = count;
+    return VKGV_OK;
+}
+
+int vkgv_view_create(vkgv_ctx* c, uint32_t width, uint32_t height, vkgv_view** out)
+{
+    if (!c || !out) return VKGV_ERROR_INVALID_ARGUMENT;
+    *out = nullptr;
+    if (width == 0 || height == 0 || width > 16384 || height > 16384 || std::max(width, height) < 2)
+        return fail(c, VKGV_ERROR_INVALID_ARGUMENT, "view_create: size must be 1..16384 with max(W,H) >= 2");
+    HIP_TRY(c, hipSetDevice(c->device));
+    auto* v = new vkgv_view();
+    v->ctx = c;
+    ViewDev& d = v->dev;
+    d.width = width;
+    d.height = height;
+    // pyramid geometry (application.cpp:472-483)
+    const uint32_t m = std::max(width, height);
+    uint32_t levels = 0;
+    while ((m >> (levels + 1)) != 0) ++levels;
+    d.pyr.levels = levels;
+    uint32_t off = 0;
+    for (uint32_t k = 0; k < 16; ++k) {
+        d.pyr.w[k] = d.pyr.h[k] = d.pyr.off[k] = 0;
+        if (k < levels) {
+            d.pyr.w[k] = std::max(1u, width >> (k + 1));
+            d.pyr.h[k] = std::max(1u, height >> (k + 1));
+            d.pyr.off[k] = off;
+            off += d.pyr.w[k] * d.pyr.h[k];
+        }
+    }
+    const size_t px = (size_t)width * height;
+    int rc = VKGV_OK;
+    if (!rc) rc = dalloc(c, v->keys, 8 * px);
+    if (!rc) rc = dalloc(c, v->color, 4 * px);
+    if (!rc) rc = dalloc(c, v->pyramid, 4 * (size_t)std::max(off, 1u));
+    if (!rc) rc = dalloc(c, v->counters, sizeof(Counters));
+    if (!rc) rc = dalloc(c, v->big, sizeof(BigTri) * (size_t)c->big_capacity);
+    if (!rc) rc = dalloc(c, v->big_start, sizeof(uint32_t) * (size_t)c->big_capacity);
+    if (rc) { vkgv_view_destroy(v); return rc; }
+    d.keys = static_cast<unsigned long long*>(v->keys.p);
+    d.color = static_cast<uint32_t*>(v->color.p);
+    d.pyramid = static_cast<float*>(v->pyramid.p);
+    d.counters = static_cast<Counters*>(v->counters.p);
+    d.big = static_cast<BigTri*>(v->big.p);
+    d.big_tile_start = static_cast<uint32_t*>(v->big_start.p);
+    d.big_capacity = c->big_capacity;
+    // first frame: pyramid = far (0.0), so the first cull is frustum-only
+    // (the reference leaves it undefined, application.cpp:743); empty visbuffer.
+    if (hipMemsetAsync(v->pyramid.p, 0, v->pyramid.bytes, c->stream) != hipSuccess
+        || hipMemsetAsync(v->counters.p, 0, v->counters.bytes, c->stream) != hipSuccess
+        || hipMemsetAsync(v->color.p, 0, v->color.bytes, c->stream) != hipSuccess) {
+        vkgv_view_destroy(v);
+        return fail(c, VKGV_ERROR_HIP, "view_create: clear failed");
+    }
+    launch_clear_keys(d.keys, px, c->stream);
+    for (auto& e : v->ev)
+        if (hipEventCreate(&e) != hipSuccess) { vkgv_view_destroy(v); return fail(c, VKGV_ERROR_HIP, "hipEventCreate"); }
+    if (hipStreamSynchronize(c->stream) != hipSuccess) { vkgv_view_destroy(v); return fail(c, VKGV_ERROR_HIP, "sync"); }
+    *out = v;
+    return VKGV_OK;
+}
+
+void vkgv_view_destroy(vkgv_view* v)
+{
+    if (!v) return;
+    vkgv_ctx* c = v->ctx;
+    if (c) {
+        (void)hipSetDevice(c->device);
+        (void)hipStreamSynchronize(c->stream);
+        if (v->last_stream) (void)hipStreamSynchronize(v->last_stream);
+    }
+    for (DevBuf* b : { &v->keys, &v->color, &v->pyramid, &v->visible0, &v->visible1, &v->history, &v->counters, &v->big,
+                       &v->big_start })
+        dfree(*b);
+    for (auto& e : v->ev)
+        if (e) (void)hipEventDestroy(e);
+    delete v;
+}
+
+int vkgv_render(vkgv_ctx* c, vkgv_view* v, const vkgv_frame_desc* f)
+{
+    int rc = check_view(c, v);
+    if (rc) return rc;
+    if (!f) return fail(c, VKGV_ERROR_INVALID_ARGUMENT, "render: null frame");
+    if (!c->draws_set) return fail(c, VKGV_ERROR_STATE, "render: no draw list");
+    if (c->draw_count && c->max_transform >= c->transform_count)
+        return fail(c, VKGV_ERROR_STATE, "render: draw references transform " + std::to_string(c->max_transform)
+                                             + " but only " + std::to_string(c->transform_count) + " are set");
+    const uint32_t flags = f->flags;
+    const bool two_pass = flags & VKGV_CULL_TWO_PASS;
+    if (two_pass && (flags & VKGV_CULL_OCCLUSION_PREV))
+        return fail(c, VKGV_ERROR_INVALID_ARGUMENT, "render: CULL_TWO_PASS and CULL_OCCLUSION_PREV are exclusive");
+    HIP_TRY(c, hipSetDevice(c->device));
+    if ((rc = ensure_lists(c, v))) return rc;
+    hipStream_t st = f->stream ? static_cast<hipStream_t>(f->stream) : c->stream;
+    if (v->last_stream && v->last_stream != st) HIP_TRY(c, hipStreamSynchronize(v->last_stream));
+    v->last_stream = st;
+    const bool timed = flags & VKGV_TIMING;
+    auto mark = [&](Ev e) { if (timed) (void)hipEventRecord(v->ev[e], st); };
+
+    const SceneDev s = scene_dev(c);
+    CameraArg cam;
+    cam.cam = f->camera;
+    cam.eye[0] = f->eye[0]; cam.eye[1] = f->eye[1]; cam.eye[2] = f->eye[2];
+    const Mat4 vp = to_mat4(f->camera.view_projection);
+    const uint32_t raster_grid = (uint32_t)c->cus * c->tuning.raster_blocks_per_cu;
+    const uint64_t npx = (uint64_t)v->dev.width * v->dev.height;
+
+    mark(EV_START);
+    launch_reset_counters(v->dev.counters, 0xFFFFFFFFu, st);
+    launch_clear_keys(v->dev.keys, npx, st);
+    mark(EV_CLEAR);
+    launch_cull(s, v->dev, cam, flags, two_pass ? CULL_EARLY : CULL_SINGLE, 0, st);
+    mark(EV_CULL);
+    launch_raster(s, v->dev, vp, flags, 0, raster_grid, c->tuning.small_max_extent, st);
+    launch_raster_big(v->dev, c->tuning.big_blocks, st);
+    mark(EV_RASTER);
+    if (two_pass) {
+        launch_hiz(v->dev, st);
+        mark(EV_HIZ);
+        launch_reset_counters(v->dev.counters, 1u, st);
+        launch_cull(s, v->dev, cam, flags, CULL_LATE, 1, st);
+        mark(EV_CULL_LATE);
+        launch_raster(s, v->dev, vp, flags, 1, raster_grid, c->tuning.small_max_extent, st);
+        launch_raster_big(v->dev, c->tuning.big_blocks, st);
+        mark(EV_RASTER_LATE);
+    }
+    if (!(flags & VKGV_SKIP_RESOLVE)) launch_resolve(s, v->dev, vp, flags, st);
+    mark(EV_RESOLVE);
+    if (!two_pass && (flags & VKGV_BUILD_PYRAMID)) launch_hiz(v->dev, st);
+    mark(EV_PYRAMID);
+    HIP_TRY(c, hipGetLastError());
+    v->timed = timed;
+    v->last_flags = flags;
+    return VKGV_OK;
+}
+
+int vkgv_view_build_pyramid(vkgv_ctx* c, vkgv_view* v, void* stream)
+{
+    int rc = check_view(c, v);
+    if (rc) return rc;
+    HIP_TRY(c, hipSetDevice(c->device));
+    hipStream_t st = stream ? static_cast<hipStream_t>(stream) : c->stream;
+    launch_hiz(v->dev, st);
+    HIP_TRY(c, hipGetLastError());
+    v->last_stream = st;
+    return VKGV_OK;
+}
+
+int vkgv_view_reset_history(vkgv_ctx* c, vkgv_view* v)
+{
+    int rc = check_view(c, v);
+    if (rc) return rc;
+    if (!v->history.p) return VKGV_OK;
+    HIP_TRY(c, hipSetDevice(c->device));
+    if ((rc = sync_view(c, v))) return rc;
+    HIP_TRY(c, hipMemsetAsync(v->history.p, 0, v->history.bytes, c->stream));
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    return VKGV_OK;
+}
+
+int vkgv_view_pyramid_info(const vkgv_view* v, uint32_t* levels, uint32_t* sizes, uint64_t* texels)
+{
+    if (!v) return VKGV_ERROR_INVALID_ARGUMENT;
+    const PyramidDesc& p = v->dev.pyr;
+    if (levels) *levels = p.levels;
+    uint64_t n = 0;
+    for (uint32_t k = 0; k < p.levels; ++k) {
+        if (sizes) { sizes[2 * k] = p.w[k]; sizes[2 * k + 1] = p.h[k]; }
+        n += (uint64_t)p.w[k] * p.h[k];
+    }
+    if (texels) *texels = n;
+    return VKGV_OK;
+}
+
+static int copy_out(vkgv_ctx* c, vkgv_view* v, void* dst, const void* src, size_t bytes)
+{
+    int rc = check_view(c, v);
+    if (rc) return rc;
+    if (!dst) return fail(c, VKGV_ERROR_INVALID_ARGUMENT, "readback: null destination");
+    HIP_TRY(c, hipSetDevice(c->device));
+    if ((rc = sync_view(c, v))) return rc;
+    HIP_TRY(c, hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    return VKGV_OK;
+}
+
+int vkgv_view_read_visbuffer(vkgv_ctx* c, vkgv_view* v, uint64_t* keys)
+{
+    if (check_view(c, v)) return check_view(c, v);
+    return copy_out(c, v, keys, v->keys.p, 8ull * v->dev.width * v->dev.height);
+}
+
+int vkgv_view_read_ids_depth(vkgv_ctx* c, vkgv_view* v, uint32_t* ids, float* depth)
+{
+    int rc = check_view(c, v);
+    if (rc) return rc;
+    const size_t n = (size_t)v->dev.width * v->dev.height;
+    std::vector<uint64_t> k(n);
+    if ((rc = copy_out(c, v, k.data(), v->keys.p, 8 * n))) return rc;
+    for (size_t i = 0; i < n; ++i) {
+        if (ids) ids[i] = 0xFFFFFFFEu - (uint32_t)k[i];
+        if (depth) {
+            const uint32_t b = ~(uint32_t)(k[i] >> 32);
+            std::memcpy(&depth[i], &b, 4);
+        }
+    }
+    return VKGV_OK;
+}
+
+int vkgv_view_read_color(vkgv_ctx* c, vkgv_view* v, uint32_t* rgba8)
+{
+    if (check_view(c, v)) return check_view(c, v);
+    return copy_out(c, v, rgba8, v->color.p, 4ull * v->dev.width * v->dev.height);
+}
+
+int vkgv_view_read_pyramid(vkgv_ctx* c, vkgv_view* v, float* texels)
+{
+    if (check_view(c, v)) return check_view(c, v);
+    uint64_t n = 0;
+    vkgv_view_pyramid_info(v, nullptr, nullptr, &n);
+    return copy_out(c, v, texels, v->pyramid.p, 4 * n);
+}
+
+int vkgv_view_write_pyramid(vkgv_ctx* c, vkgv_view* v, const float* texels)
+{
+    int rc = check_view(c, v);
+    if (rc) return rc;
+    if (!texels) return fail(c, VKGV_ERROR_INVALID_ARGUMENT, "write_pyramid: null source");
+    uint64_t n = 0;
+    vkgv_view_pyramid_info(v, nullptr, nullptr, &n);
+    HIP_TRY(c, hipSetDevice(c->device));
+    if ((rc = sync_view(c, v))) return rc;
+    HIP_TRY(c, hipMemcpyAsync(v->pyramid.p, texels, 4 * n, hipMemcpyHostToDevice, c->stream));
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    return VKGV_OK;
+}
+
+int vkgv_view_write_depth(vkgv_ctx* c, vkgv_view* v, const float* depth)
+{
+    int rc = check_view(c, v);
+    if (rc) return rc;
+    if (!depth) return fail(c, VKGV_ERROR_INVALID_ARGUMENT, "write_depth: null source");
+    HIP_TRY(c, hipSetDevice(c->device));
+    if ((rc = sync_view(c, v))) return rc;
+    const size_t n = (size_t)v->dev.width * v->dev.height;
+    DevBuf tmp;
+    if ((rc = upload(c, tmp, depth, 4 * n))) return rc;
+    launch_write_depth_keys(v->dev.keys, static_cast<const float*>(tmp.p), n, c->stream);
+    hipError_t e = hipStreamSynchronize(c->stream);
+    dfree(tmp);
+    if (e != hipSuccess) return fail(c, VKGV_ERROR_HIP, std::string("write_depth: ") + hipGetErrorString(e));
+    return VKGV_OK;
+}
+
+int vkgv_view_read_visible(vkgv_ctx* c, vkgv_view* v, uint32_t pass, uint32_t* out, uint32_t capacity, uint32_t* count)
+{
+    int rc = check_view(c, v);
+    if (rc) return rc;
+    if (pass > 1 || !count) return fail(c, VKGV_ERROR_INVALID_ARGUMENT, "read_visible: bad pass or null count");
+    Counters h{};
+    if ((rc = copy_out(c, v, &h, v->counters.p, sizeof h))) return rc;
+    *count = h.vis_count[pass];
+    const uint32_t n = std::min(h.vis_count[pass], capacity);
+    if (n && out) {
+        const void* src = pass ? v->visible1.p : v->visible0.p;
+        if (!src) return fail(c, VKGV_ERROR_STATE, "read_visible: no frame rendered");
+        if ((rc = copy_out(c, v, out, src, 4ull * n))) return rc;
+    }
+    return VKGV_OK;
+}
+
+int vkgv_view_read_stats(vkgv_ctx* c, vkgv_view* v, vkgv_frame_stats* s)
+{
+    int rc = check_view(c, v);
+    if (rc) return rc;
+    if (!s) return fail(c, VKGV_ERROR_INVALID_ARGUMENT, "read_stats: null");
+    Counters h{};
+    if ((rc = copy_out(c, v, &h, v->counters.p, sizeof h))) return rc;
+    std::memset(s, 0, sizeof *s);
+    s->draws = c->draw_count;
+    s->visible_early = h.vis_count[0];
+    s->visible_late = h.vis_count[1];
+    s->big_triangles = h.big_total_tris;
+    s->big_tiles = h.big_total_tiles;
+    s->clipped_triangles = h.clipped;
+    s->overflow = h.overflow;
+    if (v->timed) {
+        auto el = [&](Ev a, Ev b) {
+            float ms = 0.0f;
+            (void)hipEventElapsedTime(&ms, v->ev[a], v->ev[b]);
+            return ms;
+        };
+        const bool two = v->last_flags & VKGV_CULL_TWO_PASS;
+        s->ms_clear = el(EV_START, EV_CLEAR);
+        s->ms_cull = el(EV_CLEAR, EV_CULL);
+        s->ms_raster = el(EV_CULL, EV_RASTER);
+        if (two) {
+            s->ms_hiz = el(EV_RASTER, EV_HIZ);
+            s->ms_cull_late = el(EV_HIZ, EV_CULL_LATE);
+            s->ms_raster_late = el(EV_CULL_LATE, EV_RASTER_LATE);
+            s->ms_resolve = el(EV_RASTER_LATE, EV_RESOLVE);
+        } else {
+            s->ms_resolve = el(EV_RASTER, EV_RESOLVE);
+        }
+        s->ms_pyramid = el(EV_RESOLVE, EV_PYRAMID);
+        s->ms_total = el(EV_START, EV_PYRAMID);
+    }
+    return VKGV_OK;
+}
+
+} // extern "C"

@@ -1,0 +1,255 @@
+// Meshlet cull + compaction (visbuffer.task.glsl:25-76, culling.h.glsl:8-56).
+//
+// One lane per meshlet draw.  A wave reads 64 consecutive 32-byte cull
+// records (one contiguous 2 KiB span), transforms the meshlet AABB, runs the
+// frustum test, the optional normal-cone test and the Hi-Z probe, and appends
+// its visible draws to the list with one atomicAdd per wave (ballot + mbcnt
+// prefix).  The list order across waves is unspecified; the visible SET is
+// what the reference defines (its per-task-workgroup payload is re-packed by
+// the mesh-shader launch anyway).
+#include "internal.hpp"
+
+namespace vkgv {
+namespace {
+
+constexpr uint32_t kBlock = 256;
+
+// aabbPositions (culling.h.glsl:32-41).
+__constant__ float kCorner[8][3] = {
+    { 1, -1, -1 }, { 1, 1, -1 }, { -1, 1, -1 }, { -1, -1, -1 },
+    { 1, -1, 1 },  { 1, 1, 1 },  { -1, -1, 1 }, { -1, 1, 1 },
+};
+
+__device__ __forceinline__ bool aabb_in_frustum(const float c[3], const float e[3], const vkgv_camera& cam)
+{
+#pragma unroll
+    for (int i = 0; i < 6; ++i) {
+        const float* p = cam.frustum[i];
+        const float radius = (e[0] * fabsf(p[0]) + e[1] * fabsf(p[1])) + e[2] * fabsf(p[2]);
+        const float distance = ((p[0] * c[0] + p[1] * c[1]) + p[2] * c[2]) - p[3];
+        if (-radius > distance) return false;
+    }
+    return true;
+}
+
+// projectAabb (culling.h.glsl:44-56) + the probe (visbuffer.task.glsl:56-64).
+__device__ __forceinline__ bool occlusion_visible(const float c[3], const float e[3], const float* vpm,
+                                                  const float* __restrict__ pyr, const PyramidDesc& pd)
+{
+    Mat4 vp;
+#pragma unroll
+    for (int i = 0; i < 16; ++i) vp.m[i] = vpm[i];
+    float mnx = 1.0f, mny = 1.0f, mnz = 1.0f, mxx = -1.0f, mxy = -1.0f, mxz = -1.0f;
+#pragma unroll
+    for (int i = 0; i < 8; ++i) {
+        const float px = kCorner[i][0] * e[0] + c[0];
+        const float py = kCorner[i][1] * e[1] + c[1];
+        const float pz = kCorner[i][2] * e[2] + c[2];
+        const float4 clip = mat4_point(vp, px, py, pz);
+        const float nx = gclamp(clip.x / clip.w, -1.0f, 1.0f);
+        const float ny = gclamp(clip.y / clip.w, -1.0f, 1.0f);
+        const float u = nx * 0.5f + 0.5f, v = ny * 0.5f + 0.5f, z = clip.z / clip.w;
+        mnx = gmin(mnx, u); mny = gmin(mny, v); mnz = gmin(mnz, z);
+        mxx = gmax(mxx, u); mxy = gmax(mxy, v); mxz = gmax(mxz, z);
+    }
+    (void)mnz;
+    const float width = (mxx - mnx) * (float)pd.w[0];
+    const float height = (mxy - mny) * (float)pd.h[0];
+    const float level = floor_log2(gmax(width, height));
+    const float depth = pyramid_probe(pyr, pd, (mnx + mxx) * 0.5f, (mny + mxy) * 0.5f, level);
+    return depth < mxz;
+}
+
+__device__ __forceinline__ float3 mat4_dir(const Mat4& m, float x, float y, float z)
+{
+    float3 o;
+    o.x = ((m.m[0] * x + m.m[4] * y) + m.m[8] * z) + m.m[12] * 0.0f;
+    o.y = ((m.m[1] * x + m.m[5] * y) + m.m[9] * z) + m.m[13] * 0.0f;
+    o.z = ((m.m[2] * x + m.m[6] * y) + m.m[10] * z) + m.m[14] * 0.0f;
+    return o;
+}
+
+__device__ __forceinline__ float dot3(float ax, float ay, float az, float bx, float by, float bz)
+{
+    return (ax * bx + ay * by) + az * bz;
+}
+
+// Normal-cone back-face test (meshoptimizer's published cone test), skipped
+// unless the instance transform is a proper similarity.
+__device__ __forceinline__ bool cone_culled(const vkgv_meshlet_cone& cone, const Mat4& m, const float* eye)
+{
+    if (!(cone.cutoff < 1.0f)) return false;
+    const float* a = m.m;
+    const float l0 = dot3(a[0], a[1], a[2], a[0], a[1], a[2]);
+    const float l1 = dot3(a[4], a[5], a[6], a[4], a[5], a[6]);
+    const float l2 = dot3(a[8], a[9], a[10], a[8], a[9], a[10]);
+    const float tol = l0 * 1e-5f;
+    if (!(fabsf(l0 - l1) <= tol && fabsf(l0 - l2) <= tol && fabsf(dot3(a[0], a[1], a[2], a[4], a[5], a[6])) <= tol
+          && fabsf(dot3(a[0], a[1], a[2], a[8], a[9], a[10])) <= tol
+          && fabsf(dot3(a[4], a[5], a[6], a[8], a[9], a[10])) <= tol))
+        return false;
+    if (!(det4(m) > 0.0f)) return false;
+    const float4 apex = mat4_point(m, cone.apex[0], cone.apex[1], cone.apex[2]);
+    float3 ax = mat4_dir(m, cone.axis[0], cone.axis[1], cone.axis[2]);
+    const float al = sqrtf(dot3(ax.x, ax.y, ax.z, ax.x, ax.y, ax.z));
+    ax.x = ax.x / al; ax.y = ax.y / al; ax.z = ax.z / al;
+    float dx = apex.x - eye[0], dy = apex.y - eye[1], dz = apex.z - eye[2];
+    const float dl = sqrtf(dot3(dx, dy, dz, dx, dy, dz));
+    dx = dx / dl; dy = dy / dl; dz = dz / dl;
+    return dot3(dx, dy, dz, ax.x, ax.y, ax.z) >= cone.cutoff;
+}
+
+struct CullArgs {
+    SceneDev s;
+    CameraArg cam;
+    const float* pyr;
+    PyramidDesc pd;
+    unsigned long long* history;    // one 64-bit word per 64 draws
+    uint32_t* out;
+    uint32_t* out_count;
+    uint32_t flags;
+    uint32_t mode;
+};
+
+__global__ __launch_bounds__(kBlock) void cull_kernel(CullArgs a)
+{
+    const uint32_t d = blockIdx.x * kBlock + threadIdx.x;
+    const uint32_t lane = threadIdx.x & 63u;
+    bool vis = false, newbit = false;
+    if (d < a.s.draw_count) {
+        const float4* rp = reinterpret_cast<const float4*>(a.s.cull + d);
+        const float4 r0 = rp[0], r1 = rp[1];
+        const uint32_t tidx = __float_as_uint(r1.z), cone = __float_as_uint(r1.w);
+        const Mat4 m = load_mat4(a.s.transforms + 16ull * tidx);
+        // world AABB: centre (M * vec4(c,1)).xyz, extent mat3(|M0|,|M1|,|M2|) * e
+        const float4 wc4 = mat4_point(m, r0.x, r0.y, r0.z);
+        const float wc[3] = { wc4.x, wc4.y, wc4.z };
+        const float e[3] = { r0.w, r1.x, r1.y };
+        float we[3];
+#pragma unroll
+        for (int r = 0; r < 3; ++r)
+            we[r] = (fabsf(m.m[r]) * e[0] + fabsf(m.m[4 + r]) * e[1]) + fabsf(m.m[8 + r]) * e[2];
+        bool pass = true;
+        if (a.flags & VKGV_CULL_FRUSTUM) pass = aabb_in_frustum(wc, we, a.cam.cam);
+        if (pass && (a.flags & VKGV_CULL_CONE) && cone != 0xFFFFFFFFu)
+            pass = !cone_culled(a.s.cones[cone], m, a.cam.eye);
+        if (a.mode == CULL_SINGLE) {
+            vis = pass;
+            if (vis && (a.flags & VKGV_CULL_OCCLUSION_PREV))
+                vis = occlusion_visible(wc, we, a.cam.cam.prev_occlusion_view_projection, a.pyr, a.pd);
+        } else {
+            const bool old = (a.history[d >> 6] >> (d & 63u)) & 1ull;
+            if (a.mode == CULL_EARLY) {
+                vis = pass && old;
+            } else {
+                const bool occ = pass && occlusion_visible(wc, we, a.cam.cam.occlusion_view_projection, a.pyr, a.pd);
+                newbit = occ;
+                vis = occ && !(pass && old);
+            }
+        }
+    }
+    if (a.mode == CULL_LATE) {
+        const unsigned long long bits = __ballot(newbit);
+        if (lane == 0 && (d - lane) < a.s.draw_count) a.history[d >> 6] = bits;
+    }
+    const unsigned long long mask = __ballot(vis);
+    const uint32_t cnt = (uint32_t)__popcll(mask);
+    if (cnt == 0) return;
+    uint32_t base = 0;
+    if (lane == 0) base = atomicAdd(a.out_count, cnt);
+    base = __shfl(base, 0);
+    if (vis) {
+        const uint32_t rank = __builtin_amdgcn_mbcnt_hi((uint32_t)(mask >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)mask, 0u));
+        a.out[base + rank] = d;
+    }
+}
+
+// Cull records from the contract arrays: draw -> primitive -> meshlet AABB.
+__global__ __launch_bounds__(kBlock) void build_cull_records_kernel(SceneDev s, CullRec* out)
+{
+    const uint32_t d = blockIdx.x * kBlock + threadIdx.x;
+    if (d >= s.draw_count) return;
+    const vkgv_meshlet_draw dr = s.draws[d];
+    const vkgv_primitive& p = s.prims[dr.primitive_index];
+    const vkgv_meshlet* ml = reinterpret_cast<const vkgv_meshlet*>(p.meshlet_buffer) + dr.meshlet_index;
+    CullRec r;
+    r.cx = ml->aabb_center[0]; r.cy = ml->aabb_center[1]; r.cz = ml->aabb_center[2];
+    r.ex = ml->aabb_extents[0]; r.ey = ml->aabb_extents[1]; r.ez = ml->aabb_extents[2];
+    r.transform = dr.transform_index;
+    const uint32_t cb = s.prim_cone_base[dr.primitive_index];
+    r.cone = cb == 0xFFFFFFFFu ? 0xFFFFFFFFu : cb + dr.meshlet_index;
+    out[d] = r;
+}
+
+__global__ __launch_bounds__(kBlock) void clear_keys_kernel(ulonglong2* keys, uint64_t n2, unsigned long long* tail, uint32_t odd)
+{
+    const ulonglong2 v = make_ulonglong2(VKGV_KEY_CLEAR, VKGV_KEY_CLEAR);
+    for (uint64_t i = (uint64_t)blockIdx.x * kBlock + threadIdx.x; i < n2; i += (uint64_t)gridDim.x * kBlock) keys[i] = v;
+    if (odd && blockIdx.x == 0 && threadIdx.x == 0) *tail = VKGV_KEY_CLEAR;
+}
+
+__global__ void reset_counters_kernel(Counters* c, uint32_t what)
+{
+    const uint32_t t = threadIdx.x;
+    uint32_t* w = reinterpret_cast<uint32_t*>(c);
+    if (what == 0xFFFFFFFFu) {
+        if (t < sizeof(Counters) / 4) w[t] = 0;
+    } else if (t == 0) {
+        // reset of the per-raster-pass big-triangle counter only
+        c->big = 0ull;
+    }
+}
+
+__global__ __launch_bounds__(kBlock) void write_depth_keys_kernel(unsigned long long* keys, const float* depth, uint64_t n)
+{
+    const uint64_t i = (uint64_t)blockIdx.x * kBlock + threadIdx.x;
+    if (i < n) keys[i] = ((unsigned long long)(~__float_as_uint(depth[i])) << 32) | 0xFFFFFFFFull;
+}
+
+} // namespace
+
+void launch_build_cull_records(const SceneDev& s, CullRec* out, hipStream_t st)
+{
+    if (!s.draw_count) return;
+    build_cull_records_kernel<<<(s.draw_count + kBlock - 1) / kBlock, kBlock, 0, st>>>(s, out);
+}
+
+void launch_clear_keys(unsigned long long* keys, uint64_t n, hipStream_t st)
+{
+    const uint64_t n2 = n / 2;
+    uint64_t blocks = (n2 + kBlock - 1) / kBlock;
+    if (blocks > 4096) blocks = 4096;
+    if (blocks == 0) blocks = 1;
+    clear_keys_kernel<<<(uint32_t)blocks, kBlock, 0, st>>>(reinterpret_cast<ulonglong2*>(keys), n2, keys + n - 1,
+                                                           (uint32_t)(n & 1));
+}
+
+void launch_reset_counters(Counters* c, uint32_t what, hipStream_t st)
+{
+    reset_counters_kernel<<<1, 64, 0, st>>>(c, what);
+}
+
+void launch_cull(const SceneDev& s, const ViewDev& v, const CameraArg& cam, uint32_t flags, CullMode mode,
+                 uint32_t list, hipStream_t st)
+{
+    if (!s.draw_count) return;
+    CullArgs a;
+    a.s = s;
+    a.cam = cam;
+    a.pyr = v.pyramid;
+    a.pd = v.pyr;
+    a.history = reinterpret_cast<unsigned long long*>(v.history);
+    a.out = v.visible[list];
+    a.out_count = &v.counters->vis_count[list];
+    a.flags = flags;
+    a.mode = mode;
+    cull_kernel<<<(s.draw_count + kBlock - 1) / kBlock, kBlock, 0, st>>>(a);
+}
+
+void launch_write_depth_keys(unsigned long long* keys, const float* depth, uint64_t n, hipStream_t st)
+{
+    if (!n) return;
+    write_depth_keys_kernel<<<(uint32_t)((n + kBlock - 1) / kBlock), kBlock, 0, st>>>(keys, depth, n);
+}
+
+} // namespace vkgv

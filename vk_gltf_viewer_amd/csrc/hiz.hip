@@ -1,0 +1,132 @@
+// Hi-Z depth pyramid (hiz_reduce.comp.glsl:21-31, application.cpp:435-529, 951-1003).
+//
+// Level k (0-based) is max(1, W >> (k+1)) x max(1, H >> (k+1)); texel p of a
+// level samples its source (the depth image for level 0, level k-1 after) at
+// (p + 0.5) / dstSize through the MIN-reduction linear sampler, i.e. the
+// minimum of the footprint texels with non-zero bilinear weight.
+//
+// Two launches instead of one per level:
+//  1. hiz_tiles: one 256-thread block per 32x32 texel tile of level 0, read
+//     straight from the 64-bit visibility buffer; the following levels are
+//     reduced in LDS for as long as every source level is exactly twice its
+//     destination (then the footprint of texel p is {2p, 2p+1} and stays
+//     inside the tile), writing each level once;
+//  2. hiz_tail: one 1024-thread block for the remaining (small) levels with
+//     the general footprint.
+#include "internal.hpp"
+
+namespace vkgv {
+namespace {
+
+__device__ __forceinline__ void footprint(uint32_t p, uint32_t dst, uint32_t src, int& t0, int& t1)
+{
+    const float u = (((float)p + 0.5f) / (float)dst) * (float)src - 0.5f;
+    axis_footprint(u, (int)src, t0, t1);
+}
+
+struct HizArgs {
+    const unsigned long long* keys;
+    uint32_t W, H;
+    float* pyr;
+    PyramidDesc pd;
+    uint32_t tile_levels;   // levels produced by hiz_tiles (>= 1)
+};
+
+__global__ __launch_bounds__(256) void hiz_tiles_kernel(HizArgs a)
+{
+    __shared__ float s_a[32 * 32];
+    __shared__ float s_b[16 * 16];
+    const uint32_t w0 = a.pd.w[0], h0 = a.pd.h[0];
+    const uint32_t bx = blockIdx.x * 32u, by = blockIdx.y * 32u;
+    float* lvl0 = a.pyr + a.pd.off[0];
+    for (uint32_t t = threadIdx.x; t < 1024u; t += 256u) {
+        const uint32_t x = bx + (t & 31u), y = by + (t >> 5);
+        float d = 0.0f;
+        if (x < w0 && y < h0) {
+            int x0, x1, y0, y1;
+            footprint(x, w0, a.W, x0, x1);
+            footprint(y, h0, a.H, y0, y1);
+            const unsigned long long* r0 = a.keys + (size_t)y0 * a.W;
+            const unsigned long long* r1 = a.keys + (size_t)y1 * a.W;
+            d = key_depth(r0[x0]);
+            d = gmin(d, key_depth(r0[x1]));
+            d = gmin(d, key_depth(r1[x0]));
+            d = gmin(d, key_depth(r1[x1]));
+            lvl0[(size_t)y * w0 + x] = d;
+        }
+        s_a[t] = d;
+    }
+    float* src = s_a;
+    float* dst = s_b;
+    uint32_t ts = 32u;   // tile size of the source level in LDS
+    for (uint32_t k = 1; k < a.tile_levels; ++k) {
+        __syncthreads();
+        const uint32_t nts = ts >> 1;
+        const uint32_t wk = a.pd.w[k], hk = a.pd.h[k];
+        const uint32_t ox = bx >> k, oy = by >> k;
+        float* out = a.pyr + a.pd.off[k];
+        for (uint32_t t = threadIdx.x; t < nts * nts; t += 256u) {
+            const uint32_t i = t % nts, j = t / nts;
+            const uint32_t x = ox + i, y = oy + j;
+            const float* s0 = src + (2u * j) * ts + 2u * i;
+            // footprint {2i, 2i+1} x {2j, 2j+1}, taken in the order (x0,y0),(x1,y0),(x0,y1),(x1,y1)
+            float d = s0[0];
+            d = gmin(d, s0[1]);
+            d = gmin(d, s0[ts]);
+            d = gmin(d, s0[ts + 1]);
+            if (x < wk && y < hk) out[(size_t)y * wk + x] = d;
+            dst[t] = d;
+        }
+        float* tmp = src; src = dst; dst = tmp;
+        ts = nts;
+    }
+}
+
+__global__ __launch_bounds__(1024) void hiz_tail_kernel(float* pyr, PyramidDesc pd, uint32_t first)
+{
+    for (uint32_t k = first; k < pd.levels; ++k) {
+        const uint32_t sw = pd.w[k - 1], sh = pd.h[k - 1], dw = pd.w[k], dh = pd.h[k];
+        const float* src = pyr + pd.off[k - 1];
+        float* dst = pyr + pd.off[k];
+        for (uint32_t t = threadIdx.x; t < dw * dh; t += 1024u) {
+            const uint32_t x = t % dw, y = t / dw;
+            int x0, x1, y0, y1;
+            footprint(x, dw, sw, x0, x1);
+            footprint(y, dh, sh, y0, y1);
+            float d = src[y0 * sw + x0];
+            d = gmin(d, src[y0 * sw + x1]);
+            d = gmin(d, src[y1 * sw + x0]);
+            d = gmin(d, src[y1 * sw + x1]);
+            dst[t] = d;
+        }
+        __syncthreads();
+    }
+}
+
+} // namespace
+
+// Number of levels the tile kernel can produce: level 0 plus every following
+// level whose source is exactly twice its size, at most 6 (32 -> 1).
+uint32_t hiz_tile_levels(const PyramidDesc& pd)
+{
+    uint32_t n = 1;
+    while (n < pd.levels && n < 6 && pd.w[n - 1] == 2 * pd.w[n] && pd.h[n - 1] == 2 * pd.h[n]) ++n;
+    return n;
+}
+
+void launch_hiz(const ViewDev& v, hipStream_t st)
+{
+    if (v.pyr.levels == 0) return;
+    HizArgs a;
+    a.keys = v.keys;
+    a.W = v.width;
+    a.H = v.height;
+    a.pyr = v.pyramid;
+    a.pd = v.pyr;
+    a.tile_levels = hiz_tile_levels(v.pyr);
+    const dim3 grid((v.pyr.w[0] + 31) / 32, (v.pyr.h[0] + 31) / 32);
+    hiz_tiles_kernel<<<grid, 256, 0, st>>>(a);
+    if (a.tile_levels < v.pyr.levels) hiz_tail_kernel<<<1, 1024, 0, st>>>(v.pyramid, v.pyr, a.tile_levels);
+}
+
+} // namespace vkgv

@@ -1,0 +1,182 @@
+"""Host-side mirror of the reference's frame interface over libvkgv_hip.so.
+
+The reference drives its frame from ``Application::run()`` with a World
+(draws, transforms, primitives, materials) and a Camera
+(application.cpp:604-1280).  Here:
+
+    ctx  = Renderer()                         # Device + World GPU buffers
+    ctx.upload_scene(scene_arrays)            # assets.cpp:375-429, world.cpp:10-185
+    ctx.set_draws(draws); ctx.set_transforms(mats)   # world.cpp:230-345
+    view = ctx.create_view(W, H)              # the pass images (application.cpp:181-576)
+    view.render(camera, flags)                # the recorded frame (application.cpp:843-1003)
+    view.ids_depth(), view.color(), view.pyramid(), view.visible()
+
+Every call goes through the C ABI of include/vkgv.h; there is no Python or CPU
+fallback: a missing library or a failing call raises.
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+
+from . import _native as N
+from .scene import SceneArrays
+
+
+class VkgvError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"vkgv error {code}: {msg}")
+        self.code = code
+
+
+class Renderer:
+    def __init__(self, device: int = 0, big_triangle_capacity: int = 0):
+        self._lib = N.hip()
+        info = N.CreateInfo(device, big_triangle_capacity)
+        h = C.c_void_p()
+        rc = self._lib.vkgv_create(C.byref(info), C.byref(h))
+        if rc != 0:
+            raise VkgvError(rc, "vkgv_create failed (no HIP device?)")
+        self._h = h
+        self.views = []
+
+    # -- errors -----------------------------------------------------------------
+    def _check(self, rc):
+        if rc != 0:
+            raise VkgvError(rc, self._lib.vkgv_last_error(self._h).decode())
+        return rc
+
+    def close(self):
+        for v in list(self.views):
+            v.close()
+        if self._h:
+            self._lib.vkgv_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # -- scene ------------------------------------------------------------------
+    def upload_scene(self, sc: SceneArrays):
+        meshes, keep = sc.mesh_descs()
+        prims = np.ascontiguousarray(sc.primitives)
+        mats = np.ascontiguousarray(sc.materials)
+        self._check(self._lib.vkgv_upload_scene(
+            self._h, meshes, len(sc.meshes),
+            C.cast(prims.ctypes.data, C.POINTER(N.PrimitiveDesc)), len(prims),
+            C.cast(mats.ctypes.data, C.POINTER(N.Material)), len(mats)))
+        del keep
+
+    def set_draws(self, draws: np.ndarray):
+        d = np.ascontiguousarray(draws)
+        self._check(self._lib.vkgv_set_draws(self._h, d.ctypes.data, len(d)))
+
+    def set_transforms(self, transforms: np.ndarray):
+        t = np.ascontiguousarray(transforms, dtype=np.float32).reshape(-1, 16)
+        self._check(self._lib.vkgv_set_transforms(self._h, t.ctypes.data, t.shape[0]))
+
+    def load(self, sc: SceneArrays):
+        self.upload_scene(sc)
+        self.set_transforms(sc.transforms)
+        self.set_draws(sc.draws)
+
+    def sync(self):
+        self._check(self._lib.vkgv_sync(self._h))
+
+    def create_view(self, width: int, height: int) -> "View":
+        h = C.c_void_p()
+        self._check(self._lib.vkgv_view_create(self._h, width, height, C.byref(h)))
+        v = View(self, h, width, height)
+        self.views.append(v)
+        return v
+
+
+class View:
+    def __init__(self, r: Renderer, h, width, height):
+        self.r, self._h, self.width, self.height = r, h, width, height
+        lv = C.c_uint32()
+        n = C.c_uint64()
+        sizes = (C.c_uint32 * 64)()
+        r._lib.vkgv_view_pyramid_info(h, C.byref(lv), sizes, C.byref(n))
+        self.levels = lv.value
+        self.level_sizes = [(sizes[2 * k], sizes[2 * k + 1]) for k in range(self.levels)]
+        self.pyramid_texels = n.value
+
+    def close(self):
+        if self._h:
+            self.r._lib.vkgv_view_destroy(self._h)
+            self._h = None
+            self.r.views.remove(self)
+
+    def render(self, camera: N.Camera, flags: int = N.MODE_REFERENCE, eye=(0.0, 0.0, 0.0), stream=None):
+        f = N.FrameDesc()
+        f.camera = camera
+        f.eye[:] = list(eye)
+        f.flags = flags
+        f.stream = stream
+        self.r._check(self.r._lib.vkgv_render(self.r._h, self._h, C.byref(f)))
+
+    def build_pyramid(self):
+        self.r._check(self.r._lib.vkgv_view_build_pyramid(self.r._h, self._h, None))
+
+    def reset_history(self):
+        self.r._check(self.r._lib.vkgv_view_reset_history(self.r._h, self._h))
+
+    # -- readbacks ----------------------------------------------------------------
+    def keys(self) -> np.ndarray:
+        out = np.empty(self.width * self.height, dtype=np.uint64)
+        self.r._check(self.r._lib.vkgv_view_read_visbuffer(self.r._h, self._h, out.ctypes.data))
+        return out.reshape(self.height, self.width)
+
+    def ids_depth(self):
+        ids = np.empty(self.width * self.height, dtype=np.uint32)
+        dep = np.empty(self.width * self.height, dtype=np.float32)
+        self.r._check(self.r._lib.vkgv_view_read_ids_depth(self.r._h, self._h, ids.ctypes.data, dep.ctypes.data))
+        return ids.reshape(self.height, self.width), dep.reshape(self.height, self.width)
+
+    def color(self) -> np.ndarray:
+        out = np.empty(self.width * self.height, dtype=np.uint32)
+        self.r._check(self.r._lib.vkgv_view_read_color(self.r._h, self._h, out.ctypes.data))
+        return out.reshape(self.height, self.width)
+
+    def pyramid(self) -> np.ndarray:
+        out = np.empty(self.pyramid_texels, dtype=np.float32)
+        self.r._check(self.r._lib.vkgv_view_read_pyramid(self.r._h, self._h, out.ctypes.data))
+        return out
+
+    def pyramid_levels(self):
+        flat = self.pyramid()
+        res, off = [], 0
+        for (w, h) in self.level_sizes:
+            res.append(flat[off:off + w * h].reshape(h, w))
+            off += w * h
+        return res
+
+    def write_pyramid(self, texels: np.ndarray):
+        t = np.ascontiguousarray(texels, dtype=np.float32).ravel()
+        if t.size != self.pyramid_texels:
+            raise ValueError(f"pyramid has {self.pyramid_texels} texels, got {t.size}")
+        self.r._check(self.r._lib.vkgv_view_write_pyramid(self.r._h, self._h, t.ctypes.data))
+
+    def write_depth(self, depth: np.ndarray):
+        d = np.ascontiguousarray(depth, dtype=np.float32).ravel()
+        if d.size != self.width * self.height:
+            raise ValueError("depth size mismatch")
+        self.r._check(self.r._lib.vkgv_view_write_depth(self.r._h, self._h, d.ctypes.data))
+
+    def visible(self, pass_index: int = 0) -> np.ndarray:
+        n = C.c_uint32()
+        self.r._check(self.r._lib.vkgv_view_read_visible(self.r._h, self._h, pass_index, None, 0, C.byref(n)))
+        out = np.empty(max(1, n.value), dtype=np.uint32)
+        self.r._check(self.r._lib.vkgv_view_read_visible(self.r._h, self._h, pass_index, out.ctypes.data,
+                                                         out.size, C.byref(n)))
+        return np.sort(out[:n.value])
+
+    def stats(self) -> N.FrameStats:
+        s = N.FrameStats()
+        self.r._check(self.r._lib.vkgv_view_read_stats(self.r._h, self._h, C.byref(s)))
+        return s
