@@ -165,14 +165,15 @@ typedef struct vkgv_frame_stats {
     uint32_t visible_early;          /* draws rasterized in the first (or only) pass */
     uint32_t visible_late;           /* draws rasterized in the late pass (two-pass) */
     uint32_t big_triangles;          /* triangles sent to the tiled path */
-    uint32_t big_tiles;              /* 16x16 tiles they cover */
+    uint32_t big_tiles;              /* (32x32 screen tile, triangle) pairs they were binned to */
     uint32_t clipped_triangles;      /* triangles that needed near/guard-band clipping */
     uint32_t overflow;               /* non-zero if a queue overflowed (frame incomplete) */
-    uint32_t pad;
-    /* per-pass GPU milliseconds, filled when the frame ran with VKGV_TIMING */
+    uint32_t timed_frames;           /* frames averaged in the ms_* fields */
+    /* per-pass GPU milliseconds (HIP events), averaged over the frames rendered
+     * with VKGV_TIMING since the previous vkgv_view_read_stats (at most 64) */
     float    ms_clear, ms_cull, ms_raster, ms_hiz, ms_cull_late, ms_raster_late,
              ms_resolve, ms_pyramid, ms_total;
-    float    pad2;
+    uint32_t max_tile_tris;          /* most big triangles binned to one 32x32 tile */
 } vkgv_frame_stats;
 
 typedef struct vkgv_create_info {
@@ -182,6 +183,13 @@ typedef struct vkgv_create_info {
 
 typedef struct vkgv_ctx  vkgv_ctx;
 typedef struct vkgv_view vkgv_view;
+
+/* Raster tunables (the autotuner's knobs; per context). */
+typedef struct vkgv_tuning {
+    uint32_t small_max_extent;       /* pixels: triangles with a smaller bounding box are rasterized by
+                                        the meshlet's own wave, larger ones are binned (1..64, default 16) */
+    uint32_t raster_blocks_per_cu;   /* persistent meshlet-raster grid = CUs * this (1..16, default 3) */
+} vkgv_tuning;
 
 /* Status codes */
 #define VKGV_OK                       0
@@ -198,6 +206,15 @@ int         vkgv_create(const vkgv_create_info* info, vkgv_ctx** out);
 void        vkgv_destroy(vkgv_ctx* ctx);
 const char* vkgv_last_error(const vkgv_ctx* ctx);
 int         vkgv_sync(vkgv_ctx* ctx);      /* waits for the context's stream */
+/* Raster tunables (no counterpart in the reference, whose raster is fixed
+ * function): set by the autotuner, take effect at the next frame. */
+int         vkgv_set_tuning(vkgv_ctx* ctx, const vkgv_tuning* tuning);
+int         vkgv_get_tuning(const vkgv_ctx* ctx, vkgv_tuning* tuning);
+
+/* The 255 ascending thresholds the resolve uses for fromLinear + unorm8
+ * (srgb.h.glsl:26-32): the 8-bit code of a linear value c is the number of
+ * thresholds <= c.  Host only; needs no device. */
+void        vkgv_srgb_thresholds(float out[255]);
 
 /* Copies meshes, primitives and materials to device memory owned by the
  * context (assets.cpp:375-429 staging upload, world.cpp:10-185 primitive and

@@ -620,6 +620,7 @@ void ref_resolve(const ref_scene* s, const float* vp, const uint64_t* keys,
                  uint32_t w, uint32_t h, uint32_t flags, uint32_t* rgba)
 {
     const float hw = (float)w * 0.5f, hh = (float)h * 0.5f;
+    const float inv_hw = 1.0f / hw, inv_hh = 1.0f / hh;
     for (uint32_t py = 0; py < h; ++py) {
         for (uint32_t px = 0; px < w; ++px) {
             const uint64_t i = (uint64_t)py * w + px;
@@ -647,15 +648,16 @@ void ref_resolve(const ref_scene* s, const float* vp, const uint64_t* keys,
                     hv[k][0] = cl[0]; hv[k][1] = cl[1]; hv[k][2] = cl[3];
                     for (int j = 0; j < 4; ++j) col[k][j] = (float)vx->color[j] / 255.0f;
                 }
-                const float xn = (((float)px + 0.5f) - hw) / hw;
-                const float yn = (((float)py + 0.5f) - hh) / hh;
+                const float xn = (((float)px + 0.5f) - hw) * inv_hw;
+                const float yn = (((float)py + 0.5f) - hh) * inv_hh;
                 float r[3][3], l[3];
                 cross3(hv[1], hv[2], r[0]);
                 cross3(hv[2], hv[0], r[1]);
                 cross3(hv[0], hv[1], r[2]);
                 for (int k = 0; k < 3; ++k) l[k] = (r[k][0] * xn + r[k][1] * yn) + r[k][2];
                 const float sum = (l[0] + l[1]) + l[2];
-                for (int k = 0; k < 3; ++k) l[k] = l[k] / sum;
+                const float inv = 1.0f / sum;
+                for (int k = 0; k < 3; ++k) l[k] = l[k] * inv;
                 for (int j = 0; j < 4; ++j)
                     c[j] = c[j] * ((l[0] * col[0][j] + l[1] * col[1][j]) + l[2] * col[2][j]);
             }

@@ -104,10 +104,14 @@ class FrameDesc(C.Structure):
 class FrameStats(C.Structure):
     _fields_ = [("draws", C.c_uint32), ("visible_early", C.c_uint32), ("visible_late", C.c_uint32),
                 ("big_triangles", C.c_uint32), ("big_tiles", C.c_uint32), ("clipped_triangles", C.c_uint32),
-                ("overflow", C.c_uint32), ("pad", C.c_uint32),
+                ("overflow", C.c_uint32), ("timed_frames", C.c_uint32),
                 ("ms_clear", C.c_float), ("ms_cull", C.c_float), ("ms_raster", C.c_float), ("ms_hiz", C.c_float),
                 ("ms_cull_late", C.c_float), ("ms_raster_late", C.c_float), ("ms_resolve", C.c_float),
-                ("ms_pyramid", C.c_float), ("ms_total", C.c_float), ("pad2", C.c_float)]
+                ("ms_pyramid", C.c_float), ("ms_total", C.c_float), ("max_tile_tris", C.c_uint32)]
+
+
+class Tuning(C.Structure):
+    _fields_ = [("small_max_extent", C.c_uint32), ("raster_blocks_per_cu", C.c_uint32)]
 
 
 class CreateInfo(C.Structure):
@@ -189,7 +193,7 @@ HIP_SYMBOLS = [
     "vkgv_set_transforms", "vkgv_view_create", "vkgv_view_destroy", "vkgv_render", "vkgv_view_build_pyramid",
     "vkgv_view_reset_history", "vkgv_view_pyramid_info", "vkgv_view_read_visbuffer", "vkgv_view_read_ids_depth",
     "vkgv_view_read_color", "vkgv_view_read_pyramid", "vkgv_view_write_pyramid", "vkgv_view_write_depth",
-    "vkgv_view_read_visible", "vkgv_view_read_stats",
+    "vkgv_view_read_visible", "vkgv_view_read_stats", "vkgv_set_tuning", "vkgv_get_tuning", "vkgv_srgb_thresholds",
 ]
 
 
@@ -219,5 +223,8 @@ def hip():
         _bind(lib, "vkgv_view_read_ids_depth", C.c_int, _p, _p, _p, _p)
         _bind(lib, "vkgv_view_read_visible", C.c_int, _p, _p, _u32, _p, _u32, C.POINTER(_u32))
         _bind(lib, "vkgv_view_read_stats", C.c_int, _p, _p, C.POINTER(FrameStats))
+        _bind(lib, "vkgv_set_tuning", C.c_int, _p, C.POINTER(Tuning))
+        _bind(lib, "vkgv_get_tuning", C.c_int, _p, C.POINTER(Tuning))
+        _bind(lib, "vkgv_srgb_thresholds", None, _p)
         _hip = lib
     return _hip

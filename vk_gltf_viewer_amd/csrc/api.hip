@@ -53,16 +53,25 @@ struct vkgv_ctx {
     uint32_t draw_count = 0, transform_count = 0;
     uint32_t max_transform = 0;
     bool draws_set = false;
-    RasterTuning tuning{ 32u, 3u, 2048u };
-    uint32_t big_capacity = 1u << 20;
+    RasterTuning tuning{ 16u, 3u };
+    DevBuf srgb;                     // 255 sRGB-encode thresholds
+    uint32_t big_capacity = 1u << 18;
+    uint32_t clip_capacity = 1u << 16;
+    uint32_t cache_slots = 8192;     // resolve-cache meshlet slots per view
 };
 
 struct vkgv_view {
     vkgv_ctx* ctx = nullptr;
     ViewDev dev{};
-    DevBuf keys, color, pyramid, visible0, visible1, history, counters, big, big_start;
+    DevBuf keys, color, pyramid, visible0, visible1, history, counters, big, clipq, bin_count, bin_start, bins;
+    DevBuf setups, pairs, tri_cache, slot_albedo, draw_slot, big_chunk;
     uint32_t list_capacity = 0;
-    hipEvent_t ev[EV_COUNT] = {};
+    // Ring of per-frame event sets: timed frames are averaged at the next
+    // read_stats without synchronizing between frames.
+    static constexpr uint32_t kRing = 64;
+    hipEvent_t ev[kRing][EV_COUNT] = {};
+    uint32_t ring_next = 0, ring_frames = 0;
+    uint32_t ring_flags[kRing] = {};
     bool timed = false;
     uint32_t last_flags = 0;
     hipStream_t last_stream = nullptr;
@@ -126,7 +135,37 @@ SceneDev scene_dev(const vkgv_ctx* c)
     s.prim_cone_base = static_cast<const uint32_t*>(c->cone_base.p);
     s.cones = static_cast<const vkgv_meshlet_cone*>(c->cones.p);
     s.cull = static_cast<const CullRec*>(c->cull.p);
+    s.srgb = c->srgb.p;
     return s;
+}
+
+// fromLinear (srgb.h.glsl:9-15) followed by the unorm8 store.
+uint32_t srgb_code_ref(float c)
+{
+    const bool cutoff = c < 0.0031308f;
+    const float higher = 1.055f * std::pow(c, 1.0f / 2.4f) - 0.055f;
+    const float lower = c * 12.92f;
+    float v = cutoff ? lower : higher;
+    v = 1.0f < v ? 1.0f : v;
+    v = v < 0.0f ? 0.0f : v;
+    if (!(v >= 0.0f)) v = 0.0f;
+    return (uint32_t)(v * 255.0f + 0.5f);
+}
+
+// Thresholds t[k-1] = the smallest float c in [0, 1] whose code is >= k,
+// found by bisection over the bit patterns (the code is non-decreasing in c).
+void srgb_thresholds(float* t)
+{
+    for (uint32_t k = 1; k <= 255; ++k) {
+        uint32_t lo = 0, hi = 0x3F800000u;   // code(lo) < k <= code(hi)
+        while (hi - lo > 1) {
+            const uint32_t mid = lo + (hi - lo) / 2;
+            float f;
+            std::memcpy(&f, &mid, 4);
+            if (srgb_code_ref(f) >= k) hi = mid; else lo = mid;
+        }
+        std::memcpy(&t[k - 1], &hi, 4);
+    }
 }
 
 Mat4 to_mat4(const float* m)
@@ -145,11 +184,14 @@ int ensure_lists(vkgv_ctx* c, vkgv_view* v)
     if ((rc = dalloc(c, v->visible1, sizeof(uint32_t) * cap))) return rc;
     const size_t hwords = (cap + 63u) / 64u;
     if ((rc = dalloc(c, v->history, sizeof(uint64_t) * hwords))) return rc;
+    if ((rc = dalloc(c, v->draw_slot, sizeof(uint32_t) * cap))) return rc;
     HIP_TRY(c, hipMemsetAsync(v->history.p, 0, v->history.bytes, c->stream));
+    HIP_TRY(c, hipStreamSynchronize(c->stream));   // frames may run on another stream
     v->list_capacity = cap;
     v->dev.visible[0] = static_cast<uint32_t*>(v->visible0.p);
     v->dev.visible[1] = static_cast<uint32_t*>(v->visible1.p);
     v->dev.history = static_cast<uint32_t*>(v->history.p);
+    v->dev.draw_slot = static_cast<uint32_t*>(v->draw_slot.p);
     return VKGV_OK;
 }
 
@@ -190,7 +232,53 @@ int vkgv_create(const vkgv_create_info* info, vkgv_ctx** out)
     if (hipGetDeviceProperties(&prop, c->device) == hipSuccess && prop.multiProcessorCount > 0)
         c->cus = prop.multiProcessorCount;
     if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) { delete c; return VKGV_ERROR_HIP; }
+    SrgbTables tab;
+    std::memset(&tab, 0, sizeof tab);
+    srgb_thresholds(tab.thr);
+    const uint32_t inf = 0x7F800000u;
+    std::memcpy(&tab.thr[255], &inf, 4);
+    uint32_t b0;
+    std::memcpy(&b0, &tab.thr[0], 4);
+    tab.key0 = b0 >> 16;
+    tab.nbuckets = (0x3F800000u >> 16) - tab.key0 + 1u;
+    bool ok = tab.nbuckets <= sizeof tab.base;
+    for (uint32_t b = 0, k = 0; ok && b < tab.nbuckets; ++b) {
+        // thresholds below bucket b, and at most one inside it
+        uint32_t kb;
+        while (k < 255 && (std::memcpy(&kb, &tab.thr[k], 4), (kb >> 16) < tab.key0 + b)) ++k;
+        tab.base[b] = (uint8_t)k;
+        uint32_t k2 = k;
+        while (k2 < 255 && (std::memcpy(&kb, &tab.thr[k2], 4), (kb >> 16) == tab.key0 + b)) ++k2;
+        ok = k2 - k <= 1;
+    }
+    if (!ok || upload(c, c->srgb, &tab, sizeof tab) != VKGV_OK || hipStreamSynchronize(c->stream) != hipSuccess) {
+        vkgv_destroy(c);
+        return VKGV_ERROR_HIP;
+    }
     *out = c;
+    return VKGV_OK;
+}
+
+void vkgv_srgb_thresholds(float out[255])
+{
+    if (out) srgb_thresholds(out);
+}
+
+int vkgv_set_tuning(vkgv_ctx* c, const vkgv_tuning* t)
+{
+    if (!c || !t) return VKGV_ERROR_INVALID_ARGUMENT;
+    if (t->small_max_extent < 1 || t->small_max_extent > 64 || t->raster_blocks_per_cu < 1 || t->raster_blocks_per_cu > 16)
+        return fail(c, VKGV_ERROR_INVALID_ARGUMENT, "set_tuning: small_max_extent 1..64, raster_blocks_per_cu 1..16");
+    c->tuning.small_max_extent = t->small_max_extent;
+    c->tuning.raster_blocks_per_cu = t->raster_blocks_per_cu;
+    return VKGV_OK;
+}
+
+int vkgv_get_tuning(const vkgv_ctx* c, vkgv_tuning* t)
+{
+    if (!c || !t) return VKGV_ERROR_INVALID_ARGUMENT;
+    t->small_max_extent = c->tuning.small_max_extent;
+    t->raster_blocks_per_cu = c->tuning.raster_blocks_per_cu;
     return VKGV_OK;
 }
 
@@ -201,7 +289,7 @@ void vkgv_destroy(vkgv_ctx* c)
     (void)hipStreamSynchronize(c->stream);
     for (auto& b : c->mesh_bufs) dfree(b);
     dfree(c->prims); dfree(c->materials); dfree(c->cone_base); dfree(c->cones);
-    dfree(c->draws); dfree(c->cull); dfree(c->transforms);
+    dfree(c->draws); dfree(c->cull); dfree(c->transforms); dfree(c->srgb);
     (void)hipStreamDestroy(c->stream);
     delete c;
 }
@@ -252,20 +340,49 @@ int vkgv_upload_scene(vkgv_ctx* c, const vkgv_mesh_desc* meshes, uint32_t mesh_c
         if (prims[p].mesh_index >= mesh_count || prims[p].material_index >= material_count)
             return fail(c, VKGV_ERROR_INVALID_ARGUMENT, "upload_scene: primitive " + std::to_string(p) + " out of range");
 
-    // ---- device copies (assets.cpp:375-429 staging upload)
+    // ---- device copies (assets.cpp:375-429 staging upload).  The four
+    // buffer kinds of all meshes are packed into one device allocation each
+    // (every mesh's part 256-byte aligned); a primitive's four addresses
+    // point into them, as the reference's per-primitive buffer addresses do.
     HIP_TRY(c, hipStreamSynchronize(c->stream));
     for (auto& b : c->mesh_bufs) dfree(b);
-    c->mesh_bufs.assign((size_t)mesh_count * 4, DevBuf());
+    c->mesh_bufs.assign(4, DevBuf());
     c->mesh_meshlets.assign(mesh_count, 0);
     std::vector<uint32_t> cone_first(mesh_count, 0xFFFFFFFFu);
     std::vector<vkgv_meshlet_cone> all_cones;
+    std::vector<uint64_t> off((size_t)mesh_count * 4);
+    uint64_t total[4] = { 0, 0, 0, 0 };
+    auto part_bytes = [&](const vkgv_mesh_desc& m, int k) -> uint64_t {
+        switch (k) {
+        case 0: return 4ull * m.vertex_index_count;
+        case 1: return m.triangle_index_bytes;
+        case 2: return sizeof(vkgv_vertex) * m.vertex_count;
+        default: return sizeof(vkgv_meshlet) * (uint64_t)m.meshlet_count;
+        }
+    };
+    auto part_src = [&](const vkgv_mesh_desc& m, int k) -> const void* {
+        switch (k) {
+        case 0: return m.vertex_indices;
+        case 1: return m.triangle_indices;
+        case 2: return m.vertices;
+        default: return m.meshlets;
+        }
+    };
+    for (uint32_t mi = 0; mi < mesh_count; ++mi)
+        for (int k = 0; k < 4; ++k) {
+            off[4 * mi + k] = total[k];
+            total[k] += (part_bytes(meshes[mi], k) + 255ull) & ~255ull;
+        }
     int rc;
+    for (int k = 0; k < 4; ++k) {
+        std::vector<uint8_t> staging(total[k] ? total[k] : 1, 0);
+        for (uint32_t mi = 0; mi < mesh_count; ++mi)
+            if (part_bytes(meshes[mi], k)) std::memcpy(staging.data() + off[4 * mi + k], part_src(meshes[mi], k), part_bytes(meshes[mi], k));
+        if ((rc = upload(c, c->mesh_bufs[k], staging.data(), total[k]))) return rc;
+        HIP_TRY(c, hipStreamSynchronize(c->stream));   // staging is freed at the end of this iteration
+    }
     for (uint32_t mi = 0; mi < mesh_count; ++mi) {
         const vkgv_mesh_desc& m = meshes[mi];
-        if ((rc = upload(c, c->mesh_bufs[4 * mi + 0], m.vertex_indices, 4 * m.vertex_index_count))) return rc;
-        if ((rc = upload(c, c->mesh_bufs[4 * mi + 1], m.triangle_indices, m.triangle_index_bytes))) return rc;
-        if ((rc = upload(c, c->mesh_bufs[4 * mi + 2], m.vertices, sizeof(vkgv_vertex) * m.vertex_count))) return rc;
-        if ((rc = upload(c, c->mesh_bufs[4 * mi + 3], m.meshlets, sizeof(vkgv_meshlet) * m.meshlet_count))) return rc;
         c->mesh_meshlets[mi] = m.meshlet_count;
         if (m.cones) {
             cone_first[mi] = (uint32_t)all_cones.size();
@@ -278,10 +395,10 @@ int vkgv_upload_scene(vkgv_ctx* c, const vkgv_mesh_desc* meshes, uint32_t mesh_c
     for (uint32_t p = 0; p < prim_count; ++p) {
         const uint32_t mi = prims[p].mesh_index;
         vkgv_primitive& q = dp[p];
-        q.vertex_index_buffer = reinterpret_cast<uint64_t>(c->mesh_bufs[4 * mi + 0].p);
-        q.primitive_index_buffer = reinterpret_cast<uint64_t>(c->mesh_bufs[4 * mi + 1].p);
-        q.vertex_buffer = reinterpret_cast<uint64_t>(c->mesh_bufs[4 * mi + 2].p);
-        q.meshlet_buffer = reinterpret_cast<uint64_t>(c->mesh_bufs[4 * mi + 3].p);
+        q.vertex_index_buffer = reinterpret_cast<uint64_t>(c->mesh_bufs[0].p) + off[4 * mi + 0];
+        q.primitive_index_buffer = reinterpret_cast<uint64_t>(c->mesh_bufs[1].p) + off[4 * mi + 1];
+        q.vertex_buffer = reinterpret_cast<uint64_t>(c->mesh_bufs[2].p) + off[4 * mi + 2];
+        q.meshlet_buffer = reinterpret_cast<uint64_t>(c->mesh_bufs[3].p) + off[4 * mi + 3];
         std::memcpy(q.aabb_extents, prims[p].aabb_extents, 12);
         std::memcpy(q.aabb_center, prims[p].aabb_center, 12);
         q.meshlet_count = meshes[mi].meshlet_count;
@@ -381,26 +498,52 @@ int vkgv_view_create(vkgv_ctx* c, uint32_t width, uint32_t height, vkgv_view** o
     if (!rc) rc = dalloc(c, v->pyramid, 4 * (size_t)std::max(off, 1u));
     if (!rc) rc = dalloc(c, v->counters, sizeof(Counters));
     if (!rc) rc = dalloc(c, v->big, sizeof(BigTri) * (size_t)c->big_capacity);
-    if (!rc) rc = dalloc(c, v->big_start, sizeof(uint32_t) * (size_t)c->big_capacity);
+    if (!rc) rc = dalloc(c, v->big_chunk, sizeof(uint32_t) * (size_t)c->big_capacity);
+    const uint32_t tiles_x = (width + kBinTile - 1) / kBinTile, tiles_y = (height + kBinTile - 1) / kBinTile;
+    const uint32_t ntiles = tiles_x * tiles_y;
+    const uint32_t bin_cap = std::max<uint32_t>(1u << 21, 64u * ntiles);
+    if (!rc) rc = dalloc(c, v->clipq, sizeof(ClipTri) * (size_t)c->clip_capacity);
+    if (!rc) rc = dalloc(c, v->bin_count, sizeof(uint32_t) * ntiles);
+    if (!rc) rc = dalloc(c, v->bin_start, sizeof(uint32_t) * ntiles);
+    if (!rc) rc = dalloc(c, v->bins, sizeof(uint32_t) * (size_t)bin_cap);
+    if (!rc) rc = dalloc(c, v->setups, sizeof(BigSetup) * (size_t)c->big_capacity);
+    if (!rc) rc = dalloc(c, v->pairs, sizeof(BinPair) * (size_t)bin_cap);
+    if (!rc) rc = dalloc(c, v->tri_cache, sizeof(TriCache) * 128u * (size_t)c->cache_slots);
+    if (!rc) rc = dalloc(c, v->slot_albedo, sizeof(float4) * (size_t)c->cache_slots);
     if (rc) { vkgv_view_destroy(v); return rc; }
+    d.setups = static_cast<BigSetup*>(v->setups.p);
+    d.pairs = static_cast<BinPair*>(v->pairs.p);
+    d.tri_cache = static_cast<TriCache*>(v->tri_cache.p);
+    d.slot_albedo = static_cast<float4*>(v->slot_albedo.p);
+    d.cache_slots = c->cache_slots;
     d.keys = static_cast<unsigned long long*>(v->keys.p);
     d.color = static_cast<uint32_t*>(v->color.p);
     d.pyramid = static_cast<float*>(v->pyramid.p);
     d.counters = static_cast<Counters*>(v->counters.p);
     d.big = static_cast<BigTri*>(v->big.p);
-    d.big_tile_start = static_cast<uint32_t*>(v->big_start.p);
+    d.big_chunk = static_cast<uint32_t*>(v->big_chunk.p);
     d.big_capacity = c->big_capacity;
+    d.clipq = static_cast<ClipTri*>(v->clipq.p);
+    d.clip_capacity = c->clip_capacity;
+    d.tiles_x = tiles_x;
+    d.tiles_y = tiles_y;
+    d.bin_count = static_cast<uint32_t*>(v->bin_count.p);
+    d.bin_start = static_cast<uint32_t*>(v->bin_start.p);
+    d.bins = static_cast<uint32_t*>(v->bins.p);
+    d.bin_capacity = bin_cap;
     // first frame: pyramid = far (0.0), so the first cull is frustum-only
     // (the reference leaves it undefined, application.cpp:743); empty visbuffer.
     if (hipMemsetAsync(v->pyramid.p, 0, v->pyramid.bytes, c->stream) != hipSuccess
         || hipMemsetAsync(v->counters.p, 0, v->counters.bytes, c->stream) != hipSuccess
-        || hipMemsetAsync(v->color.p, 0, v->color.bytes, c->stream) != hipSuccess) {
+        || hipMemsetAsync(v->color.p, 0, v->color.bytes, c->stream) != hipSuccess
+        || hipMemsetAsync(v->bin_count.p, 0, v->bin_count.bytes, c->stream) != hipSuccess) {
         vkgv_view_destroy(v);
         return fail(c, VKGV_ERROR_HIP, "view_create: clear failed");
     }
     launch_clear_keys(d.keys, px, c->stream);
-    for (auto& e : v->ev)
-        if (hipEventCreate(&e) != hipSuccess) { vkgv_view_destroy(v); return fail(c, VKGV_ERROR_HIP, "hipEventCreate"); }
+    for (auto& set : v->ev)
+        for (auto& e : set)
+            if (hipEventCreate(&e) != hipSuccess) { vkgv_view_destroy(v); return fail(c, VKGV_ERROR_HIP, "hipEventCreate"); }
     if (hipStreamSynchronize(c->stream) != hipSuccess) { vkgv_view_destroy(v); return fail(c, VKGV_ERROR_HIP, "sync"); }
     *out = v;
     return VKGV_OK;
@@ -416,10 +559,12 @@ void vkgv_view_destroy(vkgv_view* v)
         if (v->last_stream) (void)hipStreamSynchronize(v->last_stream);
     }
     for (DevBuf* b : { &v->keys, &v->color, &v->pyramid, &v->visible0, &v->visible1, &v->history, &v->counters, &v->big,
-                       &v->big_start })
+                       &v->clipq, &v->bin_count, &v->bin_start, &v->bins, &v->setups, &v->pairs, &v->tri_cache,
+                       &v->slot_albedo, &v->draw_slot, &v->big_chunk })
         dfree(*b);
-    for (auto& e : v->ev)
-        if (e) (void)hipEventDestroy(e);
+    for (auto& set : v->ev)
+        for (auto& e : set)
+            if (e) (void)hipEventDestroy(e);
     delete v;
 }
 
@@ -442,7 +587,8 @@ int vkgv_render(vkgv_ctx* c, vkgv_view* v, const vkgv_frame_desc* f)
     if (v->last_stream && v->last_stream != st) HIP_TRY(c, hipStreamSynchronize(v->last_stream));
     v->last_stream = st;
     const bool timed = flags & VKGV_TIMING;
-    auto mark = [&](Ev e) { if (timed) (void)hipEventRecord(v->ev[e], st); };
+    const uint32_t slot = v->ring_next;
+    auto mark = [&](Ev e) { if (timed) (void)hipEventRecord(v->ev[slot][e], st); };
 
     const SceneDev s = scene_dev(c);
     CameraArg cam;
@@ -453,22 +599,22 @@ int vkgv_render(vkgv_ctx* c, vkgv_view* v, const vkgv_frame_desc* f)
     const uint64_t npx = (uint64_t)v->dev.width * v->dev.height;
 
     mark(EV_START);
-    launch_reset_counters(v->dev.counters, 0xFFFFFFFFu, st);
+    launch_reset_counters(v->dev, 0xFFFFFFFFu, st);
     launch_clear_keys(v->dev.keys, npx, st);
     mark(EV_CLEAR);
     launch_cull(s, v->dev, cam, flags, two_pass ? CULL_EARLY : CULL_SINGLE, 0, st);
     mark(EV_CULL);
     launch_raster(s, v->dev, vp, flags, 0, raster_grid, c->tuning.small_max_extent, st);
-    launch_raster_big(v->dev, c->tuning.big_blocks, st);
+    launch_raster_big(v->dev, st);
     mark(EV_RASTER);
     if (two_pass) {
         launch_hiz(v->dev, st);
         mark(EV_HIZ);
-        launch_reset_counters(v->dev.counters, 1u, st);
+        launch_reset_counters(v->dev, 1u, st);
         launch_cull(s, v->dev, cam, flags, CULL_LATE, 1, st);
         mark(EV_CULL_LATE);
         launch_raster(s, v->dev, vp, flags, 1, raster_grid, c->tuning.small_max_extent, st);
-        launch_raster_big(v->dev, c->tuning.big_blocks, st);
+        launch_raster_big(v->dev, st);
         mark(EV_RASTER_LATE);
     }
     if (!(flags & VKGV_SKIP_RESOLVE)) launch_resolve(s, v->dev, vp, flags, st);
@@ -476,6 +622,11 @@ int vkgv_render(vkgv_ctx* c, vkgv_view* v, const vkgv_frame_desc* f)
     if (!two_pass && (flags & VKGV_BUILD_PYRAMID)) launch_hiz(v->dev, st);
     mark(EV_PYRAMID);
     HIP_TRY(c, hipGetLastError());
+    if (timed) {
+        v->ring_flags[slot] = flags;
+        v->ring_next = (slot + 1) % vkgv_view::kRing;
+        v->ring_frames = std::min(v->ring_frames + 1, vkgv_view::kRing);
+    }
     v->timed = timed;
     v->last_flags = flags;
     return VKGV_OK;
@@ -628,30 +779,37 @@ int vkgv_view_read_stats(vkgv_ctx* c, vkgv_view* v, vkgv_frame_stats* s)
     s->visible_early = h.vis_count[0];
     s->visible_late = h.vis_count[1];
     s->big_triangles = h.big_total_tris;
-    s->big_tiles = h.big_total_tiles;
+    s->big_tiles = h.big_total_pairs;
     s->clipped_triangles = h.clipped;
     s->overflow = h.overflow;
-    if (v->timed) {
+    s->max_tile_tris = h.max_tile_tris;
+    // Per-pass milliseconds averaged over the timed frames recorded since
+    // the previous read_stats (at most the last 64).
+    const uint32_t n = v->ring_frames;
+    for (uint32_t f = 0; f < n; ++f) {
+        const uint32_t slot = (v->ring_next + vkgv_view::kRing - n + f) % vkgv_view::kRing;
         auto el = [&](Ev a, Ev b) {
             float ms = 0.0f;
-            (void)hipEventElapsedTime(&ms, v->ev[a], v->ev[b]);
-            return ms;
+            (void)hipEventElapsedTime(&ms, v->ev[slot][a], v->ev[slot][b]);
+            return ms / (float)n;
         };
-        const bool two = v->last_flags & VKGV_CULL_TWO_PASS;
-        s->ms_clear = el(EV_START, EV_CLEAR);
-        s->ms_cull = el(EV_CLEAR, EV_CULL);
-        s->ms_raster = el(EV_CULL, EV_RASTER);
+        const bool two = v->ring_flags[slot] & VKGV_CULL_TWO_PASS;
+        s->ms_clear += el(EV_START, EV_CLEAR);
+        s->ms_cull += el(EV_CLEAR, EV_CULL);
+        s->ms_raster += el(EV_CULL, EV_RASTER);
         if (two) {
-            s->ms_hiz = el(EV_RASTER, EV_HIZ);
-            s->ms_cull_late = el(EV_HIZ, EV_CULL_LATE);
-            s->ms_raster_late = el(EV_CULL_LATE, EV_RASTER_LATE);
-            s->ms_resolve = el(EV_RASTER_LATE, EV_RESOLVE);
+            s->ms_hiz += el(EV_RASTER, EV_HIZ);
+            s->ms_cull_late += el(EV_HIZ, EV_CULL_LATE);
+            s->ms_raster_late += el(EV_CULL_LATE, EV_RASTER_LATE);
+            s->ms_resolve += el(EV_RASTER_LATE, EV_RESOLVE);
         } else {
-            s->ms_resolve = el(EV_RASTER, EV_RESOLVE);
+            s->ms_resolve += el(EV_RASTER, EV_RESOLVE);
         }
-        s->ms_pyramid = el(EV_RESOLVE, EV_PYRAMID);
-        s->ms_total = el(EV_START, EV_PYRAMID);
+        s->ms_pyramid += el(EV_RESOLVE, EV_PYRAMID);
+        s->ms_total += el(EV_START, EV_PYRAMID);
     }
+    s->timed_frames = n;
+    v->ring_frames = 0;
     return VKGV_OK;
 }
 

@@ -188,15 +188,19 @@ __global__ __launch_bounds__(kBlock) void clear_keys_kernel(ulonglong2* keys, ui
     if (odd && blockIdx.x == 0 && threadIdx.x == 0) *tail = VKGV_KEY_CLEAR;
 }
 
-__global__ void reset_counters_kernel(Counters* c, uint32_t what)
+__global__ __launch_bounds__(kBlock) void reset_counters_kernel(Counters* c, uint32_t what, uint32_t* bin_count,
+                                                              uint32_t ntiles)
 {
-    const uint32_t t = threadIdx.x;
+    const uint32_t t = blockIdx.x * kBlock + threadIdx.x;
+    for (uint32_t i = t; i < ntiles; i += gridDim.x * kBlock) bin_count[i] = 0u;
+    if (blockIdx.x != 0) return;
     uint32_t* w = reinterpret_cast<uint32_t*>(c);
     if (what == 0xFFFFFFFFu) {
         if (t < sizeof(Counters) / 4) w[t] = 0;
     } else if (t == 0) {
-        // reset of the per-raster-pass big-triangle counter only
-        c->big = 0ull;
+        // per-raster-pass queues only
+        c->big_packed = 0ull;
+        c->clip_count = 0u;
     }
 }
 
@@ -224,9 +228,12 @@ void launch_clear_keys(unsigned long long* keys, uint64_t n, hipStream_t st)
                                                            (uint32_t)(n & 1));
 }
 
-void launch_reset_counters(Counters* c, uint32_t what, hipStream_t st)
+void launch_reset_counters(const ViewDev& v, uint32_t what, hipStream_t st)
 {
-    reset_counters_kernel<<<1, 64, 0, st>>>(c, what);
+    const uint32_t ntiles = v.tiles_x * v.tiles_y;
+    uint32_t blocks = (ntiles + kBlock - 1) / kBlock;
+    blocks = blocks > 64u ? 64u : (blocks ? blocks : 1u);
+    reset_counters_kernel<<<blocks, kBlock, 0, st>>>(v.counters, what, v.bin_count, ntiles);
 }
 
 void launch_cull(const SceneDev& s, const ViewDev& v, const CameraArg& cam, uint32_t flags, CullMode mode,

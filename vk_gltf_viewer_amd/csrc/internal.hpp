@@ -34,6 +34,18 @@ struct SceneDev {
     const uint32_t* prim_cone_base;      // per primitive: first index of its mesh's cones (~0u = none)
     const vkgv_meshlet_cone* cones;      // cones of all meshes, concatenated
     const CullRec* cull;                 // per draw
+    // sRGB encode + unorm8 tables (SrgbTables)
+    const void* srgb;
+};
+
+// The 8-bit code of fromLinear(c) is the number of the 255 ascending
+// thresholds <= c (the encode is monotonic).  Buckets of the float's top 16
+// bits hold at most one threshold each: code = base[bucket] + (c >= thr[base]).
+struct SrgbTables {
+    float thr[256];             // 255 thresholds, thr[255] = +inf
+    uint32_t key0;              // top 16 bits of thr[0]
+    uint32_t nbuckets;          // buckets from key0 to the top 16 bits of 1.0f
+    uint8_t base[2048];         // thresholds below each bucket
 };
 
 // Triangle handed to the tiled (large / clipped) path: snapped vertices,
@@ -47,19 +59,66 @@ struct BigTri {
 };
 static_assert(sizeof(BigTri) == 48, "BigTri layout");
 
+// Rasterization setup of a big triangle, computed once when it is binned.
+// Edge k runs from vertex (k+1)%3 to (k+2)%3 of the oriented triangle;
+// e0[k] is its edge function at the centre of the bounding box's first pixel
+// minus the top-left bias, so a pixel is covered iff every value is >= 0.
+struct BigSetup {
+    long long e0[3];
+    int ex[3], ey[3];
+    float za, zb, z0;
+    int x0, y0;             // vertex 0, 1/256 pixel
+    uint32_t id;
+    uint32_t bb0, bb1;      // px0 | py0 << 16, px1 | py1 << 16
+};
+static_assert(sizeof(BigSetup) == 80, "BigSetup layout");
+
+// A (screen tile, big triangle) pair found by the binning pass; tile ~0u
+// marks a bounding-box tile that the triangle cannot cover.
+struct BinPair {
+    uint32_t tile, tri;
+};
+
+// Data of one rasterized triangle kept for the resolve: with v_k the clip
+// (x, y, w) of vertex k, r[k] = v_{k+1} x v_{k+2} (the barycentric of vertex
+// k is proportional to r[k] . (xn, yn, 1)), and the packed RGBA8 colours.
+struct TriCache {
+    float r[3][3];
+    uint32_t col[3];
+};
+static_assert(sizeof(TriCache) == 48, "TriCache layout");
+
+// Triangle waiting for near-plane / guard-band clipping (clip-space vertices).
+struct ClipTri {
+    float4 c[3];
+    uint32_t id;
+    uint32_t pad[3];
+};
+static_assert(sizeof(ClipTri) == 64, "ClipTri layout");
+
 // Per-view frame counters in device memory.
 struct Counters {
+    // big triangles of the current raster pass: (count << 32) | 64-tile
+    // chunks reserved; one 64-bit atomicAdd gives a triangle both its slot
+    // and its first chunk, so chunk starts ascend with the slots
+    unsigned long long big_packed;
     uint32_t vis_count[2];          // pass 0 / late-pass visible lists
     uint32_t raster_cursor[2];      // meshlet work cursors
-    unsigned long long big;         // (triangles << 32) | tiles of the current raster pass
-    uint32_t clipped;
-    uint32_t overflow;
-    uint32_t hiz_done;
-    uint32_t big_total_tris;        // statistics over the frame
-    uint32_t big_total_tiles;
-    uint32_t pad[5];
+    uint32_t clip_count;            // triangles queued for clipping in the current pass
+    uint32_t bin_total;             // (tile, triangle) pairs binned in the current pass
+    uint32_t clipped;               // statistics over the frame
+    uint32_t overflow;              // bit 0 big queue, bit 1 clip queue, bit 2 bins
+    uint32_t bin_done;              // blocks of the binning kernel that finished
+    uint32_t big_total_tris;
+    uint32_t big_total_pairs;
+    uint32_t max_tile_tris;         // most big triangles binned to one tile this frame
+    uint32_t pad[2];
 };
 static_assert(sizeof(Counters) == 64, "Counters layout");
+
+constexpr uint32_t kTileSplit = 8;  // most blocks sharing one screen tile's triangle list
+
+constexpr uint32_t kBinTile = 32;   // screen tile of the binned path, pixels
 
 struct ViewDev {
     uint32_t width, height;
@@ -70,9 +129,24 @@ struct ViewDev {
     uint32_t* visible[2];           // compacted draw lists
     uint32_t* history;              // two-pass visibility bits, ceil(draws/64)*2 words
     Counters* counters;
-    BigTri* big;
-    uint32_t* big_tile_start;
+    BigTri* big;                    // big triangles of the current raster pass
+    uint32_t* big_chunk;            // first 64-tile chunk of each big triangle
     uint32_t big_capacity;
+    ClipTri* clipq;
+    uint32_t clip_capacity;
+    uint32_t tiles_x, tiles_y;      // kBinTile screen tiles
+    uint32_t* bin_count;            // per tile; zero between passes
+    uint32_t* bin_start;            // per tile
+    uint32_t* bins;                 // big-triangle indices, grouped by tile
+    uint32_t bin_capacity;          // also the capacity of `pairs` (64 per chunk)
+    BigSetup* setups;               // per big triangle
+    BinPair* pairs;
+    // resolve cache: triangle t of the meshlet in visible-list slot s is
+    // tri_cache[s * 128 + t]; draw_slot[d] is the slot of draw d this frame
+    TriCache* tri_cache;
+    float4* slot_albedo;            // albedo factor of each slot's material
+    uint32_t* draw_slot;
+    uint32_t cache_slots;
 };
 
 // Camera as kernels take it (a by-value copy of vkgv_camera).
@@ -86,12 +160,15 @@ enum CullMode : uint32_t { CULL_SINGLE = 0, CULL_EARLY = 1, CULL_LATE = 2 };
 // ---- launchers (stream-ordered, no host synchronization) -----------------
 void launch_build_cull_records(const SceneDev& s, CullRec* out, hipStream_t st);
 void launch_clear_keys(unsigned long long* keys, uint64_t n, hipStream_t st);
-void launch_reset_counters(Counters* c, uint32_t what, hipStream_t st);
+// what = ~0u: all counters (frame start); 1: the per-raster-pass queues.
+// Both also zero the per-tile bin counts.
+void launch_reset_counters(const ViewDev& v, uint32_t what, hipStream_t st);
 void launch_cull(const SceneDev& s, const ViewDev& v, const CameraArg& cam, uint32_t flags, CullMode mode,
                  uint32_t list, hipStream_t st);
 void launch_raster(const SceneDev& s, const ViewDev& v, const Mat4& vp, uint32_t flags, uint32_t list,
                    uint32_t grid_blocks, uint32_t small_max_extent, hipStream_t st);
-void launch_raster_big(const ViewDev& v, uint32_t grid_blocks, hipStream_t st);
+// Clipping, binning and tiled rasterization of the big-triangle queue.
+void launch_raster_big(const ViewDev& v, hipStream_t st);
 void launch_hiz(const ViewDev& v, hipStream_t st);
 void launch_resolve(const SceneDev& s, const ViewDev& v, const Mat4& vp, uint32_t flags, hipStream_t st);
 void launch_write_depth_keys(unsigned long long* keys, const float* depth, uint64_t n, hipStream_t st);
@@ -100,7 +177,6 @@ void launch_write_depth_keys(unsigned long long* keys, const float* depth, uint6
 struct RasterTuning {
     uint32_t small_max_extent;      // triangles whose bbox extent is below this (pixels) stay in the meshlet wave
     uint32_t raster_blocks_per_cu;  // persistent raster grid = CUs * this
-    uint32_t big_blocks;            // tiled-path grid size
 };
 
 } // namespace vkgv

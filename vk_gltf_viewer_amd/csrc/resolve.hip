@@ -2,28 +2,38 @@
 //
 // Per pixel: key -> id -> draw -> primitive -> material; sRGB-encode the
 // albedo factor (alpha passes through) to RGBA8; the clear key gives (0,0,0,0).
-// With VKGV_RESOLVE_BARY the triangle is re-fetched from the id, its vertices
-// re-transformed, and perspective-correct barycentrics recomputed at the pixel
-// centre (2D homogeneous form, valid for clipped triangles too) to interpolate
-// the vertex colour, which multiplies the albedo factor.
+// With VKGV_RESOLVE_BARY perspective-correct barycentrics are recomputed at
+// the pixel centre from the triangle's clip-space vertices (2D homogeneous
+// form, valid for clipped triangles too) to interpolate the vertex colour,
+// which multiplies the albedo factor.
+//
+// The raster leaves each rasterized meshlet's albedo and, per triangle, the
+// cross products of its clip-space (x, y, w) vertices and its vertex colours
+// in a per-view cache indexed by the meshlet's slot in the visible list, so a
+// pixel needs key -> slot -> triangle (three dependent loads); meshlets beyond
+// the cache capacity are re-fetched and re-transformed from the scene buffers
+// (the same arithmetic).  sRGB encode + unorm8 is a two-level table lookup
+// (exact for the 8-bit result).
 #include "internal.hpp"
 
 namespace vkgv {
 namespace {
-
-__device__ __forceinline__ float from_linear(float c)
-{
-    const bool cutoff = c < 0.0031308f;
-    const float higher = 1.055f * powf(c, 1.0f / 2.4f) - 0.055f;
-    const float lower = c * 12.92f;
-    return cutoff ? lower : higher;
-}
 
 __device__ __forceinline__ uint32_t unorm8(float x)
 {
     float v = gclamp(x, 0.0f, 1.0f);
     if (!(v >= 0.0f)) v = 0.0f;
     return (uint32_t)(v * 255.0f + 0.5f);
+}
+
+// unorm8(fromLinear(c)): 0 below the first threshold (and for NaN), 255 from
+// 1.0 up, else base[bucket] plus the one threshold the bucket may hold.
+__device__ __forceinline__ uint32_t srgb_code(const float* thr, const uint8_t* base, uint32_t key0, float c)
+{
+    if (!(c >= thr[0])) return 0u;
+    if (c >= 1.0f) return 255u;
+    const uint32_t k = base[(__float_as_uint(c) >> 16) - key0];
+    return k + (c >= thr[k] ? 1u : 0u);
 }
 
 struct ResolveArgs {
@@ -33,6 +43,9 @@ struct ResolveArgs {
     uint32_t* color;
     uint32_t W, H;
     uint32_t flags;
+    const uint32_t* draw_slot;
+    const TriCache* tri_cache;
+    const float4* slot_albedo;
 };
 
 __device__ __forceinline__ float3 cross3(float3 a, float3 b)
@@ -42,6 +55,19 @@ __device__ __forceinline__ float3 cross3(float3 a, float3 b)
 
 __global__ __launch_bounds__(256) void resolve_kernel(ResolveArgs a)
 {
+    // LDS tables: sRGB thresholds and buckets, exact quotients b / 255 of a byte
+    __shared__ float s_thr[256];
+    __shared__ float s_byte[256];
+    __shared__ uint32_t s_base[512];
+    const SrgbTables* tab = static_cast<const SrgbTables*>(a.s.srgb);
+    s_thr[threadIdx.x] = tab->thr[threadIdx.x];
+    s_byte[threadIdx.x] = (float)threadIdx.x / 255.0f;
+    s_base[threadIdx.x] = reinterpret_cast<const uint32_t*>(tab->base)[threadIdx.x];
+    s_base[threadIdx.x + 256] = reinterpret_cast<const uint32_t*>(tab->base)[threadIdx.x + 256];
+    const uint32_t key0 = tab->key0;
+    __syncthreads();
+    const uint8_t* base = reinterpret_cast<const uint8_t*>(s_base);
+
     const uint32_t px = blockIdx.x * 64u + (threadIdx.x & 63u);
     const uint32_t py = blockIdx.y * 4u + (threadIdx.x >> 6);
     if (px >= a.W || py >= a.H) return;
@@ -51,42 +77,71 @@ __global__ __launch_bounds__(256) void resolve_kernel(ResolveArgs a)
     const uint32_t id = key_id(key);
     const uint32_t d = id >> VKGV_TRIANGLE_BITS, t = id & ((1u << VKGV_TRIANGLE_BITS) - 1u);
     if (d >= a.s.draw_count) { a.color[i] = 0u; return; }   // keys not written by this draw list
-    const vkgv_meshlet_draw dr = a.s.draws[d];
-    const vkgv_primitive& prim = a.s.prims[dr.primitive_index];
-    const vkgv_material& mat = a.s.materials[prim.material_index];
-    float c[4] = { mat.albedo_factor[0], mat.albedo_factor[1], mat.albedo_factor[2], mat.albedo_factor[3] };
-    if (a.flags & VKGV_RESOLVE_BARY) {
-        const vkgv_meshlet ml = reinterpret_cast<const vkgv_meshlet*>(prim.meshlet_buffer)[dr.meshlet_index];
-        const Mat4 mvp = mat4_mul(a.vp, load_mat4(a.s.transforms + 16ull * dr.transform_index));
-        const uint8_t* tri = reinterpret_cast<const uint8_t*>(prim.primitive_index_buffer) + ml.triangle_offset + 3u * t;
-        const uint32_t* vidx = reinterpret_cast<const uint32_t*>(prim.vertex_index_buffer) + ml.vertex_offset;
-        const vkgv_vertex* verts = reinterpret_cast<const vkgv_vertex*>(prim.vertex_buffer);
-        float3 hv[3];
-        float col[3][4];
+    const uint32_t slot = a.draw_slot[d];
+    const bool bary = (a.flags & VKGV_RESOLVE_BARY) != 0;
+    float c[4];
+    float3 r[3];
+    uint32_t col[3];
+    if (slot != 0xFFFFFFFFu) {
+        // the raster kept this meshlet's albedo and triangles
+        const float4 alb = a.slot_albedo[slot];
+        c[0] = alb.x; c[1] = alb.y; c[2] = alb.z; c[3] = alb.w;
+        if (bary) {
+            const TriCache tc = a.tri_cache[(size_t)slot * 128u + t];
 #pragma unroll
-        for (int k = 0; k < 3; ++k) {
-            const vkgv_vertex& vx = verts[vidx[tri[k]]];
-            const float4 cl = mat4_point(mvp, vx.position[0], vx.position[1], vx.position[2]);
-            hv[k] = make_float3(cl.x, cl.y, cl.w);
-            const uint32_t packed = *reinterpret_cast<const uint32_t*>(vx.color);
-#pragma unroll
-            for (int j = 0; j < 4; ++j) col[k][j] = (float)((packed >> (8 * j)) & 0xFFu) / 255.0f;
+            for (int k = 0; k < 3; ++k) {
+                r[k] = make_float3(tc.r[k][0], tc.r[k][1], tc.r[k][2]);
+                col[k] = tc.col[k];
+            }
         }
-        const float hw = (float)a.W * 0.5f, hh = (float)a.H * 0.5f;
-        const float xn = (((float)px + 0.5f) - hw) / hw;
-        const float yn = (((float)py + 0.5f) - hh) / hh;
-        const float3 r0 = cross3(hv[1], hv[2]), r1 = cross3(hv[2], hv[0]), r2 = cross3(hv[0], hv[1]);
-        float l0 = (r0.x * xn + r0.y * yn) + r0.z;
-        float l1 = (r1.x * xn + r1.y * yn) + r1.z;
-        float l2 = (r2.x * xn + r2.y * yn) + r2.z;
-        const float sum = (l0 + l1) + l2;
-        l0 = l0 / sum; l1 = l1 / sum; l2 = l2 / sum;
+    } else {
+        const vkgv_meshlet_draw dr = a.s.draws[d];
+        const vkgv_primitive& prim = a.s.prims[dr.primitive_index];
+        const vkgv_material& mat = a.s.materials[prim.material_index];
+        c[0] = mat.albedo_factor[0]; c[1] = mat.albedo_factor[1]; c[2] = mat.albedo_factor[2]; c[3] = mat.albedo_factor[3];
+        if (bary) {
+            const vkgv_meshlet ml = reinterpret_cast<const vkgv_meshlet*>(prim.meshlet_buffer)[dr.meshlet_index];
+            const Mat4 mvp = mat4_mul(a.vp, load_mat4(a.s.transforms + 16ull * dr.transform_index));
+            const uint8_t* tri = reinterpret_cast<const uint8_t*>(prim.primitive_index_buffer) + ml.triangle_offset + 3u * t;
+            const uint32_t* vidx = reinterpret_cast<const uint32_t*>(prim.vertex_index_buffer) + ml.vertex_offset;
+            const vkgv_vertex* verts = reinterpret_cast<const vkgv_vertex*>(prim.vertex_buffer);
+            float3 hv[3];
 #pragma unroll
-        for (int j = 0; j < 4; ++j) c[j] = c[j] * ((l0 * col[0][j] + l1 * col[1][j]) + l2 * col[2][j]);
+            for (int k = 0; k < 3; ++k) {
+                const vkgv_vertex& vx = verts[vidx[tri[k]]];
+                const float4 cl = mat4_point(mvp, vx.position[0], vx.position[1], vx.position[2]);
+                hv[k] = make_float3(cl.x, cl.y, cl.w);
+                col[k] = *reinterpret_cast<const uint32_t*>(vx.color);
+            }
+            r[0] = cross3(hv[1], hv[2]);
+            r[1] = cross3(hv[2], hv[0]);
+            r[2] = cross3(hv[0], hv[1]);
+        }
     }
-    const uint32_t r8 = unorm8(from_linear(c[0])), g8 = unorm8(from_linear(c[1])), b8 = unorm8(from_linear(c[2]));
-    const uint32_t a8 = unorm8(c[3]);
-    a.color[i] = r8 | (g8 << 8) | (b8 << 16) | (a8 << 24);
+    if (bary) {
+        // perspective-correct barycentrics at the pixel centre:
+        // lambda_k ~ r[k] . (xn, yn, 1), normalized; c *= interpolated colour
+        const float hw = (float)a.W * 0.5f, hh = (float)a.H * 0.5f;
+        const float xn = (((float)px + 0.5f) - hw) * (1.0f / hw);
+        const float yn = (((float)py + 0.5f) - hh) * (1.0f / hh);
+        float l[3];
+#pragma unroll
+        for (int k = 0; k < 3; ++k) l[k] = (r[k].x * xn + r[k].y * yn) + r[k].z;
+        const float inv = 1.0f / ((l[0] + l[1]) + l[2]);
+#pragma unroll
+        for (int k = 0; k < 3; ++k) l[k] = l[k] * inv;
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            const float c0 = s_byte[(col[0] >> (8 * j)) & 0xFFu];
+            const float c1 = s_byte[(col[1] >> (8 * j)) & 0xFFu];
+            const float c2 = s_byte[(col[2] >> (8 * j)) & 0xFFu];
+            c[j] = c[j] * ((l[0] * c0 + l[1] * c1) + l[2] * c2);
+        }
+    }
+    const uint32_t r8 = srgb_code(s_thr, base, key0, c[0]);
+    const uint32_t g8 = srgb_code(s_thr, base, key0, c[1]);
+    const uint32_t b8 = srgb_code(s_thr, base, key0, c[2]);
+    a.color[i] = r8 | (g8 << 8) | (b8 << 16) | (unorm8(c[3]) << 24);
 }
 
 } // namespace
@@ -101,6 +156,9 @@ void launch_resolve(const SceneDev& s, const ViewDev& v, const Mat4& vp, uint32_
     a.W = v.width;
     a.H = v.height;
     a.flags = flags;
+    a.draw_slot = v.draw_slot;
+    a.tri_cache = v.tri_cache;
+    a.slot_albedo = v.slot_albedo;
     const dim3 grid((v.width + 63) / 64, (v.height + 3) / 4);
     resolve_kernel<<<grid, 256, 0, st>>>(a);
 }

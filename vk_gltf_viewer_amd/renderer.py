@@ -87,6 +87,14 @@ class Renderer:
     def sync(self):
         self._check(self._lib.vkgv_sync(self._h))
 
+    def set_tuning(self, small_max_extent: int, raster_blocks_per_cu: int):
+        self._check(self._lib.vkgv_set_tuning(self._h, C.byref(N.Tuning(small_max_extent, raster_blocks_per_cu))))
+
+    def get_tuning(self):
+        t = N.Tuning()
+        self._check(self._lib.vkgv_get_tuning(self._h, C.byref(t)))
+        return t.small_max_extent, t.raster_blocks_per_cu
+
     def create_view(self, width: int, height: int) -> "View":
         h = C.c_void_p()
         self._check(self._lib.vkgv_view_create(self._h, width, height, C.byref(h)))
