@@ -64,7 +64,7 @@ struct vkgv_view {
     vkgv_ctx* ctx = nullptr;
     ViewDev dev{};
     DevBuf keys, color, pyramid, visible0, visible1, history, counters, big, clipq, bin_count, bin_start, bins;
-    DevBuf setups, pairs, tri_cache, slot_albedo, draw_slot, big_chunk;
+    DevBuf setups, pairs, tri_cache, slot_albedo, draw_slot, big_chunk, chunk_tri, tile_items;
     uint32_t list_capacity = 0;
     // Ring of per-frame event sets: timed frames are averaged at the next
     // read_stats without synchronizing between frames.
@@ -510,7 +510,12 @@ int vkgv_view_create(vkgv_ctx* c, uint32_t width, uint32_t height, vkgv_view** o
     if (!rc) rc = dalloc(c, v->pairs, sizeof(BinPair) * (size_t)bin_cap);
     if (!rc) rc = dalloc(c, v->tri_cache, sizeof(TriCache) * 128u * (size_t)c->cache_slots);
     if (!rc) rc = dalloc(c, v->slot_albedo, sizeof(float4) * (size_t)c->cache_slots);
+    if (!rc) rc = dalloc(c, v->chunk_tri, sizeof(uint32_t) * (size_t)(bin_cap / 64u));
+    if (!rc) rc = dalloc(c, v->tile_items, sizeof(TileItem) * (size_t)ntiles * kTileSplit);
     if (rc) { vkgv_view_destroy(v); return rc; }
+    d.chunk_tri = static_cast<uint32_t*>(v->chunk_tri.p);
+    d.tile_items = static_cast<TileItem*>(v->tile_items.p);
+    d.tile_grid = std::max<uint32_t>(1u, (uint32_t)c->cus) * 8u;
     d.setups = static_cast<BigSetup*>(v->setups.p);
     d.pairs = static_cast<BinPair*>(v->pairs.p);
     d.tri_cache = static_cast<TriCache*>(v->tri_cache.p);
@@ -560,7 +565,7 @@ void vkgv_view_destroy(vkgv_view* v)
     }
     for (DevBuf* b : { &v->keys, &v->color, &v->pyramid, &v->visible0, &v->visible1, &v->history, &v->counters, &v->big,
                        &v->clipq, &v->bin_count, &v->bin_start, &v->bins, &v->setups, &v->pairs, &v->tri_cache,
-                       &v->slot_albedo, &v->draw_slot, &v->big_chunk })
+                       &v->slot_albedo, &v->draw_slot, &v->big_chunk, &v->chunk_tri, &v->tile_items })
         dfree(*b);
     for (auto& set : v->ev)
         for (auto& e : set)
@@ -610,7 +615,6 @@ int vkgv_render(vkgv_ctx* c, vkgv_view* v, const vkgv_frame_desc* f)
     if (two_pass) {
         launch_hiz(v->dev, st);
         mark(EV_HIZ);
-        launch_reset_counters(v->dev, 1u, st);
         launch_cull(s, v->dev, cam, flags, CULL_LATE, 1, st);
         mark(EV_CULL_LATE);
         launch_raster(s, v->dev, vp, flags, 1, raster_grid, c->tuning.small_max_extent, st);

@@ -79,6 +79,14 @@ struct BinPair {
     uint32_t tile, tri;
 };
 
+// Work item of the tile pass: part `part` of `split` of screen tile `tile`
+// rasterizes triangles part, part + split, ... of the tile's list
+// bins[start .. start + n).  code = tile | part << 24 | (split - 1) << 28.
+struct TileItem {
+    uint32_t code, start, n, pad;
+};
+static_assert(sizeof(TileItem) == 16, "TileItem layout");
+
 // Data of one rasterized triangle kept for the resolve: with v_k the clip
 // (x, y, w) of vertex k, r[k] = v_{k+1} x v_{k+2} (the barycentric of vertex
 // k is proportional to r[k] . (xn, yn, 1)), and the packed RGBA8 colours.
@@ -103,20 +111,22 @@ struct Counters {
     // and its first chunk, so chunk starts ascend with the slots
     unsigned long long big_packed;
     uint32_t vis_count[2];          // pass 0 / late-pass visible lists
-    uint32_t raster_cursor[2];      // meshlet work cursors
+    uint32_t reserved[2];
     uint32_t clip_count;            // triangles queued for clipping in the current pass
     uint32_t bin_total;             // (tile, triangle) pairs binned in the current pass
     uint32_t clipped;               // statistics over the frame
     uint32_t overflow;              // bit 0 big queue, bit 1 clip queue, bit 2 bins
-    uint32_t bin_done;              // blocks of the binning kernel that finished
+    uint32_t reserved2;
     uint32_t big_total_tris;
     uint32_t big_total_pairs;
     uint32_t max_tile_tris;         // most big triangles binned to one tile this frame
-    uint32_t pad[2];
+    uint32_t tile_items;            // work items of the tile pass (TileItem)
+    uint32_t pad;
 };
 static_assert(sizeof(Counters) == 64, "Counters layout");
 
-constexpr uint32_t kTileSplit = 8;  // most blocks sharing one screen tile's triangle list
+constexpr uint32_t kTileSplit = 8;      // most work items sharing one screen tile's triangle list
+constexpr uint32_t kTileItemTris = 16;  // triangles per work item before a tile is split
 
 constexpr uint32_t kBinTile = 32;   // screen tile of the binned path, pixels
 
@@ -131,6 +141,7 @@ struct ViewDev {
     Counters* counters;
     BigTri* big;                    // big triangles of the current raster pass
     uint32_t* big_chunk;            // first 64-tile chunk of each big triangle
+    uint32_t* chunk_tri;            // big triangle of each 64-tile chunk
     uint32_t big_capacity;
     ClipTri* clipq;
     uint32_t clip_capacity;
@@ -139,6 +150,8 @@ struct ViewDev {
     uint32_t* bin_start;            // per tile
     uint32_t* bins;                 // big-triangle indices, grouped by tile
     uint32_t bin_capacity;          // also the capacity of `pairs` (64 per chunk)
+    TileItem* tile_items;           // tiles * kTileSplit
+    uint32_t tile_grid;             // blocks of the tile pass (a multiple of the CU count)
     BigSetup* setups;               // per big triangle
     BinPair* pairs;
     // resolve cache: triangle t of the meshlet in visible-list slot s is

@@ -14,26 +14,25 @@
 //    GREATER_OR_EQUAL with ties to the later primitive.
 //
 // Work split.
-//  raster_kernel  persistent grid of 4-wave blocks; each wave takes one visible
-//                 meshlet at a time (the first by its wave index, the rest from
-//                 an atomic cursor), transforms its <= 64 vertices into LDS,
-//                 back-face culls its <= 124 triangles (2 per lane) and
+//  raster_kernel  persistent grid of 4-wave blocks; wave g takes visible
+//                 meshlets g, g + waves, ..., transforms its <= 64 vertices into
+//                 LDS, back-face culls its <= 124 triangles (2 per lane) and
 //                 rasterizes the small ones cooperatively: a wave prefix sum
-//                 over their bounding-box pixel counts spreads the pixels evenly
-//                 over the 64 lanes.  Large triangles go to the big queue (one
+//                 over their bounding-box pixel counts gives each lane an equal
+//                 run of pixel slots, walked with integer edge steps.  Large triangles go to the big queue (one
 //                 64-bit atomic reserves the slot and the 64-tile chunks of the
 //                 triangle's 32x32-pixel bounding-box tiles), triangles needing
 //                 clipping to the clip queue.
 //  clip_kernel    clips the queued triangles; their fans join the big queue.
 //  bin_kernel     one wave per chunk: stores the triangle's setup (chunk 0),
 //                 tests 64 tiles, writes a (tile, triangle) pair per tile that
-//                 can hold a covered pixel centre and counts it in its tile; the
-//                 last block to finish scans the per-tile counts.
+//                 can hold a covered pixel centre and counts it in its tile.
+//  bin_scan       one block: scans the per-tile counts and cuts each tile's
+//                 list into up to kTileSplit work items.
 //  bin_fill       scatters the pairs into per-tile triangle lists.
-//  tile_kernel    kTileSplit blocks per screen tile, each rasterizing its share
-//                 of the tile's triangles into a 32x32 key tile in LDS (LDS
-//                 atomicMin), then merging it into the visibility buffer with
-//                 one global atomicMin per covered pixel.
+//  tile_kernel    grid-stride loop over the work items, each rasterizing its
+//                 share of a tile's triangles into a 32x32 key tile in LDS (LDS
+//                 atomicMin), then merging it into the visibility buffer.
 #include "internal.hpp"
 
 namespace vkgv {
@@ -67,7 +66,6 @@ struct RasterArgs {
     Mat4 vp;
     const uint32_t* list;
     const uint32_t* list_count;
-    uint32_t* cursor;
     unsigned long long* keys;
     uint32_t W, H;
     uint32_t flags;
@@ -75,6 +73,8 @@ struct RasterArgs {
     BigTri* big;
     uint32_t* big_chunk;
     uint32_t big_cap;
+    uint32_t* chunk_tri;
+    uint32_t chunk_cap;
     ClipTri* clipq;
     uint32_t clip_cap;
     Counters* ctr;
@@ -141,25 +141,31 @@ __device__ __forceinline__ void key_min(unsigned long long* p, unsigned long lon
     (void)__hip_atomic_fetch_min(p, key, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
+// No early depth read: the wave does not wait on memory inside the pixel loop.
 __device__ __forceinline__ void write_fragment(unsigned long long* keys, uint32_t W, int px, int py, float z, uint32_t id)
 {
-    const unsigned long long key = make_key(z, id);
-    unsigned long long* p = keys + (size_t)py * W + px;
-    if (key < *p) key_min(p, key);
+    key_min(keys + (size_t)py * W + px, make_key(z, id));
 }
 
-// Appends a big triangle: one 64-bit atomic gives its slot (high word) and the
-// first of the 64-tile chunks its bounding box needs (low word).
-__device__ __forceinline__ void push_big(const TriSetup& t, uint32_t id, BigTri* big, uint32_t* big_chunk, uint32_t cap,
-                                         Counters* ctr)
+__device__ __forceinline__ uint32_t big_chunks(const TriSetup& t)
 {
     const uint32_t ntx = (uint32_t)(t.px1 / (int)kBinTile - t.px0 / (int)kBinTile + 1);
     const uint32_t nty = (uint32_t)(t.py1 / (int)kBinTile - t.py0 / (int)kBinTile + 1);
-    const uint32_t chunks = (ntx * nty + 63u) >> 6;
-    const unsigned long long packed = atomicAdd(&ctr->big_packed, (1ull << 32) | (unsigned long long)chunks);
+    return (ntx * nty + 63u) >> 6;
+}
+
+// Stores a big triangle at the reservation `packed` = slot << 32 | first
+// chunk, and marks its chunks as the triangle's in chunk_tri.
+__device__ __forceinline__ void store_big(const TriSetup& t, uint32_t id, unsigned long long packed, uint32_t chunks,
+                                          BigTri* big, uint32_t* big_chunk, uint32_t cap, uint32_t* chunk_tri,
+                                          uint32_t chunk_cap, Counters* ctr)
+{
     const uint32_t slot = (uint32_t)(packed >> 32);
     if (slot >= cap) { atomicOr(&ctr->overflow, 1u); return; }
-    big_chunk[slot] = (uint32_t)packed;
+    const uint32_t c0 = (uint32_t)packed;
+    big_chunk[slot] = c0;
+    // past chunk_cap the pass overflows its pair list and is dropped whole
+    for (uint32_t q = 0; q < chunks && c0 + q < chunk_cap; ++q) chunk_tri[c0 + q] = slot;
     BigTri b;
 #pragma unroll
     for (int k = 0; k < 3; ++k) { b.x[k] = t.x[k]; b.y[k] = t.y[k]; b.z[k] = t.z[k]; }
@@ -167,6 +173,40 @@ __device__ __forceinline__ void push_big(const TriSetup& t, uint32_t id, BigTri*
     b.bb0 = (uint32_t)t.px0 | ((uint32_t)t.py0 << 16);
     b.bb1 = (uint32_t)t.px1 | ((uint32_t)t.py1 << 16);
     big[slot] = b;
+}
+
+// Appends one big triangle: one 64-bit atomic gives its slot (high word) and
+// the first of the 64-tile chunks its bounding box needs (low word), so chunk
+// starts ascend with the slots.
+__device__ __forceinline__ void push_big(const TriSetup& t, uint32_t id, BigTri* big, uint32_t* big_chunk, uint32_t cap,
+                                         uint32_t* chunk_tri, uint32_t chunk_cap, Counters* ctr)
+{
+    const uint32_t chunks = big_chunks(t);
+    const unsigned long long packed = atomicAdd(&ctr->big_packed, (1ull << 32) | (unsigned long long)chunks);
+    store_big(t, id, packed, chunks, big, big_chunk, cap, chunk_tri, chunk_cap, ctr);
+}
+
+// Wave-wide append (called by all 64 lanes): a prefix sum of the lanes'
+// (1, chunks) pairs and one atomic per wave reserve consecutive slots and
+// chunks in lane order.
+__device__ __forceinline__ void push_big_wave(bool want, const TriSetup& t, uint32_t id, uint32_t lane, BigTri* big,
+                                              uint32_t* big_chunk, uint32_t cap, uint32_t* chunk_tri, uint32_t chunk_cap,
+                                              Counters* ctr)
+{
+    const uint32_t chunks = want ? big_chunks(t) : 0u;
+    const unsigned long long mine = want ? ((1ull << 32) | (unsigned long long)chunks) : 0ull;
+    unsigned long long x = mine;
+#pragma unroll
+    for (int off = 1; off < 64; off <<= 1) {
+        const unsigned long long y = __shfl_up(x, off);
+        if (lane >= (uint32_t)off) x += y;
+    }
+    const unsigned long long total = __shfl(x, 63);
+    if (total == 0ull) return;   // wave-uniform
+    unsigned long long base = 0ull;
+    if (lane == 0) base = atomicAdd(&ctr->big_packed, total);
+    base = __shfl(base, 0);
+    if (want) store_big(t, id, base + (x - mine), chunks, big, big_chunk, cap, chunk_tri, chunk_cap, ctr);
 }
 
 __device__ __forceinline__ uint32_t tri_byte(const uint32_t* words, uint32_t b)
@@ -191,7 +231,7 @@ __global__ __launch_bounds__(kBlock) void raster_kernel(RasterArgs a)
     const float hw = (float)a.W * 0.5f, hh = (float)a.H * 0.5f;
     const int W = (int)a.W, H = (int)a.H;
 
-    uint32_t idx = blockIdx.x * kWaves + (uint32_t)w;   // first meshlet: static
+    uint32_t idx = blockIdx.x * kWaves + (uint32_t)w;
     while (idx < count) {
         const uint32_t d = __builtin_amdgcn_readfirstlane(a.list[idx]);
         const vkgv_meshlet_draw dr = a.s.draws[d];
@@ -236,71 +276,76 @@ __global__ __launch_bounds__(kBlock) void raster_kernel(RasterArgs a)
 #pragma unroll
         for (int sl = 0; sl < 2; ++sl) {
             const uint32_t t = lane + 64u * (uint32_t)sl;
-            if (t >= tc) continue;
-            const uint32_t i0 = tri_byte(s_tri[w], 3u * t), i1 = tri_byte(s_tri[w], 3u * t + 1u),
-                           i2 = tri_byte(s_tri[w], 3u * t + 2u);
-            const float4 c0 = s_clip[w][i0], c1 = s_clip[w][i1], c2 = s_clip[w][i2];
-            if (cached) {
-                // r[k] = v_{k+1} x v_{k+2} over the clip (x, y, w) of the vertices
-                TriCache tcache;
-                const float4 cv[3] = { c0, c1, c2 };
-#pragma unroll
-                for (int k = 0; k < 3; ++k) {
-                    const float4 p = cv[(k + 1) % 3], q = cv[(k + 2) % 3];
-                    tcache.r[k][0] = p.y * q.w - p.w * q.y;
-                    tcache.r[k][1] = p.w * q.x - p.x * q.w;
-                    tcache.r[k][2] = p.x * q.y - p.y * q.x;
-                }
-                tcache.col[0] = s_col[w][i0]; tcache.col[1] = s_col[w][i1]; tcache.col[2] = s_col[w][i2];
-                a.tri_cache[(size_t)slot * 128u + t] = tcache;
-            }
-            if (cull_faces) {
-                const float det = det3_xyw(c0, c1, c2);
-                const bool culled = tdet < 0.0f ? det < 0.0f : det > 0.0f;
-                if (culled) continue;
-            }
-            const uint32_t id = (d << VKGV_TRIANGLE_BITS) | t;
-            const SVtx v0 = s_sv[w][i0], v1 = s_sv[w][i1], v2 = s_sv[w][i2];
-            if (!(v0.ok & v1.ok & v2.ok)) {
-                const uint32_t slot = atomicAdd(&a.ctr->clip_count, 1u);
-                if (slot < a.clip_cap) {
-                    ClipTri ct;
-                    ct.c[0] = c0; ct.c[1] = c1; ct.c[2] = c2;
-                    ct.id = id;
-                    ct.pad[0] = ct.pad[1] = ct.pad[2] = 0u;
-                    a.clipq[slot] = ct;
-                } else {
-                    atomicOr(&a.ctr->overflow, 2u);
-                }
-                continue;
-            }
+            bool want_big = false;
             TriSetup ts;
-            if (!setup_tri(v0, v1, v2, W, H, ts)) continue;
-            if (ts.maxx - ts.minx < a.small_lim && ts.maxy - ts.miny < a.small_lim) {
-                SmallTri st;
-                const int sx0 = ts.px0 * 256 + 128, sy0 = ts.py0 * 256 + 128;
+            uint32_t id = 0;
+            do {   // per-lane triangle work; `continue` leaves it
+                if (t >= tc) continue;
+                const uint32_t i0 = tri_byte(s_tri[w], 3u * t), i1 = tri_byte(s_tri[w], 3u * t + 1u),
+                               i2 = tri_byte(s_tri[w], 3u * t + 2u);
+                const float4 c0 = s_clip[w][i0], c1 = s_clip[w][i1], c2 = s_clip[w][i2];
+                if (cached) {
+                    // r[k] = v_{k+1} x v_{k+2} over the clip (x, y, w) of the vertices
+                    TriCache tcache;
+                    const float4 cv[3] = { c0, c1, c2 };
 #pragma unroll
-                for (int k = 0; k < 3; ++k) {
-                    const int ka = (k + 1) % 3, kb = (k + 2) % 3;
-                    const int ex = ts.x[kb] - ts.x[ka], ey = ts.y[kb] - ts.y[ka];
-                    const bool tl = ey < 0 || (ey == 0 && ex > 0);
-                    st.C[k] = ex * (sy0 - ts.y[ka]) - ey * (sx0 - ts.x[ka]) - (tl ? 0 : 1);
-                    st.A[k] = -ey * 256;
-                    st.B[k] = ex * 256;
+                    for (int k = 0; k < 3; ++k) {
+                        const float4 p = cv[(k + 1) % 3], q = cv[(k + 2) % 3];
+                        tcache.r[k][0] = p.y * q.w - p.w * q.y;
+                        tcache.r[k][1] = p.w * q.x - p.x * q.w;
+                        tcache.r[k][2] = p.x * q.y - p.y * q.x;
+                    }
+                    tcache.col[0] = s_col[w][i0]; tcache.col[1] = s_col[w][i1]; tcache.col[2] = s_col[w][i2];
+                    a.tri_cache[(size_t)slot * 128u + t] = tcache;
                 }
-                depth_plane(ts.x, ts.y, ts.z, ts.area, st.za, st.zb);
-                st.z0 = ts.z[0];
-                st.oxf = (float)(sx0 - ts.x[0]) * 0.00390625f;
-                st.oyf = (float)(sy0 - ts.y[0]) * 0.00390625f;
-                st.org = (uint32_t)ts.px0 | ((uint32_t)ts.py0 << 16);
-                const uint32_t bw = (uint32_t)(ts.px1 - ts.px0 + 1), bh = (uint32_t)(ts.py1 - ts.py0 + 1);
-                st.dim = bw | (bh << 8);
-                st.id = id;
-                s_st[w][t] = st;
-                npix[sl] = bw * bh;
-            } else {
-                push_big(ts, id, a.big, a.big_chunk, a.big_cap, a.ctr);
-            }
+                if (cull_faces) {
+                    const float det = det3_xyw(c0, c1, c2);
+                    const bool culled = tdet < 0.0f ? det < 0.0f : det > 0.0f;
+                    if (culled) continue;
+                }
+                id = (d << VKGV_TRIANGLE_BITS) | t;
+                const SVtx v0 = s_sv[w][i0], v1 = s_sv[w][i1], v2 = s_sv[w][i2];
+                if (!(v0.ok & v1.ok & v2.ok)) {
+                    const uint32_t slot = atomicAdd(&a.ctr->clip_count, 1u);
+                    if (slot < a.clip_cap) {
+                        ClipTri ct;
+                        ct.c[0] = c0; ct.c[1] = c1; ct.c[2] = c2;
+                        ct.id = id;
+                        ct.pad[0] = ct.pad[1] = ct.pad[2] = 0u;
+                        a.clipq[slot] = ct;
+                    } else {
+                        atomicOr(&a.ctr->overflow, 2u);
+                    }
+                    continue;
+                }
+                if (!setup_tri(v0, v1, v2, W, H, ts)) continue;
+                if (ts.maxx - ts.minx < a.small_lim && ts.maxy - ts.miny < a.small_lim) {
+                    SmallTri st;
+                    const int sx0 = ts.px0 * 256 + 128, sy0 = ts.py0 * 256 + 128;
+#pragma unroll
+                    for (int k = 0; k < 3; ++k) {
+                        const int ka = (k + 1) % 3, kb = (k + 2) % 3;
+                        const int ex = ts.x[kb] - ts.x[ka], ey = ts.y[kb] - ts.y[ka];
+                        const bool tl = ey < 0 || (ey == 0 && ex > 0);
+                        st.C[k] = ex * (sy0 - ts.y[ka]) - ey * (sx0 - ts.x[ka]) - (tl ? 0 : 1);
+                        st.A[k] = -ey * 256;
+                        st.B[k] = ex * 256;
+                    }
+                    depth_plane(ts.x, ts.y, ts.z, ts.area, st.za, st.zb);
+                    st.z0 = ts.z[0];
+                    st.oxf = (float)(sx0 - ts.x[0]) * 0.00390625f;
+                    st.oyf = (float)(sy0 - ts.y[0]) * 0.00390625f;
+                    st.org = (uint32_t)ts.px0 | ((uint32_t)ts.py0 << 16);
+                    const uint32_t bw = (uint32_t)(ts.px1 - ts.px0 + 1), bh = (uint32_t)(ts.py1 - ts.py0 + 1);
+                    st.dim = bw | (bh << 8);
+                    st.id = id;
+                    s_st[w][t] = st;
+                    npix[sl] = bw * bh;
+                } else {
+                    want_big = true;
+                }
+            } while (false);
+            push_big_wave(want_big, ts, id, lane, a.big, a.big_chunk, a.big_cap, a.chunk_tri, a.chunk_cap, a.ctr);
         }
 
         // exclusive prefix of pixel counts over the 128 triangle slots
@@ -316,32 +361,59 @@ __global__ __launch_bounds__(kBlock) void raster_kernel(RasterArgs a)
         const uint32_t total = tot0 + tot1;
         wave_sync();
 
-        for (uint32_t k = lane; k < total; k += 64u) {
-            uint32_t lo = 0, hi = 127;
+        // Lane l takes the contiguous run of pixel slots [l * per, (l + 1) * per):
+        // one search finds the triangle of its first slot, then it walks the
+        // run with integer edge steps and moves to the next triangle when the
+        // run crosses a triangle's start (no search per pixel).
+        const uint32_t per = (total + 63u) >> 6;
+        uint32_t k = lane * per;
+        const uint32_t kend = min(k + per, total);
+        if (k < kend) {
+            uint32_t lo = 0, hi = 127;   // last triangle whose first slot is <= k
             while (lo < hi) {
                 const uint32_t mid = (lo + hi + 1u) >> 1;
                 if (s_pre[w][mid] <= k) lo = mid; else hi = mid - 1u;
             }
-            const SmallTri& st = s_st[w][lo];
+            uint32_t next = lo < 127u ? s_pre[w][lo + 1u] : total;
+            SmallTri st = s_st[w][lo];
+            uint32_t bw = st.dim & 0xFFu;
             const uint32_t r = k - s_pre[w][lo];
-            const uint32_t bw = st.dim & 0xFFu;
-            const uint32_t j = (uint32_t)(((float)r + 0.5f) * __builtin_amdgcn_rcpf((float)bw));
-            const uint32_t i = r - j * bw;
-            if (i >= bw) continue;   // never taken: (r + 0.5) / bw is at least 0.5/bw from an integer
-            const int ii = (int)i, jj = (int)j;
-            const int e0 = st.C[0] + st.A[0] * ii + st.B[0] * jj;
-            const int e1 = st.C[1] + st.A[1] * ii + st.B[1] * jj;
-            const int e2 = st.C[2] + st.A[2] * ii + st.B[2] * jj;
-            if ((e0 | e1 | e2) < 0) continue;
-            const float z = st.z0 + (st.za * (st.oxf + (float)ii) + st.zb * (st.oyf + (float)jj));
-            if (!(z >= 0.0f && z <= 1.0f)) continue;
-            write_fragment(a.keys, a.W, (int)(st.org & 0xFFFFu) + ii, (int)(st.org >> 16) + jj, z, st.id);
+            int jj = (int)(r / bw), ii = (int)(r - (uint32_t)jj * bw);
+            int e0 = st.C[0] + st.A[0] * ii + st.B[0] * jj;
+            int e1 = st.C[1] + st.A[1] * ii + st.B[1] * jj;
+            int e2 = st.C[2] + st.A[2] * ii + st.B[2] * jj;
+            for (;;) {
+                if ((e0 | e1 | e2) >= 0) {
+                    const float z = st.z0 + (st.za * (st.oxf + (float)ii) + st.zb * (st.oyf + (float)jj));
+                    if (z >= 0.0f && z <= 1.0f)
+                        write_fragment(a.keys, a.W, (int)(st.org & 0xFFFFu) + ii, (int)(st.org >> 16) + jj, z, st.id);
+                }
+                if (++k >= kend) break;
+                if (k == next) {   // first slot of a later triangle (empty ones skipped)
+                    while (lo < 127u && s_pre[w][lo + 1u] <= k) ++lo;
+                    next = lo < 127u ? s_pre[w][lo + 1u] : total;
+                    st = s_st[w][lo];
+                    bw = st.dim & 0xFFu;
+                    ii = 0;
+                    jj = 0;
+                    e0 = st.C[0];
+                    e1 = st.C[1];
+                    e2 = st.C[2];
+                } else if (++ii == (int)bw) {   // next row of the bounding box
+                    ii = 0;
+                    ++jj;
+                    e0 = st.C[0] + st.B[0] * jj;
+                    e1 = st.C[1] + st.B[1] * jj;
+                    e2 = st.C[2] + st.B[2] * jj;
+                } else {
+                    e0 += st.A[0];
+                    e1 += st.A[1];
+                    e2 += st.A[2];
+                }
+            }
         }
         wave_sync();
-
-        uint32_t nxt = 0;
-        if (lane == 0) nxt = atomicAdd(a.cursor, 1u);
-        idx = nwaves + __builtin_amdgcn_readfirstlane(__shfl(nxt, 0));
+        idx += nwaves;   // static stride: no shared cursor
     }
 }
 
@@ -350,6 +422,8 @@ struct BigArgs {
     BigTri* big;
     uint32_t* big_chunk;
     uint32_t big_cap;
+    uint32_t* chunk_tri;
+    uint32_t chunk_cap;
     const ClipTri* clipq;
     uint32_t clip_cap;
     Counters* ctr;
@@ -362,6 +436,7 @@ struct BigArgs {
     uint32_t bin_cap;
     BigSetup* setups;
     BinPair* pairs;
+    TileItem* items;
 };
 
 // Clipping (rare): Sutherland-Hodgman against pa*x + pb*y + pc*z + pd*w >= 0.
@@ -413,7 +488,7 @@ __global__ __launch_bounds__(256) void clip_kernel(BigArgs a)
         for (int k = 1; k + 1 < m; ++k) {
             TriSetup t;
             if (setup_tri(sv[0], sv[k], sv[k + 1], (int)a.W, (int)a.H, t))
-                push_big(t, ct.id, a.big, a.big_chunk, a.big_cap, a.ctr);
+                push_big(t, ct.id, a.big, a.big_chunk, a.big_cap, a.chunk_tri, a.chunk_cap, a.ctr);
         }
     }
 }
@@ -462,16 +537,20 @@ __device__ __forceinline__ void pass_extent(const BigArgs& a, uint32_t& nbig, ui
     if (raw > a.big_cap && nbig) nchunks = a.big_chunk[nbig - 1] + chunks_of(a.big[nbig - 1]);
 }
 
+// Work items of a tile whose list holds n triangles (0 for an empty tile).
+__device__ __forceinline__ uint32_t tile_split(uint32_t n)
+{
+    return min(kTileSplit, (n + kTileItemTris - 1u) / kTileItemTris);
+}
+
 // One wave per 64-tile chunk of a big triangle's bounding-box tiles.  The
 // wave of chunk 0 stores the triangle's rasterization setup.  Lane l tests
 // tile 64*chunk + l: a tile is skipped when one edge is negative at all four
 // corner pixel centres of the tile's part of the bounding box (no centre there
 // can be covered); kept tiles write pair 64*chunk + l and count themselves.
-// The last block to finish scans the per-tile counts.
+// bin_scan_kernel then scans the per-tile counts.
 __global__ __launch_bounds__(256) void bin_kernel(BigArgs a)
 {
-    __shared__ uint32_t s_scan[4];
-    __shared__ bool s_last;
     const uint32_t w = threadIdx.x >> 6, lane = threadIdx.x & 63u;
     const uint32_t gw = blockIdx.x * 4u + w, nw = gridDim.x * 4u;
     uint32_t nbig, nchunks;
@@ -479,12 +558,7 @@ __global__ __launch_bounds__(256) void bin_kernel(BigArgs a)
     if (nchunks == 0) return;   // uniform over the grid: no big triangle this pass
     const bool pair_overflow = (unsigned long long)nchunks * 64ull > a.bin_cap;
     for (uint32_t c = gw; c < nchunks && !pair_overflow; c += nw) {
-        uint32_t lo = 0, hi = nbig - 1u;   // last triangle whose first chunk is <= c
-        while (lo < hi) {
-            const uint32_t mid = (lo + hi + 1u) >> 1;
-            if (a.big_chunk[mid] <= c) lo = mid; else hi = mid - 1u;
-        }
-        const uint32_t ti = lo;
+        const uint32_t ti = a.chunk_tri[c];
         const BigTri t = a.big[ti];
         const uint32_t j = c - a.big_chunk[ti];
         Edge e[3];
@@ -540,51 +614,93 @@ __global__ __launch_bounds__(256) void bin_kernel(BigArgs a)
         a.pairs[64u * c + lane] = bp;
     }
 
-    // last block: exclusive scan of the tile counts
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        __threadfence();
-        s_last = atomicAdd(&a.ctr->bin_done, 1u) == gridDim.x - 1u;
+    if (pair_overflow && blockIdx.x == 0 && threadIdx.x == 0) atomicOr(&a.ctr->overflow, 4u);
+}
+
+// Exclusive scans (one block) of the tile counts and of the tile pass's work
+// items, kScanChunk tiles at a time through LDS so that every global load and
+// store is coalesced: writes the per-tile list starts, zeroes the counts for
+// the fill pass (which counts again) and writes the work items.
+constexpr uint32_t kScanThreads = 1024;
+constexpr uint32_t kScanChunk = 4096;
+constexpr uint32_t kScanPer = kScanChunk / kScanThreads;
+
+__global__ __launch_bounds__(kScanThreads) void bin_scan_kernel(BigArgs a)
+{
+    __shared__ uint32_t s_cnt[kScanChunk], s_start[kScanChunk], s_istart[kScanChunk];
+    __shared__ uint32_t s_wsum[kScanThreads / 64], s_wisum[kScanThreads / 64], s_wmax[kScanThreads / 64];
+    const uint32_t tid = threadIdx.x, w = tid >> 6, lane = tid & 63u;
+    uint32_t nbig, nchunks;
+    pass_extent(a, nbig, nchunks);
+    if (nchunks == 0) {   // no big triangle this pass: no work items
+        if (tid == 0) a.ctr->tile_items = 0u;
+        return;
     }
-    __syncthreads();
-    if (!s_last) return;
-    __atomic_thread_fence(__ATOMIC_ACQUIRE);   // the other blocks' counts are visible
     const uint32_t ntiles = a.tiles_x * a.tiles_y;
-    const uint32_t per = (ntiles + 255u) / 256u;
-    const uint32_t b = threadIdx.x * per, e = min(b + per, ntiles);
-    uint32_t sum = 0, most = 0;
-    for (uint32_t i = b; i < e; ++i) {
-        const uint32_t cnt = a.bin_count[i];
-        sum += cnt;
-        most = max(most, cnt);
+    uint32_t run_base = 0, irun_base = 0, most = 0;
+    for (uint32_t c0 = 0; c0 < ntiles; c0 += kScanChunk) {
+        const uint32_t m = min(kScanChunk, ntiles - c0);
+        for (uint32_t i = tid; i < m; i += kScanThreads) s_cnt[i] = a.bin_count[c0 + i];
+        __syncthreads();
+        const uint32_t b = tid * kScanPer;
+        uint32_t v[kScanPer], sum = 0, isum = 0;
+#pragma unroll
+        for (uint32_t u = 0; u < kScanPer; ++u) {
+            v[u] = b + u < m ? s_cnt[b + u] : 0u;
+            sum += v[u];
+            isum += tile_split(v[u]);
+            most = max(most, v[u]);
+        }
+        uint32_t x = sum, ix = isum;
+#pragma unroll
+        for (int off = 1; off < 64; off <<= 1) {
+            const uint32_t y = __shfl_up(x, off), iy = __shfl_up(ix, off);
+            if (lane >= (uint32_t)off) { x += y; ix += iy; }
+        }
+        if (lane == 63u) { s_wsum[w] = x; s_wisum[w] = ix; }
+        __syncthreads();
+        uint32_t wbase = 0, iwbase = 0, total = 0, itotal = 0;
+        for (uint32_t q = 0; q < kScanThreads / 64; ++q) {
+            if (q < w) { wbase += s_wsum[q]; iwbase += s_wisum[q]; }
+            total += s_wsum[q];
+            itotal += s_wisum[q];
+        }
+        uint32_t run = run_base + wbase + x - sum, irun = irun_base + iwbase + ix - isum;
+#pragma unroll
+        for (uint32_t u = 0; u < kScanPer; ++u) {
+            if (b + u < m) { s_start[b + u] = run; s_istart[b + u] = irun; }
+            run += v[u];
+            irun += tile_split(v[u]);
+        }
+        __syncthreads();
+        for (uint32_t i = tid; i < m; i += kScanThreads) {
+            const uint32_t n = s_cnt[i], st = s_start[i], sp = tile_split(n);
+            a.bin_start[c0 + i] = st;
+            a.bin_count[c0 + i] = 0u;   // the fill pass counts again; the tile pass leaves it zero
+            for (uint32_t p = 0; p < sp; ++p) {
+                TileItem it;
+                it.code = (c0 + i) | (p << 24) | ((sp - 1u) << 28);
+                it.start = st;
+                it.n = n;
+                it.pad = 0u;
+                a.items[s_istart[i] + p] = it;
+            }
+        }
+        run_base += total;
+        irun_base += itotal;
+        __syncthreads();   // LDS reused by the next chunk
     }
 #pragma unroll
     for (int off = 32; off >= 1; off >>= 1) most = max(most, __shfl_xor(most, off));
-    if (lane == 0) atomicMax(&a.ctr->max_tile_tris, most);
-    uint32_t x = sum;
-#pragma unroll
-    for (int off = 1; off < 64; off <<= 1) {
-        const uint32_t y = __shfl_up(x, off);
-        if (lane >= (uint32_t)off) x += y;
-    }
-    if (lane == 63u) s_scan[w] = x;
+    if (lane == 0) s_wmax[w] = most;
     __syncthreads();
-    uint32_t wbase = 0;
-    for (uint32_t q = 0; q < w; ++q) wbase += s_scan[q];
-    const uint32_t total = s_scan[0] + s_scan[1] + s_scan[2] + s_scan[3];
-    uint32_t run = wbase + x - sum;
-    for (uint32_t i = b; i < e; ++i) {
-        const uint32_t cnt = a.bin_count[i];
-        a.bin_start[i] = run;
-        a.bin_count[i] = 0u;   // the fill pass counts again (and leaves zero on overflow)
-        run += cnt;
-    }
-    if (threadIdx.x == 0) {
-        a.ctr->bin_total = total;
+    if (tid == 0) {
+        for (uint32_t q = 1; q < kScanThreads / 64; ++q) most = max(most, s_wmax[q]);
+        atomicMax(&a.ctr->max_tile_tris, most);
+        a.ctr->bin_total = run_base;
+        a.ctr->tile_items = irun_base;
         a.ctr->big_total_tris += nbig;
-        a.ctr->big_total_pairs += total;
-        a.ctr->bin_done = 0u;
-        if (pair_overflow) atomicOr(&a.ctr->overflow, 4u);
+        a.ctr->big_total_pairs += run_base;
     }
 }
 
@@ -603,79 +719,111 @@ __global__ __launch_bounds__(256) void bin_fill_kernel(BigArgs a)
     }
 }
 
-// Up to kTileSplit blocks per 32x32 screen tile, one per kPerBlock triangles
-// of its list: block s of S rasterizes triangles s, s + S, ... into LDS keys,
-// then merges its tile into the visibility buffer.  Setups are staged through
-// LDS kChunk at a time.  Thread t owns pixels (t & 31, (t >> 5) + 8q), q = 0..3.
+// Tile pass: each wave runs a grid-stride loop over the halves (32x16
+// pixels) of the work items.  Item (tile, part, S) rasterizes triangles
+// [n*part/S, n*(part+1)/S) of the tile's list of n; a half keeps its keys in
+// registers (lane l owns column l & 31, rows (l >> 5) + 2r, r = 0..7 of the
+// half), then merges them into the visibility buffer:
+// plain read-min-store when the tile has one item (the meshlet raster of this
+// pass has finished and tiles are disjoint; the old keys are read when the
+// item starts), atomicMin otherwise.  Setups are loaded 64 at a time, one per
+// lane, and broadcast with readlane.  Items of one tile are adjacent, so
+// they run on different XCDs.  Wave 0 also clears the pass's queue counters
+// (their last reader, the fill pass, has finished), and part 0 of each tile
+// zeroes the tile's count for the next pass.
+constexpr int kTileRows = (int)(kBinTile * kBinTile / 128u);   // pixels per lane in a half tile
+constexpr int kSetupWords = (int)(sizeof(BigSetup) / 4);
+static_assert(kSetupWords % 4 == 0, "setups are loaded 16 bytes at a time");
+
 __global__ __launch_bounds__(256) void tile_kernel(BigArgs a)
 {
-    constexpr uint32_t kChunk = 32;
-    constexpr uint32_t kPerBlock = 32;
-    constexpr uint32_t kWords = sizeof(BigSetup) / 4;
-    __shared__ unsigned long long s_key[kBinTile * kBinTile];
-    __shared__ uint32_t s_idx[kChunk];
-    __shared__ BigSetup s_set[kChunk];
-    const uint32_t tile = blockIdx.x / kTileSplit, part = blockIdx.x % kTileSplit;
-    const uint32_t n_all = a.bin_count[tile];
-    const uint32_t split = min(kTileSplit, (n_all + kPerBlock - 1u) / kPerBlock);
-    if (part >= split) return;
-    const uint32_t n = (n_all - part + split - 1u) / split;   // this block's share
-    const uint32_t start = a.bin_start[tile];
-    const int ox = (int)((tile % a.tiles_x) * kBinTile), oy = (int)((tile / a.tiles_x) * kBinTile);
-    const int lx = (int)(threadIdx.x & 31u), ly = (int)(threadIdx.x >> 5);
-    const int px = ox + lx, py_first = oy + ly;
+    const uint32_t lane = threadIdx.x & 63u;
+    const uint32_t wave = blockIdx.x * 4u + (threadIdx.x >> 6), nwaves = gridDim.x * 4u;
+    if (wave == 0 && lane == 0) {
+        a.ctr->big_packed = 0ull;
+        a.ctr->clip_count = 0u;
+    }
+    const uint32_t nitems = __builtin_amdgcn_readfirstlane(a.ctr->tile_items);
+    const int lx = (int)(lane & 31u), ly = (int)(lane >> 5);
+    const BigSetup* __restrict__ setups = a.setups;
+    for (uint32_t hv = wave; hv < 2u * nitems; hv += nwaves) {
+        const TileItem item = a.items[hv >> 1];
+        const uint32_t code = __builtin_amdgcn_readfirstlane(item.code);
+        const uint32_t n_all = __builtin_amdgcn_readfirstlane(item.n);
+        const uint32_t start = __builtin_amdgcn_readfirstlane(item.start);
+        const uint32_t tile = code & 0xFFFFFFu, part = (code >> 24) & 15u, split = (code >> 28) + 1u;
+        const uint32_t b = start + n_all * part / split, e = start + n_all * (part + 1u) / split;
+        if (part == 0 && (hv & 1u) == 0 && lane == 0) a.bin_count[tile] = 0u;
+        const int px = (int)((tile % a.tiles_x) * kBinTile) + lx;
+        const int py0 = (int)((tile / a.tiles_x) * kBinTile + (hv & 1u) * (kBinTile / 2u)) + ly;
+        const bool col_in = px < (int)a.W;
+        unsigned long long old[kTileRows];
+        if (split == 1) {
 #pragma unroll
-    for (int q = 0; q < 4; ++q) s_key[threadIdx.x + 256 * q] = VKGV_KEY_CLEAR;
-    const int sx = px * 256 + 128;
-    for (uint32_t c0 = 0; c0 < n; c0 += kChunk) {
-        const uint32_t m = min(kChunk, n - c0);
-        if (threadIdx.x < m) s_idx[threadIdx.x] = a.bins[start + part + (c0 + threadIdx.x) * split];
-        __syncthreads();
-        uint32_t* dst = reinterpret_cast<uint32_t*>(s_set);
-        for (uint32_t wd = threadIdx.x; wd < m * kWords; wd += 256u)
-            dst[wd] = reinterpret_cast<const uint32_t*>(a.setups + s_idx[wd / kWords])[wd % kWords];
-        __syncthreads();
-        for (uint32_t k = 0; k < m; ++k) {
-            const BigSetup& s = s_set[k];
-            const int bx0 = (int)(s.bb0 & 0xFFFFu), by0 = (int)(s.bb0 >> 16);
-            const int bx1 = (int)(s.bb1 & 0xFFFFu), by1 = (int)(s.bb1 >> 16);
-            if (px < bx0 || px > bx1) continue;
-            long long ev[3];
-#pragma unroll
-            for (int r = 0; r < 3; ++r)
-                ev[r] = s.e0[r] - (long long)(px - bx0) * (long long)s.ey[r] * 256
-                        + (long long)(py_first - by0) * (long long)s.ex[r] * 256;
-            const float dxf = (float)(sx - s.x0) * 0.00390625f;
-#pragma unroll
-            for (int q = 0; q < 4; ++q) {
-                const int py = py_first + 8 * q;
-                if (q) {
-#pragma unroll
-                    for (int r = 0; r < 3; ++r) ev[r] += (long long)s.ex[r] * 2048;   // +8 rows
-                }
-                if (py < by0 || py > by1) continue;
-                if ((ev[0] | ev[1] | ev[2]) < 0) continue;
-                const float dyf = (float)(py * 256 + 128 - s.y0) * 0.00390625f;
-                const float z = s.z0 + (s.za * dxf + s.zb * dyf);
-                if (!(z >= 0.0f && z <= 1.0f)) continue;
-                (void)__hip_atomic_fetch_min(&s_key[threadIdx.x + 256 * q], make_key(z, s.id), __ATOMIC_RELAXED,
-                                             __HIP_MEMORY_SCOPE_WORKGROUP);
+            for (int r = 0; r < kTileRows; ++r) {
+                const int py = py0 + 2 * r;
+                old[r] = col_in && py < (int)a.H ? a.keys[(size_t)py * a.W + px] : 0ull;
             }
         }
-        __syncthreads();
-    }
-    if (px >= (int)a.W) return;
-    // A tile owned by one block has no concurrent writer (the meshlet raster
-    // of this pass has finished; tiles are disjoint): plain read-min-store.
+        unsigned long long key[kTileRows];
 #pragma unroll
-    for (int q = 0; q < 4; ++q) {
-        const int py = py_first + 8 * q;
-        const unsigned long long key = s_key[threadIdx.x + 256 * q];
-        if (key == VKGV_KEY_CLEAR || py >= (int)a.H) continue;
-        unsigned long long* p = a.keys + (size_t)py * a.W + px;
-        if (key < *p) {
-            if (split == 1) *p = key;
-            else key_min(p, key);
+        for (int r = 0; r < kTileRows; ++r) key[r] = VKGV_KEY_CLEAR;
+        const int sx = px * 256 + 128;
+        for (uint32_t t0 = b; t0 < e; t0 += 64u) {
+            // batch of up to 64 triangles: lane j holds the setup of triangle t0 + j
+            // (two loads for the batch), broadcast one at a time below
+            const uint32_t m = min(64u, e - t0);
+            uint32_t sw[kSetupWords];
+            if (lane < m) {
+                const uint32_t* src = reinterpret_cast<const uint32_t*>(setups + a.bins[t0 + lane]);
+#pragma unroll
+                for (int q = 0; q < kSetupWords; q += 4) {
+                    const uint4 v4 = *reinterpret_cast<const uint4*>(src + q);
+                    sw[q] = v4.x; sw[q + 1] = v4.y; sw[q + 2] = v4.z; sw[q + 3] = v4.w;
+                }
+            }
+            for (uint32_t j = 0; j < m; ++j) {
+                BigSetup s;
+                uint32_t* dst = reinterpret_cast<uint32_t*>(&s);
+#pragma unroll
+                for (int q = 0; q < kSetupWords; ++q) dst[q] = __builtin_amdgcn_readlane(sw[q], j);
+                const int bx0 = (int)(s.bb0 & 0xFFFFu), by0 = (int)(s.bb0 >> 16);
+                const int bx1 = (int)(s.bb1 & 0xFFFFu), by1 = (int)(s.bb1 >> 16);
+                if (px < bx0 || px > bx1) continue;
+                long long ev[3];
+#pragma unroll
+                for (int q = 0; q < 3; ++q)
+                    ev[q] = s.e0[q] - (long long)(px - bx0) * (long long)s.ey[q] * 256
+                            + (long long)(py0 - by0) * (long long)s.ex[q] * 256;
+                const float dxf = (float)(sx - s.x0) * 0.00390625f;
+#pragma unroll
+                for (int r = 0; r < kTileRows; ++r) {
+                    const int py = py0 + 2 * r;
+                    if (r) {
+#pragma unroll
+                        for (int q = 0; q < 3; ++q) ev[q] += (long long)s.ex[q] * 512;   // +2 rows
+                    }
+                    if (py < by0 || py > by1) continue;
+                    if ((ev[0] | ev[1] | ev[2]) < 0) continue;
+                    const float dyf = (float)(py * 256 + 128 - s.y0) * 0.00390625f;
+                    const float z = s.z0 + (s.za * dxf + s.zb * dyf);
+                    if (!(z >= 0.0f && z <= 1.0f)) continue;
+                    const unsigned long long k = make_key(z, s.id);
+                    key[r] = k < key[r] ? k : key[r];
+                }
+            }
+        }
+        if (!col_in) continue;
+#pragma unroll
+        for (int r = 0; r < kTileRows; ++r) {
+            const int py = py0 + 2 * r;
+            if (key[r] == VKGV_KEY_CLEAR || py >= (int)a.H) continue;
+            unsigned long long* p = a.keys + (size_t)py * a.W + px;
+            if (split == 1) {
+                if (key[r] < old[r]) *p = key[r];
+            } else {
+                key_min(p, key[r]);
+            }
         }
     }
 }
@@ -690,7 +838,6 @@ void launch_raster(const SceneDev& s, const ViewDev& v, const Mat4& vp, uint32_t
     a.vp = vp;
     a.list = v.visible[list];
     a.list_count = &v.counters->vis_count[list];
-    a.cursor = &v.counters->raster_cursor[list];
     a.keys = v.keys;
     a.W = v.width;
     a.H = v.height;
@@ -699,6 +846,8 @@ void launch_raster(const SceneDev& s, const ViewDev& v, const Mat4& vp, uint32_t
     a.big = v.big;
     a.big_chunk = v.big_chunk;
     a.big_cap = v.big_capacity;
+    a.chunk_tri = v.chunk_tri;
+    a.chunk_cap = v.bin_capacity / 64u;
     a.clipq = v.clipq;
     a.clip_cap = v.clip_capacity;
     a.ctr = v.counters;
@@ -716,6 +865,8 @@ void launch_raster_big(const ViewDev& v, hipStream_t st)
     a.big = v.big;
     a.big_chunk = v.big_chunk;
     a.big_cap = v.big_capacity;
+    a.chunk_tri = v.chunk_tri;
+    a.chunk_cap = v.bin_capacity / 64u;
     a.clipq = v.clipq;
     a.clip_cap = v.clip_capacity;
     a.ctr = v.counters;
@@ -730,10 +881,12 @@ void launch_raster_big(const ViewDev& v, hipStream_t st)
     a.bin_cap = v.bin_capacity;
     a.setups = v.setups;
     a.pairs = v.pairs;
+    a.items = v.tile_items;
     clip_kernel<<<16, 256, 0, st>>>(a);
     bin_kernel<<<1024, 256, 0, st>>>(a);
+    bin_scan_kernel<<<1, kScanThreads, 0, st>>>(a);
     bin_fill_kernel<<<256, 256, 0, st>>>(a);
-    tile_kernel<<<v.tiles_x * v.tiles_y * kTileSplit, 256, 0, st>>>(a);
+    tile_kernel<<<v.tile_grid, 256, 0, st>>>(a);
 }
 
 } // namespace vkgv

@@ -53,6 +53,39 @@ __device__ __forceinline__ float3 cross3(float3 a, float3 b)
     return make_float3(a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x);
 }
 
+// Albedo and triangle data of a draw whose meshlet is not in the resolve
+// cache: re-fetched from the scene and re-transformed (same arithmetic as the
+// raster's cache fill).
+__device__ __forceinline__ void fetch_uncached(const ResolveArgs& a, uint32_t d, uint32_t t, bool bary, float* c, float3* r,
+                                               uint32_t* col)
+{
+    const vkgv_meshlet_draw dr = a.s.draws[d];
+    const vkgv_primitive& prim = a.s.prims[dr.primitive_index];
+    const vkgv_material& mat = a.s.materials[prim.material_index];
+    c[0] = mat.albedo_factor[0]; c[1] = mat.albedo_factor[1]; c[2] = mat.albedo_factor[2]; c[3] = mat.albedo_factor[3];
+    if (!bary) return;
+    const vkgv_meshlet ml = reinterpret_cast<const vkgv_meshlet*>(prim.meshlet_buffer)[dr.meshlet_index];
+    const Mat4 mvp = mat4_mul(a.vp, load_mat4(a.s.transforms + 16ull * dr.transform_index));
+    const uint8_t* tri = reinterpret_cast<const uint8_t*>(prim.primitive_index_buffer) + ml.triangle_offset + 3u * t;
+    const uint32_t* vidx = reinterpret_cast<const uint32_t*>(prim.vertex_index_buffer) + ml.vertex_offset;
+    const vkgv_vertex* verts = reinterpret_cast<const vkgv_vertex*>(prim.vertex_buffer);
+    float3 hv[3];
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        const vkgv_vertex& vx = verts[vidx[tri[k]]];
+        const float4 cl = mat4_point(mvp, vx.position[0], vx.position[1], vx.position[2]);
+        hv[k] = make_float3(cl.x, cl.y, cl.w);
+        col[k] = *reinterpret_cast<const uint32_t*>(vx.color);
+    }
+    r[0] = cross3(hv[1], hv[2]);
+    r[1] = cross3(hv[2], hv[0]);
+    r[2] = cross3(hv[0], hv[1]);
+}
+
+// Block = 64 x 16 pixels; thread (x, y) resolves rows y, y + 4, y + 8, y + 12
+// of its column, with the loads of the four pixels issued together.
+constexpr int kRows = 4;
+
 __global__ __launch_bounds__(256) void resolve_kernel(ResolveArgs a)
 {
     // LDS tables: sRGB thresholds and buckets, exact quotients b / 255 of a byte
@@ -69,79 +102,79 @@ __global__ __launch_bounds__(256) void resolve_kernel(ResolveArgs a)
     const uint8_t* base = reinterpret_cast<const uint8_t*>(s_base);
 
     const uint32_t px = blockIdx.x * 64u + (threadIdx.x & 63u);
-    const uint32_t py = blockIdx.y * 4u + (threadIdx.x >> 6);
-    if (px >= a.W || py >= a.H) return;
-    const size_t i = (size_t)py * a.W + px;
-    const unsigned long long key = a.keys[i];
-    if (key == VKGV_KEY_CLEAR) { a.color[i] = 0u; return; }
-    const uint32_t id = key_id(key);
-    const uint32_t d = id >> VKGV_TRIANGLE_BITS, t = id & ((1u << VKGV_TRIANGLE_BITS) - 1u);
-    if (d >= a.s.draw_count) { a.color[i] = 0u; return; }   // keys not written by this draw list
-    const uint32_t slot = a.draw_slot[d];
+    if (px >= a.W) return;
+    const uint32_t py0 = blockIdx.y * (4u * kRows) + (threadIdx.x >> 6);
     const bool bary = (a.flags & VKGV_RESOLVE_BARY) != 0;
-    float c[4];
-    float3 r[3];
-    uint32_t col[3];
-    if (slot != 0xFFFFFFFFu) {
-        // the raster kept this meshlet's albedo and triangles
-        const float4 alb = a.slot_albedo[slot];
-        c[0] = alb.x; c[1] = alb.y; c[2] = alb.z; c[3] = alb.w;
-        if (bary) {
-            const TriCache tc = a.tri_cache[(size_t)slot * 128u + t];
+    const unsigned long long* __restrict__ keys = a.keys;
+    const uint32_t* __restrict__ draw_slot = a.draw_slot;
+
+    unsigned long long key[kRows];
 #pragma unroll
-            for (int k = 0; k < 3; ++k) {
-                r[k] = make_float3(tc.r[k][0], tc.r[k][1], tc.r[k][2]);
-                col[k] = tc.col[k];
-            }
-        }
-    } else {
-        const vkgv_meshlet_draw dr = a.s.draws[d];
-        const vkgv_primitive& prim = a.s.prims[dr.primitive_index];
-        const vkgv_material& mat = a.s.materials[prim.material_index];
-        c[0] = mat.albedo_factor[0]; c[1] = mat.albedo_factor[1]; c[2] = mat.albedo_factor[2]; c[3] = mat.albedo_factor[3];
-        if (bary) {
-            const vkgv_meshlet ml = reinterpret_cast<const vkgv_meshlet*>(prim.meshlet_buffer)[dr.meshlet_index];
-            const Mat4 mvp = mat4_mul(a.vp, load_mat4(a.s.transforms + 16ull * dr.transform_index));
-            const uint8_t* tri = reinterpret_cast<const uint8_t*>(prim.primitive_index_buffer) + ml.triangle_offset + 3u * t;
-            const uint32_t* vidx = reinterpret_cast<const uint32_t*>(prim.vertex_index_buffer) + ml.vertex_offset;
-            const vkgv_vertex* verts = reinterpret_cast<const vkgv_vertex*>(prim.vertex_buffer);
-            float3 hv[3];
+    for (int k = 0; k < kRows; ++k) {
+        const uint32_t py = py0 + 4u * k;
+        key[k] = py < a.H ? keys[(size_t)py * a.W + px] : VKGV_KEY_CLEAR;
+    }
+    // draw of each pixel; ~0u: no shading (clear key or a key not written by this draw list)
+    uint32_t drw[kRows], slot[kRows];
 #pragma unroll
-            for (int k = 0; k < 3; ++k) {
-                const vkgv_vertex& vx = verts[vidx[tri[k]]];
-                const float4 cl = mat4_point(mvp, vx.position[0], vx.position[1], vx.position[2]);
-                hv[k] = make_float3(cl.x, cl.y, cl.w);
-                col[k] = *reinterpret_cast<const uint32_t*>(vx.color);
-            }
-            r[0] = cross3(hv[1], hv[2]);
-            r[1] = cross3(hv[2], hv[0]);
-            r[2] = cross3(hv[0], hv[1]);
+    for (int k = 0; k < kRows; ++k) {
+        drw[k] = 0xFFFFFFFFu;
+        slot[k] = 0xFFFFFFFFu;
+        if (key[k] != VKGV_KEY_CLEAR) {
+            const uint32_t d = key_id(key[k]) >> VKGV_TRIANGLE_BITS;
+            if (d < a.s.draw_count) { drw[k] = d; slot[k] = draw_slot[d]; }
         }
     }
-    if (bary) {
-        // perspective-correct barycentrics at the pixel centre:
-        // lambda_k ~ r[k] . (xn, yn, 1), normalized; c *= interpolated colour
-        const float hw = (float)a.W * 0.5f, hh = (float)a.H * 0.5f;
-        const float xn = (((float)px + 0.5f) - hw) * (1.0f / hw);
-        const float yn = (((float)py + 0.5f) - hh) * (1.0f / hh);
-        float l[3];
+    const float hw = (float)a.W * 0.5f, hh = (float)a.H * 0.5f;
+    const float xn = (((float)px + 0.5f) - hw) * (1.0f / hw);
 #pragma unroll
-        for (int k = 0; k < 3; ++k) l[k] = (r[k].x * xn + r[k].y * yn) + r[k].z;
-        const float inv = 1.0f / ((l[0] + l[1]) + l[2]);
+    for (int k = 0; k < kRows; ++k) {
+        const uint32_t py = py0 + 4u * k;
+        if (py >= a.H) break;
+        const size_t i = (size_t)py * a.W + px;
+        if (drw[k] == 0xFFFFFFFFu) { a.color[i] = 0u; continue; }
+        const uint32_t t = key_id(key[k]) & ((1u << VKGV_TRIANGLE_BITS) - 1u);
+        float c[4];
+        float3 r[3];
+        uint32_t col[3];
+        if (slot[k] != 0xFFFFFFFFu) {
+            // the raster kept this meshlet's albedo and triangles
+            const float4 alb = a.slot_albedo[slot[k]];
+            c[0] = alb.x; c[1] = alb.y; c[2] = alb.z; c[3] = alb.w;
+            if (bary) {
+                const TriCache tc = a.tri_cache[(size_t)slot[k] * 128u + t];
 #pragma unroll
-        for (int k = 0; k < 3; ++k) l[k] = l[k] * inv;
-#pragma unroll
-        for (int j = 0; j < 4; ++j) {
-            const float c0 = s_byte[(col[0] >> (8 * j)) & 0xFFu];
-            const float c1 = s_byte[(col[1] >> (8 * j)) & 0xFFu];
-            const float c2 = s_byte[(col[2] >> (8 * j)) & 0xFFu];
-            c[j] = c[j] * ((l[0] * c0 + l[1] * c1) + l[2] * c2);
+                for (int q = 0; q < 3; ++q) {
+                    r[q] = make_float3(tc.r[q][0], tc.r[q][1], tc.r[q][2]);
+                    col[q] = tc.col[q];
+                }
+            }
+        } else {
+            fetch_uncached(a, drw[k], t, bary, c, r, col);
         }
+        if (bary) {
+            // perspective-correct barycentrics at the pixel centre:
+            // lambda_q ~ r[q] . (xn, yn, 1), normalized; c *= interpolated colour
+            const float yn = (((float)py + 0.5f) - hh) * (1.0f / hh);
+            float l[3];
+#pragma unroll
+            for (int q = 0; q < 3; ++q) l[q] = (r[q].x * xn + r[q].y * yn) + r[q].z;
+            const float inv = 1.0f / ((l[0] + l[1]) + l[2]);
+#pragma unroll
+            for (int q = 0; q < 3; ++q) l[q] = l[q] * inv;
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+                const float c0 = s_byte[(col[0] >> (8 * j)) & 0xFFu];
+                const float c1 = s_byte[(col[1] >> (8 * j)) & 0xFFu];
+                const float c2 = s_byte[(col[2] >> (8 * j)) & 0xFFu];
+                c[j] = c[j] * ((l[0] * c0 + l[1] * c1) + l[2] * c2);
+            }
+        }
+        const uint32_t r8 = srgb_code(s_thr, base, key0, c[0]);
+        const uint32_t g8 = srgb_code(s_thr, base, key0, c[1]);
+        const uint32_t b8 = srgb_code(s_thr, base, key0, c[2]);
+        a.color[i] = r8 | (g8 << 8) | (b8 << 16) | (unorm8(c[3]) << 24);
     }
-    const uint32_t r8 = srgb_code(s_thr, base, key0, c[0]);
-    const uint32_t g8 = srgb_code(s_thr, base, key0, c[1]);
-    const uint32_t b8 = srgb_code(s_thr, base, key0, c[2]);
-    a.color[i] = r8 | (g8 << 8) | (b8 << 16) | (unorm8(c[3]) << 24);
 }
 
 } // namespace
@@ -159,7 +192,7 @@ void launch_resolve(const SceneDev& s, const ViewDev& v, const Mat4& vp, uint32_
     a.draw_slot = v.draw_slot;
     a.tri_cache = v.tri_cache;
     a.slot_albedo = v.slot_albedo;
-    const dim3 grid((v.width + 63) / 64, (v.height + 3) / 4);
+    const dim3 grid((v.width + 63) / 64, (v.height + 4 * kRows - 1) / (4 * kRows));
     resolve_kernel<<<grid, 256, 0, st>>>(a);
 }
 
