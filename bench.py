@@ -39,6 +39,42 @@ def pyramid_texels(w, h):
     return total
 
 
+def visible_triangles(sc, draw_ids):
+    """Triangles of the listed meshlet draws."""
+    d = sc.draws[np.asarray(draw_ids, dtype=np.int64)]
+    mesh_of = sc.primitives["mesh_index"][d["primitive_index"]]
+    total = 0
+    for mi, m in enumerate(sc.meshes):
+        sel = mesh_of == mi
+        if sel.any():
+            total += int(m.meshlets["triangle_count"][d["meshlet_index"][sel]].astype(np.int64).sum())
+    return total
+
+
+# HBM bytes per launch from rocprofv3 FETCH_SIZE / WRITE_SIZE passes over this
+# benchmark, each scaled by the same counter's reading for a plain 256 MiB
+# copy kernel (profiles/traffic_c2_4k.json records the commands and values).
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "traffic_c2_4k.json")
+PASS_KERNELS = {"clear": ["clear_keys_kernel"], "hiz": ["hiz_tiles_kernel", "hiz_tail_kernel"],
+                "resolve": ["resolve_kernel"]}
+
+
+def measured_traffic(pass_name, W, H):
+    try:
+        with open(TRAFFIC_FILE) as f:
+            t = json.load(f)
+    except (OSError, ValueError):
+        return None
+    if (t.get("width"), t.get("height")) != (W, H):
+        return None
+    total = 0.0
+    for c in ("FETCH_SIZE", "WRITE_SIZE"):
+        rec = t[c]
+        for k in PASS_KERNELS[pass_name]:
+            total += rec["per_kernel_kib"][k] * rec["factor"] * 1024.0
+    return round(total)
+
+
 def cpu_baseline(sc, cam, W, H, flags, budget_s=12.0, max_frames=8):
     """The oracle (scalar C restatement, 1 thread) on frames of the same workload."""
     import oracle
@@ -65,7 +101,19 @@ def main():
     ap.add_argument("--verbose", action="store_true")
     ap.add_argument("--small-max", type=int, default=0, help="raster tunable (0: library default)")
     ap.add_argument("--blocks-per-cu", type=int, default=0, help="raster tunable (0: library default)")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="only run three plain 256 MiB device-to-device copies (the reference kernel for scaling "
+                         "rocprofv3 FETCH_SIZE / WRITE_SIZE to bytes) and exit")
     args = ap.parse_args()
+    if args.copy_calibration:
+        import torch
+        a = torch.ones(64 << 20, dtype=torch.float32, device="cuda:0")
+        b = torch.empty_like(a)
+        for _ in range(3):
+            b.copy_(a)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_calibration_bytes": a.numel() * 4, "launches": 3}))
+        return
     if args.gpus != 1:
         print(f"bench.py: this build measures one GPU; --gpus {args.gpus} is not supported "
               "(views shard one per GPU, see the design notes)", file=sys.stderr)
@@ -106,16 +154,27 @@ def main():
         "clear": st.ms_clear, "cull_early": st.ms_cull, "raster_early": st.ms_raster, "hiz": st.ms_hiz,
         "cull_late": st.ms_cull_late, "raster_late": st.ms_raster_late, "resolve": st.ms_resolve,
     }
-    # Algorithmic bytes per launch (HBM-bound passes):
-    #  clear   8 B per pixel written
-    #  hiz     4 B per depth pixel read + 4 B per pyramid texel written
-    #  resolve 8 B key read + 4 B RGBA8 written per pixel
+    # Algorithmic bytes per launch of the HBM-bound passes (per unit x units):
+    #  clear    8 B per pixel written (64-bit key)
+    #  cull     36 B per draw (12 B record + 24 B AABB) + 64 B per instance
+    #           transform + 16 B per Hi-Z probe (2x2 fp32; at most one per
+    #           draw) + 4 B per visible draw id written
+    #  hiz      4 B per depth pixel read + 4 B per pyramid texel written
+    #  resolve  8 B key read + 4 B RGBA8 written per pixel + 39 B per visible
+    #           triangle (3 x 12 B positions + 3 B indices)
     npx = W * H
-    alg = {"clear": 8.0 * npx, "hiz": 4.0 * npx + 4.0 * pyramid_texels(W, H), "resolve": 12.0 * npx}
-    dom = max(alg, key=lambda k: passes[k])
+    vis = np.concatenate([view.visible(0), view.visible(1)])
+    vis_tris = visible_triangles(sc, vis)
+    n_draw, n_inst = len(sc.draws), len(sc.transforms)
+    alg = {"clear": 8.0 * npx,
+           "cull_early": 36.0 * n_draw + 64.0 * n_inst + 16.0 * n_draw + 4.0 * st.visible_early,
+           "hiz": 4.0 * npx + 4.0 * pyramid_texels(W, H),
+           "resolve": 12.0 * npx + 39.0 * vis_tris}
+    gbs = {k: round(alg[k] / (passes[k] * 1e-3) / 1e9, 1) for k in alg if passes[k] > 0}
+    dom = max(("clear", "hiz", "resolve"), key=lambda k: passes[k])
     ach = alg[dom] / (passes[dom] * 1e-3) / 1e9
     roofline = {"bound": "hbm", "kernel": dom, "achieved": round(ach, 1), "peak": HBM_PEAK_GBS,
-                "unit": "GB/s", "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": None}
+                "unit": "GB/s", "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": measured_traffic(dom, W, H)}
 
     cpu = None
     if not args.no_cpu_baseline:
@@ -134,6 +193,7 @@ def main():
                    "visible_early": st.visible_early, "visible_late": st.visible_late,
                    "parallelism": "single GPU"},
         "passes_ms": {k: round(v, 4) for k, v in passes.items()},
+        "passes_alg_gbs": gbs,
         "gpu_frame_ms_events": round(st.ms_total, 4),
         "triangles_per_s": round(tris / (ms_frame * 1e-3), 1),
         "roofline": roofline,

@@ -68,6 +68,7 @@ def _setup(L):
         "ref_keys_depth": (None, [p, C.c_uint64, p]),
         "ref_keys_ids": (None, [p, C.c_uint64, p]),
         "ref_from_linear": (f32, [f32]),
+        "ref_srgb_codes": (None, [p, C.c_uint64, p]),
         "ref_resolve": (None, [p, p, p, u32, u32, u32, p]),
         "ref_frame_reference": (u32, [p, p, p, u32, u32, u32, p, p, p, p]),
         "ref_frame_two_pass": (None, [p, p, p, u32, u32, u32, p, p, p, p, p, p]),
@@ -145,6 +146,22 @@ def resolve(scene: Scene, vp, keys, flags=0) -> np.ndarray:
     k = np.ascontiguousarray(keys, dtype=np.uint64)
     lib().ref_resolve(scene.ref, vpa.ctypes.data, k.ctypes.data, w, h, flags, out.ctypes.data)
     return out.reshape(h, w)
+
+
+def srgb_codes(c) -> np.ndarray:
+    c = np.ascontiguousarray(c, dtype=np.float32)
+    out = np.zeros(c.size, dtype=np.uint8)
+    lib().ref_srgb_codes(c.ctypes.data, c.size, out.ctypes.data)
+    return out
+
+
+def raster_triangle(c0, c1, c2, tid, w, h, keys=None) -> np.ndarray:
+    """One clip-space triangle (x, y, z, w per vertex) into `keys` (h, w)."""
+    if keys is None:
+        keys = np.full(w * h, 0xFFFFFFFFFFFFFFFF, dtype=np.uint64)
+    v = [np.ascontiguousarray(c, dtype=np.float32) for c in (c0, c1, c2)]
+    lib().ref_raster_triangle(v[0].ctypes.data, v[1].ctypes.data, v[2].ctypes.data, tid, w, h, keys.ctypes.data)
+    return keys.reshape(h, w)
 
 
 def keys_depth(keys):

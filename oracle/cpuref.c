@@ -613,6 +613,12 @@ static uint32_t unorm8(float x)
     return (uint32_t)(v * 255.0f + 0.5f);
 }
 
+/* unorm8(fromLinear(c)) of n values: the resolve's colour encode. */
+void ref_srgb_codes(const float* c, uint64_t n, uint8_t* out)
+{
+    for (uint64_t i = 0; i < n; ++i) out[i] = (uint8_t)unorm8(ref_from_linear(c[i]));
+}
+
 /* visbuffer_resolve.comp.glsl:17-41; with VKGV_RESOLVE_BARY the colour is
  * albedoFactor * interpolated vertex colour, using perspective-correct
  * barycentrics recomputed from the ids. */

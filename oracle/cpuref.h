@@ -94,6 +94,7 @@ void ref_keys_ids(const uint64_t* keys, uint64_t n, uint32_t* ids);
 
 /* ---- visbuffer_resolve.comp.glsl + srgb.h.glsl -------------------------- */
 float ref_from_linear(float c);
+void  ref_srgb_codes(const float* c, uint64_t n, uint8_t* out);
 void  ref_resolve(const ref_scene* s, const float* vp, const uint64_t* keys,
                   uint32_t w, uint32_t h, uint32_t flags, uint32_t* rgba);
 

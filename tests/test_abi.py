@@ -1,0 +1,92 @@
+"""The C ABI: both libraries load on a host without a GPU and export every
+function their headers declare; the Python bindings' struct layouts equal the
+headers' (compiled with gcc).  No device call is made here."""
+import os
+import re
+import subprocess
+import tempfile
+
+import numpy as np
+import pytest
+
+from vk_gltf_viewer_amd import _native as N
+from vk_gltf_viewer_amd import scene as S
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+INC = os.path.join(ROOT, "include")
+
+
+def declared(header):
+    text = open(os.path.join(INC, header)).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"\b(vkgv_[a-z0-9_]+)\s*\(", text)))
+
+
+@pytest.mark.parametrize("header,loader", [("vkgv.h", N.hip), ("vkgv_host.h", N.host)])
+def test_library_exports_every_declared_function(header, loader):
+    lib = loader()
+    names = declared(header)
+    assert len(names) > 10
+    missing = [n for n in names if not hasattr(lib, n)]
+    assert not missing, missing
+
+
+def test_python_binding_lists_match_header():
+    assert sorted(N.HIP_SYMBOLS) == declared("vkgv.h")
+
+
+def test_struct_layouts_match_headers():
+    fields = {
+        "vkgv_meshlet": (S.MESHLET_DTYPE, ["vertex_offset", "triangle_offset", "vertex_count", "triangle_count",
+                                           "aabb_extents", "aabb_center"]),
+        "vkgv_vertex": (S.VERTEX_DTYPE, ["position", "color", "normal", "uv"]),
+        "vkgv_meshlet_draw": (S.DRAW_DTYPE, ["primitive_index", "meshlet_index", "transform_index"]),
+        "vkgv_material": (S.MATERIAL_DTYPE, ["albedo_factor", "albedo_index", "uv_offset", "uv_scale", "uv_rotation",
+                                             "alpha_cutoff", "double_sided"]),
+        "vkgv_meshlet_cone": (S.CONE_DTYPE, ["apex", "axis", "cutoff"]),
+        "vkgv_primitive_desc": (S.PRIM_DTYPE, ["mesh_index", "material_index", "aabb_extents", "aabb_center"]),
+    }
+    ctypes_structs = {"vkgv_camera": N.Camera, "vkgv_frame_desc": N.FrameDesc, "vkgv_frame_stats": N.FrameStats,
+                      "vkgv_create_info": N.CreateInfo, "vkgv_tuning": N.Tuning, "vkgv_mesh_desc": N.MeshDesc,
+                      "vkgv_primitive": N.Primitive, "vkgv_view_desc": N.ViewDesc, "vkgv_node": N.Node}
+    lines = ["#include <stdio.h>", "#include <stddef.h>", '#include "vkgv.h"', '#include "vkgv_host.h"',
+             "int main(void) {"]
+    for st, (_, fl) in fields.items():
+        lines.append(f'printf("{st} %zu\\n", sizeof({st}));')
+        for f in fl:
+            lines.append(f'printf("{st}.{f} %zu\\n", offsetof({st}, {f}));')
+    for st, cs in ctypes_structs.items():
+        lines.append(f'printf("{st} %zu\\n", sizeof({st}));')
+        for f, _ in cs._fields_:
+            lines.append(f'printf("{st}.{f} %zu\\n", offsetof({st}, {f}));')
+    lines.append("return 0; }")
+    with tempfile.TemporaryDirectory() as d:
+        src, exe = os.path.join(d, "layout.c"), os.path.join(d, "layout")
+        open(src, "w").write("\n".join(lines))
+        subprocess.run(["gcc", "-std=c11", "-I", INC, src, "-o", exe], check=True)
+        out = dict(l.rsplit(" ", 1) for l in subprocess.run([exe], capture_output=True, text=True,
+                                                            check=True).stdout.splitlines())
+    for st, (dt, fl) in fields.items():
+        assert int(out[st]) == dt.itemsize, st
+        for f in fl:
+            assert int(out[f"{st}.{f}"]) == dt.fields[f][1], (st, f)
+    for st, cs in ctypes_structs.items():
+        import ctypes
+        assert int(out[st]) == ctypes.sizeof(cs), st
+        for f, _ in cs._fields_:
+            assert int(out[f"{st}.{f}"]) == getattr(cs, f).offset, (st, f)
+
+
+def test_status_codes_and_flags_match_header():
+    text = open(os.path.join(INC, "vkgv.h")).read()
+    consts = dict(re.findall(r"#define\s+(VKGV_[A-Z0-9_]+)\s+\(?([^\n/]+?)\)?\s*(?:/\*|\n)", text))
+
+    def val(name):
+        v = consts[name].strip().rstrip("u").rstrip("U")
+        m = re.fullmatch(r"1u?\s*<<\s*(\d+)", v)
+        return 1 << int(m.group(1)) if m else int(v, 0)
+    for py, hdr in [("CULL_FRUSTUM", "VKGV_CULL_FRUSTUM"), ("CULL_OCCLUSION_PREV", "VKGV_CULL_OCCLUSION_PREV"),
+                    ("CULL_TWO_PASS", "VKGV_CULL_TWO_PASS"), ("CULL_CONE", "VKGV_CULL_CONE"),
+                    ("RESOLVE_BARY", "VKGV_RESOLVE_BARY"), ("BUILD_PYRAMID", "VKGV_BUILD_PYRAMID"),
+                    ("TIMING", "VKGV_TIMING"), ("NO_BACKFACE", "VKGV_NO_BACKFACE"), ("SKIP_RESOLVE", "VKGV_SKIP_RESOLVE")]:
+        assert getattr(N, py) == val(hdr), py

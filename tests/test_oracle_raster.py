@@ -1,0 +1,311 @@
+"""Pins of the oracle's raster, back-face, clipping, resolve and frame logic
+(CPU only), each against something independent of the oracle's code: exact
+rational coverage fixtures, watertightness and area invariants, float64
+geometry, IEC 61966-2-1, and the conservativeness of occlusion culling."""
+import os
+
+import numpy as np
+import pytest
+from scipy.spatial import Delaunay
+
+import oracle
+from vk_gltf_viewer_amd import _native as N
+from vk_gltf_viewer_amd.scene import Scene, camera_advance, camera_from_view, view_desc
+
+from helpers import icosphere, tiny_scene, translate_scale
+
+GOLDEN = os.path.join(os.path.dirname(__file__), "golden")
+CLEAR = np.uint64(0xFFFFFFFFFFFFFFFF)
+
+
+def px_to_clip(x, y, z, W, H):
+    return [x / (W / 2) - 1.0, y / (H / 2) - 1.0, z, 1.0]
+
+
+# ---------------------------------------------------------------------------
+def test_coverage_matches_exact_rational_golden():
+    g = np.load(os.path.join(GOLDEN, "raster.npz"))
+    W, H = int(g["width"]), int(g["height"])
+    for tri, mask in zip(g["tris"], g["masks"]):
+        c = [px_to_clip(x, y, 0.5, W, H) for (x, y) in tri]
+        keys = oracle.raster_triangle(c[0], c[1], c[2], 3, W, H)
+        got = (keys != CLEAR).astype(np.uint8)
+        assert np.array_equal(got, mask), tri
+
+
+def test_watertight_triangulation():
+    """A Delaunay triangulation on the 1/256 grid covers every pixel centre
+    strictly inside its hull exactly once and nothing outside it."""
+    rng = np.random.default_rng(11)
+    W, H = 40, 28
+    for _ in range(6):
+        pts = np.round(rng.uniform([-3, -3], [W + 3, H + 3], (40, 2)) * 256) / 256
+        pts = np.unique(pts, axis=0)
+        tri = Delaunay(pts)
+        count = np.zeros((H, W), np.int32)
+        for k, s in enumerate(tri.simplices):
+            c = [px_to_clip(pts[i][0], pts[i][1], 0.5, W, H) for i in s]
+            keys = oracle.raster_triangle(c[0], c[1], c[2], k, W, H)
+            count += keys != CLEAR
+        centres = np.stack(np.meshgrid(np.arange(W) + 0.5, np.arange(H) + 0.5), -1).reshape(-1, 2)
+        inside = (tri.find_simplex(centres) >= 0).reshape(H, W)
+        # distance of each centre to the hull boundary, to leave boundary ties out
+        hull = pts[tri.convex_hull]
+        a, b = hull[:, 0], hull[:, 1]
+        d = np.full(len(centres), np.inf)
+        for (p0, p1) in zip(a, b):
+            t = np.clip(((centres - p0) @ (p1 - p0)) / ((p1 - p0) @ (p1 - p0)), 0, 1)
+            d = np.minimum(d, np.linalg.norm(centres - (p0 + t[:, None] * (p1 - p0)), axis=1))
+        d = d.reshape(H, W)
+        assert count.max() <= 1
+        assert np.all(count[inside & (d > 1e-9)] == 1)
+        assert np.all(count[~inside & (d > 1e-9)] == 0)
+
+
+def test_coverage_area_and_depth_plane():
+    """Covered count ~ triangle area (within a perimeter band); depth at each
+    covered centre equals the float64 plane through the snapped vertices."""
+    rng = np.random.default_rng(12)
+    W, H = 64, 48
+    for _ in range(40):
+        p = np.round(rng.uniform([0, 0], [W, H], (3, 2)) * 256) / 256
+        z = rng.uniform(0.05, 0.95, 3)
+        area = 0.5 * abs((p[1, 0] - p[0, 0]) * (p[2, 1] - p[0, 1]) - (p[1, 1] - p[0, 1]) * (p[2, 0] - p[0, 0]))
+        if area < 4:
+            continue
+        c = [px_to_clip(p[i][0], p[i][1], z[i], W, H) for i in range(3)]
+        keys = oracle.raster_triangle(c[0], c[1], c[2], 5, W, H)
+        cov = keys != CLEAR
+        perim = sum(np.linalg.norm(p[i] - p[(i + 1) % 3]) for i in range(3))
+        assert abs(cov.sum() - area) <= perim + 2
+        ys, xs = np.nonzero(cov)
+        A = np.c_[p, np.ones(3)]
+        coef = np.linalg.solve(A, np.asarray(z, np.float32).astype(np.float64))
+        want = coef[0] * (xs + 0.5) + coef[1] * (ys + 0.5) + coef[2]
+        got = oracle.keys_depth(keys[cov])
+        assert np.all(np.abs(got - want) <= 2e-6 + 1e-6 * np.abs(want))
+        assert np.all(oracle.keys_ids(keys[cov]) == 5)
+
+
+def test_near_plane_clipping_against_float64_polygon():
+    """A triangle crossing the near plane (z = w in reverse-Z clip space)
+    covers the centres inside its float64 clipped-and-projected polygon
+    (pixels within 1 px of an edge excluded)."""
+    rng = np.random.default_rng(13)
+    W, H = 64, 64
+    checked = 0
+    for _ in range(30):
+        v = [np.r_[rng.uniform(-1.5, 1.5, 2), 0.0, 0.0] for _ in range(3)]
+        ws = [rng.uniform(0.3, 2.0), rng.uniform(0.3, 2.0), rng.uniform(-1.0, -0.05)]
+        cl = []
+        for k in range(3):
+            x, y = v[k][0] * ws[k], v[k][1] * ws[k]
+            zc = 0.1 if ws[k] > 0 else 0.1 - 2 * ws[k]   # points behind the eye have z > w
+            cl.append(np.array([x, y, zc, ws[k]], np.float64))
+        c32 = [np.asarray(c, np.float32) for c in cl]
+        keys = oracle.raster_triangle(c32[0], c32[1], c32[2], 1, W, H)
+        poly = []
+        for i in range(3):
+            p, q = c32[i].astype(np.float64), c32[(i + 1) % 3].astype(np.float64)
+            dp, dq = p[3] - p[2], q[3] - q[2]
+            if dp >= 0:
+                poly.append(p)
+            if (dp >= 0) != (dq >= 0):
+                poly.append(p + dp / (dp - dq) * (q - p))
+        if len(poly) < 3:
+            continue
+        scr = np.array([[(c[0] / c[3]) * W / 2 + W / 2, (c[1] / c[3]) * H / 2 + H / 2] for c in poly])
+        centres = np.stack(np.meshgrid(np.arange(W) + 0.5, np.arange(H) + 0.5), -1).reshape(-1, 2)
+        sgn = []
+        mind = np.full(len(centres), np.inf)
+        for i in range(len(scr)):
+            a, b = scr[i], scr[(i + 1) % len(scr)]
+            e = b - a
+            s = e[0] * (centres[:, 1] - a[1]) - e[1] * (centres[:, 0] - a[0])
+            sgn.append(s)
+            t = np.clip(((centres - a) @ e) / (e @ e), 0, 1)
+            mind = np.minimum(mind, np.linalg.norm(centres - (a + t[:, None] * e), axis=1))
+        sgn = np.array(sgn)
+        inside = np.all(sgn > 0, axis=0) | np.all(sgn < 0, axis=0)
+        far = mind > 1.0
+        got = (keys != CLEAR).ravel()
+        assert np.array_equal(got[far], inside[far])
+        checked += int(inside[far].sum())
+    assert checked > 500
+
+
+def test_backface_matches_screen_winding():
+    """Back-face culling (visbuffer.mesh.glsl:90-100) keeps exactly the
+    triangles of one screen-space winding, and a mirroring transform
+    (det(M) < 0) keeps the other one."""
+    rng = np.random.default_rng(14)
+    W, H = 96, 64
+    cam = camera_from_view(view_desc(W, H, (0, 0, 6), (0, 0, 0)))
+    vp = np.asarray(cam.view_projection, np.float32)
+    VP = vp.reshape(4, 4).T.astype(np.float64)
+    front = None
+    for mirror in (False, True):
+        seen = []
+        for _ in range(60):
+            pos = rng.uniform(-1.2, 1.2, (3, 3)).astype(np.float32)
+            M = np.diag([-1.0 if mirror else 1.0, 1, 1, 1])
+            sc = tiny_scene([(pos, [0, 1, 2])], [(0, M, (1, 1, 1, 1), False)])
+            rs = oracle.Scene(sc)
+            keys = oracle.raster(rs, vp, [0], W, H)
+            clip = (VP @ M @ np.c_[pos, np.ones(3)].T).T
+            s = clip[:, :2] / clip[:, 3:4]
+            area = (s[1, 0] - s[0, 0]) * (s[2, 1] - s[0, 1]) - (s[1, 1] - s[0, 1]) * (s[2, 0] - s[0, 0])
+            if abs(area) < 1e-3:
+                continue
+            both = oracle.raster(rs, vp, [0], W, H, flags=N.NO_BACKFACE)
+            if not (both != CLEAR).any():
+                continue   # covers no pixel centre: nothing to compare
+            seen.append((np.sign(area), bool((keys != CLEAR).any())))
+        kept = {sg for sg, r in seen if r}
+        culled = {sg for sg, r in seen if not r}
+        assert len(kept) == 1 and len(culled) == 1 and kept != culled, seen
+        if not mirror:
+            front = kept.pop()
+        else:
+            assert kept.pop() == -front
+
+
+def test_icosphere_front_faces_render():
+    """A closed outward-wound sphere renders the same with or without
+    back-face culling (only far faces are culled, and they are hidden)."""
+    pos, idx = icosphere(3)
+    n = np.cross(pos[idx[1::3]] - pos[idx[0::3]], pos[idx[2::3]] - pos[idx[0::3]])
+    assert np.all((n * pos[idx[0::3]]).sum(1) > 0)      # counter-clockwise from outside
+    sc = tiny_scene([(pos, idx)], [(0, translate_scale((0, 0, 0), 2.0), (1, 1, 1, 1), False)])
+    rs = oracle.Scene(sc)
+    cam = camera_from_view(view_desc(128, 96, (0.5, 0.7, 8), (0, 0, 0)))
+    lst = np.arange(len(sc.draws), dtype=np.uint32)
+    a = oracle.raster(rs, cam.view_projection, lst, 128, 96)
+    b = oracle.raster(rs, cam.view_projection, lst, 128, 96, flags=N.NO_BACKFACE)
+    assert (a != CLEAR).sum() > 1000
+    assert np.array_equal(a, b)
+
+
+# ---------------------------------------------------------------------------
+def test_from_linear_and_codes_against_iec_golden():
+    g = np.load(os.path.join(GOLDEN, "srgb.npz"))
+    c, enc, code = g["c"], g["encoded"], g["code"]
+    got = np.array([oracle.lib().ref_from_linear(np.float32(x)) for x in c])
+    assert np.allclose(got, enc, rtol=2e-6, atol=1e-7)
+    codes = oracle.srgb_codes(c)
+    frac = enc * 255.0 + 0.5
+    clear_of_tie = np.abs(frac - np.round(frac)) > 1e-3
+    assert np.array_equal(codes[clear_of_tie], code[clear_of_tie])
+    assert np.abs(codes.astype(int) - code.astype(int)).max() <= 1
+
+
+def test_srgb_threshold_table_equals_encode():
+    """The product's 255 sRGB thresholds (libvkgv_hip host code, no device
+    call) count exactly unorm8(fromLinear(c)) for every tested float: random
+    bit patterns in [0, 1], each threshold and its neighbours, specials."""
+    t = np.zeros(255, np.float32)
+    N.hip().vkgv_srgb_thresholds(t.ctypes.data)
+    assert np.all(np.diff(t) > 0)
+    rng = np.random.default_rng(15)
+    bits = rng.integers(0, 0x3F800001, 2_000_000, dtype=np.int64).astype(np.uint32)
+    xs = [bits.view(np.float32), t, np.nextafter(t, np.float32(0)), np.nextafter(t, np.float32(2)),
+          np.array([0.0, -0.0, 1.0, 1.5, -1.0, 1e-30, np.inf, -np.inf], np.float32)]
+    c = np.concatenate(xs).astype(np.float32)
+    want = oracle.srgb_codes(c)
+    got = np.searchsorted(t, c, side="right").astype(np.int64)
+    got[~(c >= t[0])] = 0
+    got[c >= 1.0] = 255
+    assert np.array_equal(got, want.astype(np.int64))
+    # every 16-bit bucket of the float holds at most one threshold
+    keys = t.view(np.uint32) >> 16
+    assert len(np.unique(keys)) == len(keys)
+
+
+def test_resolve_flat_and_bary():
+    """Flat resolve = sRGB(albedo); with barycentrics and equal vertex colours
+    the result is unchanged up to 1 LSB; with differing colours it matches a
+    float64 perspective-correct interpolation (3x3 solve) within 1 LSB."""
+    pos, idx = icosphere(2)
+    rng = np.random.default_rng(16)
+    col_equal = np.tile([255, 255, 255, 255], (len(pos), 1)).astype(np.uint8)
+    col_rand = rng.integers(0, 256, (len(pos), 4)).astype(np.uint8)
+    W, H = 96, 72
+    cam = camera_from_view(view_desc(W, H, (0.4, 0.3, 5), (0, 0, 0)))
+    vp = np.asarray(cam.view_projection, np.float32)
+    albedo = (0.8, 0.35, 0.05, 0.7)
+    for colors in (col_equal, col_rand):
+        sc = tiny_scene([(pos, idx)], [(0, translate_scale((0.2, -0.1, 0), 1.5), albedo, False)], colors=[colors])
+        rs = oracle.Scene(sc)
+        keys = oracle.raster(rs, vp, np.arange(len(sc.draws), dtype=np.uint32), W, H)
+        flat = oracle.resolve(rs, vp, keys)
+        bary = oracle.resolve(rs, vp, keys, flags=N.RESOLVE_BARY)
+        cov = keys != CLEAR
+        assert cov.sum() > 1000
+        enc = lambda x: np.floor(np.clip(np.where(x < 0.0031308, 12.92 * x, 1.055 * np.power(np.maximum(x, 0), 1 / 2.4) - 0.055), 0, 1) * 255 + 0.5)
+        want_flat = np.array([enc(albedo[0]), enc(albedo[1]), enc(albedo[2]), np.floor(albedo[3] * 255 + 0.5)])
+        fb = flat.view(np.uint8).reshape(H, W, 4)
+        assert np.abs(fb[cov].astype(int) - want_flat.astype(int)).max() <= 1
+        assert np.all(flat[~cov] == 0)
+        bb = bary.view(np.uint8).reshape(H, W, 4).astype(int)
+        if colors is col_equal:
+            assert np.abs(bb[cov] - fb[cov].astype(int)).max() <= 1
+            continue
+        ids = oracle.keys_ids(keys)
+        VP = vp.reshape(4, 4).T.astype(np.float64)
+        M = translate_scale((0.2, -0.1, 0), 1.5)
+        m = sc.meshes[0]
+        bad = 0
+        ys, xs = np.nonzero(cov)
+        for y, x in zip(ys[::7], xs[::7]):
+            i = ids[y, x]
+            d, t = i >> 7, i & 127
+            ml = m.meshlets[sc.draws[d]["meshlet_index"]]
+            tri = m.triangle_indices[ml["triangle_offset"] + 3 * t: ml["triangle_offset"] + 3 * t + 3]
+            vi = m.vertex_indices[ml["vertex_offset"] + tri.astype(np.int64)]
+            P = (VP @ M @ np.c_[m.vertices["position"][vi].astype(np.float64), np.ones(3)].T)   # 4x3 clip
+            ndc = np.array([(x + 0.5) / (W / 2) - 1, (y + 0.5) / (H / 2) - 1, 1.0])
+            # perspective-correct barycentrics: lambda solves [x; y; w] lambda ~ (ndc.x, ndc.y, 1)
+            lam = np.linalg.solve(P[[0, 1, 3]], ndc)
+            lam /= lam.sum()
+            colv = m.vertices["color"][vi].astype(np.float64) / 255.0
+            lin = np.asarray(albedo) * (lam @ colv)
+            want = np.array([enc(lin[0]), enc(lin[1]), enc(lin[2]), np.floor(np.clip(lin[3], 0, 1) * 255 + 0.5)])
+            bad += int(np.abs(bb[y, x] - want).max() > 1)
+        assert bad == 0
+
+
+# ---------------------------------------------------------------------------
+def test_occlusion_culling_is_conservative():
+    """Two-pass occlusion culling renders the same visibility buffer as
+    frustum culling alone (power-of-two view: every Hi-Z level is an exact 2x2
+    reduction), over a moving camera."""
+    s = Scene(1, 1, 0).arrays()
+    rs = oracle.Scene(s)
+    W, H = 512, 256
+    hist = np.zeros(len(s.draws), np.uint8)
+    pyr = np.zeros(oracle.pyramid_info(W, H)[2], np.float32)
+    cam = None
+    culled = 0
+    for f in range(4):
+        v = view_desc(W, H, (4.0 * f, 2.0, 75.0 - 3 * f), (0, 0, 0))
+        cam = camera_from_view(v) if cam is None else camera_advance(cam, v)
+        keys, _, _, ve, vl = oracle.frame_two_pass(rs, cam, W, H, N.CULL_FRUSTUM | N.CULL_TWO_PASS | N.SKIP_RESOLVE,
+                                                   hist)
+        ref, _, vis = oracle.frame_reference(rs, cam, W, H, N.CULL_FRUSTUM | N.SKIP_RESOLVE, pyr)
+        assert np.array_equal(keys, ref), f
+        assert set(ve.tolist()).isdisjoint(vl.tolist())
+        assert set(ve.tolist()) | set(vl.tolist()) <= set(vis.tolist())
+        culled += len(vis) - len(ve) - len(vl)
+    assert culled > 500      # occlusion culling removed draws without changing the image
+
+
+def test_cull_sets_subset_and_order():
+    s = Scene(1, 1, 64).arrays()
+    rs = oracle.Scene(s)
+    v = s.views[0]
+    cam = camera_from_view(v)
+    pyr = np.zeros(oracle.pyramid_info(v.width, v.height)[2], np.float32)
+    allv = oracle.cull(rs, cam, pyr, v.width, v.height, 0)
+    fr = oracle.cull(rs, cam, pyr, v.width, v.height, N.CULL_FRUSTUM)
+    assert len(allv) == len(s.draws)
+    assert np.all(np.diff(fr.astype(np.int64)) > 0) and set(fr.tolist()) <= set(allv.tolist())
