@@ -75,6 +75,55 @@ def measured_traffic(pass_name, W, H):
     return round(total)
 
 
+def batched_views(steps, warmup, log):
+    """The batched form: the 1.31M-triangle instanced-sphere scene rendered to
+    64 views of uneven size (4 orthographic cascades at 2048x2048, 6 cube
+    faces at 512x512, 54 random perspective views up to 1920x1080) as one
+    captured-graph batch frame, against the plain loop of 64 single-view
+    frames on the same views.  Two-pass Hi-Z, barycentric resolve; tunables
+    chosen by the autotuner on the batch frame."""
+    from vk_gltf_viewer_amd.autotune import autotune
+    sc = Scene(3, 3).arrays()
+    r = Renderer(0)
+    try:
+        r.load(sc)
+        views = [r.create_view(v.width, v.height) for v in sc.views]
+        cams = [camera_from_view(v) for v in sc.views]
+        batch = r.create_batch(views)
+        flags = N.CULL_FRUSTUM | N.CULL_TWO_PASS | N.RESOLVE_BARY
+        tuning, _ = autotune(r, lambda: batch.render(cams, flags | N.GRAPH), key="c3-64views", use_cache=False,
+                             log=log)
+
+        def run(fn, n):
+            r.sync()
+            t0 = time.perf_counter()
+            for _ in range(n):
+                fn()
+            r.sync()
+            return 1000.0 * (time.perf_counter() - t0) / n
+
+        batch_frame = lambda: batch.render(cams, flags | N.GRAPH)
+
+        def loop_frame():
+            for v, c in zip(views, cams):
+                v.render(c, flags)
+        run(batch_frame, warmup)
+        ms_batch = run(batch_frame, steps)
+        run(loop_frame, warmup)
+        ms_loop = run(loop_frame, steps)
+        st = batch.stats()
+        px = sum(v.width * v.height for v in sc.views)
+        out = {"views": len(views), "pixels": px, "triangles": Scene(3, 3).triangle_count,
+               "ms_per_batch_frame": round(ms_batch, 4), "views_per_s": round(len(views) / ms_batch * 1000.0, 1),
+               "loop_ms_per_frame": round(ms_loop, 4), "loop_views_per_s": round(len(views) / ms_loop * 1000.0, 1),
+               "speedup_vs_loop": round(ms_loop / ms_batch, 3), "tuning": list(tuning),
+               "overflow": int(st.overflow)}
+        batch.close()
+        return out
+    finally:
+        r.close()
+
+
 def cpu_baseline(sc, cam, W, H, flags, budget_s=12.0, max_frames=8):
     """The oracle (scalar C restatement, 1 thread) on frames of the same workload."""
     import oracle
@@ -101,6 +150,8 @@ def main():
     ap.add_argument("--verbose", action="store_true")
     ap.add_argument("--small-max", type=int, default=0, help="raster tunable (0: library default)")
     ap.add_argument("--blocks-per-cu", type=int, default=0, help="raster tunable (0: library default)")
+    ap.add_argument("--no-batch", action="store_true", help="skip the 64-view batched-vs-loop measurement")
+    ap.add_argument("--batch-only", action="store_true", help="only the 64-view batched-vs-loop measurement")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run three plain 256 MiB device-to-device copies (the reference kernel for scaling "
                          "rocprofv3 FETCH_SIZE / WRITE_SIZE to bytes) and exit")
@@ -119,6 +170,11 @@ def main():
               "(views shard one per GPU, see the design notes)", file=sys.stderr)
         sys.exit(2)
 
+    log = (lambda m: print(m, file=sys.stderr)) if args.verbose else None
+    if args.batch_only:
+        print(json.dumps({"batched": batched_views(args.steps, args.warmup, log)}))
+        return
+
     t0 = time.perf_counter()
     scene = Scene(args.config, args.config, args.variant)
     sc = scene.arrays()
@@ -130,8 +186,8 @@ def main():
 
     r = Renderer(0)
     if args.small_max or args.blocks_per_cu:
-        sm, bpc = r.get_tuning()
-        r.set_tuning(args.small_max or sm, args.blocks_per_cu or bpc)
+        sm, bpc, tit = r.get_tuning()
+        r.set_tuning(args.small_max or sm, args.blocks_per_cu or bpc, tit)
     r.load(sc)
     view = r.create_view(W, H)
     flags = N.CULL_FRUSTUM | N.CULL_TWO_PASS | N.RESOLVE_BARY
@@ -203,8 +259,10 @@ def main():
         print(f"scene generation {gen_s:.2f}s; tuning {r.get_tuning()}; stats: big tris {st.big_triangles} "
               f"tile pairs {st.big_tiles} max/tile {st.max_tile_tris} clipped {st.clipped_triangles}; "
               f"passes {json.dumps({k: round(v, 4) for k, v in passes.items()})}", file=sys.stderr)
-    print(json.dumps(out))
     r.close()
+    if not args.no_batch:
+        out["batched"] = batched_views(min(args.steps, 20), args.warmup, log)
+    print(json.dumps(out))
 
 
 if __name__ == "__main__":

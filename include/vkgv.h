@@ -148,6 +148,7 @@ typedef struct vkgv_primitive_desc {
 #define VKGV_TIMING              (1u << 6)  /* record per-pass HIP events (vkgv_view_read_stats) */
 #define VKGV_NO_BACKFACE         (1u << 7)  /* disable per-triangle back-face culling */
 #define VKGV_SKIP_RESOLVE        (1u << 8)  /* leave the colour image untouched */
+#define VKGV_GRAPH               (1u << 9)  /* vkgv_render_batch: replay the batch's frame as one captured hipGraph */
 
 /* The reference's own frame: frustum + previous-frame Hi-Z occlusion, pyramid
  * rebuilt after the resolve (application.cpp:843-1003). */
@@ -183,12 +184,15 @@ typedef struct vkgv_create_info {
 
 typedef struct vkgv_ctx  vkgv_ctx;
 typedef struct vkgv_view vkgv_view;
+typedef struct vkgv_batch vkgv_batch;
 
 /* Raster tunables (the autotuner's knobs; per context). */
 typedef struct vkgv_tuning {
     uint32_t small_max_extent;       /* pixels: triangles with a smaller bounding box are rasterized by
                                         the meshlet's own wave, larger ones are binned (1..64, default 16) */
     uint32_t raster_blocks_per_cu;   /* persistent meshlet-raster grid = CUs * this (1..16, default 3) */
+    uint32_t tile_item_tris;         /* binned path: a tile whose list holds more triangles is split into
+                                        up to 8 work items of about this many (1..256, default 16) */
 } vkgv_tuning;
 
 /* Status codes */
@@ -265,6 +269,24 @@ int vkgv_view_write_depth(vkgv_ctx* ctx, vkgv_view* view, const float* depth);  
 int vkgv_view_read_visible(vkgv_ctx* ctx, vkgv_view* view, uint32_t pass,
                            uint32_t* out, uint32_t capacity, uint32_t* count);
 int vkgv_view_read_stats(vkgv_ctx* ctx, vkgv_view* view, vkgv_frame_stats* stats);
+
+/* ------------------------------------------------------------------------ */
+/* Batches: many views of uneven size rendered by one sequence of launches
+ * (every kernel covers all views), optionally replayed as one captured
+ * hipGraph (VKGV_GRAPH).  No counterpart in the reference, which records one
+ * view per frame (application.cpp:843-1003); this is the batched form of the
+ * same frame.  A batch refers to its views (at most 256, each once); destroy
+ * the batch before its views.  After vkgv_render_batch the views' images and
+ * readbacks are those of the batch frame. */
+int  vkgv_batch_create(vkgv_ctx* ctx, vkgv_view* const* views, uint32_t count, vkgv_batch** out);
+void vkgv_batch_destroy(vkgv_batch* batch);
+/* frames[i] is view i's camera (and eye); flags and stream must be the same
+ * for every view.  With VKGV_GRAPH the frame is captured on first use (and
+ * again after the flags, stream, tuning or draw list change) and replayed. */
+int  vkgv_render_batch(vkgv_ctx* ctx, vkgv_batch* batch, const vkgv_frame_desc* frames, uint32_t count);
+/* Counters summed over the views; ms_total averaged over the VKGV_TIMING
+ * frames since the previous call. */
+int  vkgv_batch_read_stats(vkgv_ctx* ctx, vkgv_batch* batch, vkgv_frame_stats* stats);
 
 /* Visibility-buffer key layout of this implementation: one 64-bit word per
  * pixel, (~float_bits(depth) << 32) | (0xFFFFFFFE - id), min-reduced with

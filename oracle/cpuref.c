@@ -494,7 +494,10 @@ static void fill_triangle(svtx v0, svtx v1, svtx v2, uint32_t id, uint32_t w, ui
     }
 }
 
-/* Sutherland-Hodgman against a*x + b*y + c*z + d*w >= 0. */
+/* Sutherland-Hodgman against a*x + b*y + c*z + d*w >= 0.  A convex polygon
+ * gains at most one vertex per plane (3 + 5 <= CLIP_MAX); input that crosses
+ * a plane more often (non-finite or degenerate) is dropped (returns 0). */
+#define CLIP_MAX 16
 static int clip_plane(float (*in)[4], int n, float (*out)[4], float a, float b, float c, float d)
 {
     int m = 0;
@@ -503,14 +506,15 @@ static int clip_plane(float (*in)[4], int n, float (*out)[4], float a, float b, 
         const float* q = in[(i + 1) % n];
         const float dp = ((a * p[0] + b * p[1]) + c * p[2]) + d * p[3];
         const float dq = ((a * q[0] + b * q[1]) + c * q[2]) + d * q[3];
-        if (dp >= 0.0f) { memcpy(out[m], p, 16); ++m; }
+        if (dp >= 0.0f) { if (m < CLIP_MAX) memcpy(out[m], p, 16); ++m; }
         if ((dp >= 0.0f) != (dq >= 0.0f)) {
             const float t = dp / (dp - dq);
-            for (int k = 0; k < 4; ++k) out[m][k] = p[k] + t * (q[k] - p[k]);
+            if (m < CLIP_MAX)
+                for (int k = 0; k < 4; ++k) out[m][k] = p[k] + t * (q[k] - p[k]);
             ++m;
         }
     }
-    return m;
+    return m <= CLIP_MAX ? m : 0;
 }
 
 void ref_raster_triangle(const float* c0, const float* c1, const float* c2, uint32_t id,
@@ -522,7 +526,7 @@ void ref_raster_triangle(const float* c0, const float* c1, const float* c2, uint
         fill_triangle(s0, s1, s2, id, w, h, keys);
         return;
     }
-    float bufa[16][4], bufb[16][4];
+    float bufa[CLIP_MAX][4], bufb[CLIP_MAX][4];
     memcpy(bufa[0], c0, 16); memcpy(bufa[1], c1, 16); memcpy(bufa[2], c2, 16);
     const float gx = CLIP_GUARD_PX / (float)w, gy = CLIP_GUARD_PX / (float)h;
     int n = 3;
@@ -532,7 +536,7 @@ void ref_raster_triangle(const float* c0, const float* c1, const float* c2, uint
     n = clip_plane(bufb, n, bufa, 0.0f, -1.0f, 0.0f, gy);     /* gy*w - y >= 0 */
     n = clip_plane(bufa, n, bufb, 0.0f, 1.0f, 0.0f, gy);      /* gy*w + y >= 0 */
     if (n < 3) return;
-    svtx sv[16];
+    svtx sv[CLIP_MAX];
     for (int i = 0; i < n; ++i)
         if (!project_vtx(bufb[i], w, h, &sv[i])) return;
     for (int i = 1; i + 1 < n; ++i)

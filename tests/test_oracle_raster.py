@@ -309,3 +309,27 @@ def test_cull_sets_subset_and_order():
     fr = oracle.cull(rs, cam, pyr, v.width, v.height, N.CULL_FRUSTUM)
     assert len(allv) == len(s.draws)
     assert np.all(np.diff(fr.astype(np.int64)) > 0) and set(fr.tolist()) <= set(allv.tolist())
+
+
+def test_clipping_non_finite_and_degenerate_input_is_bounded():
+    """Triangles with infinite, NaN or huge clip coordinates go through the
+    clip path without writing past its polygon buffers: they rasterize to
+    nothing or to in-viewport pixels (regression: a plane crossed more often
+    than a convex polygon allows)."""
+    W, H = 32, 24
+    inf, nan = np.inf, np.nan
+    cases = [
+        ([inf, 0, 0.5, 1], [0, inf, 0.5, 1], [-inf, -inf, 0.5, 1]),
+        ([nan, 0, 0.5, 1], [0.5, 0.5, 0.5, 1], [nan, nan, nan, nan]),
+        ([1e30, -1e30, 0.2, -1], [-1e30, 1e30, 0.9, 2], [0.1, 0.1, 5, -0.5]),
+        ([inf, nan, -inf, 1], [-inf, nan, inf, -1], [nan, inf, -inf, 0]),
+        ([0.1, 0.1, 0.5, 1e-38], [0.2, -0.3, 0.5, -1e-38], [3e38, 3e38, 3e38, 3e38]),
+    ]
+    rng = np.random.default_rng(17)
+    for _ in range(300):
+        c = rng.choice([0.0, 1.0, -1.0, 1e-30, 1e30, inf, -inf, nan], size=(3, 4))
+        cases.append(tuple(c))
+    for c0, c1, c2 in cases:
+        keys = oracle.raster_triangle(c0, c1, c2, 9, W, H)
+        cov = keys != CLEAR
+        assert np.all(oracle.keys_ids(keys[cov]) == 9)

@@ -36,6 +36,7 @@ BUILD_PYRAMID = 1 << 5
 TIMING = 1 << 6
 NO_BACKFACE = 1 << 7
 SKIP_RESOLVE = 1 << 8
+GRAPH = 1 << 9
 MODE_REFERENCE = CULL_FRUSTUM | CULL_OCCLUSION_PREV | BUILD_PYRAMID
 
 OK = 0
@@ -111,7 +112,7 @@ class FrameStats(C.Structure):
 
 
 class Tuning(C.Structure):
-    _fields_ = [("small_max_extent", C.c_uint32), ("raster_blocks_per_cu", C.c_uint32)]
+    _fields_ = [("small_max_extent", C.c_uint32), ("raster_blocks_per_cu", C.c_uint32), ("tile_item_tris", C.c_uint32)]
 
 
 class CreateInfo(C.Structure):
@@ -194,6 +195,7 @@ HIP_SYMBOLS = [
     "vkgv_view_reset_history", "vkgv_view_pyramid_info", "vkgv_view_read_visbuffer", "vkgv_view_read_ids_depth",
     "vkgv_view_read_color", "vkgv_view_read_pyramid", "vkgv_view_write_pyramid", "vkgv_view_write_depth",
     "vkgv_view_read_visible", "vkgv_view_read_stats", "vkgv_set_tuning", "vkgv_get_tuning", "vkgv_srgb_thresholds",
+    "vkgv_batch_create", "vkgv_batch_destroy", "vkgv_render_batch", "vkgv_batch_read_stats",
 ]
 
 
@@ -226,5 +228,9 @@ def hip():
         _bind(lib, "vkgv_set_tuning", C.c_int, _p, C.POINTER(Tuning))
         _bind(lib, "vkgv_get_tuning", C.c_int, _p, C.POINTER(Tuning))
         _bind(lib, "vkgv_srgb_thresholds", None, _p)
+        _bind(lib, "vkgv_batch_create", C.c_int, _p, _p, _u32, C.POINTER(_p))
+        _bind(lib, "vkgv_batch_destroy", None, _p)
+        _bind(lib, "vkgv_render_batch", C.c_int, _p, _p, _p, _u32)
+        _bind(lib, "vkgv_batch_read_stats", C.c_int, _p, _p, C.POINTER(FrameStats))
         _hip = lib
     return _hip

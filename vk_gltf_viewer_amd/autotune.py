@@ -1,0 +1,87 @@
+"""Autotuner for the raster tunables of a context (vkgv_set_tuning).
+
+The knobs:
+- `small_max_extent` sets the bin boundary. Triangles whose bounding box is
+  smaller stay in their meshlet's wave; larger ones go to the binned tile pass.
+- `raster_blocks_per_cu` sets the persistent raster grid, i.e. the
+  wave-to-meshlet mapping.
+- `tile_item_tris` is the tile pass's work-item size.
+
+A candidate is timed on the caller's own frame: a function that records one
+frame (a batch or a view) on the context's stream. The fastest candidate is
+kept and cached in a JSON file under a key the caller chooses (device, scene,
+views).
+
+The tuner only changes how work is split. It never changes the frame's
+results, which are identical for every tuning: the parity tests run at
+several tunings.
+"""
+from __future__ import annotations
+
+import itertools
+import json
+import os
+import time
+
+DEFAULT_CANDIDATES = {
+    "small_max_extent": (4, 8, 16, 32),
+    "raster_blocks_per_cu": (2, 3),
+    "tile_item_tris": (8, 16, 32),
+}
+
+
+def cache_path() -> str:
+    return os.environ.get("VKGV_TUNE_CACHE",
+                          os.path.join(os.path.expanduser("~"), ".cache", "vk_gltf_viewer_amd", "tune.json"))
+
+
+def _load(path):
+    try:
+        with open(path) as f:
+            return json.load(f)
+    except (OSError, ValueError):
+        return {}
+
+
+def _store(path, data):
+    try:
+        os.makedirs(os.path.dirname(path), exist_ok=True)
+        tmp = path + ".tmp"
+        with open(tmp, "w") as f:
+            json.dump(data, f, indent=1, sort_keys=True)
+        os.replace(tmp, path)
+    except OSError:
+        pass        # a read-only home is not an error: the result is still applied
+
+
+def autotune(renderer, frame, key: str, candidates=None, warmup: int = 2, frames: int = 5, use_cache: bool = True,
+             log=None):
+    """Times `frame()` (which records one frame on the renderer's stream)
+    under each candidate tuning and applies the fastest.  Returns
+    (tuning tuple, {tuning: ms per frame})."""
+    path = cache_path()
+    cache = _load(path) if use_cache else {}
+    if use_cache and key in cache:
+        t = tuple(cache[key]["tuning"])
+        renderer.set_tuning(*t)
+        return t, {}
+    cand = candidates or DEFAULT_CANDIDATES
+    times = {}
+    for t in itertools.product(cand["small_max_extent"], cand["raster_blocks_per_cu"], cand["tile_item_tris"]):
+        renderer.set_tuning(*t)
+        for _ in range(warmup):
+            frame()
+        renderer.sync()
+        t0 = time.perf_counter()
+        for _ in range(frames):
+            frame()
+        renderer.sync()
+        times[t] = 1000.0 * (time.perf_counter() - t0) / frames
+        if log:
+            log(f"autotune {key}: {t} {times[t]:.4f} ms")
+    best = min(times, key=times.get)
+    renderer.set_tuning(*best)
+    if use_cache:
+        cache[key] = {"tuning": list(best), "ms": times[best]}
+        _store(path, cache)
+    return best, times

@@ -49,15 +49,16 @@ struct vkgv_ctx {
     std::vector<uint32_t> prim_meshlets;     // meshlet count per primitive (host copy for validation)
     bool scene = false;
     // frame inputs
-    DevBuf draws, cull, transforms;
+    DevBuf draws, cull, rrec, transforms;
     uint32_t draw_count = 0, transform_count = 0;
     uint32_t max_transform = 0;
     bool draws_set = false;
-    RasterTuning tuning{ 16u, 3u };
+    RasterTuning tuning{ 16u, 3u, 16u };
     DevBuf srgb;                     // 255 sRGB-encode thresholds
     uint32_t big_capacity = 1u << 18;
     uint32_t clip_capacity = 1u << 16;
     uint32_t cache_slots = 8192;     // resolve-cache meshlet slots per view
+    uint64_t generation = 0;         // bumped when view tables must be rebuilt (draw lists reallocated)
 };
 
 struct vkgv_view {
@@ -74,6 +75,37 @@ struct vkgv_view {
     uint32_t ring_flags[kRing] = {};
     bool timed = false;
     uint32_t last_flags = 0;
+    hipStream_t last_stream = nullptr;
+    // the view as a one-view batch: block maps and launch sizes
+    DevBuf maps;
+    BatchDev batch{};
+    BatchGrid grid{};
+};
+
+// A set of views rendered by one sequence of launches (optionally one
+// captured hipGraph).
+struct vkgv_batch {
+    vkgv_ctx* ctx = nullptr;
+    std::vector<vkgv_view*> views;
+    DevBuf table, maps, cams;
+    BatchDev batch{};
+    BatchGrid grid{};
+    uint64_t generation = ~0ull;
+    // pinned staging ring for the per-frame cameras
+    static constexpr uint32_t kStage = 4;
+    CameraArg* staging[kStage] = {};
+    hipEvent_t staged[kStage] = {};
+    bool stage_used[kStage] = {};
+    uint32_t stage_next = 0;
+    // captured frame
+    hipGraphExec_t exec = nullptr;
+    uint32_t exec_flags = 0;
+    hipStream_t exec_stream = nullptr;
+    RasterTuning exec_tuning{ 0u, 0u, 0u };
+    // whole-frame timing ring
+    static constexpr uint32_t kRing = 64;
+    hipEvent_t ev0[kRing] = {}, ev1[kRing] = {};
+    uint32_t ring_next = 0, ring_frames = 0;
     hipStream_t last_stream = nullptr;
 };
 
@@ -135,6 +167,12 @@ SceneDev scene_dev(const vkgv_ctx* c)
     s.prim_cone_base = static_cast<const uint32_t*>(c->cone_base.p);
     s.cones = static_cast<const vkgv_meshlet_cone*>(c->cones.p);
     s.cull = static_cast<const CullRec*>(c->cull.p);
+    s.rrec = static_cast<const RasterRec*>(c->rrec.p);
+    if (c->mesh_bufs.size() == 4) {
+        s.vidx_base = static_cast<const uint32_t*>(c->mesh_bufs[0].p);
+        s.tri_base = static_cast<const uint8_t*>(c->mesh_bufs[1].p);
+        s.vtx_base = static_cast<const vkgv_vertex*>(c->mesh_bufs[2].p);
+    }
     s.srgb = c->srgb.p;
     return s;
 }
@@ -180,16 +218,17 @@ int ensure_lists(vkgv_ctx* c, vkgv_view* v)
     if (v->list_capacity >= c->draw_count && v->visible0.p) return VKGV_OK;
     const uint32_t cap = std::max<uint32_t>(c->draw_count, 64u);
     int rc;
-    if ((rc = dalloc(c, v->visible0, sizeof(uint32_t) * cap))) return rc;
-    if ((rc = dalloc(c, v->visible1, sizeof(uint32_t) * cap))) return rc;
+    if ((rc = dalloc(c, v->visible0, sizeof(RasterRec) * cap))) return rc;
+    if ((rc = dalloc(c, v->visible1, sizeof(RasterRec) * cap))) return rc;
     const size_t hwords = (cap + 63u) / 64u;
     if ((rc = dalloc(c, v->history, sizeof(uint64_t) * hwords))) return rc;
     if ((rc = dalloc(c, v->draw_slot, sizeof(uint32_t) * cap))) return rc;
     HIP_TRY(c, hipMemsetAsync(v->history.p, 0, v->history.bytes, c->stream));
     HIP_TRY(c, hipStreamSynchronize(c->stream));   // frames may run on another stream
     v->list_capacity = cap;
-    v->dev.visible[0] = static_cast<uint32_t*>(v->visible0.p);
-    v->dev.visible[1] = static_cast<uint32_t*>(v->visible1.p);
+    c->generation++;
+    v->dev.visible[0] = static_cast<RasterRec*>(v->visible0.p);
+    v->dev.visible[1] = static_cast<RasterRec*>(v->visible1.p);
     v->dev.history = static_cast<uint32_t*>(v->history.p);
     v->dev.draw_slot = static_cast<uint32_t*>(v->draw_slot.p);
     return VKGV_OK;
@@ -206,6 +245,102 @@ int sync_view(vkgv_ctx* c, vkgv_view* v)
 {
     if (v->last_stream && v->last_stream != c->stream) HIP_TRY(c, hipStreamSynchronize(v->last_stream));
     HIP_TRY(c, hipStreamSynchronize(c->stream));
+    return VKGV_OK;
+}
+
+// Block maps of the pixel-parallel launches of a set of views (entry = view |
+// local block << 8) and the launch sizes.
+int build_maps(vkgv_ctx* c, const std::vector<const ViewDev*>& views, DevBuf& maps, BatchDev& bd, BatchGrid& g)
+{
+    std::vector<uint32_t> clear, hiz, resolve;
+    g = BatchGrid{};
+    g.nviews = (uint32_t)views.size();
+    for (uint32_t vi = 0; vi < views.size(); ++vi) {
+        const ViewDev& v = *views[vi];
+        const uint64_t npx = (uint64_t)v.width * v.height;
+        const uint64_t nclear = (npx + kClearKeysPerBlock - 1) / kClearKeysPerBlock;
+        const uint64_t nhiz = v.pyr.levels ? (uint64_t)((v.pyr.w[0] + 31) / 32) * ((v.pyr.h[0] + 31) / 32) : 0;
+        const uint64_t nres = (uint64_t)v.resolve_blocks_x * ((v.height + 15) / 16);
+        if (std::max({ nclear, nhiz, nres }) >= (1ull << 24))
+            return fail(c, VKGV_ERROR_LIMIT, "view too large for a batch block map");
+        for (uint64_t k = 0; k < nclear; ++k) clear.push_back(vi | (uint32_t)(k << 8));
+        for (uint64_t k = 0; k < nhiz; ++k) hiz.push_back(vi | (uint32_t)(k << 8));
+        for (uint64_t k = 0; k < nres; ++k) resolve.push_back(vi | (uint32_t)(k << 8));
+        if (v.hiz_tile_levels < v.pyr.levels) g.max_tail_levels = std::max(g.max_tail_levels, v.pyr.levels);
+    }
+    g.clear_blocks = (uint32_t)clear.size();
+    g.hiz_blocks = (uint32_t)hiz.size();
+    g.resolve_blocks = (uint32_t)resolve.size();
+    std::vector<uint32_t> all;
+    all.insert(all.end(), clear.begin(), clear.end());
+    all.insert(all.end(), hiz.begin(), hiz.end());
+    all.insert(all.end(), resolve.begin(), resolve.end());
+    int rc = dalloc(c, maps, sizeof(uint32_t) * std::max<size_t>(all.size(), 1));
+    if (rc) return rc;
+    HIP_TRY(c, hipMemcpy(maps.p, all.data(), sizeof(uint32_t) * all.size(), hipMemcpyHostToDevice));
+    const uint32_t* base = static_cast<const uint32_t*>(maps.p);
+    bd.map_clear = base;
+    bd.map_hiz = base + clear.size();
+    bd.map_resolve = base + clear.size() + hiz.size();
+    bd.nviews = g.nviews;
+    return VKGV_OK;
+}
+
+// Device copy of the views' descriptors (synchronous; only when they change).
+int upload_table(vkgv_ctx* c, const std::vector<const ViewDev*>& views, DevBuf& table, BatchDev& bd)
+{
+    std::vector<ViewDev> host;
+    for (const ViewDev* v : views) host.push_back(*v);
+    if (!table.p || table.bytes < sizeof(ViewDev) * host.size()) {
+        int rc = dalloc(c, table, sizeof(ViewDev) * host.size());
+        if (rc) return rc;
+    }
+    HIP_TRY(c, hipMemcpyAsync(table.p, host.data(), sizeof(ViewDev) * host.size(), hipMemcpyHostToDevice, c->stream));
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    bd.views = static_cast<const ViewDev*>(table.p);
+    return VKGV_OK;
+}
+
+// The frame's launches for a set of views (application.cpp:843-1003); `mark`
+// records the pass boundaries when timing.
+template <typename Mark>
+void frame_launches(vkgv_ctx* c, const BatchDev& b, const BatchGrid& g, uint32_t flags, hipStream_t st, Mark mark)
+{
+    const SceneDev s = scene_dev(c);
+    const bool two_pass = flags & VKGV_CULL_TWO_PASS;
+    const uint32_t raster_grid = (uint32_t)c->cus * c->tuning.raster_blocks_per_cu;
+    const uint32_t tile_grid = (uint32_t)c->cus * 8u;
+    mark(EV_START);
+    launch_frame_start(b, g, st);
+    mark(EV_CLEAR);
+    launch_cull(s, b, g, flags, two_pass ? CULL_EARLY : CULL_SINGLE, 0, st);
+    mark(EV_CULL);
+    launch_raster(s, b, flags, 0, raster_grid, c->tuning.small_max_extent, st);
+    launch_raster_big(b, g, tile_grid, c->tuning.tile_item_tris, st);
+    mark(EV_RASTER);
+    if (two_pass) {
+        launch_hiz(b, g, st);
+        mark(EV_HIZ);
+        launch_cull(s, b, g, flags, CULL_LATE, 1, st);
+        mark(EV_CULL_LATE);
+        launch_raster(s, b, flags, 1, raster_grid, c->tuning.small_max_extent, st);
+        launch_raster_big(b, g, tile_grid, c->tuning.tile_item_tris, st);
+        mark(EV_RASTER_LATE);
+    }
+    if (!(flags & VKGV_SKIP_RESOLVE)) launch_resolve(s, b, g, flags, st);
+    mark(EV_RESOLVE);
+    if (!two_pass && (flags & VKGV_BUILD_PYRAMID)) launch_hiz(b, g, st);
+    mark(EV_PYRAMID);
+}
+
+int check_frame_inputs(vkgv_ctx* c, uint32_t flags)
+{
+    if (!c->draws_set) return fail(c, VKGV_ERROR_STATE, "render: no draw list");
+    if (c->draw_count && c->max_transform >= c->transform_count)
+        return fail(c, VKGV_ERROR_STATE, "render: draw references transform " + std::to_string(c->max_transform)
+                                             + " but only " + std::to_string(c->transform_count) + " are set");
+    if ((flags & VKGV_CULL_TWO_PASS) && (flags & VKGV_CULL_OCCLUSION_PREV))
+        return fail(c, VKGV_ERROR_INVALID_ARGUMENT, "render: CULL_TWO_PASS and CULL_OCCLUSION_PREV are exclusive");
     return VKGV_OK;
 }
 
@@ -267,10 +402,13 @@ void vkgv_srgb_thresholds(float out[255])
 int vkgv_set_tuning(vkgv_ctx* c, const vkgv_tuning* t)
 {
     if (!c || !t) return VKGV_ERROR_INVALID_ARGUMENT;
-    if (t->small_max_extent < 1 || t->small_max_extent > 64 || t->raster_blocks_per_cu < 1 || t->raster_blocks_per_cu > 16)
-        return fail(c, VKGV_ERROR_INVALID_ARGUMENT, "set_tuning: small_max_extent 1..64, raster_blocks_per_cu 1..16");
+    if (t->small_max_extent < 1 || t->small_max_extent > 64 || t->raster_blocks_per_cu < 1 || t->raster_blocks_per_cu > 16
+        || t->tile_item_tris < 1 || t->tile_item_tris > 256)
+        return fail(c, VKGV_ERROR_INVALID_ARGUMENT,
+                    "set_tuning: small_max_extent 1..64, raster_blocks_per_cu 1..16, tile_item_tris 1..256");
     c->tuning.small_max_extent = t->small_max_extent;
     c->tuning.raster_blocks_per_cu = t->raster_blocks_per_cu;
+    c->tuning.tile_item_tris = t->tile_item_tris;
     return VKGV_OK;
 }
 
@@ -279,6 +417,7 @@ int vkgv_get_tuning(const vkgv_ctx* c, vkgv_tuning* t)
     if (!c || !t) return VKGV_ERROR_INVALID_ARGUMENT;
     t->small_max_extent = c->tuning.small_max_extent;
     t->raster_blocks_per_cu = c->tuning.raster_blocks_per_cu;
+    t->tile_item_tris = c->tuning.tile_item_tris;
     return VKGV_OK;
 }
 
@@ -289,7 +428,7 @@ void vkgv_destroy(vkgv_ctx* c)
     (void)hipStreamSynchronize(c->stream);
     for (auto& b : c->mesh_bufs) dfree(b);
     dfree(c->prims); dfree(c->materials); dfree(c->cone_base); dfree(c->cones);
-    dfree(c->draws); dfree(c->cull); dfree(c->transforms); dfree(c->srgb);
+    dfree(c->draws); dfree(c->cull); dfree(c->rrec); dfree(c->transforms); dfree(c->srgb);
     (void)hipStreamDestroy(c->stream);
     delete c;
 }
@@ -375,6 +514,10 @@ int vkgv_upload_scene(vkgv_ctx* c, const vkgv_mesh_desc* meshes, uint32_t mesh_c
         }
     int rc;
     for (int k = 0; k < 4; ++k) {
+        // raster records address the packed buffers with 32-bit offsets (elements
+        // for vertex indices, bytes for triangles and vertices)
+        if (total[k] / (k == 0 ? 4u : 1u) >= (1ull << 32))
+            return fail(c, VKGV_ERROR_LIMIT, "upload_scene: packed mesh buffer beyond 32-bit offsets");
         std::vector<uint8_t> staging(total[k] ? total[k] : 1, 0);
         for (uint32_t mi = 0; mi < mesh_count; ++mi)
             if (part_bytes(meshes[mi], k)) std::memcpy(staging.data() + off[4 * mi + k], part_src(meshes[mi], k), part_bytes(meshes[mi], k));
@@ -438,10 +581,12 @@ int vkgv_set_draws(vkgv_ctx* c, const vkgv_meshlet_draw* draws, uint32_t count)
     int rc;
     if ((rc = upload(c, c->draws, draws, sizeof(vkgv_meshlet_draw) * count))) return rc;
     if ((rc = dalloc(c, c->cull, sizeof(CullRec) * std::max<uint32_t>(count, 1u)))) return rc;
+    if ((rc = dalloc(c, c->rrec, sizeof(RasterRec) * std::max<uint32_t>(count, 1u)))) return rc;
     c->draw_count = count;
     c->max_transform = maxt;
     c->draws_set = true;
-    launch_build_cull_records(scene_dev(c), static_cast<CullRec*>(c->cull.p), c->stream);
+    launch_build_cull_records(scene_dev(c), static_cast<CullRec*>(c->cull.p), static_cast<RasterRec*>(c->rrec.p),
+                              c->stream);
     HIP_TRY(c, hipGetLastError());
     HIP_TRY(c, hipStreamSynchronize(c->stream));
     return VKGV_OK;
@@ -515,7 +660,6 @@ int vkgv_view_create(vkgv_ctx* c, uint32_t width, uint32_t height, vkgv_view** o
     if (rc) { vkgv_view_destroy(v); return rc; }
     d.chunk_tri = static_cast<uint32_t*>(v->chunk_tri.p);
     d.tile_items = static_cast<TileItem*>(v->tile_items.p);
-    d.tile_grid = std::max<uint32_t>(1u, (uint32_t)c->cus) * 8u;
     d.setups = static_cast<BigSetup*>(v->setups.p);
     d.pairs = static_cast<BinPair*>(v->pairs.p);
     d.tri_cache = static_cast<TriCache*>(v->tri_cache.p);
@@ -536,6 +680,13 @@ int vkgv_view_create(vkgv_ctx* c, uint32_t width, uint32_t height, vkgv_view** o
     d.bin_start = static_cast<uint32_t*>(v->bin_start.p);
     d.bins = static_cast<uint32_t*>(v->bins.p);
     d.bin_capacity = bin_cap;
+    d.hiz_tile_levels = hiz_tile_levels(d.pyr);
+    d.hiz_tiles_x = (d.pyr.w[0] + 31) / 32;
+    d.resolve_blocks_x = (width + 63) / 64;
+    {
+        const std::vector<const ViewDev*> one{ &v->dev };
+        if ((rc = build_maps(c, one, v->maps, v->batch, v->grid))) { vkgv_view_destroy(v); return rc; }
+    }
     // first frame: pyramid = far (0.0), so the first cull is frustum-only
     // (the reference leaves it undefined, application.cpp:743); empty visbuffer.
     if (hipMemsetAsync(v->pyramid.p, 0, v->pyramid.bytes, c->stream) != hipSuccess
@@ -565,7 +716,7 @@ void vkgv_view_destroy(vkgv_view* v)
     }
     for (DevBuf* b : { &v->keys, &v->color, &v->pyramid, &v->visible0, &v->visible1, &v->history, &v->counters, &v->big,
                        &v->clipq, &v->bin_count, &v->bin_start, &v->bins, &v->setups, &v->pairs, &v->tri_cache,
-                       &v->slot_albedo, &v->draw_slot, &v->big_chunk, &v->chunk_tri, &v->tile_items })
+                       &v->slot_albedo, &v->draw_slot, &v->big_chunk, &v->chunk_tri, &v->tile_items, &v->maps })
         dfree(*b);
     for (auto& set : v->ev)
         for (auto& e : set)
@@ -578,14 +729,8 @@ int vkgv_render(vkgv_ctx* c, vkgv_view* v, const vkgv_frame_desc* f)
     int rc = check_view(c, v);
     if (rc) return rc;
     if (!f) return fail(c, VKGV_ERROR_INVALID_ARGUMENT, "render: null frame");
-    if (!c->draws_set) return fail(c, VKGV_ERROR_STATE, "render: no draw list");
-    if (c->draw_count && c->max_transform >= c->transform_count)
-        return fail(c, VKGV_ERROR_STATE, "render: draw references transform " + std::to_string(c->max_transform)
-                                             + " but only " + std::to_string(c->transform_count) + " are set");
     const uint32_t flags = f->flags;
-    const bool two_pass = flags & VKGV_CULL_TWO_PASS;
-    if (two_pass && (flags & VKGV_CULL_OCCLUSION_PREV))
-        return fail(c, VKGV_ERROR_INVALID_ARGUMENT, "render: CULL_TWO_PASS and CULL_OCCLUSION_PREV are exclusive");
+    if ((rc = check_frame_inputs(c, flags))) return rc;
     HIP_TRY(c, hipSetDevice(c->device));
     if ((rc = ensure_lists(c, v))) return rc;
     hipStream_t st = f->stream ? static_cast<hipStream_t>(f->stream) : c->stream;
@@ -593,38 +738,13 @@ int vkgv_render(vkgv_ctx* c, vkgv_view* v, const vkgv_frame_desc* f)
     v->last_stream = st;
     const bool timed = flags & VKGV_TIMING;
     const uint32_t slot = v->ring_next;
-    auto mark = [&](Ev e) { if (timed) (void)hipEventRecord(v->ev[slot][e], st); };
-
-    const SceneDev s = scene_dev(c);
-    CameraArg cam;
-    cam.cam = f->camera;
-    cam.eye[0] = f->eye[0]; cam.eye[1] = f->eye[1]; cam.eye[2] = f->eye[2];
-    const Mat4 vp = to_mat4(f->camera.view_projection);
-    const uint32_t raster_grid = (uint32_t)c->cus * c->tuning.raster_blocks_per_cu;
-    const uint64_t npx = (uint64_t)v->dev.width * v->dev.height;
-
-    mark(EV_START);
-    launch_reset_counters(v->dev, 0xFFFFFFFFu, st);
-    launch_clear_keys(v->dev.keys, npx, st);
-    mark(EV_CLEAR);
-    launch_cull(s, v->dev, cam, flags, two_pass ? CULL_EARLY : CULL_SINGLE, 0, st);
-    mark(EV_CULL);
-    launch_raster(s, v->dev, vp, flags, 0, raster_grid, c->tuning.small_max_extent, st);
-    launch_raster_big(v->dev, st);
-    mark(EV_RASTER);
-    if (two_pass) {
-        launch_hiz(v->dev, st);
-        mark(EV_HIZ);
-        launch_cull(s, v->dev, cam, flags, CULL_LATE, 1, st);
-        mark(EV_CULL_LATE);
-        launch_raster(s, v->dev, vp, flags, 1, raster_grid, c->tuning.small_max_extent, st);
-        launch_raster_big(v->dev, st);
-        mark(EV_RASTER_LATE);
-    }
-    if (!(flags & VKGV_SKIP_RESOLVE)) launch_resolve(s, v->dev, vp, flags, st);
-    mark(EV_RESOLVE);
-    if (!two_pass && (flags & VKGV_BUILD_PYRAMID)) launch_hiz(v->dev, st);
-    mark(EV_PYRAMID);
+    BatchDev b = v->batch;
+    b.views = nullptr;
+    b.view0 = v->dev;
+    b.cams = nullptr;
+    b.cam0.cam = f->camera;
+    b.cam0.eye[0] = f->eye[0]; b.cam0.eye[1] = f->eye[1]; b.cam0.eye[2] = f->eye[2];
+    frame_launches(c, b, v->grid, flags, st, [&](Ev e) { if (timed) (void)hipEventRecord(v->ev[slot][e], st); });
     HIP_TRY(c, hipGetLastError());
     if (timed) {
         v->ring_flags[slot] = flags;
@@ -642,7 +762,11 @@ int vkgv_view_build_pyramid(vkgv_ctx* c, vkgv_view* v, void* stream)
     if (rc) return rc;
     HIP_TRY(c, hipSetDevice(c->device));
     hipStream_t st = stream ? static_cast<hipStream_t>(stream) : c->stream;
-    launch_hiz(v->dev, st);
+    BatchDev b = v->batch;
+    b.views = nullptr;
+    b.view0 = v->dev;
+    b.cams = nullptr;
+    launch_hiz(b, v->grid, st);
     HIP_TRY(c, hipGetLastError());
     v->last_stream = st;
     return VKGV_OK;
@@ -766,7 +890,9 @@ int vkgv_view_read_visible(vkgv_ctx* c, vkgv_view* v, uint32_t pass, uint32_t* o
     if (n && out) {
         const void* src = pass ? v->visible1.p : v->visible0.p;
         if (!src) return fail(c, VKGV_ERROR_STATE, "read_visible: no frame rendered");
-        if ((rc = copy_out(c, v, out, src, 4ull * n))) return rc;
+        std::vector<RasterRec> recs(n);
+        if ((rc = copy_out(c, v, recs.data(), src, sizeof(RasterRec) * n))) return rc;
+        for (uint32_t i = 0; i < n; ++i) out[i] = recs[i].draw;
     }
     return VKGV_OK;
 }
@@ -814,6 +940,179 @@ int vkgv_view_read_stats(vkgv_ctx* c, vkgv_view* v, vkgv_frame_stats* s)
     }
     s->timed_frames = n;
     v->ring_frames = 0;
+    return VKGV_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Batches
+
+int vkgv_batch_create(vkgv_ctx* c, vkgv_view* const* views, uint32_t count, vkgv_batch** out)
+{
+    if (!c || !out) return VKGV_ERROR_INVALID_ARGUMENT;
+    *out = nullptr;
+    if (!views || count == 0 || count > kMaxBatchViews)
+        return fail(c, VKGV_ERROR_INVALID_ARGUMENT, "batch_create: 1.." + std::to_string(kMaxBatchViews) + " views");
+    for (uint32_t i = 0; i < count; ++i) {
+        if (!views[i] || views[i]->ctx != c)
+            return fail(c, VKGV_ERROR_INVALID_ARGUMENT, "batch_create: view " + std::to_string(i) + " is not of this context");
+        for (uint32_t j = 0; j < i; ++j)
+            if (views[j] == views[i]) return fail(c, VKGV_ERROR_INVALID_ARGUMENT, "batch_create: a view appears twice");
+    }
+    HIP_TRY(c, hipSetDevice(c->device));
+    auto* b = new vkgv_batch();
+    b->ctx = c;
+    b->views.assign(views, views + count);
+    std::vector<const ViewDev*> devs;
+    for (vkgv_view* v : b->views) devs.push_back(&v->dev);
+    int rc = build_maps(c, devs, b->maps, b->batch, b->grid);
+    if (!rc) rc = dalloc(c, b->cams, sizeof(CameraArg) * count);
+    for (uint32_t k = 0; !rc && k < vkgv_batch::kStage; ++k) {
+        if (hipHostMalloc(reinterpret_cast<void**>(&b->staging[k]), sizeof(CameraArg) * count, hipHostMallocDefault)
+                != hipSuccess
+            || hipEventCreateWithFlags(&b->staged[k], hipEventDisableTiming) != hipSuccess)
+            rc = fail(c, VKGV_ERROR_HIP, "batch_create: pinned staging");
+    }
+    for (uint32_t k = 0; !rc && k < vkgv_batch::kRing; ++k)
+        if (hipEventCreate(&b->ev0[k]) != hipSuccess || hipEventCreate(&b->ev1[k]) != hipSuccess)
+            rc = fail(c, VKGV_ERROR_HIP, "batch_create: events");
+    if (rc) { vkgv_batch_destroy(b); return rc; }
+    b->batch.cams = static_cast<const CameraArg*>(b->cams.p);
+    *out = b;
+    return VKGV_OK;
+}
+
+void vkgv_batch_destroy(vkgv_batch* b)
+{
+    if (!b) return;
+    vkgv_ctx* c = b->ctx;
+    if (c) {
+        (void)hipSetDevice(c->device);
+        if (b->last_stream) (void)hipStreamSynchronize(b->last_stream);
+        (void)hipStreamSynchronize(c->stream);
+    }
+    if (b->exec) (void)hipGraphExecDestroy(b->exec);
+    for (DevBuf* d : { &b->table, &b->maps, &b->cams }) dfree(*d);
+    for (uint32_t k = 0; k < vkgv_batch::kStage; ++k) {
+        if (b->staging[k]) (void)hipHostFree(b->staging[k]);
+        if (b->staged[k]) (void)hipEventDestroy(b->staged[k]);
+    }
+    for (uint32_t k = 0; k < vkgv_batch::kRing; ++k) {
+        if (b->ev0[k]) (void)hipEventDestroy(b->ev0[k]);
+        if (b->ev1[k]) (void)hipEventDestroy(b->ev1[k]);
+    }
+    delete b;
+}
+
+int vkgv_render_batch(vkgv_ctx* c, vkgv_batch* b, const vkgv_frame_desc* frames, uint32_t count)
+{
+    if (!c) return VKGV_ERROR_INVALID_ARGUMENT;
+    if (!b || b->ctx != c) return fail(c, VKGV_ERROR_INVALID_ARGUMENT, "render_batch: batch does not belong to this context");
+    if (!frames || count != b->views.size())
+        return fail(c, VKGV_ERROR_INVALID_ARGUMENT, "render_batch: one frame description per view of the batch");
+    const uint32_t flags = frames[0].flags;
+    for (uint32_t i = 1; i < count; ++i)
+        if (frames[i].flags != flags || frames[i].stream != frames[0].stream)
+            return fail(c, VKGV_ERROR_INVALID_ARGUMENT, "render_batch: all views of a batch share flags and stream");
+    int rc = check_frame_inputs(c, flags);
+    if (rc) return rc;
+    HIP_TRY(c, hipSetDevice(c->device));
+    for (vkgv_view* v : b->views)
+        if ((rc = ensure_lists(c, v))) return rc;
+    if (b->generation != c->generation) {
+        std::vector<const ViewDev*> devs;
+        for (vkgv_view* v : b->views) devs.push_back(&v->dev);
+        if ((rc = upload_table(c, devs, b->table, b->batch))) return rc;
+        b->generation = c->generation;
+        if (b->exec) { (void)hipGraphExecDestroy(b->exec); b->exec = nullptr; }
+    }
+    hipStream_t st = frames[0].stream ? static_cast<hipStream_t>(frames[0].stream) : c->stream;
+    for (vkgv_view* v : b->views)
+        if (v->last_stream && v->last_stream != st) HIP_TRY(c, hipStreamSynchronize(v->last_stream));
+    // cameras: pinned staging slot -> device (the slot is reused once its copy has run)
+    const uint32_t k = b->stage_next;
+    if (b->stage_used[k]) HIP_TRY(c, hipEventSynchronize(b->staged[k]));
+    for (uint32_t i = 0; i < count; ++i) {
+        b->staging[k][i].cam = frames[i].camera;
+        std::memcpy(b->staging[k][i].eye, frames[i].eye, sizeof(float) * 3);
+    }
+    HIP_TRY(c, hipMemcpyAsync(b->cams.p, b->staging[k], sizeof(CameraArg) * count, hipMemcpyHostToDevice, st));
+    HIP_TRY(c, hipEventRecord(b->staged[k], st));
+    b->stage_used[k] = true;
+    b->stage_next = (k + 1) % vkgv_batch::kStage;
+
+    const bool timed = flags & VKGV_TIMING;
+    const uint32_t fl = flags & ~(VKGV_TIMING | VKGV_GRAPH);
+    const uint32_t slot = b->ring_next;
+    if (timed) HIP_TRY(c, hipEventRecord(b->ev0[slot], st));
+    if (flags & VKGV_GRAPH) {
+        const bool stale = !b->exec || b->exec_flags != fl || b->exec_stream != st
+                           || b->exec_tuning.small_max_extent != c->tuning.small_max_extent
+                           || b->exec_tuning.raster_blocks_per_cu != c->tuning.raster_blocks_per_cu
+                           || b->exec_tuning.tile_item_tris != c->tuning.tile_item_tris;
+        if (stale) {
+            if (b->exec) { (void)hipGraphExecDestroy(b->exec); b->exec = nullptr; }
+            HIP_TRY(c, hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+            frame_launches(c, b->batch, b->grid, fl, st, [](Ev) {});
+            hipGraph_t g = nullptr;
+            const hipError_t ce = hipStreamEndCapture(st, &g);
+            if (ce != hipSuccess || !g)
+                return fail(c, VKGV_ERROR_HIP, std::string("render_batch: graph capture: ") + hipGetErrorString(ce));
+            const hipError_t ie = hipGraphInstantiate(&b->exec, g, nullptr, nullptr, 0);
+            (void)hipGraphDestroy(g);
+            if (ie != hipSuccess) {
+                b->exec = nullptr;
+                return fail(c, VKGV_ERROR_HIP, std::string("render_batch: graph instantiate: ") + hipGetErrorString(ie));
+            }
+            b->exec_flags = fl;
+            b->exec_stream = st;
+            b->exec_tuning = c->tuning;
+        }
+        HIP_TRY(c, hipGraphLaunch(b->exec, st));
+    } else {
+        frame_launches(c, b->batch, b->grid, fl, st, [](Ev) {});
+    }
+    HIP_TRY(c, hipGetLastError());
+    if (timed) {
+        HIP_TRY(c, hipEventRecord(b->ev1[slot], st));
+        b->ring_next = (slot + 1) % vkgv_batch::kRing;
+        b->ring_frames = std::min(b->ring_frames + 1, vkgv_batch::kRing);
+    }
+    b->last_stream = st;
+    for (vkgv_view* v : b->views) {
+        v->last_stream = st;
+        v->last_flags = fl;
+    }
+    return VKGV_OK;
+}
+
+int vkgv_batch_read_stats(vkgv_ctx* c, vkgv_batch* b, vkgv_frame_stats* s)
+{
+    if (!c) return VKGV_ERROR_INVALID_ARGUMENT;
+    if (!b || b->ctx != c || !s) return fail(c, VKGV_ERROR_INVALID_ARGUMENT, "batch_read_stats: bad arguments");
+    HIP_TRY(c, hipSetDevice(c->device));
+    if (b->last_stream) HIP_TRY(c, hipStreamSynchronize(b->last_stream));
+    std::memset(s, 0, sizeof *s);
+    for (vkgv_view* v : b->views) {
+        Counters h{};
+        HIP_TRY(c, hipMemcpy(&h, v->counters.p, sizeof h, hipMemcpyDeviceToHost));
+        s->draws += c->draw_count;
+        s->visible_early += h.vis_count[0];
+        s->visible_late += h.vis_count[1];
+        s->big_triangles += h.big_total_tris;
+        s->big_tiles += h.big_total_pairs;
+        s->clipped_triangles += h.clipped;
+        s->overflow |= h.overflow;
+        s->max_tile_tris = std::max(s->max_tile_tris, h.max_tile_tris);
+    }
+    const uint32_t n = b->ring_frames;
+    for (uint32_t f = 0; f < n; ++f) {
+        const uint32_t slot = (b->ring_next + vkgv_batch::kRing - n + f) % vkgv_batch::kRing;
+        float ms = 0.0f;
+        (void)hipEventElapsedTime(&ms, b->ev0[slot], b->ev1[slot]);
+        s->ms_total += ms / (float)n;
+    }
+    s->timed_frames = n;
+    b->ring_frames = 0;
     return VKGV_OK;
 }
 

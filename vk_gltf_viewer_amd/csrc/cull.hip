@@ -101,20 +101,20 @@ __device__ __forceinline__ bool cone_culled(const vkgv_meshlet_cone& cone, const
 
 struct CullArgs {
     SceneDev s;
-    CameraArg cam;
-    const float* pyr;
-    PyramidDesc pd;
-    unsigned long long* history;    // one 64-bit word per 64 draws
-    uint32_t* out;
-    uint32_t* out_count;
+    BatchDev b;
     uint32_t flags;
     uint32_t mode;
+    uint32_t list;
 };
 
+// Grid: (draw blocks, views).
 __global__ __launch_bounds__(kBlock) void cull_kernel(CullArgs a)
 {
     const uint32_t d = blockIdx.x * kBlock + threadIdx.x;
     const uint32_t lane = threadIdx.x & 63u;
+    const ViewDev& view = batch_view(a.b, blockIdx.y);
+    const CameraArg& cam = batch_camera(a.b, blockIdx.y);
+    unsigned long long* history = reinterpret_cast<unsigned long long*>(view.history);
     bool vis = false, newbit = false;
     if (d < a.s.draw_count) {
         const float4* rp = reinterpret_cast<const float4*>(a.s.cull + d);
@@ -130,19 +130,20 @@ __global__ __launch_bounds__(kBlock) void cull_kernel(CullArgs a)
         for (int r = 0; r < 3; ++r)
             we[r] = (fabsf(m.m[r]) * e[0] + fabsf(m.m[4 + r]) * e[1]) + fabsf(m.m[8 + r]) * e[2];
         bool pass = true;
-        if (a.flags & VKGV_CULL_FRUSTUM) pass = aabb_in_frustum(wc, we, a.cam.cam);
+        if (a.flags & VKGV_CULL_FRUSTUM) pass = aabb_in_frustum(wc, we, cam.cam);
         if (pass && (a.flags & VKGV_CULL_CONE) && cone != 0xFFFFFFFFu)
-            pass = !cone_culled(a.s.cones[cone], m, a.cam.eye);
+            pass = !cone_culled(a.s.cones[cone], m, cam.eye);
         if (a.mode == CULL_SINGLE) {
             vis = pass;
             if (vis && (a.flags & VKGV_CULL_OCCLUSION_PREV))
-                vis = occlusion_visible(wc, we, a.cam.cam.prev_occlusion_view_projection, a.pyr, a.pd);
+                vis = occlusion_visible(wc, we, cam.cam.prev_occlusion_view_projection, view.pyramid, view.pyr);
         } else {
-            const bool old = (a.history[d >> 6] >> (d & 63u)) & 1ull;
+            const bool old = (history[d >> 6] >> (d & 63u)) & 1ull;
             if (a.mode == CULL_EARLY) {
                 vis = pass && old;
             } else {
-                const bool occ = pass && occlusion_visible(wc, we, a.cam.cam.occlusion_view_projection, a.pyr, a.pd);
+                const bool occ = pass && occlusion_visible(wc, we, cam.cam.occlusion_view_projection, view.pyramid,
+                                                           view.pyr);
                 newbit = occ;
                 vis = occ && !(pass && old);
             }
@@ -150,22 +151,47 @@ __global__ __launch_bounds__(kBlock) void cull_kernel(CullArgs a)
     }
     if (a.mode == CULL_LATE) {
         const unsigned long long bits = __ballot(newbit);
-        if (lane == 0 && (d - lane) < a.s.draw_count) a.history[d >> 6] = bits;
+        if (lane == 0 && (d - lane) < a.s.draw_count) history[d >> 6] = bits;
     }
     const unsigned long long mask = __ballot(vis);
     const uint32_t cnt = (uint32_t)__popcll(mask);
     if (cnt == 0) return;
     uint32_t base = 0;
-    if (lane == 0) base = atomicAdd(a.out_count, cnt);
+    if (lane == 0) base = atomicAdd(&view.counters->vis_count[a.list], cnt);
     base = __shfl(base, 0);
     if (vis) {
         const uint32_t rank = __builtin_amdgcn_mbcnt_hi((uint32_t)(mask >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)mask, 0u));
-        a.out[base + rank] = d;
+        const uint4* src = reinterpret_cast<const uint4*>(a.s.rrec + d);
+        uint4* dst = reinterpret_cast<uint4*>(view.visible[a.list] + base + rank);
+        const uint4 r0 = src[0], r1 = src[1];
+        dst[0] = r0;
+        dst[1] = r1;
     }
 }
 
-// Cull records from the contract arrays: draw -> primitive -> meshlet AABB.
-__global__ __launch_bounds__(kBlock) void build_cull_records_kernel(SceneDev s, CullRec* out)
+// Frame start: each block clears kClearKeysPerBlock keys of its view (16 per
+// thread, 16-byte stores); the view's first block also zeroes its counters.
+__global__ __launch_bounds__(kBlock) void frame_start_kernel(BatchDev b)
+{
+    const uint32_t e = b.map_clear[blockIdx.x];
+    const uint32_t vi = e & 0xFFu, local = e >> 8;
+    const ViewDev& v = batch_view(b, vi);
+    if (local == 0 && threadIdx.x < sizeof(Counters) / 4) reinterpret_cast<uint32_t*>(v.counters)[threadIdx.x] = 0u;
+    const uint64_t n = (uint64_t)v.width * v.height;
+    const uint64_t first = (uint64_t)local * kClearKeysPerBlock;
+    const ulonglong2 c2 = make_ulonglong2(VKGV_KEY_CLEAR, VKGV_KEY_CLEAR);
+    ulonglong2* k2 = reinterpret_cast<ulonglong2*>(v.keys + first);
+    if (first + kClearKeysPerBlock <= n) {
+#pragma unroll
+        for (uint32_t q = 0; q < kClearKeysPerBlock / 2u / kBlock; ++q) k2[q * kBlock + threadIdx.x] = c2;
+    } else {
+        for (uint64_t i = first + threadIdx.x; i < n; i += kBlock) v.keys[i] = VKGV_KEY_CLEAR;
+    }
+}
+
+// Cull and raster records from the contract arrays: draw -> primitive ->
+// meshlet AABB, buffer offsets, counts and material.
+__global__ __launch_bounds__(kBlock) void build_cull_records_kernel(SceneDev s, CullRec* out, RasterRec* rrec)
 {
     const uint32_t d = blockIdx.x * kBlock + threadIdx.x;
     if (d >= s.draw_count) return;
@@ -179,6 +205,17 @@ __global__ __launch_bounds__(kBlock) void build_cull_records_kernel(SceneDev s, 
     const uint32_t cb = s.prim_cone_base[dr.primitive_index];
     r.cone = cb == 0xFFFFFFFFu ? 0xFFFFFFFFu : cb + dr.meshlet_index;
     out[d] = r;
+    RasterRec q;
+    q.draw = d;
+    q.transform = dr.transform_index;
+    q.vidx = (uint32_t)((reinterpret_cast<const uint32_t*>(p.vertex_index_buffer) - s.vidx_base) + ml->vertex_offset);
+    q.tri = (uint32_t)((reinterpret_cast<const uint8_t*>(p.primitive_index_buffer) - s.tri_base) + ml->triangle_offset);
+    // byte offset: meshes are packed at 256-byte boundaries, not at multiples of a vertex
+    q.vtx = (uint32_t)(reinterpret_cast<const uint8_t*>(p.vertex_buffer) - reinterpret_cast<const uint8_t*>(s.vtx_base));
+    q.material = p.material_index | (s.materials[p.material_index].double_sided ? 0x80000000u : 0u);
+    q.counts = (uint32_t)ml->vertex_count | ((uint32_t)ml->triangle_count << 8);
+    q.pad = 0u;
+    rrec[d] = q;
 }
 
 __global__ __launch_bounds__(kBlock) void clear_keys_kernel(ulonglong2* keys, uint64_t n2, unsigned long long* tail, uint32_t odd)
@@ -186,22 +223,6 @@ __global__ __launch_bounds__(kBlock) void clear_keys_kernel(ulonglong2* keys, ui
     const ulonglong2 v = make_ulonglong2(VKGV_KEY_CLEAR, VKGV_KEY_CLEAR);
     for (uint64_t i = (uint64_t)blockIdx.x * kBlock + threadIdx.x; i < n2; i += (uint64_t)gridDim.x * kBlock) keys[i] = v;
     if (odd && blockIdx.x == 0 && threadIdx.x == 0) *tail = VKGV_KEY_CLEAR;
-}
-
-__global__ __launch_bounds__(kBlock) void reset_counters_kernel(Counters* c, uint32_t what, uint32_t* bin_count,
-                                                              uint32_t ntiles)
-{
-    const uint32_t t = blockIdx.x * kBlock + threadIdx.x;
-    for (uint32_t i = t; i < ntiles; i += gridDim.x * kBlock) bin_count[i] = 0u;
-    if (blockIdx.x != 0) return;
-    uint32_t* w = reinterpret_cast<uint32_t*>(c);
-    if (what == 0xFFFFFFFFu) {
-        if (t < sizeof(Counters) / 4) w[t] = 0;
-    } else if (t == 0) {
-        // per-raster-pass queues only
-        c->big_packed = 0ull;
-        c->clip_count = 0u;
-    }
 }
 
 __global__ __launch_bounds__(kBlock) void write_depth_keys_kernel(unsigned long long* keys, const float* depth, uint64_t n)
@@ -212,10 +233,10 @@ __global__ __launch_bounds__(kBlock) void write_depth_keys_kernel(unsigned long 
 
 } // namespace
 
-void launch_build_cull_records(const SceneDev& s, CullRec* out, hipStream_t st)
+void launch_build_cull_records(const SceneDev& s, CullRec* out, RasterRec* rrec, hipStream_t st)
 {
     if (!s.draw_count) return;
-    build_cull_records_kernel<<<(s.draw_count + kBlock - 1) / kBlock, kBlock, 0, st>>>(s, out);
+    build_cull_records_kernel<<<(s.draw_count + kBlock - 1) / kBlock, kBlock, 0, st>>>(s, out, rrec);
 }
 
 void launch_clear_keys(unsigned long long* keys, uint64_t n, hipStream_t st)
@@ -228,29 +249,22 @@ void launch_clear_keys(unsigned long long* keys, uint64_t n, hipStream_t st)
                                                            (uint32_t)(n & 1));
 }
 
-void launch_reset_counters(const ViewDev& v, uint32_t what, hipStream_t st)
+void launch_frame_start(const BatchDev& b, const BatchGrid& g, hipStream_t st)
 {
-    const uint32_t ntiles = v.tiles_x * v.tiles_y;
-    uint32_t blocks = (ntiles + kBlock - 1) / kBlock;
-    blocks = blocks > 64u ? 64u : (blocks ? blocks : 1u);
-    reset_counters_kernel<<<blocks, kBlock, 0, st>>>(v.counters, what, v.bin_count, ntiles);
+    frame_start_kernel<<<g.clear_blocks, kBlock, 0, st>>>(b);
 }
 
-void launch_cull(const SceneDev& s, const ViewDev& v, const CameraArg& cam, uint32_t flags, CullMode mode,
+void launch_cull(const SceneDev& s, const BatchDev& b, const BatchGrid& g, uint32_t flags, CullMode mode,
                  uint32_t list, hipStream_t st)
 {
     if (!s.draw_count) return;
     CullArgs a;
     a.s = s;
-    a.cam = cam;
-    a.pyr = v.pyramid;
-    a.pd = v.pyr;
-    a.history = reinterpret_cast<unsigned long long*>(v.history);
-    a.out = v.visible[list];
-    a.out_count = &v.counters->vis_count[list];
+    a.b = b;
     a.flags = flags;
     a.mode = mode;
-    cull_kernel<<<(s.draw_count + kBlock - 1) / kBlock, kBlock, 0, st>>>(a);
+    a.list = list;
+    cull_kernel<<<dim3((s.draw_count + kBlock - 1) / kBlock, g.nviews), kBlock, 0, st>>>(a);
 }
 
 void launch_write_depth_keys(unsigned long long* keys, const float* depth, uint64_t n, hipStream_t st)

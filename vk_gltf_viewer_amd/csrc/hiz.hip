@@ -11,8 +11,9 @@
 //     reduced in LDS for as long as every source level is exactly twice its
 //     destination (then the footprint of texel p is {2p, 2p+1} and stays
 //     inside the tile), writing each level once;
-//  2. hiz_tail: one 1024-thread block for the remaining (small) levels with
-//     the general footprint.
+//  2. hiz_tail: one 1024-thread block per view for the remaining (small)
+//     levels with the general footprint.
+// Both cover every view of a batch in one launch.
 #include "internal.hpp"
 
 namespace vkgv {
@@ -24,30 +25,31 @@ __device__ __forceinline__ void footprint(uint32_t p, uint32_t dst, uint32_t src
     axis_footprint(u, (int)src, t0, t1);
 }
 
-struct HizArgs {
-    const unsigned long long* keys;
-    uint32_t W, H;
-    float* pyr;
-    PyramidDesc pd;
-    uint32_t tile_levels;   // levels produced by hiz_tiles (>= 1)
-};
-
-__global__ __launch_bounds__(256) void hiz_tiles_kernel(HizArgs a)
+// One block per 32x32 tile of a view's level 0 (block map entry = view |
+// tile << 8).
+__global__ __launch_bounds__(256) void hiz_tiles_kernel(BatchDev b)
 {
     __shared__ float s_a[32 * 32];
     __shared__ float s_b[16 * 16];
-    const uint32_t w0 = a.pd.w[0], h0 = a.pd.h[0];
-    const uint32_t bx = blockIdx.x * 32u, by = blockIdx.y * 32u;
-    float* lvl0 = a.pyr + a.pd.off[0];
+    const uint32_t e = b.map_hiz[blockIdx.x];
+    const ViewDev& v = batch_view(b, e & 0xFFu);
+    const uint32_t tile = e >> 8;
+    const unsigned long long* keys = v.keys;
+    const uint32_t W = v.width, H = v.height;
+    const PyramidDesc& pd = v.pyr;
+    const uint32_t tile_levels = v.hiz_tile_levels;
+    const uint32_t w0 = pd.w[0], h0 = pd.h[0];
+    const uint32_t bx = (tile % v.hiz_tiles_x) * 32u, by = (tile / v.hiz_tiles_x) * 32u;
+    float* lvl0 = v.pyramid + pd.off[0];
     for (uint32_t t = threadIdx.x; t < 1024u; t += 256u) {
         const uint32_t x = bx + (t & 31u), y = by + (t >> 5);
         float d = 0.0f;
         if (x < w0 && y < h0) {
             int x0, x1, y0, y1;
-            footprint(x, w0, a.W, x0, x1);
-            footprint(y, h0, a.H, y0, y1);
-            const unsigned long long* r0 = a.keys + (size_t)y0 * a.W;
-            const unsigned long long* r1 = a.keys + (size_t)y1 * a.W;
+            footprint(x, w0, W, x0, x1);
+            footprint(y, h0, H, y0, y1);
+            const unsigned long long* r0 = keys + (size_t)y0 * W;
+            const unsigned long long* r1 = keys + (size_t)y1 * W;
             d = key_depth(r0[x0]);
             d = gmin(d, key_depth(r0[x1]));
             d = gmin(d, key_depth(r1[x0]));
@@ -59,12 +61,12 @@ __global__ __launch_bounds__(256) void hiz_tiles_kernel(HizArgs a)
     float* src = s_a;
     float* dst = s_b;
     uint32_t ts = 32u;   // tile size of the source level in LDS
-    for (uint32_t k = 1; k < a.tile_levels; ++k) {
+    for (uint32_t k = 1; k < tile_levels; ++k) {
         __syncthreads();
         const uint32_t nts = ts >> 1;
-        const uint32_t wk = a.pd.w[k], hk = a.pd.h[k];
+        const uint32_t wk = pd.w[k], hk = pd.h[k];
         const uint32_t ox = bx >> k, oy = by >> k;
-        float* out = a.pyr + a.pd.off[k];
+        float* out = v.pyramid + pd.off[k];
         for (uint32_t t = threadIdx.x; t < nts * nts; t += 256u) {
             const uint32_t i = t % nts, j = t / nts;
             const uint32_t x = ox + i, y = oy + j;
@@ -82,9 +84,13 @@ __global__ __launch_bounds__(256) void hiz_tiles_kernel(HizArgs a)
     }
 }
 
-__global__ __launch_bounds__(1024) void hiz_tail_kernel(float* pyr, PyramidDesc pd, uint32_t first)
+// The levels after the tile pass's, one block per view.
+__global__ __launch_bounds__(1024) void hiz_tail_kernel(BatchDev b)
 {
-    for (uint32_t k = first; k < pd.levels; ++k) {
+    const ViewDev& v = batch_view(b, blockIdx.x);
+    const PyramidDesc& pd = v.pyr;
+    float* pyr = v.pyramid;
+    for (uint32_t k = v.hiz_tile_levels; k < pd.levels; ++k) {
         const uint32_t sw = pd.w[k - 1], sh = pd.h[k - 1], dw = pd.w[k], dh = pd.h[k];
         const float* src = pyr + pd.off[k - 1];
         float* dst = pyr + pd.off[k];
@@ -114,19 +120,10 @@ uint32_t hiz_tile_levels(const PyramidDesc& pd)
     return n;
 }
 
-void launch_hiz(const ViewDev& v, hipStream_t st)
+void launch_hiz(const BatchDev& b, const BatchGrid& g, hipStream_t st)
 {
-    if (v.pyr.levels == 0) return;
-    HizArgs a;
-    a.keys = v.keys;
-    a.W = v.width;
-    a.H = v.height;
-    a.pyr = v.pyramid;
-    a.pd = v.pyr;
-    a.tile_levels = hiz_tile_levels(v.pyr);
-    const dim3 grid((v.pyr.w[0] + 31) / 32, (v.pyr.h[0] + 31) / 32);
-    hiz_tiles_kernel<<<grid, 256, 0, st>>>(a);
-    if (a.tile_levels < v.pyr.levels) hiz_tail_kernel<<<1, 1024, 0, st>>>(v.pyramid, v.pyr, a.tile_levels);
+    if (g.hiz_blocks) hiz_tiles_kernel<<<g.hiz_blocks, 256, 0, st>>>(b);
+    if (g.max_tail_levels) hiz_tail_kernel<<<g.nviews, 1024, 0, st>>>(b);
 }
 
 } // namespace vkgv

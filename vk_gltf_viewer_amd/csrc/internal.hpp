@@ -21,6 +21,22 @@ struct CullRec {
 };
 static_assert(sizeof(CullRec) == 32, "CullRec layout");
 
+// What the raster needs of one meshlet draw, built per draw when the draw
+// list changes and copied into the visible lists by the cull, so that a wave
+// reaches a meshlet's indices in one dependent load instead of four
+// (draw -> primitive -> meshlet -> buffers).  32 bytes.
+struct RasterRec {
+    uint32_t draw;                  // draw index
+    uint32_t transform;             // transform index
+    uint32_t vidx;                  // first vertex index, elements into the packed vertex-index buffer
+    uint32_t tri;                   // first triangle byte, into the packed triangle buffer
+    uint32_t vtx;                   // first vertex of the mesh, byte offset into the packed vertex buffer
+    uint32_t material;              // material index | double_sided << 31
+    uint32_t counts;                // vertex_count | triangle_count << 8
+    uint32_t pad;
+};
+static_assert(sizeof(RasterRec) == 32, "RasterRec layout");
+
 // Device view of the uploaded scene.
 struct SceneDev {
     const vkgv_meshlet_draw* draws;
@@ -34,6 +50,11 @@ struct SceneDev {
     const uint32_t* prim_cone_base;      // per primitive: first index of its mesh's cones (~0u = none)
     const vkgv_meshlet_cone* cones;      // cones of all meshes, concatenated
     const CullRec* cull;                 // per draw
+    const RasterRec* rrec;               // per draw
+    // packed mesh buffers of the scene (the primitives' device addresses point into them)
+    const uint32_t* vidx_base;
+    const uint8_t* tri_base;
+    const vkgv_vertex* vtx_base;
     // sRGB encode + unorm8 tables (SrgbTables)
     const void* srgb;
 };
@@ -126,7 +147,6 @@ struct Counters {
 static_assert(sizeof(Counters) == 64, "Counters layout");
 
 constexpr uint32_t kTileSplit = 8;      // most work items sharing one screen tile's triangle list
-constexpr uint32_t kTileItemTris = 16;  // triangles per work item before a tile is split
 
 constexpr uint32_t kBinTile = 32;   // screen tile of the binned path, pixels
 
@@ -136,7 +156,7 @@ struct ViewDev {
     uint32_t* color;                // W*H RGBA8
     float* pyramid;
     PyramidDesc pyr;
-    uint32_t* visible[2];           // compacted draw lists
+    RasterRec* visible[2];          // compacted visible draws (first pass / late pass)
     uint32_t* history;              // two-pass visibility bits, ceil(draws/64)*2 words
     Counters* counters;
     BigTri* big;                    // big triangles of the current raster pass
@@ -151,7 +171,6 @@ struct ViewDev {
     uint32_t* bins;                 // big-triangle indices, grouped by tile
     uint32_t bin_capacity;          // also the capacity of `pairs` (64 per chunk)
     TileItem* tile_items;           // tiles * kTileSplit
-    uint32_t tile_grid;             // blocks of the tile pass (a multiple of the CU count)
     BigSetup* setups;               // per big triangle
     BinPair* pairs;
     // resolve cache: triangle t of the meshlet in visible-list slot s is
@@ -160,6 +179,9 @@ struct ViewDev {
     float4* slot_albedo;            // albedo factor of each slot's material
     uint32_t* draw_slot;
     uint32_t cache_slots;
+    uint32_t hiz_tile_levels;       // pyramid levels produced by the tile pass of the Hi-Z build
+    uint32_t hiz_tiles_x;           // 32x32 level-0 tiles per row
+    uint32_t resolve_blocks_x;      // 64-pixel resolve blocks per row
 };
 
 // Camera as kernels take it (a by-value copy of vkgv_camera).
@@ -168,28 +190,66 @@ struct CameraArg {
     float eye[3];
 };
 
+// The views of one frame as kernels see them.  A single-view frame passes its
+// view and camera by value (views == cams == nullptr); a batch keeps one
+// descriptor and one camera per view in device memory.  Kernels over pixels find their view through a block map:
+// entry = view | local block << 8 (at most 256 views per batch).
+struct BatchDev {
+    const ViewDev* views;           // [nviews], device memory; nullptr: the one view is view0
+    const CameraArg* cams;          // [nviews] or nullptr
+    CameraArg cam0;
+    ViewDev view0;                  // a single-view frame's view, by value (no table load)
+    uint32_t nviews;
+    const uint32_t* map_clear;      // frame-start (counter reset + key clear) blocks
+    const uint32_t* map_hiz;        // Hi-Z tile blocks
+    const uint32_t* map_resolve;    // resolve blocks
+};
+
+constexpr uint32_t kMaxBatchViews = 256;
+constexpr uint32_t kClearKeysPerBlock = 256u * 16u;   // keys cleared by one frame-start block
+
+__device__ __forceinline__ const CameraArg& batch_camera(const BatchDev& b, uint32_t v)
+{
+    return b.cams ? b.cams[v] : b.cam0;
+}
+
+__device__ __forceinline__ const ViewDev& batch_view(const BatchDev& b, uint32_t v)
+{
+    return b.views ? b.views[v] : b.view0;
+}
+
+// Host-side sizes of a batch's launches.
+struct BatchGrid {
+    uint32_t nviews;
+    uint32_t clear_blocks, hiz_blocks, resolve_blocks;
+    uint32_t max_tail_levels;       // any view needs the one-block Hi-Z tail
+    uint32_t max_draws;
+};
+
 enum CullMode : uint32_t { CULL_SINGLE = 0, CULL_EARLY = 1, CULL_LATE = 2 };
 
 // ---- launchers (stream-ordered, no host synchronization) -----------------
-void launch_build_cull_records(const SceneDev& s, CullRec* out, hipStream_t st);
+void launch_build_cull_records(const SceneDev& s, CullRec* out, RasterRec* rrec, hipStream_t st);
 void launch_clear_keys(unsigned long long* keys, uint64_t n, hipStream_t st);
-// what = ~0u: all counters (frame start); 1: the per-raster-pass queues.
-// Both also zero the per-tile bin counts.
-void launch_reset_counters(const ViewDev& v, uint32_t what, hipStream_t st);
-void launch_cull(const SceneDev& s, const ViewDev& v, const CameraArg& cam, uint32_t flags, CullMode mode,
+// Frame start of every view of the batch: all counters zeroed, keys cleared.
+void launch_frame_start(const BatchDev& b, const BatchGrid& g, hipStream_t st);
+void launch_cull(const SceneDev& s, const BatchDev& b, const BatchGrid& g, uint32_t flags, CullMode mode,
                  uint32_t list, hipStream_t st);
-void launch_raster(const SceneDev& s, const ViewDev& v, const Mat4& vp, uint32_t flags, uint32_t list,
-                   uint32_t grid_blocks, uint32_t small_max_extent, hipStream_t st);
-// Clipping, binning and tiled rasterization of the big-triangle queue.
-void launch_raster_big(const ViewDev& v, hipStream_t st);
-void launch_hiz(const ViewDev& v, hipStream_t st);
-void launch_resolve(const SceneDev& s, const ViewDev& v, const Mat4& vp, uint32_t flags, hipStream_t st);
+void launch_raster(const SceneDev& s, const BatchDev& b, uint32_t flags, uint32_t list, uint32_t grid_blocks,
+                   uint32_t small_max_extent, hipStream_t st);
+// Clipping, binning and tiled rasterization of the views' big-triangle queues.
+void launch_raster_big(const BatchDev& b, const BatchGrid& g, uint32_t tile_grid, uint32_t tile_item_tris,
+                       hipStream_t st);
+void launch_hiz(const BatchDev& b, const BatchGrid& g, hipStream_t st);
+void launch_resolve(const SceneDev& s, const BatchDev& b, const BatchGrid& g, uint32_t flags, hipStream_t st);
 void launch_write_depth_keys(unsigned long long* keys, const float* depth, uint64_t n, hipStream_t st);
+uint32_t hiz_tile_levels(const PyramidDesc& pd);
 
 // Tunables of the raster (set by the autotuner; defaults otherwise).
 struct RasterTuning {
     uint32_t small_max_extent;      // triangles whose bbox extent is below this (pixels) stay in the meshlet wave
     uint32_t raster_blocks_per_cu;  // persistent raster grid = CUs * this
+    uint32_t tile_item_tris;        // triangles per tile-pass work item before a tile is split
 };
 
 } // namespace vkgv

@@ -28,7 +28,7 @@
 //                 tests 64 tiles, writes a (tile, triangle) pair per tile that
 //                 can hold a covered pixel centre and counts it in its tile.
 //  bin_scan       one block: scans the per-tile counts and cuts each tile's
-//                 list into up to kTileSplit work items.
+//                 list into up to kTileSplit work items of a tunable size.
 //  bin_fill       scatters the pairs into per-tile triangle lists.
 //  tile_kernel    grid-stride loop over the work items, each rasterizing its
 //                 share of a tile's triangles into a 32x32 key tile in LDS (LDS
@@ -64,8 +64,7 @@ struct SmallTri {
 struct RasterArgs {
     SceneDev s;
     Mat4 vp;
-    const uint32_t* list;
-    const uint32_t* list_count;
+    const RasterRec* list;
     unsigned long long* keys;
     uint32_t W, H;
     uint32_t flags;
@@ -78,7 +77,8 @@ struct RasterArgs {
     ClipTri* clipq;
     uint32_t clip_cap;
     Counters* ctr;
-    const uint32_t* slot_base;      // first resolve-cache slot of this list (null: 0)
+    uint32_t count;                 // meshlets in the list
+    uint32_t slot0;                 // first resolve-cache slot of this list
     TriCache* tri_cache;
     float4* slot_albedo;
     uint32_t* draw_slot;
@@ -214,7 +214,9 @@ __device__ __forceinline__ uint32_t tri_byte(const uint32_t* words, uint32_t b)
     return (words[b >> 2] >> ((b & 3u) * 8u)) & 0xFFu;
 }
 
-__global__ __launch_bounds__(kBlock) void raster_kernel(RasterArgs a)
+// One view's share of the meshlet raster (wave w of the grid takes meshlets
+// w, w + waves, ... of the view's list).
+__device__ __forceinline__ void raster_view(const RasterArgs& a)
 {
     __shared__ float4 s_clip[kWaves][64];
     __shared__ SVtx s_sv[kWaves][64];
@@ -225,38 +227,50 @@ __global__ __launch_bounds__(kBlock) void raster_kernel(RasterArgs a)
     const int w = threadIdx.x >> 6;
     const uint32_t lane = threadIdx.x & 63u;
     __shared__ uint32_t s_col[kWaves][64];
-    const uint32_t count = __builtin_amdgcn_readfirstlane(*a.list_count);
-    const uint32_t slot0 = a.slot_base ? __builtin_amdgcn_readfirstlane(*a.slot_base) : 0u;
+    const uint32_t count = a.count;
+    const uint32_t slot0 = a.slot0;
     const uint32_t nwaves = gridDim.x * kWaves;
     const float hw = (float)a.W * 0.5f, hh = (float)a.H * 0.5f;
     const int W = (int)a.W, H = (int)a.H;
 
+    // The raster record of the wave's next meshlet is loaded while the
+    // current one is processed.
+    const uint4* list4 = reinterpret_cast<const uint4*>(a.list);
     uint32_t idx = blockIdx.x * kWaves + (uint32_t)w;
+    uint4 nx0 = make_uint4(0u, 0u, 0u, 0u), nx1 = nx0;
+    if (idx < count) { nx0 = list4[2u * idx]; nx1 = list4[2u * idx + 1u]; }
     while (idx < count) {
-        const uint32_t d = __builtin_amdgcn_readfirstlane(a.list[idx]);
-        const vkgv_meshlet_draw dr = a.s.draws[d];
-        const vkgv_primitive& prim = a.s.prims[dr.primitive_index];
-        const vkgv_meshlet ml = reinterpret_cast<const vkgv_meshlet*>(prim.meshlet_buffer)[dr.meshlet_index];
-        const vkgv_material& mat = a.s.materials[prim.material_index];
-        const Mat4 m = load_mat4(a.s.transforms + 16ull * dr.transform_index);
+        const uint32_t d = __builtin_amdgcn_readfirstlane(nx0.x);
+        const uint32_t tr = __builtin_amdgcn_readfirstlane(nx0.y);
+        const uint32_t vo = __builtin_amdgcn_readfirstlane(nx0.z);
+        const uint32_t to = __builtin_amdgcn_readfirstlane(nx0.w);
+        const uint32_t vb = __builtin_amdgcn_readfirstlane(nx1.x);
+        const uint32_t matw = __builtin_amdgcn_readfirstlane(nx1.y);
+        const uint32_t counts = __builtin_amdgcn_readfirstlane(nx1.z);
+        const uint32_t nidx = idx + nwaves;
+        if (nidx < count) { nx0 = list4[2u * nidx]; nx1 = list4[2u * nidx + 1u]; }
+        const Mat4 m = load_mat4(a.s.transforms + 16ull * tr);
         const Mat4 mvp = mat4_mul(a.vp, m);
         const float tdet = det4(m);
-        const bool cull_faces = !mat.double_sided && !(a.flags & VKGV_NO_BACKFACE);
-        const uint32_t vc = ml.vertex_count, tc = ml.triangle_count;
+        const bool cull_faces = !(matw >> 31) && !(a.flags & VKGV_NO_BACKFACE);
+        const uint32_t vc = counts & 0xFFu, tc = (counts >> 8) & 0xFFu;
         // resolve-cache slot of this meshlet (none when the cache is full)
         const uint32_t slot = slot0 + idx;
         const bool cached = slot < a.cache_slots;
         if (lane == 0) {
             a.draw_slot[d] = cached ? slot : 0xFFFFFFFFu;
-            if (cached)
+            if (cached) {
+                const vkgv_material& mat = a.s.materials[matw & 0x7FFFFFFFu];
                 a.slot_albedo[slot] = make_float4(mat.albedo_factor[0], mat.albedo_factor[1], mat.albedo_factor[2],
                                                   mat.albedo_factor[3]);
+            }
         }
 
         // vertices: clip = (VP * M) * (pos, 1) (visbuffer.mesh.glsl:44-65)
         if (lane < vc) {
-            const uint32_t vi = reinterpret_cast<const uint32_t*>(prim.vertex_index_buffer)[ml.vertex_offset + lane];
-            const float* pp = reinterpret_cast<const float*>(prim.vertex_buffer) + (size_t)vi * 6;
+            const uint32_t vi = a.s.vidx_base[vo + lane];
+            const float* pp = reinterpret_cast<const float*>(reinterpret_cast<const uint8_t*>(a.s.vtx_base) + vb)
+                              + (size_t)vi * 6;
             const float4 c = mat4_point(mvp, pp[0], pp[1], pp[2]);
             s_clip[w][lane] = c;
             s_col[w][lane] = reinterpret_cast<const uint32_t*>(pp)[3];
@@ -266,7 +280,7 @@ __global__ __launch_bounds__(kBlock) void raster_kernel(RasterArgs a)
             s_sv[w][lane] = sv;
         }
         {
-            const uint32_t* src = reinterpret_cast<const uint32_t*>(prim.primitive_index_buffer + ml.triangle_offset);
+            const uint32_t* src = reinterpret_cast<const uint32_t*>(a.s.tri_base + to);
             const uint32_t ndw = (tc * 3u + 3u) >> 2;
             for (uint32_t k = lane; k < ndw; k += 64u) s_tri[w][k] = src[k];
         }
@@ -413,7 +427,7 @@ __global__ __launch_bounds__(kBlock) void raster_kernel(RasterArgs a)
             }
         }
         wave_sync();
-        idx += nwaves;   // static stride: no shared cursor
+        idx = nidx;      // static stride: no shared cursor
     }
 }
 
@@ -440,6 +454,11 @@ struct BigArgs {
 };
 
 // Clipping (rare): Sutherland-Hodgman against pa*x + pb*y + pc*z + pd*w >= 0.
+// A convex polygon gains at most one vertex per plane (3 + 5 <= kClipMax);
+// non-finite or degenerate input can produce more crossings: such a polygon
+// is dropped (returns 0) and nothing is written past kClipMax.
+constexpr int kClipMax = 16;
+
 __device__ __forceinline__ int clip_plane(const float4* in, int n, float4* out, float pa, float pb, float pc, float pd)
 {
     int m = 0;
@@ -448,7 +467,10 @@ __device__ __forceinline__ int clip_plane(const float4* in, int n, float4* out, 
         const float4 q = in[i + 1 < n ? i + 1 : 0];
         const float dp = ((pa * p.x + pb * p.y) + pc * p.z) + pd * p.w;
         const float dq = ((pa * q.x + pb * q.y) + pc * q.z) + pd * q.w;
-        if (dp >= 0.0f) out[m++] = p;
+        if (dp >= 0.0f) {
+            if (m < kClipMax) out[m] = p;
+            ++m;
+        }
         if ((dp >= 0.0f) != (dq >= 0.0f)) {
             const float t = dp / (dp - dq);
             float4 r;
@@ -456,22 +478,23 @@ __device__ __forceinline__ int clip_plane(const float4* in, int n, float4* out, 
             r.y = p.y + t * (q.y - p.y);
             r.z = p.z + t * (q.z - p.z);
             r.w = p.w + t * (q.w - p.w);
-            out[m++] = r;
+            if (m < kClipMax) out[m] = r;
+            ++m;
         }
     }
-    return m;
+    return m <= kClipMax ? m : 0;
 }
 
 // Near plane (z <= w) then the four guard planes; the fan (v0, vi, vi+1) of
 // the clipped polygon (at most 3 + 5 vertices) joins the big triangles.
-__global__ __launch_bounds__(256) void clip_kernel(BigArgs a)
+__device__ __forceinline__ void clip_view(const BigArgs& a)
 {
     const uint32_t n = min(a.ctr->clip_count, a.clip_cap);
     const float gx = kClipGuardPx / (float)a.W, gy = kClipGuardPx / (float)a.H;
     const float hw = (float)a.W * 0.5f, hh = (float)a.H * 0.5f;
     for (uint32_t i = blockIdx.x * 256u + threadIdx.x; i < n; i += gridDim.x * 256u) {
         const ClipTri ct = a.clipq[i];
-        float4 A[10], B[10];
+        float4 A[kClipMax], B[kClipMax];
         A[0] = ct.c[0]; A[1] = ct.c[1]; A[2] = ct.c[2];
         int m = 3;
         m = clip_plane(A, m, B, 0.0f, 0.0f, -1.0f, 1.0f);
@@ -481,7 +504,7 @@ __global__ __launch_bounds__(256) void clip_kernel(BigArgs a)
         m = clip_plane(A, m, B, 0.0f, 1.0f, 0.0f, gy);
         if (m < 3) continue;
         atomicAdd(&a.ctr->clipped, 1u);
-        SVtx sv[10];
+        SVtx sv[kClipMax];
         bool ok = true;
         for (int k = 0; k < m; ++k) ok = ok && project_vtx(B[k], hw, hh, sv[k]);
         if (!ok) continue;
@@ -538,9 +561,9 @@ __device__ __forceinline__ void pass_extent(const BigArgs& a, uint32_t& nbig, ui
 }
 
 // Work items of a tile whose list holds n triangles (0 for an empty tile).
-__device__ __forceinline__ uint32_t tile_split(uint32_t n)
+__device__ __forceinline__ uint32_t tile_split(uint32_t n, uint32_t item_tris)
 {
-    return min(kTileSplit, (n + kTileItemTris - 1u) / kTileItemTris);
+    return min(kTileSplit, (n + item_tris - 1u) / item_tris);
 }
 
 // One wave per 64-tile chunk of a big triangle's bounding-box tiles.  The
@@ -549,7 +572,7 @@ __device__ __forceinline__ uint32_t tile_split(uint32_t n)
 // corner pixel centres of the tile's part of the bounding box (no centre there
 // can be covered); kept tiles write pair 64*chunk + l and count themselves.
 // bin_scan_kernel then scans the per-tile counts.
-__global__ __launch_bounds__(256) void bin_kernel(BigArgs a)
+__device__ __forceinline__ void bin_view(const BigArgs& a)
 {
     const uint32_t w = threadIdx.x >> 6, lane = threadIdx.x & 63u;
     const uint32_t gw = blockIdx.x * 4u + w, nw = gridDim.x * 4u;
@@ -625,7 +648,7 @@ constexpr uint32_t kScanThreads = 1024;
 constexpr uint32_t kScanChunk = 4096;
 constexpr uint32_t kScanPer = kScanChunk / kScanThreads;
 
-__global__ __launch_bounds__(kScanThreads) void bin_scan_kernel(BigArgs a)
+__device__ __forceinline__ void bin_scan_view(const BigArgs& a, uint32_t item_tris)
 {
     __shared__ uint32_t s_cnt[kScanChunk], s_start[kScanChunk], s_istart[kScanChunk];
     __shared__ uint32_t s_wsum[kScanThreads / 64], s_wisum[kScanThreads / 64], s_wmax[kScanThreads / 64];
@@ -648,7 +671,7 @@ __global__ __launch_bounds__(kScanThreads) void bin_scan_kernel(BigArgs a)
         for (uint32_t u = 0; u < kScanPer; ++u) {
             v[u] = b + u < m ? s_cnt[b + u] : 0u;
             sum += v[u];
-            isum += tile_split(v[u]);
+            isum += tile_split(v[u], item_tris);
             most = max(most, v[u]);
         }
         uint32_t x = sum, ix = isum;
@@ -670,11 +693,11 @@ __global__ __launch_bounds__(kScanThreads) void bin_scan_kernel(BigArgs a)
         for (uint32_t u = 0; u < kScanPer; ++u) {
             if (b + u < m) { s_start[b + u] = run; s_istart[b + u] = irun; }
             run += v[u];
-            irun += tile_split(v[u]);
+            irun += tile_split(v[u], item_tris);
         }
         __syncthreads();
         for (uint32_t i = tid; i < m; i += kScanThreads) {
-            const uint32_t n = s_cnt[i], st = s_start[i], sp = tile_split(n);
+            const uint32_t n = s_cnt[i], st = s_start[i], sp = tile_split(n, item_tris);
             a.bin_start[c0 + i] = st;
             a.bin_count[c0 + i] = 0u;   // the fill pass counts again; the tile pass leaves it zero
             for (uint32_t p = 0; p < sp; ++p) {
@@ -705,7 +728,7 @@ __global__ __launch_bounds__(kScanThreads) void bin_scan_kernel(BigArgs a)
 }
 
 // Scatter of the pairs into per-tile triangle lists (counts rebuilt).
-__global__ __launch_bounds__(256) void bin_fill_kernel(BigArgs a)
+__device__ __forceinline__ void bin_fill_view(const BigArgs& a)
 {
     uint32_t nbig, nchunks;
     pass_extent(a, nbig, nchunks);
@@ -728,21 +751,16 @@ __global__ __launch_bounds__(256) void bin_fill_kernel(BigArgs a)
 // pass has finished and tiles are disjoint; the old keys are read when the
 // item starts), atomicMin otherwise.  Setups are loaded 64 at a time, one per
 // lane, and broadcast with readlane.  Items of one tile are adjacent, so
-// they run on different XCDs.  Wave 0 also clears the pass's queue counters
-// (their last reader, the fill pass, has finished), and part 0 of each tile
-// zeroes the tile's count for the next pass.
+// they run on different XCDs.  Part 0 of each tile zeroes the tile's count
+// for the next pass.
 constexpr int kTileRows = (int)(kBinTile * kBinTile / 128u);   // pixels per lane in a half tile
 constexpr int kSetupWords = (int)(sizeof(BigSetup) / 4);
 static_assert(kSetupWords % 4 == 0, "setups are loaded 16 bytes at a time");
 
-__global__ __launch_bounds__(256) void tile_kernel(BigArgs a)
+__device__ __forceinline__ void tile_view(const BigArgs& a)
 {
     const uint32_t lane = threadIdx.x & 63u;
     const uint32_t wave = blockIdx.x * 4u + (threadIdx.x >> 6), nwaves = gridDim.x * 4u;
-    if (wave == 0 && lane == 0) {
-        a.ctr->big_packed = 0ull;
-        a.ctr->clip_count = 0u;
-    }
     const uint32_t nitems = __builtin_amdgcn_readfirstlane(a.ctr->tile_items);
     const int lx = (int)(lane & 31u), ly = (int)(lane >> 5);
     const BigSetup* __restrict__ setups = a.setups;
@@ -828,38 +846,10 @@ __global__ __launch_bounds__(256) void tile_kernel(BigArgs a)
     }
 }
 
-} // namespace
+// ---------------------------------------------------------------------------
+// Kernels over a batch of views: every view's work in one launch.
 
-void launch_raster(const SceneDev& s, const ViewDev& v, const Mat4& vp, uint32_t flags, uint32_t list,
-                   uint32_t grid_blocks, uint32_t small_max_extent, hipStream_t st)
-{
-    RasterArgs a;
-    a.s = s;
-    a.vp = vp;
-    a.list = v.visible[list];
-    a.list_count = &v.counters->vis_count[list];
-    a.keys = v.keys;
-    a.W = v.width;
-    a.H = v.height;
-    a.flags = flags;
-    a.small_lim = (int)(small_max_extent * 256u);
-    a.big = v.big;
-    a.big_chunk = v.big_chunk;
-    a.big_cap = v.big_capacity;
-    a.chunk_tri = v.chunk_tri;
-    a.chunk_cap = v.bin_capacity / 64u;
-    a.clipq = v.clipq;
-    a.clip_cap = v.clip_capacity;
-    a.ctr = v.counters;
-    a.slot_base = list ? &v.counters->vis_count[0] : nullptr;
-    a.tri_cache = v.tri_cache;
-    a.slot_albedo = v.slot_albedo;
-    a.draw_slot = v.draw_slot;
-    a.cache_slots = v.cache_slots;
-    raster_kernel<<<grid_blocks, kBlock, 0, st>>>(a);
-}
-
-void launch_raster_big(const ViewDev& v, hipStream_t st)
+__device__ __forceinline__ BigArgs big_args(const ViewDev& v)
 {
     BigArgs a;
     a.big = v.big;
@@ -882,11 +872,151 @@ void launch_raster_big(const ViewDev& v, hipStream_t st)
     a.setups = v.setups;
     a.pairs = v.pairs;
     a.items = v.tile_items;
-    clip_kernel<<<16, 256, 0, st>>>(a);
-    bin_kernel<<<1024, 256, 0, st>>>(a);
-    bin_scan_kernel<<<1, kScanThreads, 0, st>>>(a);
-    bin_fill_kernel<<<256, 256, 0, st>>>(a);
-    tile_kernel<<<v.tile_grid, 256, 0, st>>>(a);
+    return a;
+}
+
+struct RasterBatchArgs {
+    SceneDev s;
+    BatchDev b;
+    uint32_t flags;
+    uint32_t list;
+    int small_lim;
+};
+
+// The views of a batch are visited 64 at a time: lane l of each wave reads
+// the counter of view v0 + l, so that a view without work for the wave costs
+// one readlane instead of a dependent load.
+__global__ __launch_bounds__(kBlock) void raster_kernel(RasterBatchArgs ab)
+{
+    const uint32_t lane = threadIdx.x & 63u;
+    const uint32_t gw = blockIdx.x * kWaves + (threadIdx.x >> 6);
+    for (uint32_t v0 = 0; v0 < ab.b.nviews; v0 += 64u) {
+        uint32_t cnt = 0, base = 0;
+        if (v0 + lane < ab.b.nviews) {
+            const Counters* c = batch_view(ab.b, v0 + lane).counters;
+            cnt = c->vis_count[ab.list];
+            base = ab.list ? c->vis_count[0] : 0u;
+        }
+        const uint32_t m = min(64u, ab.b.nviews - v0);
+        for (uint32_t j = 0; j < m; ++j) {
+            const uint32_t count = __builtin_amdgcn_readlane(cnt, j);
+            if (gw >= count) continue;
+            const uint32_t vi = v0 + j;
+            const ViewDev& v = batch_view(ab.b, vi);
+            RasterArgs a;
+            a.s = ab.s;
+            a.vp = load_mat4(batch_camera(ab.b, vi).cam.view_projection);
+            a.list = v.visible[ab.list];
+            a.keys = v.keys;
+            a.W = v.width;
+            a.H = v.height;
+            a.flags = ab.flags;
+            a.small_lim = ab.small_lim;
+            a.big = v.big;
+            a.big_chunk = v.big_chunk;
+            a.big_cap = v.big_capacity;
+            a.chunk_tri = v.chunk_tri;
+            a.chunk_cap = v.bin_capacity / 64u;
+            a.clipq = v.clipq;
+            a.clip_cap = v.clip_capacity;
+            a.ctr = v.counters;
+            a.count = count;
+            a.slot0 = __builtin_amdgcn_readlane(base, j);
+            a.tri_cache = v.tri_cache;
+            a.slot_albedo = v.slot_albedo;
+            a.draw_slot = v.draw_slot;
+            a.cache_slots = v.cache_slots;
+            raster_view(a);
+        }
+    }
+}
+
+// Lane l of the wave reads field(view v0 + l) (0 past the batch).
+template <typename F>
+__device__ __forceinline__ uint32_t lane_view_value(const BatchDev& b, uint32_t v0, F field)
+{
+    const uint32_t v = v0 + (threadIdx.x & 63u);
+    return v < b.nviews ? field(*batch_view(b, v).counters) : 0u;
+}
+
+__device__ __forceinline__ uint32_t big_count(const Counters& c) { return (uint32_t)(c.big_packed >> 32); }
+
+__global__ __launch_bounds__(256) void clip_kernel(BatchDev b)
+{
+    for (uint32_t v0 = 0; v0 < b.nviews; v0 += 64u) {
+        const uint32_t n = lane_view_value(b, v0, [](const Counters& c) { return c.clip_count; });
+        const uint32_t m = min(64u, b.nviews - v0);
+        for (uint32_t j = 0; j < m; ++j)
+            if (__builtin_amdgcn_readlane(n, j)) clip_view(big_args(batch_view(b, v0 + j)));
+    }
+}
+
+__global__ __launch_bounds__(256) void bin_kernel(BatchDev b)
+{
+    for (uint32_t v0 = 0; v0 < b.nviews; v0 += 64u) {
+        const uint32_t n = lane_view_value(b, v0, big_count);
+        const uint32_t m = min(64u, b.nviews - v0);
+        for (uint32_t j = 0; j < m; ++j)
+            if (__builtin_amdgcn_readlane(n, j)) bin_view(big_args(batch_view(b, v0 + j)));
+    }
+}
+
+// One block per view.
+__global__ __launch_bounds__(kScanThreads) void bin_scan_kernel(BatchDev b, uint32_t item_tris)
+{
+    bin_scan_view(big_args(batch_view(b, blockIdx.x)), item_tris);
+}
+
+__global__ __launch_bounds__(256) void bin_fill_kernel(BatchDev b)
+{
+    for (uint32_t v0 = 0; v0 < b.nviews; v0 += 64u) {
+        const uint32_t n = lane_view_value(b, v0, big_count);
+        const uint32_t m = min(64u, b.nviews - v0);
+        for (uint32_t j = 0; j < m; ++j)
+            if (__builtin_amdgcn_readlane(n, j)) bin_fill_view(big_args(batch_view(b, v0 + j)));
+    }
+}
+
+// Wave 0 also clears every view's queue counters for the next pass (their
+// last reader, the fill pass, has finished).
+__global__ __launch_bounds__(256) void tile_kernel(BatchDev b)
+{
+    const uint32_t wave = blockIdx.x * 4u + (threadIdx.x >> 6);
+    for (uint32_t v0 = 0; v0 < b.nviews; v0 += 64u) {
+        const uint32_t n = lane_view_value(b, v0, [](const Counters& c) { return c.tile_items; });
+        if (wave == 0 && v0 + (threadIdx.x & 63u) < b.nviews) {
+            Counters* c = batch_view(b, v0 + (threadIdx.x & 63u)).counters;
+            c->big_packed = 0ull;
+            c->clip_count = 0u;
+        }
+        const uint32_t m = min(64u, b.nviews - v0);
+        for (uint32_t j = 0; j < m; ++j)
+            if (__builtin_amdgcn_readlane(n, j)) tile_view(big_args(batch_view(b, v0 + j)));
+    }
+}
+
+} // namespace
+
+void launch_raster(const SceneDev& s, const BatchDev& b, uint32_t flags, uint32_t list, uint32_t grid_blocks,
+                   uint32_t small_max_extent, hipStream_t st)
+{
+    RasterBatchArgs a;
+    a.s = s;
+    a.b = b;
+    a.flags = flags;
+    a.list = list;
+    a.small_lim = (int)(small_max_extent * 256u);
+    raster_kernel<<<grid_blocks, kBlock, 0, st>>>(a);
+}
+
+void launch_raster_big(const BatchDev& b, const BatchGrid& g, uint32_t tile_grid, uint32_t tile_item_tris,
+                       hipStream_t st)
+{
+    clip_kernel<<<16, 256, 0, st>>>(b);
+    bin_kernel<<<1024, 256, 0, st>>>(b);
+    bin_scan_kernel<<<g.nviews, kScanThreads, 0, st>>>(b, tile_item_tris);
+    bin_fill_kernel<<<256, 256, 0, st>>>(b);
+    tile_kernel<<<tile_grid, 256, 0, st>>>(b);
 }
 
 } // namespace vkgv

@@ -86,8 +86,31 @@ __device__ __forceinline__ void fetch_uncached(const ResolveArgs& a, uint32_t d,
 // of its column, with the loads of the four pixels issued together.
 constexpr int kRows = 4;
 
-__global__ __launch_bounds__(256) void resolve_kernel(ResolveArgs a)
+struct ResolveBatchArgs {
+    SceneDev s;
+    BatchDev b;
+    uint32_t flags;
+};
+
+// One block per 64x16 pixels of a view (block map entry = view | block << 8).
+__global__ __launch_bounds__(256) void resolve_kernel(ResolveBatchArgs ab)
 {
+    const uint32_t me = ab.b.map_resolve[blockIdx.x];
+    const ViewDev& view = batch_view(ab.b, me & 0xFFu);
+    const uint32_t local = me >> 8;
+    ResolveArgs a;
+    a.s = ab.s;
+    a.vp = load_mat4(batch_camera(ab.b, me & 0xFFu).cam.view_projection);
+    a.keys = view.keys;
+    a.color = view.color;
+    a.W = view.width;
+    a.H = view.height;
+    a.flags = ab.flags;
+    a.draw_slot = view.draw_slot;
+    a.tri_cache = view.tri_cache;
+    a.slot_albedo = view.slot_albedo;
+    const uint32_t blk_x = local % view.resolve_blocks_x, blk_y = local / view.resolve_blocks_x;
+
     // LDS tables: sRGB thresholds and buckets, exact quotients b / 255 of a byte
     __shared__ float s_thr[256];
     __shared__ float s_byte[256];
@@ -101,9 +124,9 @@ __global__ __launch_bounds__(256) void resolve_kernel(ResolveArgs a)
     __syncthreads();
     const uint8_t* base = reinterpret_cast<const uint8_t*>(s_base);
 
-    const uint32_t px = blockIdx.x * 64u + (threadIdx.x & 63u);
+    const uint32_t px = blk_x * 64u + (threadIdx.x & 63u);
     if (px >= a.W) return;
-    const uint32_t py0 = blockIdx.y * (4u * kRows) + (threadIdx.x >> 6);
+    const uint32_t py0 = blk_y * (4u * kRows) + (threadIdx.x >> 6);
     const bool bary = (a.flags & VKGV_RESOLVE_BARY) != 0;
     const unsigned long long* __restrict__ keys = a.keys;
     const uint32_t* __restrict__ draw_slot = a.draw_slot;
@@ -179,21 +202,13 @@ __global__ __launch_bounds__(256) void resolve_kernel(ResolveArgs a)
 
 } // namespace
 
-void launch_resolve(const SceneDev& s, const ViewDev& v, const Mat4& vp, uint32_t flags, hipStream_t st)
+void launch_resolve(const SceneDev& s, const BatchDev& b, const BatchGrid& g, uint32_t flags, hipStream_t st)
 {
-    ResolveArgs a;
+    ResolveBatchArgs a;
     a.s = s;
-    a.vp = vp;
-    a.keys = v.keys;
-    a.color = v.color;
-    a.W = v.width;
-    a.H = v.height;
+    a.b = b;
     a.flags = flags;
-    a.draw_slot = v.draw_slot;
-    a.tri_cache = v.tri_cache;
-    a.slot_albedo = v.slot_albedo;
-    const dim3 grid((v.width + 63) / 64, (v.height + 4 * kRows - 1) / (4 * kRows));
-    resolve_kernel<<<grid, 256, 0, st>>>(a);
+    if (g.resolve_blocks) resolve_kernel<<<g.resolve_blocks, 256, 0, st>>>(a);
 }
 
 } // namespace vkgv

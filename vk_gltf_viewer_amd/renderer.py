@@ -40,6 +40,7 @@ class Renderer:
             raise VkgvError(rc, "vkgv_create failed (no HIP device?)")
         self._h = h
         self.views = []
+        self.batches = []
 
     # -- errors -----------------------------------------------------------------
     def _check(self, rc):
@@ -48,6 +49,8 @@ class Renderer:
         return rc
 
     def close(self):
+        for b in list(self.batches):
+            b.close()
         for v in list(self.views):
             v.close()
         if self._h:
@@ -87,13 +90,17 @@ class Renderer:
     def sync(self):
         self._check(self._lib.vkgv_sync(self._h))
 
-    def set_tuning(self, small_max_extent: int, raster_blocks_per_cu: int):
-        self._check(self._lib.vkgv_set_tuning(self._h, C.byref(N.Tuning(small_max_extent, raster_blocks_per_cu))))
+    def set_tuning(self, small_max_extent: int, raster_blocks_per_cu: int, tile_item_tris: int = 16):
+        t = N.Tuning(small_max_extent, raster_blocks_per_cu, tile_item_tris)
+        self._check(self._lib.vkgv_set_tuning(self._h, C.byref(t)))
 
     def get_tuning(self):
         t = N.Tuning()
         self._check(self._lib.vkgv_get_tuning(self._h, C.byref(t)))
-        return t.small_max_extent, t.raster_blocks_per_cu
+        return t.small_max_extent, t.raster_blocks_per_cu, t.tile_item_tris
+
+    def create_batch(self, views) -> "Batch":
+        return Batch(self, views)
 
     def create_view(self, width: int, height: int) -> "View":
         h = C.c_void_p()
@@ -101,6 +108,41 @@ class Renderer:
         v = View(self, h, width, height)
         self.views.append(v)
         return v
+
+
+class Batch:
+    """Views rendered together by one sequence of launches (vkgv_render_batch)."""
+
+    def __init__(self, r: Renderer, views):
+        self.r, self.views = r, list(views)
+        arr = (C.c_void_p * len(self.views))(*[v._h.value for v in self.views])
+        h = C.c_void_p()
+        r._check(r._lib.vkgv_batch_create(r._h, arr, len(self.views), C.byref(h)))
+        self._h = h
+        self._frames = (N.FrameDesc * len(self.views))()
+        r.batches.append(self)
+
+    def close(self):
+        if self._h:
+            self.r._lib.vkgv_batch_destroy(self._h)
+            self._h = None
+            self.r.batches.remove(self)
+
+    def render(self, cameras, flags: int = N.MODE_REFERENCE, eyes=None, stream=None):
+        if len(cameras) != len(self.views):
+            raise ValueError("one camera per view")
+        for i, cam in enumerate(cameras):
+            f = self._frames[i]
+            f.camera = cam
+            f.eye[:] = list(eyes[i]) if eyes is not None else [0.0, 0.0, 0.0]
+            f.flags = flags
+            f.stream = stream
+        self.r._check(self.r._lib.vkgv_render_batch(self.r._h, self._h, self._frames, len(self.views)))
+
+    def stats(self) -> N.FrameStats:
+        s = N.FrameStats()
+        self.r._check(self.r._lib.vkgv_batch_read_stats(self.r._h, self._h, C.byref(s)))
+        return s
 
 
 class View:
