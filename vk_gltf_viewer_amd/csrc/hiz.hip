@@ -84,16 +84,27 @@ __global__ __launch_bounds__(256) void hiz_tiles_kernel(BatchDev b)
     }
 }
 
-// The levels after the tile pass's, one block per view.
+// The levels after the tile pass's, one block per view.  A level small
+// enough for LDS is also kept there, so that the following levels read LDS
+// instead of global memory (one global round trip for the whole tail when
+// its first level fits).
+constexpr uint32_t kTailA = 8192, kTailB = 4096;   // floats of the two LDS level buffers
+
 __global__ __launch_bounds__(1024) void hiz_tail_kernel(BatchDev b)
 {
+    __shared__ float s_a[kTailA];
+    __shared__ float s_b[kTailB];
     const ViewDev& v = batch_view(b, blockIdx.x);
     const PyramidDesc& pd = v.pyr;
     float* pyr = v.pyramid;
+    const float* lsrc = nullptr;          // LDS copy of level k-1, if any
+    bool next_a = true;                   // LDS buffer for the next level kept in LDS
     for (uint32_t k = v.hiz_tile_levels; k < pd.levels; ++k) {
         const uint32_t sw = pd.w[k - 1], sh = pd.h[k - 1], dw = pd.w[k], dh = pd.h[k];
-        const float* src = pyr + pd.off[k - 1];
+        const float* src = lsrc ? lsrc : pyr + pd.off[k - 1];
         float* dst = pyr + pd.off[k];
+        float* ldst = nullptr;
+        if (dw * dh <= (next_a ? kTailA : kTailB)) ldst = next_a ? s_a : s_b;
         for (uint32_t t = threadIdx.x; t < dw * dh; t += 1024u) {
             const uint32_t x = t % dw, y = t / dw;
             int x0, x1, y0, y1;
@@ -104,8 +115,11 @@ __global__ __launch_bounds__(1024) void hiz_tail_kernel(BatchDev b)
             d = gmin(d, src[y1 * sw + x0]);
             d = gmin(d, src[y1 * sw + x1]);
             dst[t] = d;
+            if (ldst) ldst[t] = d;
         }
         __syncthreads();
+        lsrc = ldst;
+        if (ldst) next_a = !next_a;
     }
 }
 

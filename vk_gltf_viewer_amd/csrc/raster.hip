@@ -458,8 +458,16 @@ struct BigArgs {
 // non-finite or degenerate input can produce more crossings: such a polygon
 // is dropped (returns 0) and nothing is written past kClipMax.
 constexpr int kClipMax = 16;
+constexpr uint32_t kClipThreads = 64;   // one wave per clip block; polygons live in LDS
 
-__device__ __forceinline__ int clip_plane(const float4* in, int n, float4* out, float pa, float pb, float pc, float pd)
+// A lane's polygon in LDS: vertex k at base[k * kClipThreads] (bank-conflict free).
+struct StridedPoly {
+    float4* base;
+    __device__ __forceinline__ float4& operator[](int k) const { return base[k * (int)kClipThreads]; }
+};
+
+__device__ __forceinline__ int clip_plane(const StridedPoly in, int n, const StridedPoly out, float pa, float pb, float pc,
+                                          float pd)
 {
     int m = 0;
     for (int i = 0; i < n; ++i) {
@@ -486,31 +494,42 @@ __device__ __forceinline__ int clip_plane(const float4* in, int n, float4* out, 
 }
 
 // Near plane (z <= w) then the four guard planes; the fan (v0, vi, vi+1) of
-// the clipped polygon (at most 3 + 5 vertices) joins the big triangles.
+// the clipped polygon (at most 3 + 5 vertices) joins the big triangles.  Each
+// lane clips one queued triangle in its own LDS polygon buffers.
 __device__ __forceinline__ void clip_view(const BigArgs& a)
 {
+    __shared__ float4 s_poly[2][kClipMax][kClipThreads];
+    float4* A = &s_poly[0][0][threadIdx.x];
+    float4* B = &s_poly[1][0][threadIdx.x];
     const uint32_t n = min(a.ctr->clip_count, a.clip_cap);
     const float gx = kClipGuardPx / (float)a.W, gy = kClipGuardPx / (float)a.H;
     const float hw = (float)a.W * 0.5f, hh = (float)a.H * 0.5f;
-    for (uint32_t i = blockIdx.x * 256u + threadIdx.x; i < n; i += gridDim.x * 256u) {
+    for (uint32_t i = blockIdx.x * kClipThreads + threadIdx.x; i < n; i += gridDim.x * kClipThreads) {
         const ClipTri ct = a.clipq[i];
-        float4 A[kClipMax], B[kClipMax];
-        A[0] = ct.c[0]; A[1] = ct.c[1]; A[2] = ct.c[2];
+        StridedPoly pa{ A }, pb{ B };
+        pa[0] = ct.c[0]; pa[1] = ct.c[1]; pa[2] = ct.c[2];
         int m = 3;
-        m = clip_plane(A, m, B, 0.0f, 0.0f, -1.0f, 1.0f);
-        m = clip_plane(B, m, A, -1.0f, 0.0f, 0.0f, gx);
-        m = clip_plane(A, m, B, 1.0f, 0.0f, 0.0f, gx);
-        m = clip_plane(B, m, A, 0.0f, -1.0f, 0.0f, gy);
-        m = clip_plane(A, m, B, 0.0f, 1.0f, 0.0f, gy);
+        m = clip_plane(pa, m, pb, 0.0f, 0.0f, -1.0f, 1.0f);
+        m = clip_plane(pb, m, pa, -1.0f, 0.0f, 0.0f, gx);
+        m = clip_plane(pa, m, pb, 1.0f, 0.0f, 0.0f, gx);
+        m = clip_plane(pb, m, pa, 0.0f, -1.0f, 0.0f, gy);
+        m = clip_plane(pa, m, pb, 0.0f, 1.0f, 0.0f, gy);
         if (m < 3) continue;
         atomicAdd(&a.ctr->clipped, 1u);
-        SVtx sv[kClipMax];
         bool ok = true;
-        for (int k = 0; k < m; ++k) ok = ok && project_vtx(B[k], hw, hh, sv[k]);
+        for (int k = 0; k < m && ok; ++k) {
+            SVtx sv;
+            ok = project_vtx(pb[k], hw, hh, sv);
+        }
         if (!ok) continue;
+        SVtx s0;
+        project_vtx(pb[0], hw, hh, s0);
         for (int k = 1; k + 1 < m; ++k) {
+            SVtx s1, s2;
+            project_vtx(pb[k], hw, hh, s1);
+            project_vtx(pb[k + 1], hw, hh, s2);
             TriSetup t;
-            if (setup_tri(sv[0], sv[k], sv[k + 1], (int)a.W, (int)a.H, t))
+            if (setup_tri(s0, s1, s2, (int)a.W, (int)a.H, t))
                 push_big(t, ct.id, a.big, a.big_chunk, a.big_cap, a.chunk_tri, a.chunk_cap, a.ctr);
         }
     }
@@ -941,7 +960,7 @@ __device__ __forceinline__ uint32_t lane_view_value(const BatchDev& b, uint32_t 
 
 __device__ __forceinline__ uint32_t big_count(const Counters& c) { return (uint32_t)(c.big_packed >> 32); }
 
-__global__ __launch_bounds__(256) void clip_kernel(BatchDev b)
+__global__ __launch_bounds__(kClipThreads) void clip_kernel(BatchDev b)
 {
     for (uint32_t v0 = 0; v0 < b.nviews; v0 += 64u) {
         const uint32_t n = lane_view_value(b, v0, [](const Counters& c) { return c.clip_count; });
@@ -1012,7 +1031,7 @@ void launch_raster(const SceneDev& s, const BatchDev& b, uint32_t flags, uint32_
 void launch_raster_big(const BatchDev& b, const BatchGrid& g, uint32_t tile_grid, uint32_t tile_item_tris,
                        hipStream_t st)
 {
-    clip_kernel<<<16, 256, 0, st>>>(b);
+    clip_kernel<<<64, kClipThreads, 0, st>>>(b);
     bin_kernel<<<1024, 256, 0, st>>>(b);
     bin_scan_kernel<<<g.nviews, kScanThreads, 0, st>>>(b, tile_item_tris);
     bin_fill_kernel<<<256, 256, 0, st>>>(b);
