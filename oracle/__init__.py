@@ -69,6 +69,13 @@ def _setup(L):
         "ref_keys_ids": (None, [p, C.c_uint64, p]),
         "ref_from_linear": (f32, [f32]),
         "ref_srgb_codes": (None, [p, C.c_uint64, p]),
+        "ref_diff_raster": (None, [p, p, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, p, p]),
+        "ref_diff_forward": (C.c_double, [p, p, C.c_int, p, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int,
+                                          p, p, p, C.c_int, p]),
+        "ref_diff_backward": (None, [p, p, C.c_int, p, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int,
+                                     p, p, p, C.c_int, p, p]),
+        "ref_diff_loss_at": (C.c_double, [p, p, C.c_int, p, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int,
+                                          p, p, p, C.c_int, p, C.c_int64, p, C.c_int]),
         "ref_resolve": (None, [p, p, p, u32, u32, u32, p]),
         "ref_frame_reference": (u32, [p, p, p, u32, u32, u32, p, p, p, p]),
         "ref_frame_two_pass": (None, [p, p, p, u32, u32, u32, p, p, p, p, p, p]),
@@ -198,3 +205,62 @@ def frame_two_pass(scene: Scene, camera, w, h, flags, history, eye=None):
                              pyr.ctypes.data, keys.ctypes.data, rgba.ctypes.data, ve.ctypes.data, vl.ctypes.data)
     return (keys.reshape(h, w), rgba.reshape(h, w), pyr,
             np.nonzero(ve)[0].astype(np.uint32), np.nonzero(vl)[0].astype(np.uint32))
+
+
+# ---------------------------------------------------------------------------
+# Differentiable form (diffref.c)
+
+def diff_raster(clip, tris, H, W):
+    """clip [C, I, V, 4] float32, tris [T, 3] int32 -> (ids [C, H, W] int32, depth float32)."""
+    c = np.ascontiguousarray(clip, dtype=np.float32)
+    t = np.ascontiguousarray(tris, dtype=np.int32)
+    C_, I_, V_ = c.shape[:3]
+    ids = np.zeros((C_, H, W), np.int32)
+    depth = np.zeros((C_, H, W), np.float32)
+    lib().ref_diff_raster(c.ctypes.data, t.ctypes.data, C_, I_, V_, t.shape[0], H, W, ids.ctypes.data, depth.ctypes.data)
+    return ids, depth
+
+
+def _diff_args(clip, attr, tris, ids, depth, target):
+    c = np.ascontiguousarray(clip, dtype=np.float64)
+    a = np.ascontiguousarray(attr, dtype=np.float64)
+    t = np.ascontiguousarray(tris, dtype=np.int32)
+    i = np.ascontiguousarray(ids, dtype=np.int32)
+    d = np.ascontiguousarray(depth, dtype=np.float32)
+    g = None if target is None else np.ascontiguousarray(target, dtype=np.float64)
+    C_, I_, V_ = c.shape[:3]
+    H, W = i.shape[1:]
+    return (c, a, t, i, d, g), (c.ctypes.data, a.ctypes.data, a.shape[1], t.ctypes.data, C_, I_, V_, t.shape[0], H, W,
+                                 i.ctypes.data, d.ctypes.data, None if g is None else g.ctypes.data)
+
+
+def diff_forward(clip, attr, tris, ids, depth, target=None, aa=True):
+    """(loss, image [C, H, W, A]) in float64 for fixed ids / depth."""
+    keep, args = _diff_args(clip, attr, tris, ids, depth, target)
+    C_ = keep[0].shape[0]
+    H, W = keep[3].shape[1:]
+    out = np.zeros((C_, H, W, keep[1].shape[1]), np.float64)
+    loss = lib().ref_diff_forward(*args, 1 if aa else 0, out.ctypes.data)
+    return loss, out
+
+
+def diff_backward(clip, attr, tris, ids, depth, target=None, aa=True):
+    """(d loss / d clip [C, I, V, 4], d loss / d attr [V, A]) in float64."""
+    keep, args = _diff_args(clip, attr, tris, ids, depth, target)
+    g_clip = np.zeros(keep[0].shape, np.float64)
+    g_attr = np.zeros(keep[1].shape, np.float64)
+    lib().ref_diff_backward(*args, 1 if aa else 0, g_clip.ctypes.data, g_attr.ctypes.data)
+    return g_clip, g_attr
+
+
+def diff_loss_at(clip, attr, tris, ids, depth, pix, target=None, aa=True, state=None, mode=0):
+    """Loss over the listed pixels (flat indices into [C, H, W]).  state:
+    int8 [len(pix), 4] antialiasing state, recorded (mode 1) or held fixed
+    (mode 2)."""
+    keep, args = _diff_args(clip, attr, tris, ids, depth, target)
+    px = np.ascontiguousarray(pix, dtype=np.int64)
+    st = None
+    if state is not None:
+        assert state.dtype == np.int8 and state.shape == (px.size, 4) and state.flags.c_contiguous
+        st = state.ctypes.data
+    return lib().ref_diff_loss_at(*args, 1 if aa else 0, px.ctypes.data, px.size, st, mode)

@@ -1,0 +1,153 @@
+"""The FP64 reference of the differentiable form, pinned on the CPU.
+
+- its raster equals the frame's triangle raster (same snapping, edge and
+  depth rules) triangle by triangle;
+- interpolating the clip coordinates themselves returns each pixel's ray
+  (perspective-correct barycentrics, pixel-centre convention);
+- antialiasing of axis-aligned edges gives the exact box-filter coverage;
+- the pixel-list loss equals the full forward's loss;
+- the analytic gradient equals Ridders-extrapolated central differences on
+  200 seeded parameters (positions, albedo, instance transforms).
+"""
+import numpy as np
+import pytest
+
+import oracle
+from diffcheck import (Problem, analytic, central_differences, choose_params, clip32, pick, reaching_fraction,
+                       relative_errors)
+
+
+@pytest.fixture(scope="module")
+def small():
+    from vk_gltf_viewer_amd.diff import c4_scene, c4_target
+    s = c4_scene(instances=3, cameras=3, size=96)
+    tp, ta = c4_target(s)
+    cl = clip32(s.pos, s.xforms, s.view_projs)
+    ids, dep = oracle.diff_raster(cl, s.tris, 96, 96)
+    tcl = clip32(tp, s.xforms, s.view_projs)
+    tids, tdep = oracle.diff_raster(tcl, s.tris, 96, 96)
+    _, timg = oracle.diff_forward(tcl.astype(np.float64), ta, s.tris, tids, tdep)
+    return s, ids, dep, timg
+
+
+def test_raster_matches_frame_triangle_raster():
+    rng = np.random.default_rng(11)
+    W, H = 61, 47
+    for trial in range(40):
+        n = int(rng.integers(1, 6))
+        clip = np.zeros((1, 1, 3 * n, 4), np.float32)
+        w = rng.uniform(0.5, 3.0, (3 * n, 1))
+        clip[0, 0, :, :2] = rng.uniform(-1.3, 1.3, (3 * n, 2)) * w
+        clip[0, 0, :, 2:3] = rng.uniform(0.05, 0.95, (3 * n, 1)) * w
+        clip[0, 0, :, 3:4] = w
+        tris = np.arange(3 * n, dtype=np.int32).reshape(n, 3)
+        ids, depth = oracle.diff_raster(clip, tris, H, W)
+        keys = None
+        for t in range(n):
+            c = clip[0, 0, 3 * t:3 * t + 3]
+            keys = oracle.raster_triangle(c[0], c[1], c[2], t + 1, W, H, keys)
+        kid = oracle.keys_ids(keys.ravel()).reshape(H, W)
+        covered = keys.ravel() != np.uint64(0xFFFFFFFFFFFFFFFF)
+        ref_ids = np.where(covered.reshape(H, W), kid, 0)
+        assert np.array_equal(ids[0], ref_ids.astype(np.int32)), trial
+        ref_depth = np.where(covered, oracle.keys_depth(keys.ravel()), 0.0).reshape(H, W)
+        assert np.array_equal(depth[0].view(np.uint32), ref_depth.astype(np.float32).view(np.uint32)), trial
+
+
+def test_interpolated_clip_coordinates_give_pixel_rays(small):
+    s, ids, dep, _ = small
+    clip = clip32(s.pos, s.xforms, s.view_projs).astype(np.float64)
+    H = W = 96
+    for c in range(clip.shape[0]):
+        for i in range(clip.shape[1]):
+            attr = clip[c, i][:, [0, 1, 3]]
+            own = np.where((ids[c] > 0) & ((ids[c] - 1) // len(s.tris) == i), ids[c], 0)[None]
+            _, img = oracle.diff_forward(clip[c:c + 1], attr, s.tris, own, dep[c:c + 1], aa=False)
+            m = own[0] > 0
+            assert m.sum() > 100
+            ys, xs = np.nonzero(m)
+            xn = (xs + 0.5) / (W / 2) - 1
+            yn = (ys + 0.5) / (H / 2) - 1
+            v = img[0][m]
+            np.testing.assert_allclose(v[:, 0] / v[:, 2], xn, atol=1e-10)
+            np.testing.assert_allclose(v[:, 1] / v[:, 2], yn, atol=1e-10)
+
+
+def _rect(x0, x1, y0, y1, W, H, w=1.0):
+    """Clip positions of a screen rectangle [x0, x1] x [y0, y1] (pixels)."""
+    def ndc(x, n):
+        return x / (n / 2) - 1
+    corners = [(x0, y0), (x1, y0), (x1, y1), (x0, y1)]
+    clip = np.array([[ndc(x, W) * w, ndc(y, H) * w, 0.5 * w, w] for x, y in corners], np.float64)
+    tris = np.array([[0, 1, 2], [0, 2, 3]], np.int32)
+    return clip[None, None], tris
+
+
+def test_antialias_axis_aligned_edges_give_box_coverage():
+    W, H = 40, 32
+    for (x0, x1, y0, y1) in [(5.3, 30.8, 4.6, 25.2), (7.9, 21.1, 9.45, 20.55), (3.5, 33.25, 2.75, 28.5)]:
+        clip, tris = _rect(x0, x1, y0, y1, W, H, w=1.7)
+        ids, dep = oracle.diff_raster(clip.astype(np.float32), tris, H, W)
+        _, img = oracle.diff_forward(clip, np.ones((4, 1)), tris, ids, dep)
+        img = img[0, :, :, 0]
+        cov_x = np.clip(np.minimum(np.arange(W) + 1, x1) - np.maximum(np.arange(W), x0), 0, 1)
+        cov_y = np.clip(np.minimum(np.arange(H) + 1, y1) - np.maximum(np.arange(H), y0), 0, 1)
+        # rows / columns at least 3 pixels inside: near the corners the quad's
+        # internal diagonal is closer than one pixel to the outer edge and is
+        # the first edge a pair's segment meets in the front triangle
+        inner_y = (np.arange(H) >= y0 + 3) & (np.arange(H) + 1 <= y1 - 3)
+        inner_x = (np.arange(W) >= x0 + 3) & (np.arange(W) + 1 <= x1 - 3)
+        # columns across the vertical edges, on rows inside vertically
+        np.testing.assert_allclose(img[inner_y][:, cov_x < 1], np.broadcast_to(cov_x[cov_x < 1], (inner_y.sum(), (cov_x < 1).sum())), atol=1e-12)
+        np.testing.assert_allclose(img[:, inner_x][cov_y < 1].T, np.broadcast_to(cov_y[cov_y < 1], (inner_x.sum(), (cov_y < 1).sum())), atol=1e-12)
+        # and the interior keeps its colour across the diagonal between the two triangles
+        np.testing.assert_allclose(img[np.ix_(inner_y, inner_x)], 1.0, atol=1e-12)
+        assert ((cov_x > 0) & (cov_x < 1)).sum() == 2 and ((cov_y > 0) & (cov_y < 1)).sum() == 2
+
+
+def test_antialias_blend_uses_the_nearer_surface():
+    """Two overlapping rectangles: the nearer one (larger z/w) is the front of
+    their shared pairs, so its edge decides the blend and the far colour
+    shows through in proportion to the coverage."""
+    W, H = 40, 30
+    near, t1 = _rect(10.0, 25.3, 5.0, 25.0, W, H)
+    far, _ = _rect(5.0, 35.0, 2.0, 28.0, W, H)
+    near[..., 2] = 0.9
+    far[..., 2] = 0.2
+    clip = np.concatenate([near, far], 2)
+    tris = np.concatenate([t1, t1 + 4])
+    attr = np.concatenate([np.ones((4, 1)), np.full((4, 1), 0.25)])
+    ids, dep = oracle.diff_raster(clip.astype(np.float32), tris, H, W)
+    _, img = oracle.diff_forward(clip, attr, tris, ids, dep)
+    row = img[0, 15, :, 0]
+    assert row[25] == pytest.approx(0.3 * 1.0 + 0.7 * 0.25, abs=1e-12)
+    assert row[24] == pytest.approx(1.0, abs=1e-12) and row[26] == pytest.approx(0.25, abs=1e-12)
+
+
+def test_pixel_list_loss_equals_forward(small):
+    s, ids, dep, timg = small
+    clip = clip32(s.pos, s.xforms, s.view_projs).astype(np.float64)
+    loss, _ = oracle.diff_forward(clip, s.albedo, s.tris, ids, dep, timg)
+    pix = np.random.default_rng(3).permutation(ids.size)
+    state = np.zeros((pix.size, 4), np.int8)
+    l1 = oracle.diff_loss_at(clip, s.albedo, s.tris, ids, dep, pix, timg, True, state, 1)
+    l2 = oracle.diff_loss_at(clip, s.albedo, s.tris, ids, dep, pix, timg, True, state, 2)
+    assert l1 == pytest.approx(loss, rel=1e-12)
+    assert l2 == l1
+    assert (state > 0).sum() > 1000
+
+
+@pytest.mark.parametrize("aa", [True, False])
+def test_backward_matches_central_differences(small, aa):
+    s, ids, dep, timg = small
+    pr = Problem(s.pos.astype(np.float64), s.albedo.astype(np.float64), s.xforms.astype(np.float64),
+                 s.view_projs.astype(np.float64), s.tris, ids, dep, timg, aa)
+    g = analytic(pr)
+    params = choose_params(pr, 200, 5)
+    assert len({p for p in params}) == 200
+    fd = central_differences(pr, params, 200)
+    err = relative_errors(fd, g)
+    worst = fd[int(np.argmax(err))]
+    assert err.max() < 1e-6, (worst, pick(g, worst[0], worst[1]))
+    # the parameters reach the loss: most of them have a clearly non-zero gradient
+    assert reaching_fraction(fd, g, 1e-5) > 0.8

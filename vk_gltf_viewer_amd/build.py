@@ -2,6 +2,8 @@
 
 libvkgv_hip.so   hipcc for gfx950, all HIP translation units in one shared library
 libvkgv_host.so  g++ (host producers: camera, World, meshlets, scenes)
+vkgv_diff.so     LibTorch extension of the differentiable form (csrc/diff/), built
+                 by torch.utils.cpp_extension with hipcc for gfx950
 
 Both are built in-tree under vk_gltf_viewer_amd/lib/.
 """
@@ -55,14 +57,51 @@ def build_hip(verbose=True):
     return out
 
 
+def build_diff(verbose=False):
+    """The differentiable form: its kernels (csrc/diff/diff.hip) with hipcc into
+    lib/libvkgv_diffk.so, and the LibTorch binding (csrc/diff/diff_ext.cpp, host
+    C++ only) with torch.utils.cpp_extension into lib/vkgv_diff.so, linked to
+    the kernel library (resolved by preloading it with RTLD_GLOBAL before the
+    binding is imported)."""
+    import shutil
+    os.makedirs(LIB, exist_ok=True)
+    hipcc = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
+    kout = os.path.join(LIB, "libvkgv_diffk.so")
+    _run([hipcc, *HIPCC_FLAGS, "-Wl,-soname,libvkgv_diffk.so", "-I", os.path.join(CSRC, "diff"), "-o", kout + ".tmp",
+          os.path.join(CSRC, "diff", "diff.hip")])
+    os.replace(kout + ".tmp", kout)
+    import ctypes
+    import torch
+    # the binding's NEEDED libvkgv_diffk.so resolves to this already-loaded library
+    ctypes.CDLL(kout, mode=ctypes.RTLD_GLOBAL)
+    from torch.utils import cpp_extension
+    torch_lib = os.path.join(os.path.dirname(torch.__file__), "lib")
+    bdir = os.path.join(LIB, "_build_diff")
+    os.makedirs(bdir, exist_ok=True)
+    mod = cpp_extension.load(
+        name="vkgv_diff", sources=[os.path.join(CSRC, "diff", "diff_ext.cpp")],
+        extra_cflags=["-O2", "-std=c++17", "-D__HIP_PLATFORM_AMD__=1", "-DUSE_ROCM=1", "-I/opt/rocm/include",
+                      "-I" + os.path.join(CSRC, "diff")],
+        extra_ldflags=["-L" + LIB, "-lvkgv_diffk", "-L/opt/rocm/lib", "-lamdhip64",
+                       "-L" + torch_lib, "-lc10_hip"],
+        with_cuda=False, build_directory=bdir, verbose=verbose, is_python_module=True)
+    out = os.path.join(LIB, "vkgv_diff.so")
+    shutil.copyfile(mod.__file__, out + ".tmp")
+    os.replace(out + ".tmp", out)
+    return out
+
+
 def build_all():
     build_host()
     build_hip()
+    build_diff()
 
 
 if __name__ == "__main__":
-    what = sys.argv[1:] or ["host", "hip"]
+    what = sys.argv[1:] or ["host", "hip", "diff"]
     if "host" in what:
         build_host()
     if "hip" in what:
         build_hip()
+    if "diff" in what:
+        build_diff()
