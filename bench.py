@@ -124,6 +124,62 @@ def batched_views(steps, warmup, log):
         r.close()
 
 
+def uneven_views(steps, warmup, log):
+    """The uneven stress configuration: 200 meshes with log-uniform meshlet
+    counts (1 to 20k; 39.7M unique triangles) whose source positions are
+    fp32, int16-normalized or uint8-normalized, 1-64 instances each (15.2M
+    meshlet draws), rendered to 16 views of mixed resolution up to 3840x2160
+    as one captured-graph batch frame, against the loop of 16 single-view
+    frames.  Two-pass Hi-Z, barycentric resolve; tunables from the
+    autotuner on the batch frame."""
+    from vk_gltf_viewer_amd.autotune import autotune
+    t0 = time.perf_counter()
+    scene = Scene(5, 5)
+    sc = scene.arrays()
+    gen_s = time.perf_counter() - t0
+    r = Renderer(0)
+    try:
+        r.load(sc)
+        views = [r.create_view(v.width, v.height) for v in sc.views]
+        cams = [camera_from_view(v) for v in sc.views]
+        batch = r.create_batch(views)
+        flags = N.CULL_FRUSTUM | N.CULL_TWO_PASS | N.RESOLVE_BARY
+        tuning, _ = autotune(r, lambda: batch.render(cams, flags | N.GRAPH), key="c5-16views", use_cache=False,
+                             log=log, warmup=1, frames=3)
+
+        def run(fn, n):
+            r.sync()
+            t = time.perf_counter()
+            for _ in range(n):
+                fn()
+            r.sync()
+            return 1000.0 * (time.perf_counter() - t) / n
+
+        batch_frame = lambda: batch.render(cams, flags | N.GRAPH)
+
+        def loop_frame():
+            for v, c in zip(views, cams):
+                v.render(c, flags)
+        run(batch_frame, warmup)
+        ms_batch = run(batch_frame, steps)
+        run(loop_frame, warmup)
+        ms_loop = run(loop_frame, steps)
+        st = batch.stats()
+        vis_draws = [int(len(v.visible(0)) + len(v.visible(1))) for v in views]
+        vis_tris = sum(visible_triangles(sc, np.concatenate([v.visible(0), v.visible(1)])) for v in views)
+        px = sum(v.width * v.height for v in sc.views)
+        out = {"views": len(views), "pixels": px, "meshes": len(sc.meshes), "draws": len(sc.draws),
+               "triangles": scene.triangle_count, "visible_draws": sum(vis_draws), "visible_triangles": vis_tris,
+               "ms_per_batch_frame": round(ms_batch, 4), "views_per_s": round(len(views) / ms_batch * 1000.0, 1),
+               "visible_triangles_per_s": round(vis_tris / (ms_batch * 1e-3), 1),
+               "loop_ms_per_frame": round(ms_loop, 4), "speedup_vs_loop": round(ms_loop / ms_batch, 3),
+               "tuning": list(tuning), "overflow": int(st.overflow), "scene_generation_s": round(gen_s, 2)}
+        batch.close()
+        return out
+    finally:
+        r.close()
+
+
 def differentiable(steps, warmup, precisions=("mixed", "float32")):
     """The differentiable form: the subdivision-6 icosphere (40,962 vertices,
     81,920 triangles) as 8 instances, 8 ring cameras at 512x512, L2 loss of
@@ -196,6 +252,8 @@ def main():
     ap.add_argument("--no-batch", action="store_true", help="skip the 64-view batched-vs-loop measurement")
     ap.add_argument("--batch-only", action="store_true", help="only the 64-view batched-vs-loop measurement")
     ap.add_argument("--no-diff", action="store_true", help="skip the differentiable-form measurement")
+    ap.add_argument("--no-uneven", action="store_true", help="skip the 16-view uneven stress measurement")
+    ap.add_argument("--uneven-only", action="store_true", help="only the 16-view uneven stress measurement")
     ap.add_argument("--diff-only", action="store_true", help="only the differentiable-form measurement")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run three plain 256 MiB device-to-device copies (the reference kernel for scaling "
@@ -218,6 +276,9 @@ def main():
     log = (lambda m: print(m, file=sys.stderr)) if args.verbose else None
     if args.batch_only:
         print(json.dumps({"batched": batched_views(args.steps, args.warmup, log)}))
+        return
+    if args.uneven_only:
+        print(json.dumps({"uneven": uneven_views(args.steps, args.warmup, log)}))
         return
     if args.diff_only:
         print(json.dumps({"differentiable": differentiable(args.steps, args.warmup)}))
@@ -310,6 +371,8 @@ def main():
     r.close()
     if not args.no_batch:
         out["batched"] = batched_views(min(args.steps, 20), args.warmup, log)
+    if not args.no_uneven:
+        out["uneven"] = uneven_views(min(args.steps, 10), min(args.warmup, 2), log)
     if not args.no_diff:
         out["differentiable"] = differentiable(min(args.steps, 20), args.warmup)
     print(json.dumps(out))
