@@ -92,7 +92,7 @@ def batched_views(steps, warmup, log):
         batch = r.create_batch(views)
         flags = N.CULL_FRUSTUM | N.CULL_TWO_PASS | N.RESOLVE_BARY
         tuning, _ = autotune(r, lambda: batch.render(cams, flags | N.GRAPH), key="c3-64views", use_cache=False,
-                             log=log)
+                             log=log, valid=lambda: batch.stats().overflow == 0)
 
         def run(fn, n):
             r.sync()
@@ -145,7 +145,7 @@ def uneven_views(steps, warmup, log):
         batch = r.create_batch(views)
         flags = N.CULL_FRUSTUM | N.CULL_TWO_PASS | N.RESOLVE_BARY
         tuning, _ = autotune(r, lambda: batch.render(cams, flags | N.GRAPH), key="c5-16views", use_cache=False,
-                             log=log, warmup=1, frames=3)
+                             log=log, warmup=1, frames=3, valid=lambda: batch.stats().overflow == 0)
 
         def run(fn, n):
             r.sync()
@@ -165,6 +165,32 @@ def uneven_views(steps, warmup, log):
         run(loop_frame, warmup)
         ms_loop = run(loop_frame, steps)
         st = batch.stats()
+        # backward to the per-instance transforms: every view's antialiased
+        # linear image and its backward (an L2 loss against a zero target,
+        # d loss / d image = 2 image), accumulated into one gradient
+        import torch
+        batch_frame()
+        r.sync()
+        T = sc.transforms.shape[0]
+        g = torch.zeros((T, 16), dtype=torch.float64, device="cuda")
+        imgs = [torch.empty((v.height, v.width, 3), dtype=torch.float32, device="cuda") for v in views]
+
+        def grad_step():
+            g.zero_()
+            for v, c, im in zip(views, cams, imgs):
+                v.aa_image(c, im)
+                v.aa_backward(c, im, g)
+        for _ in range(max(1, warmup)):
+            grad_step()
+        torch.cuda.synchronize()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        for _ in range(steps):
+            grad_step()
+        e1.record()
+        torch.cuda.synchronize()
+        ms_grad = e0.elapsed_time(e1) / steps
+        grad_live = int((g.abs().amax(1) > 0).sum().item())
         vis_draws = [int(len(v.visible(0)) + len(v.visible(1))) for v in views]
         vis_tris = sum(visible_triangles(sc, np.concatenate([v.visible(0), v.visible(1)])) for v in views)
         px = sum(v.width * v.height for v in sc.views)
@@ -173,7 +199,8 @@ def uneven_views(steps, warmup, log):
                "ms_per_batch_frame": round(ms_batch, 4), "views_per_s": round(len(views) / ms_batch * 1000.0, 1),
                "visible_triangles_per_s": round(vis_tris / (ms_batch * 1e-3), 1),
                "loop_ms_per_frame": round(ms_loop, 4), "speedup_vs_loop": round(ms_loop / ms_batch, 3),
-               "tuning": list(tuning), "overflow": int(st.overflow), "scene_generation_s": round(gen_s, 2)}
+               "tuning": list(tuning), "overflow": int(st.overflow), "scene_generation_s": round(gen_s, 2),
+               "transform_backward_ms": round(ms_grad, 4), "transforms": T, "transforms_with_gradient": grad_live}
         batch.close()
         return out
     finally:

@@ -288,6 +288,27 @@ int  vkgv_render_batch(vkgv_ctx* ctx, vkgv_batch* batch, const vkgv_frame_desc* 
  * frames since the previous call. */
 int  vkgv_batch_read_stats(vkgv_ctx* ctx, vkgv_batch* batch, vkgv_frame_stats* stats);
 
+/* ------------------------------------------------------------------------ */
+/* Differentiable frame: gradients of an image loss with respect to the
+ * instance transforms, through analytic edge antialiasing of the view's last
+ * frame.  No counterpart in the reference, which has no differentiable path;
+ * the image is its flat resolve (visbuffer_resolve.comp.glsl:17-41) in linear
+ * colour.  `camera` must be the one the view's last frame was rendered with,
+ * and the scene and transforms those of that frame.  Device pointers; the
+ * calls are stream-ordered (stream NULL: the context's stream).
+ *
+ * image: float [H][W][3], the albedo factor of each pixel's material
+ * (background 0) with each pair of horizontally or vertically adjacent
+ * pixels of different colour blended along the front triangle's crossed
+ * edge (see csrc/grad.hip). */
+int vkgv_view_aa_image(vkgv_ctx* ctx, vkgv_view* view, const vkgv_camera* camera, float* image, void* stream);
+/* g_transforms (double [transform_count][16], column-major like
+ * vkgv_set_transforms) += d(sum g_image . image) / d transforms, for
+ * g_image float [H][W][3].  transform_count must be at least the count last
+ * given to vkgv_set_transforms.  The rasterization has no gradient. */
+int vkgv_view_aa_backward(vkgv_ctx* ctx, vkgv_view* view, const vkgv_camera* camera, const float* g_image,
+                          double* g_transforms, uint32_t transform_count, void* stream);
+
 /* Visibility-buffer key layout of this implementation: one 64-bit word per
  * pixel, (~float_bits(depth) << 32) | (0xFFFFFFFE - id), min-reduced with
  * atomicMin.  Reversed-Z GREATER_OR_EQUAL (larger depth wins, ties go to

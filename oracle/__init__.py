@@ -76,6 +76,9 @@ def _setup(L):
                                      p, p, p, C.c_int, p, p]),
         "ref_diff_loss_at": (C.c_double, [p, p, C.c_int, p, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int,
                                           p, p, p, C.c_int, p, C.c_int64, p, C.c_int]),
+        "ref_frame_aa_image": (None, [p, p, p, C.c_int, C.c_int, p, p]),
+        "ref_frame_aa_backward": (None, [p, p, p, C.c_int, C.c_int, p, p, p]),
+        "ref_frame_aa_loss_at": (C.c_double, [p, p, p, C.c_int, C.c_int, p, p, p, C.c_int64, p, C.c_int]),
         "ref_resolve": (None, [p, p, p, u32, u32, u32, p]),
         "ref_frame_reference": (u32, [p, p, p, u32, u32, u32, p, p, p, p]),
         "ref_frame_two_pass": (None, [p, p, p, u32, u32, u32, p, p, p, p, p, p]),
@@ -264,3 +267,42 @@ def diff_loss_at(clip, attr, tris, ids, depth, pix, target=None, aa=True, state=
         assert state.dtype == np.int8 and state.shape == (px.size, 4) and state.flags.c_contiguous
         st = state.ctypes.data
     return lib().ref_diff_loss_at(*args, 1 if aa else 0, px.ctypes.data, px.size, st, mode)
+
+
+def _frame_args(keys, vp, xf):
+    k = np.ascontiguousarray(keys, dtype=np.uint64)
+    v = np.ascontiguousarray(vp, dtype=np.float32).ravel()
+    x = None if xf is None else np.ascontiguousarray(xf, dtype=np.float64)
+    return k, v, x
+
+
+def frame_aa_image(scene: Scene, vp, keys, W, H, xf=None):
+    """Antialiased linear image [H, W, 3] (float64) of a frame's keys; vp is
+    the camera's column-major view-projection, xf optional float64
+    transforms [T, 16] (default: the scene's)."""
+    k, v, x = _frame_args(keys, vp, xf)
+    out = np.zeros((H, W, 3), np.float64)
+    lib().ref_frame_aa_image(scene.ref, v.ctypes.data, k.ctypes.data, W, H, _ptr(x), out.ctypes.data)
+    return out
+
+
+def frame_aa_backward(scene: Scene, vp, keys, W, H, g_image, xf=None):
+    """d(sum g_image * image) / d transforms, float64 [T, 16] (column-major)."""
+    k, v, x = _frame_args(keys, vp, xf)
+    g = np.ascontiguousarray(g_image, dtype=np.float64)
+    out = np.zeros((scene.sc.transforms.reshape(-1, 16).shape[0], 16), np.float64)
+    lib().ref_frame_aa_backward(scene.ref, v.ctypes.data, k.ctypes.data, W, H, _ptr(x), g.ctypes.data, out.ctypes.data)
+    return out
+
+
+def frame_aa_loss_at(scene: Scene, vp, keys, W, H, xf, target, pix, state=None, mode=0):
+    """sum over the listed pixels of |image - target|^2; state int8 [len(pix), 4]."""
+    k, v, x = _frame_args(keys, vp, xf)
+    t = np.ascontiguousarray(target, dtype=np.float64)
+    px = np.ascontiguousarray(pix, dtype=np.int64)
+    st = None
+    if state is not None:
+        assert state.dtype == np.int8 and state.shape == (px.size, 4) and state.flags.c_contiguous
+        st = state.ctypes.data
+    return lib().ref_frame_aa_loss_at(scene.ref, v.ctypes.data, k.ctypes.data, W, H, _ptr(x), t.ctypes.data,
+                                      px.ctypes.data, px.size, st, mode)

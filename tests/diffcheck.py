@@ -147,20 +147,30 @@ def ridders(f, h0: float = 1e-3, shrink: float = 2.0, levels: int = 16, rtol: fl
     """(derivative at 0, error estimate) of the scalar function f.  The
     tableau keeps shrinking the step while its error estimate is above
     rtol * |estimate| (when h0 exceeds the distance to a singularity of f,
-    the first levels are not yet in the range where the expansion holds)."""
+    the first levels are not yet in the range where the expansion holds).
+    No error estimate is taken below the rounding noise of the level's
+    difference quotient, 4 eps (|f(h)| + |f(-h)|) / 2h: below it two
+    differences can agree by accident."""
+    eps = np.finfo(np.float64).eps
     a = np.zeros((levels, levels))
+    noise = np.zeros(levels)
     h = h0
-    a[0, 0] = (f(h) - f(-h)) / (2 * h)
+
+    def cd(h):
+        fp, fm = f(h), f(-h)
+        return (fp - fm) / (2 * h), 4 * eps * (abs(fp) + abs(fm)) / (2 * h)
+
+    a[0, 0], noise[0] = cd(h)
     best, err = a[0, 0], np.inf
     c2 = shrink * shrink
     for i in range(1, levels):
         h /= shrink
-        a[0, i] = (f(h) - f(-h)) / (2 * h)
+        a[0, i], noise[i] = cd(h)
         fac = c2
         for j in range(1, i + 1):
             a[j, i] = (a[j - 1, i] * fac - a[j - 1, i - 1]) / (fac - 1.0)
             fac *= c2
-            e = max(abs(a[j, i] - a[j - 1, i]), abs(a[j, i] - a[j - 1, i - 1]))
+            e = max(abs(a[j, i] - a[j - 1, i]), abs(a[j, i] - a[j - 1, i - 1]), noise[i])
             if e <= err:
                 err, best = e, a[j, i]
         if abs(a[i, i] - a[i - 1, i - 1]) >= 2.0 * err and err <= rtol * abs(best):

@@ -130,3 +130,43 @@ def test_world_build_depth_first_order():
         for n in chain:
             M = M @ T(n)
         assert np.allclose(tr[k].reshape(4, 4).T, M), k
+
+
+class _FakeRenderer:
+    """Stands in for a Renderer: a frame's duration depends on the tuning."""
+
+    def __init__(self, cost):
+        self.t, self.cost = (16, 3, 16), cost
+
+    def set_tuning(self, *t):
+        self.t = tuple(t)
+
+    def get_tuning(self):
+        return self.t
+
+    def sync(self):
+        pass
+
+    def frame(self):
+        import time
+        time.sleep(self.cost(self.t))
+
+
+def test_autotune_picks_fastest_valid_candidate(tmp_path, monkeypatch):
+    from vk_gltf_viewer_amd.autotune import autotune
+    monkeypatch.setenv("VKGV_TUNE_CACHE", str(tmp_path / "tune.json"))
+    cand = {"small_max_extent": (4, 16), "raster_blocks_per_cu": (2, 3), "tile_item_tris": (16,)}
+    # small extent 4 is the fastest but "overflows"
+    r = _FakeRenderer(lambda t: 0.0005 if t[0] == 4 else (0.002 if t[1] == 2 else 0.001))
+    best, times = autotune(r, r.frame, "k", candidates=cand, warmup=0, frames=2, use_cache=True,
+                           valid=lambda: r.t[0] != 4)
+    assert best == (16, 3, 16) and r.t == best
+    assert times[(4, 2, 16)] == float("inf") and times[(4, 3, 16)] == float("inf")
+    # cached: applied without timing
+    r.set_tuning(4, 2, 16)
+    best2, times2 = autotune(r, r.frame, "k", candidates=cand, use_cache=True)
+    assert best2 == best and r.t == best and times2 == {}
+    # every candidate rejected: the tuning is left as it was
+    r.set_tuning(32, 2, 8)
+    best3, _ = autotune(r, r.frame, "k2", candidates=cand, warmup=0, frames=1, use_cache=False, valid=lambda: False)
+    assert best3 == (32, 2, 8) and r.t == (32, 2, 8)

@@ -196,7 +196,20 @@ HIP_SYMBOLS = [
     "vkgv_view_read_color", "vkgv_view_read_pyramid", "vkgv_view_write_pyramid", "vkgv_view_write_depth",
     "vkgv_view_read_visible", "vkgv_view_read_stats", "vkgv_set_tuning", "vkgv_get_tuning", "vkgv_srgb_thresholds",
     "vkgv_batch_create", "vkgv_batch_destroy", "vkgv_render_batch", "vkgv_batch_read_stats",
+    "vkgv_view_aa_image", "vkgv_view_aa_backward",
 ]
+
+
+def _one_hip_runtime():
+    """PyTorch-ROCm wheels bundle their own libamdhip64.so.7 and
+    libhsa-runtime64.so.  When this library's HIP runtime (the system ROCm)
+    is loaded first, a later torch import binds torch to it by soname while
+    loading its own HSA runtime, and torch then finds no GPU.  Loading torch
+    first (when it is installed) makes the library share torch's runtime."""
+    try:
+        import torch  # noqa: F401
+    except ImportError:
+        pass
 
 
 def hip():
@@ -205,6 +218,7 @@ def hip():
     if _hip is None:
         if not os.path.exists(HIP_LIB):
             raise RuntimeError(f"{HIP_LIB} is not built: run __graft_entry__.build()")
+        _one_hip_runtime()
         lib = C.CDLL(HIP_LIB)
         _bind(lib, "vkgv_create", C.c_int, C.POINTER(CreateInfo), C.POINTER(_p))
         _bind(lib, "vkgv_destroy", None, _p)
@@ -232,5 +246,7 @@ def hip():
         _bind(lib, "vkgv_batch_destroy", None, _p)
         _bind(lib, "vkgv_render_batch", C.c_int, _p, _p, _p, _u32)
         _bind(lib, "vkgv_batch_read_stats", C.c_int, _p, _p, C.POINTER(FrameStats))
+        _bind(lib, "vkgv_view_aa_image", C.c_int, _p, _p, C.POINTER(Camera), _p, _p)
+        _bind(lib, "vkgv_view_aa_backward", C.c_int, _p, _p, C.POINTER(Camera), _p, _p, _u32, _p)
         _hip = lib
     return _hip

@@ -12,9 +12,12 @@ frame (a batch or a view) on the context's stream. The fastest candidate is
 kept and cached in a JSON file under a key the caller chooses (device, scene,
 views).
 
-The tuner only changes how work is split. It never changes the frame's
-results, which are identical for every tuning: the parity tests run at
-several tunings.
+The tuner only changes how work is split, and the frame's results are
+identical for every tuning (the parity tests run at several tunings) as long
+as the big-triangle queue does not overflow: a lower `small_max_extent`
+sends more triangles to it.  A caller passes `valid` (e.g. "the frame's
+overflow counter is zero"), and candidates whose frames fail it are not
+chosen.
 """
 from __future__ import annotations
 
@@ -55,10 +58,13 @@ def _store(path, data):
 
 
 def autotune(renderer, frame, key: str, candidates=None, warmup: int = 2, frames: int = 5, use_cache: bool = True,
-             log=None):
+             log=None, valid=None):
     """Times `frame()` (which records one frame on the renderer's stream)
-    under each candidate tuning and applies the fastest.  Returns
-    (tuning tuple, {tuning: ms per frame})."""
+    under each candidate tuning and applies the fastest for which `valid()`
+    (called after the candidate's frames; default: always true) holds.
+    Returns (tuning tuple, {tuning: ms per frame}); rejected candidates are
+    listed with an infinite time.  If every candidate is rejected the
+    context's tuning is left as it was and that tuning is returned."""
     path = cache_path()
     cache = _load(path) if use_cache else {}
     if use_cache and key in cache:
@@ -66,6 +72,7 @@ def autotune(renderer, frame, key: str, candidates=None, warmup: int = 2, frames
         renderer.set_tuning(*t)
         return t, {}
     cand = candidates or DEFAULT_CANDIDATES
+    before = tuple(renderer.get_tuning())
     times = {}
     for t in itertools.product(cand["small_max_extent"], cand["raster_blocks_per_cu"], cand["tile_item_tris"]):
         renderer.set_tuning(*t)
@@ -77,9 +84,15 @@ def autotune(renderer, frame, key: str, candidates=None, warmup: int = 2, frames
             frame()
         renderer.sync()
         times[t] = 1000.0 * (time.perf_counter() - t0) / frames
+        ok = valid is None or valid()
         if log:
-            log(f"autotune {key}: {t} {times[t]:.4f} ms")
+            log(f"autotune {key}: {t} {times[t]:.4f} ms" + ("" if ok else " (rejected)"))
+        if not ok:
+            times[t] = float("inf")
     best = min(times, key=times.get)
+    if times[best] == float("inf"):
+        renderer.set_tuning(*before)
+        return before, times
     renderer.set_tuning(*best)
     if use_cache:
         cache[key] = {"tuning": list(best), "ms": times[best]}

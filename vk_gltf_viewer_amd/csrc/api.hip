@@ -833,6 +833,41 @@ int vkgv_view_read_ids_depth(vkgv_ctx* c, vkgv_view* v, uint32_t* ids, float* de
     return VKGV_OK;
 }
 
+static int check_grad(vkgv_ctx* c, vkgv_view* v, const vkgv_camera* cam, const void* a, const void* b)
+{
+    int rc = check_view(c, v);
+    if (rc) return rc;
+    if (!cam || !a || !b) return fail(c, VKGV_ERROR_INVALID_ARGUMENT, "differentiable frame: null argument");
+    if (!c->scene || !c->draws_set) return fail(c, VKGV_ERROR_STATE, "differentiable frame: no scene or draws");
+    return check_frame_inputs(c, 0);
+}
+
+int vkgv_view_aa_image(vkgv_ctx* c, vkgv_view* v, const vkgv_camera* cam, float* image, void* stream)
+{
+    int rc = check_grad(c, v, cam, image, image);
+    if (rc) return rc;
+    hipStream_t st = stream ? static_cast<hipStream_t>(stream) : c->stream;
+    if (v->last_stream && v->last_stream != st) HIP_TRY(c, hipStreamSynchronize(v->last_stream));
+    launch_aa_image(scene_dev(c), v->dev.keys, v->dev.width, v->dev.height, cam->view_projection, image, st);
+    HIP_TRY(c, hipGetLastError());
+    return VKGV_OK;
+}
+
+int vkgv_view_aa_backward(vkgv_ctx* c, vkgv_view* v, const vkgv_camera* cam, const float* g_image,
+                          double* g_transforms, uint32_t transform_count, void* stream)
+{
+    int rc = check_grad(c, v, cam, g_image, g_transforms);
+    if (rc) return rc;
+    if (transform_count < c->transform_count)
+        return fail(c, VKGV_ERROR_INVALID_ARGUMENT, "aa_backward: g_transforms holds fewer transforms than the scene");
+    hipStream_t st = stream ? static_cast<hipStream_t>(stream) : c->stream;
+    if (v->last_stream && v->last_stream != st) HIP_TRY(c, hipStreamSynchronize(v->last_stream));
+    launch_aa_backward(scene_dev(c), v->dev.keys, v->dev.width, v->dev.height, cam->view_projection, g_image,
+                       g_transforms, st);
+    HIP_TRY(c, hipGetLastError());
+    return VKGV_OK;
+}
+
 int vkgv_view_read_color(vkgv_ctx* c, vkgv_view* v, uint32_t* rgba8)
 {
     if (check_view(c, v)) return check_view(c, v);

@@ -173,6 +173,37 @@ class View:
     def build_pyramid(self):
         self.r._check(self.r._lib.vkgv_view_build_pyramid(self.r._h, self._h, None))
 
+    # -- differentiable frame (torch device tensors, torch's current stream) ---------
+    def _tensor(self, t, shape, dtype, name):
+        import torch
+        if not (isinstance(t, torch.Tensor) and t.is_cuda and t.dtype == dtype and t.is_contiguous()
+                and tuple(t.shape) == tuple(shape)):
+            raise ValueError(f"{name} must be a contiguous {dtype} device tensor of shape {tuple(shape)}")
+        return t
+
+    def aa_image(self, camera: N.Camera, out=None):
+        """Antialiased linear image [H, W, 3] float32 of the last frame (the
+        frame's flat resolve before the sRGB encode, blended along the front
+        triangle's edge for every pixel pair of different colour)."""
+        import torch
+        if out is None:
+            out = torch.empty((self.height, self.width, 3), dtype=torch.float32, device="cuda")
+        self._tensor(out, (self.height, self.width, 3), torch.float32, "out")
+        st = torch.cuda.current_stream().cuda_stream
+        self.r._check(self.r._lib.vkgv_view_aa_image(self.r._h, self._h, C.byref(camera), out.data_ptr(), st))
+        return out
+
+    def aa_backward(self, camera: N.Camera, g_image, g_transforms):
+        """g_transforms (float64 [T, 16] device tensor) += d(sum g_image * image) / d transforms."""
+        import torch
+        self._tensor(g_image, (self.height, self.width, 3), torch.float32, "g_image")
+        if not (g_transforms.is_cuda and g_transforms.dtype == torch.float64 and g_transforms.is_contiguous()
+                and g_transforms.dim() == 2 and g_transforms.shape[1] == 16):
+            raise ValueError("g_transforms must be a contiguous float64 [T, 16] device tensor")
+        st = torch.cuda.current_stream().cuda_stream
+        self.r._check(self.r._lib.vkgv_view_aa_backward(self.r._h, self._h, C.byref(camera), g_image.data_ptr(),
+                                                        g_transforms.data_ptr(), g_transforms.shape[0], st))
+
     def reset_history(self):
         self.r._check(self.r._lib.vkgv_view_reset_history(self.r._h, self._h))
 
