@@ -180,15 +180,18 @@ def uneven_views(steps, warmup, log):
             for v, c, im in zip(views, cams, imgs):
                 v.aa_image(c, im)
                 v.aa_backward(c, im, g)
-        for _ in range(max(1, warmup)):
-            grad_step()
-        torch.cuda.synchronize()
-        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-        e0.record()
-        for _ in range(steps):
-            grad_step()
-        e1.record()
-        torch.cuda.synchronize()
+        # a stream of its own (not torch's default, which the library would fence)
+        gs = torch.cuda.Stream()
+        with torch.cuda.stream(gs):
+            for _ in range(max(1, warmup)):
+                grad_step()
+            torch.cuda.synchronize()
+            e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            e0.record(gs)
+            for _ in range(steps):
+                grad_step()
+            e1.record(gs)
+            torch.cuda.synchronize()
         ms_grad = e0.elapsed_time(e1) / steps
         grad_live = int((g.abs().amax(1) > 0).sum().item())
         vis_draws = [int(len(v.visible(0)) + len(v.visible(1))) for v in views]

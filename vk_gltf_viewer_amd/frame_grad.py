@@ -59,7 +59,10 @@ class DifferentiableFrame:
         self._count = host.shape[0]
         self._device = transforms.device
         self._dtype = transforms.dtype
-        self.batch.render(self.cams, self.flags, stream=torch.cuda.current_stream().cuda_stream)
+        s = torch.cuda.current_stream()
+        # the default torch stream (handle 0) means the context's own stream to the
+        # library; aa_image below orders it with torch's stream
+        self.batch.render(self.cams, self.flags, stream=s.cuda_stream or None)
         return [v.aa_image(c) for v, c in zip(self.views, self.cams)]
 
     def _backward(self, g_images):

@@ -174,6 +174,22 @@ class View:
         self.r._check(self.r._lib.vkgv_view_build_pyramid(self.r._h, self._h, None))
 
     # -- differentiable frame (torch device tensors, torch's current stream) ---------
+    def _torch_call(self, fn):
+        """Runs fn(stream) ordered with torch's current stream.  A torch
+        stream other than the default one is passed through, so the call
+        stays asynchronous on it.  torch's default stream is the legacy null
+        stream (handle 0), which this API reads as "the context's stream",
+        and that stream is non-blocking, so the call is then fenced: torch's
+        work is finished first and the context's after."""
+        import torch
+        s = torch.cuda.current_stream()
+        if s.cuda_stream:
+            return fn(s.cuda_stream)
+        s.synchronize()
+        out = fn(None)
+        self.r.sync()
+        return out
+
     def _tensor(self, t, shape, dtype, name):
         import torch
         if not (isinstance(t, torch.Tensor) and t.is_cuda and t.dtype == dtype and t.is_contiguous()
@@ -189,8 +205,8 @@ class View:
         if out is None:
             out = torch.empty((self.height, self.width, 3), dtype=torch.float32, device="cuda")
         self._tensor(out, (self.height, self.width, 3), torch.float32, "out")
-        st = torch.cuda.current_stream().cuda_stream
-        self.r._check(self.r._lib.vkgv_view_aa_image(self.r._h, self._h, C.byref(camera), out.data_ptr(), st))
+        self._torch_call(lambda st: self.r._check(
+            self.r._lib.vkgv_view_aa_image(self.r._h, self._h, C.byref(camera), out.data_ptr(), st)))
         return out
 
     def aa_backward(self, camera: N.Camera, g_image, g_transforms):
@@ -200,9 +216,9 @@ class View:
         if not (g_transforms.is_cuda and g_transforms.dtype == torch.float64 and g_transforms.is_contiguous()
                 and g_transforms.dim() == 2 and g_transforms.shape[1] == 16):
             raise ValueError("g_transforms must be a contiguous float64 [T, 16] device tensor")
-        st = torch.cuda.current_stream().cuda_stream
-        self.r._check(self.r._lib.vkgv_view_aa_backward(self.r._h, self._h, C.byref(camera), g_image.data_ptr(),
-                                                        g_transforms.data_ptr(), g_transforms.shape[0], st))
+        self._torch_call(lambda st: self.r._check(self.r._lib.vkgv_view_aa_backward(
+            self.r._h, self._h, C.byref(camera), g_image.data_ptr(), g_transforms.data_ptr(), g_transforms.shape[0],
+            st)))
 
     def reset_history(self):
         self.r._check(self.r._lib.vkgv_view_reset_history(self.r._h, self._h))
