@@ -66,6 +66,7 @@ struct vkgv_view {
     ViewDev dev{};
     DevBuf keys, color, pyramid, visible0, visible1, history, counters, big, clipq, bin_count, bin_start, bins;
     DevBuf setups, pairs, tri_cache, slot_albedo, draw_slot, big_chunk, chunk_tri, tile_items;
+    DevBuf aa_mat;                   // per-pixel material indices of the differentiable frame (allocated on first use)
     uint32_t list_capacity = 0;
     // Ring of per-frame event sets: timed frames are averaged at the next
     // read_stats without synchronizing between frames.
@@ -716,7 +717,8 @@ void vkgv_view_destroy(vkgv_view* v)
     }
     for (DevBuf* b : { &v->keys, &v->color, &v->pyramid, &v->visible0, &v->visible1, &v->history, &v->counters, &v->big,
                        &v->clipq, &v->bin_count, &v->bin_start, &v->bins, &v->setups, &v->pairs, &v->tri_cache,
-                       &v->slot_albedo, &v->draw_slot, &v->big_chunk, &v->chunk_tri, &v->tile_items, &v->maps })
+                       &v->slot_albedo, &v->draw_slot, &v->big_chunk, &v->chunk_tri, &v->tile_items, &v->maps,
+                       &v->aa_mat })
         dfree(*b);
     for (auto& set : v->ev)
         for (auto& e : set)
@@ -833,6 +835,13 @@ int vkgv_view_read_ids_depth(vkgv_ctx* c, vkgv_view* v, uint32_t* ids, float* de
     return VKGV_OK;
 }
 
+static int ensure_aa_mat(vkgv_ctx* c, vkgv_view* v)
+{
+    if (v->aa_mat.p) return VKGV_OK;
+    HIP_TRY(c, hipDeviceSynchronize());   // the allocation must not race work of other streams on the buffer list
+    return dalloc(c, v->aa_mat, 4ull * v->dev.width * v->dev.height);
+}
+
 static int check_grad(vkgv_ctx* c, vkgv_view* v, const vkgv_camera* cam, const void* a, const void* b)
 {
     int rc = check_view(c, v);
@@ -848,7 +857,9 @@ int vkgv_view_aa_image(vkgv_ctx* c, vkgv_view* v, const vkgv_camera* cam, float*
     if (rc) return rc;
     hipStream_t st = stream ? static_cast<hipStream_t>(stream) : c->stream;
     if (v->last_stream && v->last_stream != st) HIP_TRY(c, hipStreamSynchronize(v->last_stream));
-    launch_aa_image(scene_dev(c), v->dev.keys, v->dev.width, v->dev.height, cam->view_projection, image, st);
+    if ((rc = ensure_aa_mat(c, v))) return rc;
+    launch_aa_image(scene_dev(c), v->dev.keys, static_cast<uint32_t*>(v->aa_mat.p), v->dev.width, v->dev.height,
+                    cam->view_projection, image, st);
     HIP_TRY(c, hipGetLastError());
     return VKGV_OK;
 }
@@ -862,8 +873,9 @@ int vkgv_view_aa_backward(vkgv_ctx* c, vkgv_view* v, const vkgv_camera* cam, con
         return fail(c, VKGV_ERROR_INVALID_ARGUMENT, "aa_backward: g_transforms holds fewer transforms than the scene");
     hipStream_t st = stream ? static_cast<hipStream_t>(stream) : c->stream;
     if (v->last_stream && v->last_stream != st) HIP_TRY(c, hipStreamSynchronize(v->last_stream));
-    launch_aa_backward(scene_dev(c), v->dev.keys, v->dev.width, v->dev.height, cam->view_projection, g_image,
-                       g_transforms, st);
+    if ((rc = ensure_aa_mat(c, v))) return rc;
+    launch_aa_backward(scene_dev(c), v->dev.keys, static_cast<uint32_t*>(v->aa_mat.p), v->dev.width, v->dev.height,
+                       cam->view_projection, g_image, g_transforms, st);
     HIP_TRY(c, hipGetLastError());
     return VKGV_OK;
 }

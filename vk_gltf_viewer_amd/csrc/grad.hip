@@ -22,6 +22,9 @@
 // pixel's centre: the edge term scales as 1 / (edge function difference)^2,
 // which float32 clip positions do not resolve.
 //
+// A first pass writes each pixel's material index, so that the pairs of equal
+// colour (nearly all of them) are skipped after two coalesced loads.
+//
 // Backward: one thread per pair; d loss / d M_tr = sum over the crossed
 // edge's two vertices of (VP^T g_clip) (p, 1)^T.  Lanes of a wave mostly
 // share a transform, so each wave reduces its 16-value contributions per
@@ -38,9 +41,12 @@ constexpr uint32_t kGradBlock = 256;
 struct GradArgs {
     SceneDev s;
     const unsigned long long* keys;
+    const uint32_t* mat;    // per pixel: material index, kNoMaterial on background
     uint32_t W, H;
     double vp[16];          // column-major
 };
+
+constexpr uint32_t kNoMaterial = 0xFFFFFFFFu;
 
 __device__ __forceinline__ bool key_id(unsigned long long k, uint32_t& id)
 {
@@ -49,17 +55,21 @@ __device__ __forceinline__ bool key_id(unsigned long long k, uint32_t& id)
     return true;
 }
 
-// albedo factor of the pixel's material; false on background or a bad id
-__device__ __forceinline__ bool color_of(const GradArgs& a, unsigned long long k, float3& c, uint32_t& mat)
+// material of a key's draw; kNoMaterial on background or an id outside the draw list
+__device__ __forceinline__ uint32_t material_of_key(const SceneDev& s, unsigned long long k)
 {
     uint32_t id;
-    if (!key_id(k, id)) { c = make_float3(0.0f, 0.0f, 0.0f); mat = 0xFFFFFFFFu; return false; }
+    if (!key_id(k, id)) return kNoMaterial;
     const uint32_t d = id >> VKGV_TRIANGLE_BITS;
-    if (d >= a.s.draw_count) { c = make_float3(0.0f, 0.0f, 0.0f); mat = 0xFFFFFFFFu; return false; }
-    mat = a.s.rrec[d].material & 0x7FFFFFFFu;
-    const vkgv_material& m = a.s.materials[mat];
-    c = make_float3(m.albedo_factor[0], m.albedo_factor[1], m.albedo_factor[2]);
-    return true;
+    if (d >= s.draw_count) return kNoMaterial;
+    return s.rrec[d].material & 0x7FFFFFFFu;
+}
+
+__device__ __forceinline__ float3 albedo(const SceneDev& s, uint32_t m)
+{
+    if (m == kNoMaterial) return make_float3(0.0f, 0.0f, 0.0f);
+    const vkgv_material& mt = s.materials[m];
+    return make_float3(mt.albedo_factor[0], mt.albedo_factor[1], mt.albedo_factor[2]);
 }
 
 struct DPair {
@@ -83,13 +93,10 @@ __device__ bool pair_eval(const GradArgs& a, int px, int py, int dir, DPair& e)
     const int qx = px + (dir == 0), qy = py + (dir == 1);
     if (qx >= (int)a.W || qy >= (int)a.H) return false;
     const size_t ip = (size_t)py * a.W + px, iq = (size_t)qy * a.W + qx;
+    const uint32_t mp = a.mat[ip], mq = a.mat[iq];
+    if (mp == mq) return false;                        // equal colours (or both background): no blend
+    const bool hp = mp != kNoMaterial, hq = mq != kNoMaterial;
     const unsigned long long kp = a.keys[ip], kq = a.keys[iq];
-    if (kp == kq) return false;
-    float3 cp, cq;
-    uint32_t mp, mq;
-    const bool hp = color_of(a, kp, cp, mp), hq = color_of(a, kq, cq, mq);
-    if (!hp && !hq) return false;
-    if (hp && hq && mp == mq) return false;            // equal colours: no blend
     const bool p_front = !hp ? false : (!hq ? true : kp <= kq);
     e.f = p_front ? ip : iq;
     e.b = p_front ? iq : ip;
@@ -148,15 +155,21 @@ __device__ bool pair_eval(const GradArgs& a, int px, int py, int dir, DPair& e)
     return true;
 }
 
+// Per pixel: the material of its draw (kNoMaterial on background).
+__global__ __launch_bounds__(kGradBlock) void pixel_material_kernel(SceneDev s, const unsigned long long* __restrict__ keys,
+                                                                   size_t n, uint32_t* __restrict__ mat)
+{
+    const size_t p = (size_t)blockIdx.x * kGradBlock + threadIdx.x;
+    if (p < n) mat[p] = material_of_key(s, keys[p]);
+}
+
 // Gather: pixel p = its colour + the blends of its four pairs that target it.
 __global__ __launch_bounds__(kGradBlock) void aa_image_kernel(GradArgs a, float* __restrict__ image)
 {
     const size_t p = (size_t)blockIdx.x * kGradBlock + threadIdx.x;
     if (p >= (size_t)a.W * a.H) return;
     const int px = (int)(p % a.W), py = (int)(p / a.W);
-    float3 c;
-    uint32_t mat;
-    color_of(a, a.keys[p], c, mat);
+    const float3 c = albedo(a.s, a.mat[p]);
     double acc[3] = { c.x, c.y, c.z };
     const int sx[4] = { px - 1, px, px, px }, sy[4] = { py, py, py - 1, py }, dr[4] = { 0, 0, 1, 1 };
     for (int q = 0; q < 4; ++q) {
@@ -165,9 +178,7 @@ __global__ __launch_bounds__(kGradBlock) void aa_image_kernel(GradArgs a, float*
         if (!pair_eval(a, sx[q], sy[q], dr[q], e)) continue;
         const size_t tgt = e.to_front ? e.f : e.b, src = e.to_front ? e.b : e.f;
         if (tgt != p) continue;
-        float3 cs;
-        uint32_t ms;
-        color_of(a, a.keys[src], cs, ms);
+        const float3 cs = albedo(a.s, a.mat[src]);
         acc[0] += e.alpha * ((double)cs.x - c.x);
         acc[1] += e.alpha * ((double)cs.y - c.y);
         acc[2] += e.alpha * ((double)cs.z - c.z);
@@ -198,10 +209,7 @@ __global__ __launch_bounds__(kGradBlock) void aa_backward_kernel(GradArgs a, con
         DPair e;
         if (pair_eval(a, (int)(p % a.W), (int)(p / a.W), dir, e)) {
             const size_t tgt = e.to_front ? e.f : e.b, src = e.to_front ? e.b : e.f;
-            float3 ct, cs;
-            uint32_t m0, m1;
-            color_of(a, a.keys[tgt], ct, m0);
-            color_of(a, a.keys[src], cs, m1);
+            const float3 ct = albedo(a.s, a.mat[tgt]), cs = albedo(a.s, a.mat[src]);
             const double galpha = (double)g_image[3 * tgt] * ((double)cs.x - ct.x) +
                                   (double)g_image[3 * tgt + 1] * ((double)cs.y - ct.y) +
                                   (double)g_image[3 * tgt + 2] * ((double)cs.z - ct.z);
@@ -248,11 +256,13 @@ __global__ __launch_bounds__(kGradBlock) void aa_backward_kernel(GradArgs a, con
 
 inline unsigned grad_blocks(size_t n) { return (unsigned)((n + kGradBlock - 1) / kGradBlock); }
 
-GradArgs make_args(const SceneDev& s, const unsigned long long* keys, uint32_t W, uint32_t H, const float* vp)
+GradArgs make_args(const SceneDev& s, const unsigned long long* keys, const uint32_t* mat, uint32_t W, uint32_t H,
+                   const float* vp)
 {
     GradArgs a;
     a.s = s;
     a.keys = keys;
+    a.mat = mat;
     a.W = W;
     a.H = H;
     for (int i = 0; i < 16; ++i) a.vp[i] = (double)vp[i];
@@ -261,20 +271,23 @@ GradArgs make_args(const SceneDev& s, const unsigned long long* keys, uint32_t W
 
 } // namespace
 
-void launch_aa_image(const SceneDev& s, const unsigned long long* keys, uint32_t W, uint32_t H, const float* vp,
-                     float* image, hipStream_t st)
+void launch_aa_image(const SceneDev& s, const unsigned long long* keys, uint32_t* mat, uint32_t W, uint32_t H,
+                     const float* vp, float* image, hipStream_t st)
 {
     const size_t n = (size_t)W * H;
     if (n == 0) return;
-    aa_image_kernel<<<grad_blocks(n), kGradBlock, 0, st>>>(make_args(s, keys, W, H, vp), image);
+    pixel_material_kernel<<<grad_blocks(n), kGradBlock, 0, st>>>(s, keys, n, mat);
+    aa_image_kernel<<<grad_blocks(n), kGradBlock, 0, st>>>(make_args(s, keys, mat, W, H, vp), image);
 }
 
-void launch_aa_backward(const SceneDev& s, const unsigned long long* keys, uint32_t W, uint32_t H, const float* vp,
-                        const float* g_image, double* g_transforms, hipStream_t st)
+void launch_aa_backward(const SceneDev& s, const unsigned long long* keys, uint32_t* mat, uint32_t W, uint32_t H,
+                        const float* vp, const float* g_image, double* g_transforms, hipStream_t st)
 {
-    const size_t n = 2 * (size_t)W * H;
+    const size_t n = (size_t)W * H;
     if (n == 0) return;
-    aa_backward_kernel<<<grad_blocks(n), kGradBlock, 0, st>>>(make_args(s, keys, W, H, vp), g_image, g_transforms);
+    pixel_material_kernel<<<grad_blocks(n), kGradBlock, 0, st>>>(s, keys, n, mat);
+    aa_backward_kernel<<<grad_blocks(2 * n), kGradBlock, 0, st>>>(make_args(s, keys, mat, W, H, vp), g_image,
+                                                                  g_transforms);
 }
 
 } // namespace vkgv

@@ -245,10 +245,11 @@ void launch_resolve(const SceneDev& s, const BatchDev& b, const BatchGrid& g, ui
 void launch_write_depth_keys(unsigned long long* keys, const float* depth, uint64_t n, hipStream_t st);
 // Differentiable frame (grad.hip): antialiased linear image of a view's keys,
 // and its backward to the instance transforms (accumulated into g_transforms).
-void launch_aa_image(const SceneDev& s, const unsigned long long* keys, uint32_t W, uint32_t H, const float* vp,
-                     float* image, hipStream_t st);
-void launch_aa_backward(const SceneDev& s, const unsigned long long* keys, uint32_t W, uint32_t H, const float* vp,
-                        const float* g_image, double* g_transforms, hipStream_t st);
+// mat: W*H scratch of per-pixel material indices (written, then read).
+void launch_aa_image(const SceneDev& s, const unsigned long long* keys, uint32_t* mat, uint32_t W, uint32_t H,
+                     const float* vp, float* image, hipStream_t st);
+void launch_aa_backward(const SceneDev& s, const unsigned long long* keys, uint32_t* mat, uint32_t W, uint32_t H,
+                        const float* vp, const float* g_image, double* g_transforms, hipStream_t st);
 uint32_t hiz_tile_levels(const PyramidDesc& pd);
 
 // Tunables of the raster (set by the autotuner; defaults otherwise).
