@@ -325,8 +325,8 @@ def main():
 
     r = Renderer(0)
     if args.small_max or args.blocks_per_cu:
-        sm, bpc, tit = r.get_tuning()
-        r.set_tuning(args.small_max or sm, args.blocks_per_cu or bpc, tit)
+        sm, bpc, tit, rcs = r.get_tuning()
+        r.set_tuning(args.small_max or sm, args.blocks_per_cu or bpc, tit, rcs)
     r.load(sc)
     view = r.create_view(W, H)
     flags = N.CULL_FRUSTUM | N.CULL_TWO_PASS | N.RESOLVE_BARY

@@ -82,7 +82,7 @@ def structured_depth(w, h, seed):
 
 
 @pytest.mark.parametrize("w,h", [(1280, 720), (3840, 2160), (1337, 893), (64, 64), (97, 31), (2, 1), (1, 7),
-                                 (2048, 2048), (513, 1025)])
+                                 (2048, 2048), (513, 1025), (1999, 1001), (4095, 4095)])
 def test_hiz_bit_exact(rend, w, h):
     sc = scene(1, 1)
     load(rend, sc)

@@ -112,7 +112,8 @@ class FrameStats(C.Structure):
 
 
 class Tuning(C.Structure):
-    _fields_ = [("small_max_extent", C.c_uint32), ("raster_blocks_per_cu", C.c_uint32), ("tile_item_tris", C.c_uint32)]
+    _fields_ = [("small_max_extent", C.c_uint32), ("raster_blocks_per_cu", C.c_uint32), ("tile_item_tris", C.c_uint32),
+                ("resolve_cache_slots", C.c_uint32)]
 
 
 class CreateInfo(C.Structure):

@@ -6,6 +6,10 @@ The knobs:
 - `raster_blocks_per_cu` sets the persistent raster grid, i.e. the
   wave-to-meshlet mapping.
 - `tile_item_tris` is the tile pass's work-item size.
+- `resolve_cache_slots` is how many rasterized meshlets per view leave their
+  triangles for the barycentric resolve: the raster writes 48 bytes per
+  triangle of a cached meshlet, the resolve recomputes the others from the
+  scene buffers.  Many more rasterized triangles than pixels favour 0.
 
 A candidate is timed on the caller's own frame: a function that records one
 frame (a batch or a view) on the context's stream. The fastest candidate is
@@ -30,7 +34,9 @@ DEFAULT_CANDIDATES = {
     "small_max_extent": (4, 8, 16, 32),
     "raster_blocks_per_cu": (2, 3),
     "tile_item_tris": (8, 16, 32),
+    "resolve_cache_slots": (0, 8192),
 }
+KNOBS = ("small_max_extent", "raster_blocks_per_cu", "tile_item_tris", "resolve_cache_slots")
 
 
 def cache_path() -> str:
@@ -73,8 +79,10 @@ def autotune(renderer, frame, key: str, candidates=None, warmup: int = 2, frames
         return t, {}
     cand = candidates or DEFAULT_CANDIDATES
     before = tuple(renderer.get_tuning())
+    # one axis per knob the renderer reports; a knob the candidates leave out keeps its value
+    axes = [cand.get(k, (before[i],)) for i, k in enumerate(KNOBS[:len(before)])]
     times = {}
-    for t in itertools.product(cand["small_max_extent"], cand["raster_blocks_per_cu"], cand["tile_item_tris"]):
+    for t in itertools.product(*axes):
         renderer.set_tuning(*t)
         for _ in range(warmup):
             frame()

@@ -53,7 +53,7 @@ struct vkgv_ctx {
     uint32_t draw_count = 0, transform_count = 0;
     uint32_t max_transform = 0;
     bool draws_set = false;
-    RasterTuning tuning{ 16u, 3u, 16u };
+    RasterTuning tuning{ 16u, 3u, 16u, 8192u };
     DevBuf srgb;                     // 255 sRGB-encode thresholds
     uint32_t big_capacity = 1u << 18;
     uint32_t clip_capacity = 1u << 16;
@@ -102,7 +102,7 @@ struct vkgv_batch {
     hipGraphExec_t exec = nullptr;
     uint32_t exec_flags = 0;
     hipStream_t exec_stream = nullptr;
-    RasterTuning exec_tuning{ 0u, 0u, 0u };
+    RasterTuning exec_tuning{ 0u, 0u, 0u, 0u };
     // whole-frame timing ring
     static constexpr uint32_t kRing = 64;
     hipEvent_t ev0[kRing] = {}, ev1[kRing] = {};
@@ -267,7 +267,9 @@ int build_maps(vkgv_ctx* c, const std::vector<const ViewDev*>& views, DevBuf& ma
         for (uint64_t k = 0; k < nclear; ++k) clear.push_back(vi | (uint32_t)(k << 8));
         for (uint64_t k = 0; k < nhiz; ++k) hiz.push_back(vi | (uint32_t)(k << 8));
         for (uint64_t k = 0; k < nres; ++k) resolve.push_back(vi | (uint32_t)(k << 8));
-        if (v.hiz_tile_levels < v.pyr.levels) g.max_tail_levels = std::max(g.max_tail_levels, v.pyr.levels);
+        if (v.hiz_tail_start < v.pyr.levels) g.max_tail_levels = std::max(g.max_tail_levels, v.pyr.levels);
+        for (uint32_t k = v.hiz_tile_levels; k < v.hiz_tail_start && k < kHizMaxLevels; ++k)
+            g.hiz_level_blocks[k] = std::max(g.hiz_level_blocks[k], (uint32_t)(((uint64_t)v.pyr.w[k] * v.pyr.h[k] + 255) / 256));
     }
     g.clear_blocks = (uint32_t)clear.size();
     g.hiz_blocks = (uint32_t)hiz.size();
@@ -316,7 +318,7 @@ void frame_launches(vkgv_ctx* c, const BatchDev& b, const BatchGrid& g, uint32_t
     mark(EV_CLEAR);
     launch_cull(s, b, g, flags, two_pass ? CULL_EARLY : CULL_SINGLE, 0, st);
     mark(EV_CULL);
-    launch_raster(s, b, flags, 0, raster_grid, c->tuning.small_max_extent, st);
+    launch_raster(s, b, flags, 0, raster_grid, c->tuning.small_max_extent, c->tuning.resolve_cache_slots, st);
     launch_raster_big(b, g, tile_grid, c->tuning.tile_item_tris, st);
     mark(EV_RASTER);
     if (two_pass) {
@@ -324,7 +326,7 @@ void frame_launches(vkgv_ctx* c, const BatchDev& b, const BatchGrid& g, uint32_t
         mark(EV_HIZ);
         launch_cull(s, b, g, flags, CULL_LATE, 1, st);
         mark(EV_CULL_LATE);
-        launch_raster(s, b, flags, 1, raster_grid, c->tuning.small_max_extent, st);
+        launch_raster(s, b, flags, 1, raster_grid, c->tuning.small_max_extent, c->tuning.resolve_cache_slots, st);
         launch_raster_big(b, g, tile_grid, c->tuning.tile_item_tris, st);
         mark(EV_RASTER_LATE);
     }
@@ -404,12 +406,14 @@ int vkgv_set_tuning(vkgv_ctx* c, const vkgv_tuning* t)
 {
     if (!c || !t) return VKGV_ERROR_INVALID_ARGUMENT;
     if (t->small_max_extent < 1 || t->small_max_extent > 64 || t->raster_blocks_per_cu < 1 || t->raster_blocks_per_cu > 16
-        || t->tile_item_tris < 1 || t->tile_item_tris > 256)
+        || t->tile_item_tris < 1 || t->tile_item_tris > 256 || t->resolve_cache_slots > c->cache_slots)
         return fail(c, VKGV_ERROR_INVALID_ARGUMENT,
-                    "set_tuning: small_max_extent 1..64, raster_blocks_per_cu 1..16, tile_item_tris 1..256");
+                    "set_tuning: small_max_extent 1..64, raster_blocks_per_cu 1..16, tile_item_tris 1..256, "
+                    "resolve_cache_slots 0.." + std::to_string(c->cache_slots));
     c->tuning.small_max_extent = t->small_max_extent;
     c->tuning.raster_blocks_per_cu = t->raster_blocks_per_cu;
     c->tuning.tile_item_tris = t->tile_item_tris;
+    c->tuning.resolve_cache_slots = t->resolve_cache_slots;
     return VKGV_OK;
 }
 
@@ -419,6 +423,7 @@ int vkgv_get_tuning(const vkgv_ctx* c, vkgv_tuning* t)
     t->small_max_extent = c->tuning.small_max_extent;
     t->raster_blocks_per_cu = c->tuning.raster_blocks_per_cu;
     t->tile_item_tris = c->tuning.tile_item_tris;
+    t->resolve_cache_slots = c->tuning.resolve_cache_slots;
     return VKGV_OK;
 }
 
@@ -682,6 +687,7 @@ int vkgv_view_create(vkgv_ctx* c, uint32_t width, uint32_t height, vkgv_view** o
     d.bins = static_cast<uint32_t*>(v->bins.p);
     d.bin_capacity = bin_cap;
     d.hiz_tile_levels = hiz_tile_levels(d.pyr);
+    d.hiz_tail_start = hiz_tail_start(d.pyr, d.hiz_tile_levels);
     d.hiz_tiles_x = (d.pyr.w[0] + 31) / 32;
     d.resolve_blocks_x = (width + 63) / 64;
     {
@@ -1095,7 +1101,8 @@ int vkgv_render_batch(vkgv_ctx* c, vkgv_batch* b, const vkgv_frame_desc* frames,
         const bool stale = !b->exec || b->exec_flags != fl || b->exec_stream != st
                            || b->exec_tuning.small_max_extent != c->tuning.small_max_extent
                            || b->exec_tuning.raster_blocks_per_cu != c->tuning.raster_blocks_per_cu
-                           || b->exec_tuning.tile_item_tris != c->tuning.tile_item_tris;
+                           || b->exec_tuning.tile_item_tris != c->tuning.tile_item_tris
+                           || b->exec_tuning.resolve_cache_slots != c->tuning.resolve_cache_slots;
         if (stale) {
             if (b->exec) { (void)hipGraphExecDestroy(b->exec); b->exec = nullptr; }
             HIP_TRY(c, hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));

@@ -88,7 +88,7 @@ __global__ __launch_bounds__(256) void hiz_tiles_kernel(BatchDev b)
 // enough for LDS is also kept there, so that the following levels read LDS
 // instead of global memory (one global round trip for the whole tail when
 // its first level fits).
-constexpr uint32_t kTailA = 8192, kTailB = 4096;   // floats of the two LDS level buffers
+constexpr uint32_t kTailA = kHizTailTexels, kTailB = kHizTailTexels / 2;   // floats of the two LDS level buffers
 
 __global__ __launch_bounds__(1024) void hiz_tail_kernel(BatchDev b)
 {
@@ -99,7 +99,7 @@ __global__ __launch_bounds__(1024) void hiz_tail_kernel(BatchDev b)
     float* pyr = v.pyramid;
     const float* lsrc = nullptr;          // LDS copy of level k-1, if any
     bool next_a = true;                   // LDS buffer for the next level kept in LDS
-    for (uint32_t k = v.hiz_tile_levels; k < pd.levels; ++k) {
+    for (uint32_t k = v.hiz_tail_start; k < pd.levels; ++k) {
         const uint32_t sw = pd.w[k - 1], sh = pd.h[k - 1], dw = pd.w[k], dh = pd.h[k];
         const float* src = lsrc ? lsrc : pyr + pd.off[k - 1];
         float* dst = pyr + pd.off[k];
@@ -123,7 +123,40 @@ __global__ __launch_bounds__(1024) void hiz_tail_kernel(BatchDev b)
     }
 }
 
+// Levels after the tile pass that are too large for the tail's single block
+// (a source level of odd size ends the tile pass early, e.g. at level 1 of a
+// view of odd width): one launch per level for every view, 256 texels per
+// block; block (x, view).  Same footprint as the tail.
+__global__ __launch_bounds__(256) void hiz_level_kernel(BatchDev b, uint32_t k)
+{
+    const ViewDev& v = batch_view(b, blockIdx.y);
+    if (k < v.hiz_tile_levels || k >= v.hiz_tail_start) return;
+    const PyramidDesc& pd = v.pyr;
+    const uint32_t sw = pd.w[k - 1], sh = pd.h[k - 1], dw = pd.w[k], dh = pd.h[k];
+    const uint32_t t = blockIdx.x * 256u + threadIdx.x;
+    if (t >= dw * dh) return;
+    const float* src = v.pyramid + pd.off[k - 1];
+    const uint32_t x = t % dw, y = t / dw;
+    int x0, x1, y0, y1;
+    footprint(x, dw, sw, x0, x1);
+    footprint(y, dh, sh, y0, y1);
+    float d = src[y0 * sw + x0];
+    d = gmin(d, src[y0 * sw + x1]);
+    d = gmin(d, src[y1 * sw + x0]);
+    d = gmin(d, src[y1 * sw + x1]);
+    v.pyramid[pd.off[k] + t] = d;
+}
+
 } // namespace
+
+// First level of the tail: the first level after the tile pass small enough
+// for the tail's LDS buffer.
+uint32_t hiz_tail_start(const PyramidDesc& pd, uint32_t tile_levels)
+{
+    uint32_t s = tile_levels;
+    while (s < pd.levels && (uint64_t)pd.w[s] * pd.h[s] > kHizTailTexels) ++s;
+    return s;
+}
 
 // Number of levels the tile kernel can produce: level 0 plus every following
 // level whose source is exactly twice its size, at most 6 (32 -> 1).
@@ -137,6 +170,8 @@ uint32_t hiz_tile_levels(const PyramidDesc& pd)
 void launch_hiz(const BatchDev& b, const BatchGrid& g, hipStream_t st)
 {
     if (g.hiz_blocks) hiz_tiles_kernel<<<g.hiz_blocks, 256, 0, st>>>(b);
+    for (uint32_t k = 1; k < kHizMaxLevels; ++k)
+        if (g.hiz_level_blocks[k]) hiz_level_kernel<<<dim3(g.hiz_level_blocks[k], g.nviews), 256, 0, st>>>(b, k);
     if (g.max_tail_levels) hiz_tail_kernel<<<g.nviews, 1024, 0, st>>>(b);
 }
 

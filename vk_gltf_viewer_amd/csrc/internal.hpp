@@ -180,6 +180,7 @@ struct ViewDev {
     uint32_t* draw_slot;
     uint32_t cache_slots;
     uint32_t hiz_tile_levels;       // pyramid levels produced by the tile pass of the Hi-Z build
+    uint32_t hiz_tail_start;        // first level of the one-block tail; levels in between: hiz_level_kernel
     uint32_t hiz_tiles_x;           // 32x32 level-0 tiles per row
     uint32_t resolve_blocks_x;      // 64-pixel resolve blocks per row
 };
@@ -219,11 +220,15 @@ __device__ __forceinline__ const ViewDev& batch_view(const BatchDev& b, uint32_t
 }
 
 // Host-side sizes of a batch's launches.
+constexpr uint32_t kHizMaxLevels = 16;
+constexpr uint32_t kHizTailTexels = 8192;   // largest first tail level (the tail keeps it in LDS)
+
 struct BatchGrid {
     uint32_t nviews;
     uint32_t clear_blocks, hiz_blocks, resolve_blocks;
     uint32_t max_tail_levels;       // any view needs the one-block Hi-Z tail
     uint32_t max_draws;
+    uint32_t hiz_level_blocks[kHizMaxLevels];   // per level: 256-texel blocks of the largest view between tiles and tail
 };
 
 enum CullMode : uint32_t { CULL_SINGLE = 0, CULL_EARLY = 1, CULL_LATE = 2 };
@@ -236,7 +241,7 @@ void launch_frame_start(const BatchDev& b, const BatchGrid& g, hipStream_t st);
 void launch_cull(const SceneDev& s, const BatchDev& b, const BatchGrid& g, uint32_t flags, CullMode mode,
                  uint32_t list, hipStream_t st);
 void launch_raster(const SceneDev& s, const BatchDev& b, uint32_t flags, uint32_t list, uint32_t grid_blocks,
-                   uint32_t small_max_extent, hipStream_t st);
+                   uint32_t small_max_extent, uint32_t cache_slots, hipStream_t st);
 // Clipping, binning and tiled rasterization of the views' big-triangle queues.
 void launch_raster_big(const BatchDev& b, const BatchGrid& g, uint32_t tile_grid, uint32_t tile_item_tris,
                        hipStream_t st);
@@ -251,12 +256,14 @@ void launch_aa_image(const SceneDev& s, const unsigned long long* keys, uint32_t
 void launch_aa_backward(const SceneDev& s, const unsigned long long* keys, uint32_t* mat, uint32_t W, uint32_t H,
                         const float* vp, const float* g_image, double* g_transforms, hipStream_t st);
 uint32_t hiz_tile_levels(const PyramidDesc& pd);
+uint32_t hiz_tail_start(const PyramidDesc& pd, uint32_t tile_levels);
 
 // Tunables of the raster (set by the autotuner; defaults otherwise).
 struct RasterTuning {
     uint32_t small_max_extent;      // triangles whose bbox extent is below this (pixels) stay in the meshlet wave
     uint32_t raster_blocks_per_cu;  // persistent raster grid = CUs * this
     uint32_t tile_item_tris;        // triangles per tile-pass work item before a tile is split
+    uint32_t resolve_cache_slots;   // meshlets per view cached for the barycentric resolve
 };
 
 } // namespace vkgv

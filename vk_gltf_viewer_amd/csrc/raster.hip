@@ -78,6 +78,7 @@ struct RasterArgs {
     uint32_t clip_cap;
     Counters* ctr;
     uint32_t count;                 // meshlets in the list
+    uint32_t first;                 // the wave's first meshlet of the list (then every nwaves-th)
     uint32_t slot0;                 // first resolve-cache slot of this list
     TriCache* tri_cache;
     float4* slot_albedo;
@@ -236,7 +237,7 @@ __device__ __forceinline__ void raster_view(const RasterArgs& a)
     // The raster record of the wave's next meshlet is loaded while the
     // current one is processed.
     const uint4* list4 = reinterpret_cast<const uint4*>(a.list);
-    uint32_t idx = blockIdx.x * kWaves + (uint32_t)w;
+    uint32_t idx = a.first;
     uint4 nx0 = make_uint4(0u, 0u, 0u, 0u), nx1 = nx0;
     if (idx < count) { nx0 = list4[2u * idx]; nx1 = list4[2u * idx + 1u]; }
     while (idx < count) {
@@ -253,6 +254,7 @@ __device__ __forceinline__ void raster_view(const RasterArgs& a)
         const Mat4 mvp = mat4_mul(a.vp, m);
         const float tdet = det4(m);
         const bool cull_faces = !(matw >> 31) && !(a.flags & VKGV_NO_BACKFACE);
+        const bool bary = (a.flags & VKGV_RESOLVE_BARY) != 0;   // the flat resolve reads only the slot albedo
         const uint32_t vc = counts & 0xFFu, tc = (counts >> 8) & 0xFFu;
         // resolve-cache slot of this meshlet (none when the cache is full)
         const uint32_t slot = slot0 + idx;
@@ -298,7 +300,7 @@ __device__ __forceinline__ void raster_view(const RasterArgs& a)
                 const uint32_t i0 = tri_byte(s_tri[w], 3u * t), i1 = tri_byte(s_tri[w], 3u * t + 1u),
                                i2 = tri_byte(s_tri[w], 3u * t + 2u);
                 const float4 c0 = s_clip[w][i0], c1 = s_clip[w][i1], c2 = s_clip[w][i2];
-                if (cached) {
+                if (cached && bary) {
                     // r[k] = v_{k+1} x v_{k+2} over the clip (x, y, w) of the vertices
                     TriCache tcache;
                     const float4 cv[3] = { c0, c1, c2 };
@@ -900,15 +902,21 @@ struct RasterBatchArgs {
     uint32_t flags;
     uint32_t list;
     int small_lim;
+    uint32_t cache_slots;           // tuning: meshlets per view cached for the resolve
 };
 
 // The views of a batch are visited 64 at a time: lane l of each wave reads
 // the counter of view v0 + l, so that a view without work for the wave costs
-// one readlane instead of a dependent load.
+// one readlane instead of a dependent load.  The views' lists are dealt out
+// as one concatenated list: view j's meshlet i goes to wave
+// (i + meshlets of the views before j) mod waves, so every wave gets within
+// one meshlet of the average however the meshlets spread over the views.
 __global__ __launch_bounds__(kBlock) void raster_kernel(RasterBatchArgs ab)
 {
     const uint32_t lane = threadIdx.x & 63u;
     const uint32_t gw = blockIdx.x * kWaves + (threadIdx.x >> 6);
+    const uint32_t nwaves = gridDim.x * kWaves;
+    uint32_t carry = 0;   // meshlets of the views before v0, mod nwaves
     for (uint32_t v0 = 0; v0 < ab.b.nviews; v0 += 64u) {
         uint32_t cnt = 0, base = 0;
         if (v0 + lane < ab.b.nviews) {
@@ -916,10 +924,20 @@ __global__ __launch_bounds__(kBlock) void raster_kernel(RasterBatchArgs ab)
             cnt = c->vis_count[ab.list];
             base = ab.list ? c->vis_count[0] : 0u;
         }
+        // exclusive prefix of the counts over the 64 views, modulo nwaves
+        uint32_t incl = cnt % nwaves;
+#pragma unroll
+        for (uint32_t off = 1; off < 64u; off <<= 1) {
+            const uint32_t y = __shfl_up(incl, off);
+            if (lane >= off) incl = (incl + y) % nwaves;
+        }
+        const uint32_t excl = (incl + nwaves - cnt % nwaves) % nwaves;
         const uint32_t m = min(64u, ab.b.nviews - v0);
         for (uint32_t j = 0; j < m; ++j) {
             const uint32_t count = __builtin_amdgcn_readlane(cnt, j);
-            if (gw >= count) continue;
+            const uint32_t rot = (carry + __builtin_amdgcn_readlane(excl, j)) % nwaves;
+            const uint32_t first = (gw + nwaves - rot) % nwaves;
+            if (first >= count) continue;
             const uint32_t vi = v0 + j;
             const ViewDev& v = batch_view(ab.b, vi);
             RasterArgs a;
@@ -940,13 +958,15 @@ __global__ __launch_bounds__(kBlock) void raster_kernel(RasterBatchArgs ab)
             a.clip_cap = v.clip_capacity;
             a.ctr = v.counters;
             a.count = count;
+            a.first = first;
             a.slot0 = __builtin_amdgcn_readlane(base, j);
             a.tri_cache = v.tri_cache;
             a.slot_albedo = v.slot_albedo;
             a.draw_slot = v.draw_slot;
-            a.cache_slots = v.cache_slots;
+            a.cache_slots = min(v.cache_slots, ab.cache_slots);
             raster_view(a);
         }
+        carry = (carry + __builtin_amdgcn_readlane(incl, 63)) % nwaves;
     }
 }
 
@@ -1017,7 +1037,7 @@ __global__ __launch_bounds__(256) void tile_kernel(BatchDev b)
 } // namespace
 
 void launch_raster(const SceneDev& s, const BatchDev& b, uint32_t flags, uint32_t list, uint32_t grid_blocks,
-                   uint32_t small_max_extent, hipStream_t st)
+                   uint32_t small_max_extent, uint32_t cache_slots, hipStream_t st)
 {
     RasterBatchArgs a;
     a.s = s;
@@ -1025,6 +1045,7 @@ void launch_raster(const SceneDev& s, const BatchDev& b, uint32_t flags, uint32_
     a.flags = flags;
     a.list = list;
     a.small_lim = (int)(small_max_extent * 256u);
+    a.cache_slots = cache_slots;
     raster_kernel<<<grid_blocks, kBlock, 0, st>>>(a);
 }
 

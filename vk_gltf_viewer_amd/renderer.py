@@ -90,14 +90,16 @@ class Renderer:
     def sync(self):
         self._check(self._lib.vkgv_sync(self._h))
 
-    def set_tuning(self, small_max_extent: int, raster_blocks_per_cu: int, tile_item_tris: int = 16):
-        t = N.Tuning(small_max_extent, raster_blocks_per_cu, tile_item_tris)
+    def set_tuning(self, small_max_extent: int, raster_blocks_per_cu: int, tile_item_tris: int = 16,
+                   resolve_cache_slots: int = 8192):
+        t = N.Tuning(small_max_extent, raster_blocks_per_cu, tile_item_tris, resolve_cache_slots)
         self._check(self._lib.vkgv_set_tuning(self._h, C.byref(t)))
 
     def get_tuning(self):
+        """(small_max_extent, raster_blocks_per_cu, tile_item_tris, resolve_cache_slots)"""
         t = N.Tuning()
         self._check(self._lib.vkgv_get_tuning(self._h, C.byref(t)))
-        return t.small_max_extent, t.raster_blocks_per_cu, t.tile_item_tris
+        return t.small_max_extent, t.raster_blocks_per_cu, t.tile_item_tris, t.resolve_cache_slots
 
     def create_batch(self, views) -> "Batch":
         return Batch(self, views)
