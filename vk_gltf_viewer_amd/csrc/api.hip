@@ -67,6 +67,7 @@ struct vkgv_view {
     DevBuf keys, color, pyramid, visible0, visible1, history, counters, big, clipq, bin_count, bin_start, bins;
     DevBuf setups, pairs, tri_cache, slot_albedo, draw_slot, big_chunk, chunk_tri, tile_items;
     DevBuf aa_mat;                   // per-pixel material indices of the differentiable frame (allocated on first use)
+    DevBuf mvp_tab;                  // per-frame VP M table, kMvpStride floats per transform
     uint32_t list_capacity = 0;
     // Ring of per-frame event sets: timed frames are averaged at the next
     // read_stats without synchronizing between frames.
@@ -214,8 +215,23 @@ Mat4 to_mat4(const float* m)
     return r;
 }
 
+int ensure_mvp(vkgv_ctx* c, vkgv_view* v)
+{
+    if (v->dev.mvp_capacity >= c->transform_count && v->mvp_tab.p) return VKGV_OK;
+    const uint32_t cap = std::max<uint32_t>(c->transform_count, 64u);
+    HIP_TRY(c, hipDeviceSynchronize());   // frames of other streams may still read the old table
+    int rc;
+    if ((rc = dalloc(c, v->mvp_tab, sizeof(float) * kMvpStride * (size_t)cap))) return rc;
+    v->dev.mvp = static_cast<float*>(v->mvp_tab.p);
+    v->dev.mvp_capacity = cap;
+    c->generation++;
+    return VKGV_OK;
+}
+
 int ensure_lists(vkgv_ctx* c, vkgv_view* v)
 {
+    int rc0 = ensure_mvp(c, v);
+    if (rc0) return rc0;
     if (v->list_capacity >= c->draw_count && v->visible0.p) return VKGV_OK;
     const uint32_t cap = std::max<uint32_t>(c->draw_count, 64u);
     int rc;
@@ -314,7 +330,7 @@ void frame_launches(vkgv_ctx* c, const BatchDev& b, const BatchGrid& g, uint32_t
     const uint32_t raster_grid = (uint32_t)c->cus * c->tuning.raster_blocks_per_cu;
     const uint32_t tile_grid = (uint32_t)c->cus * 8u;
     mark(EV_START);
-    launch_frame_start(b, g, st);
+    launch_frame_start(s, b, g, st);
     mark(EV_CLEAR);
     launch_cull(s, b, g, flags, two_pass ? CULL_EARLY : CULL_SINGLE, 0, st);
     mark(EV_CULL);
@@ -456,6 +472,7 @@ int vkgv_upload_scene(vkgv_ctx* c, const vkgv_mesh_desc* meshes, uint32_t mesh_c
     if ((mesh_count && !meshes) || (prim_count && !prims) || !materials || material_count == 0)
         return fail(c, VKGV_ERROR_INVALID_ARGUMENT, "upload_scene: null array or no materials");
     HIP_TRY(c, hipSetDevice(c->device));
+    HIP_TRY(c, hipDeviceSynchronize());   // frames of any stream may still read the old buffers
     // ---- validation: everything a kernel dereferences stays in bounds
     for (uint32_t mi = 0; mi < mesh_count; ++mi) {
         const vkgv_mesh_desc& m = meshes[mi];
@@ -566,6 +583,7 @@ int vkgv_upload_scene(vkgv_ctx* c, const vkgv_mesh_desc* meshes, uint32_t mesh_c
     // a new scene invalidates the draw list (world.cpp:181-182)
     c->draws_set = false;
     c->draw_count = 0;
+    c->generation++;   // captured batch frames hold the old buffers
     return VKGV_OK;
 }
 
@@ -583,7 +601,7 @@ int vkgv_set_draws(vkgv_ctx* c, const vkgv_meshlet_draw* draws, uint32_t count)
         maxt = std::max(maxt, dr.transform_index);
     }
     HIP_TRY(c, hipSetDevice(c->device));
-    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    HIP_TRY(c, hipDeviceSynchronize());   // frames of any stream may still read the old buffers
     int rc;
     if ((rc = upload(c, c->draws, draws, sizeof(vkgv_meshlet_draw) * count))) return rc;
     if ((rc = dalloc(c, c->cull, sizeof(CullRec) * std::max<uint32_t>(count, 1u)))) return rc;
@@ -591,6 +609,7 @@ int vkgv_set_draws(vkgv_ctx* c, const vkgv_meshlet_draw* draws, uint32_t count)
     c->draw_count = count;
     c->max_transform = maxt;
     c->draws_set = true;
+    c->generation++;   // captured batch frames hold the old buffers and grids
     launch_build_cull_records(scene_dev(c), static_cast<CullRec*>(c->cull.p), static_cast<RasterRec*>(c->rrec.p),
                               c->stream);
     HIP_TRY(c, hipGetLastError());
@@ -605,12 +624,14 @@ int vkgv_set_transforms(vkgv_ctx* c, const float* mat4s, uint32_t count)
     HIP_TRY(c, hipSetDevice(c->device));
     const size_t bytes = sizeof(float) * 16 * (size_t)count;
     if (c->transforms.bytes < bytes || !c->transforms.p) {
-        HIP_TRY(c, hipStreamSynchronize(c->stream));
+        HIP_TRY(c, hipDeviceSynchronize());   // frames of any stream may still read the old buffer
         int rc = dalloc(c, c->transforms, bytes);
         if (rc) return rc;
+        c->generation++;
     }
     if (bytes) HIP_TRY(c, hipMemcpyAsync(c->transforms.p, mat4s, bytes, hipMemcpyHostToDevice, c->stream));
     HIP_TRY(c, hipStreamSynchronize(c->stream));
+    if (count != c->transform_count) c->generation++;   // captured batch frames hold the count
     c->transform_count = count;
     return VKGV_OK;
 }
@@ -724,7 +745,7 @@ void vkgv_view_destroy(vkgv_view* v)
     for (DevBuf* b : { &v->keys, &v->color, &v->pyramid, &v->visible0, &v->visible1, &v->history, &v->counters, &v->big,
                        &v->clipq, &v->bin_count, &v->bin_start, &v->bins, &v->setups, &v->pairs, &v->tri_cache,
                        &v->slot_albedo, &v->draw_slot, &v->big_chunk, &v->chunk_tri, &v->tile_items, &v->maps,
-                       &v->aa_mat })
+                       &v->aa_mat, &v->mvp_tab })
         dfree(*b);
     for (auto& set : v->ev)
         for (auto& e : set)

@@ -171,13 +171,32 @@ __global__ __launch_bounds__(kBlock) void cull_kernel(CullArgs a)
 
 // Frame start: each block clears kClearKeysPerBlock keys of its view (16 per
 // thread, 16-byte stores); the view's first block also zeroes its counters.
-__global__ __launch_bounds__(kBlock) void frame_start_kernel(BatchDev b)
+__global__ __launch_bounds__(kBlock) void frame_start_kernel(SceneDev s, BatchDev b)
 {
     const uint32_t e = b.map_clear[blockIdx.x];
     const uint32_t vi = e & 0xFFu, local = e >> 8;
     const ViewDev& v = batch_view(b, vi);
     if (local == 0 && threadIdx.x < sizeof(Counters) / 4) reinterpret_cast<uint32_t*>(v.counters)[threadIdx.x] = 0u;
     const uint64_t n = (uint64_t)v.width * v.height;
+    // this frame's VP M and det(M) of every transform, computed once per view
+    // and read by the raster and the resolve (the same arithmetic they used
+    // per meshlet and per pixel); the view's clear blocks share the entries
+    {
+        const uint32_t nblk = (uint32_t)((n + kClearKeysPerBlock - 1) / kClearKeysPerBlock);
+        const uint32_t count = min(s.transform_count, v.mvp_capacity);
+        if ((uint64_t)local * kBlock < count) {
+            const Mat4 vp = load_mat4(batch_camera(b, vi).cam.view_projection);
+            for (uint32_t t = local * kBlock + threadIdx.x; t < count; t += nblk * kBlock) {
+                const Mat4 m = load_mat4(s.transforms + 16ull * t);
+                const Mat4 mvp = mat4_mul(vp, m);
+                float4* o = reinterpret_cast<float4*>(v.mvp + (size_t)t * kMvpStride);
+#pragma unroll
+                for (int c = 0; c < 4; ++c)
+                    o[c] = make_float4(mvp.m[4 * c], mvp.m[4 * c + 1], mvp.m[4 * c + 2], mvp.m[4 * c + 3]);
+                o[4] = make_float4(det4(m), 0.0f, 0.0f, 0.0f);
+            }
+        }
+    }
     const uint64_t first = (uint64_t)local * kClearKeysPerBlock;
     const ulonglong2 c2 = make_ulonglong2(VKGV_KEY_CLEAR, VKGV_KEY_CLEAR);
     ulonglong2* k2 = reinterpret_cast<ulonglong2*>(v.keys + first);
@@ -249,9 +268,9 @@ void launch_clear_keys(unsigned long long* keys, uint64_t n, hipStream_t st)
                                                            (uint32_t)(n & 1));
 }
 
-void launch_frame_start(const BatchDev& b, const BatchGrid& g, hipStream_t st)
+void launch_frame_start(const SceneDev& s, const BatchDev& b, const BatchGrid& g, hipStream_t st)
 {
-    frame_start_kernel<<<g.clear_blocks, kBlock, 0, st>>>(b);
+    frame_start_kernel<<<g.clear_blocks, kBlock, 0, st>>>(s, b);
 }
 
 void launch_cull(const SceneDev& s, const BatchDev& b, const BatchGrid& g, uint32_t flags, CullMode mode,

@@ -179,6 +179,8 @@ struct ViewDev {
     float4* slot_albedo;            // albedo factor of each slot's material
     uint32_t* draw_slot;
     uint32_t cache_slots;
+    float* mvp;                     // per transform, this frame: VP M (16 floats, column-major), det(M) at [16]
+    uint32_t mvp_capacity;          // transforms the table holds
     uint32_t hiz_tile_levels;       // pyramid levels produced by the tile pass of the Hi-Z build
     uint32_t hiz_tail_start;        // first level of the one-block tail; levels in between: hiz_level_kernel
     uint32_t hiz_tiles_x;           // 32x32 level-0 tiles per row
@@ -220,6 +222,7 @@ __device__ __forceinline__ const ViewDev& batch_view(const BatchDev& b, uint32_t
 }
 
 // Host-side sizes of a batch's launches.
+constexpr uint32_t kMvpStride = 20;          // floats per transform in a view's VP M table (80-byte rows)
 constexpr uint32_t kHizMaxLevels = 16;
 constexpr uint32_t kHizTailTexels = 8192;   // largest first tail level (the tail keeps it in LDS)
 
@@ -236,8 +239,9 @@ enum CullMode : uint32_t { CULL_SINGLE = 0, CULL_EARLY = 1, CULL_LATE = 2 };
 // ---- launchers (stream-ordered, no host synchronization) -----------------
 void launch_build_cull_records(const SceneDev& s, CullRec* out, RasterRec* rrec, hipStream_t st);
 void launch_clear_keys(unsigned long long* keys, uint64_t n, hipStream_t st);
-// Frame start of every view of the batch: all counters zeroed, keys cleared.
-void launch_frame_start(const BatchDev& b, const BatchGrid& g, hipStream_t st);
+// Frame start of every view of the batch: all counters zeroed, keys cleared,
+// the view's table of VP M and det(M) per transform computed.
+void launch_frame_start(const SceneDev& s, const BatchDev& b, const BatchGrid& g, hipStream_t st);
 void launch_cull(const SceneDev& s, const BatchDev& b, const BatchGrid& g, uint32_t flags, CullMode mode,
                  uint32_t list, hipStream_t st);
 void launch_raster(const SceneDev& s, const BatchDev& b, uint32_t flags, uint32_t list, uint32_t grid_blocks,

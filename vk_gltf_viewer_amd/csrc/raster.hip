@@ -63,7 +63,6 @@ struct SmallTri {
 
 struct RasterArgs {
     SceneDev s;
-    Mat4 vp;
     const RasterRec* list;
     unsigned long long* keys;
     uint32_t W, H;
@@ -77,6 +76,7 @@ struct RasterArgs {
     ClipTri* clipq;
     uint32_t clip_cap;
     Counters* ctr;
+    const float* mvp;               // the view's VP M table (kMvpStride floats per transform)
     uint32_t count;                 // meshlets in the list
     uint32_t first;                 // the wave's first meshlet of the list (then every nwaves-th)
     uint32_t slot0;                 // first resolve-cache slot of this list
@@ -250,9 +250,10 @@ __device__ __forceinline__ void raster_view(const RasterArgs& a)
         const uint32_t counts = __builtin_amdgcn_readfirstlane(nx1.z);
         const uint32_t nidx = idx + nwaves;
         if (nidx < count) { nx0 = list4[2u * nidx]; nx1 = list4[2u * nidx + 1u]; }
-        const Mat4 m = load_mat4(a.s.transforms + 16ull * tr);
-        const Mat4 mvp = mat4_mul(a.vp, m);
-        const float tdet = det4(m);
+        // VP M and det(M) of the transform, computed once per frame at frame start
+        const float* mrow = a.mvp + (size_t)tr * kMvpStride;
+        const Mat4 mvp = load_mat4(mrow);
+        const float tdet = mrow[16];
         const bool cull_faces = !(matw >> 31) && !(a.flags & VKGV_NO_BACKFACE);
         const bool bary = (a.flags & VKGV_RESOLVE_BARY) != 0;   // the flat resolve reads only the slot albedo
         const uint32_t vc = counts & 0xFFu, tc = (counts >> 8) & 0xFFu;
@@ -942,7 +943,6 @@ __global__ __launch_bounds__(kBlock) void raster_kernel(RasterBatchArgs ab)
             const ViewDev& v = batch_view(ab.b, vi);
             RasterArgs a;
             a.s = ab.s;
-            a.vp = load_mat4(batch_camera(ab.b, vi).cam.view_projection);
             a.list = v.visible[ab.list];
             a.keys = v.keys;
             a.W = v.width;
@@ -957,6 +957,7 @@ __global__ __launch_bounds__(kBlock) void raster_kernel(RasterBatchArgs ab)
             a.clipq = v.clipq;
             a.clip_cap = v.clip_capacity;
             a.ctr = v.counters;
+            a.mvp = v.mvp;
             a.count = count;
             a.first = first;
             a.slot0 = __builtin_amdgcn_readlane(base, j);

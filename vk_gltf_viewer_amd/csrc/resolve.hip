@@ -38,7 +38,6 @@ __device__ __forceinline__ uint32_t srgb_code(const float* thr, const uint8_t* b
 
 struct ResolveArgs {
     SceneDev s;
-    Mat4 vp;
     const unsigned long long* keys;
     uint32_t* color;
     uint32_t W, H;
@@ -46,6 +45,7 @@ struct ResolveArgs {
     const uint32_t* draw_slot;
     const TriCache* tri_cache;
     const float4* slot_albedo;
+    const float* mvp;               // the view's VP M table (kMvpStride floats per transform)
 };
 
 __device__ __forceinline__ float3 cross3(float3 a, float3 b)
@@ -65,7 +65,7 @@ __device__ __forceinline__ void fetch_uncached(const ResolveArgs& a, uint32_t d,
     c[0] = mat.albedo_factor[0]; c[1] = mat.albedo_factor[1]; c[2] = mat.albedo_factor[2]; c[3] = mat.albedo_factor[3];
     if (!bary) return;
     const vkgv_meshlet ml = reinterpret_cast<const vkgv_meshlet*>(prim.meshlet_buffer)[dr.meshlet_index];
-    const Mat4 mvp = mat4_mul(a.vp, load_mat4(a.s.transforms + 16ull * dr.transform_index));
+    const Mat4 mvp = load_mat4(a.mvp + (size_t)dr.transform_index * kMvpStride);   // VP M of this frame
     const uint8_t* tri = reinterpret_cast<const uint8_t*>(prim.primitive_index_buffer) + ml.triangle_offset + 3u * t;
     const uint32_t* vidx = reinterpret_cast<const uint32_t*>(prim.vertex_index_buffer) + ml.vertex_offset;
     const vkgv_vertex* verts = reinterpret_cast<const vkgv_vertex*>(prim.vertex_buffer);
@@ -100,7 +100,6 @@ __global__ __launch_bounds__(256) void resolve_kernel(ResolveBatchArgs ab)
     const uint32_t local = me >> 8;
     ResolveArgs a;
     a.s = ab.s;
-    a.vp = load_mat4(batch_camera(ab.b, me & 0xFFu).cam.view_projection);
     a.keys = view.keys;
     a.color = view.color;
     a.W = view.width;
@@ -109,6 +108,7 @@ __global__ __launch_bounds__(256) void resolve_kernel(ResolveBatchArgs ab)
     a.draw_slot = view.draw_slot;
     a.tri_cache = view.tri_cache;
     a.slot_albedo = view.slot_albedo;
+    a.mvp = view.mvp;
     const uint32_t blk_x = local % view.resolve_blocks_x, blk_y = local / view.resolve_blocks_x;
 
     // LDS tables: sRGB thresholds and buckets, exact quotients b / 255 of a byte
