@@ -55,7 +55,7 @@ def visible_triangles(sc, draw_ids):
 # benchmark, each scaled by the same counter's reading for a plain 256 MiB
 # copy kernel (profiles/traffic_c2_4k.json records the commands and values).
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "traffic_c2_4k.json")
-PASS_KERNELS = {"clear": ["clear_keys_kernel"], "hiz": ["hiz_tiles_kernel", "hiz_tail_kernel"],
+PASS_KERNELS = {"clear": ["frame_start_kernel"], "hiz": ["hiz_tiles_kernel", "hiz_level_kernel", "hiz_tail_kernel"],
                 "resolve": ["resolve_kernel"]}
 
 
@@ -71,7 +71,7 @@ def measured_traffic(pass_name, W, H):
     for c in ("FETCH_SIZE", "WRITE_SIZE"):
         rec = t[c]
         for k in PASS_KERNELS[pass_name]:
-            total += rec["per_kernel_kib"][k] * rec["factor"] * 1024.0
+            total += rec["per_kernel_kib"].get(k, 0.0) * rec["factor"] * 1024.0
     return round(total)
 
 
