@@ -9,7 +9,8 @@ every timed frame is a steady-state two-pass frame.
 
 A step is one frame.  W warm-up frames, then K timed frames bracketed by a
 stream synchronization on both sides.  Per-pass durations come from HIP
-events recorded on the frame's stream inside the timed region.
+events recorded on the frame's stream in K further frames after the timed
+region.
 """
 from __future__ import annotations
 
@@ -337,10 +338,15 @@ def main():
 
     t0 = time.perf_counter()
     for _ in range(args.steps):
-        view.render(cam, flags | N.TIMING)
+        view.render(cam, flags)
     r.sync()
     elapsed = time.perf_counter() - t0
     ms_frame = 1000.0 * elapsed / args.steps
+    # per-pass durations from HIP events, in frames of their own after the timed
+    # region (the event records would add their own cost to the timed frames)
+    for _ in range(args.steps):
+        view.render(cam, flags | N.TIMING)
+    r.sync()
     st = view.stats()
     if st.overflow:
         print("bench.py: WARNING big-triangle queue overflow", file=sys.stderr)
