@@ -1,0 +1,262 @@
+"""glTF ingestion on the CPU: accessor decoding (KHR_mesh_quantization
+normalization, strides, sparse), generated indices, materials with the
+default at index 0, node TRS and decomposed matrices through the World
+traversal, .gltf / .glb equivalence and the rejected forms."""
+import json
+
+import numpy as np
+import pytest
+
+from gltfgen import Builder, instanced_spheres, rotation_quat, trs_matrix, uv_sphere
+from vk_gltf_viewer_amd import gltf
+from vk_gltf_viewer_amd.scene import Scene
+
+
+def mesh_triangles(m):
+    """Triangles of a loaded mesh, rebuilt from its meshlets, as sorted tuples
+    of vertex indices (rotation-invariant)."""
+    out = []
+    for ml in m.meshlets:
+        vo, to, tc = int(ml["vertex_offset"]), int(ml["triangle_offset"]), int(ml["triangle_count"])
+        loc = m.triangle_indices[to:to + 3 * tc].reshape(-1, 3).astype(np.int64)
+        out.append(m.vertex_indices[vo + loc])
+    t = np.concatenate(out) if out else np.zeros((0, 3), np.uint32)
+    return t
+
+
+def canon(tris):
+    """Each triangle rotated so its smallest index comes first (winding kept)."""
+    t = np.asarray(tris, np.int64).reshape(-1, 3)
+    k = np.argmin(t, 1)
+    r = np.stack([t[np.arange(len(t)), (k + i) % 3] for i in range(3)], 1)
+    return sorted(map(tuple, r))
+
+
+def one_triangle_builder(pos, idx=None, **acc):
+    b = Builder()
+    pa = b.accessor(pos, minmax=True, **acc)
+    prim = {"attributes": {"POSITION": pa}}
+    if idx is not None:
+        prim["indices"] = b.accessor(np.asarray(idx).reshape(-1, 1))
+    b.mesh([prim])
+    b.scene([b.node(mesh=0)])
+    return b
+
+
+@pytest.mark.parametrize("dtype,scale,signed", [(np.int8, 127, True), (np.uint8, 255, False),
+                                               (np.int16, 32767, True), (np.uint16, 65535, False)])
+def test_quantized_positions_normalized(tmp_path, dtype, scale, signed):
+    """KHR_mesh_quantization: normalized integers map to c / max, signed
+    ones clamped at -1 (the most negative code)."""
+    info = np.iinfo(dtype)
+    q = np.array([[info.min, 0, info.max], [info.max, info.min, 1], [1, 2, 3]], dtype)
+    b = one_triangle_builder(q, normalized=True)
+    p = tmp_path / "q.gltf"
+    b.write_gltf(p)
+    sc = gltf.load_gltf(str(p))
+    got = sc.meshes[0].vertices["position"]
+    want = q.astype(np.float64) / scale
+    if signed:
+        want = np.maximum(want, -1.0)
+    np.testing.assert_array_equal(got, want.astype(np.float32))
+    # the primitive AABB from the accessor min / max, normalized the same way
+    lo, hi = want.min(0), want.max(0)
+    np.testing.assert_allclose(sc.primitives[0]["aabb_center"], (lo + hi) / 2, rtol=0, atol=1e-7)
+    np.testing.assert_allclose(sc.primitives[0]["aabb_extents"], (hi - lo) / 2, rtol=0, atol=1e-7)
+
+
+def test_unnormalized_integer_positions(tmp_path):
+    q = np.array([[-3, 0, 7], [5, -2, 1], [0, 9, -9]], np.int16)
+    b = one_triangle_builder(q)
+    p = tmp_path / "u.gltf"
+    b.write_gltf(p)
+    got = gltf.load_gltf(str(p)).meshes[0].vertices["position"]
+    np.testing.assert_array_equal(got, q.astype(np.float32))
+
+
+def test_generated_indices_and_meshlets_cover_triangles(tmp_path):
+    pos, tris = uv_sphere(12, 16)
+    soup = pos[tris.ravel()]                       # non-indexed triangle list
+    b = one_triangle_builder(soup)
+    p = tmp_path / "soup.glb"
+    b.write_glb(p)
+    sc = gltf.load_gltf(str(p))
+    m = sc.meshes[0]
+    assert canon(mesh_triangles(m)) == canon(np.arange(len(soup)).reshape(-1, 3))
+    assert m.meshlets["vertex_count"].max() <= 64 and m.meshlets["triangle_count"].max() <= 124
+
+
+def test_indexed_meshlets_cover_triangles(tmp_path):
+    pos, tris = uv_sphere(30, 40)
+    for idt in (np.uint16, np.uint32, np.uint8):
+        if idt == np.uint8 and len(pos) > 255:
+            continue
+        b = one_triangle_builder(pos, tris.astype(idt).ravel())
+        p = tmp_path / f"i{np.dtype(idt).itemsize}.gltf"
+        b.write_gltf(p, external=True)
+        m = gltf.load_gltf(str(p)).meshes[0]
+        assert canon(mesh_triangles(m)) == canon(tris)
+        np.testing.assert_array_equal(m.vertices["position"], pos)
+
+
+def test_interleaved_stride_and_sparse(tmp_path):
+    """A position accessor inside an interleaved buffer view (byteStride 24)
+    with a sparse override of two vertices."""
+    pos = np.array([[0, 0, 0], [1, 0, 0], [0, 1, 0], [1, 1, 0]], np.float32)
+    other = np.full((4, 3), 7.0, np.float32)
+    inter = np.concatenate([pos, other], 1).astype(np.float32)
+    b = Builder()
+    v = b.view(inter.tobytes(), stride=24)
+    pa = b.accessor(pos, view=v)
+    sp_idx = b.view(np.array([1, 3], np.uint16).tobytes())
+    sp_val = b.view(np.array([[5, 5, 5], [6, 6, 6]], np.float32).tobytes())
+    b.js["accessors"][pa]["sparse"] = {"count": 2, "indices": {"bufferView": sp_idx, "componentType": 5123},
+                                       "values": {"bufferView": sp_val}}
+    ia = b.accessor(np.array([0, 1, 2, 2, 1, 3], np.uint16).reshape(-1, 1))
+    b.mesh([{"attributes": {"POSITION": pa}, "indices": ia}])
+    b.scene([b.node(mesh=0)])
+    p = tmp_path / "s.gltf"
+    b.write_gltf(p)
+    got = gltf.load_gltf(str(p)).meshes[0].vertices["position"]
+    want = pos.copy()
+    want[1], want[3] = 5, 6
+    np.testing.assert_array_equal(got, want)
+
+
+def test_materials_default_at_zero(tmp_path):
+    """assets.cpp:478-518: material 0 is the default; glTF material i is i+1;
+    a primitive without a material draws with material 0."""
+    pos, tris = uv_sphere(4, 6)
+    b = Builder()
+    pa = b.accessor(pos, minmax=True)
+    ia = b.accessor(tris.astype(np.uint16).reshape(-1, 1))
+    m0 = b.material([0.1, 0.2, 0.3, 0.4], cutoff=0.25, double_sided=True)
+    m1 = b.material([0.5, 0.6, 0.7, 1.0],
+                    extra={"baseColorTexture": {"index": 0, "extensions": {"KHR_texture_transform": {
+                        "offset": [0.5, 0.25], "scale": [2.0, 3.0], "rotation": 0.5}}}})
+    b.mesh([{"attributes": {"POSITION": pa}, "indices": ia, "material": m1},
+            {"attributes": {"POSITION": pa}, "indices": ia},
+            {"attributes": {"POSITION": pa}, "indices": ia, "material": m0}])
+    b.scene([b.node(mesh=0)])
+    p = tmp_path / "m.gltf"
+    b.write_gltf(p)
+    sc = gltf.load_gltf(str(p))
+    mats = sc.materials
+    assert len(mats) == 3
+    np.testing.assert_array_equal(mats[0]["albedo_factor"], [1, 1, 1, 1])
+    assert mats[0]["alpha_cutoff"] == np.float32(0.5) and mats[0]["double_sided"] == 0
+    assert mats[0]["albedo_index"] == gltf.INVALID_HANDLE
+    np.testing.assert_array_equal(mats[1]["albedo_factor"], np.float32([0.1, 0.2, 0.3, 0.4]))
+    assert mats[1]["alpha_cutoff"] == np.float32(0.25) and mats[1]["double_sided"] == 1
+    np.testing.assert_array_equal(mats[2]["uv_offset"], [0.5, 0.25])
+    np.testing.assert_array_equal(mats[2]["uv_scale"], [2.0, 3.0])
+    assert mats[2]["uv_rotation"] == np.float32(0.5)
+    assert list(sc.primitives["material_index"]) == [2, 0, 1]
+    # one draw per (primitive, meshlet), primitives in order
+    assert list(np.unique(sc.draws["primitive_index"])) == [0, 1, 2]
+    assert len(sc.draws) == sum(len(m.meshlets) for m in sc.meshes)
+
+
+def test_decompose_matches_matrix():
+    rng = np.random.default_rng(3)
+    for k in range(50):
+        t = rng.normal(size=3)
+        q = rotation_quat(rng.normal(size=3), rng.uniform(-np.pi, np.pi))
+        s = rng.uniform(0.1, 3.0, 3) * (-1 if k % 5 == 0 else 1) ** np.array([1, 0, 0])
+        M = trs_matrix(t, q, s)
+        t2, q2, s2 = gltf.decompose(M.T.ravel())
+        np.testing.assert_allclose(trs_matrix(t2, q2, s2), M, atol=1e-12)
+
+
+def test_node_hierarchy_world_transforms(tmp_path):
+    """World traversal of a matrix root over TRS children (world.cpp:187-228):
+    transform = parent * T * R * S, depth-first order."""
+    b = instanced_spheres(grid=3, seed=5)
+    p = tmp_path / "h.glb"
+    b.write_glb(p)
+    sc = gltf.load_gltf(str(p))
+    js = b.js
+    root = js["nodes"][-1]
+    R = np.asarray(root["matrix"], np.float64).reshape(4, 4).T
+    kids = root["children"]
+    assert sc.transforms.shape == (len(kids), 16)
+    for k, c in enumerate(kids):
+        n = js["nodes"][c]
+        want = R @ trs_matrix(n["translation"], n["rotation"], n["scale"])
+        np.testing.assert_allclose(sc.transforms[k].reshape(4, 4).T, want, rtol=0, atol=2e-6)
+    # each draw's transform is its node's, in visit order
+    first = np.r_[0, np.nonzero(np.diff(sc.draws["transform_index"].astype(np.int64)))[0] + 1]
+    assert list(sc.draws["transform_index"][first]) == list(range(len(kids)))
+
+
+def test_gltf_and_glb_load_identically(tmp_path):
+    b = instanced_spheres(grid=4, seed=9)
+    b.write_glb(tmp_path / "a.glb")
+    b.write_gltf(tmp_path / "a.gltf")
+    b.write_gltf(tmp_path / "b.gltf", external=True)
+    scs = [gltf.load_gltf(str(tmp_path / n)) for n in ("a.glb", "a.gltf", "b.gltf")]
+    for s in scs[1:]:
+        for k in ("primitives", "materials", "draws", "transforms"):
+            assert np.array_equal(getattr(s, k), getattr(scs[0], k)), k
+        for ma, mb in zip(s.meshes, scs[0].meshes):
+            for k in ("vertex_indices", "triangle_indices", "vertices", "meshlets", "cones"):
+                assert np.array_equal(getattr(ma, k), getattr(mb, k)), k
+
+
+def test_vertex_colors(tmp_path):
+    pos = np.array([[0, 0, 0], [1, 0, 0], [0, 1, 0]], np.float32)
+    b = Builder()
+    pa = b.accessor(pos)
+    ca = b.accessor(np.array([[255, 0, 0], [0, 128, 0], [0, 0, 255]], np.uint8), normalized=True)
+    b.mesh([{"attributes": {"POSITION": pa, "COLOR_0": ca}}])
+    b.scene([b.node(mesh=0)])
+    p = tmp_path / "c.gltf"
+    b.write_gltf(p)
+    col = gltf.load_gltf(str(p)).meshes[0].vertices["color"]
+    np.testing.assert_array_equal(col, [[255, 0, 0, 255], [0, 128, 0, 255], [0, 0, 255, 255]])
+
+
+def test_rejected_forms(tmp_path):
+    pos = np.array([[0, 0, 0], [1, 0, 0], [0, 1, 0]], np.float32)
+    b = one_triangle_builder(pos)
+    b.js["extensionsRequired"] = ["EXT_meshopt_compression"]
+    b.write_gltf(tmp_path / "mo.gltf")
+    with pytest.raises(NotImplementedError, match="EXT_meshopt_compression"):
+        gltf.load_gltf(str(tmp_path / "mo.gltf"))
+    b = one_triangle_builder(pos)
+    b.js["meshes"][0]["primitives"][0]["mode"] = 1
+    b.write_gltf(tmp_path / "lines.gltf")
+    with pytest.raises(ValueError, match="triangle"):
+        gltf.load_gltf(str(tmp_path / "lines.gltf"))
+    b = one_triangle_builder(pos, np.array([0, 1, 5], np.uint16))
+    b.write_gltf(tmp_path / "oob.gltf")
+    with pytest.raises(ValueError, match="out of range"):
+        gltf.load_gltf(str(tmp_path / "oob.gltf"))
+    b = one_triangle_builder(pos)
+    b.js["extensionsRequired"] = ["KHR_draco_mesh_compression"]
+    b.write_gltf(tmp_path / "d.gltf")
+    with pytest.raises(ValueError, match="not supported"):
+        gltf.load_gltf(str(tmp_path / "d.gltf"))
+    with open(tmp_path / "bad.glb", "wb") as f:
+        f.write(b"glTF" + (1).to_bytes(4, "little") + (12).to_bytes(4, "little"))
+    with pytest.raises(ValueError, match="2.0"):
+        gltf.load_gltf(str(tmp_path / "bad.glb"))
+
+
+def test_renders_like_procedural_scene_arrays(tmp_path):
+    """A glTF written from a procedural scene's own mesh and transforms loads
+    back to the same triangles and world transforms."""
+    sc = Scene(4, 4).arrays()
+    m = sc.meshes[0]
+    pos = m.vertices["position"].astype(np.float32)
+    tris = mesh_triangles(m)
+    b = Builder()
+    pa = b.accessor(pos, minmax=True)
+    ia = b.accessor(tris.astype(np.uint32).reshape(-1, 1))
+    b.mesh([{"attributes": {"POSITION": pa}, "indices": ia}])
+    M = sc.transforms[0].reshape(4, 4).T.astype(np.float64)
+    b.scene([b.node(mesh=0, matrix=M.T.ravel().tolist())])
+    b.write_glb(tmp_path / "p.glb")
+    g = gltf.load_gltf(str(tmp_path / "p.glb"))
+    assert canon(mesh_triangles(g.meshes[0])) == canon(tris)
+    np.testing.assert_allclose(g.transforms[0], sc.transforms[0], rtol=0, atol=1e-6)

@@ -1,0 +1,320 @@
+"""glTF 2.0 scene ingestion (.gltf with external or data-URI buffers, .glb).
+
+Follows the reference's asset loading (assets.cpp):
+- positions through their accessor, with KHR_mesh_quantization normalization
+  for normalized integer components (int8: max(c / 127, -1), uint8: c / 255,
+  int16: max(c / 32767, -1), uint16: c / 65535) and integer values taken as
+  they are when not normalized (fastgltf's iterateAccessor, assets.cpp:310-314);
+- indices copied to u32, generated as 0..n-1 when a primitive has none
+  (fastgltf::Options::GenerateMeshIndices, assets.cpp:533-534);
+- the primitive AABB from the position accessor's min / max (assets.cpp:299-304);
+- meshlets at the reference's limits (64 vertices, 124 triangles) by the
+  host builder (assets.cpp:322-373);
+- materials: index 0 is the default (albedo 1, alpha cutoff 0.5, single
+  sided), glTF material i is i + 1 with its baseColorFactor, alphaCutoff and
+  doubleSided (assets.cpp:478-518); a primitive's material index is its glTF
+  index + 1, 0 without one (assets.cpp:292-294); the albedo texture handle is
+  the invalid handle, as in the reference, and a KHR_texture_transform of the
+  base-colour texture is kept in the uv fields;
+- node matrices decomposed to translation / rotation / scale
+  (fastgltf::Options::DecomposeNodeMatrices), then the World traversal
+  (world.cpp:187-293, the host library's vkgv_world_build) of the default
+  scene's roots gives the draws and transforms.
+Beyond the reference, COLOR_0 (as unorm8 RGBA) is loaded into the vertex
+colour that the barycentric resolve interpolates; without it the colour is
+white.
+
+Not supported: EXT_meshopt_compression buffer views (raises
+NotImplementedError), primitive modes other than triangles (ValueError).
+"""
+from __future__ import annotations
+
+import base64
+import ctypes as C
+import json
+import math
+import os
+import struct
+
+import numpy as np
+
+from . import _native as N
+from .scene import MATERIAL_DTYPE, PRIM_DTYPE, DRAW_DTYPE, SceneArrays, build_meshlets
+
+_COMPONENT = {5120: np.int8, 5121: np.uint8, 5122: np.int16, 5123: np.uint16, 5125: np.uint32, 5126: np.float32}
+_COUNT = {"SCALAR": 1, "VEC2": 2, "VEC3": 3, "VEC4": 4, "MAT2": 4, "MAT3": 9, "MAT4": 16}
+_SUPPORTED_REQUIRED = {"KHR_mesh_quantization", "KHR_texture_transform", "KHR_materials_emissive_strength",
+                       "KHR_lights_punctual", "KHR_texture_basisu", "MSFT_texture_dds"}
+INVALID_HANDLE = 0xFFFFFFFF
+
+
+class GltfError(ValueError):
+    pass
+
+
+def _read_glb(data: bytes):
+    magic, version, length = struct.unpack_from("<4sII", data, 0)
+    if magic != b"glTF" or version != 2:
+        raise GltfError("not a glTF 2.0 binary")
+    off, js, binc = 12, None, None
+    while off < min(length, len(data)):
+        clen, ctype = struct.unpack_from("<I4s", data, off)
+        chunk = data[off + 8:off + 8 + clen]
+        if ctype == b"JSON":
+            js = json.loads(chunk.decode("utf-8"))
+        elif ctype == b"BIN\x00" and binc is None:
+            binc = chunk
+        off += 8 + clen
+    if js is None:
+        raise GltfError("glTF binary without a JSON chunk")
+    return js, binc
+
+
+class _Asset:
+    def __init__(self, path: str):
+        self.dir = os.path.dirname(os.path.abspath(path))
+        with open(path, "rb") as f:
+            data = f.read()
+        if data[:4] == b"glTF":
+            self.js, self.glb_bin = _read_glb(data)
+        else:
+            self.js, self.glb_bin = json.loads(data.decode("utf-8")), None
+        if self.js.get("asset", {}).get("version", "2.0").split(".")[0] != "2":
+            raise GltfError("only glTF 2.x is supported")
+        required = set(self.js.get("extensionsRequired", []))
+        if "EXT_meshopt_compression" in required:
+            raise NotImplementedError("EXT_meshopt_compression buffers are not supported by this loader")
+        unknown = required - _SUPPORTED_REQUIRED
+        if unknown:
+            raise GltfError(f"required extensions not supported: {sorted(unknown)}")
+        self._buffers = {}
+
+    def buffer(self, i: int) -> bytes:
+        if i not in self._buffers:
+            b = self.js["buffers"][i]
+            uri = b.get("uri")
+            if uri is None:
+                if self.glb_bin is None:
+                    raise GltfError(f"buffer {i} has no uri and there is no GLB binary chunk")
+                data = self.glb_bin
+            elif uri.startswith("data:"):
+                data = base64.b64decode(uri.split(",", 1)[1])
+            else:
+                from urllib.parse import unquote
+                with open(os.path.join(self.dir, unquote(uri)), "rb") as f:
+                    data = f.read()
+            if len(data) < b["byteLength"]:
+                raise GltfError(f"buffer {i} is shorter than its byteLength")
+            self._buffers[i] = data
+        return self._buffers[i]
+
+    def view_bytes(self, vi: int):
+        v = self.js["bufferViews"][vi]
+        if "EXT_meshopt_compression" in v.get("extensions", {}):
+            raise NotImplementedError("EXT_meshopt_compression buffer views are not supported by this loader")
+        data = self.buffer(v["buffer"])
+        off = v.get("byteOffset", 0)
+        return data[off:off + v["byteLength"]], v.get("byteStride")
+
+    def accessor(self, ai: int, as_float: bool):
+        """(count, components) array; with as_float, normalized integers are
+        mapped as KHR_mesh_quantization / the core spec define."""
+        a = self.js["accessors"][ai]
+        dt = np.dtype(_COMPONENT[a["componentType"]]).newbyteorder("<")
+        nc = _COUNT[a["type"]]
+        count = a["count"]
+        if "bufferView" in a:
+            raw, stride = self.view_bytes(a["bufferView"])
+            off = a.get("byteOffset", 0)
+            elem = dt.itemsize * nc
+            stride = stride or elem
+            need = off + stride * (count - 1) + elem if count else 0
+            if need > len(raw):
+                raise GltfError(f"accessor {ai} reads past its buffer view")
+            arr = np.ndarray((count, nc), dtype=dt, buffer=raw, offset=off, strides=(stride, dt.itemsize)).copy()
+        else:
+            arr = np.zeros((count, nc), dtype=dt)
+        sp = a.get("sparse")
+        if sp:
+            idt = np.dtype(_COMPONENT[sp["indices"]["componentType"]]).newbyteorder("<")
+            iraw, _ = self.view_bytes(sp["indices"]["bufferView"])
+            idx = np.frombuffer(iraw, dtype=idt, count=sp["count"], offset=sp["indices"].get("byteOffset", 0))
+            vraw, _ = self.view_bytes(sp["values"]["bufferView"])
+            vals = np.frombuffer(vraw, dtype=dt, count=sp["count"] * nc,
+                                 offset=sp["values"].get("byteOffset", 0)).reshape(-1, nc)
+            if len(idx) and int(idx.max()) >= count:
+                raise GltfError(f"accessor {ai}: sparse index out of range")
+            arr[idx.astype(np.int64)] = vals
+        if not as_float:
+            return arr
+        if a.get("normalized", False) and dt.kind in "iu":
+            if dt == np.int8:
+                return np.maximum(arr.astype(np.float32) / 127.0, -1.0).astype(np.float32)
+            if dt == np.uint8:
+                return (arr.astype(np.float32) / 255.0).astype(np.float32)
+            if dt == np.int16:
+                return np.maximum(arr.astype(np.float32) / 32767.0, -1.0).astype(np.float32)
+            if dt == np.uint16:
+                return (arr.astype(np.float32) / 65535.0).astype(np.float32)
+            raise GltfError(f"accessor {ai}: normalized {dt} is not allowed")
+        return arr.astype(np.float32)
+
+
+def _quat_from_matrix(R):
+    """Unit quaternion (x, y, z, w) of a rotation matrix (Shepperd's method)."""
+    t = R[0, 0] + R[1, 1] + R[2, 2]
+    if t > 0:
+        s = math.sqrt(t + 1.0) * 2
+        w, x, y, z = 0.25 * s, (R[2, 1] - R[1, 2]) / s, (R[0, 2] - R[2, 0]) / s, (R[1, 0] - R[0, 1]) / s
+    elif R[0, 0] > R[1, 1] and R[0, 0] > R[2, 2]:
+        s = math.sqrt(1.0 + R[0, 0] - R[1, 1] - R[2, 2]) * 2
+        w, x, y, z = (R[2, 1] - R[1, 2]) / s, 0.25 * s, (R[0, 1] + R[1, 0]) / s, (R[0, 2] + R[2, 0]) / s
+    elif R[1, 1] > R[2, 2]:
+        s = math.sqrt(1.0 + R[1, 1] - R[0, 0] - R[2, 2]) * 2
+        w, x, y, z = (R[0, 2] - R[2, 0]) / s, (R[0, 1] + R[1, 0]) / s, 0.25 * s, (R[1, 2] + R[2, 1]) / s
+    else:
+        s = math.sqrt(1.0 + R[2, 2] - R[0, 0] - R[1, 1]) * 2
+        w, x, y, z = (R[1, 0] - R[0, 1]) / s, (R[0, 2] + R[2, 0]) / s, (R[1, 2] + R[2, 1]) / s, 0.25 * s
+    q = np.array([x, y, z, w])
+    return q / np.linalg.norm(q)
+
+
+def decompose(matrix16):
+    """glTF column-major node matrix -> (translation, rotation xyzw, scale),
+    for matrices without shear (as glTF requires of node matrices); a negative
+    determinant is carried by the x scale."""
+    M = np.asarray(matrix16, np.float64).reshape(4, 4).T
+    t = M[:3, 3].copy()
+    A = M[:3, :3].copy()
+    s = np.linalg.norm(A, axis=0)
+    if np.linalg.det(A) < 0:
+        s[0] = -s[0]
+    R = A / np.where(s == 0, 1.0, s)
+    return t, _quat_from_matrix(R), s
+
+
+def load_gltf(path: str, scene: int | None = None) -> SceneArrays:
+    """The glTF file at `path` as SceneArrays (meshes, primitives, materials,
+    draws and transforms of the default scene, or of `scene`)."""
+    asset = _Asset(path)
+    js = asset.js
+    # materials: 0 = default, glTF i -> i + 1
+    gmats = js.get("materials", [])
+    mats = np.zeros(len(gmats) + 1, MATERIAL_DTYPE)
+    mats["albedo_factor"] = 1.0
+    mats["albedo_index"] = INVALID_HANDLE
+    mats["uv_scale"] = 1.0
+    mats["alpha_cutoff"] = 0.5
+    for i, m in enumerate(gmats):
+        pbr = m.get("pbrMetallicRoughness", {})
+        mats[i + 1]["albedo_factor"] = pbr.get("baseColorFactor", [1.0, 1.0, 1.0, 1.0])
+        mats[i + 1]["alpha_cutoff"] = m.get("alphaCutoff", 0.5)
+        mats[i + 1]["double_sided"] = 1 if m.get("doubleSided", False) else 0
+        tt = pbr.get("baseColorTexture", {}).get("extensions", {}).get("KHR_texture_transform")
+        if tt:
+            mats[i + 1]["uv_offset"] = tt.get("offset", [0.0, 0.0])
+            mats[i + 1]["uv_scale"] = tt.get("scale", [1.0, 1.0])
+            mats[i + 1]["uv_rotation"] = tt.get("rotation", 0.0)
+    # meshes -> one vkgv mesh + primitive per glTF primitive
+    meshes, prims, mesh_prims, prim_list = [], [], [], []
+    for gm in js.get("meshes", []):
+        first = len(prims)
+        for gp in gm["primitives"]:
+            if gp.get("mode", 4) != 4:
+                raise ValueError("only triangle-list primitives are supported (mode 4)")
+            if "extensions" in gp and "KHR_draco_mesh_compression" in gp["extensions"]:
+                raise NotImplementedError("KHR_draco_mesh_compression is not supported")
+            pa = gp["attributes"]["POSITION"]
+            pos = asset.accessor(pa, True)
+            if pos.shape[1] != 3:
+                raise GltfError("POSITION must be VEC3")
+            if "indices" in gp:
+                idx = asset.accessor(gp["indices"], False).ravel().astype(np.uint32)
+            else:
+                idx = np.arange(pos.shape[0], dtype=np.uint32)
+            if len(idx) % 3:
+                raise GltfError("triangle list with an index count not divisible by 3")
+            if len(idx) and int(idx.max()) >= pos.shape[0]:
+                raise GltfError("index out of range")
+            ma = build_meshlets(pos, idx)
+            if "COLOR_0" in gp["attributes"]:
+                col = asset.accessor(gp["attributes"]["COLOR_0"], True)
+                rgba = np.ones((col.shape[0], 4), np.float32)
+                rgba[:, :col.shape[1]] = col
+                ma.vertices["color"] = np.clip(np.rint(rgba * 255.0), 0, 255).astype(np.uint8)
+            acc = js["accessors"][pa]
+            if "min" in acc and "max" in acc:
+                lo, hi = np.asarray(acc["min"], np.float32), np.asarray(acc["max"], np.float32)
+                if acc.get("normalized", False):
+                    lo = asset_normalize(acc, lo)
+                    hi = asset_normalize(acc, hi)
+            else:
+                lo, hi = pos.min(0), pos.max(0)
+            p = np.zeros(1, PRIM_DTYPE)[0]
+            p["mesh_index"] = len(meshes)
+            p["material_index"] = gp["material"] + 1 if "material" in gp else 0
+            center = (lo + hi) / 2.0
+            p["aabb_center"] = center
+            p["aabb_extents"] = hi - center
+            meshes.append(ma)
+            prims.append(p)
+            prim_list.append(len(prims) - 1)
+        mesh_prims += [first, len(prims) - first]
+    prims_arr = np.array(prims, dtype=PRIM_DTYPE) if prims else np.zeros(0, PRIM_DTYPE)
+    # nodes of the scene -> draws and transforms (World traversal)
+    gnodes = js.get("nodes", [])
+    nodes = (N.Node * max(1, len(gnodes)))()
+    children = []
+    for i, n in enumerate(gnodes):
+        nd = nodes[i]
+        nd.mesh = n.get("mesh", -1)
+        kids = n.get("children", [])
+        nd.first_child, nd.child_count = len(children), len(kids)
+        children += kids
+        if "matrix" in n:
+            t, r, s = decompose(n["matrix"])
+        else:
+            t = n.get("translation", [0.0, 0.0, 0.0])
+            r = n.get("rotation", [0.0, 0.0, 0.0, 1.0])
+            s = n.get("scale", [1.0, 1.0, 1.0])
+        nd.translation[:] = [float(x) for x in t]
+        nd.rotation[:] = [float(x) for x in r]
+        nd.scale[:] = [float(x) for x in s]
+    scenes = js.get("scenes", [])
+    si = scene if scene is not None else js.get("scene", 0)
+    roots = scenes[si]["nodes"] if scenes else list(range(len(gnodes)))
+    draws, transforms = _world(nodes, len(gnodes), children, roots, mesh_prims, prim_list,
+                               [len(m.meshlets) for m in meshes])
+    return SceneArrays(meshes, prims_arr, mats, draws, transforms)
+
+
+def asset_normalize(acc, v):
+    """Accessor min / max of a normalized integer accessor, in float."""
+    ct = acc["componentType"]
+    if ct == 5120:
+        return np.maximum(v / 127.0, -1.0).astype(np.float32)
+    if ct == 5121:
+        return (v / 255.0).astype(np.float32)
+    if ct == 5122:
+        return np.maximum(v / 32767.0, -1.0).astype(np.float32)
+    if ct == 5123:
+        return (v / 65535.0).astype(np.float32)
+    return v
+
+
+def _world(nodes, node_count, children, roots, mesh_prims, prim_list, meshlet_counts):
+    L = N.host()
+    ch = np.ascontiguousarray(children if children else [0], np.uint32)
+    rt = np.ascontiguousarray(roots if roots else [0], np.uint32)
+    mp = np.ascontiguousarray(mesh_prims if mesh_prims else [0, 0], np.uint32)
+    pl = np.ascontiguousarray(prim_list if prim_list else [0], np.uint32)
+    mc = np.ascontiguousarray(meshlet_counts if meshlet_counts else [0], np.uint32)
+    args = [nodes, node_count, ch.ctypes.data, rt.ctypes.data, len(roots), mp.ctypes.data, pl.ctypes.data, mc.ctypes.data]
+    nd, nt = C.c_uint32(0), C.c_uint32(0)
+    if L.vkgv_world_build(*args, None, C.byref(nd), None, C.byref(nt)) != 0:
+        raise GltfError("World traversal failed: " + L.vkgv_scene_error().decode())
+    draws = np.zeros(nd.value, DRAW_DTYPE)
+    tr = np.zeros((nt.value, 16), np.float32)
+    if L.vkgv_world_build(*args, draws.ctypes.data, C.byref(nd), tr.ctypes.data, C.byref(nt)) != 0:
+        raise GltfError("World traversal failed: " + L.vkgv_scene_error().decode())
+    return draws, tr
