@@ -6,7 +6,8 @@
  *     World::updateTransformBuffer, world.cpp:187-345),
  *   - a meshlet builder at the reference's limits (64 vertices, 124
  *     triangles, assets.cpp:322-373), with per-meshlet normal cones,
- *   - seeded synthetic scenes for the benchmark configurations.
+ *   - seeded synthetic scenes for the benchmark configurations,
+ *   - EXT_meshopt_compression decoders for glTF buffer views.
  * Arrays returned by accessors stay valid until vkgv_scene_free().
  */
 #ifndef VKGV_HOST_H
@@ -111,6 +112,40 @@ int vkgv_world_build(const vkgv_node* nodes, uint32_t node_count,
                      const uint32_t* meshlet_counts,
                      vkgv_meshlet_draw* draws, uint32_t* draw_count,
                      float* transforms, uint32_t* transform_count);
+
+/* ---- EXT_meshopt_compression ------------------------------------------- */
+
+/* Decoders of compressed glTF buffer views, replacing the meshoptimizer calls
+ * of CompressedBufferDataAdapter::ExecuteRange (assets.cpp:111-171):
+ * meshopt_decodeVertexBuffer (assets.cpp:132-133), meshopt_decodeIndexBuffer
+ * (:137-138), meshopt_decodeIndexSequence (:142-143) and
+ * meshopt_decodeFilterOct / Quat / Exp (:152-167), with the same arguments
+ * and return codes: 0 success, -1 bad header, version or argument, -2 the
+ * stream ends early, -3 trailing data.  Vertex streams: codec version 0
+ * (0xA0), vertex_size a multiple of 4 up to 256.  Triangle streams: versions 0
+ * and 1 (0xE0, 0xE1), index_size 2 or 4.  Index sequences: 0xD0 / 0xD1. */
+int vkgv_meshopt_decode_vertex_buffer(void* destination, size_t vertex_count, size_t vertex_size,
+                                      const unsigned char* buffer, size_t buffer_size);
+int vkgv_meshopt_decode_index_buffer(void* destination, size_t index_count, size_t index_size,
+                                     const unsigned char* buffer, size_t buffer_size);
+int vkgv_meshopt_decode_index_sequence(void* destination, size_t index_count, size_t index_size,
+                                       const unsigned char* buffer, size_t buffer_size);
+/* In-place filters over count elements of `stride` bytes: octahedral
+ * normals (stride 4: int8, 8: int16), quaternions (stride 8), exponential
+ * floats (stride a multiple of 4). */
+void vkgv_meshopt_decode_filter_oct(void* buffer, size_t count, size_t stride);
+void vkgv_meshopt_decode_filter_quat(void* buffer, size_t count, size_t stride);
+void vkgv_meshopt_decode_filter_exp(void* buffer, size_t count, size_t stride);
+
+enum { VKGV_MESHOPT_ATTRIBUTES = 0, VKGV_MESHOPT_TRIANGLES = 1, VKGV_MESHOPT_INDICES = 2 };
+enum { VKGV_MESHOPT_FILTER_NONE = 0, VKGV_MESHOPT_FILTER_OCTAHEDRAL = 1, VKGV_MESHOPT_FILTER_QUATERNION = 2,
+       VKGV_MESHOPT_FILTER_EXPONENTIAL = 3 };
+/* One buffer view as the adapter decodes it (assets.cpp:111-171): the mode's
+ * decoder into count * stride bytes, then the filter (ATTRIBUTES only; the
+ * stride must suit the filter).  Returns the decoder's code, or -1 for an
+ * invalid mode / filter combination. */
+int vkgv_meshopt_decode(void* destination, size_t count, size_t stride, const unsigned char* buffer,
+                        size_t buffer_size, int mode, int filter);
 
 #ifdef __cplusplus
 }

@@ -14,6 +14,7 @@ TYPE = {1: "SCALAR", 2: "VEC2", 3: "VEC3", 4: "VEC4"}
 class Builder:
     def __init__(self):
         self.bin = bytearray()
+        self.fallback = 0          # byteLength of the meshopt fallback buffer (buffer 1), 0 = none
         self.js = {"asset": {"version": "2.0"}, "buffers": [], "bufferViews": [], "accessors": [],
                    "meshes": [], "materials": [], "nodes": [], "scenes": [], "scene": 0}
 
@@ -30,6 +31,28 @@ class Builder:
             v["target"] = target
         self.bin += data
         self.js["bufferViews"].append(v)
+        return len(self.js["bufferViews"]) - 1
+
+    def compressed_view(self, stream: bytes, count, stride, mode, filter=None, target=None):
+        """An EXT_meshopt_compression buffer view: `stream` goes into buffer 0,
+        the view itself points into the fallback buffer 1."""
+        self._align()
+        ext = {"buffer": 0, "byteOffset": len(self.bin), "byteLength": len(stream), "byteStride": stride,
+               "count": count, "mode": mode}
+        if filter:
+            ext["filter"] = filter
+        self.bin += stream
+        v = {"buffer": 1, "byteOffset": self.fallback, "byteLength": count * stride,
+             "extensions": {"EXT_meshopt_compression": ext}}
+        if mode == "ATTRIBUTES":
+            v["byteStride"] = stride
+        if target:
+            v["target"] = target
+        self.fallback += (count * stride + 3) & ~3
+        self.js["bufferViews"].append(v)
+        for k in ("extensionsUsed", "extensionsRequired"):
+            if "EXT_meshopt_compression" not in self.js.setdefault(k, []):
+                self.js[k].append("EXT_meshopt_compression")
         return len(self.js["bufferViews"]) - 1
 
     def accessor(self, arr, normalized=False, minmax=False, view=None, offset=0, count=None):
@@ -83,6 +106,9 @@ class Builder:
         if uri is not None:
             b["uri"] = uri
         js["buffers"] = [b]
+        if self.fallback:
+            js["buffers"].append({"byteLength": self.fallback,
+                                  "extensions": {"EXT_meshopt_compression": {"fallback": True}}})
         return js
 
     def write_gltf(self, path, external=False):

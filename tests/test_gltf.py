@@ -1,13 +1,15 @@
 """glTF ingestion on the CPU: accessor decoding (KHR_mesh_quantization
-normalization, strides, sparse), generated indices, materials with the
-default at index 0, node TRS and decomposed matrices through the World
-traversal, .gltf / .glb equivalence and the rejected forms."""
+normalization, strides, sparse, EXT_meshopt_compression views), generated
+indices, materials with the default at index 0, node TRS and decomposed
+matrices through the World traversal, .gltf / .glb equivalence and the
+rejected forms."""
 import json
 
 import numpy as np
 import pytest
 
-from gltfgen import Builder, instanced_spheres, rotation_quat, trs_matrix, uv_sphere
+from gltfgen import Builder, instanced_spheres, quantize_snorm16, rotation_quat, trs_matrix, uv_sphere
+from meshopt_enc import encode_index_buffer, encode_index_sequence, encode_vertex_buffer
 from vk_gltf_viewer_amd import gltf
 from vk_gltf_viewer_amd.scene import Scene
 
@@ -219,10 +221,10 @@ def test_vertex_colors(tmp_path):
 def test_rejected_forms(tmp_path):
     pos = np.array([[0, 0, 0], [1, 0, 0], [0, 1, 0]], np.float32)
     b = one_triangle_builder(pos)
-    b.js["extensionsRequired"] = ["EXT_meshopt_compression"]
-    b.write_gltf(tmp_path / "mo.gltf")
-    with pytest.raises(NotImplementedError, match="EXT_meshopt_compression"):
-        gltf.load_gltf(str(tmp_path / "mo.gltf"))
+    b.js["meshes"][0]["primitives"][0]["extensions"] = {"KHR_draco_mesh_compression": {}}
+    b.write_gltf(tmp_path / "dr.gltf")
+    with pytest.raises(NotImplementedError, match="draco"):
+        gltf.load_gltf(str(tmp_path / "dr.gltf"))
     b = one_triangle_builder(pos)
     b.js["meshes"][0]["primitives"][0]["mode"] = 1
     b.write_gltf(tmp_path / "lines.gltf")
@@ -260,3 +262,60 @@ def test_renders_like_procedural_scene_arrays(tmp_path):
     g = gltf.load_gltf(str(tmp_path / "p.glb"))
     assert canon(mesh_triangles(g.meshes[0])) == canon(tris)
     np.testing.assert_allclose(g.transforms[0], sc.transforms[0], rtol=0, atol=1e-6)
+
+
+def test_meshopt_compressed_views_load_like_plain_ones(tmp_path):
+    """EXT_meshopt_compression (assets.cpp:111-171): int16 positions (stride 8,
+    KHR_mesh_quantization) through the vertex codec, u16 triangles through the
+    index codec and a u32 index sequence; the loaded scene equals the same
+    data stored plainly."""
+    rng = np.random.default_rng(8)
+    pos, tris = uv_sphere(20, 28)
+    q, c, s = quantize_snorm16(pos)
+    q8 = np.zeros((len(q), 4), np.int16)
+    q8[:, :3] = q
+    vstream = encode_vertex_buffer(q8.view(np.uint8).reshape(len(q), 8), rng)
+    istream, dec = encode_index_buffer(tris, 1, rng)
+    pos2, tris2 = uv_sphere(6, 8)
+    seq = tris2.astype(np.uint32).ravel()
+    sstream = encode_index_sequence(seq, rng)
+
+    def build(compressed):
+        b = Builder()
+        if compressed:
+            pv = b.compressed_view(vstream, len(q), 8, "ATTRIBUTES")
+            iv = b.compressed_view(istream, dec.size, 2, "TRIANGLES")
+            sv = b.compressed_view(sstream, seq.size, 4, "INDICES")
+        else:
+            pv = b.view(q8.tobytes(), stride=8)
+            iv = b.view(dec.astype(np.uint16).tobytes())
+            sv = b.view(seq.tobytes())
+        pa = b.accessor(q, normalized=True, view=pv)
+        lo, hi = q.min(0), q.max(0)
+        b.js["accessors"][pa]["min"], b.js["accessors"][pa]["max"] = lo.tolist(), hi.tolist()
+        ia = b.accessor(dec.astype(np.uint16).reshape(-1, 1), view=iv)
+        p2 = b.accessor(pos2, minmax=True)
+        sa = b.accessor(seq.reshape(-1, 1), view=sv)
+        b.mesh([{"attributes": {"POSITION": pa}, "indices": ia}, {"attributes": {"POSITION": p2}, "indices": sa}])
+        b.scene([b.node(mesh=0, scale=[s, s, s], translation=c.tolist())])
+        b.js.setdefault("extensionsRequired", []).append("KHR_mesh_quantization")
+        return b
+
+    build(True).write_glb(tmp_path / "mo.glb")
+    build(False).write_glb(tmp_path / "plain.glb")
+    a = gltf.load_gltf(str(tmp_path / "mo.glb"))
+    b = gltf.load_gltf(str(tmp_path / "plain.glb"))
+    for k in ("primitives", "materials", "draws", "transforms"):
+        assert np.array_equal(getattr(a, k), getattr(b, k)), k
+    for ma, mb in zip(a.meshes, b.meshes):
+        for k in ("vertex_indices", "triangle_indices", "vertices", "meshlets", "cones"):
+            assert np.array_equal(getattr(ma, k), getattr(mb, k)), k
+    assert canon(mesh_triangles(a.meshes[0])) == canon(tris)
+    assert canon(mesh_triangles(a.meshes[1])) == canon(tris2)
+    # a corrupted stream is an error, not garbage geometry
+    bad = build(True)
+    ext = bad.js["bufferViews"][0]["extensions"]["EXT_meshopt_compression"]
+    ext["byteLength"] -= 1
+    bad.write_glb(tmp_path / "bad.glb")
+    with pytest.raises(ValueError, match="meshopt"):
+        gltf.load_gltf(str(tmp_path / "bad.glb"))

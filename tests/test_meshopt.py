@@ -1,0 +1,226 @@
+"""EXT_meshopt_compression decoders (csrc/host/meshopt.cpp), pinned by
+
+- a known-answer triangle stream (index codec v0, upstream test data) that
+  exercises the table, edge-fifo and both slow-path codes with free indices;
+- a hand-assembled vertex stream (2-bit groups with escapes, all-zero groups,
+  the zero-padded tail);
+- round trips through test-side encoders that choose at random among every
+  encoding the format allows, so each decoder branch is reached;
+- the filters against closed forms: EXPONENTIAL is ldexp(m, e) exactly;
+  OCTAHEDRAL and QUATERNION reconstruct the encoded unit vectors within their
+  quantization step;
+- the status codes of malformed streams.
+"""
+import numpy as np
+import pytest
+
+from meshopt_enc import CODEAUX_TABLE, encode_index_buffer, encode_index_sequence, encode_vertex_buffer
+from vk_gltf_viewer_amd import meshopt as MO
+
+KNOWN_INDEX_V0 = bytes([0xE0, 0xF0, 0x10, 0xFE, 0xFF, 0xF0, 0x0C, 0xFF, 0x02, 0x02, 0x02, 0x00, 0x76, 0x87, 0x56,
+                        0x67, 0x78, 0xA9, 0x86, 0x65, 0x89, 0x68, 0x98, 0x01, 0x69, 0x00, 0x00])
+KNOWN_INDICES = [0, 1, 2, 2, 1, 3, 4, 6, 5, 7, 8, 9]
+
+
+@pytest.mark.parametrize("size,dt", [(2, np.uint16), (4, np.uint32)])
+def test_known_index_stream(size, dt):
+    rc, out = MO.decode_index_buffer(KNOWN_INDEX_V0, 12, size)
+    assert rc == 0
+    assert out.view(dt).tolist() == KNOWN_INDICES
+    assert KNOWN_INDEX_V0[-16:] == CODEAUX_TABLE
+
+
+def test_known_vertex_stream():
+    """Four 12-byte vertices (u16 x, y, z; u8 pair; u16 pair), x and y in
+    {0, 300}: byte channels 0-3 in 2-bit groups (zigzag deltas 0/88/87/88,
+    0/2/1/2, 0/0/88/0, 0/0/2/0; 88 and 87 escaped), the rest all-zero groups,
+    then a 32-byte tail holding the zero first vertex."""
+    stream = bytes([0xA0,
+                    0x01, 0x3F, 0x00, 0x00, 0x00, 0x58, 0x57, 0x58,
+                    0x01, 0x26, 0x00, 0x00, 0x00,
+                    0x01, 0x0C, 0x00, 0x00, 0x00, 0x58,
+                    0x01, 0x08, 0x00, 0x00, 0x00]) + bytes(8) + bytes(32)
+    rc, out = MO.decode_vertex_buffer(stream, 4, 12)
+    assert rc == 0
+    v = out.reshape(4, 12)
+    assert v[:, 0:6].copy().view(np.uint16).tolist() == [[0, 0, 0], [300, 0, 0], [0, 300, 0], [300, 300, 0]]
+    assert not v[:, 6:].any()
+    # and the encoder's smallest choice is exactly this stream
+    assert encode_vertex_buffer(v) == stream
+
+
+@pytest.mark.parametrize("stride", [4, 8, 12, 16, 20, 64, 132, 256])
+def test_vertex_round_trip(stride):
+    rng = np.random.default_rng(stride)
+    for n in (1, 15, 16, 17, 255, 256, 257, 700):
+        smooth = np.cumsum(rng.integers(-3, 4, (n, stride)), 0).astype(np.uint8)
+        noise = rng.integers(0, 256, (n, stride)).astype(np.uint8)
+        mixed = np.where(rng.random((n, stride)) < 0.1, noise, smooth).astype(np.uint8)
+        for data in (smooth, noise, mixed, np.zeros((n, stride), np.uint8)):
+            for choice in (None, rng):
+                s = encode_vertex_buffer(data, choice)
+                rc, out = MO.decode_vertex_buffer(s, n, stride)
+                assert rc == 0, (n, stride)
+                assert np.array_equal(out.reshape(n, stride), data), (n, stride)
+
+
+def test_vertex_zero_count():
+    rc, _ = MO.decode_vertex_buffer(bytes([0xA0]) + bytes(32), 0, 16)
+    assert rc == 0
+
+
+def mesh_triangle_sets(rng):
+    grid = []
+    for y in range(12):
+        for x in range(12):
+            a, b, c, d = y * 13 + x, y * 13 + x + 1, (y + 1) * 13 + x, (y + 1) * 13 + x + 1
+            grid += [(a, b, d), (a, d, c)]
+    strip = [(i, i + 1, i + 2) if i % 2 == 0 else (i + 1, i, i + 2) for i in range(60)]
+    rand = rng.integers(0, 40, (80, 3))
+    rand = rand[(rand[:, 0] != rand[:, 1]) & (rand[:, 1] != rand[:, 2]) & (rand[:, 0] != rand[:, 2])]
+    far = rng.integers(0, 1 << 30, (30, 3))
+    restarts = [(0, 1, 2), (2, 1, 3), (0, 1, 2), (0, 1, 2), (3, 4, 5)]
+    return [np.array(grid), np.array(strip), rand, far, np.array(restarts), np.array(grid[::-1])]
+
+
+@pytest.mark.parametrize("version", [0, 1])
+def test_index_round_trip(version):
+    rng = np.random.default_rng(100 + version)
+    for tris in mesh_triangle_sets(rng):
+        for trial in range(4):
+            s, want = encode_index_buffer(tris, version, rng)
+            for size, dt in ((4, np.uint32), (2, np.uint16)):
+                if size == 2 and tris.max() >= 65536:
+                    continue
+                rc, out = MO.decode_index_buffer(s, want.size, size)
+                assert rc == 0
+                assert np.array_equal(out.view(dt).reshape(-1, 3), want.astype(dt))
+            # each decoded triangle is a rotation of the input (winding kept)
+            for t, w in zip(tris, want):
+                assert any(tuple(np.roll(t, -r)) == tuple(w) for r in range(3))
+
+
+def test_index_stream_uses_every_code():
+    """The random encoder reaches every code class of the index codec."""
+    rng = np.random.default_rng(7)
+    seen = set()
+    for tris in mesh_triangle_sets(rng):
+        for _ in range(6):
+            s, want = encode_index_buffer(tris, 1, rng)
+            codes = s[1:1 + len(want)]
+            for c in codes:
+                if c < 0xF0:
+                    f = c & 15
+                    seen.add("edge-next" if f == 0 else "edge-fifo" if f < 13 else f"edge-{f}")
+                else:
+                    seen.add("table" if c < 0xFE else hex(c))
+    assert seen >= {"edge-next", "edge-fifo", "edge-13", "edge-14", "edge-15", "table", "0xfe", "0xff"}, seen
+
+
+def test_index_sequence_round_trip():
+    rng = np.random.default_rng(5)
+    for seq in (np.arange(300), rng.integers(0, 1 << 20, 500), np.repeat(rng.integers(0, 9000, 50), 3),
+                np.array([], np.int64)):
+        s = encode_index_sequence(seq, rng)
+        for size, dt in ((4, np.uint32), (2, np.uint16)):
+            if size == 2 and seq.size and seq.max() >= 65536:
+                continue
+            rc, out = MO.decode_index_sequence(s, seq.size, size)
+            assert rc == 0
+            assert np.array_equal(out.view(dt), seq.astype(dt))
+
+
+def test_malformed_streams():
+    tris = np.array([(0, 1, 2), (2, 1, 3), (7, 9, 8)])
+    s, want = encode_index_buffer(tris, 1, np.random.default_rng(0))
+    assert MO.decode_index_buffer(s, 9, 4)[0] == 0
+    assert MO.decode_index_buffer(bytes([0xE2]) + s[1:], 9, 4)[0] == -1          # version 2
+    assert MO.decode_index_buffer(bytes([0xA1]) + s[1:], 9, 4)[0] == -1          # not a triangle stream
+    assert MO.decode_index_buffer(s[:-17] + s[-16:], 9, 4)[0] in (-2, -3)        # data cut short
+    assert MO.decode_index_buffer(s[:-16] + b"\x00" + s[-16:], 9, 4)[0] == -3    # trailing byte
+    assert MO.decode_index_buffer(s, 9, 3)[0] == -1                              # index size
+    assert MO.decode_index_buffer(s, 10, 4)[0] == -1                             # not whole triangles
+    assert MO.decode_index_buffer(s[:10], 9, 4)[0] == -2                         # shorter than codes + table
+    v = np.arange(64, dtype=np.uint8).reshape(4, 16)
+    sv = encode_vertex_buffer(v)
+    assert MO.decode_vertex_buffer(sv, 4, 16)[0] == 0
+    assert MO.decode_vertex_buffer(bytes([0xA1]) + sv[1:], 4, 16)[0] == -1       # vertex codec v1 not accepted
+    assert MO.decode_vertex_buffer(sv, 4, 6)[0] == -1                            # stride not a multiple of 4
+    assert MO.decode_vertex_buffer(sv[:1] + sv[2:], 4, 16)[0] in (-2, -3)
+    assert MO.decode_vertex_buffer(sv[:1] + b"\x00" + sv[1:], 4, 16)[0] in (-2, -3)
+    assert MO.decode_vertex_buffer(sv[:20], 4, 16)[0] in (-2, -3)
+    sq = encode_index_sequence(np.arange(10))
+    assert MO.decode_index_sequence(sq, 10, 4)[0] == 0
+    assert MO.decode_index_sequence(sq, 11, 4)[0] == -2
+    assert MO.decode_index_sequence(sq[:-4] + b"\x00" + sq[-4:], 10, 4)[0] == -3
+    with pytest.raises(MO.MeshoptError, match="trailing"):
+        MO.decode(s[:-16] + b"\x00" + s[-16:], 9, 4, "TRIANGLES")
+    with pytest.raises(MO.MeshoptError):
+        MO.decode(sv, 4, 16, "TRIANGLES")
+    with pytest.raises(MO.MeshoptError):
+        MO.decode(sv, 4, 16, "ATTRIBUTES", "QUATERNION")                      # quaternions need stride 8
+
+
+def test_filter_exponential_is_ldexp():
+    rng = np.random.default_rng(11)
+    m = rng.integers(-(1 << 23), 1 << 23, 4000)
+    e = rng.integers(-100, 101, 4000)
+    raw = ((e.astype(np.int64) & 0xFF) << 24) | (m.astype(np.int64) & 0xFFFFFF)
+    arr = raw.astype(np.uint32)
+    MO.filter_in_place("exp", arr, 1000, 16)
+    want = np.ldexp(m.astype(np.float64), e).astype(np.float32)
+    assert np.array_equal(arr.view(np.float32), want)
+
+
+def oct_encode(n, bits):
+    """Octahedral map of unit vectors, quantized to `bits`; z holds 1.0."""
+    maxv = (1 << (bits - 1)) - 1
+    p = n / np.abs(n).sum(1, keepdims=True)
+    x, y = p[:, 0].copy(), p[:, 1].copy()
+    lower = p[:, 2] < 0
+    x2 = (1 - np.abs(p[:, 1])) * np.where(p[:, 0] >= 0, 1, -1)
+    y2 = (1 - np.abs(p[:, 0])) * np.where(p[:, 1] >= 0, 1, -1)
+    x, y = np.where(lower, x2, x), np.where(lower, y2, y)
+    q = np.zeros((len(n), 4), np.int64)
+    q[:, 0] = np.rint(x * maxv)
+    q[:, 1] = np.rint(y * maxv)
+    q[:, 2] = maxv
+    q[:, 3] = np.arange(len(n)) % 100
+    return q
+
+
+@pytest.mark.parametrize("bits,stride,dt", [(8, 4, np.int8), (16, 8, np.int16)])
+def test_filter_octahedral_reconstructs_unit_vectors(bits, stride, dt):
+    rng = np.random.default_rng(bits)
+    n = rng.normal(size=(3000, 3))
+    n /= np.linalg.norm(n, axis=1, keepdims=True)
+    q = oct_encode(n, bits).astype(dt)
+    w = q[:, 3].copy()
+    MO.filter_in_place("oct", q, len(n), stride)
+    maxv = (1 << (bits - 1)) - 1
+    d = q[:, :3].astype(np.float64)
+    # unit length at the component scale, within the output rounding
+    assert np.abs(np.linalg.norm(d, axis=1) - maxv).max() < 1.5
+    # the encoded direction, within the input quantization (one step of 1/maxv on the octahedron)
+    ang = np.arccos(np.clip((d / np.linalg.norm(d, axis=1, keepdims=True) * n).sum(1), -1, 1))
+    assert ang.max() < 4.0 / maxv
+    assert np.array_equal(q[:, 3], w)
+
+
+def test_filter_quaternion_reconstructs_unit_quaternions():
+    rng = np.random.default_rng(12)
+    bits = 12
+    rmax = (1 << (bits - 1)) - 1
+    q = rng.normal(size=(3000, 4))
+    q /= np.linalg.norm(q, axis=1, keepdims=True)
+    qc = np.abs(q).argmax(1)
+    q *= np.where(q[np.arange(len(q)), qc] < 0, -1, 1)[:, None]
+    enc = np.zeros((len(q), 4), np.int16)
+    for j in range(3):
+        comp = q[np.arange(len(q)), (qc + 1 + j) & 3]
+        enc[:, j] = np.rint(comp * np.sqrt(2) * rmax)
+    enc[:, 3] = (rmax & ~3) | qc
+    MO.filter_in_place("quat", enc, len(q), 8)
+    got = enc.astype(np.float64) / 32767.0
+    assert np.abs(got - q).max() < 2.0 / rmax
+    assert (enc[np.arange(len(q)), qc] > 0).all()

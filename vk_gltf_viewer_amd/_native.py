@@ -162,7 +162,7 @@ _hip = None
 
 
 def host():
-    """libvkgv_host.so (camera, World, meshlets, scenes)."""
+    """libvkgv_host.so (camera, World, meshlets, scenes, meshopt decoders)."""
     global _host
     if _host is None:
         if not os.path.exists(HOST_LIB):
@@ -186,6 +186,11 @@ def host():
         _bind(lib, "vkgv_scene_triangle_count", _u64, _p)
         _bind(lib, "vkgv_world_build", C.c_int, C.POINTER(Node), _u32, _p, _p, _u32, _p, _p, _p,
               _p, C.POINTER(_u32), _p, C.POINTER(_u32))
+        for nm in ("vertex_buffer", "index_buffer", "index_sequence"):
+            _bind(lib, f"vkgv_meshopt_decode_{nm}", C.c_int, _p, C.c_size_t, C.c_size_t, _p, C.c_size_t)
+        for nm in ("oct", "quat", "exp"):
+            _bind(lib, f"vkgv_meshopt_decode_filter_{nm}", None, _p, C.c_size_t, C.c_size_t)
+        _bind(lib, "vkgv_meshopt_decode", C.c_int, _p, C.c_size_t, C.c_size_t, _p, C.c_size_t, C.c_int, C.c_int)
         _host = lib
     return _host
 

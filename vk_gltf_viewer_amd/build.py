@@ -42,7 +42,7 @@ def build_host(verbose=True):
     out = os.path.join(LIB, "libvkgv_host.so")
     _run(["g++", "-O2", "-std=c++17", "-fPIC", "-shared", "-fopenmp", "-ffp-contract=off",
           "-Wall", "-Wextra", "-Wno-unused-parameter", "-I", INCLUDE, "-o", out,
-          os.path.join(CSRC, "host", "host.cpp")])
+          os.path.join(CSRC, "host", "host.cpp"), os.path.join(CSRC, "host", "meshopt.cpp")])
     return out
 
 

@@ -24,8 +24,12 @@ Beyond the reference, COLOR_0 (as unorm8 RGBA) is loaded into the vertex
 colour that the barycentric resolve interpolates; without it the colour is
 white.
 
-Not supported: EXT_meshopt_compression buffer views (raises
-NotImplementedError), primitive modes other than triangles (ValueError).
+EXT_meshopt_compression buffer views are decoded on the host
+(vk_gltf_viewer_amd.meshopt, as assets.cpp:111-171 does with meshoptimizer)
+before the accessors read them; a decoder error raises.
+
+Not supported: primitive modes other than triangles (ValueError),
+KHR_draco_mesh_compression (NotImplementedError).
 """
 from __future__ import annotations
 
@@ -39,11 +43,12 @@ import struct
 import numpy as np
 
 from . import _native as N
+from . import meshopt
 from .scene import MATERIAL_DTYPE, PRIM_DTYPE, DRAW_DTYPE, SceneArrays, build_meshlets
 
 _COMPONENT = {5120: np.int8, 5121: np.uint8, 5122: np.int16, 5123: np.uint16, 5125: np.uint32, 5126: np.float32}
 _COUNT = {"SCALAR": 1, "VEC2": 2, "VEC3": 3, "VEC4": 4, "MAT2": 4, "MAT3": 9, "MAT4": 16}
-_SUPPORTED_REQUIRED = {"KHR_mesh_quantization", "KHR_texture_transform", "KHR_materials_emissive_strength",
+_SUPPORTED_REQUIRED = {"EXT_meshopt_compression", "KHR_mesh_quantization", "KHR_texture_transform", "KHR_materials_emissive_strength",
                        "KHR_lights_punctual", "KHR_texture_basisu", "MSFT_texture_dds"}
 INVALID_HANDLE = 0xFFFFFFFF
 
@@ -82,19 +87,20 @@ class _Asset:
         if self.js.get("asset", {}).get("version", "2.0").split(".")[0] != "2":
             raise GltfError("only glTF 2.x is supported")
         required = set(self.js.get("extensionsRequired", []))
-        if "EXT_meshopt_compression" in required:
-            raise NotImplementedError("EXT_meshopt_compression buffers are not supported by this loader")
         unknown = required - _SUPPORTED_REQUIRED
         if unknown:
             raise GltfError(f"required extensions not supported: {sorted(unknown)}")
         self._buffers = {}
+        self._decoded = {}
 
     def buffer(self, i: int) -> bytes:
         if i not in self._buffers:
             b = self.js["buffers"][i]
             uri = b.get("uri")
+            if uri is None and b.get("extensions", {}).get("EXT_meshopt_compression", {}).get("fallback"):
+                raise GltfError(f"buffer {i} is a meshopt fallback buffer without data")
             if uri is None:
-                if self.glb_bin is None:
+                if i != 0 or self.glb_bin is None:
                     raise GltfError(f"buffer {i} has no uri and there is no GLB binary chunk")
                 data = self.glb_bin
             elif uri.startswith("data:"):
@@ -110,8 +116,16 @@ class _Asset:
 
     def view_bytes(self, vi: int):
         v = self.js["bufferViews"][vi]
-        if "EXT_meshopt_compression" in v.get("extensions", {}):
-            raise NotImplementedError("EXT_meshopt_compression buffer views are not supported by this loader")
+        mc = v.get("extensions", {}).get("EXT_meshopt_compression")
+        if mc is not None:
+            if vi not in self._decoded:
+                src = self.buffer(mc["buffer"])
+                off = mc.get("byteOffset", 0)
+                if off + mc["byteLength"] > len(src):
+                    raise GltfError(f"buffer view {vi}: compressed range past the end of its buffer")
+                self._decoded[vi] = meshopt.decode(src[off:off + mc["byteLength"]], mc["count"], mc["byteStride"],
+                                                   mc["mode"], mc.get("filter", "NONE"))
+            return self._decoded[vi], v.get("byteStride", mc["byteStride"] if mc["mode"] == "ATTRIBUTES" else None)
         data = self.buffer(v["buffer"])
         off = v.get("byteOffset", 0)
         return data[off:off + v["byteLength"]], v.get("byteStride")
