@@ -83,7 +83,7 @@ typedef struct vkgv_primitive {
 /* glsl::Material, mesh_common.h.glsl:115-126 (48 bytes) */
 typedef struct vkgv_material {
     float    albedo_factor[4];       /* linear RGBA */
-    uint32_t albedo_index;           /* @16, texture handle (unused on this path) */
+    uint32_t albedo_index;           /* @16, texture handle: index into vkgv_upload_textures, ~0u = none */
     float    uv_offset[2];           /* @20 */
     float    uv_scale[2];            /* @28 */
     float    uv_rotation;            /* @36 */
@@ -149,6 +149,7 @@ typedef struct vkgv_primitive_desc {
 #define VKGV_NO_BACKFACE         (1u << 7)  /* disable per-triangle back-face culling */
 #define VKGV_SKIP_RESOLVE        (1u << 8)  /* leave the colour image untouched */
 #define VKGV_GRAPH               (1u << 9)  /* vkgv_render_batch: replay the batch's frame as one captured hipGraph */
+#define VKGV_RESOLVE_TEXTURED    (1u << 10) /* as RESOLVE_BARY, times the material's albedo texture sampled at transformUv(uv) */
 
 /* The reference's own frame: frustum + previous-frame Hi-Z occlusion, pyramid
  * rebuilt after the resolve (application.cpp:843-1003). */
@@ -230,6 +231,29 @@ int vkgv_upload_scene(vkgv_ctx* ctx,
                       const vkgv_mesh_desc* meshes, uint32_t mesh_count,
                       const vkgv_primitive_desc* primitives, uint32_t primitive_count,
                       const vkgv_material* materials, uint32_t material_count);
+
+/* Albedo textures of the textured resolve: the sampling the reference's
+ * fragment shader leaves commented out (visbuffer.frag.glsl:31-32,
+ * `color * albedoFactor * toLinear(texture(textures[albedoIndex],
+ * transformUv(material, uv)))`), with transformUv of mesh_common.h.glsl:
+ * 129-135.  Texels are sRGB-encoded RGBA8, decoded per texel (as an _SRGB
+ * image format does) and filtered bilinearly at level 0 in linear space;
+ * alpha is linear.  Wrap modes per axis as glTF samplers give them. */
+#define VKGV_WRAP_REPEAT          0u   /* glTF 10497 */
+#define VKGV_WRAP_CLAMP_TO_EDGE   1u   /* glTF 33071 */
+#define VKGV_WRAP_MIRRORED_REPEAT 2u   /* glTF 33648 */
+typedef struct vkgv_texture_desc {
+    const uint8_t* rgba8;            /* width * height * 4 bytes, row-major, row 0 at v = 0 */
+    uint32_t width, height;          /* 1 .. 16384 */
+    uint32_t wrap_s, wrap_t;         /* VKGV_WRAP_* */
+} vkgv_texture_desc;
+
+/* Copies the textures to device memory owned by the context, replacing any
+ * previous set (no counterpart in the reference, whose image load task is
+ * empty: assets.cpp:467-471).  A frame with VKGV_RESOLVE_TEXTURED fails with
+ * VKGV_ERROR_STATE while a material's albedo_index is neither ~0u nor below
+ * `count`. */
+int vkgv_upload_textures(vkgv_ctx* ctx, const vkgv_texture_desc* textures, uint32_t count);
 
 /* The draw list (World::rebuildDrawBuffer, world.cpp:230-293): at most 2^25
  * draws, each index checked against the uploaded scene. */

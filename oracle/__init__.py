@@ -32,12 +32,27 @@ def lib():
     return _lib
 
 
+class RefTexture(C.Structure):
+    """vkgv_texture_desc."""
+    _fields_ = [("rgba8", C.c_void_p), ("width", C.c_uint32), ("height", C.c_uint32),
+                ("wrap_s", C.c_uint32), ("wrap_t", C.c_uint32)]
+
+
+def _textures(texs):
+    keep = [np.ascontiguousarray(t.rgba, dtype=np.uint8) for t in texs]
+    arr = (RefTexture * max(1, len(texs)))()
+    for i, (t, a) in enumerate(zip(texs, keep)):
+        arr[i] = RefTexture(a.ctypes.data, a.shape[1], a.shape[0], int(t.wrap_s), int(t.wrap_t))
+    return arr, keep
+
+
 class RefScene(C.Structure):
     _fields_ = [("meshes", C.c_void_p), ("mesh_count", C.c_uint32),
                 ("primitives", C.c_void_p), ("primitive_count", C.c_uint32),
                 ("materials", C.c_void_p), ("material_count", C.c_uint32),
                 ("draws", C.c_void_p), ("draw_count", C.c_uint32),
-                ("transforms", C.c_void_p), ("transform_count", C.c_uint32)]
+                ("transforms", C.c_void_p), ("transform_count", C.c_uint32),
+                ("textures", C.c_void_p), ("texture_count", C.c_uint32)]
 
 
 def _setup(L):
@@ -103,9 +118,12 @@ class Scene:
         self._mats = np.ascontiguousarray(sc.materials)
         self._draws = np.ascontiguousarray(sc.draws)
         self._tr = np.ascontiguousarray(sc.transforms, dtype=np.float32)
+        texs = getattr(sc, "textures", [])
+        self._tex, self._tex_keep = _textures(texs)
         self.s = RefScene(C.cast(self._mesh_arr, C.c_void_p), len(sc.meshes),
                           self._prims.ctypes.data, len(self._prims), self._mats.ctypes.data, len(self._mats),
-                          self._draws.ctypes.data, len(self._draws), self._tr.ctypes.data, self._tr.shape[0])
+                          self._draws.ctypes.data, len(self._draws), self._tr.ctypes.data, self._tr.shape[0],
+                          C.cast(self._tex, C.c_void_p), len(texs))
 
     @property
     def ref(self):

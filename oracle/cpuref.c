@@ -623,9 +623,99 @@ void ref_srgb_codes(const float* c, uint64_t n, uint8_t* out)
     for (uint64_t i = 0; i < n; ++i) out[i] = (uint8_t)unorm8(ref_from_linear(c[i]));
 }
 
+/* toLinear (srgb.h.glsl:17-23) of b / 255, as float arithmetic. */
+static float to_linear_byte(int b)
+{
+    const float v = (float)b / 255.0f;
+    const int cutoff = v < 0.04045f;
+    const float higher = powf((v + 0.055f) / 1.055f, 2.4f);
+    const float lower = v / 12.92f;
+    return cutoff ? lower : higher;
+}
+
+/* IEEE binary16 bits -> float. */
+static float half_to_float(uint32_t h)
+{
+    const uint32_t sgn = (h & 0x8000u) << 16, e = (h >> 10) & 31u, m = h & 1023u;
+    uint32_t bits;
+    float f;
+    if (e == 0u) {
+        f = ldexpf((float)m, -24);
+        return sgn ? -f : f;
+    }
+    bits = e == 31u ? (sgn | 0x7F800000u | (m << 13)) : (sgn | ((e - 15u + 127u) << 23) | (m << 13));
+    memcpy(&f, &bits, 4);
+    return f;
+}
+
+/* Texel of integer coordinate k: repeat (glTF 10497), clamp to edge
+ * (33071), mirrored repeat (33648). */
+static uint32_t wrap_texel(int64_t k, uint32_t n, uint32_t mode)
+{
+    if (mode == VKGV_WRAP_CLAMP_TO_EDGE) return (uint32_t)(k < 0 ? 0 : (k >= (int64_t)n ? (int64_t)n - 1 : k));
+    if (mode == VKGV_WRAP_MIRRORED_REPEAT) {
+        /* period 2n: 0..n-1 forwards, then n-1..0 */
+        int64_t period = 2 * (int64_t)n, m = ((k % period) + period) % period;
+        return (uint32_t)(m < (int64_t)n ? m : period - 1 - m);
+    }
+    return (uint32_t)((((k % (int64_t)n) + (int64_t)n)) % (int64_t)n);
+}
+
+/* Vulkan LINEAR filtering at level 0 along one axis (Vulkan spec,
+ * "Texel Filtering"): x = u * n - 1/2, texels floor(x) and floor(x) + 1,
+ * weight of the second frac(x).  NaN maps to 0; x is clamped to +-2^30. */
+static float texel_pair(float u, uint32_t n, uint32_t mode, uint32_t* i0, uint32_t* i1)
+{
+    float x = u * (float)n - 0.5f;
+    if (x != x) x = 0.0f;
+    if (x < -1073741824.0f) x = -1073741824.0f;
+    if (x > 1073741824.0f) x = 1073741824.0f;
+    const float fl = floorf(x);
+    const int64_t k = (int64_t)fl;
+    *i0 = wrap_texel(k, n, mode);
+    *i1 = wrap_texel(k + 1, n, mode);
+    return x - fl;
+}
+
+/* The commented-out material sampling of visbuffer.frag.glsl:31-32:
+ * c *= toLinear(texture(albedo, transformUv(material, uv))), with
+ * transformUv = mat2(cos, -sin, sin, cos) * uv * uvScale + uvOffset
+ * (mesh_common.h.glsl:129-135) and the texture's bytes decoded per texel. */
+static void sample_albedo(const ref_scene* s, const vkgv_material* mat, const float uv[2], float* c)
+{
+    const vkgv_texture_desc* tex = &s->textures[mat->albedo_index];
+    const float cs = cosf(mat->uv_rotation), sn = sinf(mat->uv_rotation);
+    const float ur = cs * uv[0] + sn * uv[1];
+    const float vr = (-sn) * uv[0] + cs * uv[1];
+    const float tu = ur * mat->uv_scale[0] + mat->uv_offset[0];
+    const float tv = vr * mat->uv_scale[1] + mat->uv_offset[1];
+    uint32_t x0, x1, y0, y1;
+    const float fx = texel_pair(tu, tex->width, tex->wrap_s, &x0, &x1);
+    const float fy = texel_pair(tv, tex->height, tex->wrap_t, &y0, &y1);
+    const float ax = 1.0f - fx, ay = 1.0f - fy;
+    const uint8_t* p00 = tex->rgba8 + 4 * ((size_t)y0 * tex->width + x0);
+    const uint8_t* p10 = tex->rgba8 + 4 * ((size_t)y0 * tex->width + x1);
+    const uint8_t* p01 = tex->rgba8 + 4 * ((size_t)y1 * tex->width + x0);
+    const uint8_t* p11 = tex->rgba8 + 4 * ((size_t)y1 * tex->width + x1);
+    for (int j = 0; j < 4; ++j) {
+        float t00, t10, t01, t11;
+        if (j < 3) {
+            t00 = to_linear_byte(p00[j]); t10 = to_linear_byte(p10[j]);
+            t01 = to_linear_byte(p01[j]); t11 = to_linear_byte(p11[j]);
+        } else {
+            t00 = (float)p00[j] / 255.0f; t10 = (float)p10[j] / 255.0f;
+            t01 = (float)p01[j] / 255.0f; t11 = (float)p11[j] / 255.0f;
+        }
+        const float r0 = t00 * ax + t10 * fx;
+        const float r1 = t01 * ax + t11 * fx;
+        c[j] = c[j] * (r0 * ay + r1 * fy);
+    }
+}
+
 /* visbuffer_resolve.comp.glsl:17-41; with VKGV_RESOLVE_BARY the colour is
  * albedoFactor * interpolated vertex colour, using perspective-correct
- * barycentrics recomputed from the ids. */
+ * barycentrics recomputed from the ids; VKGV_RESOLVE_TEXTURED multiplies in
+ * the sampled albedo texture as well. */
 void ref_resolve(const ref_scene* s, const float* vp, const uint64_t* keys,
                  uint32_t w, uint32_t h, uint32_t flags, uint32_t* rgba)
 {
@@ -641,7 +731,7 @@ void ref_resolve(const ref_scene* s, const float* vp, const uint64_t* keys,
             const vkgv_primitive_desc* p = &s->primitives[dr->primitive_index];
             const vkgv_material* mat = &s->materials[p->material_index];
             float c[4] = { mat->albedo_factor[0], mat->albedo_factor[1], mat->albedo_factor[2], mat->albedo_factor[3] };
-            if (flags & VKGV_RESOLVE_BARY) {
+            if (flags & (VKGV_RESOLVE_BARY | VKGV_RESOLVE_TEXTURED)) {
                 const vkgv_mesh_desc* mesh = &s->meshes[p->mesh_index];
                 const vkgv_meshlet* ml = &mesh->meshlets[dr->meshlet_index];
                 const float* m = s->transforms + 16ull * dr->transform_index;
@@ -649,9 +739,11 @@ void ref_resolve(const ref_scene* s, const float* vp, const uint64_t* keys,
                 ref_mat4_mul(vp, m, mvp);
                 const uint8_t* ti = mesh->triangle_indices + ml->triangle_offset + 3 * t;
                 float hv[3][3];
-                float col[3][4];
+                float col[3][4], uvs[3][2];
                 for (int k = 0; k < 3; ++k) {
                     const vkgv_vertex* vx = &mesh->vertices[mesh->vertex_indices[ml->vertex_offset + ti[k]]];
+                    uvs[k][0] = half_to_float(vx->uv[0]);
+                    uvs[k][1] = half_to_float(vx->uv[1]);
                     const float p4[4] = { vx->position[0], vx->position[1], vx->position[2], 1.0f };
                     float cl[4];
                     mat4_vec4(mvp, p4, cl);
@@ -670,6 +762,11 @@ void ref_resolve(const ref_scene* s, const float* vp, const uint64_t* keys,
                 for (int k = 0; k < 3; ++k) l[k] = l[k] * inv;
                 for (int j = 0; j < 4; ++j)
                     c[j] = c[j] * ((l[0] * col[0][j] + l[1] * col[1][j]) + l[2] * col[2][j]);
+                if ((flags & VKGV_RESOLVE_TEXTURED) && mat->albedo_index != 0xFFFFFFFFu) {
+                    const float uv[2] = { (l[0] * uvs[0][0] + l[1] * uvs[1][0]) + l[2] * uvs[2][0],
+                                          (l[0] * uvs[0][1] + l[1] * uvs[1][1]) + l[2] * uvs[2][1] };
+                    sample_albedo(s, mat, uv, c);
+                }
             }
             const uint32_t r8 = unorm8(ref_from_linear(c[0]));
             const uint32_t g8 = unorm8(ref_from_linear(c[1]));

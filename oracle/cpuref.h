@@ -42,6 +42,8 @@ typedef struct ref_scene {
     uint32_t                   draw_count;
     const float*               transforms;      /* mat4, column-major */
     uint32_t                   transform_count;
+    const vkgv_texture_desc*   textures;        /* albedo textures (VKGV_RESOLVE_TEXTURED) */
+    uint32_t                   texture_count;
 } ref_scene;
 
 /* ---- camera.cpp --------------------------------------------------------- */

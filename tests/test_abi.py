@@ -48,7 +48,8 @@ def test_struct_layouts_match_headers():
     }
     ctypes_structs = {"vkgv_camera": N.Camera, "vkgv_frame_desc": N.FrameDesc, "vkgv_frame_stats": N.FrameStats,
                       "vkgv_create_info": N.CreateInfo, "vkgv_tuning": N.Tuning, "vkgv_mesh_desc": N.MeshDesc,
-                      "vkgv_primitive": N.Primitive, "vkgv_view_desc": N.ViewDesc, "vkgv_node": N.Node}
+                      "vkgv_primitive": N.Primitive, "vkgv_view_desc": N.ViewDesc, "vkgv_node": N.Node,
+                      "vkgv_texture_desc": N.TextureDesc}
     lines = ["#include <stdio.h>", "#include <stddef.h>", '#include "vkgv.h"', '#include "vkgv_host.h"',
              "int main(void) {"]
     for st, (_, fl) in fields.items():
@@ -88,5 +89,8 @@ def test_status_codes_and_flags_match_header():
     for py, hdr in [("CULL_FRUSTUM", "VKGV_CULL_FRUSTUM"), ("CULL_OCCLUSION_PREV", "VKGV_CULL_OCCLUSION_PREV"),
                     ("CULL_TWO_PASS", "VKGV_CULL_TWO_PASS"), ("CULL_CONE", "VKGV_CULL_CONE"),
                     ("RESOLVE_BARY", "VKGV_RESOLVE_BARY"), ("BUILD_PYRAMID", "VKGV_BUILD_PYRAMID"),
-                    ("TIMING", "VKGV_TIMING"), ("NO_BACKFACE", "VKGV_NO_BACKFACE"), ("SKIP_RESOLVE", "VKGV_SKIP_RESOLVE")]:
+                    ("TIMING", "VKGV_TIMING"), ("NO_BACKFACE", "VKGV_NO_BACKFACE"), ("SKIP_RESOLVE", "VKGV_SKIP_RESOLVE"),
+                    ("GRAPH", "VKGV_GRAPH"), ("RESOLVE_TEXTURED", "VKGV_RESOLVE_TEXTURED"),
+                    ("WRAP_REPEAT", "VKGV_WRAP_REPEAT"), ("WRAP_CLAMP_TO_EDGE", "VKGV_WRAP_CLAMP_TO_EDGE"),
+                    ("WRAP_MIRRORED_REPEAT", "VKGV_WRAP_MIRRORED_REPEAT")]:
         assert getattr(N, py) == val(hdr), py

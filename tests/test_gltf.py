@@ -319,3 +319,45 @@ def test_meshopt_compressed_views_load_like_plain_ones(tmp_path):
     bad.write_glb(tmp_path / "bad.glb")
     with pytest.raises(ValueError, match="meshopt"):
         gltf.load_gltf(str(tmp_path / "bad.glb"))
+
+
+def test_textures_samplers_and_uvs(tmp_path):
+    """Base-colour textures decode to the PNG's bytes; samplers map to wrap
+    modes; materials point at the decoded textures (untextured: ~0u); the
+    vertex uv holds TEXCOORD_0 as float16."""
+    from texscene import textured_spheres
+    from vk_gltf_viewer_amd import _native as N
+    b = textured_spheres(grid=2, seed=4)
+    b.write_glb(tmp_path / "t.glb")
+    sc = gltf.load_gltf(str(tmp_path / "t.glb"))
+    assert len(sc.textures) == 3
+    from PIL import Image
+    import io
+    for k, t in enumerate(sc.textures):
+        v = b.js["bufferViews"][b.js["images"][k]["bufferView"]]
+        png = bytes(b.bin[v["byteOffset"]:v["byteOffset"] + v["byteLength"]])
+        assert np.array_equal(t.rgba, np.asarray(Image.open(io.BytesIO(png)).convert("RGBA")))
+    assert [(t.wrap_s, t.wrap_t) for t in sc.textures] == [
+        (N.WRAP_REPEAT, N.WRAP_CLAMP_TO_EDGE), (N.WRAP_MIRRORED_REPEAT, N.WRAP_CLAMP_TO_EDGE),
+        (N.WRAP_CLAMP_TO_EDGE, N.WRAP_REPEAT)]
+    assert list(sc.materials["albedo_index"]) == [gltf.INVALID_HANDLE, 0, 1, 2, gltf.INVALID_HANDLE]
+    np.testing.assert_allclose(sc.materials[2]["uv_scale"], [1.5, 0.75])
+    uv_acc = b.js["meshes"][0]["primitives"][0]["attributes"]["TEXCOORD_0"]
+    a = b.js["accessors"][uv_acc]
+    v = b.js["bufferViews"][a["bufferView"]]
+    uv = np.frombuffer(bytes(b.bin[v["byteOffset"]:v["byteOffset"] + v["byteLength"]]), np.float32).reshape(-1, 2)
+    assert np.array_equal(sc.meshes[0].vertices["uv"], uv.astype(np.float16).view(np.uint16))
+    # without textures: no decode, every albedo handle invalid
+    bare = gltf.load_gltf(str(tmp_path / "t.glb"), textures=False)
+    assert bare.textures == [] and (bare.materials["albedo_index"] == gltf.INVALID_HANDLE).all()
+
+
+def test_undecodable_image_leaves_material_untextured(tmp_path):
+    from texscene import textured_spheres
+    b = textured_spheres(grid=1, seed=5)
+    b.js["images"][1] = {"bufferView": b.view(b"KTX 20 not an image"), "mimeType": "image/ktx2"}
+    b.write_glb(tmp_path / "u.glb")
+    with pytest.warns(UserWarning, match="not decoded"):
+        sc = gltf.load_gltf(str(tmp_path / "u.glb"))
+    assert len(sc.textures) == 2
+    assert list(sc.materials["albedo_index"]) == [gltf.INVALID_HANDLE, 0, gltf.INVALID_HANDLE, 1, gltf.INVALID_HANDLE]

@@ -37,6 +37,8 @@ TIMING = 1 << 6
 NO_BACKFACE = 1 << 7
 SKIP_RESOLVE = 1 << 8
 GRAPH = 1 << 9
+RESOLVE_TEXTURED = 1 << 10
+WRAP_REPEAT, WRAP_CLAMP_TO_EDGE, WRAP_MIRRORED_REPEAT = 0, 1, 2
 MODE_REFERENCE = CULL_FRUSTUM | CULL_OCCLUSION_PREV | BUILD_PYRAMID
 
 OK = 0
@@ -67,6 +69,11 @@ class Primitive(C.Structure):
                 ("vertex_buffer", C.c_uint64), ("meshlet_buffer", C.c_uint64),
                 ("aabb_extents", C.c_float * 3), ("aabb_center", C.c_float * 3),
                 ("meshlet_count", C.c_uint32), ("material_index", C.c_uint32)]
+
+
+class TextureDesc(C.Structure):
+    _fields_ = [("rgba8", C.c_void_p), ("width", C.c_uint32), ("height", C.c_uint32),
+                ("wrap_s", C.c_uint32), ("wrap_t", C.c_uint32)]
 
 
 class Material(C.Structure):
@@ -202,7 +209,7 @@ HIP_SYMBOLS = [
     "vkgv_view_read_color", "vkgv_view_read_pyramid", "vkgv_view_write_pyramid", "vkgv_view_write_depth",
     "vkgv_view_read_visible", "vkgv_view_read_stats", "vkgv_set_tuning", "vkgv_get_tuning", "vkgv_srgb_thresholds",
     "vkgv_batch_create", "vkgv_batch_destroy", "vkgv_render_batch", "vkgv_batch_read_stats",
-    "vkgv_view_aa_image", "vkgv_view_aa_backward",
+    "vkgv_view_aa_image", "vkgv_view_aa_backward", "vkgv_upload_textures",
 ]
 
 
@@ -230,6 +237,7 @@ def hip():
         _bind(lib, "vkgv_destroy", None, _p)
         _bind(lib, "vkgv_last_error", C.c_char_p, _p)
         _bind(lib, "vkgv_sync", C.c_int, _p)
+        _bind(lib, "vkgv_upload_textures", C.c_int, _p, C.POINTER(TextureDesc), _u32)
         _bind(lib, "vkgv_upload_scene", C.c_int, _p, C.POINTER(MeshDesc), _u32, C.POINTER(PrimitiveDesc), _u32,
               C.POINTER(Material), _u32)
         _bind(lib, "vkgv_set_draws", C.c_int, _p, _p, _u32)

@@ -55,6 +55,10 @@ struct vkgv_ctx {
     bool draws_set = false;
     RasterTuning tuning{ 16u, 3u, 16u, 8192u };
     DevBuf srgb;                     // 255 sRGB-encode thresholds
+    DevBuf srgb_decode;              // toLinear of each byte / 255 (256 floats)
+    DevBuf tex_recs, texels, mat_rot;
+    uint32_t texture_count = 0;
+    int64_t max_albedo_index = -1;   // largest texture handle a material names (-1: none)
     uint32_t big_capacity = 1u << 18;
     uint32_t clip_capacity = 1u << 16;
     uint32_t cache_slots = 8192;     // resolve-cache meshlet slots per view
@@ -176,7 +180,25 @@ SceneDev scene_dev(const vkgv_ctx* c)
         s.vtx_base = static_cast<const vkgv_vertex*>(c->mesh_bufs[2].p);
     }
     s.srgb = c->srgb.p;
+    s.textures = static_cast<const TexRec*>(c->tex_recs.p);
+    s.texture_count = c->texture_count;
+    s.texels = static_cast<const uint32_t*>(c->texels.p);
+    s.srgb_decode = static_cast<const float*>(c->srgb_decode.p);
+    s.mat_rot = static_cast<const float2*>(c->mat_rot.p);
     return s;
+}
+
+// toLinear (srgb.h.glsl:17-23) of every byte value b / 255: the per-texel
+// decode of the textured resolve, evaluated once on the host.
+void srgb_decode_table(float* out)
+{
+    for (int b = 0; b < 256; ++b) {
+        const float v = (float)b / 255.0f;
+        const bool cutoff = v < 0.04045f;
+        const float higher = std::pow((v + 0.055f) / 1.055f, 2.4f);
+        const float lower = v / 12.92f;
+        out[b] = cutoff ? lower : higher;
+    }
 }
 
 // fromLinear (srgb.h.glsl:9-15) followed by the unorm8 store.
@@ -360,6 +382,9 @@ int check_frame_inputs(vkgv_ctx* c, uint32_t flags)
                                              + " but only " + std::to_string(c->transform_count) + " are set");
     if ((flags & VKGV_CULL_TWO_PASS) && (flags & VKGV_CULL_OCCLUSION_PREV))
         return fail(c, VKGV_ERROR_INVALID_ARGUMENT, "render: CULL_TWO_PASS and CULL_OCCLUSION_PREV are exclusive");
+    if ((flags & VKGV_RESOLVE_TEXTURED) && c->max_albedo_index >= (int64_t)c->texture_count)
+        return fail(c, VKGV_ERROR_STATE, "render: a material names texture " + std::to_string(c->max_albedo_index)
+                                             + " but " + std::to_string(c->texture_count) + " are uploaded");
     return VKGV_OK;
 }
 
@@ -405,7 +430,10 @@ int vkgv_create(const vkgv_create_info* info, vkgv_ctx** out)
         while (k2 < 255 && (std::memcpy(&kb, &tab.thr[k2], 4), (kb >> 16) == tab.key0 + b)) ++k2;
         ok = k2 - k <= 1;
     }
-    if (!ok || upload(c, c->srgb, &tab, sizeof tab) != VKGV_OK || hipStreamSynchronize(c->stream) != hipSuccess) {
+    float dec[256];
+    srgb_decode_table(dec);
+    if (!ok || upload(c, c->srgb, &tab, sizeof tab) != VKGV_OK || upload(c, c->srgb_decode, dec, sizeof dec) != VKGV_OK
+        || hipStreamSynchronize(c->stream) != hipSuccess) {
         vkgv_destroy(c);
         return VKGV_ERROR_HIP;
     }
@@ -451,6 +479,7 @@ void vkgv_destroy(vkgv_ctx* c)
     for (auto& b : c->mesh_bufs) dfree(b);
     dfree(c->prims); dfree(c->materials); dfree(c->cone_base); dfree(c->cones);
     dfree(c->draws); dfree(c->cull); dfree(c->rrec); dfree(c->transforms); dfree(c->srgb);
+    dfree(c->srgb_decode); dfree(c->tex_recs); dfree(c->texels); dfree(c->mat_rot);
     (void)hipStreamDestroy(c->stream);
     delete c;
 }
@@ -574,6 +603,16 @@ int vkgv_upload_scene(vkgv_ctx* c, const vkgv_mesh_desc* meshes, uint32_t mesh_c
     }
     if ((rc = upload(c, c->prims, dp.data(), sizeof(vkgv_primitive) * prim_count))) return rc;
     if ((rc = upload(c, c->materials, materials, sizeof(vkgv_material) * material_count))) return rc;
+    // (cos, sin) of each uv rotation, evaluated once on the host so that every
+    // pixel of the textured resolve uses the same rotation matrix
+    std::vector<float2> rot(material_count);
+    c->max_albedo_index = -1;
+    for (uint32_t m = 0; m < material_count; ++m) {
+        rot[m] = make_float2(std::cos(materials[m].uv_rotation), std::sin(materials[m].uv_rotation));
+        if (materials[m].albedo_index != 0xFFFFFFFFu)
+            c->max_albedo_index = std::max<int64_t>(c->max_albedo_index, materials[m].albedo_index);
+    }
+    if ((rc = upload(c, c->mat_rot, rot.data(), sizeof(float2) * material_count))) return rc;
     if ((rc = upload(c, c->cone_base, cb.data(), sizeof(uint32_t) * cb.size()))) return rc;
     if ((rc = upload(c, c->cones, all_cones.data(), sizeof(vkgv_meshlet_cone) * all_cones.size()))) return rc;
     HIP_TRY(c, hipStreamSynchronize(c->stream));
@@ -584,6 +623,36 @@ int vkgv_upload_scene(vkgv_ctx* c, const vkgv_mesh_desc* meshes, uint32_t mesh_c
     c->draws_set = false;
     c->draw_count = 0;
     c->generation++;   // captured batch frames hold the old buffers
+    return VKGV_OK;
+}
+
+int vkgv_upload_textures(vkgv_ctx* c, const vkgv_texture_desc* textures, uint32_t count)
+{
+    if (!c) return VKGV_ERROR_INVALID_ARGUMENT;
+    if (count && !textures) return fail(c, VKGV_ERROR_INVALID_ARGUMENT, "upload_textures: null array");
+    std::vector<TexRec> recs(count);
+    uint64_t total = 0;
+    for (uint32_t i = 0; i < count; ++i) {
+        const vkgv_texture_desc& t = textures[i];
+        if (!t.rgba8 || t.width == 0 || t.height == 0 || t.width > 16384 || t.height > 16384
+            || t.wrap_s > VKGV_WRAP_MIRRORED_REPEAT || t.wrap_t > VKGV_WRAP_MIRRORED_REPEAT)
+            return fail(c, VKGV_ERROR_INVALID_ARGUMENT, "upload_textures: texture " + std::to_string(i) + " is malformed");
+        recs[i] = TexRec{ total, t.width, t.height, t.wrap_s, t.wrap_t, { 0u, 0u } };
+        total += (uint64_t)t.width * t.height;
+    }
+    HIP_TRY(c, hipSetDevice(c->device));
+    HIP_TRY(c, hipDeviceSynchronize());   // frames of any stream may still read the old textures
+    c->texture_count = 0;                 // until the new set is complete
+    c->generation++;
+    int rc = dalloc(c, c->texels, 4ull * std::max<uint64_t>(total, 1));
+    if (rc) return rc;
+    for (uint32_t i = 0; i < count; ++i)
+        HIP_TRY(c, hipMemcpyAsync(static_cast<uint32_t*>(c->texels.p) + recs[i].offset, textures[i].rgba8,
+                                  4ull * textures[i].width * textures[i].height, hipMemcpyHostToDevice, c->stream));
+    if ((rc = upload(c, c->tex_recs, recs.data(), sizeof(TexRec) * count))) return rc;
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    c->texture_count = count;
+    c->generation++;   // captured batch frames hold the old texture addresses
     return VKGV_OK;
 }
 

@@ -38,6 +38,15 @@ struct RasterRec {
 static_assert(sizeof(RasterRec) == 32, "RasterRec layout");
 
 // Device view of the uploaded scene.
+// One albedo texture: `width * height` RGBA8 words from texels + offset.
+struct TexRec {
+    uint64_t offset;
+    uint32_t width, height;
+    uint32_t wrap_s, wrap_t;
+    uint32_t pad[2];
+};
+static_assert(sizeof(TexRec) == 32, "TexRec layout");
+
 struct SceneDev {
     const vkgv_meshlet_draw* draws;
     uint32_t draw_count;
@@ -57,6 +66,13 @@ struct SceneDev {
     const vkgv_vertex* vtx_base;
     // sRGB encode + unorm8 tables (SrgbTables)
     const void* srgb;
+    // textured resolve: texture records, their texels (RGBA8 words), the
+    // per-byte sRGB decode, and (cos, sin) of each material's uv rotation
+    const TexRec* textures;
+    uint32_t texture_count;
+    const uint32_t* texels;
+    const float* srgb_decode;
+    const float2* mat_rot;
 };
 
 // The 8-bit code of fromLinear(c) is the number of the 255 ascending

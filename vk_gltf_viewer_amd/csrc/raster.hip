@@ -255,7 +255,8 @@ __device__ __forceinline__ void raster_view(const RasterArgs& a)
         const Mat4 mvp = load_mat4(mrow);
         const float tdet = mrow[16];
         const bool cull_faces = !(matw >> 31) && !(a.flags & VKGV_NO_BACKFACE);
-        const bool bary = (a.flags & VKGV_RESOLVE_BARY) != 0;   // the flat resolve reads only the slot albedo
+        // the flat resolve reads only the slot albedo; the interpolating ones the triangles too
+        const bool bary = (a.flags & (VKGV_RESOLVE_BARY | VKGV_RESOLVE_TEXTURED)) != 0;
         const uint32_t vc = counts & 0xFFu, tc = (counts >> 8) & 0xFFu;
         // resolve-cache slot of this meshlet (none when the cache is full)
         const uint32_t slot = slot0 + idx;

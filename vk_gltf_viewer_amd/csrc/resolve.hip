@@ -14,6 +14,13 @@
 // the cache capacity are re-fetched and re-transformed from the scene buffers
 // (the same arithmetic).  sRGB encode + unorm8 is a two-level table lookup
 // (exact for the 8-bit result).
+//
+// VKGV_RESOLVE_TEXTURED adds the albedo texture the reference's fragment
+// shader leaves commented out (visbuffer.frag.glsl:31-32): the vertex uvs
+// (float16) are interpolated with the same barycentrics, transformed by
+// transformUv (mesh_common.h.glsl:129-135: rotation, then scale, then
+// offset), and the texture is sampled bilinearly at level 0 with per-texel
+// sRGB decode; colour = vertex colour * albedo factor * texture.
 #include "internal.hpp"
 
 namespace vkgv {
@@ -82,6 +89,95 @@ __device__ __forceinline__ void fetch_uncached(const ResolveArgs& a, uint32_t d,
     r[2] = cross3(hv[0], hv[1]);
 }
 
+// float16 bits -> float (exact: every half is a float).
+__device__ __forceinline__ float half_bits_to_float(uint32_t h)
+{
+    const uint32_t sgn = (h & 0x8000u) << 16, e = (h >> 10) & 31u, m = h & 1023u;
+    uint32_t bits;
+    if (e == 0u) {
+        const float f = (float)m * 5.9604644775390625e-8f;     // m * 2^-24
+        return sgn ? -f : f;
+    }
+    if (e == 31u) bits = sgn | 0x7F800000u | (m << 13);
+    else bits = sgn | ((e + 112u) << 23) | (m << 13);
+    return __uint_as_float(bits);
+}
+
+// Texel index of integer coordinate k under a wrap mode.
+__device__ __forceinline__ uint32_t wrap_texel(long long k, uint32_t n, uint32_t mode)
+{
+    if (mode == VKGV_WRAP_CLAMP_TO_EDGE) return (uint32_t)(k < 0 ? 0 : (k >= (long long)n ? n - 1 : k));
+    if (mode == VKGV_WRAP_MIRRORED_REPEAT) {
+        long long m = k % (2ll * n);
+        if (m < 0) m += 2ll * n;
+        return (uint32_t)(m < (long long)n ? m : 2ll * n - 1 - m);
+    }
+    long long r = k % (long long)n;
+    return (uint32_t)(r < 0 ? r + n : r);
+}
+
+// Bilinear footprint along one axis: texels i0, i1 and the weight of i1.
+// Coordinates are clamped to +-2^30 texels (beyond, the fraction is 0 anyway).
+__device__ __forceinline__ float texel_pair(float u, uint32_t n, uint32_t mode, uint32_t& i0, uint32_t& i1)
+{
+    float x = u * (float)n - 0.5f;
+    if (!(x == x)) x = 0.0f;
+    x = x < -1073741824.0f ? -1073741824.0f : (x > 1073741824.0f ? 1073741824.0f : x);
+    const float fl = floorf(x);
+    const long long k = (long long)fl;
+    i0 = wrap_texel(k, n, mode);
+    i1 = wrap_texel(k + 1, n, mode);
+    return x - fl;
+}
+
+// Albedo texture times the pixel's colour (c[0..3]): uvs of triangle t of
+// draw d interpolated with l[], transformUv, bilinear sample, multiply.
+__device__ __forceinline__ void apply_texture(const SceneDev& s, uint32_t d, uint32_t t, const float* l,
+                                              const float* s_lin, const float* s_byte, float* c)
+{
+    const vkgv_meshlet_draw dr = s.draws[d];
+    const vkgv_primitive& prim = s.prims[dr.primitive_index];
+    const uint32_t mi = prim.material_index;
+    const vkgv_material& mat = s.materials[mi];
+    const uint32_t ti = mat.albedo_index;
+    if (ti == 0xFFFFFFFFu) return;
+    const vkgv_meshlet ml = reinterpret_cast<const vkgv_meshlet*>(prim.meshlet_buffer)[dr.meshlet_index];
+    const uint8_t* tri = reinterpret_cast<const uint8_t*>(prim.primitive_index_buffer) + ml.triangle_offset + 3u * t;
+    const uint32_t* vidx = reinterpret_cast<const uint32_t*>(prim.vertex_index_buffer) + ml.vertex_offset;
+    const vkgv_vertex* verts = reinterpret_cast<const vkgv_vertex*>(prim.vertex_buffer);
+    float uq[3], vq[3];
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        const uint32_t uvw = *reinterpret_cast<const uint32_t*>(verts[vidx[tri[k]]].uv);
+        uq[k] = half_bits_to_float(uvw & 0xFFFFu);
+        vq[k] = half_bits_to_float(uvw >> 16);
+    }
+    const float u = (l[0] * uq[0] + l[1] * uq[1]) + l[2] * uq[2];
+    const float v = (l[0] * vq[0] + l[1] * vq[1]) + l[2] * vq[2];
+    // transformUv: mat2(cos, -sin, sin, cos) * uv * scale + offset
+    const float2 cs = s.mat_rot[mi];
+    const float ur = cs.x * u + cs.y * v;
+    const float vr = (-cs.y) * u + cs.x * v;
+    const float tu = ur * mat.uv_scale[0] + mat.uv_offset[0];
+    const float tv = vr * mat.uv_scale[1] + mat.uv_offset[1];
+    const TexRec tex = s.textures[ti];
+    uint32_t x0, x1, y0, y1;
+    const float fx = texel_pair(tu, tex.width, tex.wrap_s, x0, x1);
+    const float fy = texel_pair(tv, tex.height, tex.wrap_t, y0, y1);
+    const uint32_t* img = s.texels + tex.offset;
+    const uint32_t t00 = img[(size_t)y0 * tex.width + x0], t10 = img[(size_t)y0 * tex.width + x1];
+    const uint32_t t01 = img[(size_t)y1 * tex.width + x0], t11 = img[(size_t)y1 * tex.width + x1];
+    const float ax = 1.0f - fx, ay = 1.0f - fy;
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+        const float* dec = j < 3 ? s_lin : s_byte;
+        const uint32_t sh = 8u * j;
+        const float r0 = dec[(t00 >> sh) & 0xFFu] * ax + dec[(t10 >> sh) & 0xFFu] * fx;
+        const float r1 = dec[(t01 >> sh) & 0xFFu] * ax + dec[(t11 >> sh) & 0xFFu] * fx;
+        c[j] = c[j] * (r0 * ay + r1 * fy);
+    }
+}
+
 // Block = 64 x 16 pixels; thread (x, y) resolves rows y, y + 4, y + 8, y + 12
 // of its column, with the loads of the four pixels issued together.
 constexpr int kRows = 4;
@@ -115,7 +211,10 @@ __global__ __launch_bounds__(256) void resolve_kernel(ResolveBatchArgs ab)
     __shared__ float s_thr[256];
     __shared__ float s_byte[256];
     __shared__ uint32_t s_base[512];
+    __shared__ float s_lin[256];
+    const bool tex = (a.flags & VKGV_RESOLVE_TEXTURED) != 0;
     const SrgbTables* tab = static_cast<const SrgbTables*>(a.s.srgb);
+    if (tex) s_lin[threadIdx.x] = a.s.srgb_decode[threadIdx.x];
     s_thr[threadIdx.x] = tab->thr[threadIdx.x];
     s_byte[threadIdx.x] = (float)threadIdx.x / 255.0f;
     s_base[threadIdx.x] = reinterpret_cast<const uint32_t*>(tab->base)[threadIdx.x];
@@ -127,7 +226,7 @@ __global__ __launch_bounds__(256) void resolve_kernel(ResolveBatchArgs ab)
     const uint32_t px = blk_x * 64u + (threadIdx.x & 63u);
     if (px >= a.W) return;
     const uint32_t py0 = blk_y * (4u * kRows) + (threadIdx.x >> 6);
-    const bool bary = (a.flags & VKGV_RESOLVE_BARY) != 0;
+    const bool bary = (a.flags & (VKGV_RESOLVE_BARY | VKGV_RESOLVE_TEXTURED)) != 0;
     const unsigned long long* __restrict__ keys = a.keys;
     const uint32_t* __restrict__ draw_slot = a.draw_slot;
 
@@ -192,6 +291,7 @@ __global__ __launch_bounds__(256) void resolve_kernel(ResolveBatchArgs ab)
                 const float c2 = s_byte[(col[2] >> (8 * j)) & 0xFFu];
                 c[j] = c[j] * ((l[0] * c0 + l[1] * c1) + l[2] * c2);
             }
+            if (tex) apply_texture(a.s, drw[k], t, l, s_lin, s_byte, c);
         }
         const uint32_t r8 = srgb_code(s_thr, base, key0, c[0]);
         const uint32_t g8 = srgb_code(s_thr, base, key0, c[1]);

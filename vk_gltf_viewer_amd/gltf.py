@@ -13,9 +13,13 @@ Follows the reference's asset loading (assets.cpp):
 - materials: index 0 is the default (albedo 1, alpha cutoff 0.5, single
   sided), glTF material i is i + 1 with its baseColorFactor, alphaCutoff and
   doubleSided (assets.cpp:478-518); a primitive's material index is its glTF
-  index + 1, 0 without one (assets.cpp:292-294); the albedo texture handle is
-  the invalid handle, as in the reference, and a KHR_texture_transform of the
-  base-colour texture is kept in the uv fields;
+  index + 1, 0 without one (assets.cpp:292-294); a KHR_texture_transform of
+  the base-colour texture is kept in the uv fields;
+- beyond the reference (whose image load task is empty, assets.cpp:467-471,
+  so its albedo handle is always invalid): base-colour textures decoded to
+  RGBA8 with their sampler's wrap modes, the material's albedo_index pointing
+  at them, and TEXCOORD uvs as float16 in the vertex, for the textured
+  resolve;
 - node matrices decomposed to translation / rotation / scale
   (fastgltf::Options::DecomposeNodeMatrices), then the World traversal
   (world.cpp:187-293, the host library's vkgv_world_build) of the default
@@ -39,12 +43,13 @@ import json
 import math
 import os
 import struct
+import warnings
 
 import numpy as np
 
 from . import _native as N
 from . import meshopt
-from .scene import MATERIAL_DTYPE, PRIM_DTYPE, DRAW_DTYPE, SceneArrays, build_meshlets
+from .scene import MATERIAL_DTYPE, PRIM_DTYPE, DRAW_DTYPE, SceneArrays, Texture, build_meshlets
 
 _COMPONENT = {5120: np.int8, 5121: np.uint8, 5122: np.int16, 5123: np.uint16, 5125: np.uint32, 5126: np.float32}
 _COUNT = {"SCALAR": 1, "VEC2": 2, "VEC3": 3, "VEC4": 4, "MAT2": 4, "MAT3": 9, "MAT4": 16}
@@ -207,11 +212,63 @@ def decompose(matrix16):
     return t, _quat_from_matrix(R), s
 
 
-def load_gltf(path: str, scene: int | None = None) -> SceneArrays:
+_WRAP = {10497: 0, 33071: 1, 33648: 2}     # REPEAT, CLAMP_TO_EDGE, MIRRORED_REPEAT
+
+
+def _image_bytes(asset, img):
+    if "bufferView" in img:
+        raw, _ = asset.view_bytes(img["bufferView"])
+        return bytes(raw)
+    uri = img.get("uri", "")
+    if uri.startswith("data:"):
+        return base64.b64decode(uri.split(",", 1)[1])
+    from urllib.parse import unquote
+    with open(os.path.join(asset.dir, unquote(uri)), "rb") as f:
+        return f.read()
+
+
+def _load_textures(asset):
+    """One Texture per glTF texture (image + sampler), None where the image
+    cannot be decoded (no PIL, or a format PIL does not read, such as KTX2)."""
+    js = asset.js
+    out, cache = [], {}
+    try:
+        from PIL import Image
+    except ImportError:                      # pragma: no cover - PIL is part of the image
+        Image = None
+    for ti, t in enumerate(js.get("textures", [])):
+        src = t.get("source")
+        for ext in ("KHR_texture_basisu", "MSFT_texture_dds"):
+            if src is None and ext in t.get("extensions", {}):
+                src = t["extensions"][ext].get("source")
+        tex = None
+        if src is not None and Image is not None:
+            if src not in cache:
+                try:
+                    import io
+                    with Image.open(io.BytesIO(_image_bytes(asset, js["images"][src]))) as im:
+                        cache[src] = np.ascontiguousarray(np.asarray(im.convert("RGBA"), np.uint8))
+                except Exception as e:          # undecodable image: the material draws untextured
+                    warnings.warn(f"glTF image {src} not decoded ({e}); texture {ti} is skipped")
+                    cache[src] = None
+            rgba = cache[src]
+            if rgba is not None:
+                smp = js["samplers"][t["sampler"]] if "sampler" in t else {}
+                tex = Texture(rgba, _WRAP.get(smp.get("wrapS", 10497), 0), _WRAP.get(smp.get("wrapT", 10497), 0))
+        out.append(tex)
+    return out
+
+
+def load_gltf(path: str, scene: int | None = None, textures: bool = True) -> SceneArrays:
     """The glTF file at `path` as SceneArrays (meshes, primitives, materials,
-    draws and transforms of the default scene, or of `scene`)."""
+    draws and transforms of the default scene, or of `scene`).  With
+    `textures`, base-colour textures are decoded (PNG, JPEG and whatever else
+    PIL reads) and the materials' albedo_index points at them; each
+    primitive's vertex uv is the TEXCOORD set its base-colour texture uses."""
     asset = _Asset(path)
     js = asset.js
+    gtex = _load_textures(asset) if textures else []
+    tex_list, tex_map = [], {}
     # materials: 0 = default, glTF i -> i + 1
     gmats = js.get("materials", [])
     mats = np.zeros(len(gmats) + 1, MATERIAL_DTYPE)
@@ -224,7 +281,14 @@ def load_gltf(path: str, scene: int | None = None) -> SceneArrays:
         mats[i + 1]["albedo_factor"] = pbr.get("baseColorFactor", [1.0, 1.0, 1.0, 1.0])
         mats[i + 1]["alpha_cutoff"] = m.get("alphaCutoff", 0.5)
         mats[i + 1]["double_sided"] = 1 if m.get("doubleSided", False) else 0
-        tt = pbr.get("baseColorTexture", {}).get("extensions", {}).get("KHR_texture_transform")
+        bct = pbr.get("baseColorTexture")
+        if bct is not None and bct.get("index", -1) < len(gtex) and gtex[bct["index"]] is not None:
+            gi = bct["index"]
+            if gi not in tex_map:
+                tex_map[gi] = len(tex_list)
+                tex_list.append(gtex[gi])
+            mats[i + 1]["albedo_index"] = tex_map[gi]
+        tt = (bct or {}).get("extensions", {}).get("KHR_texture_transform")
         if tt:
             mats[i + 1]["uv_offset"] = tt.get("offset", [0.0, 0.0])
             mats[i + 1]["uv_scale"] = tt.get("scale", [1.0, 1.0])
@@ -251,6 +315,12 @@ def load_gltf(path: str, scene: int | None = None) -> SceneArrays:
             if len(idx) and int(idx.max()) >= pos.shape[0]:
                 raise GltfError("index out of range")
             ma = build_meshlets(pos, idx)
+            uv_set = _uv_set(js, gp)
+            if f"TEXCOORD_{uv_set}" in gp["attributes"]:
+                uv = asset.accessor(gp["attributes"][f"TEXCOORD_{uv_set}"], True)
+                if uv.shape != (pos.shape[0], 2):
+                    raise GltfError("TEXCOORD must be VEC2 with one entry per vertex")
+                ma.vertices["uv"] = uv.astype(np.float16).view(np.uint16)
             if "COLOR_0" in gp["attributes"]:
                 col = asset.accessor(gp["attributes"]["COLOR_0"], True)
                 rgba = np.ones((col.shape[0], 4), np.float32)
@@ -299,7 +369,17 @@ def load_gltf(path: str, scene: int | None = None) -> SceneArrays:
     roots = scenes[si]["nodes"] if scenes else list(range(len(gnodes)))
     draws, transforms = _world(nodes, len(gnodes), children, roots, mesh_prims, prim_list,
                                [len(m.meshlets) for m in meshes])
-    return SceneArrays(meshes, prims_arr, mats, draws, transforms)
+    return SceneArrays(meshes, prims_arr, mats, draws, transforms, textures=tex_list)
+
+
+def _uv_set(js, gp):
+    """TEXCOORD set of the primitive's base-colour texture (0 by default;
+    KHR_texture_transform may override it)."""
+    if "material" not in gp:
+        return 0
+    bct = js["materials"][gp["material"]].get("pbrMetallicRoughness", {}).get("baseColorTexture", {})
+    tt = bct.get("extensions", {}).get("KHR_texture_transform", {})
+    return int(tt.get("texCoord", bct.get("texCoord", 0)))
 
 
 def asset_normalize(acc, v):

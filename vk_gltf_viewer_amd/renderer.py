@@ -21,7 +21,7 @@ import ctypes as C
 import numpy as np
 
 from . import _native as N
-from .scene import SceneArrays
+from .scene import SceneArrays, texture_descs
 
 
 class VkgvError(RuntimeError):
@@ -72,6 +72,13 @@ class Renderer:
             self._h, meshes, len(sc.meshes),
             C.cast(prims.ctypes.data, C.POINTER(N.PrimitiveDesc)), len(prims),
             C.cast(mats.ctypes.data, C.POINTER(N.Material)), len(mats)))
+        del keep
+        self.upload_textures(sc.textures)
+
+    def upload_textures(self, textures):
+        """Albedo textures (scene.Texture list) for VKGV_RESOLVE_TEXTURED."""
+        descs, keep = texture_descs(textures)
+        self._check(self._lib.vkgv_upload_textures(self._h, descs, len(textures)))
         del keep
 
     def set_draws(self, draws: np.ndarray):

@@ -53,6 +53,27 @@ class MeshArrays:
 
 
 @dataclass
+class Texture:
+    """An albedo texture: sRGB-encoded RGBA8, row 0 at v = 0, and the glTF
+    sampler's wrap modes (WRAP_REPEAT / WRAP_CLAMP_TO_EDGE / WRAP_MIRRORED_REPEAT)."""
+    rgba: np.ndarray                # (H, W, 4) uint8
+    wrap_s: int = 0
+    wrap_t: int = 0
+
+
+def texture_descs(textures):
+    """ctypes vkgv_texture_desc array of a texture list, and the arrays it
+    points into (keep them alive while the descriptors are used)."""
+    keep = [np.ascontiguousarray(t.rgba, dtype=np.uint8) for t in textures]
+    arr = (N.TextureDesc * max(1, len(textures)))()
+    for i, (t, a) in enumerate(zip(textures, keep)):
+        if a.ndim != 3 or a.shape[2] != 4:
+            raise ValueError(f"texture {i}: expected (H, W, 4) uint8")
+        arr[i] = N.TextureDesc(a.ctypes.data, a.shape[1], a.shape[0], int(t.wrap_s), int(t.wrap_t))
+    return arr, keep
+
+
+@dataclass
 class SceneArrays:
     meshes: list[MeshArrays]
     primitives: np.ndarray          # PRIM_DTYPE
@@ -60,6 +81,7 @@ class SceneArrays:
     draws: np.ndarray               # DRAW_DTYPE
     transforms: np.ndarray          # (T, 16) f32, column-major
     views: list = field(default_factory=list)
+    textures: list = field(default_factory=list)   # Texture, indexed by the materials' albedo_index
 
     @property
     def triangle_count(self) -> int:
