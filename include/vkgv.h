@@ -150,6 +150,7 @@ typedef struct vkgv_primitive_desc {
 #define VKGV_SKIP_RESOLVE        (1u << 8)  /* leave the colour image untouched */
 #define VKGV_GRAPH               (1u << 9)  /* vkgv_render_batch: replay the batch's frame as one captured hipGraph */
 #define VKGV_RESOLVE_TEXTURED    (1u << 10) /* as RESOLVE_BARY, times the material's albedo texture sampled at transformUv(uv) */
+#define VKGV_MOTION_VECTORS      (1u << 11) /* resolve also writes per-pixel motion vectors (visbuffer.frag.glsl:38) */
 
 /* The reference's own frame: frustum + previous-frame Hi-Z occlusion, pyramid
  * rebuilt after the resolve (application.cpp:843-1003). */
@@ -259,8 +260,66 @@ int vkgv_upload_textures(vkgv_ctx* ctx, const vkgv_texture_desc* textures, uint3
  * draws, each index checked against the uploaded scene. */
 int vkgv_set_draws(vkgv_ctx* ctx, const vkgv_meshlet_draw* draws, uint32_t count);
 
-/* Per-node mat4 transforms (World::updateTransformBuffer, world.cpp:295-345). */
+/* Per-node mat4 transforms (World::updateTransformBuffer, world.cpp:295-345).
+ * The transforms they replace become the previous frame's (motion vectors). */
 int vkgv_set_transforms(vkgv_ctx* ctx, const float* mat4s, uint32_t count);
+
+/* ---- Node hierarchy and animations on the device ------------------------ */
+/* glTF node (TRS form, as fastgltf's DecomposeNodeMatrices gives it). */
+typedef struct vkgv_node {
+    int32_t  mesh;                   /* glTF mesh index or -1 */
+    uint32_t first_child, child_count;
+    uint32_t pad;
+    float    translation[3];
+    float    rotation[4];            /* quaternion x, y, z, w */
+    float    scale[3];
+} vkgv_node;
+
+/* AnimationSampler (assets.hpp:36-110): keyframe times times[input_offset ..
+ * + input_count) (ascending), output values from values[output_offset]
+ * (3 or 4 floats per output; cubic splines store in-tangent, value,
+ * out-tangent per key). */
+#define VKGV_INTERP_LINEAR       0u
+#define VKGV_INTERP_STEP         1u
+#define VKGV_INTERP_CUBICSPLINE  2u
+typedef struct vkgv_anim_sampler {
+    uint32_t input_offset, input_count;
+    uint32_t output_offset;
+    uint32_t interpolation;          /* VKGV_INTERP_* */
+} vkgv_anim_sampler;
+
+/* AnimationChannel: the node and TRS component a sampler drives.  Channels
+ * are applied in array order (the reference's animation-then-channel order,
+ * world.cpp:194-216): a later channel on the same node and path wins. */
+#define VKGV_PATH_TRANSLATION    0u
+#define VKGV_PATH_ROTATION       1u
+#define VKGV_PATH_SCALE          2u
+typedef struct vkgv_anim_channel {
+    uint32_t node, sampler, path, pad;
+} vkgv_anim_channel;
+
+/* Uploads the node hierarchy of the scene's roots and its animations for
+ * vkgv_update_transforms.  The mesh nodes, in the depth-first order of
+ * World::iterateNode, own transforms 0 .. n-1 (the order vkgv_world_build
+ * and the reference's draw list use), and the context's transform count
+ * becomes n.  Depth is limited to 64. */
+int vkgv_set_hierarchy(vkgv_ctx* ctx, const vkgv_node* nodes, uint32_t node_count, const uint32_t* children,
+                       const uint32_t* roots, uint32_t root_count,
+                       const vkgv_anim_sampler* samplers, uint32_t sampler_count,
+                       const float* times, uint32_t time_count, const float* values, uint32_t value_count,
+                       const vkgv_anim_channel* channels, uint32_t channel_count);
+
+/* The per-frame transform update of World::updateDrawBuffers
+ * (world.cpp:346-354) on the device: every animated node's TRS sampled at
+ * `time` (AnimationSampler::sample, assets.hpp:52-110, looping with
+ * fmod(time, last key)), then each mesh node's world matrix
+ * parent * T * R * S down its ancestor chain (World::iterateNode,
+ * world.cpp:187-228), written to the transform buffer the frames read.  The
+ * transforms it replaces become the previous frame's.  Kernels and one
+ * device copy only: no allocation, so it may be captured in a graph. */
+int vkgv_update_transforms(vkgv_ctx* ctx, float time, void* stream);
+/* Copies the current transforms to the host (float[count][16]). */
+int vkgv_read_transforms(vkgv_ctx* ctx, float* mat4s, uint32_t count);
 
 /* ------------------------------------------------------------------------ */
 /* Views: the per-resolution images of the frame (visbuffer, depth, Hi-Z
@@ -288,6 +347,13 @@ int vkgv_view_pyramid_info(const vkgv_view* view, uint32_t* levels, uint32_t* si
 int vkgv_view_read_visbuffer(vkgv_ctx* ctx, vkgv_view* view, uint64_t* keys);      /* W*H */
 int vkgv_view_read_ids_depth(vkgv_ctx* ctx, vkgv_view* view, uint32_t* ids, float* depth);
 int vkgv_view_read_color(vkgv_ctx* ctx, vkgv_view* view, uint32_t* rgba8);         /* W*H */
+/* Motion vectors of the last frame rendered with VKGV_MOTION_VECTORS,
+ * float[H][W][2]: ((prev.xy / prev.w) * 0.5 - 0.5) - ((pos.xy / pos.w) * 0.5 - 0.5)
+ * (visbuffer.frag.glsl:38), where prev = prevViewProjection * previous
+ * transform * vertex, interpolated with the pixel's perspective-correct
+ * barycentrics, and pos.xy / pos.w is the pixel centre's NDC; (0, 0) where
+ * nothing was drawn. */
+int vkgv_view_read_motion(vkgv_ctx* ctx, vkgv_view* view, float* motion);
 int vkgv_view_read_pyramid(vkgv_ctx* ctx, vkgv_view* view, float* texels);
 int vkgv_view_write_pyramid(vkgv_ctx* ctx, vkgv_view* view, const float* texels);
 int vkgv_view_write_depth(vkgv_ctx* ctx, vkgv_view* view, const float* depth);     /* ids cleared */

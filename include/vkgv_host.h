@@ -90,14 +90,7 @@ const vkgv_view_desc*      vkgv_scene_views(const vkgv_scene* s);
 uint64_t                   vkgv_scene_triangle_count(const vkgv_scene* s);  /* sum over draws */
 
 /* ---- World: node hierarchy -> draws + transforms ---------------------- */
-typedef struct vkgv_node {
-    int32_t  mesh;                   /* glTF mesh index or -1 */
-    uint32_t first_child, child_count;
-    uint32_t pad;
-    float    translation[3];
-    float    rotation[4];            /* quaternion x, y, z, w */
-    float    scale[3];
-} vkgv_node;
+/* (vkgv_node is declared in vkgv.h, which the device-side transform update shares.) */
 
 /* Walks the roots depth-first (World::iterateNode, world.cpp:187-228) with
  * M = parent * T * R * S; every node with a mesh takes the next transform

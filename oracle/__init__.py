@@ -95,6 +95,11 @@ def _setup(L):
         "ref_frame_aa_backward": (None, [p, p, p, C.c_int, C.c_int, p, p, p]),
         "ref_frame_aa_loss_at": (C.c_double, [p, p, p, C.c_int, C.c_int, p, p, p, C.c_int64, p, C.c_int]),
         "ref_resolve": (None, [p, p, p, u32, u32, u32, p]),
+        "ref_series_acos": (C.c_double, [C.c_double]),
+        "ref_series_sin": (C.c_double, [C.c_double]),
+        "ref_anim_sample": (None, [p, p, p, C.c_int, f32, p]),
+        "ref_update_transforms": (C.c_int, [p, u32, p, p, u32, p, p, p, p, u32, f32, p, p]),
+        "ref_motion_vectors": (None, [p, p, p, p, u32, u32, p]),
         "ref_frame_reference": (u32, [p, p, p, u32, u32, u32, p, p, p, p]),
         "ref_frame_two_pass": (None, [p, p, p, u32, u32, u32, p, p, p, p, p, p]),
     }
@@ -324,3 +329,49 @@ def frame_aa_loss_at(scene: Scene, vp, keys, W, H, xf, target, pix, state=None, 
         st = state.ctypes.data
     return lib().ref_frame_aa_loss_at(scene.ref, v.ctypes.data, k.ctypes.data, W, H, _ptr(x), t.ctypes.data,
                                       px.ctypes.data, px.size, st, mode)
+
+
+def series_acos(x):
+    return lib().ref_series_acos(float(x))
+
+
+def series_sin(a):
+    return lib().ref_series_sin(float(a))
+
+
+def anim_sample(sampler, times, values, n, time):
+    """AnimationSampler::sample of one sampler (a 1-element ANIM_SAMPLER_DTYPE record)."""
+    sp = np.ascontiguousarray(sampler)
+    t = np.ascontiguousarray(times, np.float32)
+    v = np.ascontiguousarray(values, np.float32)
+    out = np.zeros(4, np.float32)
+    lib().ref_anim_sample(sp.ctypes.data, t.ctypes.data, v.ctypes.data, n, float(time), out.ctypes.data)
+    return out[:n]
+
+
+def update_transforms(h, time):
+    """World matrices of the mesh nodes of hierarchy `h` at `time`, (T, 16) column-major."""
+    nodes = np.ascontiguousarray(h.nodes)
+    ch = np.ascontiguousarray(h.children, np.uint32)
+    rt = np.ascontiguousarray(h.roots, np.uint32)
+    sp = np.ascontiguousarray(h.samplers)
+    tm = np.ascontiguousarray(h.times, np.float32)
+    vl = np.ascontiguousarray(h.values, np.float32)
+    cn = np.ascontiguousarray(h.channels)
+    cnt = C.c_uint32(0)
+    ptr = lambda a: a.ctypes.data if a.size else None
+    args = [ptr(nodes), len(nodes), ptr(ch), ptr(rt), len(rt), ptr(sp), ptr(tm), ptr(vl), ptr(cn), len(cn), float(time)]
+    if lib().ref_update_transforms(*args, None, C.byref(cnt)) != 0:
+        raise ValueError("hierarchy walk failed")
+    out = np.zeros((cnt.value, 16), np.float32)
+    lib().ref_update_transforms(*args, out.ctypes.data if out.size else None, C.byref(cnt))
+    return out
+
+
+def motion_vectors(scene: Scene, camera, prev_transforms, keys, w, h):
+    """Motion vectors (H, W, 2) of a frame's keys."""
+    k = np.ascontiguousarray(keys, np.uint64).ravel()
+    pt = np.ascontiguousarray(prev_transforms, np.float32)
+    out = np.zeros(w * h * 2, np.float32)
+    lib().ref_motion_vectors(scene.ref, C.byref(camera), pt.ctypes.data, k.ctypes.data, w, h, out.ctypes.data)
+    return out.reshape(h, w, 2)

@@ -100,6 +100,8 @@ static void cross3(const float* a, const float* b, float* o)
     memcpy(o, t, sizeof t);
 }
 static float dot3(const float* a, const float* b) { return (a[0] * b[0] + a[1] * b[1]) + a[2] * b[2]; }
+void ref_mat4_vec4(const float* m, const float* v, float* out) { mat4_vec4(m, v, out); }
+void ref_cross3(const float* a, const float* b, float* out) { cross3(a, b, out); }
 
 /* glm::lookAtRH closed form (camera.cpp:170). */
 void ref_look_at_rh(const float* eye, const float* center, const float* up, float* out)

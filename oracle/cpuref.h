@@ -48,6 +48,8 @@ typedef struct ref_scene {
 
 /* ---- camera.cpp --------------------------------------------------------- */
 void ref_mat4_mul(const float* a, const float* b, float* out);
+void ref_mat4_vec4(const float* m, const float* v, float* out);   /* m * v */
+void ref_cross3(const float* a, const float* b, float* out);
 void ref_perspective_rh_zo(float fovy, float aspect, float znear, float zfar, float* out);
 void ref_ortho_rh_zo(float l, float r, float b, float t, float znear, float zfar, float* out);
 void ref_look_at_rh(const float* eye, const float* center, const float* up, float* out);
@@ -118,4 +120,16 @@ void ref_frame_two_pass(const ref_scene* s, const vkgv_camera* cam, const float*
 #ifdef __cplusplus
 }
 #endif
+/* ---- anim.c: transform update and motion vectors ------------------------ */
+double ref_series_acos(double x);
+double ref_series_sin(double a);
+void ref_anim_sample(const vkgv_anim_sampler* sp, const float* times, const float* values, int n, float time,
+                     float* out);
+int  ref_update_transforms(const vkgv_node* nodes, uint32_t node_count, const uint32_t* children,
+                           const uint32_t* roots, uint32_t root_count, const vkgv_anim_sampler* samplers,
+                           const float* times, const float* values, const vkgv_anim_channel* channels,
+                           uint32_t channel_count, float time, float* out, uint32_t* count);
+void ref_motion_vectors(const ref_scene* s, const vkgv_camera* cam, const float* prev_transforms,
+                        const uint64_t* keys, uint32_t w, uint32_t h, float* motion);
+
 #endif

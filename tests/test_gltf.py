@@ -361,3 +361,41 @@ def test_undecodable_image_leaves_material_untextured(tmp_path):
         sc = gltf.load_gltf(str(tmp_path / "u.glb"))
     assert len(sc.textures) == 2
     assert list(sc.materials["albedo_index"]) == [gltf.INVALID_HANDLE, 0, gltf.INVALID_HANDLE, 1, gltf.INVALID_HANDLE]
+
+
+def test_animations_flattened(tmp_path):
+    """Animations (assets.cpp:579-600): samplers of all animations in order with
+    their keyframe times and outputs (normalized int16 rotations decoded),
+    channels mapped to (node, global sampler, path); weights dropped."""
+    import oracle
+    from vk_gltf_viewer_amd import _native as N
+    pos, tris = uv_sphere(4, 6)
+    b = Builder()
+    pa = b.accessor(pos, minmax=True)
+    ia = b.accessor(tris.astype(np.uint16).reshape(-1, 1))
+    b.mesh([{"attributes": {"POSITION": pa}, "indices": ia}])
+    child = b.node(mesh=0, translation=[1.0, 0.0, 0.0])
+    root = b.node(children=[child])
+    b.scene([root])
+    t1 = b.accessor(np.array([[0.0], [1.0], [2.0]], np.float32))
+    v1 = b.accessor(np.array([[0, 0, 0], [2, 0, 0], [0, 0, 0]], np.float32))
+    t2 = b.accessor(np.array([[0.5], [1.0], [3.0]], np.float32))
+    q = np.array([[0, 0, 0, 32767], [0, 23170, 0, 23170], [0, 0, 0, 32767]], np.int16)   # identity, 90 deg about y
+    v2 = b.accessor(q, normalized=True)
+    b.js["animations"] = [
+        {"samplers": [{"input": t1, "output": v1}], "channels": [{"sampler": 0, "target": {"node": child, "path": "translation"}}]},
+        {"samplers": [{"input": t2, "output": v2, "interpolation": "STEP"}, {"input": t1, "output": v1}],
+         "channels": [{"sampler": 0, "target": {"node": root, "path": "rotation"}},
+                      {"sampler": 1, "target": {"node": child, "path": "weights"}}]},
+    ]
+    b.write_glb(tmp_path / "a.glb")
+    h = gltf.load_gltf(str(tmp_path / "a.glb")).hierarchy
+    assert h.samplers.tolist() == [(0, 3, 0, N.INTERP_LINEAR), (3, 3, 9, N.INTERP_STEP), (6, 3, 21, N.INTERP_LINEAR)]
+    assert h.channels.tolist() == [(child, 0, N.PATH_TRANSLATION, 0), (root, 1, N.PATH_ROTATION, 0)]
+    np.testing.assert_array_equal(h.times, [0, 1, 2, 0.5, 1, 3, 0, 1, 2])
+    np.testing.assert_array_equal(h.values[9:21], np.maximum(q.ravel().astype(np.float32) / np.float32(32767), -1))
+    assert h.duration == 3.0
+    # t = 1.6: the child's x = 0.4 * 2 (LINEAR between keys 1 and 2), turned by the
+    # root's 90 degrees about y (STEP holds key 1 until t = 3): x -> -z
+    m = oracle.update_transforms(h, 1.6)[0].reshape(4, 4).T
+    np.testing.assert_allclose(m[:3, 3], [0.0, 0.0, -0.8], atol=1e-4)

@@ -38,6 +38,9 @@ NO_BACKFACE = 1 << 7
 SKIP_RESOLVE = 1 << 8
 GRAPH = 1 << 9
 RESOLVE_TEXTURED = 1 << 10
+MOTION_VECTORS = 1 << 11
+INTERP_LINEAR, INTERP_STEP, INTERP_CUBICSPLINE = 0, 1, 2
+PATH_TRANSLATION, PATH_ROTATION, PATH_SCALE = 0, 1, 2
 WRAP_REPEAT, WRAP_CLAMP_TO_EDGE, WRAP_MIRRORED_REPEAT = 0, 1, 2
 MODE_REFERENCE = CULL_FRUSTUM | CULL_OCCLUSION_PREV | BUILD_PYRAMID
 
@@ -209,7 +212,8 @@ HIP_SYMBOLS = [
     "vkgv_view_read_color", "vkgv_view_read_pyramid", "vkgv_view_write_pyramid", "vkgv_view_write_depth",
     "vkgv_view_read_visible", "vkgv_view_read_stats", "vkgv_set_tuning", "vkgv_get_tuning", "vkgv_srgb_thresholds",
     "vkgv_batch_create", "vkgv_batch_destroy", "vkgv_render_batch", "vkgv_batch_read_stats",
-    "vkgv_view_aa_image", "vkgv_view_aa_backward", "vkgv_upload_textures",
+    "vkgv_view_aa_image", "vkgv_view_aa_backward", "vkgv_upload_textures", "vkgv_set_hierarchy",
+    "vkgv_update_transforms", "vkgv_read_transforms", "vkgv_view_read_motion",
 ]
 
 
@@ -237,6 +241,10 @@ def hip():
         _bind(lib, "vkgv_destroy", None, _p)
         _bind(lib, "vkgv_last_error", C.c_char_p, _p)
         _bind(lib, "vkgv_sync", C.c_int, _p)
+        _bind(lib, "vkgv_set_hierarchy", C.c_int, _p, _p, _u32, _p, _p, _u32, _p, _u32, _p, _u32, _p, _u32, _p, _u32)
+        _bind(lib, "vkgv_update_transforms", C.c_int, _p, C.c_float, _p)
+        _bind(lib, "vkgv_read_transforms", C.c_int, _p, _p, _u32)
+        _bind(lib, "vkgv_view_read_motion", C.c_int, _p, _p, _p)
         _bind(lib, "vkgv_upload_textures", C.c_int, _p, C.POINTER(TextureDesc), _u32)
         _bind(lib, "vkgv_upload_scene", C.c_int, _p, C.POINTER(MeshDesc), _u32, C.POINTER(PrimitiveDesc), _u32,
               C.POINTER(Material), _u32)

@@ -58,6 +58,12 @@ struct vkgv_ctx {
     DevBuf srgb_decode;              // toLinear of each byte / 255 (256 floats)
     DevBuf tex_recs, texels, mat_rot;
     uint32_t texture_count = 0;
+    DevBuf prev_transforms;          // the transforms the last update replaced (motion vectors)
+    uint32_t prev_count = 0;
+    // node hierarchy and animations (vkgv_set_hierarchy)
+    DevBuf an_nodes, an_chan_first, an_channels, an_samplers, an_times, an_values, an_trs, an_chain_first, an_chain;
+    AnimDev anim{};
+    bool anim_set = false;
     int64_t max_albedo_index = -1;   // largest texture handle a material names (-1: none)
     uint32_t big_capacity = 1u << 18;
     uint32_t clip_capacity = 1u << 16;
@@ -72,6 +78,7 @@ struct vkgv_view {
     DevBuf setups, pairs, tri_cache, slot_albedo, draw_slot, big_chunk, chunk_tri, tile_items;
     DevBuf aa_mat;                   // per-pixel material indices of the differentiable frame (allocated on first use)
     DevBuf mvp_tab;                  // per-frame VP M table, kMvpStride floats per transform
+    DevBuf pmvp_tab, motion;         // prevVP * previous M table and W*H motion vectors (allocated on first use)
     uint32_t list_capacity = 0;
     // Ring of per-frame event sets: timed frames are averaged at the next
     // read_stats without synchronizing between frames.
@@ -185,6 +192,7 @@ SceneDev scene_dev(const vkgv_ctx* c)
     s.texels = static_cast<const uint32_t*>(c->texels.p);
     s.srgb_decode = static_cast<const float*>(c->srgb_decode.p);
     s.mat_rot = static_cast<const float2*>(c->mat_rot.p);
+    s.prev_transforms = static_cast<const float*>(c->prev_transforms.p ? c->prev_transforms.p : c->transforms.p);
     return s;
 }
 
@@ -246,6 +254,26 @@ int ensure_mvp(vkgv_ctx* c, vkgv_view* v)
     if ((rc = dalloc(c, v->mvp_tab, sizeof(float) * kMvpStride * (size_t)cap))) return rc;
     v->dev.mvp = static_cast<float*>(v->mvp_tab.p);
     v->dev.mvp_capacity = cap;
+    c->generation++;
+    return VKGV_OK;
+}
+
+// Motion-vector buffers of a view: the prevVP * previous M table and the image.
+int ensure_motion(vkgv_ctx* c, vkgv_view* v)
+{
+    if (v->dev.pmvp_capacity >= c->transform_count && v->pmvp_tab.p && v->motion.p) return VKGV_OK;
+    HIP_TRY(c, hipDeviceSynchronize());   // frames of other streams may still read the old buffers
+    const uint32_t cap = std::max<uint32_t>(c->transform_count, 64u);
+    int rc;
+    if ((rc = dalloc(c, v->pmvp_tab, sizeof(float) * 16 * (size_t)cap))) return rc;
+    v->dev.pmvp = static_cast<float*>(v->pmvp_tab.p);
+    v->dev.pmvp_capacity = cap;
+    if (!v->motion.p) {
+        if ((rc = dalloc(c, v->motion, sizeof(float2) * (size_t)v->dev.width * v->dev.height))) return rc;
+        HIP_TRY(c, hipMemsetAsync(v->motion.p, 0, v->motion.bytes, c->stream));
+        HIP_TRY(c, hipStreamSynchronize(c->stream));
+        v->dev.motion = static_cast<float2*>(v->motion.p);
+    }
     c->generation++;
     return VKGV_OK;
 }
@@ -353,6 +381,7 @@ void frame_launches(vkgv_ctx* c, const BatchDev& b, const BatchGrid& g, uint32_t
     const uint32_t tile_grid = (uint32_t)c->cus * 8u;
     mark(EV_START);
     launch_frame_start(s, b, g, st);
+    if (flags & VKGV_MOTION_VECTORS) launch_prev_mvp(s, b, g, st);
     mark(EV_CLEAR);
     launch_cull(s, b, g, flags, two_pass ? CULL_EARLY : CULL_SINGLE, 0, st);
     mark(EV_CULL);
@@ -479,7 +508,10 @@ void vkgv_destroy(vkgv_ctx* c)
     for (auto& b : c->mesh_bufs) dfree(b);
     dfree(c->prims); dfree(c->materials); dfree(c->cone_base); dfree(c->cones);
     dfree(c->draws); dfree(c->cull); dfree(c->rrec); dfree(c->transforms); dfree(c->srgb);
-    dfree(c->srgb_decode); dfree(c->tex_recs); dfree(c->texels); dfree(c->mat_rot);
+    dfree(c->srgb_decode); dfree(c->tex_recs); dfree(c->texels); dfree(c->mat_rot); dfree(c->prev_transforms);
+    for (DevBuf* b : { &c->an_nodes, &c->an_chan_first, &c->an_channels, &c->an_samplers, &c->an_times, &c->an_values,
+                       &c->an_trs, &c->an_chain_first, &c->an_chain })
+        dfree(*b);
     (void)hipStreamDestroy(c->stream);
     delete c;
 }
@@ -686,22 +718,183 @@ int vkgv_set_draws(vkgv_ctx* c, const vkgv_meshlet_draw* draws, uint32_t count)
     return VKGV_OK;
 }
 
+namespace {
+
+// Capacity for `count` transforms in the current and previous buffers; when
+// the count stays, the current transforms become the previous ones (stream
+// ordered on the context's stream).  Reallocation invalidates captured frames.
+int reserve_transforms(vkgv_ctx* c, uint32_t count)
+{
+    const size_t bytes = sizeof(float) * 16 * (size_t)count;
+    if (c->transforms.bytes < bytes || !c->transforms.p || c->prev_transforms.bytes < bytes || !c->prev_transforms.p) {
+        HIP_TRY(c, hipDeviceSynchronize());   // frames of any stream may still read the old buffers
+        int rc;
+        if (c->transforms.bytes < bytes || !c->transforms.p) {
+            if ((rc = dalloc(c, c->transforms, bytes))) return rc;
+            c->transform_count = 0;           // contents lost
+        }
+        if ((rc = dalloc(c, c->prev_transforms, std::max(bytes, c->transforms.bytes)))) return rc;
+        c->prev_count = 0;
+        c->generation++;
+    }
+    if (count == c->transform_count && count == c->prev_count && bytes)
+        HIP_TRY(c, hipMemcpyAsync(c->prev_transforms.p, c->transforms.p, bytes, hipMemcpyDeviceToDevice, c->stream));
+    return VKGV_OK;
+}
+
+// After new transforms are in place: a count change leaves no previous frame,
+// so the previous transforms are the new ones (zero motion).
+int finish_transforms(vkgv_ctx* c, uint32_t count, hipStream_t st)
+{
+    const size_t bytes = sizeof(float) * 16 * (size_t)count;
+    if (!(count == c->transform_count && count == c->prev_count) && bytes)
+        HIP_TRY(c, hipMemcpyAsync(c->prev_transforms.p, c->transforms.p, bytes, hipMemcpyDeviceToDevice, st));
+    if (count != c->transform_count) c->generation++;   // captured batch frames hold the count
+    c->transform_count = count;
+    c->prev_count = count;
+    return VKGV_OK;
+}
+
+} // namespace
+
 int vkgv_set_transforms(vkgv_ctx* c, const float* mat4s, uint32_t count)
 {
     if (!c) return VKGV_ERROR_INVALID_ARGUMENT;
     if (count && !mat4s) return fail(c, VKGV_ERROR_INVALID_ARGUMENT, "set_transforms: null array");
     HIP_TRY(c, hipSetDevice(c->device));
+    int rc = reserve_transforms(c, count);
+    if (rc) return rc;
     const size_t bytes = sizeof(float) * 16 * (size_t)count;
-    if (c->transforms.bytes < bytes || !c->transforms.p) {
-        HIP_TRY(c, hipDeviceSynchronize());   // frames of any stream may still read the old buffer
-        int rc = dalloc(c, c->transforms, bytes);
-        if (rc) return rc;
-        c->generation++;
-    }
     if (bytes) HIP_TRY(c, hipMemcpyAsync(c->transforms.p, mat4s, bytes, hipMemcpyHostToDevice, c->stream));
+    if ((rc = finish_transforms(c, count, c->stream))) return rc;
     HIP_TRY(c, hipStreamSynchronize(c->stream));
-    if (count != c->transform_count) c->generation++;   // captured batch frames hold the count
-    c->transform_count = count;
+    return VKGV_OK;
+}
+
+int vkgv_set_hierarchy(vkgv_ctx* c, const vkgv_node* nodes, uint32_t node_count, const uint32_t* children,
+                       const uint32_t* roots, uint32_t root_count, const vkgv_anim_sampler* samplers,
+                       uint32_t sampler_count, const float* times, uint32_t time_count, const float* values,
+                       uint32_t value_count, const vkgv_anim_channel* channels, uint32_t channel_count)
+{
+    if (!c) return VKGV_ERROR_INVALID_ARGUMENT;
+    if ((node_count && !nodes) || (root_count && !roots) || (sampler_count && !samplers) || (time_count && !times)
+        || (value_count && !values) || (channel_count && !channels))
+        return fail(c, VKGV_ERROR_INVALID_ARGUMENT, "set_hierarchy: null array");
+    // ---- validation: every index a kernel follows stays in bounds
+    for (uint32_t sp = 0; sp < sampler_count; ++sp) {
+        const vkgv_anim_sampler& a = samplers[sp];
+        if (a.input_count == 0 || (uint64_t)a.input_offset + a.input_count > time_count
+            || a.interpolation > VKGV_INTERP_CUBICSPLINE)
+            return fail(c, VKGV_ERROR_INVALID_ARGUMENT, "set_hierarchy: sampler " + std::to_string(sp) + " is malformed");
+    }
+    std::vector<uint32_t> per_node(node_count + 1, 0);
+    for (uint32_t k = 0; k < channel_count; ++k) {
+        const vkgv_anim_channel& ch = channels[k];
+        if (ch.node >= node_count || ch.sampler >= sampler_count || ch.path > VKGV_PATH_SCALE)
+            return fail(c, VKGV_ERROR_INVALID_ARGUMENT, "set_hierarchy: channel " + std::to_string(k) + " is malformed");
+        const vkgv_anim_sampler& a = samplers[ch.sampler];
+        const uint64_t comps = ch.path == VKGV_PATH_ROTATION ? 4u : 3u;
+        const uint64_t outs = (uint64_t)a.input_count * (a.interpolation == VKGV_INTERP_CUBICSPLINE ? 3u : 1u);
+        if ((uint64_t)a.output_offset + outs * comps > value_count)
+            return fail(c, VKGV_ERROR_INVALID_ARGUMENT, "set_hierarchy: channel " + std::to_string(k) + " reads past the values");
+        per_node[ch.node + 1]++;
+    }
+    // channels grouped by node, in array order within a node
+    for (uint32_t n = 0; n < node_count; ++n) per_node[n + 1] += per_node[n];
+    std::vector<vkgv_anim_channel> grouped(std::max<uint32_t>(channel_count, 1u));
+    {
+        std::vector<uint32_t> fill(per_node.begin(), per_node.end() - 1);
+        for (uint32_t k = 0; k < channel_count; ++k) grouped[fill[channels[k].node]++] = channels[k];
+    }
+    // depth-first walk of the roots (World::iterateNode order, as vkgv_world_build):
+    // every mesh node takes the next transform and records its root-to-node chain
+    std::vector<uint32_t> chain_first(1, 0), chain;
+    struct Frame { uint32_t node, depth; };
+    std::vector<Frame> stack;
+    std::vector<uint32_t> path;
+    for (uint32_t r = 0; r < root_count; ++r) {
+        if (roots[r] >= node_count) return fail(c, VKGV_ERROR_INVALID_ARGUMENT, "set_hierarchy: root out of range");
+        stack.push_back({ roots[r], 0 });
+        while (!stack.empty()) {
+            const Frame f = stack.back();
+            stack.pop_back();
+            if (f.depth >= 64) return fail(c, VKGV_ERROR_LIMIT, "set_hierarchy: deeper than 64 nodes (or a cycle)");
+            path.resize(f.depth);
+            path.push_back(f.node);
+            const vkgv_node& n = nodes[f.node];
+            if (n.mesh >= 0) {
+                chain.insert(chain.end(), path.begin(), path.end());
+                chain_first.push_back((uint32_t)chain.size());
+            }
+            if (n.child_count && !children) return fail(c, VKGV_ERROR_INVALID_ARGUMENT, "set_hierarchy: null children");
+            for (uint32_t k = n.child_count; k-- > 0;) {
+                const uint32_t child = children[n.first_child + k];
+                if (child >= node_count) return fail(c, VKGV_ERROR_INVALID_ARGUMENT, "set_hierarchy: child out of range");
+                stack.push_back({ child, f.depth + 1 });
+            }
+        }
+    }
+    const uint32_t mesh_count = (uint32_t)chain_first.size() - 1;
+    HIP_TRY(c, hipSetDevice(c->device));
+    HIP_TRY(c, hipDeviceSynchronize());   // updates of any stream may still read the old arrays
+    c->anim_set = false;
+    int rc;
+    if ((rc = upload(c, c->an_nodes, nodes, sizeof(vkgv_node) * node_count))) return rc;
+    if ((rc = upload(c, c->an_chan_first, per_node.data(), sizeof(uint32_t) * per_node.size()))) return rc;
+    if ((rc = upload(c, c->an_channels, grouped.data(), sizeof(vkgv_anim_channel) * channel_count))) return rc;
+    if ((rc = upload(c, c->an_samplers, samplers, sizeof(vkgv_anim_sampler) * sampler_count))) return rc;
+    if ((rc = upload(c, c->an_times, times, sizeof(float) * time_count))) return rc;
+    if ((rc = upload(c, c->an_values, values, sizeof(float) * value_count))) return rc;
+    if ((rc = dalloc(c, c->an_trs, sizeof(NodeTrs) * std::max<uint32_t>(node_count, 1u)))) return rc;
+    if ((rc = upload(c, c->an_chain_first, chain_first.data(), sizeof(uint32_t) * chain_first.size()))) return rc;
+    if ((rc = upload(c, c->an_chain, chain.data(), sizeof(uint32_t) * chain.size()))) return rc;
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    AnimDev& d = c->anim;
+    d.nodes = static_cast<const vkgv_node*>(c->an_nodes.p);
+    d.node_count = node_count;
+    d.chan_first = static_cast<const uint32_t*>(c->an_chan_first.p);
+    d.channels = static_cast<const vkgv_anim_channel*>(c->an_channels.p);
+    d.samplers = static_cast<const vkgv_anim_sampler*>(c->an_samplers.p);
+    d.times = static_cast<const float*>(c->an_times.p);
+    d.values = static_cast<const float*>(c->an_values.p);
+    d.trs = static_cast<NodeTrs*>(c->an_trs.p);
+    d.mesh_count = mesh_count;
+    d.chain_first = static_cast<const uint32_t*>(c->an_chain_first.p);
+    d.chain_nodes = static_cast<const uint32_t*>(c->an_chain.p);
+    // the transform buffer holds the mesh nodes' matrices at time 0 (no previous frame yet)
+    if ((rc = reserve_transforms(c, mesh_count))) return rc;
+    c->prev_count = 0;
+    launch_update_transforms(d, 0.0f, static_cast<float*>(c->transforms.p), c->stream);
+    HIP_TRY(c, hipGetLastError());
+    if ((rc = finish_transforms(c, mesh_count, c->stream))) return rc;
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    c->anim_set = true;
+    return VKGV_OK;
+}
+
+int vkgv_update_transforms(vkgv_ctx* c, float time, void* stream)
+{
+    if (!c) return VKGV_ERROR_INVALID_ARGUMENT;
+    if (!c->anim_set) return fail(c, VKGV_ERROR_STATE, "update_transforms: no hierarchy set");
+    if (c->transform_count != c->anim.mesh_count || c->prev_count != c->anim.mesh_count)
+        return fail(c, VKGV_ERROR_STATE, "update_transforms: the transforms were replaced since set_hierarchy");
+    HIP_TRY(c, hipSetDevice(c->device));
+    hipStream_t st = stream ? static_cast<hipStream_t>(stream) : c->stream;
+    const size_t bytes = sizeof(float) * 16 * (size_t)c->anim.mesh_count;
+    if (bytes) HIP_TRY(c, hipMemcpyAsync(c->prev_transforms.p, c->transforms.p, bytes, hipMemcpyDeviceToDevice, st));
+    launch_update_transforms(c->anim, time, static_cast<float*>(c->transforms.p), st);
+    HIP_TRY(c, hipGetLastError());
+    return VKGV_OK;
+}
+
+int vkgv_read_transforms(vkgv_ctx* c, float* mat4s, uint32_t count)
+{
+    if (!c) return VKGV_ERROR_INVALID_ARGUMENT;
+    if (count > c->transform_count) return fail(c, VKGV_ERROR_INVALID_ARGUMENT, "read_transforms: more than are set");
+    if (count && !mat4s) return fail(c, VKGV_ERROR_INVALID_ARGUMENT, "read_transforms: null destination");
+    HIP_TRY(c, hipSetDevice(c->device));
+    HIP_TRY(c, hipDeviceSynchronize());   // updates may run on any stream
+    if (count) HIP_TRY(c, hipMemcpy(mat4s, c->transforms.p, sizeof(float) * 16 * (size_t)count, hipMemcpyDeviceToHost));
     return VKGV_OK;
 }
 
@@ -814,7 +1007,7 @@ void vkgv_view_destroy(vkgv_view* v)
     for (DevBuf* b : { &v->keys, &v->color, &v->pyramid, &v->visible0, &v->visible1, &v->history, &v->counters, &v->big,
                        &v->clipq, &v->bin_count, &v->bin_start, &v->bins, &v->setups, &v->pairs, &v->tri_cache,
                        &v->slot_albedo, &v->draw_slot, &v->big_chunk, &v->chunk_tri, &v->tile_items, &v->maps,
-                       &v->aa_mat, &v->mvp_tab })
+                       &v->aa_mat, &v->mvp_tab, &v->pmvp_tab, &v->motion })
         dfree(*b);
     for (auto& set : v->ev)
         for (auto& e : set)
@@ -831,6 +1024,7 @@ int vkgv_render(vkgv_ctx* c, vkgv_view* v, const vkgv_frame_desc* f)
     if ((rc = check_frame_inputs(c, flags))) return rc;
     HIP_TRY(c, hipSetDevice(c->device));
     if ((rc = ensure_lists(c, v))) return rc;
+    if ((flags & VKGV_MOTION_VECTORS) && (rc = ensure_motion(c, v))) return rc;
     hipStream_t st = f->stream ? static_cast<hipStream_t>(f->stream) : c->stream;
     if (v->last_stream && v->last_stream != st) HIP_TRY(c, hipStreamSynchronize(v->last_stream));
     v->last_stream = st;
@@ -980,6 +1174,13 @@ int vkgv_view_read_color(vkgv_ctx* c, vkgv_view* v, uint32_t* rgba8)
 {
     if (check_view(c, v)) return check_view(c, v);
     return copy_out(c, v, rgba8, v->color.p, 4ull * v->dev.width * v->dev.height);
+}
+
+int vkgv_view_read_motion(vkgv_ctx* c, vkgv_view* v, float* motion)
+{
+    if (check_view(c, v)) return check_view(c, v);
+    if (!v->motion.p) return fail(c, VKGV_ERROR_STATE, "read_motion: no frame rendered with VKGV_MOTION_VECTORS");
+    return copy_out(c, v, motion, v->motion.p, 8ull * v->dev.width * v->dev.height);
 }
 
 int vkgv_view_read_pyramid(vkgv_ctx* c, vkgv_view* v, float* texels)
@@ -1159,8 +1360,10 @@ int vkgv_render_batch(vkgv_ctx* c, vkgv_batch* b, const vkgv_frame_desc* frames,
     int rc = check_frame_inputs(c, flags);
     if (rc) return rc;
     HIP_TRY(c, hipSetDevice(c->device));
-    for (vkgv_view* v : b->views)
+    for (vkgv_view* v : b->views) {
         if ((rc = ensure_lists(c, v))) return rc;
+        if ((flags & VKGV_MOTION_VECTORS) && (rc = ensure_motion(c, v))) return rc;
+    }
     if (b->generation != c->generation) {
         std::vector<const ViewDev*> devs;
         for (vkgv_view* v : b->views) devs.push_back(&v->dev);

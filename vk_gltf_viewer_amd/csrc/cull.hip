@@ -208,6 +208,22 @@ __global__ __launch_bounds__(kBlock) void frame_start_kernel(SceneDev s, BatchDe
     }
 }
 
+// prevViewProjection * previous M of every transform, per view (the mesh
+// shader's prevMvp, visbuffer.mesh.glsl:45, with the previous frame's
+// transform where the reference reuses the current one).
+__global__ __launch_bounds__(kBlock) void prev_mvp_kernel(SceneDev s, BatchDev b)
+{
+    const uint32_t vi = blockIdx.y;
+    const ViewDev& v = batch_view(b, vi);
+    const uint32_t t = blockIdx.x * kBlock + threadIdx.x;
+    if (t >= s.transform_count || t >= v.pmvp_capacity) return;
+    const Mat4 pvp = load_mat4(batch_camera(b, vi).cam.prev_view_projection);
+    const Mat4 m = mat4_mul(pvp, load_mat4(s.prev_transforms + 16ull * t));
+    float4* o = reinterpret_cast<float4*>(v.pmvp + 16ull * t);
+#pragma unroll
+    for (int c = 0; c < 4; ++c) o[c] = make_float4(m.m[4 * c], m.m[4 * c + 1], m.m[4 * c + 2], m.m[4 * c + 3]);
+}
+
 // Cull and raster records from the contract arrays: draw -> primitive ->
 // meshlet AABB, buffer offsets, counts and material.
 __global__ __launch_bounds__(kBlock) void build_cull_records_kernel(SceneDev s, CullRec* out, RasterRec* rrec)
@@ -271,6 +287,12 @@ void launch_clear_keys(unsigned long long* keys, uint64_t n, hipStream_t st)
 void launch_frame_start(const SceneDev& s, const BatchDev& b, const BatchGrid& g, hipStream_t st)
 {
     frame_start_kernel<<<g.clear_blocks, kBlock, 0, st>>>(s, b);
+}
+
+void launch_prev_mvp(const SceneDev& s, const BatchDev& b, const BatchGrid& g, hipStream_t st)
+{
+    if (!s.transform_count) return;
+    prev_mvp_kernel<<<dim3((s.transform_count + kBlock - 1) / kBlock, g.nviews), kBlock, 0, st>>>(s, b);
 }
 
 void launch_cull(const SceneDev& s, const BatchDev& b, const BatchGrid& g, uint32_t flags, CullMode mode,

@@ -38,6 +38,24 @@ struct RasterRec {
 static_assert(sizeof(RasterRec) == 32, "RasterRec layout");
 
 // Device view of the uploaded scene.
+// Animated TRS of one node for the device transform update.
+struct NodeTrs { float t[3], q[4], s[3]; };
+
+// Node hierarchy and animations on the device (vkgv_set_hierarchy).
+struct AnimDev {
+    const vkgv_node* nodes;
+    uint32_t node_count;
+    const uint32_t* chan_first;          // node_count + 1: the node's channels, in application order
+    const vkgv_anim_channel* channels;
+    const vkgv_anim_sampler* samplers;
+    const float* times;
+    const float* values;
+    NodeTrs* trs;                        // per node, this update
+    uint32_t mesh_count;                 // mesh nodes = transforms
+    const uint32_t* chain_first;         // mesh_count + 1
+    const uint32_t* chain_nodes;         // root-to-node chain of each mesh node
+};
+
 // One albedo texture: `width * height` RGBA8 words from texels + offset.
 struct TexRec {
     uint64_t offset;
@@ -73,6 +91,8 @@ struct SceneDev {
     const uint32_t* texels;
     const float* srgb_decode;
     const float2* mat_rot;
+    // the transforms of the previous update (motion vectors)
+    const float* prev_transforms;
 };
 
 // The 8-bit code of fromLinear(c) is the number of the 255 ascending
@@ -197,6 +217,9 @@ struct ViewDev {
     uint32_t cache_slots;
     float* mvp;                     // per transform, this frame: VP M (16 floats, column-major), det(M) at [16]
     uint32_t mvp_capacity;          // transforms the table holds
+    float* pmvp;                    // per transform: prevVP * previous M (motion vectors), 16 floats
+    uint32_t pmvp_capacity;
+    float2* motion;                 // W*H motion vectors (VKGV_MOTION_VECTORS)
     uint32_t hiz_tile_levels;       // pyramid levels produced by the tile pass of the Hi-Z build
     uint32_t hiz_tail_start;        // first level of the one-block tail; levels in between: hiz_level_kernel
     uint32_t hiz_tiles_x;           // 32x32 level-0 tiles per row
@@ -253,11 +276,15 @@ struct BatchGrid {
 enum CullMode : uint32_t { CULL_SINGLE = 0, CULL_EARLY = 1, CULL_LATE = 2 };
 
 // ---- launchers (stream-ordered, no host synchronization) -----------------
+// Animation sampling and world matrices of the mesh nodes into `transforms`.
+void launch_update_transforms(const AnimDev& d, float time, float* transforms, hipStream_t st);
 void launch_build_cull_records(const SceneDev& s, CullRec* out, RasterRec* rrec, hipStream_t st);
 void launch_clear_keys(unsigned long long* keys, uint64_t n, hipStream_t st);
 // Frame start of every view of the batch: all counters zeroed, keys cleared,
 // the view's table of VP M and det(M) per transform computed.
 void launch_frame_start(const SceneDev& s, const BatchDev& b, const BatchGrid& g, hipStream_t st);
+// Every view's prevViewProjection * previous transform table (motion vectors).
+void launch_prev_mvp(const SceneDev& s, const BatchDev& b, const BatchGrid& g, hipStream_t st);
 void launch_cull(const SceneDev& s, const BatchDev& b, const BatchGrid& g, uint32_t flags, CullMode mode,
                  uint32_t list, hipStream_t st);
 void launch_raster(const SceneDev& s, const BatchDev& b, uint32_t flags, uint32_t list, uint32_t grid_blocks,

@@ -256,7 +256,7 @@ __device__ __forceinline__ void raster_view(const RasterArgs& a)
         const float tdet = mrow[16];
         const bool cull_faces = !(matw >> 31) && !(a.flags & VKGV_NO_BACKFACE);
         // the flat resolve reads only the slot albedo; the interpolating ones the triangles too
-        const bool bary = (a.flags & (VKGV_RESOLVE_BARY | VKGV_RESOLVE_TEXTURED)) != 0;
+        const bool bary = (a.flags & (VKGV_RESOLVE_BARY | VKGV_RESOLVE_TEXTURED | VKGV_MOTION_VECTORS)) != 0;
         const uint32_t vc = counts & 0xFFu, tc = (counts >> 8) & 0xFFu;
         // resolve-cache slot of this meshlet (none when the cache is full)
         const uint32_t slot = slot0 + idx;

@@ -21,6 +21,8 @@
 // transformUv (mesh_common.h.glsl:129-135: rotation, then scale, then
 // offset), and the texture is sampled bilinearly at level 0 with per-texel
 // sRGB decode; colour = vertex colour * albedo factor * texture.
+//
+// VKGV_MOTION_VECTORS writes each pixel's motion vector beside the colour.
 #include "internal.hpp"
 
 namespace vkgv {
@@ -178,6 +180,32 @@ __device__ __forceinline__ void apply_texture(const SceneDev& s, uint32_t d, uin
     }
 }
 
+// Motion vector of a pixel (visbuffer.frag.glsl:38): the triangle's vertices
+// through prevViewProjection * previous M, interpolated with the pixel's
+// perspective-correct barycentrics l[], against the pixel centre's NDC.
+__device__ __forceinline__ float2 motion_vector(const SceneDev& s, const float* pmvp, uint32_t d, uint32_t t,
+                                                const float* l, float xn, float yn)
+{
+    const vkgv_meshlet_draw dr = s.draws[d];
+    const vkgv_primitive& prim = s.prims[dr.primitive_index];
+    const vkgv_meshlet ml = reinterpret_cast<const vkgv_meshlet*>(prim.meshlet_buffer)[dr.meshlet_index];
+    const uint8_t* tri = reinterpret_cast<const uint8_t*>(prim.primitive_index_buffer) + ml.triangle_offset + 3u * t;
+    const uint32_t* vidx = reinterpret_cast<const uint32_t*>(prim.vertex_index_buffer) + ml.vertex_offset;
+    const vkgv_vertex* verts = reinterpret_cast<const vkgv_vertex*>(prim.vertex_buffer);
+    const Mat4 pm = load_mat4(pmvp + 16ull * dr.transform_index);
+    float4 q[3];
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        const vkgv_vertex& vx = verts[vidx[tri[k]]];
+        q[k] = mat4_point(pm, vx.position[0], vx.position[1], vx.position[2]);
+    }
+    const float px = (l[0] * q[0].x + l[1] * q[1].x) + l[2] * q[2].x;
+    const float py = (l[0] * q[0].y + l[1] * q[1].y) + l[2] * q[2].y;
+    const float pw = (l[0] * q[0].w + l[1] * q[1].w) + l[2] * q[2].w;
+    return make_float2(((px / pw) * 0.5f - 0.5f) - (xn * 0.5f - 0.5f),
+                       ((py / pw) * 0.5f - 0.5f) - (yn * 0.5f - 0.5f));
+}
+
 // Block = 64 x 16 pixels; thread (x, y) resolves rows y, y + 4, y + 8, y + 12
 // of its column, with the loads of the four pixels issued together.
 constexpr int kRows = 4;
@@ -205,6 +233,8 @@ __global__ __launch_bounds__(256) void resolve_kernel(ResolveBatchArgs ab)
     a.tri_cache = view.tri_cache;
     a.slot_albedo = view.slot_albedo;
     a.mvp = view.mvp;
+    float2* const view_motion = view.motion;
+    const float* const view_pmvp = view.pmvp;
     const uint32_t blk_x = local % view.resolve_blocks_x, blk_y = local / view.resolve_blocks_x;
 
     // LDS tables: sRGB thresholds and buckets, exact quotients b / 255 of a byte
@@ -226,7 +256,9 @@ __global__ __launch_bounds__(256) void resolve_kernel(ResolveBatchArgs ab)
     const uint32_t px = blk_x * 64u + (threadIdx.x & 63u);
     if (px >= a.W) return;
     const uint32_t py0 = blk_y * (4u * kRows) + (threadIdx.x >> 6);
-    const bool bary = (a.flags & (VKGV_RESOLVE_BARY | VKGV_RESOLVE_TEXTURED)) != 0;
+    const bool shade = (a.flags & (VKGV_RESOLVE_BARY | VKGV_RESOLVE_TEXTURED)) != 0;   // interpolated colour
+    const bool mv = (a.flags & VKGV_MOTION_VECTORS) != 0;
+    const bool bary = shade || mv;                                                     // barycentrics needed
     const unsigned long long* __restrict__ keys = a.keys;
     const uint32_t* __restrict__ draw_slot = a.draw_slot;
 
@@ -254,7 +286,11 @@ __global__ __launch_bounds__(256) void resolve_kernel(ResolveBatchArgs ab)
         const uint32_t py = py0 + 4u * k;
         if (py >= a.H) break;
         const size_t i = (size_t)py * a.W + px;
-        if (drw[k] == 0xFFFFFFFFu) { a.color[i] = 0u; continue; }
+        if (drw[k] == 0xFFFFFFFFu) {
+            a.color[i] = 0u;
+            if (mv) view_motion[i] = make_float2(0.0f, 0.0f);
+            continue;
+        }
         const uint32_t t = key_id(key[k]) & ((1u << VKGV_TRIANGLE_BITS) - 1u);
         float c[4];
         float3 r[3];
@@ -284,14 +320,17 @@ __global__ __launch_bounds__(256) void resolve_kernel(ResolveBatchArgs ab)
             const float inv = 1.0f / ((l[0] + l[1]) + l[2]);
 #pragma unroll
             for (int q = 0; q < 3; ++q) l[q] = l[q] * inv;
+            if (shade) {
 #pragma unroll
-            for (int j = 0; j < 4; ++j) {
-                const float c0 = s_byte[(col[0] >> (8 * j)) & 0xFFu];
-                const float c1 = s_byte[(col[1] >> (8 * j)) & 0xFFu];
-                const float c2 = s_byte[(col[2] >> (8 * j)) & 0xFFu];
-                c[j] = c[j] * ((l[0] * c0 + l[1] * c1) + l[2] * c2);
+                for (int j = 0; j < 4; ++j) {
+                    const float c0 = s_byte[(col[0] >> (8 * j)) & 0xFFu];
+                    const float c1 = s_byte[(col[1] >> (8 * j)) & 0xFFu];
+                    const float c2 = s_byte[(col[2] >> (8 * j)) & 0xFFu];
+                    c[j] = c[j] * ((l[0] * c0 + l[1] * c1) + l[2] * c2);
+                }
             }
             if (tex) apply_texture(a.s, drw[k], t, l, s_lin, s_byte, c);
+            if (mv) view_motion[i] = motion_vector(a.s, view_pmvp, drw[k], t, l, xn, yn);
         }
         const uint32_t r8 = srgb_code(s_thr, base, key0, c[0]);
         const uint32_t g8 = srgb_code(s_thr, base, key0, c[1]);

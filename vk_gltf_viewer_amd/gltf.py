@@ -49,7 +49,8 @@ import numpy as np
 
 from . import _native as N
 from . import meshopt
-from .scene import MATERIAL_DTYPE, PRIM_DTYPE, DRAW_DTYPE, SceneArrays, Texture, build_meshlets
+from .scene import (ANIM_CHANNEL_DTYPE, ANIM_SAMPLER_DTYPE, DRAW_DTYPE, MATERIAL_DTYPE, NODE_DTYPE, PRIM_DTYPE,
+                    Hierarchy, SceneArrays, Texture, build_meshlets)
 
 _COMPONENT = {5120: np.int8, 5121: np.uint8, 5122: np.int16, 5123: np.uint16, 5125: np.uint32, 5126: np.float32}
 _COUNT = {"SCALAR": 1, "VEC2": 2, "VEC3": 3, "VEC4": 4, "MAT2": 4, "MAT3": 9, "MAT4": 16}
@@ -347,13 +348,12 @@ def load_gltf(path: str, scene: int | None = None, textures: bool = True) -> Sce
     prims_arr = np.array(prims, dtype=PRIM_DTYPE) if prims else np.zeros(0, PRIM_DTYPE)
     # nodes of the scene -> draws and transforms (World traversal)
     gnodes = js.get("nodes", [])
-    nodes = (N.Node * max(1, len(gnodes)))()
+    nodes = np.zeros(len(gnodes), NODE_DTYPE)
     children = []
     for i, n in enumerate(gnodes):
-        nd = nodes[i]
-        nd.mesh = n.get("mesh", -1)
         kids = n.get("children", [])
-        nd.first_child, nd.child_count = len(children), len(kids)
+        nodes[i]["mesh"] = n.get("mesh", -1)
+        nodes[i]["first_child"], nodes[i]["child_count"] = len(children), len(kids)
         children += kids
         if "matrix" in n:
             t, r, s = decompose(n["matrix"])
@@ -361,15 +361,15 @@ def load_gltf(path: str, scene: int | None = None, textures: bool = True) -> Sce
             t = n.get("translation", [0.0, 0.0, 0.0])
             r = n.get("rotation", [0.0, 0.0, 0.0, 1.0])
             s = n.get("scale", [1.0, 1.0, 1.0])
-        nd.translation[:] = [float(x) for x in t]
-        nd.rotation[:] = [float(x) for x in r]
-        nd.scale[:] = [float(x) for x in s]
+        nodes[i]["translation"], nodes[i]["rotation"], nodes[i]["scale"] = t, r, s
     scenes = js.get("scenes", [])
     si = scene if scene is not None else js.get("scene", 0)
-    roots = scenes[si]["nodes"] if scenes else list(range(len(gnodes)))
-    draws, transforms = _world(nodes, len(gnodes), children, roots, mesh_prims, prim_list,
-                               [len(m.meshlets) for m in meshes])
-    return SceneArrays(meshes, prims_arr, mats, draws, transforms, textures=tex_list)
+    # without a scene the reference draws nothing (world.cpp:233-234)
+    roots = scenes[si].get("nodes", []) if scenes else []
+    draws, transforms = _world(nodes, children, roots, mesh_prims, prim_list, [len(m.meshlets) for m in meshes])
+    hier = Hierarchy(nodes, np.asarray(children, np.uint32), np.asarray(roots, np.uint32),
+                     *_animations(asset, len(gnodes)))
+    return SceneArrays(meshes, prims_arr, mats, draws, transforms, textures=tex_list, hierarchy=hier)
 
 
 def _uv_set(js, gp):
@@ -396,14 +396,58 @@ def asset_normalize(acc, v):
     return v
 
 
-def _world(nodes, node_count, children, roots, mesh_prims, prim_list, meshlet_counts):
+_INTERP = {"LINEAR": 0, "STEP": 1, "CUBICSPLINE": 2}
+_PATH = {"translation": 0, "rotation": 1, "scale": 2}
+
+
+def _animations(asset, node_count):
+    """All animations' samplers and channels, flattened in animation order
+    (assets.cpp:579-600): keyframe times and output values as floats
+    (normalized integer outputs decoded), channels of unsupported paths
+    (morph-target weights) dropped as the reference does (world.cpp:214-215)."""
+    js = asset.js
+    samplers, channels, times, values = [], [], [], []
+    nt = nv = 0
+    for anim in js.get("animations", []):
+        base = len(samplers)
+        for sp in anim.get("samplers", []):
+            t = asset.accessor(sp["input"], True).ravel()
+            v = asset.accessor(sp["output"], True).ravel()
+            if len(t) == 0:
+                raise GltfError("animation sampler without keyframes")
+            interp = _INTERP.get(sp.get("interpolation", "LINEAR"))
+            if interp is None:
+                raise GltfError(f"unknown interpolation {sp.get('interpolation')}")
+            samplers.append((nt, len(t), nv, interp))
+            times.append(t)
+            values.append(v)
+            nt += len(t)
+            nv += len(v)
+        for ch in anim.get("channels", []):
+            path = _PATH.get(ch["target"].get("path"))
+            node = ch["target"].get("node")
+            if path is None or node is None:
+                continue
+            if node >= node_count:
+                raise GltfError("animation channel targets a missing node")
+            channels.append((node, base + ch["sampler"], path, 0))
+    sp = np.array(samplers, dtype=np.uint32).reshape(-1, 4).view(ANIM_SAMPLER_DTYPE).ravel()
+    cn = np.array(channels, dtype=np.uint32).reshape(-1, 4).view(ANIM_CHANNEL_DTYPE).ravel()
+    tm = np.concatenate(times).astype(np.float32) if times else np.zeros(0, np.float32)
+    vl = np.concatenate(values).astype(np.float32) if values else np.zeros(0, np.float32)
+    return sp, tm, vl, cn
+
+
+def _world(nodes, children, roots, mesh_prims, prim_list, meshlet_counts):
     L = N.host()
+    nd_arr = np.ascontiguousarray(nodes if len(nodes) else np.zeros(1, NODE_DTYPE))
     ch = np.ascontiguousarray(children if children else [0], np.uint32)
     rt = np.ascontiguousarray(roots if roots else [0], np.uint32)
     mp = np.ascontiguousarray(mesh_prims if mesh_prims else [0, 0], np.uint32)
     pl = np.ascontiguousarray(prim_list if prim_list else [0], np.uint32)
     mc = np.ascontiguousarray(meshlet_counts if meshlet_counts else [0], np.uint32)
-    args = [nodes, node_count, ch.ctypes.data, rt.ctypes.data, len(roots), mp.ctypes.data, pl.ctypes.data, mc.ctypes.data]
+    args = [nd_arr.ctypes.data_as(C.POINTER(N.Node)), len(nodes), ch.ctypes.data, rt.ctypes.data, len(roots),
+            mp.ctypes.data, pl.ctypes.data, mc.ctypes.data]
     nd, nt = C.c_uint32(0), C.c_uint32(0)
     if L.vkgv_world_build(*args, None, C.byref(nd), None, C.byref(nt)) != 0:
         raise GltfError("World traversal failed: " + L.vkgv_scene_error().decode())

@@ -94,6 +94,27 @@ class Renderer:
         self.set_transforms(sc.transforms)
         self.set_draws(sc.draws)
 
+    def set_hierarchy(self, h):
+        """Node tree and animations (scene.Hierarchy) for update_transforms; the
+        context's transforms become the mesh nodes' matrices at time 0."""
+        nodes, samplers, channels = (np.ascontiguousarray(a) for a in (h.nodes, h.samplers, h.channels))
+        children, roots = (np.ascontiguousarray(a, dtype=np.uint32) for a in (h.children, h.roots))
+        times, values = (np.ascontiguousarray(a, dtype=np.float32) for a in (h.times, h.values))
+        ptr = lambda a: a.ctypes.data if a.size else None
+        self._check(self._lib.vkgv_set_hierarchy(
+            self._h, ptr(nodes), len(nodes), ptr(children), ptr(roots), len(roots), ptr(samplers), len(samplers),
+            ptr(times), len(times), ptr(values), len(values), ptr(channels),
+            len(channels)))
+
+    def update_transforms(self, time: float, stream=None):
+        """Animation sampling and world matrices on the device (stream-ordered)."""
+        self._check(self._lib.vkgv_update_transforms(self._h, float(time), stream))
+
+    def read_transforms(self, count: int) -> np.ndarray:
+        out = np.empty((count, 16), dtype=np.float32)
+        self._check(self._lib.vkgv_read_transforms(self._h, out.ctypes.data, count))
+        return out
+
     def sync(self):
         self._check(self._lib.vkgv_sync(self._h))
 
@@ -243,6 +264,12 @@ class View:
         dep = np.empty(self.width * self.height, dtype=np.float32)
         self.r._check(self.r._lib.vkgv_view_read_ids_depth(self.r._h, self._h, ids.ctypes.data, dep.ctypes.data))
         return ids.reshape(self.height, self.width), dep.reshape(self.height, self.width)
+
+    def motion(self) -> np.ndarray:
+        """Motion vectors of the last VKGV_MOTION_VECTORS frame, (H, W, 2) float32."""
+        out = np.empty(self.width * self.height * 2, dtype=np.float32)
+        self.r._check(self.r._lib.vkgv_view_read_motion(self.r._h, self._h, out.ctypes.data))
+        return out.reshape(self.height, self.width, 2)
 
     def color(self) -> np.ndarray:
         out = np.empty(self.width * self.height, dtype=np.uint32)

@@ -36,6 +36,15 @@ PRIM_DTYPE = np.dtype({"names": ["mesh_index", "material_index", "aabb_extents",
                        "offsets": [0, 4, 8, 20], "itemsize": 32})
 
 
+NODE_DTYPE = np.dtype({"names": ["mesh", "first_child", "child_count", "pad", "translation", "rotation", "scale"],
+                       "formats": ["<i4", "<u4", "<u4", "<u4", ("<f4", 3), ("<f4", 4), ("<f4", 3)],
+                       "offsets": [0, 4, 8, 12, 16, 28, 44], "itemsize": 56})
+ANIM_SAMPLER_DTYPE = np.dtype({"names": ["input_offset", "input_count", "output_offset", "interpolation"],
+                               "formats": ["<u4"] * 4, "offsets": [0, 4, 8, 12], "itemsize": 16})
+ANIM_CHANNEL_DTYPE = np.dtype({"names": ["node", "sampler", "path", "pad"],
+                               "formats": ["<u4"] * 4, "offsets": [0, 4, 8, 12], "itemsize": 16})
+
+
 def _view(ptr, count, dtype):
     if count == 0:
         return np.zeros(0, dtype=dtype)
@@ -74,6 +83,27 @@ def texture_descs(textures):
 
 
 @dataclass
+class Hierarchy:
+    """glTF nodes of a scene and their animations, for the device transform
+    update (vkgv_set_hierarchy / vkgv_update_transforms)."""
+    nodes: np.ndarray                                   # NODE_DTYPE
+    children: np.ndarray                                # u32
+    roots: np.ndarray                                   # u32
+    samplers: np.ndarray = field(default_factory=lambda: np.zeros(0, ANIM_SAMPLER_DTYPE))
+    times: np.ndarray = field(default_factory=lambda: np.zeros(0, np.float32))
+    values: np.ndarray = field(default_factory=lambda: np.zeros(0, np.float32))
+    channels: np.ndarray = field(default_factory=lambda: np.zeros(0, ANIM_CHANNEL_DTYPE))
+
+    @property
+    def duration(self) -> float:
+        """Longest sampler period (the last key time)."""
+        if not len(self.samplers):
+            return 0.0
+        last = self.samplers["input_offset"].astype(np.int64) + self.samplers["input_count"] - 1
+        return float(self.times[last].max())
+
+
+@dataclass
 class SceneArrays:
     meshes: list[MeshArrays]
     primitives: np.ndarray          # PRIM_DTYPE
@@ -82,6 +112,7 @@ class SceneArrays:
     transforms: np.ndarray          # (T, 16) f32, column-major
     views: list = field(default_factory=list)
     textures: list = field(default_factory=list)   # Texture, indexed by the materials' albedo_index
+    hierarchy: Hierarchy | None = None              # node tree + animations (glTF scenes)
 
     @property
     def triangle_count(self) -> int:
