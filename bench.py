@@ -15,6 +15,7 @@ region.
 from __future__ import annotations
 
 import argparse
+import dataclasses
 import json
 import os
 import sys
@@ -27,7 +28,7 @@ sys.path.insert(0, ROOT)
 
 from vk_gltf_viewer_amd import _native as N  # noqa: E402
 from vk_gltf_viewer_amd.renderer import Renderer  # noqa: E402
-from vk_gltf_viewer_amd.scene import Scene, camera_from_view  # noqa: E402
+from vk_gltf_viewer_amd.scene import Scene, Texture, camera_from_view  # noqa: E402
 
 HBM_PEAK_GBS = 8000.0
 
@@ -71,8 +72,10 @@ def measured_traffic(pass_name, W, H):
     total = 0.0
     for c in ("FETCH_SIZE", "WRITE_SIZE"):
         rec = t[c]
-        for k in PASS_KERNELS[pass_name]:
-            total += rec["per_kernel_kib"].get(k, 0.0) * rec["factor"] * 1024.0
+        for name, kib in rec["per_kernel_kib"].items():
+            # template instantiations (resolve_kernel<...>) count as their kernel
+            if name.split("<")[0] in PASS_KERNELS[pass_name]:
+                total += kib * rec["factor"] * 1024.0
     return round(total)
 
 
@@ -254,6 +257,117 @@ def differentiable(steps, warmup, precisions=("mixed", "float32")):
     return out
 
 
+def synthetic_hierarchy(nodes, seed=0, animated_frac=0.5):
+    """A random forest of `nodes` glTF nodes (every node a mesh node), half of
+    them driven by 1-3 channels of random path and interpolation with 2-16
+    keyframes: the input of the device transform update."""
+    from vk_gltf_viewer_amd.scene import ANIM_CHANNEL_DTYPE, ANIM_SAMPLER_DTYPE, NODE_DTYPE, Hierarchy
+    rng = np.random.default_rng(seed)
+    roots = max(1, nodes // 64)
+    parent = np.full(nodes, -1, np.int64)
+    parent[roots:] = (rng.random(nodes - roots) * np.arange(roots, nodes)).astype(np.int64)
+    order = np.argsort(parent, kind="stable")                 # children grouped by parent, in index order
+    kids = order[parent[order] >= 0]
+    counts = np.bincount(parent[parent >= 0], minlength=nodes)
+    first = np.concatenate([[0], np.cumsum(counts)[:-1]])
+    nd = np.zeros(nodes, NODE_DTYPE)
+    nd["mesh"] = 0
+    nd["first_child"], nd["child_count"] = first, counts
+    nd["translation"] = rng.normal(0, 1, (nodes, 3))
+    q = rng.normal(size=(nodes, 4))
+    nd["rotation"] = q / np.linalg.norm(q, axis=1, keepdims=True)
+    nd["scale"] = rng.uniform(0.8, 1.2, (nodes, 3))
+    anim = np.nonzero(rng.random(nodes) < animated_frac)[0]
+    per = rng.integers(1, 4, len(anim))
+    ch_node = np.repeat(anim, per)
+    n = len(ch_node)
+    path = rng.integers(0, 3, n)
+    interp = rng.integers(0, 3, n)
+    keys = rng.integers(2, 17, n)
+    outs = keys * np.where(interp == 2, 3, 1)
+    comps = np.where(path == 1, 4, 3)
+    t_off = np.concatenate([[0], np.cumsum(keys)[:-1]])
+    v_off = np.concatenate([[0], np.cumsum(outs * comps)[:-1]])
+    times = np.sort(rng.uniform(0, 4, int(keys.sum())).astype(np.float32))
+    # ascending keys within each sampler: split one sorted pool, then re-sort per segment
+    times = np.concatenate([np.sort(times[o:o + k]) for o, k in zip(t_off, keys)]).astype(np.float32)
+    values = rng.normal(0, 1, int((outs * comps).sum())).astype(np.float32)
+    sp = np.zeros(n, ANIM_SAMPLER_DTYPE)
+    sp["input_offset"], sp["input_count"], sp["output_offset"], sp["interpolation"] = t_off, keys, v_off, interp
+    ch = np.zeros(n, ANIM_CHANNEL_DTYPE)
+    ch["node"], ch["sampler"], ch["path"] = ch_node, np.arange(n), path
+    return Hierarchy(nd, kids.astype(np.uint32), np.arange(roots, dtype=np.uint32), sp, times, values, ch)
+
+
+def widened_rows(sc, cam, W, H, steps, warmup, log):
+    """The rows beyond the frame: the textured resolve and motion vectors on
+    the same 4K workload, and the device transform update of a 262144-node
+    animated hierarchy."""
+    out = {}
+    sc2 = dataclasses.replace(sc)
+    ntex, tsz = 16, 1024
+    rng = np.random.default_rng(7)
+    sc2.textures = [Texture(rng.integers(0, 256, (tsz, tsz, 4), dtype=np.uint8), N.WRAP_REPEAT, N.WRAP_REPEAT)
+                    for _ in range(ntex)]
+    sc2.materials = sc.materials.copy()
+    sc2.materials["albedo_index"] = np.arange(len(sc2.materials)) % ntex
+    sc2.meshes = []
+    for m in sc.meshes:
+        v = m.vertices.copy()
+        p = v["position"]
+        v["uv"] = np.stack([p[:, 0] * 0.5, p[:, 2] * 0.5 + p[:, 1] * 0.25], 1).astype(np.float16).view(np.uint16)
+        sc2.meshes.append(dataclasses.replace(m, vertices=v))
+    r = Renderer(0)
+    r.load(sc2)
+    view = r.create_view(W, H)
+    base = N.CULL_FRUSTUM | N.CULL_TWO_PASS
+    npx = W * H
+    res = {}
+    for name, fl in (("bary", base | N.RESOLVE_BARY), ("textured", base | N.RESOLVE_TEXTURED),
+                     ("textured_motion", base | N.RESOLVE_TEXTURED | N.MOTION_VECTORS)):
+        for _ in range(max(1, warmup)):
+            view.render(cam, fl)
+        r.sync()
+        view.stats()
+        for _ in range(steps):
+            view.render(cam, fl | N.TIMING)
+        r.sync()
+        st = view.stats()
+        vis_tris = visible_triangles(sc2, np.concatenate([view.visible(0), view.visible(1)]))
+        # algorithmic bytes: key 8 + colour 4 per pixel, 39 B per visible triangle;
+        # textured: + 16 B (2x2 RGBA8 texels) per pixel + 12 B (3 float16 uv pairs)
+        # per visible triangle; motion: + 8 B (float2) per pixel
+        alg = 12.0 * npx + 39.0 * vis_tris
+        if name != "bary":
+            alg += 16.0 * npx + 12.0 * vis_tris
+        if name == "textured_motion":
+            alg += 8.0 * npx
+        res[name] = {"frame_ms": round(st.ms_total, 4), "resolve_ms": round(st.ms_resolve, 4),
+                     "resolve_alg_gbs": round(alg / (st.ms_resolve * 1e-3) / 1e9, 1)}
+    out["resolve_4k"] = res
+    view.close()
+    r.close()
+    # device transform update
+    h = synthetic_hierarchy(1 << 18)
+    r = Renderer(0)
+    r.set_hierarchy(h)
+    for i in range(max(1, warmup)):
+        r.update_transforms(0.1 * i)
+    r.sync()
+    t0 = time.perf_counter()
+    for i in range(steps):
+        r.update_transforms(0.013 * i)
+    r.sync()
+    ms = 1000.0 * (time.perf_counter() - t0) / steps
+    out["transform_update"] = {"nodes": len(h.nodes), "channels": len(h.channels), "mesh_nodes": len(h.nodes),
+                               "ms_per_update": round(ms, 4),
+                               "mesh_nodes_per_s": round(len(h.nodes) / (ms * 1e-3), 1)}
+    r.close()
+    if log:
+        log(f"widened rows: {json.dumps(out)}")
+    return out
+
+
 def cpu_baseline(sc, cam, W, H, flags, budget_s=12.0, max_frames=8):
     """The oracle (scalar C restatement, 1 thread) on frames of the same workload."""
     import oracle
@@ -286,6 +400,8 @@ def main():
     ap.add_argument("--no-uneven", action="store_true", help="skip the 16-view uneven stress measurement")
     ap.add_argument("--uneven-only", action="store_true", help="only the 16-view uneven stress measurement")
     ap.add_argument("--diff-only", action="store_true", help="only the differentiable-form measurement")
+    ap.add_argument("--no-widened", action="store_true",
+                    help="skip the textured-resolve / motion-vector / transform-update measurements")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run three plain 256 MiB device-to-device copies (the reference kernel for scaling "
                          "rocprofv3 FETCH_SIZE / WRITE_SIZE to bytes) and exit")
@@ -405,6 +521,8 @@ def main():
               f"tile pairs {st.big_tiles} max/tile {st.max_tile_tris} clipped {st.clipped_triangles}; "
               f"passes {json.dumps({k: round(v, 4) for k, v in passes.items()})}", file=sys.stderr)
     r.close()
+    if not args.no_widened:
+        out["widened"] = widened_rows(sc, cam, W, H, min(args.steps, 20), args.warmup, log)
     if not args.no_batch:
         out["batched"] = batched_views(min(args.steps, 20), args.warmup, log)
     if not args.no_uneven:

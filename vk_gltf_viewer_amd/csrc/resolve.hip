@@ -217,6 +217,10 @@ struct ResolveBatchArgs {
 };
 
 // One block per 64x16 pixels of a view (block map entry = view | block << 8).
+// One instantiation per shading mode, so that the flat and barycentric
+// resolves carry none of the texture / motion-vector code: kShade =
+// interpolated vertex colour, kTex = albedo texture, kMv = motion vectors.
+template <bool kShade, bool kTex, bool kMv>
 __global__ __launch_bounds__(256) void resolve_kernel(ResolveBatchArgs ab)
 {
     const uint32_t me = ab.b.map_resolve[blockIdx.x];
@@ -242,9 +246,9 @@ __global__ __launch_bounds__(256) void resolve_kernel(ResolveBatchArgs ab)
     __shared__ float s_byte[256];
     __shared__ uint32_t s_base[512];
     __shared__ float s_lin[256];
-    const bool tex = (a.flags & VKGV_RESOLVE_TEXTURED) != 0;
+    constexpr bool tex = kTex;
     const SrgbTables* tab = static_cast<const SrgbTables*>(a.s.srgb);
-    if (tex) s_lin[threadIdx.x] = a.s.srgb_decode[threadIdx.x];
+    if constexpr (tex) s_lin[threadIdx.x] = a.s.srgb_decode[threadIdx.x];
     s_thr[threadIdx.x] = tab->thr[threadIdx.x];
     s_byte[threadIdx.x] = (float)threadIdx.x / 255.0f;
     s_base[threadIdx.x] = reinterpret_cast<const uint32_t*>(tab->base)[threadIdx.x];
@@ -256,9 +260,9 @@ __global__ __launch_bounds__(256) void resolve_kernel(ResolveBatchArgs ab)
     const uint32_t px = blk_x * 64u + (threadIdx.x & 63u);
     if (px >= a.W) return;
     const uint32_t py0 = blk_y * (4u * kRows) + (threadIdx.x >> 6);
-    const bool shade = (a.flags & (VKGV_RESOLVE_BARY | VKGV_RESOLVE_TEXTURED)) != 0;   // interpolated colour
-    const bool mv = (a.flags & VKGV_MOTION_VECTORS) != 0;
-    const bool bary = shade || mv;                                                     // barycentrics needed
+    constexpr bool shade = kShade;                 // interpolated colour
+    constexpr bool mv = kMv;
+    constexpr bool bary = kShade || kMv;           // barycentrics needed
     const unsigned long long* __restrict__ keys = a.keys;
     const uint32_t* __restrict__ draw_slot = a.draw_slot;
 
@@ -288,7 +292,7 @@ __global__ __launch_bounds__(256) void resolve_kernel(ResolveBatchArgs ab)
         const size_t i = (size_t)py * a.W + px;
         if (drw[k] == 0xFFFFFFFFu) {
             a.color[i] = 0u;
-            if (mv) view_motion[i] = make_float2(0.0f, 0.0f);
+            if constexpr (mv) view_motion[i] = make_float2(0.0f, 0.0f);
             continue;
         }
         const uint32_t t = key_id(key[k]) & ((1u << VKGV_TRIANGLE_BITS) - 1u);
@@ -329,8 +333,8 @@ __global__ __launch_bounds__(256) void resolve_kernel(ResolveBatchArgs ab)
                     c[j] = c[j] * ((l[0] * c0 + l[1] * c1) + l[2] * c2);
                 }
             }
-            if (tex) apply_texture(a.s, drw[k], t, l, s_lin, s_byte, c);
-            if (mv) view_motion[i] = motion_vector(a.s, view_pmvp, drw[k], t, l, xn, yn);
+            if constexpr (tex) apply_texture(a.s, drw[k], t, l, s_lin, s_byte, c);
+            if constexpr (mv) view_motion[i] = motion_vector(a.s, view_pmvp, drw[k], t, l, xn, yn);
         }
         const uint32_t r8 = srgb_code(s_thr, base, key0, c[0]);
         const uint32_t g8 = srgb_code(s_thr, base, key0, c[1]);
@@ -347,7 +351,18 @@ void launch_resolve(const SceneDev& s, const BatchDev& b, const BatchGrid& g, ui
     a.s = s;
     a.b = b;
     a.flags = flags;
-    if (g.resolve_blocks) resolve_kernel<<<g.resolve_blocks, 256, 0, st>>>(a);
+    if (!g.resolve_blocks) return;
+    const bool tex = flags & VKGV_RESOLVE_TEXTURED, mv = flags & VKGV_MOTION_VECTORS;
+    const bool shade = tex || (flags & VKGV_RESOLVE_BARY);
+    if (mv) {
+        if (tex) resolve_kernel<true, true, true><<<g.resolve_blocks, 256, 0, st>>>(a);
+        else if (shade) resolve_kernel<true, false, true><<<g.resolve_blocks, 256, 0, st>>>(a);
+        else resolve_kernel<false, false, true><<<g.resolve_blocks, 256, 0, st>>>(a);
+    } else {
+        if (tex) resolve_kernel<true, true, false><<<g.resolve_blocks, 256, 0, st>>>(a);
+        else if (shade) resolve_kernel<true, false, false><<<g.resolve_blocks, 256, 0, st>>>(a);
+        else resolve_kernel<false, false, false><<<g.resolve_blocks, 256, 0, st>>>(a);
+    }
 }
 
 } // namespace vkgv
