@@ -394,6 +394,7 @@ def main():
     ap.add_argument("--verbose", action="store_true")
     ap.add_argument("--small-max", type=int, default=0, help="raster tunable (0: library default)")
     ap.add_argument("--blocks-per-cu", type=int, default=0, help="raster tunable (0: library default)")
+    ap.add_argument("--tile-item-tris", type=int, default=0, help="raster tunable (0: library default)")
     ap.add_argument("--no-batch", action="store_true", help="skip the 64-view batched-vs-loop measurement")
     ap.add_argument("--batch-only", action="store_true", help="only the 64-view batched-vs-loop measurement")
     ap.add_argument("--no-diff", action="store_true", help="skip the differentiable-form measurement")
@@ -441,9 +442,9 @@ def main():
     gen_s = time.perf_counter() - t0
 
     r = Renderer(0)
-    if args.small_max or args.blocks_per_cu:
+    if args.small_max or args.blocks_per_cu or args.tile_item_tris:
         sm, bpc, tit, rcs = r.get_tuning()
-        r.set_tuning(args.small_max or sm, args.blocks_per_cu or bpc, tit, rcs)
+        r.set_tuning(args.small_max or sm, args.blocks_per_cu or bpc, args.tile_item_tris or tit, rcs)
     r.load(sc)
     view = r.create_view(W, H)
     flags = N.CULL_FRUSTUM | N.CULL_TWO_PASS | N.RESOLVE_BARY

@@ -327,7 +327,7 @@ int build_maps(vkgv_ctx* c, const std::vector<const ViewDev*>& views, DevBuf& ma
         const uint64_t npx = (uint64_t)v.width * v.height;
         const uint64_t nclear = (npx + kClearKeysPerBlock - 1) / kClearKeysPerBlock;
         const uint64_t nhiz = v.pyr.levels ? (uint64_t)((v.pyr.w[0] + 31) / 32) * ((v.pyr.h[0] + 31) / 32) : 0;
-        const uint64_t nres = (uint64_t)v.resolve_blocks_x * ((v.height + 15) / 16);
+        const uint64_t nres = (uint64_t)v.resolve_blocks_x * ((v.height + 4 * kResolveRows - 1) / (4 * kResolveRows));
         if (std::max({ nclear, nhiz, nres }) >= (1ull << 24))
             return fail(c, VKGV_ERROR_LIMIT, "view too large for a batch block map");
         for (uint64_t k = 0; k < nclear; ++k) clear.push_back(vi | (uint32_t)(k << 8));

@@ -183,6 +183,9 @@ struct Counters {
 static_assert(sizeof(Counters) == 64, "Counters layout");
 
 constexpr uint32_t kTileSplit = 8;      // most work items sharing one screen tile's triangle list
+// Resolve block: 64 columns x (4 * kResolveRows) rows; each thread resolves
+// kResolveRows pixels of its column, their loads issued together.
+constexpr uint32_t kResolveRows = 4;
 
 constexpr uint32_t kBinTile = 32;   // screen tile of the binned path, pixels
 

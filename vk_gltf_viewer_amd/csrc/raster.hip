@@ -776,7 +776,10 @@ __device__ __forceinline__ void bin_fill_view(const BigArgs& a)
 // lane, and broadcast with readlane.  Items of one tile are adjacent, so
 // they run on different XCDs.  Part 0 of each tile zeroes the tile's count
 // for the next pass.
-constexpr int kTileRows = (int)(kBinTile * kBinTile / 128u);   // pixels per lane in a half tile
+// A tile-pass wave covers kTileParts horizontal bands of a screen tile's list:
+// 64 lanes = 32 columns x 2 rows, each lane kTileRows pixels of its column.
+constexpr int kTileParts = 2;
+constexpr int kTileRows = (int)(kBinTile * kBinTile / 64u) / kTileParts;     // pixels per lane in a band
 constexpr int kSetupWords = (int)(sizeof(BigSetup) / 4);
 static_assert(kSetupWords % 4 == 0, "setups are loaded 16 bytes at a time");
 
@@ -787,16 +790,16 @@ __device__ __forceinline__ void tile_view(const BigArgs& a)
     const uint32_t nitems = __builtin_amdgcn_readfirstlane(a.ctr->tile_items);
     const int lx = (int)(lane & 31u), ly = (int)(lane >> 5);
     const BigSetup* __restrict__ setups = a.setups;
-    for (uint32_t hv = wave; hv < 2u * nitems; hv += nwaves) {
-        const TileItem item = a.items[hv >> 1];
+    for (uint32_t hv = wave; hv < kTileParts * nitems; hv += nwaves) {
+        const TileItem item = a.items[hv / kTileParts];
         const uint32_t code = __builtin_amdgcn_readfirstlane(item.code);
         const uint32_t n_all = __builtin_amdgcn_readfirstlane(item.n);
         const uint32_t start = __builtin_amdgcn_readfirstlane(item.start);
         const uint32_t tile = code & 0xFFFFFFu, part = (code >> 24) & 15u, split = (code >> 28) + 1u;
         const uint32_t b = start + n_all * part / split, e = start + n_all * (part + 1u) / split;
-        if (part == 0 && (hv & 1u) == 0 && lane == 0) a.bin_count[tile] = 0u;
+        if (part == 0 && hv % kTileParts == 0 && lane == 0) a.bin_count[tile] = 0u;
         const int px = (int)((tile % a.tiles_x) * kBinTile) + lx;
-        const int py0 = (int)((tile / a.tiles_x) * kBinTile + (hv & 1u) * (kBinTile / 2u)) + ly;
+        const int py0 = (int)((tile / a.tiles_x) * kBinTile + (hv % kTileParts) * (kBinTile / kTileParts)) + ly;
         const bool col_in = px < (int)a.W;
         unsigned long long old[kTileRows];
         if (split == 1) {

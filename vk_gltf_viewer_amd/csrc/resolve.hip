@@ -206,9 +206,9 @@ __device__ __forceinline__ float2 motion_vector(const SceneDev& s, const float* 
                        ((py / pw) * 0.5f - 0.5f) - (yn * 0.5f - 0.5f));
 }
 
-// Block = 64 x 16 pixels; thread (x, y) resolves rows y, y + 4, y + 8, y + 12
-// of its column, with the loads of the four pixels issued together.
-constexpr int kRows = 4;
+// Block = 64 x (4 * kResolveRows) pixels; thread (x, y) resolves rows
+// y, y + 4, ... of its column, with the loads of its pixels issued together.
+constexpr int kRows = (int)kResolveRows;
 
 struct ResolveBatchArgs {
     SceneDev s;
