@@ -75,7 +75,7 @@ struct vkgv_view {
     vkgv_ctx* ctx = nullptr;
     ViewDev dev{};
     DevBuf keys, color, pyramid, visible0, visible1, history, counters, big, clipq, bin_count, bin_start, bins;
-    DevBuf setups, pairs, tri_cache, slot_albedo, draw_slot, big_chunk, chunk_tri, tile_items;
+    DevBuf setups, pairs, tri_cache, slot_albedo, slot_mat, draw_slot, big_chunk, chunk_tri, tile_items;
     DevBuf aa_mat;                   // per-pixel material indices of the differentiable frame (allocated on first use)
     DevBuf mvp_tab;                  // per-frame VP M table, kMvpStride floats per transform
     DevBuf pmvp_tab, motion;         // prevVP * previous M table and W*H motion vectors (allocated on first use)
@@ -944,6 +944,7 @@ int vkgv_view_create(vkgv_ctx* c, uint32_t width, uint32_t height, vkgv_view** o
     if (!rc) rc = dalloc(c, v->pairs, sizeof(BinPair) * (size_t)bin_cap);
     if (!rc) rc = dalloc(c, v->tri_cache, sizeof(TriCache) * 128u * (size_t)c->cache_slots);
     if (!rc) rc = dalloc(c, v->slot_albedo, sizeof(float4) * (size_t)c->cache_slots);
+    if (!rc) rc = dalloc(c, v->slot_mat, sizeof(uint32_t) * (size_t)c->cache_slots);
     if (!rc) rc = dalloc(c, v->chunk_tri, sizeof(uint32_t) * (size_t)(bin_cap / 64u));
     if (!rc) rc = dalloc(c, v->tile_items, sizeof(TileItem) * (size_t)ntiles * kTileSplit);
     if (rc) { vkgv_view_destroy(v); return rc; }
@@ -953,6 +954,7 @@ int vkgv_view_create(vkgv_ctx* c, uint32_t width, uint32_t height, vkgv_view** o
     d.pairs = static_cast<BinPair*>(v->pairs.p);
     d.tri_cache = static_cast<TriCache*>(v->tri_cache.p);
     d.slot_albedo = static_cast<float4*>(v->slot_albedo.p);
+    d.slot_mat = static_cast<uint32_t*>(v->slot_mat.p);
     d.cache_slots = c->cache_slots;
     d.keys = static_cast<unsigned long long*>(v->keys.p);
     d.color = static_cast<uint32_t*>(v->color.p);
@@ -1006,7 +1008,7 @@ void vkgv_view_destroy(vkgv_view* v)
     }
     for (DevBuf* b : { &v->keys, &v->color, &v->pyramid, &v->visible0, &v->visible1, &v->history, &v->counters, &v->big,
                        &v->clipq, &v->bin_count, &v->bin_start, &v->bins, &v->setups, &v->pairs, &v->tri_cache,
-                       &v->slot_albedo, &v->draw_slot, &v->big_chunk, &v->chunk_tri, &v->tile_items, &v->maps,
+                       &v->slot_albedo, &v->slot_mat, &v->draw_slot, &v->big_chunk, &v->chunk_tri, &v->tile_items, &v->maps,
                        &v->aa_mat, &v->mvp_tab, &v->pmvp_tab, &v->motion })
         dfree(*b);
     for (auto& set : v->ev)

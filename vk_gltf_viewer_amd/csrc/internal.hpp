@@ -150,8 +150,10 @@ static_assert(sizeof(TileItem) == 16, "TileItem layout");
 struct TriCache {
     float r[3][3];
     uint32_t col[3];
+    uint32_t uv[3];                 // float16 uv pairs of the vertices (textured frames only)
+    uint32_t pad;
 };
-static_assert(sizeof(TriCache) == 48, "TriCache layout");
+static_assert(sizeof(TriCache) == 64, "TriCache layout");
 
 // Triangle waiting for near-plane / guard-band clipping (clip-space vertices).
 struct ClipTri {
@@ -216,6 +218,7 @@ struct ViewDev {
     // tri_cache[s * 128 + t]; draw_slot[d] is the slot of draw d this frame
     TriCache* tri_cache;
     float4* slot_albedo;            // albedo factor of each slot's material
+    uint32_t* slot_mat;             // material index of each slot's meshlet
     uint32_t* draw_slot;
     uint32_t cache_slots;
     float* mvp;                     // per transform, this frame: VP M (16 floats, column-major), det(M) at [16]

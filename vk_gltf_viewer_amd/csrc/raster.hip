@@ -82,6 +82,7 @@ struct RasterArgs {
     uint32_t slot0;                 // first resolve-cache slot of this list
     TriCache* tri_cache;
     float4* slot_albedo;
+    uint32_t* slot_mat;
     uint32_t* draw_slot;
     uint32_t cache_slots;
 };
@@ -228,6 +229,7 @@ __device__ __forceinline__ void raster_view(const RasterArgs& a)
     const int w = threadIdx.x >> 6;
     const uint32_t lane = threadIdx.x & 63u;
     __shared__ uint32_t s_col[kWaves][64];
+    __shared__ uint32_t s_uv[kWaves][64];
     const uint32_t count = a.count;
     const uint32_t slot0 = a.slot0;
     const uint32_t nwaves = gridDim.x * kWaves;
@@ -257,6 +259,7 @@ __device__ __forceinline__ void raster_view(const RasterArgs& a)
         const bool cull_faces = !(matw >> 31) && !(a.flags & VKGV_NO_BACKFACE);
         // the flat resolve reads only the slot albedo; the interpolating ones the triangles too
         const bool bary = (a.flags & (VKGV_RESOLVE_BARY | VKGV_RESOLVE_TEXTURED | VKGV_MOTION_VECTORS)) != 0;
+        const bool tex = (a.flags & VKGV_RESOLVE_TEXTURED) != 0;   // the textured resolve reads the uvs too
         const uint32_t vc = counts & 0xFFu, tc = (counts >> 8) & 0xFFu;
         // resolve-cache slot of this meshlet (none when the cache is full)
         const uint32_t slot = slot0 + idx;
@@ -267,6 +270,7 @@ __device__ __forceinline__ void raster_view(const RasterArgs& a)
                 const vkgv_material& mat = a.s.materials[matw & 0x7FFFFFFFu];
                 a.slot_albedo[slot] = make_float4(mat.albedo_factor[0], mat.albedo_factor[1], mat.albedo_factor[2],
                                                   mat.albedo_factor[3]);
+                a.slot_mat[slot] = matw & 0x7FFFFFFFu;
             }
         }
 
@@ -278,6 +282,7 @@ __device__ __forceinline__ void raster_view(const RasterArgs& a)
             const float4 c = mat4_point(mvp, pp[0], pp[1], pp[2]);
             s_clip[w][lane] = c;
             s_col[w][lane] = reinterpret_cast<const uint32_t*>(pp)[3];
+            if (tex) s_uv[w][lane] = reinterpret_cast<const uint32_t*>(pp)[5];
             SVtx sv;
             sv.x = 0; sv.y = 0; sv.z = 0.0f;
             sv.ok = project_vtx(c, hw, hh, sv) ? 1u : 0u;
@@ -314,7 +319,11 @@ __device__ __forceinline__ void raster_view(const RasterArgs& a)
                         tcache.r[k][2] = p.x * q.y - p.y * q.x;
                     }
                     tcache.col[0] = s_col[w][i0]; tcache.col[1] = s_col[w][i1]; tcache.col[2] = s_col[w][i2];
-                    a.tri_cache[(size_t)slot * 128u + t] = tcache;
+                    // 48 bytes always, the uv words only for textured frames
+                    uint4* dst = reinterpret_cast<uint4*>(a.tri_cache + (size_t)slot * 128u + t);
+                    const uint4* srcw = reinterpret_cast<const uint4*>(&tcache);
+                    dst[0] = srcw[0]; dst[1] = srcw[1]; dst[2] = srcw[2];
+                    if (tex) dst[3] = make_uint4(s_uv[w][i0], s_uv[w][i1], s_uv[w][i2], 0u);
                 }
                 if (cull_faces) {
                     const float det = det3_xyw(c0, c1, c2);
@@ -967,6 +976,7 @@ __global__ __launch_bounds__(kBlock) void raster_kernel(RasterBatchArgs ab)
             a.slot0 = __builtin_amdgcn_readlane(base, j);
             a.tri_cache = v.tri_cache;
             a.slot_albedo = v.slot_albedo;
+            a.slot_mat = v.slot_mat;
             a.draw_slot = v.draw_slot;
             a.cache_slots = min(v.cache_slots, ab.cache_slots);
             raster_view(a);

@@ -17,7 +17,8 @@
 //
 // VKGV_RESOLVE_TEXTURED adds the albedo texture the reference's fragment
 // shader leaves commented out (visbuffer.frag.glsl:31-32): the vertex uvs
-// (float16) are interpolated with the same barycentrics, transformed by
+// (float16; kept in the resolve cache beside the triangle, with the slot's
+// material) are interpolated with the same barycentrics, transformed by
 // transformUv (mesh_common.h.glsl:129-135: rotation, then scale, then
 // offset), and the texture is sampled bilinearly at level 0 with per-texel
 // sRGB decode; colour = vertex colour * albedo factor * texture.
@@ -54,6 +55,7 @@ struct ResolveArgs {
     const uint32_t* draw_slot;
     const TriCache* tri_cache;
     const float4* slot_albedo;
+    const uint32_t* slot_mat;
     const float* mvp;               // the view's VP M table (kMvpStride floats per transform)
 };
 
@@ -132,27 +134,34 @@ __device__ __forceinline__ float texel_pair(float u, uint32_t n, uint32_t mode, 
     return x - fl;
 }
 
-// Albedo texture times the pixel's colour (c[0..3]): uvs of triangle t of
-// draw d interpolated with l[], transformUv, bilinear sample, multiply.
-__device__ __forceinline__ void apply_texture(const SceneDev& s, uint32_t d, uint32_t t, const float* l,
-                                              const float* s_lin, const float* s_byte, float* c)
+// Material and float16 uv words of triangle t of draw d from the scene
+// buffers (meshlets the resolve cache does not hold).
+__device__ __forceinline__ uint32_t fetch_uvs(const SceneDev& s, uint32_t d, uint32_t t, uint32_t* uvw)
 {
     const vkgv_meshlet_draw dr = s.draws[d];
     const vkgv_primitive& prim = s.prims[dr.primitive_index];
-    const uint32_t mi = prim.material_index;
-    const vkgv_material& mat = s.materials[mi];
-    const uint32_t ti = mat.albedo_index;
-    if (ti == 0xFFFFFFFFu) return;
     const vkgv_meshlet ml = reinterpret_cast<const vkgv_meshlet*>(prim.meshlet_buffer)[dr.meshlet_index];
     const uint8_t* tri = reinterpret_cast<const uint8_t*>(prim.primitive_index_buffer) + ml.triangle_offset + 3u * t;
     const uint32_t* vidx = reinterpret_cast<const uint32_t*>(prim.vertex_index_buffer) + ml.vertex_offset;
     const vkgv_vertex* verts = reinterpret_cast<const vkgv_vertex*>(prim.vertex_buffer);
+#pragma unroll
+    for (int k = 0; k < 3; ++k) uvw[k] = *reinterpret_cast<const uint32_t*>(verts[vidx[tri[k]]].uv);
+    return prim.material_index;
+}
+
+// Albedo texture of material mi times the pixel's colour (c[0..3]): the
+// vertex uvs interpolated with l[], transformUv, bilinear sample, multiply.
+__device__ __forceinline__ void apply_texture(const SceneDev& s, uint32_t mi, const uint32_t* uvw, const float* l,
+                                              const float* s_lin, const float* s_byte, float* c)
+{
+    const vkgv_material& mat = s.materials[mi];
+    const uint32_t ti = mat.albedo_index;
+    if (ti == 0xFFFFFFFFu) return;
     float uq[3], vq[3];
 #pragma unroll
     for (int k = 0; k < 3; ++k) {
-        const uint32_t uvw = *reinterpret_cast<const uint32_t*>(verts[vidx[tri[k]]].uv);
-        uq[k] = half_bits_to_float(uvw & 0xFFFFu);
-        vq[k] = half_bits_to_float(uvw >> 16);
+        uq[k] = half_bits_to_float(uvw[k] & 0xFFFFu);
+        vq[k] = half_bits_to_float(uvw[k] >> 16);
     }
     const float u = (l[0] * uq[0] + l[1] * uq[1]) + l[2] * uq[2];
     const float v = (l[0] * vq[0] + l[1] * vq[1]) + l[2] * vq[2];
@@ -236,6 +245,7 @@ __global__ __launch_bounds__(256) void resolve_kernel(ResolveBatchArgs ab)
     a.draw_slot = view.draw_slot;
     a.tri_cache = view.tri_cache;
     a.slot_albedo = view.slot_albedo;
+    a.slot_mat = view.slot_mat;
     a.mvp = view.mvp;
     float2* const view_motion = view.motion;
     const float* const view_pmvp = view.pmvp;
@@ -298,7 +308,7 @@ __global__ __launch_bounds__(256) void resolve_kernel(ResolveBatchArgs ab)
         const uint32_t t = key_id(key[k]) & ((1u << VKGV_TRIANGLE_BITS) - 1u);
         float c[4];
         float3 r[3];
-        uint32_t col[3];
+        uint32_t col[3], uvw[3], mi = 0;
         if (slot[k] != 0xFFFFFFFFu) {
             // the raster kept this meshlet's albedo and triangles
             const float4 alb = a.slot_albedo[slot[k]];
@@ -309,10 +319,13 @@ __global__ __launch_bounds__(256) void resolve_kernel(ResolveBatchArgs ab)
                 for (int q = 0; q < 3; ++q) {
                     r[q] = make_float3(tc.r[q][0], tc.r[q][1], tc.r[q][2]);
                     col[q] = tc.col[q];
+                    if constexpr (tex) uvw[q] = tc.uv[q];
                 }
+                if constexpr (tex) mi = a.slot_mat[slot[k]];
             }
         } else {
             fetch_uncached(a, drw[k], t, bary, c, r, col);
+            if constexpr (tex) mi = fetch_uvs(a.s, drw[k], t, uvw);
         }
         if (bary) {
             // perspective-correct barycentrics at the pixel centre:
@@ -333,7 +346,7 @@ __global__ __launch_bounds__(256) void resolve_kernel(ResolveBatchArgs ab)
                     c[j] = c[j] * ((l[0] * c0 + l[1] * c1) + l[2] * c2);
                 }
             }
-            if constexpr (tex) apply_texture(a.s, drw[k], t, l, s_lin, s_byte, c);
+            if constexpr (tex) apply_texture(a.s, mi, uvw, l, s_lin, s_byte, c);
             if constexpr (mv) view_motion[i] = motion_vector(a.s, view_pmvp, drw[k], t, l, xn, yn);
         }
         const uint32_t r8 = srgb_code(s_thr, base, key0, c[0]);
