@@ -85,7 +85,8 @@ def batched_views(steps, warmup, log):
     faces at 512x512, 54 random perspective views up to 1920x1080) as one
     captured-graph batch frame, against the plain loop of 64 single-view
     frames on the same views.  Two-pass Hi-Z, barycentric resolve; tunables
-    chosen by the autotuner on the batch frame."""
+    chosen by the autotuner on the batch frame; batch and loop timed in
+    alternating rounds, medians reported."""
     from vk_gltf_viewer_amd.autotune import autotune
     sc = Scene(3, 3).arrays()
     r = Renderer(0)
@@ -96,7 +97,7 @@ def batched_views(steps, warmup, log):
         batch = r.create_batch(views)
         flags = N.CULL_FRUSTUM | N.CULL_TWO_PASS | N.RESOLVE_BARY
         tuning, _ = autotune(r, lambda: batch.render(cams, flags | N.GRAPH), key="c3-64views", use_cache=False,
-                             log=log, valid=lambda: batch.stats().overflow == 0)
+                             log=log, valid=lambda: batch.stats().overflow == 0, frames=8)
 
         def run(fn, n):
             r.sync()
@@ -112,9 +113,15 @@ def batched_views(steps, warmup, log):
             for v, c in zip(views, cams):
                 v.render(c, flags)
         run(batch_frame, warmup)
-        ms_batch = run(batch_frame, steps)
-        run(loop_frame, warmup)
-        ms_loop = run(loop_frame, steps)
+        run(loop_frame, max(1, warmup // 2))
+        # batch and loop timed in alternating rounds (medians), so that clock
+        # changes over the run affect both sides alike
+        rounds = 5
+        ms_b, ms_l = [], []
+        for _ in range(rounds):
+            ms_b.append(run(batch_frame, max(2, steps // rounds)))
+            ms_l.append(run(loop_frame, max(1, steps // (2 * rounds))))
+        ms_batch, ms_loop = float(np.median(ms_b)), float(np.median(ms_l))
         st = batch.stats()
         px = sum(v.width * v.height for v in sc.views)
         out = {"views": len(views), "pixels": px, "triangles": Scene(3, 3).triangle_count,
@@ -165,9 +172,15 @@ def uneven_views(steps, warmup, log):
             for v, c in zip(views, cams):
                 v.render(c, flags)
         run(batch_frame, warmup)
-        ms_batch = run(batch_frame, steps)
-        run(loop_frame, warmup)
-        ms_loop = run(loop_frame, steps)
+        run(loop_frame, max(1, warmup // 2))
+        # batch and loop timed in alternating rounds (medians), so that clock
+        # changes over the run affect both sides alike
+        rounds = 5
+        ms_b, ms_l = [], []
+        for _ in range(rounds):
+            ms_b.append(run(batch_frame, max(2, steps // rounds)))
+            ms_l.append(run(loop_frame, max(1, steps // (2 * rounds))))
+        ms_batch, ms_loop = float(np.median(ms_b)), float(np.median(ms_l))
         st = batch.stats()
         # backward to the per-instance transforms: every view's antialiased
         # linear image and its backward (an L2 loss against a zero target,
@@ -401,6 +414,8 @@ def main():
     ap.add_argument("--no-uneven", action="store_true", help="skip the 16-view uneven stress measurement")
     ap.add_argument("--uneven-only", action="store_true", help="only the 16-view uneven stress measurement")
     ap.add_argument("--diff-only", action="store_true", help="only the differentiable-form measurement")
+    ap.add_argument("--widened-only", action="store_true",
+                    help="only the textured-resolve / motion-vector / transform-update measurements")
     ap.add_argument("--no-widened", action="store_true",
                     help="skip the textured-resolve / motion-vector / transform-update measurements")
     ap.add_argument("--copy-calibration", action="store_true",
@@ -430,6 +445,13 @@ def main():
         return
     if args.diff_only:
         print(json.dumps({"differentiable": differentiable(args.steps, args.warmup)}))
+        return
+
+    if args.widened_only:
+        sc = Scene(args.config, args.config, args.variant).arrays()
+        vd = sc.views[0]
+        print(json.dumps({"widened": widened_rows(sc, camera_from_view(vd), vd.width, vd.height, args.steps,
+                                                  args.warmup, log)}))
         return
 
     t0 = time.perf_counter()
