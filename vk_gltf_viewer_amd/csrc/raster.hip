@@ -53,13 +53,46 @@ struct TriSetup {
     long long area;
 };
 
+// A small triangle staged in LDS (52 bytes).  The edge steps A = -ey * 256
+// and B = ex * 256 are kept as the 16-bit deltas (the small path's extents
+// stay below 2^15 sub-pixel units), and the depth-plane origin offsets as the
+// 16-bit sub-pixel distances they are; both are widened exactly on load.
 struct SmallTri {
+    int C[3];
+    float za, zb, z0;
+    uint32_t org;   // px0 | py0 << 16
+    uint32_t id;
+    int16_t ex[3], ey[3];
+    int16_t ox, oy; // sub-pixel offsets of the bbox origin's centre from vertex 0
+    uint16_t dim;   // bw | bh << 8
+    uint16_t pad;
+};
+static_assert(sizeof(SmallTri) == 52, "SmallTri layout");
+
+// The register form of a staged small triangle.
+struct SmallTriR {
     int C[3], A[3], B[3];
     float za, zb, z0, oxf, oyf;
-    uint32_t org;   // px0 | py0 << 16
-    uint32_t dim;   // bw | bh << 8
-    uint32_t id;
+    uint32_t org, id, bw;
 };
+
+__device__ __forceinline__ SmallTriR load_small(const SmallTri& s)
+{
+    SmallTriR r;
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        r.C[k] = s.C[k];
+        r.A[k] = -(int)s.ey[k] * 256;
+        r.B[k] = (int)s.ex[k] * 256;
+    }
+    r.za = s.za; r.zb = s.zb; r.z0 = s.z0;
+    r.oxf = (float)s.ox * 0.00390625f;
+    r.oyf = (float)s.oy * 0.00390625f;
+    r.org = s.org;
+    r.id = s.id;
+    r.bw = s.dim & 0xFFu;
+    return r;
+}
 
 struct RasterArgs {
     SceneDev s;
@@ -223,7 +256,7 @@ __device__ __forceinline__ void raster_view(const RasterArgs& a)
     __shared__ float4 s_clip[kWaves][64];
     __shared__ SVtx s_sv[kWaves][64];
     __shared__ uint32_t s_tri[kWaves][96];
-    __shared__ SmallTri s_st[kWaves][128];
+    __shared__ SmallTri s_st[kWaves][VKGV_MESHLET_TRIANGLES];
     __shared__ uint32_t s_pre[kWaves][128];
 
     const int w = threadIdx.x >> 6;
@@ -355,16 +388,17 @@ __device__ __forceinline__ void raster_view(const RasterArgs& a)
                         const int ex = ts.x[kb] - ts.x[ka], ey = ts.y[kb] - ts.y[ka];
                         const bool tl = ey < 0 || (ey == 0 && ex > 0);
                         st.C[k] = ex * (sy0 - ts.y[ka]) - ey * (sx0 - ts.x[ka]) - (tl ? 0 : 1);
-                        st.A[k] = -ey * 256;
-                        st.B[k] = ex * 256;
+                        st.ex[k] = (int16_t)ex;
+                        st.ey[k] = (int16_t)ey;
                     }
                     depth_plane(ts.x, ts.y, ts.z, ts.area, st.za, st.zb);
                     st.z0 = ts.z[0];
-                    st.oxf = (float)(sx0 - ts.x[0]) * 0.00390625f;
-                    st.oyf = (float)(sy0 - ts.y[0]) * 0.00390625f;
+                    st.ox = (int16_t)(sx0 - ts.x[0]);
+                    st.oy = (int16_t)(sy0 - ts.y[0]);
                     st.org = (uint32_t)ts.px0 | ((uint32_t)ts.py0 << 16);
                     const uint32_t bw = (uint32_t)(ts.px1 - ts.px0 + 1), bh = (uint32_t)(ts.py1 - ts.py0 + 1);
-                    st.dim = bw | (bh << 8);
+                    st.dim = (uint16_t)(bw | (bh << 8));
+                    st.pad = 0;
                     st.id = id;
                     s_st[w][t] = st;
                     npix[sl] = bw * bh;
@@ -402,8 +436,8 @@ __device__ __forceinline__ void raster_view(const RasterArgs& a)
                 if (s_pre[w][mid] <= k) lo = mid; else hi = mid - 1u;
             }
             uint32_t next = lo < 127u ? s_pre[w][lo + 1u] : total;
-            SmallTri st = s_st[w][lo];
-            uint32_t bw = st.dim & 0xFFu;
+            SmallTriR st = load_small(s_st[w][lo]);
+            uint32_t bw = st.bw;
             const uint32_t r = k - s_pre[w][lo];
             int jj = (int)(r / bw), ii = (int)(r - (uint32_t)jj * bw);
             int e0 = st.C[0] + st.A[0] * ii + st.B[0] * jj;
@@ -419,8 +453,8 @@ __device__ __forceinline__ void raster_view(const RasterArgs& a)
                 if (k == next) {   // first slot of a later triangle (empty ones skipped)
                     while (lo < 127u && s_pre[w][lo + 1u] <= k) ++lo;
                     next = lo < 127u ? s_pre[w][lo + 1u] : total;
-                    st = s_st[w][lo];
-                    bw = st.dim & 0xFFu;
+                    st = load_small(s_st[w][lo]);
+                    bw = st.bw;
                     ii = 0;
                     jj = 0;
                     e0 = st.C[0];
