@@ -172,3 +172,53 @@ def test_state_errors(rend, tmp_path):
         r.sync()
     finally:
         r.close()
+
+
+def test_graph_replays_with_device_updates(rend, tmp_path):
+    """A captured batch frame (VKGV_GRAPH) with the textured resolve and motion
+    vectors, replayed after each device transform update: every replay equals
+    the eager batch frame of the same state (colour, keys, motion vectors)."""
+    from texscene import textured_spheres
+    p = tmp_path / "ts.glb"
+    textured_spheres(grid=6, seed=2).write_glb(p)
+    sc = gltf.load_gltf(str(p))
+    h = sc.hierarchy
+    # spin the scene's roots about y so that the transforms change between updates
+    q = lambda a: [0.0, float(np.sin(a / 2)), 0.0, float(np.cos(a / 2))]
+    nr = len(h.roots)
+    sp = np.array([(0, 3, 0, N.INTERP_LINEAR)], np.uint32).view(ANIM_SAMPLER_DTYPE).ravel()
+    times = np.array([0.0, 1.0, 2.0], np.float32)
+    values = np.array([q(0.0), q(0.5), q(1.0)], np.float32).ravel()
+    ch = np.zeros(nr, ANIM_CHANNEL_DTYPE)
+    ch["node"], ch["sampler"], ch["path"] = h.roots, 0, N.PATH_ROTATION
+    h2 = Hierarchy(h.nodes, h.children, h.roots, sp, times, values, ch)
+    rend.load(sc)
+    rend.set_hierarchy(h2)
+    specs = [(320, 240, (0.0, 6.0, 12.0)), (200, 160, (5.0, 3.0, 9.0))]
+    views = [rend.create_view(W, H) for W, H, _ in specs]
+    cams = [camera_from_view(view_desc(W, H, e, (0.0, 0.0, 0.0), fovy_deg=50.0)) for W, H, e in specs]
+    flags = N.CULL_FRUSTUM | N.RESOLVE_TEXTURED | N.MOTION_VECTORS
+    batch = rend.create_batch(views)
+    try:
+        t_prev = 0.0                                  # set_hierarchy leaves the time-0 matrices
+        for k, t in enumerate((0.2, 0.45, 0.7, 1.3)):
+            print(f"graph replay {k} at t={t}", flush=True)
+            rend.update_transforms(t)                 # previous = t_prev, current = t
+            batch.render(cams, flags | N.GRAPH)
+            rend.sync()
+            got = [(v.keys().copy(), v.color().copy(), v.motion().copy()) for v in views]
+            # the same state rebuilt eagerly: previous = t_prev, current = t
+            rend.update_transforms(t_prev)
+            rend.update_transforms(t)
+            batch.render(cams, flags)
+            rend.sync()
+            for v, (gk, gc, gm) in zip(views, got):
+                assert np.array_equal(v.keys(), gk)
+                assert np.array_equal(v.color(), gc)
+                assert np.array_equal(v.motion().view(np.uint32), gm.view(np.uint32))
+            assert np.abs(got[0][2]).max() > 0                        # the scene moves
+            t_prev = t
+    finally:
+        batch.close()
+        for v in views:
+            v.close()
