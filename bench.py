@@ -488,7 +488,7 @@ def main():
     r.sync()
     st = view.stats()
     if st.overflow:
-        print("bench.py: WARNING big-triangle queue overflow", file=sys.stderr)
+        print("bench.py: WARNING a raster queue was full (direct path taken)", file=sys.stderr)
 
     passes = {
         "clear": st.ms_clear, "cull_early": st.ms_cull, "raster_early": st.ms_raster, "hiz": st.ms_hiz,

@@ -170,7 +170,9 @@ typedef struct vkgv_frame_stats {
     uint32_t big_triangles;          /* triangles sent to the tiled path */
     uint32_t big_tiles;              /* (32x32 screen tile, triangle) pairs they were binned to */
     uint32_t clipped_triangles;      /* triangles that needed near/guard-band clipping */
-    uint32_t overflow;               /* non-zero if a queue overflowed (frame incomplete) */
+    uint32_t overflow;               /* non-zero if a queue was full: bit 0 big-triangle queue, bit 1 clip
+                                        queue, bit 2 tile-pair list.  The frame is still complete: those
+                                        triangles took the slower direct path (one lane per triangle) */
     uint32_t timed_frames;           /* frames averaged in the ms_* fields */
     /* per-pass GPU milliseconds (HIP events), averaged over the frames rendered
      * with VKGV_TIMING since the previous vkgv_view_read_stats (at most 64) */
@@ -181,7 +183,8 @@ typedef struct vkgv_frame_stats {
 
 typedef struct vkgv_create_info {
     int32_t  device;                 /* HIP device ordinal */
-    uint32_t big_triangle_capacity;  /* 0 = default (1M entries) */
+    uint32_t big_triangle_capacity;  /* per view and raster pass; 0 = default (2^18), at most 2^23 */
+    uint32_t clip_triangle_capacity; /* per view and raster pass; 0 = default (2^16), at most 2^22 */
 } vkgv_create_info;
 
 typedef struct vkgv_ctx  vkgv_ctx;

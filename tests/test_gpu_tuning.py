@@ -1,7 +1,8 @@
 """Every raster tuning gives the same frame: keys, colours and visible lists
 of single-view frames and of a captured-graph batch frame are bit-identical
 to those of the default tuning (which the parity tests check against the
-oracle), as long as the big-triangle queue does not overflow."""
+oracle), also when a low small-triangle extent fills the big-triangle queues
+and their triangles take the direct path."""
 import numpy as np
 import pytest
 
@@ -70,20 +71,18 @@ def test_batch_graph_recaptures_on_tuning_change(rend):
         rend.set_tuning(*DEFAULT)
         batch.render(cams, flags)
         ref = [(v.keys(), v.color()) for v in views]
-        compared = 0
+        full = 0
         for t in TUNINGS:
             rend.set_tuning(*t)
             batch.render(cams, flags)
             if batch.stats().overflow:
                 # a small-triangle extent below the default sends more triangles to the
-                # binned path than its queues hold (2048x2048 cascades): reported, not exact
+                # binned path than its queues hold (2048x2048 cascades): the direct path
                 assert t[0] < DEFAULT[0], t
-                continue
-            compared += 1
+                full += 1
             for i, v in enumerate(views):
                 assert np.array_equal(v.keys(), ref[i][0]) and np.array_equal(v.color(), ref[i][1]), (t, i)
-        assert compared >= 3
-        print(f"batch graph: identical frames under {compared} tunings", flush=True)
+        print(f"batch graph: identical frames under {len(TUNINGS)} tunings ({full} with a full queue)", flush=True)
     finally:
         rend.set_tuning(*DEFAULT)
         batch.close()

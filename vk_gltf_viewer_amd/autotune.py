@@ -17,11 +17,11 @@ kept and cached in a JSON file under a key the caller chooses (device, scene,
 views).
 
 The tuner only changes how work is split, and the frame's results are
-identical for every tuning (the parity tests run at several tunings) as long
-as the big-triangle queue does not overflow: a lower `small_max_extent`
-sends more triangles to it.  A caller passes `valid` (e.g. "the frame's
-overflow counter is zero"), and candidates whose frames fail it are not
-chosen.
+identical for every tuning (the parity tests run at several tunings).  A
+lower `small_max_extent` sends more triangles to the big-triangle queues;
+when they are full, those triangles take the slower direct path (the frame's
+overflow bits say so).  A caller passes `valid` (e.g. "the frame's overflow
+bits are zero"), and candidates whose frames fail it are not chosen.
 """
 from __future__ import annotations
 

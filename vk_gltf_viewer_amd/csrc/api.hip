@@ -75,7 +75,7 @@ struct vkgv_view {
     vkgv_ctx* ctx = nullptr;
     ViewDev dev{};
     DevBuf keys, color, pyramid, visible0, visible1, history, counters, big, clipq, bin_count, bin_start, bins;
-    DevBuf setups, pairs, tri_cache, slot_albedo, slot_mat, draw_slot, big_chunk, chunk_tri, tile_items;
+    DevBuf setups, pairs, tri_cache, slot_albedo, slot_mat, draw_slot, big_chunk, chunk_tri, tile_items, redo;
     DevBuf aa_mat;                   // per-pixel material indices of the differentiable frame (allocated on first use)
     DevBuf mvp_tab;                  // per-frame VP M table, kMvpStride floats per transform
     DevBuf pmvp_tab, motion;         // prevVP * previous M table and W*H motion vectors (allocated on first use)
@@ -290,6 +290,7 @@ int ensure_lists(vkgv_ctx* c, vkgv_view* v)
     const size_t hwords = (cap + 63u) / 64u;
     if ((rc = dalloc(c, v->history, sizeof(uint64_t) * hwords))) return rc;
     if ((rc = dalloc(c, v->draw_slot, sizeof(uint32_t) * cap))) return rc;
+    if ((rc = dalloc(c, v->redo, sizeof(uint32_t) * cap))) return rc;
     HIP_TRY(c, hipMemsetAsync(v->history.p, 0, v->history.bytes, c->stream));
     HIP_TRY(c, hipStreamSynchronize(c->stream));   // frames may run on another stream
     v->list_capacity = cap;
@@ -298,6 +299,7 @@ int ensure_lists(vkgv_ctx* c, vkgv_view* v)
     v->dev.visible[1] = static_cast<RasterRec*>(v->visible1.p);
     v->dev.history = static_cast<uint32_t*>(v->history.p);
     v->dev.draw_slot = static_cast<uint32_t*>(v->draw_slot.p);
+    v->dev.redo = static_cast<uint32_t*>(v->redo.p);
     return VKGV_OK;
 }
 
@@ -386,7 +388,7 @@ void frame_launches(vkgv_ctx* c, const BatchDev& b, const BatchGrid& g, uint32_t
     launch_cull(s, b, g, flags, two_pass ? CULL_EARLY : CULL_SINGLE, 0, st);
     mark(EV_CULL);
     launch_raster(s, b, flags, 0, raster_grid, c->tuning.small_max_extent, c->tuning.resolve_cache_slots, st);
-    launch_raster_big(b, g, tile_grid, c->tuning.tile_item_tris, st);
+    launch_raster_big(s, b, g, flags, 0, c->tuning.small_max_extent, tile_grid, c->tuning.tile_item_tris, st);
     mark(EV_RASTER);
     if (two_pass) {
         launch_hiz(b, g, st);
@@ -394,7 +396,7 @@ void frame_launches(vkgv_ctx* c, const BatchDev& b, const BatchGrid& g, uint32_t
         launch_cull(s, b, g, flags, CULL_LATE, 1, st);
         mark(EV_CULL_LATE);
         launch_raster(s, b, flags, 1, raster_grid, c->tuning.small_max_extent, c->tuning.resolve_cache_slots, st);
-        launch_raster_big(b, g, tile_grid, c->tuning.tile_item_tris, st);
+        launch_raster_big(s, b, g, flags, 1, c->tuning.small_max_extent, tile_grid, c->tuning.tile_item_tris, st);
         mark(EV_RASTER_LATE);
     }
     if (!(flags & VKGV_SKIP_RESOLVE)) launch_resolve(s, b, g, flags, st);
@@ -429,7 +431,8 @@ int vkgv_create(const vkgv_create_info* info, vkgv_ctx** out)
     auto* c = new vkgv_ctx();
     c->device = info ? info->device : 0;
     if (info && info->big_triangle_capacity) c->big_capacity = info->big_triangle_capacity;
-    if (c->big_capacity > (1u << 23)) { delete c; return VKGV_ERROR_LIMIT; }
+    if (info && info->clip_triangle_capacity) c->clip_capacity = info->clip_triangle_capacity;
+    if (c->big_capacity > (1u << 23) || c->clip_capacity > (1u << 22)) { delete c; return VKGV_ERROR_LIMIT; }
     int count = 0;
     if (hipGetDeviceCount(&count) != hipSuccess || count <= c->device || c->device < 0) {
         delete c;
@@ -1009,7 +1012,7 @@ void vkgv_view_destroy(vkgv_view* v)
     for (DevBuf* b : { &v->keys, &v->color, &v->pyramid, &v->visible0, &v->visible1, &v->history, &v->counters, &v->big,
                        &v->clipq, &v->bin_count, &v->bin_start, &v->bins, &v->setups, &v->pairs, &v->tri_cache,
                        &v->slot_albedo, &v->slot_mat, &v->draw_slot, &v->big_chunk, &v->chunk_tri, &v->tile_items, &v->maps,
-                       &v->aa_mat, &v->mvp_tab, &v->pmvp_tab, &v->motion })
+                       &v->aa_mat, &v->mvp_tab, &v->pmvp_tab, &v->motion, &v->redo })
         dfree(*b);
     for (auto& set : v->ev)
         for (auto& e : set)

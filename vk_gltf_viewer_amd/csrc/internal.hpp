@@ -165,16 +165,14 @@ static_assert(sizeof(ClipTri) == 64, "ClipTri layout");
 
 // Per-view frame counters in device memory.
 struct Counters {
-    // big triangles of the current raster pass: (count << 32) | 64-tile
-    // chunks reserved; one 64-bit atomicAdd gives a triangle both its slot
-    // and its first chunk, so chunk starts ascend with the slots
-    unsigned long long big_packed;
+    unsigned long long big_chunks;  // 64-tile chunks of the pair list reserved in the current raster pass
     uint32_t vis_count[2];          // pass 0 / late-pass visible lists
-    uint32_t reserved[2];
+    uint32_t big_count;             // big-triangle slots reserved in the current raster pass
+    uint32_t redo_count;            // meshlets of the current raster pass in `redo`
     uint32_t clip_count;            // triangles queued for clipping in the current pass
     uint32_t bin_total;             // (tile, triangle) pairs binned in the current pass
     uint32_t clipped;               // statistics over the frame
-    uint32_t overflow;              // bit 0 big queue, bit 1 clip queue, bit 2 bins
+    uint32_t overflow;              // slow path taken: bit 0 big queue full, bit 1 clip queue full, bit 2 pair list full
     uint32_t reserved2;
     uint32_t big_total_tris;
     uint32_t big_total_pairs;
@@ -198,6 +196,7 @@ struct ViewDev {
     float* pyramid;
     PyramidDesc pyr;
     RasterRec* visible[2];          // compacted visible draws (first pass / late pass)
+    uint32_t* redo;                 // list positions of the pass's meshlets that found a queue full
     uint32_t* history;              // two-pass visibility bits, ceil(draws/64)*2 words
     Counters* counters;
     BigTri* big;                    // big triangles of the current raster pass
@@ -296,7 +295,8 @@ void launch_cull(const SceneDev& s, const BatchDev& b, const BatchGrid& g, uint3
 void launch_raster(const SceneDev& s, const BatchDev& b, uint32_t flags, uint32_t list, uint32_t grid_blocks,
                    uint32_t small_max_extent, uint32_t cache_slots, hipStream_t st);
 // Clipping, binning and tiled rasterization of the views' big-triangle queues.
-void launch_raster_big(const BatchDev& b, const BatchGrid& g, uint32_t tile_grid, uint32_t tile_item_tris,
+void launch_raster_big(const SceneDev& s, const BatchDev& b, const BatchGrid& g, uint32_t flags, uint32_t list,
+                       uint32_t small_max_extent, uint32_t tile_grid, uint32_t tile_item_tris,
                        hipStream_t st);
 void launch_hiz(const BatchDev& b, const BatchGrid& g, hipStream_t st);
 void launch_resolve(const SceneDev& s, const BatchDev& b, const BatchGrid& g, uint32_t flags, hipStream_t st);

@@ -19,11 +19,18 @@
 //                 LDS, back-face culls its <= 124 triangles (2 per lane) and
 //                 rasterizes the small ones cooperatively: a wave prefix sum
 //                 over their bounding-box pixel counts gives each lane an equal
-//                 run of pixel slots, walked with integer edge steps.  Large triangles go to the big queue (one
-//                 64-bit atomic reserves the slot and the 64-tile chunks of the
-//                 triangle's 32x32-pixel bounding-box tiles), triangles needing
-//                 clipping to the clip queue.
-//  clip_kernel    clips the queued triangles; their fans join the big queue.
+//                 run of pixel slots, walked with integer edge steps.  Large
+//                 triangles go to the big queue (a slot, and the 64-tile chunks
+//                 of the triangle's 32x32-pixel bounding-box tiles in the pair
+//                 list), triangles needing clipping to the clip queue.  A
+//                 meshlet with a triangle that found either full is listed for
+//                 the direct path.
+//  clip_kernel    clips the queued triangles; their fans join the big queue
+//                 (or, without room, are rasterized directly by their lane).
+//                 Then redoes the listed meshlets: each of their triangles that
+//                 is not small is rasterized directly, pixel by pixel over its
+//                 bounding box by one lane, so that no triangle is lost when a
+//                 queue is full (slower, same result).
 //  bin_kernel     one wave per chunk: stores the triangle's setup (chunk 0),
 //                 tests 64 tiles, writes a (tile, triangle) pair per tile that
 //                 can hold a covered pixel centre and counts it in its tile.
@@ -109,6 +116,7 @@ struct RasterArgs {
     ClipTri* clipq;
     uint32_t clip_cap;
     Counters* ctr;
+    uint32_t* redo;
     const float* mvp;               // the view's VP M table (kMvpStride floats per transform)
     uint32_t count;                 // meshlets in the list
     uint32_t first;                 // the wave's first meshlet of the list (then every nwaves-th)
@@ -182,53 +190,124 @@ __device__ __forceinline__ void write_fragment(unsigned long long* keys, uint32_
     key_min(keys + (size_t)py * W + px, make_key(z, id));
 }
 
-__device__ __forceinline__ uint32_t big_chunks(const TriSetup& t)
+// Edge k of an oriented big triangle: from vertex (k+1)%3 to (k+2)%3.
+struct Edge {
+    long long ex, ey, ax, ay;
+    bool tl;
+};
+
+__device__ __forceinline__ void edges_of(const int* x, const int* y, Edge* e)
+{
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        const int ka = (k + 1) % 3, kb = (k + 2) % 3;
+        e[k].ex = (long long)x[kb] - x[ka];
+        e[k].ey = (long long)y[kb] - y[ka];
+        e[k].ax = x[ka];
+        e[k].ay = y[ka];
+        e[k].tl = e[k].ey < 0 || (e[k].ey == 0 && e[k].ex > 0);
+    }
+}
+
+__device__ __forceinline__ long long edge_at(const Edge& e, long long sx, long long sy)
+{
+    return e.ex * (sy - e.ay) - e.ey * (sx - e.ax);
+}
+
+// A triangle that finds its queue full is rasterized by the calling lane
+// alone, pixel by pixel over its bounding box, with the tile pass's coverage
+// rule and depth arithmetic: it writes exactly the fragments the binned path
+// would.  Slow for a large triangle, but no triangle is lost.
+__device__ __noinline__ void raster_direct(const TriSetup& t, uint32_t id, unsigned long long* keys, uint32_t W)
+{
+    Edge e[3];
+    edges_of(t.x, t.y, e);
+    float za, zb;
+    depth_plane(t.x, t.y, t.z, t.area, za, zb);
+    for (int py = t.py0; py <= t.py1; ++py) {
+        const long long sy = (long long)py * 256 + 128;
+        const float dyf = (float)(py * 256 + 128 - t.y[0]) * 0.00390625f;
+        for (int px = t.px0; px <= t.px1; ++px) {
+            const long long sx = (long long)px * 256 + 128;
+            bool in = true;
+#pragma unroll
+            for (int q = 0; q < 3; ++q) in = in && edge_at(e[q], sx, sy) - (e[q].tl ? 0 : 1) >= 0;
+            if (!in) continue;
+            const float dxf = (float)(px * 256 + 128 - t.x[0]) * 0.00390625f;
+            const float z = t.z[0] + (za * dxf + zb * dyf);
+            if (!(z >= 0.0f && z <= 1.0f)) continue;
+            write_fragment(keys, W, px, py, z, id);
+        }
+    }
+}
+
+// 64-tile chunks a big triangle reserves in the pair list: none when it
+// needs more than the whole list (it is rasterized directly).
+__device__ __forceinline__ uint32_t reserve_chunks(const TriSetup& t, uint32_t chunk_cap)
 {
     const uint32_t ntx = (uint32_t)(t.px1 / (int)kBinTile - t.px0 / (int)kBinTile + 1);
     const uint32_t nty = (uint32_t)(t.py1 / (int)kBinTile - t.py0 / (int)kBinTile + 1);
-    return (ntx * nty + 63u) >> 6;
+    const uint32_t c = (ntx * nty + 63u) >> 6;
+    return c <= chunk_cap ? c : 0u;
 }
 
-// Stores a big triangle at the reservation `packed` = slot << 32 | first
-// chunk, and marks its chunks as the triangle's in chunk_tri.
-__device__ __forceinline__ void store_big(const TriSetup& t, uint32_t id, unsigned long long packed, uint32_t chunks,
-                                          BigTri* big, uint32_t* big_chunk, uint32_t cap, uint32_t* chunk_tri,
-                                          uint32_t chunk_cap, Counters* ctr)
+// Where a big triangle goes, and what it needs to get there.
+struct BigQueue {
+    BigTri* big;
+    uint32_t* big_chunk;
+    uint32_t cap;
+    uint32_t* chunk_tri;
+    uint32_t chunk_cap;
+    Counters* ctr;
+    unsigned long long* keys;
+    uint32_t W;
+};
+
+// Stores a big triangle in `slot` with its chunks from c0 (chunk_tri marks
+// them as the triangle's) and returns true.  Without room in the queue or in
+// the pair list it returns false, marks its slot empty (big_chunk ~0u; its
+// reserved chunks keep stale chunk_tri entries, which the bin pass rejects)
+// and sets overflow bit 0 (queue) or 2 (pair list): the frame takes the
+// slower direct path for the triangle.
+__device__ __forceinline__ bool store_big(const TriSetup& t, uint32_t id, uint32_t slot, unsigned long long c0,
+                                          uint32_t chunks, const BigQueue& q)
 {
-    const uint32_t slot = (uint32_t)(packed >> 32);
-    if (slot >= cap) { atomicOr(&ctr->overflow, 1u); return; }
-    const uint32_t c0 = (uint32_t)packed;
-    big_chunk[slot] = c0;
-    // past chunk_cap the pass overflows its pair list and is dropped whole
-    for (uint32_t q = 0; q < chunks && c0 + q < chunk_cap; ++q) chunk_tri[c0 + q] = slot;
+    if (chunks == 0u || slot >= q.cap || c0 + chunks > q.chunk_cap) {
+        if (slot < q.cap) q.big_chunk[slot] = 0xFFFFFFFFu;
+        atomicOr(&q.ctr->overflow, slot >= q.cap ? 1u : 4u);
+        return false;
+    }
+    q.big_chunk[slot] = (uint32_t)c0;
+    for (uint32_t k = 0; k < chunks; ++k) q.chunk_tri[c0 + k] = slot;
     BigTri b;
 #pragma unroll
     for (int k = 0; k < 3; ++k) { b.x[k] = t.x[k]; b.y[k] = t.y[k]; b.z[k] = t.z[k]; }
     b.id = id;
     b.bb0 = (uint32_t)t.px0 | ((uint32_t)t.py0 << 16);
     b.bb1 = (uint32_t)t.px1 | ((uint32_t)t.py1 << 16);
-    big[slot] = b;
+    q.big[slot] = b;
+    return true;
 }
 
-// Appends one big triangle: one 64-bit atomic gives its slot (high word) and
-// the first of the 64-tile chunks its bounding box needs (low word), so chunk
-// starts ascend with the slots.
-__device__ __forceinline__ void push_big(const TriSetup& t, uint32_t id, BigTri* big, uint32_t* big_chunk, uint32_t cap,
-                                         uint32_t* chunk_tri, uint32_t chunk_cap, Counters* ctr)
+// Appends one big triangle (one lane): a slot and its chunks, or, without
+// room, the lane rasterizes it directly.
+__device__ __forceinline__ void push_big(const TriSetup& t, uint32_t id, const BigQueue& q)
 {
-    const uint32_t chunks = big_chunks(t);
-    const unsigned long long packed = atomicAdd(&ctr->big_packed, (1ull << 32) | (unsigned long long)chunks);
-    store_big(t, id, packed, chunks, big, big_chunk, cap, chunk_tri, chunk_cap, ctr);
+    const uint32_t chunks = reserve_chunks(t, q.chunk_cap);
+    const uint32_t slot = atomicAdd(&q.ctr->big_count, 1u);
+    const unsigned long long c0 = chunks ? atomicAdd(&q.ctr->big_chunks, (unsigned long long)chunks) : 0ull;
+    if (!store_big(t, id, slot, c0, chunks, q)) raster_direct(t, id, q.keys, q.W);
 }
 
 // Wave-wide append (called by all 64 lanes): a prefix sum of the lanes'
-// (1, chunks) pairs and one atomic per wave reserve consecutive slots and
-// chunks in lane order.
-__device__ __forceinline__ void push_big_wave(bool want, const TriSetup& t, uint32_t id, uint32_t lane, BigTri* big,
-                                              uint32_t* big_chunk, uint32_t cap, uint32_t* chunk_tri, uint32_t chunk_cap,
-                                              Counters* ctr)
+// (1, chunks) pairs and one atomic per counter and wave reserve consecutive
+// slots and chunks in lane order.  A lane's chunk count is at most the pair
+// list's chunk capacity (below 2^19), so the 64-lane sums fit their 32-bit
+// halves.  Returns true on a lane whose triangle found no room.
+__device__ __forceinline__ bool push_big_wave(bool want, const TriSetup& t, uint32_t id, uint32_t lane,
+                                              const BigQueue& q)
 {
-    const uint32_t chunks = want ? big_chunks(t) : 0u;
+    const uint32_t chunks = want ? reserve_chunks(t, q.chunk_cap) : 0u;
     const unsigned long long mine = want ? ((1ull << 32) | (unsigned long long)chunks) : 0ull;
     unsigned long long x = mine;
 #pragma unroll
@@ -237,11 +316,110 @@ __device__ __forceinline__ void push_big_wave(bool want, const TriSetup& t, uint
         if (lane >= (uint32_t)off) x += y;
     }
     const unsigned long long total = __shfl(x, 63);
-    if (total == 0ull) return;   // wave-uniform
+    if (total == 0ull) return false;   // wave-uniform
+    // slot base << 32 | chunk base (saturated: a base past the list fails either way)
     unsigned long long base = 0ull;
-    if (lane == 0) base = atomicAdd(&ctr->big_packed, total);
+    if (lane == 0) {
+        const uint32_t sb = atomicAdd(&q.ctr->big_count, (uint32_t)(total >> 32));
+        const unsigned long long cb =
+            (uint32_t)total ? atomicAdd(&q.ctr->big_chunks, (unsigned long long)(uint32_t)total) : 0ull;
+        base = ((unsigned long long)sb << 32) | (cb < 0xFFFFFFFFull ? cb : 0xFFFFFFFFull);
+    }
     base = __shfl(base, 0);
-    if (want) store_big(t, id, base + (x - mine), chunks, big, big_chunk, cap, chunk_tri, chunk_cap, ctr);
+    const unsigned long long before = x - mine;
+    return want && !store_big(t, id, (uint32_t)(base >> 32) + (uint32_t)(before >> 32),
+                              (base & 0xFFFFFFFFull) + (uint32_t)before, chunks, q);
+}
+
+// Clipping (rare): Sutherland-Hodgman against pa*x + pb*y + pc*z + pd*w >= 0.
+// A convex polygon gains at most one vertex per plane (3 + 5 <= kClipMax);
+// non-finite or degenerate input can produce more crossings: such a polygon
+// is dropped (returns 0) and nothing is written past kClipMax.
+constexpr int kClipMax = 16;
+constexpr uint32_t kClipThreads = 64;   // one wave per clip block; polygons live in LDS
+
+// A lane's polygon in LDS: vertex k at base[k * kClipThreads] (bank-conflict free).
+struct StridedPoly {
+    float4* base;
+    __device__ __forceinline__ float4& operator[](int k) const { return base[k * (int)kClipThreads]; }
+};
+
+// A lane's polygon in its private memory.
+struct LocalPoly {
+    float4* base;
+    __device__ __forceinline__ float4& operator[](int k) const { return base[k]; }
+};
+
+template <typename Poly>
+__device__ __forceinline__ int clip_plane(const Poly in, int n, const Poly out, float pa, float pb, float pc, float pd)
+{
+    int m = 0;
+    for (int i = 0; i < n; ++i) {
+        const float4 p = in[i];
+        const float4 q = in[i + 1 < n ? i + 1 : 0];
+        const float dp = ((pa * p.x + pb * p.y) + pc * p.z) + pd * p.w;
+        const float dq = ((pa * q.x + pb * q.y) + pc * q.z) + pd * q.w;
+        if (dp >= 0.0f) {
+            if (m < kClipMax) out[m] = p;
+            ++m;
+        }
+        if ((dp >= 0.0f) != (dq >= 0.0f)) {
+            const float t = dp / (dp - dq);
+            float4 r;
+            r.x = p.x + t * (q.x - p.x);
+            r.y = p.y + t * (q.y - p.y);
+            r.z = p.z + t * (q.z - p.z);
+            r.w = p.w + t * (q.w - p.w);
+            if (m < kClipMax) out[m] = r;
+            ++m;
+        }
+    }
+    return m <= kClipMax ? m : 0;
+}
+
+// Near plane (z <= w) then the four guard planes; each triangle of the fan
+// (v0, vi, vi+1) of the clipped polygon (at most 3 + 5 vertices) goes to
+// `emit`.  pa and pb are the lane's two polygon buffers.
+template <typename Poly, typename Emit>
+__device__ __forceinline__ void clip_triangle(const Poly pa, const Poly pb, float4 c0, float4 c1, float4 c2, uint32_t W,
+                                              uint32_t H, Counters* ctr, Emit emit)
+{
+    const float gx = kClipGuardPx / (float)W, gy = kClipGuardPx / (float)H;
+    const float hw = (float)W * 0.5f, hh = (float)H * 0.5f;
+    pa[0] = c0; pa[1] = c1; pa[2] = c2;
+    int m = 3;
+    m = clip_plane(pa, m, pb, 0.0f, 0.0f, -1.0f, 1.0f);
+    m = clip_plane(pb, m, pa, -1.0f, 0.0f, 0.0f, gx);
+    m = clip_plane(pa, m, pb, 1.0f, 0.0f, 0.0f, gx);
+    m = clip_plane(pb, m, pa, 0.0f, -1.0f, 0.0f, gy);
+    m = clip_plane(pa, m, pb, 0.0f, 1.0f, 0.0f, gy);
+    if (m < 3) return;
+    if (ctr) atomicAdd(&ctr->clipped, 1u);
+    bool ok = true;
+    for (int k = 0; k < m && ok; ++k) {
+        SVtx sv;
+        ok = project_vtx(pb[k], hw, hh, sv);
+    }
+    if (!ok) return;
+    SVtx s0;
+    project_vtx(pb[0], hw, hh, s0);
+    for (int k = 1; k + 1 < m; ++k) {
+        SVtx s1, s2;
+        project_vtx(pb[k], hw, hh, s1);
+        project_vtx(pb[k + 1], hw, hh, s2);
+        TriSetup t;
+        if (setup_tri(s0, s1, s2, (int)W, (int)H, t)) emit(t);
+    }
+}
+
+// The calling lane clips a triangle in private memory and rasterizes the fan
+// directly (not counted in the clip statistics).
+__device__ __noinline__ void clip_direct(float4 c0, float4 c1, float4 c2, uint32_t id, uint32_t W, uint32_t H,
+                                         unsigned long long* keys)
+{
+    float4 A[kClipMax], B[kClipMax];
+    clip_triangle(LocalPoly{ A }, LocalPoly{ B }, c0, c1, c2, W, H, nullptr,
+                  [&](const TriSetup& t) { raster_direct(t, id, keys, W); });
 }
 
 __device__ __forceinline__ uint32_t tri_byte(const uint32_t* words, uint32_t b)
@@ -268,6 +446,10 @@ __device__ __forceinline__ void raster_view(const RasterArgs& a)
     const uint32_t nwaves = gridDim.x * kWaves;
     const float hw = (float)a.W * 0.5f, hh = (float)a.H * 0.5f;
     const int W = (int)a.W, H = (int)a.H;
+    const BigQueue bq{ a.big, a.big_chunk, a.big_cap, a.chunk_tri, a.chunk_cap, a.ctr, a.keys, a.W };
+    // non-zero when a triangle of the wave's meshlet found a queue full
+    __shared__ uint32_t s_spill[kWaves];
+    if (lane == 0) s_spill[w] = 0u;
 
     // The raster record of the wave's next meshlet is loaded while the
     // current one is processed.
@@ -375,6 +557,7 @@ __device__ __forceinline__ void raster_view(const RasterArgs& a)
                         a.clipq[slot] = ct;
                     } else {
                         atomicOr(&a.ctr->overflow, 2u);
+                        s_spill[w] = 1u;
                     }
                     continue;
                 }
@@ -406,7 +589,7 @@ __device__ __forceinline__ void raster_view(const RasterArgs& a)
                     want_big = true;
                 }
             } while (false);
-            push_big_wave(want_big, ts, id, lane, a.big, a.big_chunk, a.big_cap, a.chunk_tri, a.chunk_cap, a.ctr);
+            if (push_big_wave(want_big, ts, id, lane, bq)) s_spill[w] = 1u;
         }
 
         // exclusive prefix of pixel counts over the 128 triangle slots
@@ -474,6 +657,14 @@ __device__ __forceinline__ void raster_view(const RasterArgs& a)
             }
         }
         wave_sync();
+        // the meshlet is redone by the direct path after this kernel
+        if (__builtin_amdgcn_readfirstlane(s_spill[w])) {
+            if (lane == 0) {
+                a.redo[atomicAdd(&a.ctr->redo_count, 1u)] = idx;
+                s_spill[w] = 0u;
+            }
+            wave_sync();
+        }
         idx = nidx;      // static stride: no shared cursor
     }
 }
@@ -500,110 +691,19 @@ struct BigArgs {
     TileItem* items;
 };
 
-// Clipping (rare): Sutherland-Hodgman against pa*x + pb*y + pc*z + pd*w >= 0.
-// A convex polygon gains at most one vertex per plane (3 + 5 <= kClipMax);
-// non-finite or degenerate input can produce more crossings: such a polygon
-// is dropped (returns 0) and nothing is written past kClipMax.
-constexpr int kClipMax = 16;
-constexpr uint32_t kClipThreads = 64;   // one wave per clip block; polygons live in LDS
-
-// A lane's polygon in LDS: vertex k at base[k * kClipThreads] (bank-conflict free).
-struct StridedPoly {
-    float4* base;
-    __device__ __forceinline__ float4& operator[](int k) const { return base[k * (int)kClipThreads]; }
-};
-
-__device__ __forceinline__ int clip_plane(const StridedPoly in, int n, const StridedPoly out, float pa, float pb, float pc,
-                                          float pd)
-{
-    int m = 0;
-    for (int i = 0; i < n; ++i) {
-        const float4 p = in[i];
-        const float4 q = in[i + 1 < n ? i + 1 : 0];
-        const float dp = ((pa * p.x + pb * p.y) + pc * p.z) + pd * p.w;
-        const float dq = ((pa * q.x + pb * q.y) + pc * q.z) + pd * q.w;
-        if (dp >= 0.0f) {
-            if (m < kClipMax) out[m] = p;
-            ++m;
-        }
-        if ((dp >= 0.0f) != (dq >= 0.0f)) {
-            const float t = dp / (dp - dq);
-            float4 r;
-            r.x = p.x + t * (q.x - p.x);
-            r.y = p.y + t * (q.y - p.y);
-            r.z = p.z + t * (q.z - p.z);
-            r.w = p.w + t * (q.w - p.w);
-            if (m < kClipMax) out[m] = r;
-            ++m;
-        }
-    }
-    return m <= kClipMax ? m : 0;
-}
-
-// Near plane (z <= w) then the four guard planes; the fan (v0, vi, vi+1) of
-// the clipped polygon (at most 3 + 5 vertices) joins the big triangles.  Each
-// lane clips one queued triangle in its own LDS polygon buffers.
+// One lane per queued triangle, in its own LDS polygon buffers; the fan
+// joins the big triangles.
 __device__ __forceinline__ void clip_view(const BigArgs& a)
 {
     __shared__ float4 s_poly[2][kClipMax][kClipThreads];
-    float4* A = &s_poly[0][0][threadIdx.x];
-    float4* B = &s_poly[1][0][threadIdx.x];
+    const StridedPoly pa{ &s_poly[0][0][threadIdx.x] }, pb{ &s_poly[1][0][threadIdx.x] };
+    const BigQueue q{ a.big, a.big_chunk, a.big_cap, a.chunk_tri, a.chunk_cap, a.ctr, a.keys, a.W };
     const uint32_t n = min(a.ctr->clip_count, a.clip_cap);
-    const float gx = kClipGuardPx / (float)a.W, gy = kClipGuardPx / (float)a.H;
-    const float hw = (float)a.W * 0.5f, hh = (float)a.H * 0.5f;
     for (uint32_t i = blockIdx.x * kClipThreads + threadIdx.x; i < n; i += gridDim.x * kClipThreads) {
         const ClipTri ct = a.clipq[i];
-        StridedPoly pa{ A }, pb{ B };
-        pa[0] = ct.c[0]; pa[1] = ct.c[1]; pa[2] = ct.c[2];
-        int m = 3;
-        m = clip_plane(pa, m, pb, 0.0f, 0.0f, -1.0f, 1.0f);
-        m = clip_plane(pb, m, pa, -1.0f, 0.0f, 0.0f, gx);
-        m = clip_plane(pa, m, pb, 1.0f, 0.0f, 0.0f, gx);
-        m = clip_plane(pb, m, pa, 0.0f, -1.0f, 0.0f, gy);
-        m = clip_plane(pa, m, pb, 0.0f, 1.0f, 0.0f, gy);
-        if (m < 3) continue;
-        atomicAdd(&a.ctr->clipped, 1u);
-        bool ok = true;
-        for (int k = 0; k < m && ok; ++k) {
-            SVtx sv;
-            ok = project_vtx(pb[k], hw, hh, sv);
-        }
-        if (!ok) continue;
-        SVtx s0;
-        project_vtx(pb[0], hw, hh, s0);
-        for (int k = 1; k + 1 < m; ++k) {
-            SVtx s1, s2;
-            project_vtx(pb[k], hw, hh, s1);
-            project_vtx(pb[k + 1], hw, hh, s2);
-            TriSetup t;
-            if (setup_tri(s0, s1, s2, (int)a.W, (int)a.H, t))
-                push_big(t, ct.id, a.big, a.big_chunk, a.big_cap, a.chunk_tri, a.chunk_cap, a.ctr);
-        }
+        clip_triangle(pa, pb, ct.c[0], ct.c[1], ct.c[2], a.W, a.H, a.ctr,
+                      [&](const TriSetup& t) { push_big(t, ct.id, q); });
     }
-}
-
-// Edge k of an oriented big triangle: from vertex (k+1)%3 to (k+2)%3.
-struct Edge {
-    long long ex, ey, ax, ay;
-    bool tl;
-};
-
-__device__ __forceinline__ void tri_edges(const BigTri& t, Edge* e)
-{
-#pragma unroll
-    for (int k = 0; k < 3; ++k) {
-        const int ka = (k + 1) % 3, kb = (k + 2) % 3;
-        e[k].ex = (long long)t.x[kb] - t.x[ka];
-        e[k].ey = (long long)t.y[kb] - t.y[ka];
-        e[k].ax = t.x[ka];
-        e[k].ay = t.y[ka];
-        e[k].tl = e[k].ey < 0 || (e[k].ey == 0 && e[k].ex > 0);
-    }
-}
-
-__device__ __forceinline__ long long edge_at(const Edge& e, long long sx, long long sy)
-{
-    return e.ex * (sy - e.ay) - e.ey * (sx - e.ax);
 }
 
 __device__ __forceinline__ uint32_t chunks_of(const BigTri& t)
@@ -615,15 +715,12 @@ __device__ __forceinline__ uint32_t chunks_of(const BigTri& t)
     return (ntx * nty + 63u) >> 6;
 }
 
-// Chunks of the pass and the triangles they belong to (slots past the queue
-// capacity were dropped together with their chunks).
+// Big-triangle slots of this pass (stored or not) and the chunks of the pair
+// list in use.
 __device__ __forceinline__ void pass_extent(const BigArgs& a, uint32_t& nbig, uint32_t& nchunks)
 {
-    const unsigned long long packed = a.ctr->big_packed;
-    const uint32_t raw = (uint32_t)(packed >> 32);
-    nbig = min(raw, a.big_cap);
-    nchunks = (uint32_t)packed;
-    if (raw > a.big_cap && nbig) nchunks = a.big_chunk[nbig - 1] + chunks_of(a.big[nbig - 1]);
+    nbig = min(a.ctr->big_count, a.big_cap);
+    nchunks = (uint32_t)min(a.ctr->big_chunks, (unsigned long long)a.chunk_cap);
 }
 
 // Work items of a tile whose list holds n triangles (0 for an empty tile).
@@ -645,13 +742,25 @@ __device__ __forceinline__ void bin_view(const BigArgs& a)
     uint32_t nbig, nchunks;
     pass_extent(a, nbig, nchunks);
     if (nchunks == 0) return;   // uniform over the grid: no big triangle this pass
-    const bool pair_overflow = (unsigned long long)nchunks * 64ull > a.bin_cap;
-    for (uint32_t c = gw; c < nchunks && !pair_overflow; c += nw) {
+    for (uint32_t c = gw; c < nchunks; c += nw) {
+        // the chunk is in use iff its entry names a slot stored in this pass
+        // whose range holds it (chunks reserved by a triangle that found no
+        // room keep an entry of an earlier pass, or none); uniform over the wave
         const uint32_t ti = a.chunk_tri[c];
-        const BigTri t = a.big[ti];
-        const uint32_t j = c - a.big_chunk[ti];
+        const uint32_t c0 = ti < nbig ? a.big_chunk[ti] : 0xFFFFFFFFu;
+        BigTri t;
+        bool used = c0 <= c;
+        if (used) {
+            t = a.big[ti];
+            used = c - c0 < chunks_of(t);
+        }
+        if (!used) {
+            a.pairs[64u * c + lane] = BinPair{ 0xFFFFFFFFu, 0u };
+            continue;
+        }
+        const uint32_t j = c - c0;
         Edge e[3];
-        tri_edges(t, e);
+        edges_of(t.x, t.y, e);
         const int px0 = (int)(t.bb0 & 0xFFFFu), py0 = (int)(t.bb0 >> 16);
         const int px1 = (int)(t.bb1 & 0xFFFFu), py1 = (int)(t.bb1 >> 16);
         if (j == 0 && lane == 0) {
@@ -702,8 +811,6 @@ __device__ __forceinline__ void bin_view(const BigArgs& a)
         }
         a.pairs[64u * c + lane] = bp;
     }
-
-    if (pair_overflow && blockIdx.x == 0 && threadIdx.x == 0) atomicOr(&a.ctr->overflow, 4u);
 }
 
 // Exclusive scans (one block) of the tile counts and of the tile pass's work
@@ -798,8 +905,7 @@ __device__ __forceinline__ void bin_fill_view(const BigArgs& a)
 {
     uint32_t nbig, nchunks;
     pass_extent(a, nbig, nchunks);
-    const unsigned long long np = (unsigned long long)nchunks * 64ull;
-    if (np > a.bin_cap) return;   // overflow: this pass's big triangles are dropped
+    const unsigned long long np = (unsigned long long)nchunks * 64ull;   // at most the pair capacity
     for (uint32_t i = blockIdx.x * 256u + threadIdx.x; i < (uint32_t)np; i += gridDim.x * 256u) {
         const BinPair p = a.pairs[i];
         if (p.tile == 0xFFFFFFFFu) continue;
@@ -959,7 +1065,8 @@ struct RasterBatchArgs {
 // as one concatenated list: view j's meshlet i goes to wave
 // (i + meshlets of the views before j) mod waves, so every wave gets within
 // one meshlet of the average however the meshlets spread over the views.
-__global__ __launch_bounds__(kBlock) void raster_kernel(RasterBatchArgs ab)
+// Three waves per SIMD: the register budget (168) the kernel fits without spilling.
+__global__ __launch_bounds__(kBlock, 3) void raster_kernel(RasterBatchArgs ab)
 {
     const uint32_t lane = threadIdx.x & 63u;
     const uint32_t gw = blockIdx.x * kWaves + (threadIdx.x >> 6);
@@ -999,6 +1106,7 @@ __global__ __launch_bounds__(kBlock) void raster_kernel(RasterBatchArgs ab)
             a.big = v.big;
             a.big_chunk = v.big_chunk;
             a.big_cap = v.big_capacity;
+            a.redo = v.redo;
             a.chunk_tri = v.chunk_tri;
             a.chunk_cap = v.bin_capacity / 64u;
             a.clipq = v.clipq;
@@ -1027,15 +1135,78 @@ __device__ __forceinline__ uint32_t lane_view_value(const BatchDev& b, uint32_t 
     return v < b.nviews ? field(*batch_view(b, v).counters) : 0u;
 }
 
-__device__ __forceinline__ uint32_t big_count(const Counters& c) { return (uint32_t)(c.big_packed >> 32); }
+__device__ __forceinline__ uint32_t big_count(const Counters& c) { return c.big_count; }
 
-__global__ __launch_bounds__(kClipThreads) void clip_kernel(BatchDev b)
+// What the direct path needs to redo a pass's meshlets.
+struct RedoArgs {
+    SceneDev s;
+    uint32_t flags;
+    uint32_t list;
+    int small_lim;
+};
+
+// The meshlets of this pass that had a triangle without room in a queue: one
+// wave per meshlet, one lane per triangle, with the raster's own vertex,
+// culling and setup arithmetic; every triangle of theirs that the raster did
+// not keep in its wave (not small) is rasterized directly, clipped first if
+// it needs clipping.  A triangle that did reach a queue is written twice; the
+// visibility write is a minimum, so the result is the same.
+__device__ __forceinline__ void redo_view(const RedoArgs& ra, const ViewDev& v, uint32_t n)
+{
+    const RasterRec* list = v.visible[ra.list];
+    const float hw = (float)v.width * 0.5f, hh = (float)v.height * 0.5f;
+    const int W = (int)v.width, H = (int)v.height;
+    for (uint32_t r = blockIdx.x; r < n; r += gridDim.x) {
+        const RasterRec rec = list[v.redo[r]];
+        const float* mrow = v.mvp + (size_t)rec.transform * kMvpStride;
+        const Mat4 mvp = load_mat4(mrow);
+        const float tdet = mrow[16];
+        const bool cull_faces = !(rec.material >> 31) && !(ra.flags & VKGV_NO_BACKFACE);
+        const uint32_t tc = (rec.counts >> 8) & 0xFFu;
+        const uint32_t* words = reinterpret_cast<const uint32_t*>(ra.s.tri_base + rec.tri);
+        const float* vtx = reinterpret_cast<const float*>(reinterpret_cast<const uint8_t*>(ra.s.vtx_base) + rec.vtx);
+        for (uint32_t t = threadIdx.x; t < tc; t += kClipThreads) {
+            float4 c[3];
+            SVtx sv[3];
+            bool ok = true;
+#pragma unroll
+            for (int k = 0; k < 3; ++k) {
+                const uint32_t vi = ra.s.vidx_base[rec.vidx + tri_byte(words, 3u * t + (uint32_t)k)];
+                const float* pp = vtx + (size_t)vi * 6;
+                c[k] = mat4_point(mvp, pp[0], pp[1], pp[2]);
+                sv[k].x = 0; sv[k].y = 0; sv[k].z = 0.0f;
+                sv[k].ok = project_vtx(c[k], hw, hh, sv[k]) ? 1u : 0u;
+                ok = ok && sv[k].ok;
+            }
+            if (cull_faces) {
+                const float det = det3_xyw(c[0], c[1], c[2]);
+                if (tdet < 0.0f ? det < 0.0f : det > 0.0f) continue;
+            }
+            const uint32_t id = (rec.draw << VKGV_TRIANGLE_BITS) | t;
+            if (!ok) {
+                clip_direct(c[0], c[1], c[2], id, v.width, v.height, v.keys);
+                continue;
+            }
+            TriSetup ts;
+            if (!setup_tri(sv[0], sv[1], sv[2], W, H, ts)) continue;
+            if (ts.maxx - ts.minx < ra.small_lim && ts.maxy - ts.miny < ra.small_lim) continue;
+            raster_direct(ts, id, v.keys, v.width);
+        }
+    }
+}
+
+// Per view: the clip queue, then the meshlets to redo.
+__global__ __launch_bounds__(kClipThreads) void clip_kernel(BatchDev b, RedoArgs ra)
 {
     for (uint32_t v0 = 0; v0 < b.nviews; v0 += 64u) {
         const uint32_t n = lane_view_value(b, v0, [](const Counters& c) { return c.clip_count; });
+        const uint32_t nr = lane_view_value(b, v0, [](const Counters& c) { return c.redo_count; });
         const uint32_t m = min(64u, b.nviews - v0);
-        for (uint32_t j = 0; j < m; ++j)
+        for (uint32_t j = 0; j < m; ++j) {
             if (__builtin_amdgcn_readlane(n, j)) clip_view(big_args(batch_view(b, v0 + j)));
+            const uint32_t r = __builtin_amdgcn_readlane(nr, j);
+            if (r) redo_view(ra, batch_view(b, v0 + j), r);
+        }
     }
 }
 
@@ -1074,7 +1245,9 @@ __global__ __launch_bounds__(256) void tile_kernel(BatchDev b)
         const uint32_t n = lane_view_value(b, v0, [](const Counters& c) { return c.tile_items; });
         if (wave == 0 && v0 + (threadIdx.x & 63u) < b.nviews) {
             Counters* c = batch_view(b, v0 + (threadIdx.x & 63u)).counters;
-            c->big_packed = 0ull;
+            c->big_chunks = 0ull;
+            c->big_count = 0u;
+            c->redo_count = 0u;
             c->clip_count = 0u;
         }
         const uint32_t m = min(64u, b.nviews - v0);
@@ -1098,10 +1271,15 @@ void launch_raster(const SceneDev& s, const BatchDev& b, uint32_t flags, uint32_
     raster_kernel<<<grid_blocks, kBlock, 0, st>>>(a);
 }
 
-void launch_raster_big(const BatchDev& b, const BatchGrid& g, uint32_t tile_grid, uint32_t tile_item_tris,
-                       hipStream_t st)
+void launch_raster_big(const SceneDev& s, const BatchDev& b, const BatchGrid& g, uint32_t flags, uint32_t list,
+                       uint32_t small_max_extent, uint32_t tile_grid, uint32_t tile_item_tris, hipStream_t st)
 {
-    clip_kernel<<<64, kClipThreads, 0, st>>>(b);
+    RedoArgs ra;
+    ra.s = s;
+    ra.flags = flags;
+    ra.list = list;
+    ra.small_lim = (int)(small_max_extent * 256u);
+    clip_kernel<<<64, kClipThreads, 0, st>>>(b, ra);
     bin_kernel<<<1024, 256, 0, st>>>(b);
     bin_scan_kernel<<<g.nviews, kScanThreads, 0, st>>>(b, tile_item_tris);
     bin_fill_kernel<<<256, 256, 0, st>>>(b);
