@@ -938,7 +938,9 @@ int vkgv_view_create(vkgv_ctx* c, uint32_t width, uint32_t height, vkgv_view** o
     if (!rc) rc = dalloc(c, v->big_chunk, sizeof(uint32_t) * (size_t)c->big_capacity);
     const uint32_t tiles_x = (width + kBinTile - 1) / kBinTile, tiles_y = (height + kBinTile - 1) / kBinTile;
     const uint32_t ntiles = tiles_x * tiles_y;
-    const uint32_t bin_cap = std::max<uint32_t>(1u << 21, 64u * ntiles);
+    // tile-pair list: 1024 pairs (16 chunks) per screen tile, 2^21 .. 2^24
+    // (at most 2^18 chunks, which keeps a wave's chunk sum within 32 bits)
+    const uint32_t bin_cap = std::min<uint32_t>(std::max<uint32_t>(1u << 21, 1024u * ntiles), 1u << 24);
     if (!rc) rc = dalloc(c, v->clipq, sizeof(ClipTri) * (size_t)c->clip_capacity);
     if (!rc) rc = dalloc(c, v->bin_count, sizeof(uint32_t) * ntiles);
     if (!rc) rc = dalloc(c, v->bin_start, sizeof(uint32_t) * ntiles);
