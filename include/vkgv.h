@@ -197,7 +197,7 @@ typedef struct vkgv_tuning {
                                         the meshlet's own wave, larger ones are binned (1..64, default 16) */
     uint32_t raster_blocks_per_cu;   /* persistent meshlet-raster grid = CUs * this (1..16, default 3) */
     uint32_t tile_item_tris;         /* binned path: a tile whose list holds more triangles is split into
-                                        up to 8 work items of about this many (1..256, default 16) */
+                                        up to 8 work items of about this many (1..256, default 8) */
     uint32_t resolve_cache_slots;    /* rasterized meshlets per view whose triangles are cached for the
                                         barycentric resolve (0..8192, default 8192); the resolve re-fetches
                                         and re-transforms the triangles of the others (same results) */

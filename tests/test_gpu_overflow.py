@@ -23,8 +23,8 @@ from vk_gltf_viewer_amd.scene import camera_from_view, view_desc
 pytestmark = pytest.mark.gpu
 
 W, H = 640, 480
-DEFAULT_TUNING = (16, 3, 16, 8192)
-ONE_PIXEL = (1, 3, 16, 8192)          # every triangle of a bounding box >= 1 px goes to the queues
+DEFAULT_TUNING = (16, 3, 8, 8192)
+ONE_PIXEL = (1, 3, 8, 8192)          # every triangle of a bounding box >= 1 px goes to the queues
 FLAGS = [N.MODE_REFERENCE, N.CULL_FRUSTUM | N.CULL_TWO_PASS | N.RESOLVE_BARY]
 
 

@@ -12,8 +12,8 @@ from vk_gltf_viewer_amd.scene import Scene, camera_from_view
 
 pytestmark = pytest.mark.gpu
 
-DEFAULT = (16, 3, 16, 8192)
-TUNINGS = [(4, 2, 8, 0), (8, 3, 32, 8192), (32, 1, 1, 0), (16, 16, 256, 100), (64, 3, 16, 1)]
+DEFAULT = (16, 3, 8, 8192)
+TUNINGS = [(4, 2, 16, 0), (8, 3, 32, 8192), (32, 1, 1, 0), (16, 16, 256, 100), (64, 3, 16, 1)]
 FLAGS = [N.CULL_FRUSTUM | N.CULL_TWO_PASS | N.RESOLVE_BARY, N.MODE_REFERENCE]
 
 

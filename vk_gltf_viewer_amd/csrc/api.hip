@@ -53,7 +53,7 @@ struct vkgv_ctx {
     uint32_t draw_count = 0, transform_count = 0;
     uint32_t max_transform = 0;
     bool draws_set = false;
-    RasterTuning tuning{ 16u, 3u, 16u, 8192u };
+    RasterTuning tuning{ 16u, 3u, 8u, 8192u };
     DevBuf srgb;                     // 255 sRGB-encode thresholds
     DevBuf srgb_decode;              // toLinear of each byte / 255 (256 floats)
     DevBuf tex_recs, texels, mat_rot;

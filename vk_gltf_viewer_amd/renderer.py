@@ -121,7 +121,7 @@ class Renderer:
     def sync(self):
         self._check(self._lib.vkgv_sync(self._h))
 
-    def set_tuning(self, small_max_extent: int, raster_blocks_per_cu: int, tile_item_tris: int = 16,
+    def set_tuning(self, small_max_extent: int, raster_blocks_per_cu: int, tile_item_tris: int = 8,
                    resolve_cache_slots: int = 8192):
         t = N.Tuning(small_max_extent, raster_blocks_per_cu, tile_item_tris, resolve_cache_slots)
         self._check(self._lib.vkgv_set_tuning(self._h, C.byref(t)))
