@@ -747,13 +747,14 @@ __device__ __forceinline__ void bin_view(const BigArgs& a)
         // whose range holds it (chunks reserved by a triangle that found no
         // room keep an entry of an earlier pass, or none); uniform over the wave
         const uint32_t ti = a.chunk_tri[c];
-        const uint32_t c0 = ti < nbig ? a.big_chunk[ti] : 0xFFFFFFFFu;
+        const bool known = ti < nbig;
+        uint32_t c0 = 0xFFFFFFFFu;
         BigTri t;
-        bool used = c0 <= c;
-        if (used) {
+        if (known) {   // both loads in flight together
+            c0 = a.big_chunk[ti];
             t = a.big[ti];
-            used = c - c0 < chunks_of(t);
         }
+        const bool used = known && c0 <= c && c - c0 < chunks_of(t);
         if (!used) {
             a.pairs[64u * c + lane] = BinPair{ 0xFFFFFFFFu, 0u };
             continue;
