@@ -26,11 +26,11 @@
 //                 meshlet with a triangle that found either full is listed for
 //                 the direct path.
 //  clip_kernel    clips the queued triangles; their fans join the big queue
-//                 (or, without room, are rasterized directly by their lane).
-//                 Then redoes the listed meshlets: each of their triangles that
-//                 is not small is rasterized directly, pixel by pixel over its
-//                 bounding box by one lane, so that no triangle is lost when a
-//                 queue is full (slower, same result).
+//                 (or, without room, the direct path).  Then redoes the listed
+//                 meshlets: each of their triangles that is not small takes the
+//                 direct path, where the wave sweeps one triangle's bounding
+//                 box at a time, so that no triangle is lost when a queue is
+//                 full (slower, same result).
 //  bin_kernel     one wave per chunk: stores the triangle's setup (chunk 0),
 //                 tests 64 tiles, writes a (tile, triangle) pair per tile that
 //                 can hold a covered pixel centre and counts it in its tile.
@@ -214,29 +214,53 @@ __device__ __forceinline__ long long edge_at(const Edge& e, long long sx, long l
     return e.ex * (sy - e.ay) - e.ey * (sx - e.ax);
 }
 
-// A triangle that finds its queue full is rasterized by the calling lane
-// alone, pixel by pixel over its bounding box, with the tile pass's coverage
-// rule and depth arithmetic: it writes exactly the fragments the binned path
-// would.  Slow for a large triangle, but no triangle is lost.
-__device__ __noinline__ void raster_direct(const TriSetup& t, uint32_t id, unsigned long long* keys, uint32_t W)
+// Triangles that found a queue full: every lane with `want` set hands its
+// triangle to the whole wave, one triangle at a time, and the wave sweeps its
+// bounding box in 8x8-pixel blocks with the tile pass's coverage rule and
+// depth expression, so it writes exactly the fragments the binned path would.
+// Called by all 64 lanes of a wave.  Slower than the binned path, but no
+// triangle is lost.
+__device__ void wave_sweep(bool want, const TriSetup& t, uint32_t id, unsigned long long* keys, uint32_t W)
 {
-    Edge e[3];
-    edges_of(t.x, t.y, e);
-    float za, zb;
-    depth_plane(t.x, t.y, t.z, t.area, za, zb);
-    for (int py = t.py0; py <= t.py1; ++py) {
-        const long long sy = (long long)py * 256 + 128;
-        const float dyf = (float)(py * 256 + 128 - t.y[0]) * 0.00390625f;
-        for (int px = t.px0; px <= t.px1; ++px) {
-            const long long sx = (long long)px * 256 + 128;
-            bool in = true;
+    const uint32_t lane = threadIdx.x & 63u;
+    const int lx = (int)(lane & 7u), ly = (int)(lane >> 3);
+    unsigned long long pending = __ballot(want);
+    while (pending != 0ull) {
+        const int j = __ffsll((long long)pending) - 1;
+        pending &= pending - 1ull;
+        int x[3], y[3];
+        float z[3];
 #pragma unroll
-            for (int q = 0; q < 3; ++q) in = in && edge_at(e[q], sx, sy) - (e[q].tl ? 0 : 1) >= 0;
-            if (!in) continue;
-            const float dxf = (float)(px * 256 + 128 - t.x[0]) * 0.00390625f;
-            const float z = t.z[0] + (za * dxf + zb * dyf);
-            if (!(z >= 0.0f && z <= 1.0f)) continue;
-            write_fragment(keys, W, px, py, z, id);
+        for (int k = 0; k < 3; ++k) {
+            x[k] = __shfl(t.x[k], j);
+            y[k] = __shfl(t.y[k], j);
+            z[k] = __shfl(t.z[k], j);
+        }
+        const int px0 = __shfl(t.px0, j), px1 = __shfl(t.px1, j);
+        const int py0 = __shfl(t.py0, j), py1 = __shfl(t.py1, j);
+        const long long area = __shfl(t.area, j);
+        const uint32_t tid = __shfl(id, j);
+        Edge e[3];
+        edges_of(x, y, e);
+        float za, zb;
+        depth_plane(x, y, z, area, za, zb);
+        for (int by = py0; by <= py1; by += 8) {
+            const int py = by + ly;
+            const long long sy = (long long)py * 256 + 128;
+            const float dyf = (float)(py * 256 + 128 - y[0]) * 0.00390625f;
+            for (int bx = px0; bx <= px1; bx += 8) {
+                const int px = bx + lx;
+                if (px > px1 || py > py1) continue;
+                const long long sx = (long long)px * 256 + 128;
+                bool in = true;
+#pragma unroll
+                for (int q = 0; q < 3; ++q) in = in && edge_at(e[q], sx, sy) - (e[q].tl ? 0 : 1) >= 0;
+                if (!in) continue;
+                const float dxf = (float)(px * 256 + 128 - x[0]) * 0.00390625f;
+                const float zz = z[0] + (za * dxf + zb * dyf);
+                if (!(zz >= 0.0f && zz <= 1.0f)) continue;
+                write_fragment(keys, W, px, py, zz, tid);
+            }
         }
     }
 }
@@ -289,14 +313,14 @@ __device__ __forceinline__ bool store_big(const TriSetup& t, uint32_t id, uint32
     return true;
 }
 
-// Appends one big triangle (one lane): a slot and its chunks, or, without
-// room, the lane rasterizes it directly.
-__device__ __forceinline__ void push_big(const TriSetup& t, uint32_t id, const BigQueue& q)
+// Appends one big triangle (one lane): a slot and its chunks.  Returns false
+// when there was no room.
+__device__ __forceinline__ bool push_big(const TriSetup& t, uint32_t id, const BigQueue& q)
 {
     const uint32_t chunks = reserve_chunks(t, q.chunk_cap);
     const uint32_t slot = atomicAdd(&q.ctr->big_count, 1u);
     const unsigned long long c0 = chunks ? atomicAdd(&q.ctr->big_chunks, (unsigned long long)chunks) : 0ull;
-    if (!store_big(t, id, slot, c0, chunks, q)) raster_direct(t, id, q.keys, q.W);
+    return store_big(t, id, slot, c0, chunks, q);
 }
 
 // Wave-wide append (called by all 64 lanes): a prefix sum of the lanes'
@@ -377,15 +401,14 @@ __device__ __forceinline__ int clip_plane(const Poly in, int n, const Poly out, 
     return m <= kClipMax ? m : 0;
 }
 
-// Near plane (z <= w) then the four guard planes; each triangle of the fan
-// (v0, vi, vi+1) of the clipped polygon (at most 3 + 5 vertices) goes to
-// `emit`.  pa and pb are the lane's two polygon buffers.
-template <typename Poly, typename Emit>
-__device__ __forceinline__ void clip_triangle(const Poly pa, const Poly pb, float4 c0, float4 c1, float4 c2, uint32_t W,
-                                              uint32_t H, Counters* ctr, Emit emit)
+// Near plane (z <= w) then the four guard planes.  pa and pb are the lane's
+// two polygon buffers; the clipped polygon (at most 3 + 5 vertices) ends in
+// pb.  Returns its vertex count, or 0 when fewer than 3 remain.
+template <typename Poly>
+__device__ __forceinline__ int clip_polygon(const Poly pa, const Poly pb, float4 c0, float4 c1, float4 c2, uint32_t W,
+                                            uint32_t H)
 {
     const float gx = kClipGuardPx / (float)W, gy = kClipGuardPx / (float)H;
-    const float hw = (float)W * 0.5f, hh = (float)H * 0.5f;
     pa[0] = c0; pa[1] = c1; pa[2] = c2;
     int m = 3;
     m = clip_plane(pa, m, pb, 0.0f, 0.0f, -1.0f, 1.0f);
@@ -393,33 +416,31 @@ __device__ __forceinline__ void clip_triangle(const Poly pa, const Poly pb, floa
     m = clip_plane(pa, m, pb, 1.0f, 0.0f, 0.0f, gx);
     m = clip_plane(pb, m, pa, 0.0f, -1.0f, 0.0f, gy);
     m = clip_plane(pa, m, pb, 0.0f, 1.0f, 0.0f, gy);
-    if (m < 3) return;
-    if (ctr) atomicAdd(&ctr->clipped, 1u);
+    return m >= 3 ? m : 0;
+}
+
+// Whether every vertex of the clipped polygon projects (w > 0, inside the
+// guard band); a polygon that does not is dropped.
+template <typename Poly>
+__device__ __forceinline__ bool polygon_projects(const Poly p, int m, float hw, float hh)
+{
     bool ok = true;
     for (int k = 0; k < m && ok; ++k) {
         SVtx sv;
-        ok = project_vtx(pb[k], hw, hh, sv);
+        ok = project_vtx(p[k], hw, hh, sv);
     }
-    if (!ok) return;
-    SVtx s0;
-    project_vtx(pb[0], hw, hh, s0);
-    for (int k = 1; k + 1 < m; ++k) {
-        SVtx s1, s2;
-        project_vtx(pb[k], hw, hh, s1);
-        project_vtx(pb[k + 1], hw, hh, s2);
-        TriSetup t;
-        if (setup_tri(s0, s1, s2, (int)W, (int)H, t)) emit(t);
-    }
+    return ok;
 }
 
-// The calling lane clips a triangle in private memory and rasterizes the fan
-// directly (not counted in the clip statistics).
-__device__ __noinline__ void clip_direct(float4 c0, float4 c1, float4 c2, uint32_t id, uint32_t W, uint32_t H,
-                                         unsigned long long* keys)
+// Setup of fan triangle (v0, vf, vf+1) of a projecting polygon.
+template <typename Poly>
+__device__ __forceinline__ bool fan_setup(const Poly p, int f, float hw, float hh, int W, int H, TriSetup& t)
 {
-    float4 A[kClipMax], B[kClipMax];
-    clip_triangle(LocalPoly{ A }, LocalPoly{ B }, c0, c1, c2, W, H, nullptr,
-                  [&](const TriSetup& t) { raster_direct(t, id, keys, W); });
+    SVtx s0, s1, s2;
+    project_vtx(p[0], hw, hh, s0);
+    project_vtx(p[f], hw, hh, s1);
+    project_vtx(p[f + 1], hw, hh, s2);
+    return setup_tri(s0, s1, s2, W, H, t);
 }
 
 __device__ __forceinline__ uint32_t tri_byte(const uint32_t* words, uint32_t b)
@@ -692,17 +713,35 @@ struct BigArgs {
 };
 
 // One lane per queued triangle, in its own LDS polygon buffers; the fan
-// joins the big triangles.
+// joins the big triangles, or, without room, the wave's direct sweep.  The
+// block is one wave, and every lane runs the same iterations.
 __device__ __forceinline__ void clip_view(const BigArgs& a)
 {
     __shared__ float4 s_poly[2][kClipMax][kClipThreads];
     const StridedPoly pa{ &s_poly[0][0][threadIdx.x] }, pb{ &s_poly[1][0][threadIdx.x] };
     const BigQueue q{ a.big, a.big_chunk, a.big_cap, a.chunk_tri, a.chunk_cap, a.ctr, a.keys, a.W };
     const uint32_t n = min(a.ctr->clip_count, a.clip_cap);
-    for (uint32_t i = blockIdx.x * kClipThreads + threadIdx.x; i < n; i += gridDim.x * kClipThreads) {
-        const ClipTri ct = a.clipq[i];
-        clip_triangle(pa, pb, ct.c[0], ct.c[1], ct.c[2], a.W, a.H, a.ctr,
-                      [&](const TriSetup& t) { push_big(t, ct.id, q); });
+    const float hw = (float)a.W * 0.5f, hh = (float)a.H * 0.5f;
+    for (uint32_t i0 = blockIdx.x * kClipThreads; i0 < n; i0 += gridDim.x * kClipThreads) {
+        const uint32_t i = i0 + threadIdx.x;
+        ClipTri ct;
+        ct.id = 0u;
+        int m = 0;
+        if (i < n) {
+            ct = a.clipq[i];
+            m = clip_polygon(pa, pb, ct.c[0], ct.c[1], ct.c[2], a.W, a.H);
+            if (m) {
+                atomicAdd(&a.ctr->clipped, 1u);
+                if (!polygon_projects(pb, m, hw, hh)) m = 0;
+            }
+        }
+        for (int f = 1; f + 1 < kClipMax; ++f) {
+            if (__ballot(f + 1 < m) == 0ull) break;   // wave-uniform
+            TriSetup t;
+            bool spill = false;
+            if (f + 1 < m && fan_setup(pb, f, hw, hh, (int)a.W, (int)a.H, t)) spill = !push_big(t, ct.id, q);
+            wave_sweep(spill, t, ct.id, a.keys, a.W);
+        }
     }
 }
 
@@ -1149,14 +1188,16 @@ struct RedoArgs {
 // The meshlets of this pass that had a triangle without room in a queue: one
 // wave per meshlet, one lane per triangle, with the raster's own vertex,
 // culling and setup arithmetic; every triangle of theirs that the raster did
-// not keep in its wave (not small) is rasterized directly, clipped first if
-// it needs clipping.  A triangle that did reach a queue is written twice; the
-// visibility write is a minimum, so the result is the same.
+// not keep in its wave (not small) goes to the wave's direct sweep, clipped
+// first (in the lane's private memory) when it needs clipping.  A triangle
+// that did reach a queue is written twice; the visibility write is a
+// minimum, so the result is the same.  Every lane runs the same iterations.
 __device__ __forceinline__ void redo_view(const RedoArgs& ra, const ViewDev& v, uint32_t n)
 {
     const RasterRec* list = v.visible[ra.list];
     const float hw = (float)v.width * 0.5f, hh = (float)v.height * 0.5f;
     const int W = (int)v.width, H = (int)v.height;
+    const uint32_t lane = threadIdx.x & 63u;
     for (uint32_t r = blockIdx.x; r < n; r += gridDim.x) {
         const RasterRec rec = list[v.redo[r]];
         const float* mrow = v.mvp + (size_t)rec.transform * kMvpStride;
@@ -1166,32 +1207,49 @@ __device__ __forceinline__ void redo_view(const RedoArgs& ra, const ViewDev& v, 
         const uint32_t tc = (rec.counts >> 8) & 0xFFu;
         const uint32_t* words = reinterpret_cast<const uint32_t*>(ra.s.tri_base + rec.tri);
         const float* vtx = reinterpret_cast<const float*>(reinterpret_cast<const uint8_t*>(ra.s.vtx_base) + rec.vtx);
-        for (uint32_t t = threadIdx.x; t < tc; t += kClipThreads) {
-            float4 c[3];
-            SVtx sv[3];
-            bool ok = true;
-#pragma unroll
-            for (int k = 0; k < 3; ++k) {
-                const uint32_t vi = ra.s.vidx_base[rec.vidx + tri_byte(words, 3u * t + (uint32_t)k)];
-                const float* pp = vtx + (size_t)vi * 6;
-                c[k] = mat4_point(mvp, pp[0], pp[1], pp[2]);
-                sv[k].x = 0; sv[k].y = 0; sv[k].z = 0.0f;
-                sv[k].ok = project_vtx(c[k], hw, hh, sv[k]) ? 1u : 0u;
-                ok = ok && sv[k].ok;
-            }
-            if (cull_faces) {
-                const float det = det3_xyw(c[0], c[1], c[2]);
-                if (tdet < 0.0f ? det < 0.0f : det > 0.0f) continue;
-            }
+        for (uint32_t t0 = 0; t0 < tc; t0 += 64u) {
+            const uint32_t t = t0 + lane;
             const uint32_t id = (rec.draw << VKGV_TRIANGLE_BITS) | t;
-            if (!ok) {
-                clip_direct(c[0], c[1], c[2], id, v.width, v.height, v.keys);
-                continue;
-            }
+            float4 c[3];
+            bool live = t < tc, clip = false, direct = false;
             TriSetup ts;
-            if (!setup_tri(sv[0], sv[1], sv[2], W, H, ts)) continue;
-            if (ts.maxx - ts.minx < ra.small_lim && ts.maxy - ts.miny < ra.small_lim) continue;
-            raster_direct(ts, id, v.keys, v.width);
+            if (live) {
+                SVtx sv[3];
+                bool ok = true;
+#pragma unroll
+                for (int k = 0; k < 3; ++k) {
+                    const uint32_t vi = ra.s.vidx_base[rec.vidx + tri_byte(words, 3u * t + (uint32_t)k)];
+                    const float* pp = vtx + (size_t)vi * 6;
+                    c[k] = mat4_point(mvp, pp[0], pp[1], pp[2]);
+                    sv[k].x = 0; sv[k].y = 0; sv[k].z = 0.0f;
+                    sv[k].ok = project_vtx(c[k], hw, hh, sv[k]) ? 1u : 0u;
+                    ok = ok && sv[k].ok;
+                }
+                if (cull_faces) {
+                    const float det = det3_xyw(c[0], c[1], c[2]);
+                    if (tdet < 0.0f ? det < 0.0f : det > 0.0f) live = false;
+                }
+                if (live && !ok) {
+                    clip = true;
+                } else if (live && setup_tri(sv[0], sv[1], sv[2], W, H, ts)) {
+                    direct = !(ts.maxx - ts.minx < ra.small_lim && ts.maxy - ts.miny < ra.small_lim);
+                }
+            }
+            wave_sweep(direct, ts, id, v.keys, v.width);
+            if (__ballot(clip) == 0ull) continue;   // wave-uniform
+            float4 A[kClipMax], B[kClipMax];
+            const LocalPoly pa{ A }, pb{ B };
+            int m = 0;
+            if (clip) {
+                m = clip_polygon(pa, pb, c[0], c[1], c[2], v.width, v.height);
+                if (m && !polygon_projects(pb, m, hw, hh)) m = 0;
+            }
+            for (int f = 1; f + 1 < kClipMax; ++f) {
+                if (__ballot(f + 1 < m) == 0ull) break;   // wave-uniform
+                TriSetup ft;
+                const bool want = f + 1 < m && fan_setup(pb, f, hw, hh, W, H, ft);
+                wave_sweep(want, ft, id, v.keys, v.width);
+            }
         }
     }
 }
@@ -1280,7 +1338,8 @@ void launch_raster_big(const SceneDev& s, const BatchDev& b, const BatchGrid& g,
     ra.flags = flags;
     ra.list = list;
     ra.small_lim = (int)(small_max_extent * 256u);
-    clip_kernel<<<64, kClipThreads, 0, st>>>(b, ra);
+    // one wave per block: enough waves for the redo path, which is rare
+    clip_kernel<<<std::max(64u, tile_grid / 4u), kClipThreads, 0, st>>>(b, ra);
     bin_kernel<<<1024, 256, 0, st>>>(b);
     bin_scan_kernel<<<g.nviews, kScanThreads, 0, st>>>(b, tile_item_tris);
     bin_fill_kernel<<<256, 256, 0, st>>>(b);
