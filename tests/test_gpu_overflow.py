@@ -147,6 +147,19 @@ def test_full_queues_in_a_batch(crowded):
             for v, (k, c) in zip(views, ref):
                 assert np.array_equal(v.keys(), k) and np.array_equal(v.color(), c)
             print("batch with full queues: both views identical to their roomy single-view frames", flush=True)
+            # the same frame captured as a graph and replayed, with eager frames in between
+            flags = N.CULL_FRUSTUM | N.RESOLVE_BARY
+            batch.render([c for _, _, c in specs], flags)
+            r.sync()
+            eager = [(v.keys().copy(), v.color().copy()) for v in views]
+            for rep in range(3):
+                batch.render([c for _, _, c in specs], flags | N.GRAPH)
+                r.sync()
+                for v, (k, c) in zip(views, eager):
+                    assert np.array_equal(v.keys(), k) and np.array_equal(v.color(), c), rep
+                batch.render([c for _, _, c in specs], flags)
+                r.sync()
+            print("graph replays with full queues identical to the eager frame", flush=True)
         finally:
             batch.close()
             for v in views:
