@@ -121,8 +121,8 @@ def test_full_queues_in_a_batch(crowded):
     """Two views in one batch frame, with queues too small for either: each
     view's image equals its single-view frame with room to spare."""
     specs = [(W, H, camera()),
-             (320, 240, camera_from_view(view_desc(320, 240, (0.0, -0.6, -3.0), (4.0, -1.8, 5.0), fovy_deg=80.0,
-                                                   znear=0.8)))]
+             (331, 217, camera_from_view(view_desc(331, 217, (0.0, -0.6, -3.0), (4.0, -1.8, 5.0), fovy_deg=80.0,
+                                                   znear=0.8)))]   # odd size: sweeps and tiles cross the edges
     ref = []
     r = Renderer()
     try:
