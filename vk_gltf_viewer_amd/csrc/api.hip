@@ -56,7 +56,9 @@ struct vkgv_ctx {
     RasterTuning tuning{ 16u, 3u, 8u, 8192u };
     DevBuf srgb;                     // 255 sRGB-encode thresholds
     DevBuf srgb_decode;              // toLinear of each byte / 255 (256 floats)
-    DevBuf tex_recs, texels, mat_rot;
+    DevBuf tex_recs, texels, mat_tex;
+    std::vector<vkgv_material> host_materials;   // what the per-material texture records are built from
+    std::vector<TexRec> host_textures;
     uint32_t texture_count = 0;
     DevBuf prev_transforms;          // the transforms the last update replaced (motion vectors)
     uint32_t prev_count = 0;
@@ -166,6 +168,38 @@ int upload(vkgv_ctx* c, DevBuf& b, const void* src, size_t bytes)
     return VKGV_OK;
 }
 
+// The per-material records of the textured resolve, from the materials and
+// textures uploaded so far (a material whose texture is not uploaded yet gets
+// none; frames that need it are refused until it is).  cos and sin of each uv
+// rotation are evaluated once here, so that every pixel uses the same matrix.
+static int rebuild_mat_tex(vkgv_ctx* c)
+{
+    std::vector<MatTex> mt(std::max<size_t>(c->host_materials.size(), 1));
+    for (size_t m = 0; m < c->host_materials.size(); ++m) {
+        const vkgv_material& mat = c->host_materials[m];
+        MatTex& r = mt[m];
+        r.uv_scale[0] = mat.uv_scale[0];
+        r.uv_scale[1] = mat.uv_scale[1];
+        r.uv_offset[0] = mat.uv_offset[0];
+        r.uv_offset[1] = mat.uv_offset[1];
+        r.cos_r = std::cos(mat.uv_rotation);
+        r.sin_r = std::sin(mat.uv_rotation);
+        const uint32_t ti = mat.albedo_index;
+        if (ti != 0xFFFFFFFFu && ti < c->host_textures.size()) {
+            const TexRec& t = c->host_textures[ti];
+            r.width = t.width;
+            r.height = t.height;
+            r.offset = t.offset;
+            r.wrap_s = t.wrap_s;
+            r.wrap_t = t.wrap_t;
+        }
+    }
+    int rc = upload(c, c->mat_tex, mt.data(), sizeof(MatTex) * mt.size());
+    if (rc) return rc;
+    HIP_TRY(c, hipStreamSynchronize(c->stream));   // mt is released on return
+    return VKGV_OK;
+}
+
 SceneDev scene_dev(const vkgv_ctx* c)
 {
     SceneDev s{};
@@ -191,7 +225,7 @@ SceneDev scene_dev(const vkgv_ctx* c)
     s.texture_count = c->texture_count;
     s.texels = static_cast<const uint32_t*>(c->texels.p);
     s.srgb_decode = static_cast<const float*>(c->srgb_decode.p);
-    s.mat_rot = static_cast<const float2*>(c->mat_rot.p);
+    s.mat_tex = static_cast<const MatTex*>(c->mat_tex.p);
     s.prev_transforms = static_cast<const float*>(c->prev_transforms.p ? c->prev_transforms.p : c->transforms.p);
     return s;
 }
@@ -511,7 +545,7 @@ void vkgv_destroy(vkgv_ctx* c)
     for (auto& b : c->mesh_bufs) dfree(b);
     dfree(c->prims); dfree(c->materials); dfree(c->cone_base); dfree(c->cones);
     dfree(c->draws); dfree(c->cull); dfree(c->rrec); dfree(c->transforms); dfree(c->srgb);
-    dfree(c->srgb_decode); dfree(c->tex_recs); dfree(c->texels); dfree(c->mat_rot); dfree(c->prev_transforms);
+    dfree(c->srgb_decode); dfree(c->tex_recs); dfree(c->texels); dfree(c->mat_tex); dfree(c->prev_transforms);
     for (DevBuf* b : { &c->an_nodes, &c->an_chan_first, &c->an_channels, &c->an_samplers, &c->an_times, &c->an_values,
                        &c->an_trs, &c->an_chain_first, &c->an_chain })
         dfree(*b);
@@ -638,16 +672,12 @@ int vkgv_upload_scene(vkgv_ctx* c, const vkgv_mesh_desc* meshes, uint32_t mesh_c
     }
     if ((rc = upload(c, c->prims, dp.data(), sizeof(vkgv_primitive) * prim_count))) return rc;
     if ((rc = upload(c, c->materials, materials, sizeof(vkgv_material) * material_count))) return rc;
-    // (cos, sin) of each uv rotation, evaluated once on the host so that every
-    // pixel of the textured resolve uses the same rotation matrix
-    std::vector<float2> rot(material_count);
     c->max_albedo_index = -1;
-    for (uint32_t m = 0; m < material_count; ++m) {
-        rot[m] = make_float2(std::cos(materials[m].uv_rotation), std::sin(materials[m].uv_rotation));
+    for (uint32_t m = 0; m < material_count; ++m)
         if (materials[m].albedo_index != 0xFFFFFFFFu)
             c->max_albedo_index = std::max<int64_t>(c->max_albedo_index, materials[m].albedo_index);
-    }
-    if ((rc = upload(c, c->mat_rot, rot.data(), sizeof(float2) * material_count))) return rc;
+    c->host_materials.assign(materials, materials + material_count);
+    if ((rc = rebuild_mat_tex(c))) return rc;
     if ((rc = upload(c, c->cone_base, cb.data(), sizeof(uint32_t) * cb.size()))) return rc;
     if ((rc = upload(c, c->cones, all_cones.data(), sizeof(vkgv_meshlet_cone) * all_cones.size()))) return rc;
     HIP_TRY(c, hipStreamSynchronize(c->stream));
@@ -686,6 +716,8 @@ int vkgv_upload_textures(vkgv_ctx* c, const vkgv_texture_desc* textures, uint32_
                                   4ull * textures[i].width * textures[i].height, hipMemcpyHostToDevice, c->stream));
     if ((rc = upload(c, c->tex_recs, recs.data(), sizeof(TexRec) * count))) return rc;
     HIP_TRY(c, hipStreamSynchronize(c->stream));
+    c->host_textures = recs;
+    if ((rc = rebuild_mat_tex(c))) return rc;
     c->texture_count = count;
     c->generation++;   // captured batch frames hold the old texture addresses
     return VKGV_OK;

@@ -65,6 +65,18 @@ struct TexRec {
 };
 static_assert(sizeof(TexRec) == 32, "TexRec layout");
 
+// What the textured resolve reads of a material, in one record: the uv
+// transform (scale, offset, cos and sin of the rotation, evaluated once on
+// the host) and its albedo texture (width 0: none).
+struct MatTex {
+    float uv_scale[2], uv_offset[2];
+    float cos_r, sin_r;
+    uint32_t width, height;
+    uint64_t offset;                // first texel in the texel buffer
+    uint32_t wrap_s, wrap_t;
+};
+static_assert(sizeof(MatTex) == 48, "MatTex layout");
+
 struct SceneDev {
     const vkgv_meshlet_draw* draws;
     uint32_t draw_count;
@@ -85,12 +97,12 @@ struct SceneDev {
     // sRGB encode + unorm8 tables (SrgbTables)
     const void* srgb;
     // textured resolve: texture records, their texels (RGBA8 words), the
-    // per-byte sRGB decode, and (cos, sin) of each material's uv rotation
+    // per-byte sRGB decode, and each material's uv transform and texture
     const TexRec* textures;
     uint32_t texture_count;
     const uint32_t* texels;
     const float* srgb_decode;
-    const float2* mat_rot;
+    const MatTex* mat_tex;
     // the transforms of the previous update (motion vectors)
     const float* prev_transforms;
 };

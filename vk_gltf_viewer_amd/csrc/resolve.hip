@@ -154,9 +154,8 @@ __device__ __forceinline__ uint32_t fetch_uvs(const SceneDev& s, uint32_t d, uin
 __device__ __forceinline__ void apply_texture(const SceneDev& s, uint32_t mi, const uint32_t* uvw, const float* l,
                                               const float* s_lin, const float* s_byte, float* c)
 {
-    const vkgv_material& mat = s.materials[mi];
-    const uint32_t ti = mat.albedo_index;
-    if (ti == 0xFFFFFFFFu) return;
+    const MatTex mt = s.mat_tex[mi];
+    if (mt.width == 0u) return;   // no texture
     float uq[3], vq[3];
 #pragma unroll
     for (int k = 0; k < 3; ++k) {
@@ -166,18 +165,16 @@ __device__ __forceinline__ void apply_texture(const SceneDev& s, uint32_t mi, co
     const float u = (l[0] * uq[0] + l[1] * uq[1]) + l[2] * uq[2];
     const float v = (l[0] * vq[0] + l[1] * vq[1]) + l[2] * vq[2];
     // transformUv: mat2(cos, -sin, sin, cos) * uv * scale + offset
-    const float2 cs = s.mat_rot[mi];
-    const float ur = cs.x * u + cs.y * v;
-    const float vr = (-cs.y) * u + cs.x * v;
-    const float tu = ur * mat.uv_scale[0] + mat.uv_offset[0];
-    const float tv = vr * mat.uv_scale[1] + mat.uv_offset[1];
-    const TexRec tex = s.textures[ti];
+    const float ur = mt.cos_r * u + mt.sin_r * v;
+    const float vr = (-mt.sin_r) * u + mt.cos_r * v;
+    const float tu = ur * mt.uv_scale[0] + mt.uv_offset[0];
+    const float tv = vr * mt.uv_scale[1] + mt.uv_offset[1];
     uint32_t x0, x1, y0, y1;
-    const float fx = texel_pair(tu, tex.width, tex.wrap_s, x0, x1);
-    const float fy = texel_pair(tv, tex.height, tex.wrap_t, y0, y1);
-    const uint32_t* img = s.texels + tex.offset;
-    const uint32_t t00 = img[(size_t)y0 * tex.width + x0], t10 = img[(size_t)y0 * tex.width + x1];
-    const uint32_t t01 = img[(size_t)y1 * tex.width + x0], t11 = img[(size_t)y1 * tex.width + x1];
+    const float fx = texel_pair(tu, mt.width, mt.wrap_s, x0, x1);
+    const float fy = texel_pair(tv, mt.height, mt.wrap_t, y0, y1);
+    const uint32_t* img = s.texels + mt.offset;
+    const uint32_t t00 = img[(size_t)y0 * mt.width + x0], t10 = img[(size_t)y0 * mt.width + x1];
+    const uint32_t t01 = img[(size_t)y1 * mt.width + x0], t11 = img[(size_t)y1 * mt.width + x1];
     const float ax = 1.0f - fx, ay = 1.0f - fy;
 #pragma unroll
     for (int j = 0; j < 4; ++j) {
