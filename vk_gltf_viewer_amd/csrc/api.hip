@@ -223,7 +223,7 @@ SceneDev scene_dev(const vkgv_ctx* c)
     s.srgb = c->srgb.p;
     s.textures = static_cast<const TexRec*>(c->tex_recs.p);
     s.texture_count = c->texture_count;
-    s.texels = static_cast<const uint32_t*>(c->texels.p);
+    s.texels = static_cast<const float4*>(c->texels.p);
     s.srgb_decode = static_cast<const float*>(c->srgb_decode.p);
     s.mat_tex = static_cast<const MatTex*>(c->mat_tex.p);
     s.prev_transforms = static_cast<const float*>(c->prev_transforms.p ? c->prev_transforms.p : c->transforms.p);
@@ -709,11 +709,25 @@ int vkgv_upload_textures(vkgv_ctx* c, const vkgv_texture_desc* textures, uint32_
     HIP_TRY(c, hipDeviceSynchronize());   // frames of any stream may still read the old textures
     c->texture_count = 0;                 // until the new set is complete
     c->generation++;
-    int rc = dalloc(c, c->texels, 4ull * std::max<uint64_t>(total, 1));
+    // texels stored decoded: toLinear of r, g, b and a / 255, the values the
+    // resolve's bilinear filter weights (the same table as the device's)
+    int rc = dalloc(c, c->texels, sizeof(float4) * std::max<uint64_t>(total, 1));
     if (rc) return rc;
-    for (uint32_t i = 0; i < count; ++i)
-        HIP_TRY(c, hipMemcpyAsync(static_cast<uint32_t*>(c->texels.p) + recs[i].offset, textures[i].rgba8,
-                                  4ull * textures[i].width * textures[i].height, hipMemcpyHostToDevice, c->stream));
+    {
+        float dec[256];
+        srgb_decode_table(dec);
+        std::vector<float4> lin(std::max<uint64_t>(total, 1));
+        for (uint32_t i = 0; i < count; ++i) {
+            const uint8_t* src = textures[i].rgba8;
+            const uint64_t n = (uint64_t)textures[i].width * textures[i].height;
+            for (uint64_t k = 0; k < n; ++k)
+                lin[recs[i].offset + k] = make_float4(dec[src[4 * k]], dec[src[4 * k + 1]], dec[src[4 * k + 2]],
+                                                      (float)src[4 * k + 3] / 255.0f);
+        }
+        HIP_TRY(c, hipMemcpyAsync(c->texels.p, lin.data(), sizeof(float4) * lin.size(), hipMemcpyHostToDevice,
+                                  c->stream));
+        HIP_TRY(c, hipStreamSynchronize(c->stream));   // lin is released here
+    }
     if ((rc = upload(c, c->tex_recs, recs.data(), sizeof(TexRec) * count))) return rc;
     HIP_TRY(c, hipStreamSynchronize(c->stream));
     c->host_textures = recs;

@@ -56,7 +56,7 @@ struct AnimDev {
     const uint32_t* chain_nodes;         // root-to-node chain of each mesh node
 };
 
-// One albedo texture: `width * height` RGBA8 words from texels + offset.
+// One albedo texture: `width * height` decoded texels from texels + offset.
 struct TexRec {
     uint64_t offset;
     uint32_t width, height;
@@ -100,7 +100,7 @@ struct SceneDev {
     // per-byte sRGB decode, and each material's uv transform and texture
     const TexRec* textures;
     uint32_t texture_count;
-    const uint32_t* texels;
+    const float4* texels;           // decoded: toLinear(r, g, b), a / 255
     const float* srgb_decode;
     const MatTex* mat_tex;
     // the transforms of the previous update (motion vectors)

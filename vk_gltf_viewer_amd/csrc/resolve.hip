@@ -152,7 +152,7 @@ __device__ __forceinline__ uint32_t fetch_uvs(const SceneDev& s, uint32_t d, uin
 // Albedo texture of material mi times the pixel's colour (c[0..3]): the
 // vertex uvs interpolated with l[], transformUv, bilinear sample, multiply.
 __device__ __forceinline__ void apply_texture(const SceneDev& s, uint32_t mi, const uint32_t* uvw, const float* l,
-                                              const float* s_lin, const float* s_byte, float* c)
+                                              float* c)
 {
     const MatTex mt = s.mat_tex[mi];
     if (mt.width == 0u) return;   // no texture
@@ -172,16 +172,16 @@ __device__ __forceinline__ void apply_texture(const SceneDev& s, uint32_t mi, co
     uint32_t x0, x1, y0, y1;
     const float fx = texel_pair(tu, mt.width, mt.wrap_s, x0, x1);
     const float fy = texel_pair(tv, mt.height, mt.wrap_t, y0, y1);
-    const uint32_t* img = s.texels + mt.offset;
-    const uint32_t t00 = img[(size_t)y0 * mt.width + x0], t10 = img[(size_t)y0 * mt.width + x1];
-    const uint32_t t01 = img[(size_t)y1 * mt.width + x0], t11 = img[(size_t)y1 * mt.width + x1];
+    const float4* img = s.texels + mt.offset;
+    const float4 t00 = img[(size_t)y0 * mt.width + x0], t10 = img[(size_t)y0 * mt.width + x1];
+    const float4 t01 = img[(size_t)y1 * mt.width + x0], t11 = img[(size_t)y1 * mt.width + x1];
     const float ax = 1.0f - fx, ay = 1.0f - fy;
+    const float a00[4] = { t00.x, t00.y, t00.z, t00.w }, a10[4] = { t10.x, t10.y, t10.z, t10.w };
+    const float a01[4] = { t01.x, t01.y, t01.z, t01.w }, a11[4] = { t11.x, t11.y, t11.z, t11.w };
 #pragma unroll
     for (int j = 0; j < 4; ++j) {
-        const float* dec = j < 3 ? s_lin : s_byte;
-        const uint32_t sh = 8u * j;
-        const float r0 = dec[(t00 >> sh) & 0xFFu] * ax + dec[(t10 >> sh) & 0xFFu] * fx;
-        const float r1 = dec[(t01 >> sh) & 0xFFu] * ax + dec[(t11 >> sh) & 0xFFu] * fx;
+        const float r0 = a00[j] * ax + a10[j] * fx;
+        const float r1 = a01[j] * ax + a11[j] * fx;
         c[j] = c[j] * (r0 * ay + r1 * fy);
     }
 }
@@ -252,10 +252,8 @@ __global__ __launch_bounds__(256) void resolve_kernel(ResolveBatchArgs ab)
     __shared__ float s_thr[256];
     __shared__ float s_byte[256];
     __shared__ uint32_t s_base[512];
-    __shared__ float s_lin[256];
     constexpr bool tex = kTex;
     const SrgbTables* tab = static_cast<const SrgbTables*>(a.s.srgb);
-    if constexpr (tex) s_lin[threadIdx.x] = a.s.srgb_decode[threadIdx.x];
     s_thr[threadIdx.x] = tab->thr[threadIdx.x];
     s_byte[threadIdx.x] = (float)threadIdx.x / 255.0f;
     s_base[threadIdx.x] = reinterpret_cast<const uint32_t*>(tab->base)[threadIdx.x];
@@ -343,7 +341,7 @@ __global__ __launch_bounds__(256) void resolve_kernel(ResolveBatchArgs ab)
                     c[j] = c[j] * ((l[0] * c0 + l[1] * c1) + l[2] * c2);
                 }
             }
-            if constexpr (tex) apply_texture(a.s, mi, uvw, l, s_lin, s_byte, c);
+            if constexpr (tex) apply_texture(a.s, mi, uvw, l, c);
             if constexpr (mv) view_motion[i] = motion_vector(a.s, view_pmvp, drw[k], t, l, xn, yn);
         }
         const uint32_t r8 = srgb_code(s_thr, base, key0, c[0]);
