@@ -265,7 +265,7 @@ struct BatchDev {
 };
 
 constexpr uint32_t kMaxBatchViews = 256;
-constexpr uint32_t kClearKeysPerBlock = 256u * 16u;   // keys cleared by one frame-start block
+constexpr uint32_t kClearKeysPerBlock = 256u * 32u;   // keys cleared by one frame-start block
 
 __device__ __forceinline__ const CameraArg& batch_camera(const BatchDev& b, uint32_t v)
 {
