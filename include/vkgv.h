@@ -9,7 +9,7 @@
  * The data contract mirrors the reference's dual-compiled GLSL/C++ structs
  * byte for byte (shaders/mesh_common.h.glsl, shaders/visbuffer/visbuffer.h.glsl),
  * so that a caller holding the reference's glsl:: arrays can hand them over
- * unchanged.  Offsets are pinned with static assertions in the library.
+ * unchanged.  Sizes and offsets are pinned by static assertions below.
  *
  * Status codes: 0 = ok, negative = error; vkgv_last_error() returns the text of
  * the last error raised on a context.  A context and its views are externally
@@ -111,6 +111,47 @@ typedef struct vkgv_meshlet_cone {
     float pad;
 } vkgv_meshlet_cone;                 /* 32 bytes */
 
+/* The contract layouts (sizes and offsets of the reference's scalar-layout
+ * GLSL structs, mesh_common.h.glsl:20-126), checked by every compiler that
+ * includes this header. */
+#ifdef __cplusplus
+#define VKGV_LAYOUT(cond, what) static_assert(cond, what)
+#else
+#define VKGV_LAYOUT(cond, what) _Static_assert(cond, what)
+#endif
+VKGV_LAYOUT(sizeof(vkgv_meshlet) == 36, "Meshlet is 36 bytes");
+VKGV_LAYOUT(offsetof(vkgv_meshlet, triangle_offset) == 4, "Meshlet.triangleOffset @4");
+VKGV_LAYOUT(offsetof(vkgv_meshlet, vertex_count) == 8, "Meshlet.vertexCount @8");
+VKGV_LAYOUT(offsetof(vkgv_meshlet, triangle_count) == 9, "Meshlet.triangleCount @9");
+VKGV_LAYOUT(offsetof(vkgv_meshlet, aabb_extents) == 12, "Meshlet.aabbExtents @12");
+VKGV_LAYOUT(offsetof(vkgv_meshlet, aabb_center) == 24, "Meshlet.aabbCenter @24");
+VKGV_LAYOUT(sizeof(vkgv_vertex) == 24, "Vertex is 24 bytes");
+VKGV_LAYOUT(offsetof(vkgv_vertex, color) == 12, "Vertex.color @12");
+VKGV_LAYOUT(offsetof(vkgv_vertex, normal) == 16, "Vertex.normal @16");
+VKGV_LAYOUT(offsetof(vkgv_vertex, uv) == 20, "Vertex.uv @20");
+VKGV_LAYOUT(sizeof(vkgv_meshlet_draw) == 12, "MeshletDraw is 12 bytes");
+VKGV_LAYOUT(offsetof(vkgv_meshlet_draw, meshlet_index) == 4, "MeshletDraw.meshletIndex @4");
+VKGV_LAYOUT(offsetof(vkgv_meshlet_draw, transform_index) == 8, "MeshletDraw.transformIndex @8");
+VKGV_LAYOUT(sizeof(vkgv_primitive) == 64, "Primitive is 64 bytes");
+VKGV_LAYOUT(offsetof(vkgv_primitive, meshlet_buffer) == 24, "Primitive.meshletBuffer @24");
+VKGV_LAYOUT(offsetof(vkgv_primitive, aabb_extents) == 32, "Primitive.aabbExtents @32");
+VKGV_LAYOUT(offsetof(vkgv_primitive, aabb_center) == 44, "Primitive.aabbCenter @44");
+VKGV_LAYOUT(offsetof(vkgv_primitive, meshlet_count) == 56, "Primitive.meshletCount @56");
+VKGV_LAYOUT(offsetof(vkgv_primitive, material_index) == 60, "Primitive.materialIndex @60");
+VKGV_LAYOUT(sizeof(vkgv_material) == 48, "Material is 48 bytes");
+VKGV_LAYOUT(offsetof(vkgv_material, albedo_index) == 16, "Material.albedoIndex @16");
+VKGV_LAYOUT(offsetof(vkgv_material, uv_offset) == 20, "Material.uvOffset @20");
+VKGV_LAYOUT(offsetof(vkgv_material, uv_scale) == 28, "Material.uvScale @28");
+VKGV_LAYOUT(offsetof(vkgv_material, uv_rotation) == 36, "Material.uvRotation @36");
+VKGV_LAYOUT(offsetof(vkgv_material, alpha_cutoff) == 40, "Material.alphaCutoff @40");
+VKGV_LAYOUT(offsetof(vkgv_material, double_sided) == 44, "Material.doubleSided @44");
+VKGV_LAYOUT(sizeof(vkgv_camera) == 352, "Camera is 352 bytes");
+VKGV_LAYOUT(offsetof(vkgv_camera, prev_occlusion_view_projection) == 64, "Camera.prevOcclusionViewProjection @64");
+VKGV_LAYOUT(offsetof(vkgv_camera, view_projection) == 128, "Camera.viewProjection @128");
+VKGV_LAYOUT(offsetof(vkgv_camera, occlusion_view_projection) == 192, "Camera.occlusionViewProjection @192");
+VKGV_LAYOUT(offsetof(vkgv_camera, frustum) == 256, "Camera.frustum @256");
+VKGV_LAYOUT(sizeof(vkgv_meshlet_cone) == 32, "meshlet cone is 32 bytes");
+
 /* ------------------------------------------------------------------------ */
 /* Scene upload (assets.cpp:375-429 + world.cpp:10-185 equivalent).          */
 
@@ -151,6 +192,9 @@ typedef struct vkgv_primitive_desc {
 #define VKGV_GRAPH               (1u << 9)  /* vkgv_render_batch: replay the batch's frame as one captured hipGraph */
 #define VKGV_RESOLVE_TEXTURED    (1u << 10) /* as RESOLVE_BARY, times the material's albedo texture sampled at transformUv(uv) */
 #define VKGV_MOTION_VECTORS      (1u << 11) /* resolve also writes per-pixel motion vectors (visbuffer.frag.glsl:38) */
+#define VKGV_STATS               (1u << 12) /* count rasterized triangles, fragments and visibility-buffer atomics
+                                               (vkgv_frame_stats); counting kernels are separate instantiations,
+                                               frames without the flag run uncounted code */
 
 /* The reference's own frame: frustum + previous-frame Hi-Z occlusion, pyramid
  * rebuilt after the resolve (application.cpp:843-1003). */
@@ -179,8 +223,16 @@ typedef struct vkgv_frame_stats {
     float    ms_clear, ms_cull, ms_raster, ms_hiz, ms_cull_late, ms_raster_late,
              ms_resolve, ms_pyramid, ms_total;
     uint32_t max_tile_tris;          /* most big triangles binned to one 32x32 tile */
+    /* raster counts of the last frame rendered with VKGV_STATS (0 otherwise) */
+    uint64_t raster_triangles;       /* triangles past the back-face test set up for rasterization */
+    uint64_t fragments;              /* covered pixel centres with 0 <= z <= 1: visibility-buffer updates */
+    uint64_t fragment_atomics;       /* of those, written with a 64-bit atomicMin */
+    uint64_t fragment_stores;        /* tile merges written with a plain read-min-store */
 } vkgv_frame_stats;
 
+/* Error-path testing: when the environment variable VKGV_FAULT_ALLOC_AFTER=n
+ * is set at vkgv_create, that context's first n device allocations succeed
+ * and every later one fails with VKGV_ERROR_OUT_OF_MEMORY. */
 typedef struct vkgv_create_info {
     int32_t  device;                 /* HIP device ordinal */
     uint32_t big_triangle_capacity;  /* per view and raster pass; 0 = default (2^18), at most 2^23 */
@@ -305,9 +357,12 @@ typedef struct vkgv_anim_channel {
  * vkgv_update_transforms.  The mesh nodes, in the depth-first order of
  * World::iterateNode, own transforms 0 .. n-1 (the order vkgv_world_build
  * and the reference's draw list use), and the context's transform count
- * becomes n.  Depth is limited to 64. */
+ * becomes n.  Node n's children are children[first_child .. + child_count),
+ * which must lie within the `children_count` entries.  The nodes reached
+ * from the roots must form a forest (no node reached twice), as glTF
+ * requires; depth is limited to 64. */
 int vkgv_set_hierarchy(vkgv_ctx* ctx, const vkgv_node* nodes, uint32_t node_count, const uint32_t* children,
-                       const uint32_t* roots, uint32_t root_count,
+                       uint32_t children_count, const uint32_t* roots, uint32_t root_count,
                        const vkgv_anim_sampler* samplers, uint32_t sampler_count,
                        const float* times, uint32_t time_count, const float* values, uint32_t value_count,
                        const vkgv_anim_channel* channels, uint32_t channel_count);

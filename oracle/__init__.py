@@ -78,6 +78,7 @@ def _setup(L):
         "ref_cull": (None, [p, p, p, p, u32, u32, u32, p]),
         "ref_hiz_build": (None, [p, u32, u32, p]),
         "ref_keys_clear": (None, [p, C.c_uint64]),
+        "ref_threads": (C.c_int, [C.c_int]),
         "ref_raster": (None, [p, p, p, u32, u32, u32, u32, p]),
         "ref_raster_triangle": (None, [p, p, p, u32, u32, u32, p]),
         "ref_keys_depth": (None, [p, C.c_uint64, p]),
@@ -137,6 +138,11 @@ class Scene:
     @property
     def draw_count(self):
         return len(self._draws)
+
+
+def threads(n: int = 0) -> int:
+    """OpenMP threads of the frame functions; n > 0 sets the count."""
+    return int(lib().ref_threads(int(n)))
 
 
 def pyramid_info(w, h):

@@ -8,6 +8,7 @@
 #include "cpuref.h"
 
 #include <math.h>
+#include <omp.h>
 #include <stdlib.h>
 #include <string.h>
 
@@ -266,6 +267,7 @@ void ref_hiz_build(const float* depth, uint32_t w, uint32_t h, float* pyramid)
     for (uint32_t k = 0; k < levels; ++k) {
         const uint32_t dw = sizes[2 * k], dh = sizes[2 * k + 1];
         float* dst = pyramid + offs[k];
+#pragma omp parallel for schedule(static) if ((uint64_t)dw * dh >= 65536)
         for (uint32_t y = 0; y < dh; ++y) {
             const float v = (((float)y + 0.5f) / (float)dh) * (float)sh - 0.5f;
             int32_t y0, y1;
@@ -394,6 +396,7 @@ static int cull_one(const ref_scene* s, uint32_t d, const vkgv_camera* cam, cons
 void ref_cull(const ref_scene* s, const vkgv_camera* cam, const float* eye,
               const float* pyramid, uint32_t w, uint32_t h, uint32_t flags, uint8_t* visible)
 {
+#pragma omp parallel for schedule(static)
     for (uint32_t d = 0; d < s->draw_count; ++d) {
         int vis = cull_one(s, d, cam, eye, flags);
         if (vis && (flags & VKGV_CULL_OCCLUSION_PREV)) {
@@ -439,6 +442,15 @@ static int project_vtx(const float* c, uint32_t w, uint32_t h, svtx* o)
 }
 
 static inline int64_t floor_div256(int64_t v) { return v >= 0 ? v / 256 : -((-v + 255) / 256); }
+
+/* *dst = min(*dst, key), atomically: the raster runs its draws on several
+ * threads, and a minimum does not depend on the order of the updates. */
+static inline void key_min(uint64_t* dst, uint64_t key)
+{
+    uint64_t cur = __atomic_load_n(dst, __ATOMIC_RELAXED);
+    while (key < cur && !__atomic_compare_exchange_n(dst, &cur, key, 1, __ATOMIC_RELAXED, __ATOMIC_RELAXED)) {
+    }
+}
 
 static void fill_triangle(svtx v0, svtx v1, svtx v2, uint32_t id, uint32_t w, uint32_t h, uint64_t* keys)
 {
@@ -490,8 +502,7 @@ static void fill_triangle(svtx v0, svtx v1, svtx v2, uint32_t id, uint32_t w, ui
             if (!(z >= 0.0f && z <= 1.0f)) continue;
             z = z + 0.0f;
             const uint64_t key = ((uint64_t)(~f2u(z)) << 32) | (uint64_t)(0xFFFFFFFEu - id);
-            uint64_t* dst = &keys[(uint64_t)py * w + (uint64_t)px];
-            if (key < *dst) *dst = key;
+            key_min(&keys[(uint64_t)py * w + (uint64_t)px], key);
         }
     }
 }
@@ -545,8 +556,16 @@ void ref_raster_triangle(const float* c0, const float* c1, const float* c2, uint
         fill_triangle(sv[0], sv[i], sv[i + 1], id, w, h, keys);
 }
 
+/* Threads the frame functions use (OpenMP); n <= 0 leaves the count as is. */
+int ref_threads(int n)
+{
+    if (n > 0) omp_set_num_threads(n);
+    return omp_get_max_threads();
+}
+
 void ref_keys_clear(uint64_t* keys, uint64_t n)
 {
+#pragma omp parallel for schedule(static) if (n >= 65536)
     for (uint64_t i = 0; i < n; ++i) keys[i] = VKGV_KEY_CLEAR;
 }
 
@@ -554,8 +573,9 @@ void ref_keys_clear(uint64_t* keys, uint64_t n)
 void ref_raster(const ref_scene* s, const float* vp, const uint32_t* draw_list, uint32_t n,
                 uint32_t w, uint32_t h, uint32_t flags, uint64_t* keys)
 {
-    float clip[VKGV_MAX_VERTICES][4];
+#pragma omp parallel for schedule(dynamic, 8)
     for (uint32_t li = 0; li < n; ++li) {
+        float clip[VKGV_MAX_VERTICES][4];
         const uint32_t d = draw_list[li];
         const vkgv_meshlet_draw* dr = &s->draws[d];
         const vkgv_primitive_desc* p = &s->primitives[dr->primitive_index];
@@ -723,6 +743,7 @@ void ref_resolve(const ref_scene* s, const float* vp, const uint64_t* keys,
 {
     const float hw = (float)w * 0.5f, hh = (float)h * 0.5f;
     const float inv_hw = 1.0f / hw, inv_hh = 1.0f / hh;
+#pragma omp parallel for schedule(dynamic, 4)
     for (uint32_t py = 0; py < h; ++py) {
         for (uint32_t px = 0; px < w; ++px) {
             const uint64_t i = (uint64_t)py * w + px;
@@ -820,6 +841,7 @@ void ref_frame_two_pass(const ref_scene* s, const vkgv_camera* cam, const float*
     const uint32_t nd = s->draw_count;
     uint32_t* list = (uint32_t*)malloc(sizeof(uint32_t) * (nd ? nd : 1));
     uint8_t* pass = (uint8_t*)malloc(nd ? nd : 1);
+#pragma omp parallel for schedule(static)
     for (uint32_t d = 0; d < nd; ++d) {
         pass[d] = (uint8_t)cull_one(s, d, cam, eye, flags);
         visible_early[d] = (uint8_t)(pass[d] && history[d]);
@@ -833,6 +855,7 @@ void ref_frame_two_pass(const ref_scene* s, const vkgv_camera* cam, const float*
     ref_hiz_build(depth, w, h, pyramid);
     free(depth);
 
+#pragma omp parallel for schedule(static)
     for (uint32_t d = 0; d < nd; ++d) {
         int vis = 0;
         if (pass[d]) {

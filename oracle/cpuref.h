@@ -85,6 +85,8 @@ void ref_cull(const ref_scene* s, const vkgv_camera* cam, const float* eye,
 void ref_hiz_build(const float* depth, uint32_t w, uint32_t h, float* pyramid);
 
 /* ---- visbuffer.mesh.glsl + fixed-function raster + visbuffer.frag.glsl --- */
+/* OpenMP threads of the frame functions (n <= 0: unchanged); returns the count. */
+int  ref_threads(int n);
 void ref_keys_clear(uint64_t* keys, uint64_t n);
 /* Rasterize the listed draws (ascending order is not required: the key
  * minimum is order independent). */

@@ -159,7 +159,7 @@ def test_autotune_picks_fastest_valid_candidate(tmp_path, monkeypatch):
     # small extent 4 is the fastest but "overflows"
     r = _FakeRenderer(lambda t: 0.0005 if t[0] == 4 else (0.002 if t[1] == 2 else 0.001))
     best, times = autotune(r, r.frame, "k", candidates=cand, warmup=0, frames=2, use_cache=True,
-                           valid=lambda: r.t[0] != 4)
+                           valid=lambda: r.t[0] != 4, search="grid")
     assert best == (16, 3, 16) and r.t == best
     assert times[(4, 2, 16)] == float("inf") and times[(4, 3, 16)] == float("inf")
     # cached: applied without timing
@@ -170,3 +170,35 @@ def test_autotune_picks_fastest_valid_candidate(tmp_path, monkeypatch):
     r.set_tuning(32, 2, 8)
     best3, _ = autotune(r, r.frame, "k2", candidates=cand, warmup=0, frames=1, use_cache=False, valid=lambda: False)
     assert best3 == (32, 2, 8) and r.t == (32, 2, 8)
+
+
+def test_autotune_coordinate_search_and_stale_cache(tmp_path, monkeypatch):
+    """The coordinate search finds the optimum of a separable cost with fewer
+    timings than the grid; cache entries of the wrong length, or whose tuning
+    no longer passes `valid`, are searched again; the key names the device and
+    the knob count."""
+    import json
+
+    from vk_gltf_viewer_amd.autotune import KNOBS, autotune, cache_key
+    path = tmp_path / "tune.json"
+    monkeypatch.setenv("VKGV_TUNE_CACHE", str(path))
+    cand = {"small_max_extent": (4, 8, 16, 32), "raster_blocks_per_cu": (2, 3), "tile_item_tris": (8, 16, 32)}
+    best_t = (8, 2, 32)
+    cost = lambda t: 0.0002 * (1 + sum(a != b for a, b in zip(t, best_t)))
+    r = _FakeRenderer(cost)
+    best, times = autotune(r, r.frame, "sep", candidates=cand, warmup=0, frames=1)
+    assert best == best_t and r.t == best_t
+    assert len(times) < 4 * 2 * 3
+    key = cache_key(r, "sep")
+    assert key.startswith("sep|") and f"knobs-{len(KNOBS)}" in key
+    data = json.loads(path.read_text())
+    assert data[key]["tuning"] == list(best_t)
+    # an entry of the wrong length (an older build's knob set) is not applied
+    data[key]["tuning"] = [4, 3]
+    path.write_text(json.dumps(data))
+    r.set_tuning(16, 3, 16)
+    best2, times2 = autotune(r, r.frame, "sep", candidates=cand, warmup=0, frames=1)
+    assert best2 == best_t and times2
+    # a cached tuning that fails `valid` now is searched again
+    best3, times3 = autotune(r, r.frame, "sep", candidates=cand, warmup=0, frames=1, valid=lambda: r.t[0] != 8)
+    assert best3[0] != 8 and times3

@@ -39,6 +39,7 @@ SKIP_RESOLVE = 1 << 8
 GRAPH = 1 << 9
 RESOLVE_TEXTURED = 1 << 10
 MOTION_VECTORS = 1 << 11
+STATS = 1 << 12
 INTERP_LINEAR, INTERP_STEP, INTERP_CUBICSPLINE = 0, 1, 2
 PATH_TRANSLATION, PATH_ROTATION, PATH_SCALE = 0, 1, 2
 WRAP_REPEAT, WRAP_CLAMP_TO_EDGE, WRAP_MIRRORED_REPEAT = 0, 1, 2
@@ -118,7 +119,9 @@ class FrameStats(C.Structure):
                 ("overflow", C.c_uint32), ("timed_frames", C.c_uint32),
                 ("ms_clear", C.c_float), ("ms_cull", C.c_float), ("ms_raster", C.c_float), ("ms_hiz", C.c_float),
                 ("ms_cull_late", C.c_float), ("ms_raster_late", C.c_float), ("ms_resolve", C.c_float),
-                ("ms_pyramid", C.c_float), ("ms_total", C.c_float), ("max_tile_tris", C.c_uint32)]
+                ("ms_pyramid", C.c_float), ("ms_total", C.c_float), ("max_tile_tris", C.c_uint32),
+                ("raster_triangles", C.c_uint64), ("fragments", C.c_uint64), ("fragment_atomics", C.c_uint64),
+                ("fragment_stores", C.c_uint64)]
 
 
 class Tuning(C.Structure):
@@ -212,7 +215,8 @@ HIP_SYMBOLS = [
     "vkgv_view_read_color", "vkgv_view_read_pyramid", "vkgv_view_write_pyramid", "vkgv_view_write_depth",
     "vkgv_view_read_visible", "vkgv_view_read_stats", "vkgv_set_tuning", "vkgv_get_tuning", "vkgv_srgb_thresholds",
     "vkgv_batch_create", "vkgv_batch_destroy", "vkgv_render_batch", "vkgv_batch_read_stats",
-    "vkgv_view_aa_image", "vkgv_view_aa_backward", "vkgv_upload_textures", "vkgv_set_hierarchy",
+    "vkgv_view_aa_image", "vkgv_view_aa_backward", "vkgv_upload_textures",
+    "vkgv_set_hierarchy",
     "vkgv_update_transforms", "vkgv_read_transforms", "vkgv_view_read_motion",
 ]
 
@@ -241,7 +245,8 @@ def hip():
         _bind(lib, "vkgv_destroy", None, _p)
         _bind(lib, "vkgv_last_error", C.c_char_p, _p)
         _bind(lib, "vkgv_sync", C.c_int, _p)
-        _bind(lib, "vkgv_set_hierarchy", C.c_int, _p, _p, _u32, _p, _p, _u32, _p, _u32, _p, _u32, _p, _u32, _p, _u32)
+        _bind(lib, "vkgv_set_hierarchy", C.c_int, _p, _p, _u32, _p, _u32, _p, _u32, _p, _u32, _p, _u32, _p, _u32, _p,
+              _u32)
         _bind(lib, "vkgv_update_transforms", C.c_int, _p, C.c_float, _p)
         _bind(lib, "vkgv_read_transforms", C.c_int, _p, _p, _u32)
         _bind(lib, "vkgv_view_read_motion", C.c_int, _p, _p, _p)

@@ -13,6 +13,7 @@
 
 #include <algorithm>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <string>
 #include <vector>
@@ -71,6 +72,10 @@ struct vkgv_ctx {
     uint32_t clip_capacity = 1u << 16;
     uint32_t cache_slots = 8192;     // resolve-cache meshlet slots per view
     uint64_t generation = 0;         // bumped when view tables must be rebuilt (draw lists reallocated)
+    // Allocation fault injection for error-path tests: when VKGV_FAULT_ALLOC_AFTER=n
+    // is set at vkgv_create, the context's first n device allocations succeed and
+    // every later one fails with VKGV_ERROR_OUT_OF_MEMORY (-1: off).
+    int64_t fault_alloc_after = -1;
 };
 
 struct vkgv_view {
@@ -150,6 +155,11 @@ int dalloc(vkgv_ctx* c, DevBuf& b, size_t bytes)
 {
     dfree(b);
     if (bytes == 0) bytes = 16;
+    if (c && c->fault_alloc_after >= 0) {
+        if (c->fault_alloc_after == 0)
+            return fail(c, VKGV_ERROR_OUT_OF_MEMORY, "hipMalloc: out of memory (injected by VKGV_FAULT_ALLOC_AFTER)");
+        c->fault_alloc_after--;
+    }
     hipError_t e = hipMalloc(&b.p, bytes);
     if (e != hipSuccess) {
         b.p = nullptr;
@@ -419,7 +429,7 @@ void frame_launches(vkgv_ctx* c, const BatchDev& b, const BatchGrid& g, uint32_t
     launch_frame_start(s, b, g, st);
     if (flags & VKGV_MOTION_VECTORS) launch_prev_mvp(s, b, g, st);
     mark(EV_CLEAR);
-    launch_cull(s, b, g, flags, two_pass ? CULL_EARLY : CULL_SINGLE, 0, st);
+    launch_cull(s, b, g, flags, two_pass ? CULL_EARLY : CULL_SINGLE, 0, (uint32_t)c->cus, st);
     mark(EV_CULL);
     launch_raster(s, b, flags, 0, raster_grid, c->tuning.small_max_extent, c->tuning.resolve_cache_slots, st);
     launch_raster_big(s, b, g, flags, 0, c->tuning.small_max_extent, tile_grid, c->tuning.tile_item_tris, st);
@@ -427,7 +437,7 @@ void frame_launches(vkgv_ctx* c, const BatchDev& b, const BatchGrid& g, uint32_t
     if (two_pass) {
         launch_hiz(b, g, st);
         mark(EV_HIZ);
-        launch_cull(s, b, g, flags, CULL_LATE, 1, st);
+        launch_cull(s, b, g, flags, CULL_LATE, 1, (uint32_t)c->cus, st);
         mark(EV_CULL_LATE);
         launch_raster(s, b, flags, 1, raster_grid, c->tuning.small_max_extent, c->tuning.resolve_cache_slots, st);
         launch_raster_big(s, b, g, flags, 1, c->tuning.small_max_extent, tile_grid, c->tuning.tile_item_tris, st);
@@ -464,6 +474,11 @@ int vkgv_create(const vkgv_create_info* info, vkgv_ctx** out)
     *out = nullptr;
     auto* c = new vkgv_ctx();
     c->device = info ? info->device : 0;
+    if (const char* fa = std::getenv("VKGV_FAULT_ALLOC_AFTER")) {
+        char* end = nullptr;
+        const long long n = std::strtoll(fa, &end, 10);
+        if (end != fa && n >= 0) c->fault_alloc_after = n;
+    }
     if (info && info->big_triangle_capacity) c->big_capacity = info->big_triangle_capacity;
     if (info && info->clip_triangle_capacity) c->clip_capacity = info->clip_triangle_capacity;
     if (c->big_capacity > (1u << 23) || c->clip_capacity > (1u << 22)) { delete c; return VKGV_ERROR_LIMIT; }
@@ -600,16 +615,10 @@ int vkgv_upload_scene(vkgv_ctx* c, const vkgv_mesh_desc* meshes, uint32_t mesh_c
         if (prims[p].mesh_index >= mesh_count || prims[p].material_index >= material_count)
             return fail(c, VKGV_ERROR_INVALID_ARGUMENT, "upload_scene: primitive " + std::to_string(p) + " out of range");
 
-    // ---- device copies (assets.cpp:375-429 staging upload).  The four
-    // buffer kinds of all meshes are packed into one device allocation each
-    // (every mesh's part 256-byte aligned); a primitive's four addresses
-    // point into them, as the reference's per-primitive buffer addresses do.
-    HIP_TRY(c, hipStreamSynchronize(c->stream));
-    for (auto& b : c->mesh_bufs) dfree(b);
-    c->mesh_bufs.assign(4, DevBuf());
-    c->mesh_meshlets.assign(mesh_count, 0);
-    std::vector<uint32_t> cone_first(mesh_count, 0xFFFFFFFFu);
-    std::vector<vkgv_meshlet_cone> all_cones;
+    // ---- packing (assets.cpp:375-429 staging upload).  The four buffer
+    // kinds of all meshes are packed into one device allocation each (every
+    // mesh's part 256-byte aligned); a primitive's four addresses point into
+    // them, as the reference's per-primitive buffer addresses do.
     std::vector<uint64_t> off((size_t)mesh_count * 4);
     uint64_t total[4] = { 0, 0, 0, 0 };
     auto part_bytes = [&](const vkgv_mesh_desc& m, int k) -> uint64_t {
@@ -633,12 +642,31 @@ int vkgv_upload_scene(vkgv_ctx* c, const vkgv_mesh_desc* meshes, uint32_t mesh_c
             off[4 * mi + k] = total[k];
             total[k] += (part_bytes(meshes[mi], k) + 255ull) & ~255ull;
         }
-    int rc;
-    for (int k = 0; k < 4; ++k) {
-        // raster records address the packed buffers with 32-bit offsets (elements
-        // for vertex indices, bytes for triangles and vertices)
+    // raster records address the packed buffers with 32-bit offsets (elements
+    // for vertex indices, bytes for triangles and vertices)
+    for (int k = 0; k < 4; ++k)
         if (total[k] / (k == 0 ? 4u : 1u) >= (1ull << 32))
             return fail(c, VKGV_ERROR_LIMIT, "upload_scene: packed mesh buffer beyond 32-bit offsets");
+
+    // ---- device copies.  From here on the old scene is gone: the context
+    // holds no scene (and no draw list, which points into it) until every
+    // buffer of the new one is in place, so a failure leaves frames refused
+    // with VKGV_ERROR_STATE instead of reading freed memory.
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    c->scene = false;
+    c->draws_set = false;
+    c->draw_count = 0;
+    c->prim_count = 0;
+    c->material_count = 0;
+    c->prim_meshlets.clear();
+    c->generation++;   // captured batch frames hold the old buffers
+    for (auto& b : c->mesh_bufs) dfree(b);
+    c->mesh_bufs.assign(4, DevBuf());
+    c->mesh_meshlets.assign(mesh_count, 0);
+    std::vector<uint32_t> cone_first(mesh_count, 0xFFFFFFFFu);
+    std::vector<vkgv_meshlet_cone> all_cones;
+    int rc;
+    for (int k = 0; k < 4; ++k) {
         std::vector<uint8_t> staging(total[k] ? total[k] : 1, 0);
         for (uint32_t mi = 0; mi < mesh_count; ++mi)
             if (part_bytes(meshes[mi], k)) std::memcpy(staging.data() + off[4 * mi + k], part_src(meshes[mi], k), part_bytes(meshes[mi], k));
@@ -655,7 +683,6 @@ int vkgv_upload_scene(vkgv_ctx* c, const vkgv_mesh_desc* meshes, uint32_t mesh_c
     }
     std::vector<vkgv_primitive> dp(prim_count);
     std::vector<uint32_t> cb(std::max<uint32_t>(prim_count, 1u), 0xFFFFFFFFu);
-    c->prim_meshlets.assign(prim_count, 0);
     for (uint32_t p = 0; p < prim_count; ++p) {
         const uint32_t mi = prims[p].mesh_index;
         vkgv_primitive& q = dp[p];
@@ -668,7 +695,6 @@ int vkgv_upload_scene(vkgv_ctx* c, const vkgv_mesh_desc* meshes, uint32_t mesh_c
         q.meshlet_count = meshes[mi].meshlet_count;
         q.material_index = prims[p].material_index;
         cb[p] = cone_first[mi];
-        c->prim_meshlets[p] = meshes[mi].meshlet_count;
     }
     if ((rc = upload(c, c->prims, dp.data(), sizeof(vkgv_primitive) * prim_count))) return rc;
     if ((rc = upload(c, c->materials, materials, sizeof(vkgv_material) * material_count))) return rc;
@@ -681,13 +707,12 @@ int vkgv_upload_scene(vkgv_ctx* c, const vkgv_mesh_desc* meshes, uint32_t mesh_c
     if ((rc = upload(c, c->cone_base, cb.data(), sizeof(uint32_t) * cb.size()))) return rc;
     if ((rc = upload(c, c->cones, all_cones.data(), sizeof(vkgv_meshlet_cone) * all_cones.size()))) return rc;
     HIP_TRY(c, hipStreamSynchronize(c->stream));
+    c->prim_meshlets.assign(prim_count, 0);
+    for (uint32_t p = 0; p < prim_count; ++p) c->prim_meshlets[p] = meshes[prims[p].mesh_index].meshlet_count;
     c->prim_count = prim_count;
     c->material_count = material_count;
+    // a new scene invalidates the draw list (world.cpp:181-182), cleared above
     c->scene = true;
-    // a new scene invalidates the draw list (world.cpp:181-182)
-    c->draws_set = false;
-    c->draw_count = 0;
-    c->generation++;   // captured batch frames hold the old buffers
     return VKGV_OK;
 }
 
@@ -752,6 +777,11 @@ int vkgv_set_draws(vkgv_ctx* c, const vkgv_meshlet_draw* draws, uint32_t count)
     }
     HIP_TRY(c, hipSetDevice(c->device));
     HIP_TRY(c, hipDeviceSynchronize());   // frames of any stream may still read the old buffers
+    // no draw list until the new one is complete: a failed allocation leaves
+    // frames refused (VKGV_ERROR_STATE), not reading the old list's freed records
+    c->draws_set = false;
+    c->draw_count = 0;
+    c->generation++;
     int rc;
     if ((rc = upload(c, c->draws, draws, sizeof(vkgv_meshlet_draw) * count))) return rc;
     if ((rc = dalloc(c, c->cull, sizeof(CullRec) * std::max<uint32_t>(count, 1u)))) return rc;
@@ -778,13 +808,13 @@ int reserve_transforms(vkgv_ctx* c, uint32_t count)
     if (c->transforms.bytes < bytes || !c->transforms.p || c->prev_transforms.bytes < bytes || !c->prev_transforms.p) {
         HIP_TRY(c, hipDeviceSynchronize());   // frames of any stream may still read the old buffers
         int rc;
+        c->generation++;
+        c->prev_count = 0;
         if (c->transforms.bytes < bytes || !c->transforms.p) {
+            c->transform_count = 0;           // contents lost, also if the allocation fails
             if ((rc = dalloc(c, c->transforms, bytes))) return rc;
-            c->transform_count = 0;           // contents lost
         }
         if ((rc = dalloc(c, c->prev_transforms, std::max(bytes, c->transforms.bytes)))) return rc;
-        c->prev_count = 0;
-        c->generation++;
     }
     if (count == c->transform_count && count == c->prev_count && bytes)
         HIP_TRY(c, hipMemcpyAsync(c->prev_transforms.p, c->transforms.p, bytes, hipMemcpyDeviceToDevice, c->stream));
@@ -821,14 +851,19 @@ int vkgv_set_transforms(vkgv_ctx* c, const float* mat4s, uint32_t count)
 }
 
 int vkgv_set_hierarchy(vkgv_ctx* c, const vkgv_node* nodes, uint32_t node_count, const uint32_t* children,
-                       const uint32_t* roots, uint32_t root_count, const vkgv_anim_sampler* samplers,
+                       uint32_t children_count, const uint32_t* roots, uint32_t root_count,
+                       const vkgv_anim_sampler* samplers,
                        uint32_t sampler_count, const float* times, uint32_t time_count, const float* values,
                        uint32_t value_count, const vkgv_anim_channel* channels, uint32_t channel_count)
 {
     if (!c) return VKGV_ERROR_INVALID_ARGUMENT;
-    if ((node_count && !nodes) || (root_count && !roots) || (sampler_count && !samplers) || (time_count && !times)
-        || (value_count && !values) || (channel_count && !channels))
+    if ((node_count && !nodes) || (children_count && !children) || (root_count && !roots) || (sampler_count && !samplers)
+        || (time_count && !times) || (value_count && !values) || (channel_count && !channels))
         return fail(c, VKGV_ERROR_INVALID_ARGUMENT, "set_hierarchy: null array");
+    for (uint32_t n = 0; n < node_count; ++n)
+        if ((uint64_t)nodes[n].first_child + nodes[n].child_count > children_count)
+            return fail(c, VKGV_ERROR_INVALID_ARGUMENT,
+                        "set_hierarchy: node " + std::to_string(n) + " lists children past the children array");
     // ---- validation: every index a kernel follows stays in bounds
     for (uint32_t sp = 0; sp < sampler_count; ++sp) {
         const vkgv_anim_sampler& a = samplers[sp];
@@ -857,7 +892,11 @@ int vkgv_set_hierarchy(vkgv_ctx* c, const vkgv_node* nodes, uint32_t node_count,
     }
     // depth-first walk of the roots (World::iterateNode order, as vkgv_world_build):
     // every mesh node takes the next transform and records its root-to-node chain
+    // glTF node trees form a forest: a node reached twice (listed under two
+    // parents, twice under one, or as a root and a child) is rejected, which
+    // also rules out cycles and bounds the walk by the node count
     std::vector<uint32_t> chain_first(1, 0), chain;
+    std::vector<uint8_t> seen(node_count, 0);
     struct Frame { uint32_t node, depth; };
     std::vector<Frame> stack;
     std::vector<uint32_t> path;
@@ -867,7 +906,11 @@ int vkgv_set_hierarchy(vkgv_ctx* c, const vkgv_node* nodes, uint32_t node_count,
         while (!stack.empty()) {
             const Frame f = stack.back();
             stack.pop_back();
-            if (f.depth >= 64) return fail(c, VKGV_ERROR_LIMIT, "set_hierarchy: deeper than 64 nodes (or a cycle)");
+            if (seen[f.node])
+                return fail(c, VKGV_ERROR_INVALID_ARGUMENT,
+                            "set_hierarchy: node " + std::to_string(f.node) + " is reached twice (not a forest)");
+            seen[f.node] = 1;
+            if (f.depth >= 64) return fail(c, VKGV_ERROR_LIMIT, "set_hierarchy: deeper than 64 nodes");
             path.resize(f.depth);
             path.push_back(f.node);
             const vkgv_node& n = nodes[f.node];
@@ -875,7 +918,6 @@ int vkgv_set_hierarchy(vkgv_ctx* c, const vkgv_node* nodes, uint32_t node_count,
                 chain.insert(chain.end(), path.begin(), path.end());
                 chain_first.push_back((uint32_t)chain.size());
             }
-            if (n.child_count && !children) return fail(c, VKGV_ERROR_INVALID_ARGUMENT, "set_hierarchy: null children");
             for (uint32_t k = n.child_count; k-- > 0;) {
                 const uint32_t child = children[n.first_child + k];
                 if (child >= node_count) return fail(c, VKGV_ERROR_INVALID_ARGUMENT, "set_hierarchy: child out of range");
@@ -1310,6 +1352,10 @@ int vkgv_view_read_stats(vkgv_ctx* c, vkgv_view* v, vkgv_frame_stats* s)
     s->clipped_triangles = h.clipped;
     s->overflow = h.overflow;
     s->max_tile_tris = h.max_tile_tris;
+    s->raster_triangles = h.raster_tris;
+    s->fragments = h.fragments;
+    s->fragment_atomics = h.frag_atomics;
+    s->fragment_stores = h.frag_stores;
     // Per-pass milliseconds averaged over the timed frames recorded since
     // the previous read_stats (at most the last 64).
     const uint32_t n = v->ring_frames;
@@ -1503,6 +1549,10 @@ int vkgv_batch_read_stats(vkgv_ctx* c, vkgv_batch* b, vkgv_frame_stats* s)
         s->clipped_triangles += h.clipped;
         s->overflow |= h.overflow;
         s->max_tile_tris = std::max(s->max_tile_tris, h.max_tile_tris);
+        s->raster_triangles += h.raster_tris;
+        s->fragments += h.fragments;
+        s->fragment_atomics += h.frag_atomics;
+        s->fragment_stores += h.frag_stores;
     }
     const uint32_t n = b->ring_frames;
     for (uint32_t f = 0; f < n; ++f) {

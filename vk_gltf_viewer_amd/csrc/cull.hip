@@ -1,12 +1,16 @@
 // Meshlet cull + compaction (visbuffer.task.glsl:25-76, culling.h.glsl:8-56).
 //
-// One lane per meshlet draw.  A wave reads 64 consecutive 32-byte cull
-// records (one contiguous 2 KiB span), transforms the meshlet AABB, runs the
-// frustum test, the optional normal-cone test and the Hi-Z probe, and appends
-// its visible draws to the list with one atomicAdd per wave (ballot + mbcnt
-// prefix).  The list order across waves is unspecified; the visible SET is
-// what the reference defines (its per-task-workgroup payload is re-packed by
-// the mesh-shader launch anyway).
+// A persistent grid of waves strides over work units of 64 * P consecutive
+// draws and a group of views.  A lane reads the 32-byte cull records of its
+// P draws (each load of the wave one contiguous 2 KiB span), transforms the
+// meshlet AABBs to world space once, then for every view of the group runs
+// the frustum test, the optional normal-cone test and the Hi-Z probe, and
+// appends the unit's visible draws to the view's list with one atomicAdd per
+// (unit, view) (ballot + mbcnt prefix).  The list order across units is
+// unspecified; the visible SET is what the reference defines (its
+// per-task-workgroup payload is re-packed by the mesh-shader launch anyway).
+#include <algorithm>
+
 #include "internal.hpp"
 
 namespace vkgv {
@@ -75,8 +79,11 @@ __device__ __forceinline__ float dot3(float ax, float ay, float az, float bx, fl
 }
 
 // Normal-cone back-face test (meshoptimizer's published cone test), skipped
-// unless the instance transform is a proper similarity.
-__device__ __forceinline__ bool cone_culled(const vkgv_meshlet_cone& cone, const Mat4& m, const float* eye)
+// unless the instance transform is a proper similarity.  The part that does
+// not depend on the eye: the cone in world space (apex, unit axis), or false
+// when the test does not apply.
+__device__ __forceinline__ bool cone_setup(const vkgv_meshlet_cone& cone, const Mat4& m, float apex_out[3],
+                                           float axis_out[3], float& cutoff)
 {
     if (!(cone.cutoff < 1.0f)) return false;
     const float* a = m.m;
@@ -93,10 +100,20 @@ __device__ __forceinline__ bool cone_culled(const vkgv_meshlet_cone& cone, const
     float3 ax = mat4_dir(m, cone.axis[0], cone.axis[1], cone.axis[2]);
     const float al = sqrtf(dot3(ax.x, ax.y, ax.z, ax.x, ax.y, ax.z));
     ax.x = ax.x / al; ax.y = ax.y / al; ax.z = ax.z / al;
-    float dx = apex.x - eye[0], dy = apex.y - eye[1], dz = apex.z - eye[2];
+    apex_out[0] = apex.x; apex_out[1] = apex.y; apex_out[2] = apex.z;
+    axis_out[0] = ax.x; axis_out[1] = ax.y; axis_out[2] = ax.z;
+    cutoff = cone.cutoff;
+    return true;
+}
+
+// The eye-dependent part: back-facing for this eye.
+__device__ __forceinline__ bool cone_culled_from(const float apex[3], const float axis[3], float cutoff,
+                                                 const float* eye)
+{
+    float dx = apex[0] - eye[0], dy = apex[1] - eye[1], dz = apex[2] - eye[2];
     const float dl = sqrtf(dot3(dx, dy, dz, dx, dy, dz));
     dx = dx / dl; dy = dy / dl; dz = dz / dl;
-    return dot3(dx, dy, dz, ax.x, ax.y, ax.z) >= cone.cutoff;
+    return dot3(dx, dy, dz, axis[0], axis[1], axis[2]) >= cutoff;
 }
 
 struct CullArgs {
@@ -105,72 +122,128 @@ struct CullArgs {
     uint32_t flags;
     uint32_t mode;
     uint32_t list;
+    uint32_t units;          // work units: chunks * groups
+    uint32_t groups;         // view groups per chunk
+    uint32_t group_views;    // views per group (the last group may hold fewer)
 };
 
-// Grid: (draw blocks, views).
-__global__ __launch_bounds__(kBlock) void cull_kernel(CullArgs a)
+// The view-independent part of a draw's cull: its world-space AABB and, for
+// cone culling, the transformed cone.  Computed once per draw and reused for
+// every view of the unit's view group.
+struct DrawCull {
+    float wc[3], we[3];
+    float apex[3], axis[3], cutoff;
+    bool valid, cone;
+};
+
+template <bool CONE>
+__device__ __forceinline__ void draw_cull_setup(const SceneDev& s, uint32_t d, DrawCull& dc)
 {
-    const uint32_t d = blockIdx.x * kBlock + threadIdx.x;
-    const uint32_t lane = threadIdx.x & 63u;
-    const ViewDev& view = batch_view(a.b, blockIdx.y);
-    const CameraArg& cam = batch_camera(a.b, blockIdx.y);
-    unsigned long long* history = reinterpret_cast<unsigned long long*>(view.history);
-    bool vis = false, newbit = false;
-    if (d < a.s.draw_count) {
-        const float4* rp = reinterpret_cast<const float4*>(a.s.cull + d);
-        const float4 r0 = rp[0], r1 = rp[1];
-        const uint32_t tidx = __float_as_uint(r1.z), cone = __float_as_uint(r1.w);
-        const Mat4 m = load_mat4(a.s.transforms + 16ull * tidx);
-        // world AABB: centre (M * vec4(c,1)).xyz, extent mat3(|M0|,|M1|,|M2|) * e
-        const float4 wc4 = mat4_point(m, r0.x, r0.y, r0.z);
-        const float wc[3] = { wc4.x, wc4.y, wc4.z };
-        const float e[3] = { r0.w, r1.x, r1.y };
-        float we[3];
+    dc.valid = d < s.draw_count;
+    dc.cone = false;
+    if (!dc.valid) return;
+    const float4* rp = reinterpret_cast<const float4*>(s.cull + d);
+    const float4 r0 = rp[0], r1 = rp[1];
+    const uint32_t tidx = __float_as_uint(r1.z);
+    const Mat4 m = load_mat4(s.transforms + 16ull * tidx);
+    // world AABB: centre (M * vec4(c,1)).xyz, extent mat3(|M0|,|M1|,|M2|) * e
+    const float4 wc4 = mat4_point(m, r0.x, r0.y, r0.z);
+    dc.wc[0] = wc4.x; dc.wc[1] = wc4.y; dc.wc[2] = wc4.z;
+    const float e[3] = { r0.w, r1.x, r1.y };
 #pragma unroll
-        for (int r = 0; r < 3; ++r)
-            we[r] = (fabsf(m.m[r]) * e[0] + fabsf(m.m[4 + r]) * e[1]) + fabsf(m.m[8 + r]) * e[2];
-        bool pass = true;
-        if (a.flags & VKGV_CULL_FRUSTUM) pass = aabb_in_frustum(wc, we, cam.cam);
-        if (pass && (a.flags & VKGV_CULL_CONE) && cone != 0xFFFFFFFFu)
-            pass = !cone_culled(a.s.cones[cone], m, cam.eye);
-        if (a.mode == CULL_SINGLE) {
-            vis = pass;
-            if (vis && (a.flags & VKGV_CULL_OCCLUSION_PREV))
-                vis = occlusion_visible(wc, we, cam.cam.prev_occlusion_view_projection, view.pyramid, view.pyr);
-        } else {
-            const bool old = (history[d >> 6] >> (d & 63u)) & 1ull;
-            if (a.mode == CULL_EARLY) {
-                vis = pass && old;
-            } else {
-                const bool occ = pass && occlusion_visible(wc, we, cam.cam.occlusion_view_projection, view.pyramid,
-                                                           view.pyr);
-                newbit = occ;
-                vis = occ && !(pass && old);
-            }
-        }
-    }
-    if (a.mode == CULL_LATE) {
-        const unsigned long long bits = __ballot(newbit);
-        if (lane == 0 && (d - lane) < a.s.draw_count) history[d >> 6] = bits;
-    }
-    const unsigned long long mask = __ballot(vis);
-    const uint32_t cnt = (uint32_t)__popcll(mask);
-    if (cnt == 0) return;
-    uint32_t base = 0;
-    if (lane == 0) base = atomicAdd(&view.counters->vis_count[a.list], cnt);
-    base = __shfl(base, 0);
-    if (vis) {
-        const uint32_t rank = __builtin_amdgcn_mbcnt_hi((uint32_t)(mask >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)mask, 0u));
-        const uint4* src = reinterpret_cast<const uint4*>(a.s.rrec + d);
-        uint4* dst = reinterpret_cast<uint4*>(view.visible[a.list] + base + rank);
-        const uint4 r0 = src[0], r1 = src[1];
-        dst[0] = r0;
-        dst[1] = r1;
+    for (int r = 0; r < 3; ++r)
+        dc.we[r] = (fabsf(m.m[r]) * e[0] + fabsf(m.m[4 + r]) * e[1]) + fabsf(m.m[8 + r]) * e[2];
+    if (CONE) {
+        const uint32_t ci = __float_as_uint(r1.w);
+        if (ci != 0xFFFFFFFFu) dc.cone = cone_setup(s.cones[ci], m, dc.apex, dc.axis, dc.cutoff);
     }
 }
 
-// Frame start: each block clears kClearKeysPerBlock keys of its view (16 per
-// thread, 16-byte stores); the view's first block also zeroes its counters.
+// Persistent cull + compaction.  A work unit is 64 * P consecutive draws
+// (lane l takes draws base + 64 i + l, i < P, so each of the P record loads
+// of a wave is one contiguous 2 KiB span) and a group of views.  Waves stride
+// over the units.  Per view, the P ballots of the unit give its visible count,
+// reserved in the view's list with one atomicAdd per (unit, view), and each
+// visible lane's position (mbcnt); the lists come out in draw order within a
+// unit.  In the late pass the same ballots write the history words.
+template <int P, bool CONE>
+__global__ __launch_bounds__(kBlock) void cull_kernel(CullArgs a)
+{
+    const uint32_t lane = threadIdx.x & 63u;
+    const uint32_t nwaves = (gridDim.x * kBlock) >> 6;
+    for (uint32_t u = (blockIdx.x * kBlock + threadIdx.x) >> 6; u < a.units; u += nwaves) {
+        const uint32_t chunk = u / a.groups, group = u - chunk * a.groups;
+        const uint32_t base = chunk * 64u * (uint32_t)P;
+        DrawCull dc[P];
+#pragma unroll
+        for (int i = 0; i < P; ++i) draw_cull_setup<CONE>(a.s, base + 64u * i + lane, dc[i]);
+        const uint32_t v0 = group * a.group_views, v1 = min(v0 + a.group_views, a.b.nviews);
+        for (uint32_t vi = v0; vi < v1; ++vi) {
+            const ViewDev& view = batch_view(a.b, vi);
+            const CameraArg& cam = batch_camera(a.b, vi);
+            unsigned long long* history = reinterpret_cast<unsigned long long*>(view.history);
+            unsigned long long mask[P];
+            uint32_t total = 0;
+#pragma unroll
+            for (int i = 0; i < P; ++i) {
+                const uint32_t d = base + 64u * i + lane;
+                bool vis = false, newbit = false;
+                if (dc[i].valid) {
+                    bool pass = true;
+                    if (a.flags & VKGV_CULL_FRUSTUM) pass = aabb_in_frustum(dc[i].wc, dc[i].we, cam.cam);
+                    if (CONE && pass && dc[i].cone)
+                        pass = !cone_culled_from(dc[i].apex, dc[i].axis, dc[i].cutoff, cam.eye);
+                    if (a.mode == CULL_SINGLE) {
+                        vis = pass;
+                        if (vis && (a.flags & VKGV_CULL_OCCLUSION_PREV))
+                            vis = occlusion_visible(dc[i].wc, dc[i].we, cam.cam.prev_occlusion_view_projection,
+                                                    view.pyramid, view.pyr);
+                    } else {
+                        const bool old = (history[d >> 6] >> (d & 63u)) & 1ull;
+                        if (a.mode == CULL_EARLY) {
+                            vis = pass && old;
+                        } else {
+                            const bool occ = pass && occlusion_visible(dc[i].wc, dc[i].we,
+                                                                       cam.cam.occlusion_view_projection, view.pyramid,
+                                                                       view.pyr);
+                            newbit = occ;
+                            vis = occ && !(pass && old);
+                        }
+                    }
+                }
+                if (a.mode == CULL_LATE) {
+                    const unsigned long long bits = __ballot(newbit);
+                    if (lane == 0 && base + 64u * i < a.s.draw_count) history[(base >> 6) + i] = bits;
+                }
+                mask[i] = __ballot(vis);
+                total += (uint32_t)__popcll(mask[i]);
+            }
+            if (total == 0) continue;
+            uint32_t out = 0;
+            if (lane == 0) out = atomicAdd(&view.counters->vis_count[a.list], total);
+            out = __shfl(out, 0);
+            RasterRec* dst = view.visible[a.list];
+#pragma unroll
+            for (int i = 0; i < P; ++i) {
+                if ((mask[i] >> lane) & 1ull) {
+                    const uint32_t rank =
+                        __builtin_amdgcn_mbcnt_hi((uint32_t)(mask[i] >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)mask[i], 0u));
+                    const uint4* src = reinterpret_cast<const uint4*>(a.s.rrec + base + 64u * i + lane);
+                    uint4* o = reinterpret_cast<uint4*>(dst + out + rank);
+                    const uint4 q0 = src[0], q1 = src[1];
+                    o[0] = q0;
+                    o[1] = q1;
+                }
+                out += (uint32_t)__popcll(mask[i]);
+            }
+        }
+    }
+}
+
+// Frame start: each block clears kClearKeysPerBlock keys of its view
+// (kClearKeysPerBlock / kBlock = 32 per thread, two per 16-byte store); the
+// view's first block also zeroes its counters.
+static_assert(kClearKeysPerBlock % (2 * kBlock) == 0, "the unrolled clear stores whole key pairs per thread");
 __global__ __launch_bounds__(kBlock) void frame_start_kernel(SceneDev s, BatchDev b)
 {
     const uint32_t e = b.map_clear[blockIdx.x];
@@ -296,16 +369,37 @@ void launch_prev_mvp(const SceneDev& s, const BatchDev& b, const BatchGrid& g, h
 }
 
 void launch_cull(const SceneDev& s, const BatchDev& b, const BatchGrid& g, uint32_t flags, CullMode mode,
-                 uint32_t list, hipStream_t st)
+                 uint32_t list, uint32_t cus, hipStream_t st)
 {
-    if (!s.draw_count) return;
+    if (!s.draw_count || !g.nviews) return;
     CullArgs a;
     a.s = s;
     a.b = b;
     a.flags = flags;
     a.mode = mode;
     a.list = list;
-    cull_kernel<<<dim3((s.draw_count + kBlock - 1) / kBlock, g.nviews), kBlock, 0, st>>>(a);
+    // Four draws per lane from 2M draws on (fewer atomics and loop trips per
+    // draw), one below (more waves for small lists).  Views are grouped so
+    // that the units still number at least 32 per CU: a group shares each
+    // record and transform load, and a many-view batch of a small scene gets
+    // its parallelism from the groups.
+    const uint32_t per_lane = s.draw_count >= (1u << 21) ? 4u : 1u;
+    const uint32_t chunks = (s.draw_count + 64u * per_lane - 1) / (64u * per_lane);
+    const uint32_t target = cus * 32u;
+    uint32_t groups = std::min<uint32_t>(g.nviews, std::max<uint32_t>(1u, (target + chunks - 1) / chunks));
+    a.group_views = (g.nviews + groups - 1) / groups;
+    a.groups = (g.nviews + a.group_views - 1) / a.group_views;
+    a.units = chunks * a.groups;
+    // persistent grid: at most 16 waves per CU, each striding over the units
+    const uint32_t blocks = std::min<uint32_t>((a.units + 3u) / 4u, cus * 4u);
+    const bool cone = flags & VKGV_CULL_CONE;
+    if (per_lane == 4) {
+        if (cone) cull_kernel<4, true><<<blocks, kBlock, 0, st>>>(a);
+        else cull_kernel<4, false><<<blocks, kBlock, 0, st>>>(a);
+    } else {
+        if (cone) cull_kernel<1, true><<<blocks, kBlock, 0, st>>>(a);
+        else cull_kernel<1, false><<<blocks, kBlock, 0, st>>>(a);
+    }
 }
 
 void launch_write_depth_keys(unsigned long long* keys, const float* depth, uint64_t n, hipStream_t st)

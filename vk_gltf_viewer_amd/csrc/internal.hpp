@@ -191,8 +191,14 @@ struct Counters {
     uint32_t max_tile_tris;         // most big triangles binned to one tile this frame
     uint32_t tile_items;            // work items of the tile pass (TileItem)
     uint32_t pad;
+    // raster statistics over the frame, counted only by frames with VKGV_STATS
+    unsigned long long raster_tris;     // triangles past the back-face test that were set up for raster
+    unsigned long long fragments;       // covered pixel centres with 0 <= z <= 1 (visibility-buffer updates)
+    unsigned long long frag_atomics;    // global 64-bit atomicMin writes of those updates
+    unsigned long long frag_stores;     // tile merges written with a plain read-min-store instead
+    unsigned long long pad2[4];
 };
-static_assert(sizeof(Counters) == 64, "Counters layout");
+static_assert(sizeof(Counters) == 128, "Counters layout");
 
 constexpr uint32_t kTileSplit = 8;      // most work items sharing one screen tile's triangle list
 // Resolve block: 64 columns x (4 * kResolveRows) rows; each thread resolves
@@ -302,8 +308,9 @@ void launch_clear_keys(unsigned long long* keys, uint64_t n, hipStream_t st);
 void launch_frame_start(const SceneDev& s, const BatchDev& b, const BatchGrid& g, hipStream_t st);
 // Every view's prevViewProjection * previous transform table (motion vectors).
 void launch_prev_mvp(const SceneDev& s, const BatchDev& b, const BatchGrid& g, hipStream_t st);
+// Persistent cull + compaction of every view's draws; `cus` sizes the grid.
 void launch_cull(const SceneDev& s, const BatchDev& b, const BatchGrid& g, uint32_t flags, CullMode mode,
-                 uint32_t list, hipStream_t st);
+                 uint32_t list, uint32_t cus, hipStream_t st);
 void launch_raster(const SceneDev& s, const BatchDev& b, uint32_t flags, uint32_t list, uint32_t grid_blocks,
                    uint32_t small_max_extent, uint32_t cache_slots, hipStream_t st);
 // Clipping, binning and tiled rasterization of the views' big-triangle queues.

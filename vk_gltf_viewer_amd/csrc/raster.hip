@@ -190,6 +190,38 @@ __device__ __forceinline__ void write_fragment(unsigned long long* keys, uint32_
     key_min(keys + (size_t)py * W + px, make_key(z, id));
 }
 
+// Per-lane raster statistics of frames with VKGV_STATS: triangles set up,
+// fragments (covered centres with 0 <= z <= 1), and how their visibility
+// updates reached memory (64-bit atomicMin, or a tile's plain
+// read-min-store).  The STATS = false instantiations count nothing.
+template <bool STATS>
+struct RasterCount {
+    uint32_t tris = 0, frags = 0, atomics = 0, stores = 0;
+    __device__ __forceinline__ void tri() { if constexpr (STATS) ++tris; }
+    __device__ __forceinline__ void frag_atomic() { if constexpr (STATS) { ++frags; ++atomics; } }
+    __device__ __forceinline__ void frag() { if constexpr (STATS) ++frags; }
+    __device__ __forceinline__ void atomic() { if constexpr (STATS) ++atomics; }
+    __device__ __forceinline__ void store() { if constexpr (STATS) ++stores; }
+    // Wave sums into the view's counters (all 64 lanes call it), then zero.
+    __device__ __forceinline__ void flush(Counters* c)
+    {
+        if constexpr (STATS) {
+            unsigned long long v[4] = { tris, frags, atomics, stores };
+#pragma unroll
+            for (int k = 0; k < 4; ++k)
+#pragma unroll
+                for (int off = 32; off > 0; off >>= 1) v[k] += __shfl_xor(v[k], off, 64);
+            if ((threadIdx.x & 63u) == 0) {
+                if (v[0]) atomicAdd(&c->raster_tris, v[0]);
+                if (v[1]) atomicAdd(&c->fragments, v[1]);
+                if (v[2]) atomicAdd(&c->frag_atomics, v[2]);
+                if (v[3]) atomicAdd(&c->frag_stores, v[3]);
+            }
+            tris = frags = atomics = stores = 0;
+        }
+    }
+};
+
 // Edge k of an oriented big triangle: from vertex (k+1)%3 to (k+2)%3.
 struct Edge {
     long long ex, ey, ax, ay;
@@ -220,7 +252,9 @@ __device__ __forceinline__ long long edge_at(const Edge& e, long long sx, long l
 // depth expression, so it writes exactly the fragments the binned path would.
 // Called by all 64 lanes of a wave.  Slower than the binned path, but no
 // triangle is lost.
-__device__ void wave_sweep(bool want, const TriSetup& t, uint32_t id, unsigned long long* keys, uint32_t W)
+template <bool STATS>
+__device__ void wave_sweep(bool want, const TriSetup& t, uint32_t id, unsigned long long* keys, uint32_t W,
+                           RasterCount<STATS>& rc)
 {
     const uint32_t lane = threadIdx.x & 63u;
     const int lx = (int)(lane & 7u), ly = (int)(lane >> 3);
@@ -260,6 +294,7 @@ __device__ void wave_sweep(bool want, const TriSetup& t, uint32_t id, unsigned l
                 const float zz = z[0] + (za * dxf + zb * dyf);
                 if (!(zz >= 0.0f && zz <= 1.0f)) continue;
                 write_fragment(keys, W, px, py, zz, tid);
+                rc.frag_atomic();
             }
         }
     }
@@ -450,8 +485,10 @@ __device__ __forceinline__ uint32_t tri_byte(const uint32_t* words, uint32_t b)
 
 // One view's share of the meshlet raster (wave w of the grid takes meshlets
 // w, w + waves, ... of the view's list).
+template <bool STATS>
 __device__ __forceinline__ void raster_view(const RasterArgs& a)
 {
+    RasterCount<STATS> rc;
     __shared__ float4 s_clip[kWaves][64];
     __shared__ SVtx s_sv[kWaves][64];
     __shared__ uint32_t s_tri[kWaves][96];
@@ -567,6 +604,7 @@ __device__ __forceinline__ void raster_view(const RasterArgs& a)
                     if (culled) continue;
                 }
                 id = (d << VKGV_TRIANGLE_BITS) | t;
+                rc.tri();
                 const SVtx v0 = s_sv[w][i0], v1 = s_sv[w][i1], v2 = s_sv[w][i2];
                 if (!(v0.ok & v1.ok & v2.ok)) {
                     const uint32_t slot = atomicAdd(&a.ctr->clip_count, 1u);
@@ -650,8 +688,10 @@ __device__ __forceinline__ void raster_view(const RasterArgs& a)
             for (;;) {
                 if ((e0 | e1 | e2) >= 0) {
                     const float z = st.z0 + (st.za * (st.oxf + (float)ii) + st.zb * (st.oyf + (float)jj));
-                    if (z >= 0.0f && z <= 1.0f)
+                    if (z >= 0.0f && z <= 1.0f) {
                         write_fragment(a.keys, a.W, (int)(st.org & 0xFFFFu) + ii, (int)(st.org >> 16) + jj, z, st.id);
+                        rc.frag_atomic();
+                    }
                 }
                 if (++k >= kend) break;
                 if (k == next) {   // first slot of a later triangle (empty ones skipped)
@@ -688,6 +728,7 @@ __device__ __forceinline__ void raster_view(const RasterArgs& a)
         }
         idx = nidx;      // static stride: no shared cursor
     }
+    rc.flush(a.ctr);
 }
 
 // ---------------------------------------------------------------------------
@@ -715,8 +756,10 @@ struct BigArgs {
 // One lane per queued triangle, in its own LDS polygon buffers; the fan
 // joins the big triangles, or, without room, the wave's direct sweep.  The
 // block is one wave, and every lane runs the same iterations.
+template <bool STATS>
 __device__ __forceinline__ void clip_view(const BigArgs& a)
 {
+    RasterCount<STATS> rc;
     __shared__ float4 s_poly[2][kClipMax][kClipThreads];
     const StridedPoly pa{ &s_poly[0][0][threadIdx.x] }, pb{ &s_poly[1][0][threadIdx.x] };
     const BigQueue q{ a.big, a.big_chunk, a.big_cap, a.chunk_tri, a.chunk_cap, a.ctr, a.keys, a.W };
@@ -740,9 +783,10 @@ __device__ __forceinline__ void clip_view(const BigArgs& a)
             TriSetup t;
             bool spill = false;
             if (f + 1 < m && fan_setup(pb, f, hw, hh, (int)a.W, (int)a.H, t)) spill = !push_big(t, ct.id, q);
-            wave_sweep(spill, t, ct.id, a.keys, a.W);
+            wave_sweep(spill, t, ct.id, a.keys, a.W, rc);
         }
     }
+    rc.flush(a.ctr);
 }
 
 __device__ __forceinline__ uint32_t chunks_of(const BigTri& t)
@@ -972,8 +1016,10 @@ constexpr int kTileRows = (int)(kBinTile * kBinTile / 64u) / kTileParts;     // 
 constexpr int kSetupWords = (int)(sizeof(BigSetup) / 4);
 static_assert(kSetupWords % 4 == 0, "setups are loaded 16 bytes at a time");
 
+template <bool STATS>
 __device__ __forceinline__ void tile_view(const BigArgs& a)
 {
+    RasterCount<STATS> rc;
     const uint32_t lane = threadIdx.x & 63u;
     const uint32_t wave = blockIdx.x * 4u + (threadIdx.x >> 6), nwaves = gridDim.x * 4u;
     const uint32_t nitems = __builtin_amdgcn_readfirstlane(a.ctr->tile_items);
@@ -1043,6 +1089,7 @@ __device__ __forceinline__ void tile_view(const BigArgs& a)
                     if (!(z >= 0.0f && z <= 1.0f)) continue;
                     const unsigned long long k = make_key(z, s.id);
                     key[r] = k < key[r] ? k : key[r];
+                    rc.frag();
                 }
             }
         }
@@ -1054,11 +1101,14 @@ __device__ __forceinline__ void tile_view(const BigArgs& a)
             unsigned long long* p = a.keys + (size_t)py * a.W + px;
             if (split == 1) {
                 if (key[r] < old[r]) *p = key[r];
+                rc.store();
             } else {
                 key_min(p, key[r]);
+                rc.atomic();
             }
         }
     }
+    rc.flush(a.ctr);
 }
 
 // ---------------------------------------------------------------------------
@@ -1106,6 +1156,7 @@ struct RasterBatchArgs {
 // (i + meshlets of the views before j) mod waves, so every wave gets within
 // one meshlet of the average however the meshlets spread over the views.
 // Three waves per SIMD: the register budget (168) the kernel fits without spilling.
+template <bool STATS>
 __global__ __launch_bounds__(kBlock, 3) void raster_kernel(RasterBatchArgs ab)
 {
     const uint32_t lane = threadIdx.x & 63u;
@@ -1161,7 +1212,7 @@ __global__ __launch_bounds__(kBlock, 3) void raster_kernel(RasterBatchArgs ab)
             a.slot_mat = v.slot_mat;
             a.draw_slot = v.draw_slot;
             a.cache_slots = min(v.cache_slots, ab.cache_slots);
-            raster_view(a);
+            raster_view<STATS>(a);
         }
         carry = (carry + __builtin_amdgcn_readlane(incl, 63)) % nwaves;
     }
@@ -1192,8 +1243,10 @@ struct RedoArgs {
 // first (in the lane's private memory) when it needs clipping.  A triangle
 // that did reach a queue is written twice; the visibility write is a
 // minimum, so the result is the same.  Every lane runs the same iterations.
+template <bool STATS>
 __device__ __forceinline__ void redo_view(const RedoArgs& ra, const ViewDev& v, uint32_t n)
 {
+    RasterCount<STATS> rc;
     const RasterRec* list = v.visible[ra.list];
     const float hw = (float)v.width * 0.5f, hh = (float)v.height * 0.5f;
     const int W = (int)v.width, H = (int)v.height;
@@ -1235,7 +1288,7 @@ __device__ __forceinline__ void redo_view(const RedoArgs& ra, const ViewDev& v, 
                     direct = !(ts.maxx - ts.minx < ra.small_lim && ts.maxy - ts.miny < ra.small_lim);
                 }
             }
-            wave_sweep(direct, ts, id, v.keys, v.width);
+            wave_sweep(direct, ts, id, v.keys, v.width, rc);
             if (__ballot(clip) == 0ull) continue;   // wave-uniform
             float4 A[kClipMax], B[kClipMax];
             const LocalPoly pa{ A }, pb{ B };
@@ -1248,13 +1301,15 @@ __device__ __forceinline__ void redo_view(const RedoArgs& ra, const ViewDev& v, 
                 if (__ballot(f + 1 < m) == 0ull) break;   // wave-uniform
                 TriSetup ft;
                 const bool want = f + 1 < m && fan_setup(pb, f, hw, hh, W, H, ft);
-                wave_sweep(want, ft, id, v.keys, v.width);
+                wave_sweep(want, ft, id, v.keys, v.width, rc);
             }
         }
     }
+    rc.flush(v.counters);
 }
 
 // Per view: the clip queue, then the meshlets to redo.
+template <bool STATS>
 __global__ __launch_bounds__(kClipThreads) void clip_kernel(BatchDev b, RedoArgs ra)
 {
     for (uint32_t v0 = 0; v0 < b.nviews; v0 += 64u) {
@@ -1262,9 +1317,9 @@ __global__ __launch_bounds__(kClipThreads) void clip_kernel(BatchDev b, RedoArgs
         const uint32_t nr = lane_view_value(b, v0, [](const Counters& c) { return c.redo_count; });
         const uint32_t m = min(64u, b.nviews - v0);
         for (uint32_t j = 0; j < m; ++j) {
-            if (__builtin_amdgcn_readlane(n, j)) clip_view(big_args(batch_view(b, v0 + j)));
+            if (__builtin_amdgcn_readlane(n, j)) clip_view<STATS>(big_args(batch_view(b, v0 + j)));
             const uint32_t r = __builtin_amdgcn_readlane(nr, j);
-            if (r) redo_view(ra, batch_view(b, v0 + j), r);
+            if (r) redo_view<STATS>(ra, batch_view(b, v0 + j), r);
         }
     }
 }
@@ -1297,6 +1352,7 @@ __global__ __launch_bounds__(256) void bin_fill_kernel(BatchDev b)
 
 // Wave 0 also clears every view's queue counters for the next pass (their
 // last reader, the fill pass, has finished).
+template <bool STATS>
 __global__ __launch_bounds__(256) void tile_kernel(BatchDev b)
 {
     const uint32_t wave = blockIdx.x * 4u + (threadIdx.x >> 6);
@@ -1311,7 +1367,7 @@ __global__ __launch_bounds__(256) void tile_kernel(BatchDev b)
         }
         const uint32_t m = min(64u, b.nviews - v0);
         for (uint32_t j = 0; j < m; ++j)
-            if (__builtin_amdgcn_readlane(n, j)) tile_view(big_args(batch_view(b, v0 + j)));
+            if (__builtin_amdgcn_readlane(n, j)) tile_view<STATS>(big_args(batch_view(b, v0 + j)));
     }
 }
 
@@ -1327,7 +1383,8 @@ void launch_raster(const SceneDev& s, const BatchDev& b, uint32_t flags, uint32_
     a.list = list;
     a.small_lim = (int)(small_max_extent * 256u);
     a.cache_slots = cache_slots;
-    raster_kernel<<<grid_blocks, kBlock, 0, st>>>(a);
+    if (flags & VKGV_STATS) raster_kernel<true><<<grid_blocks, kBlock, 0, st>>>(a);
+    else raster_kernel<false><<<grid_blocks, kBlock, 0, st>>>(a);
 }
 
 void launch_raster_big(const SceneDev& s, const BatchDev& b, const BatchGrid& g, uint32_t flags, uint32_t list,
@@ -1339,11 +1396,14 @@ void launch_raster_big(const SceneDev& s, const BatchDev& b, const BatchGrid& g,
     ra.list = list;
     ra.small_lim = (int)(small_max_extent * 256u);
     // one wave per block: enough waves for the redo path, which is rare
-    clip_kernel<<<std::max(64u, tile_grid / 4u), kClipThreads, 0, st>>>(b, ra);
+    const bool stats = flags & VKGV_STATS;
+    if (stats) clip_kernel<true><<<std::max(64u, tile_grid / 4u), kClipThreads, 0, st>>>(b, ra);
+    else clip_kernel<false><<<std::max(64u, tile_grid / 4u), kClipThreads, 0, st>>>(b, ra);
     bin_kernel<<<1024, 256, 0, st>>>(b);
     bin_scan_kernel<<<g.nviews, kScanThreads, 0, st>>>(b, tile_item_tris);
     bin_fill_kernel<<<256, 256, 0, st>>>(b);
-    tile_kernel<<<tile_grid, 256, 0, st>>>(b);
+    if (stats) tile_kernel<true><<<tile_grid, 256, 0, st>>>(b);
+    else tile_kernel<false><<<tile_grid, 256, 0, st>>>(b);
 }
 
 } // namespace vkgv

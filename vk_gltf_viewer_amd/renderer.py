@@ -36,6 +36,7 @@ class Renderer:
         defaults); a triangle that finds its queue full is rasterized by the
         slower direct path, so they change speed, never the image."""
         self._lib = N.hip()
+        self.device = device
         info = N.CreateInfo(device, big_triangle_capacity, clip_triangle_capacity)
         h = C.c_void_p()
         rc = self._lib.vkgv_create(C.byref(info), C.byref(h))
@@ -105,7 +106,8 @@ class Renderer:
         times, values = (np.ascontiguousarray(a, dtype=np.float32) for a in (h.times, h.values))
         ptr = lambda a: a.ctypes.data if a.size else None
         self._check(self._lib.vkgv_set_hierarchy(
-            self._h, ptr(nodes), len(nodes), ptr(children), ptr(roots), len(roots), ptr(samplers), len(samplers),
+            self._h, ptr(nodes), len(nodes), ptr(children), len(children), ptr(roots), len(roots), ptr(samplers),
+            len(samplers),
             ptr(times), len(times), ptr(values), len(values), ptr(channels),
             len(channels)))
 
@@ -131,6 +133,16 @@ class Renderer:
         t = N.Tuning()
         self._check(self._lib.vkgv_get_tuning(self._h, C.byref(t)))
         return t.small_max_extent, t.raster_blocks_per_cu, t.tile_item_tris, t.resolve_cache_slots
+
+    def device_identity(self) -> str:
+        """Architecture and compute-unit count of the context's device (the
+        autotuner's cache key)."""
+        try:
+            import torch
+            p = torch.cuda.get_device_properties(self.device)
+            return f"{p.gcnArchName}-{p.multi_processor_count}cu"
+        except Exception:
+            return f"hip-device-{self.device}"
 
     def create_batch(self, views) -> "Batch":
         return Batch(self, views)
