@@ -53,30 +53,48 @@ def visible_triangles(sc, draw_ids):
     return total
 
 
-# HBM bytes per launch from rocprofv3 FETCH_SIZE / WRITE_SIZE passes over this
-# benchmark, each scaled by the same counter's reading for a plain 256 MiB
-# copy kernel (profiles/traffic_c2_4k.json records the commands and values).
+# HBM bytes per frame of each pass, from rocprofv3 FETCH_SIZE / WRITE_SIZE
+# passes over this benchmark, each scaled by the same counter's reading for a
+# plain 256 MiB copy kernel (profiles/traffic_c2_4k.json records the commands,
+# the kernels of each pass and the per-dispatch values they are summed from).
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "traffic_c2_4k.json")
-PASS_KERNELS = {"clear": ["frame_start_kernel"], "hiz": ["hiz_tiles_kernel", "hiz_level_kernel", "hiz_tail_kernel"],
-                "resolve": ["resolve_kernel"]}
 
 
 def measured_traffic(pass_name, W, H):
-    try:
-        with open(TRAFFIC_FILE) as f:
-            t = json.load(f)
-    except (OSError, ValueError):
-        return None
-    if (t.get("width"), t.get("height")) != (W, H):
+    """HBM bytes of one frame's launches of the pass; None without a profile
+    of this size or pass."""
+    t = profile_json(os.path.basename(TRAFFIC_FILE))
+    if not t or (t.get("width"), t.get("height")) != (W, H):
         return None
     total = 0.0
     for c in ("FETCH_SIZE", "WRITE_SIZE"):
-        rec = t[c]
-        for name, kib in rec["per_kernel_kib"].items():
-            # template instantiations (resolve_kernel<...>) count as their kernel
-            if name.split("<")[0] in PASS_KERNELS[pass_name]:
-                total += kib * rec["factor"] * 1024.0
+        kib = t[c].get("per_pass_kib", {}).get(pass_name)
+        if kib is None:
+            return None
+        total += kib * t[c]["factor"] * 1024.0
     return round(total)
+
+
+def profile_json(name):
+    try:
+        with open(os.path.join(ROOT, "profiles", name)) as f:
+            return json.load(f)
+    except (OSError, ValueError):
+        return None
+
+
+def meshlet_bytes(sc, draw_ids):
+    """Sum over the listed meshlet draws of 12 B per vertex (position) and 3 B
+    per triangle (indices): the raster's algorithmic reads."""
+    d = sc.draws[np.asarray(draw_ids, dtype=np.int64)]
+    mesh_of = sc.primitives["mesh_index"][d["primitive_index"]]
+    total = 0
+    for mi, m in enumerate(sc.meshes):
+        sel = mesh_of == mi
+        if sel.any():
+            ml = m.meshlets[d["meshlet_index"][sel]]
+            total += int(12 * ml["vertex_count"].astype(np.int64).sum() + 3 * ml["triangle_count"].astype(np.int64).sum())
+    return total
 
 
 def batched_views(steps, warmup, log):
@@ -381,19 +399,207 @@ def widened_rows(sc, cam, W, H, steps, warmup, log):
     return out
 
 
+def cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
 def cpu_baseline(sc, cam, W, H, flags, budget_s=12.0, max_frames=8):
-    """The oracle (scalar C restatement, 1 thread) on frames of the same workload."""
+    """The oracle (the scalar C restatement, its frame loops parallel over all
+    the host's threads with OpenMP) on frames of the same workload: two
+    warm-up frames (the first fills the history, the second still rasterizes
+    every draw it listed), then steady-state frames within the budget."""
     import oracle
+    threads = oracle.threads(int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or (os.cpu_count() or 1))
     rs = oracle.Scene(sc)
     hist = np.zeros(len(sc.draws), np.uint8)
-    oracle.frame_two_pass(rs, cam, W, H, flags, hist)        # first frame: fills the history
+    for _ in range(2):
+        oracle.frame_two_pass(rs, cam, W, H, flags, hist)
     times = []
     t_end = time.perf_counter() + budget_s
     while len(times) < max_frames and time.perf_counter() < t_end:
         t0 = time.perf_counter()
         oracle.frame_two_pass(rs, cam, W, H, flags, hist)
         times.append(time.perf_counter() - t0)
-    return 1000.0 * float(np.median(times)), len(times)
+    return 1000.0 * float(np.median(times)), len(times), threads
+
+
+def timed_frames(r, view, cam, flags, steps, warmup):
+    """Wall ms per frame over `steps` frames after `warmup` (stream
+    synchronizations on both sides), then the per-pass event timings of
+    `steps` further frames, then the raster counts of one VKGV_STATS frame."""
+    for _ in range(max(1, warmup)):
+        view.render(cam, flags)
+    r.sync()
+    view.stats()                                  # drop warm-up timings
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        view.render(cam, flags)
+    r.sync()
+    ms = 1000.0 * (time.perf_counter() - t0) / steps
+    for _ in range(steps):
+        view.render(cam, flags | N.TIMING)
+    r.sync()
+    st = view.stats()
+    view.render(cam, flags | N.STATS)
+    counts = view.stats()
+    return ms, st, counts
+
+
+def pass_table(sc, view, st, counts, W, H):
+    """Per-pass ms and algorithmic bytes (per unit x units):
+      clear       8 B per pixel written (64-bit key)
+      cull_early  36 B per draw (12 B record + 24 B AABB) + 64 B per instance
+                  transform + 4 B per visible draw id written
+      cull_late   the same + 16 B per Hi-Z probe (2x2 fp32; at most one per draw)
+      raster      12 B per vertex + 3 B per triangle of every rasterized meshlet
+                  + 8 B per fragment (one 64-bit key update), both passes
+      hiz         4 B per depth pixel read + 4 B per pyramid texel written
+      resolve     8 B key read + 4 B RGBA8 written per pixel + 39 B per visible
+                  triangle (3 x 12 B positions + 3 B indices)"""
+    npx = W * H
+    vis = np.concatenate([view.visible(0), view.visible(1)])
+    n_draw, n_inst = len(sc.draws), len(sc.transforms)
+    ms = {"clear": st.ms_clear, "cull_early": st.ms_cull, "cull_late": st.ms_cull_late,
+          "raster": st.ms_raster + st.ms_raster_late, "hiz": st.ms_hiz, "resolve": st.ms_resolve}
+    alg = {"clear": 8.0 * npx,
+           "cull_early": 36.0 * n_draw + 64.0 * n_inst + 4.0 * st.visible_early,
+           "cull_late": 36.0 * n_draw + 64.0 * n_inst + 16.0 * n_draw + 4.0 * st.visible_late,
+           "raster": float(meshlet_bytes(sc, vis)) + 8.0 * counts.fragments,
+           "hiz": 4.0 * npx + 4.0 * pyramid_texels(W, H),
+           "resolve": 12.0 * npx + 39.0 * visible_triangles(sc, vis)}
+    return ms, alg, vis
+
+
+def raster_object(r, st, counts, ms_frame, log):
+    """Raster throughput against its ceilings: rasterized triangles, fragments
+    and 64-bit atomics per second over the raster passes' event time, the
+    atomics against the device's measured atomicMin rate, and the raster
+    kernels' VALU busy from the committed counter profile."""
+    ms_r = st.ms_raster + st.ms_raster_late
+    peak_c = r.measure_atomic_rate(False)
+    peak_s = r.measure_atomic_rate(True)
+    a_per_s = counts.fragment_atomics / (ms_r * 1e-3)
+    out = {"ms": round(ms_r, 4), "triangles": int(counts.raster_triangles), "fragments": int(counts.fragments),
+           "atomics": int(counts.fragment_atomics), "tile_stores": int(counts.fragment_stores),
+           "triangles_per_s": round(counts.raster_triangles / (ms_r * 1e-3), 1),
+           "fragments_per_s": round(counts.fragments / (ms_r * 1e-3), 1),
+           "atomics_per_s": round(a_per_s, 1),
+           "atomic_peak_coalesced_per_s": round(peak_c, 1), "atomic_peak_scattered_per_s": round(peak_s, 1),
+           "atomic_frac": round(a_per_s / peak_c, 4),
+           "rasterized_triangles_per_s_frame": round(counts.raster_triangles / (ms_frame * 1e-3), 1)}
+    pmc = profile_json("raster_pmc_c2_4k.json")
+    if pmc:
+        out["valu_busy"] = pmc.get("valu_busy")
+        out["valu_busy_source"] = "profiles/raster_pmc_c2_4k.json"
+    if log:
+        log(f"raster: {json.dumps(out)}")
+    return out
+
+
+def cold_frames(view, cam, flags, steps):
+    """The first frame of a view (no history, pyramid of an empty early
+    pass): the early pass draws nothing and the late pass every draw that
+    survives the frustum.  Event time per frame."""
+    tot, late = 0.0, 0
+    for _ in range(steps):
+        view.reset_history()
+        view.render(cam, flags | N.TIMING)
+        st = view.stats()
+        tot += st.ms_total
+        late = st.visible_late
+    return {"ms_per_frame_events": round(tot / steps, 4), "visible_early": 0, "visible_late": int(late)}
+
+
+def moving_frames(r, view, vd, flags, steps, warmup, step_m=1.5):
+    """Two-pass frames with the camera moving `step_m` metres per frame down
+    the street (previous-frame matrices from the frame before): the late pass
+    draws what the motion uncovers."""
+    from vk_gltf_viewer_amd.scene import camera_advance
+    vd = N.ViewDesc.from_buffer_copy(vd)
+    cam = camera_from_view(vd)
+    view.reset_history()
+    cams = []
+    for _ in range(warmup + steps):
+        vd.eye[0] += step_m
+        vd.target[0] += step_m
+        cam = camera_advance(cam, vd)
+        cams.append(cam)
+    for c in cams[:warmup]:
+        view.render(c, flags)
+    r.sync()
+    late = 0
+    t0 = time.perf_counter()
+    for c in cams[warmup:]:
+        view.render(c, flags)
+    r.sync()
+    ms = 1000.0 * (time.perf_counter() - t0) / steps
+    st = view.stats()
+    late = st.visible_late
+    return {"ms_per_frame": round(ms, 4), "metres_per_frame": step_m, "frames": steps,
+            "last_visible_early": int(st.visible_early), "last_visible_late": int(late)}
+
+
+def frame_workload(config, variant, steps, warmup, log, with_cpu, extra=True, tuning=(0, 0, 0)):
+    """The occlusion-heavy 4K city frame: steady-state two-pass frames
+    (value), per-pass rooflines, raster counts, and the cold and moving
+    frames of the same scene."""
+    t0 = time.perf_counter()
+    scene = Scene(config, config, variant)
+    sc = scene.arrays()
+    tris = scene.triangle_count
+    vd = sc.views[0]
+    W, H = vd.width, vd.height
+    cam = camera_from_view(vd)
+    gen_s = time.perf_counter() - t0
+    r = Renderer(0)
+    try:
+        if any(tuning):   # raster tunables given on the command line (0: the library default)
+            cur = r.get_tuning()
+            r.set_tuning(*[t or c for t, c in zip(tuning, cur[:3])], cur[3])
+        r.load(sc)
+        view = r.create_view(W, H)
+        flags = N.CULL_FRUSTUM | N.CULL_TWO_PASS | N.RESOLVE_BARY
+        ms_frame, st, counts = timed_frames(r, view, cam, flags, steps, warmup)
+        if st.overflow:
+            print("bench.py: WARNING a raster queue was full (direct path taken)", file=sys.stderr)
+        ms, alg, vis = pass_table(sc, view, st, counts, W, H)
+        gbs = {k: round(alg[k] / (ms[k] * 1e-3) / 1e9, 1) for k in alg if ms[k] > 0}
+        dom = max(ms, key=lambda k: ms[k])
+        ach = alg[dom] / (ms[dom] * 1e-3) / 1e9
+        roofline = {"bound": "hbm", "kernel": dom, "achieved": round(ach, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                    "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": measured_traffic(dom, W, H),
+                    "alg_bytes": round(alg[dom]), "ms": round(ms[dom], 4)}
+        out = {"scene": {"config": config, "variant": variant, "width": W, "height": H, "triangles": tris,
+                         "draws": len(sc.draws), "visible_early": int(st.visible_early),
+                         "visible_late": int(st.visible_late), "generation_s": round(gen_s, 2)},
+               "ms_frame": ms_frame, "events_ms_total": round(st.ms_total, 4),
+               "passes_ms": {k: round(v, 4) for k, v in ms.items()}, "passes_alg_gbs": gbs,
+               "passes_traffic": {k: measured_traffic(k, W, H) for k in ms},
+               "roofline": roofline, "raster": raster_object(r, st, counts, ms_frame, log), "sc": sc, "cam": cam}
+        if extra:
+            out["cold"] = cold_frames(view, cam, flags, max(3, steps // 4))
+            out["moving"] = moving_frames(r, view, vd, flags, steps, warmup)
+        if log:
+            log(f"frame {config}.{variant}: tuning {r.get_tuning()}; stats: big tris {st.big_triangles} "
+                f"tile pairs {st.big_tiles} max/tile {st.max_tile_tris} clipped {st.clipped_triangles}; "
+                f"passes {json.dumps(out['passes_ms'])}")
+        view.close()
+        if with_cpu:
+            cms, nframes, threads = cpu_baseline(sc, cam, W, H, flags)
+            out["cpu"] = {"value": round(cms, 1), "unit": "ms/frame", "cores": threads, "kind": "port",
+                          "cpu_model": cpu_model(), "nproc": os.cpu_count(),
+                          "sample": f"oracle (C restatement, OpenMP over {threads} threads) steady-state two-pass "
+                                    f"frames of the same {W}x{H} workload, median of {nframes}"}
+        return out
+    finally:
+        r.close()
 
 
 def main():
@@ -412,6 +618,8 @@ def main():
     ap.add_argument("--batch-only", action="store_true", help="only the 64-view batched-vs-loop measurement")
     ap.add_argument("--no-diff", action="store_true", help="skip the differentiable-form measurement")
     ap.add_argument("--no-uneven", action="store_true", help="skip the 16-view uneven stress measurement")
+    ap.add_argument("--no-literal", action="store_true",
+                    help="skip the literal 83.9M-triangle city (4096 level-5 spheres)")
     ap.add_argument("--uneven-only", action="store_true", help="only the 16-view uneven stress measurement")
     ap.add_argument("--diff-only", action="store_true", help="only the differentiable-form measurement")
     ap.add_argument("--widened-only", action="store_true",
@@ -454,96 +662,40 @@ def main():
                                                   args.warmup, log)}))
         return
 
-    t0 = time.perf_counter()
-    scene = Scene(args.config, args.config, args.variant)
-    sc = scene.arrays()
-    tris = scene.triangle_count
-    vd = sc.views[0]
-    W, H = vd.width, vd.height
-    cam = camera_from_view(vd)
-    gen_s = time.perf_counter() - t0
-
-    r = Renderer(0)
-    if args.small_max or args.blocks_per_cu or args.tile_item_tris:
-        sm, bpc, tit, rcs = r.get_tuning()
-        r.set_tuning(args.small_max or sm, args.blocks_per_cu or bpc, args.tile_item_tris or tit, rcs)
-    r.load(sc)
-    view = r.create_view(W, H)
-    flags = N.CULL_FRUSTUM | N.CULL_TWO_PASS | N.RESOLVE_BARY
-    for _ in range(max(1, args.warmup)):
-        view.render(cam, flags)
-    r.sync()
-    view.stats()                       # drop warm-up timings
-
-    t0 = time.perf_counter()
-    for _ in range(args.steps):
-        view.render(cam, flags)
-    r.sync()
-    elapsed = time.perf_counter() - t0
-    ms_frame = 1000.0 * elapsed / args.steps
-    # per-pass durations from HIP events, in frames of their own after the timed
-    # region (the event records would add their own cost to the timed frames)
-    for _ in range(args.steps):
-        view.render(cam, flags | N.TIMING)
-    r.sync()
-    st = view.stats()
-    if st.overflow:
-        print("bench.py: WARNING a raster queue was full (direct path taken)", file=sys.stderr)
-
-    passes = {
-        "clear": st.ms_clear, "cull_early": st.ms_cull, "raster_early": st.ms_raster, "hiz": st.ms_hiz,
-        "cull_late": st.ms_cull_late, "raster_late": st.ms_raster_late, "resolve": st.ms_resolve,
-    }
-    # Algorithmic bytes per launch of the HBM-bound passes (per unit x units):
-    #  clear    8 B per pixel written (64-bit key)
-    #  cull     36 B per draw (12 B record + 24 B AABB) + 64 B per instance
-    #           transform + 16 B per Hi-Z probe (2x2 fp32; at most one per
-    #           draw) + 4 B per visible draw id written
-    #  hiz      4 B per depth pixel read + 4 B per pyramid texel written
-    #  resolve  8 B key read + 4 B RGBA8 written per pixel + 39 B per visible
-    #           triangle (3 x 12 B positions + 3 B indices)
-    npx = W * H
-    vis = np.concatenate([view.visible(0), view.visible(1)])
-    vis_tris = visible_triangles(sc, vis)
-    n_draw, n_inst = len(sc.draws), len(sc.transforms)
-    alg = {"clear": 8.0 * npx,
-           "cull_early": 36.0 * n_draw + 64.0 * n_inst + 16.0 * n_draw + 4.0 * st.visible_early,
-           "hiz": 4.0 * npx + 4.0 * pyramid_texels(W, H),
-           "resolve": 12.0 * npx + 39.0 * vis_tris}
-    gbs = {k: round(alg[k] / (passes[k] * 1e-3) / 1e9, 1) for k in alg if passes[k] > 0}
-    dom = max(("clear", "hiz", "resolve"), key=lambda k: passes[k])
-    ach = alg[dom] / (passes[dom] * 1e-3) / 1e9
-    roofline = {"bound": "hbm", "kernel": dom, "achieved": round(ach, 1), "peak": HBM_PEAK_GBS,
-                "unit": "GB/s", "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": measured_traffic(dom, W, H)}
-
-    cpu = None
-    if not args.no_cpu_baseline:
-        cms, nframes = cpu_baseline(sc, cam, W, H, N.CULL_FRUSTUM | N.CULL_TWO_PASS | N.RESOLVE_BARY)
-        cpu = {"value": round(cms, 1), "unit": "ms/frame", "cores": 1, "kind": "port",
-               "sample": f"oracle (scalar C restatement) two-pass frames of the same 4K workload, median of {nframes}"}
-
+    tuning = (args.small_max, args.blocks_per_cu, args.tile_item_tris)
+    f = frame_workload(args.config, args.variant, args.steps, args.warmup, log, not args.no_cpu_baseline,
+                       tuning=tuning)
+    sc, cam = f.pop("sc"), f.pop("cam")
+    scn = f["scene"]
+    W, H, tris = scn["width"], scn["height"], scn["triangles"]
     out = {
         "metric": "ms/frame (cull+HiZ+raster+resolve) 4K 10M tris; batched views/s vs plain loop",
-        "value": round(ms_frame, 4), "unit": "ms/frame", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
-        "ms_per_step": round(ms_frame, 4), "higher_is_better": False, "scaling": "weak", "vs_baseline": None,
-        "dtype": "f32", "data": "synthetic (seeded generator)",
+        "value": round(f["ms_frame"], 4), "unit": "ms/frame", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": round(f["ms_frame"], 4), "higher_is_better": False, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f32", "data": "synthetic (seeded generator)",
         "config": {"workload": f"occlusion-heavy city config {args.config}.{args.variant}: {W}x{H}, "
-                               f"{len(sc.draws)} meshlet draws, {tris} triangles, two-pass Hi-Z, barycentric resolve",
-                   "width": W, "height": H, "triangles": tris, "draws": len(sc.draws),
-                   "visible_early": st.visible_early, "visible_late": st.visible_late,
+                               f"{scn['draws']} meshlet draws, {tris} triangles, two-pass Hi-Z, barycentric resolve, "
+                               f"static camera (steady state)",
+                   "width": W, "height": H, "triangles": tris, "draws": scn["draws"],
+                   "visible_early": scn["visible_early"], "visible_late": scn["visible_late"],
                    "parallelism": "single GPU"},
-        "passes_ms": {k: round(v, 4) for k, v in passes.items()},
-        "passes_alg_gbs": gbs,
-        "gpu_frame_ms_events": round(st.ms_total, 4),
-        "triangles_per_s": round(tris / (ms_frame * 1e-3), 1),
-        "roofline": roofline,
-        "cpu_baseline": cpu,
+        "passes_ms": f["passes_ms"], "passes_alg_gbs": f["passes_alg_gbs"], "passes_traffic": f["passes_traffic"],
+        "gpu_frame_ms_events": f["events_ms_total"],
+        "rasterized_triangles_per_s": f["raster"]["rasterized_triangles_per_s_frame"],
+        "roofline": f["roofline"],
+        "raster": f["raster"],
+        "cpu_baseline": f.get("cpu"),
+        "c2_cold": f.get("cold"),
+        "c2_moving": f.get("moving"),
     }
-    if args.verbose:
-        print(f"scene generation {gen_s:.2f}s; tuning {r.get_tuning()}; stats: big tris {st.big_triangles} "
-              f"tile pairs {st.big_tiles} max/tile {st.max_tile_tris} clipped {st.clipped_triangles}; "
-              f"passes {json.dumps({k: round(v, 4) for k, v in passes.items()})}", file=sys.stderr)
-    r.close()
+    if not args.no_literal:
+        g = frame_workload(2, 1, min(args.steps, 10), min(args.warmup, 3), log, False)
+        g.pop("sc"), g.pop("cam")
+        out["c2_literal"] = {"scene": g["scene"], "ms_per_frame": round(g["ms_frame"], 4),
+                             "passes_ms": g["passes_ms"], "raster": {k: g["raster"][k] for k in
+                                                                     ("triangles", "fragments", "triangles_per_s",
+                                                                      "fragments_per_s", "atomics_per_s")},
+                             "cold": g["cold"], "moving": g["moving"]}
     if not args.no_widened:
         out["widened"] = widened_rows(sc, cam, W, H, min(args.steps, 20), args.warmup, log)
     if not args.no_batch:

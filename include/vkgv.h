@@ -280,6 +280,13 @@ int         vkgv_get_tuning(const vkgv_ctx* ctx, vkgv_tuning* tuning);
  * thresholds <= c.  Host only; needs no device. */
 void        vkgv_srgb_thresholds(float out[255]);
 
+/* Calibration (no counterpart in the reference, whose raster is fixed
+ * function): the device's rate of 64-bit atomicMin without a returned value,
+ * the raster's fragment write, on a 3840 x 2160 key buffer the call allocates
+ * and frees.  scattered = 0: the lanes of a wave update consecutive keys;
+ * 1: pseudo-random keys.  Best of three timed launches; synchronous. */
+int         vkgv_measure_atomic_rate(vkgv_ctx* ctx, uint32_t scattered, double* atomics_per_second);
+
 /* Copies meshes, primitives and materials to device memory owned by the
  * context (assets.cpp:375-429 staging upload, world.cpp:10-185 primitive and
  * material buffers).  Replaces any previous scene. */
@@ -287,6 +294,19 @@ int vkgv_upload_scene(vkgv_ctx* ctx,
                       const vkgv_mesh_desc* meshes, uint32_t mesh_count,
                       const vkgv_primitive_desc* primitives, uint32_t primitive_count,
                       const vkgv_material* materials, uint32_t material_count);
+
+/* vkgv_upload_scene with the mesh arrays in caller-owned device memory: the
+ * pointers of each vkgv_mesh_desc (indices, vertices, meshlets, cones) are
+ * device addresses, as the reference's Primitive holds buffer device
+ * addresses (mesh_common.h.glsl:102-113, filled in world.cpp:10-185); the
+ * descriptors, primitives and materials themselves are host arrays.  The
+ * library copies device to device into its packed buffers and checks every
+ * meshlet on the device (counts, ranges, vertex and triangle indices) before
+ * accepting the scene; the caller's buffers may be freed on return. */
+int vkgv_upload_scene_device(vkgv_ctx* ctx,
+                             const vkgv_mesh_desc* meshes, uint32_t mesh_count,
+                             const vkgv_primitive_desc* primitives, uint32_t primitive_count,
+                             const vkgv_material* materials, uint32_t material_count);
 
 /* Albedo textures of the textured resolve: the sampling the reference's
  * fragment shader leaves commented out (visbuffer.frag.glsl:31-32,
@@ -318,6 +338,12 @@ int vkgv_set_draws(vkgv_ctx* ctx, const vkgv_meshlet_draw* draws, uint32_t count
 /* Per-node mat4 transforms (World::updateTransformBuffer, world.cpp:295-345).
  * The transforms they replace become the previous frame's (motion vectors). */
 int vkgv_set_transforms(vkgv_ctx* ctx, const float* mat4s, uint32_t count);
+/* The same from device memory (a transform buffer another kernel wrote, e.g.
+ * a training step's parameters): copied device to device on `stream` (NULL:
+ * the context's stream), with no host synchronization unless the context's
+ * transform buffers must grow.  `mat4s` may be reused once the stream has
+ * passed the call. */
+int vkgv_set_transforms_device(vkgv_ctx* ctx, const float* mat4s, uint32_t count, void* stream);
 
 /* ---- Node hierarchy and animations on the device ------------------------ */
 /* glTF node (TRS form, as fastgltf's DecomposeNodeMatrices gives it). */

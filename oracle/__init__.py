@@ -79,6 +79,7 @@ def _setup(L):
         "ref_hiz_build": (None, [p, u32, u32, p]),
         "ref_keys_clear": (None, [p, C.c_uint64]),
         "ref_threads": (C.c_int, [C.c_int]),
+        "ref_raster_stats": (None, [C.c_int, p, p]),
         "ref_raster": (None, [p, p, p, u32, u32, u32, u32, p]),
         "ref_raster_triangle": (None, [p, p, p, u32, u32, u32, p]),
         "ref_keys_depth": (None, [p, C.c_uint64, p]),
@@ -143,6 +144,14 @@ class Scene:
 def threads(n: int = 0) -> int:
     """OpenMP threads of the frame functions; n > 0 sets the count."""
     return int(lib().ref_threads(int(n)))
+
+
+def raster_stats(reset: bool = True):
+    """(triangles past the back-face test, fragments) rasterized since the
+    last reset; reset zeroes the counters."""
+    t, f = C.c_uint64(), C.c_uint64()
+    lib().ref_raster_stats(1 if reset else 0, C.byref(t), C.byref(f))
+    return t.value, f.value
 
 
 def pyramid_info(w, h):

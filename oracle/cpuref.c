@@ -443,6 +443,10 @@ static int project_vtx(const float* c, uint32_t w, uint32_t h, svtx* o)
 
 static inline int64_t floor_div256(int64_t v) { return v >= 0 ? v / 256 : -((-v + 255) / 256); }
 
+/* Raster statistics (ref_raster_stats): triangles past the back-face test,
+ * and fragments (covered pixel centres with 0 <= z <= 1). */
+static uint64_t g_stat_tris, g_stat_frags;
+
 /* *dst = min(*dst, key), atomically: the raster runs its draws on several
  * threads, and a minimum does not depend on the order of the updates. */
 static inline void key_min(uint64_t* dst, uint64_t key)
@@ -486,6 +490,7 @@ static void fill_triangle(svtx v0, svtx v1, svtx v2, uint32_t id, uint32_t w, ui
     const float za = (float)((dz1 * e2y - dz2 * e1y) / det);
     const float zb = (float)((dz2 * e1x - dz1 * e2x) / det);
 
+    uint64_t frags = 0;
     for (int64_t py = py0; py <= py1; ++py) {
         const int64_t sy = py * 256 + 128;
         for (int64_t px = px0; px <= px1; ++px) {
@@ -503,7 +508,19 @@ static void fill_triangle(svtx v0, svtx v1, svtx v2, uint32_t id, uint32_t w, ui
             z = z + 0.0f;
             const uint64_t key = ((uint64_t)(~f2u(z)) << 32) | (uint64_t)(0xFFFFFFFEu - id);
             key_min(&keys[(uint64_t)py * w + (uint64_t)px], key);
+            ++frags;
         }
+    }
+    if (frags) __atomic_fetch_add(&g_stat_frags, frags, __ATOMIC_RELAXED);
+}
+
+void ref_raster_stats(int reset, uint64_t* tris, uint64_t* frags)
+{
+    if (tris) *tris = __atomic_load_n(&g_stat_tris, __ATOMIC_RELAXED);
+    if (frags) *frags = __atomic_load_n(&g_stat_frags, __ATOMIC_RELAXED);
+    if (reset) {
+        __atomic_store_n(&g_stat_tris, 0, __ATOMIC_RELAXED);
+        __atomic_store_n(&g_stat_frags, 0, __ATOMIC_RELAXED);
     }
 }
 
@@ -602,6 +619,7 @@ void ref_raster(const ref_scene* s, const float* vp, const uint32_t* draw_list, 
                 const int culled = tdet < 0.0f ? det < 0.0f : det > 0.0f;
                 if (culled) continue;
             }
+            __atomic_fetch_add(&g_stat_tris, 1, __ATOMIC_RELAXED);
             ref_raster_triangle(a, b, c, (d << VKGV_TRIANGLE_BITS) | t, w, h, keys);
         }
     }

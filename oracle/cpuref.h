@@ -85,6 +85,10 @@ void ref_cull(const ref_scene* s, const vkgv_camera* cam, const float* eye,
 void ref_hiz_build(const float* depth, uint32_t w, uint32_t h, float* pyramid);
 
 /* ---- visbuffer.mesh.glsl + fixed-function raster + visbuffer.frag.glsl --- */
+/* Raster statistics accumulated by ref_raster (all threads) since the last
+ * reset: triangles past the back-face test, and fragments (covered pixel
+ * centres with 0 <= z <= 1).  reset != 0 zeroes them after reading. */
+void ref_raster_stats(int reset, uint64_t* tris, uint64_t* frags);
 /* OpenMP threads of the frame functions (n <= 0: unchanged); returns the count. */
 int  ref_threads(int n);
 void ref_keys_clear(uint64_t* keys, uint64_t n);

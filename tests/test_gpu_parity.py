@@ -274,3 +274,41 @@ def test_error_paths(rend):
     finally:
         v.close()
     rend.set_transforms(sc.transforms)
+
+
+@pytest.mark.parametrize("cfg", [1, 2])
+def test_raster_stats_match_oracle(rend, cfg):
+    """VKGV_STATS frames: the same keys as uncounted frames, and the device's
+    raster counts equal the oracle's on the same two-pass frames (triangles
+    past the back-face test; fragments = covered centres with 0 <= z <= 1);
+    each fragment reaches memory through at most one atomic or tile store.
+    Two views from the same start: one renders counted frames, one not."""
+    sc = scene(cfg, cfg)
+    load(rend, sc)
+    vd = sc.views[0]
+    cam = camera_from_view(vd)
+    W, H = vd.width, vd.height
+    plain, counted = rend.create_view(W, H), rend.create_view(W, H)
+    flags = N.CULL_FRUSTUM | N.CULL_TWO_PASS
+    try:
+        rs = oracle.Scene(sc)
+        hist = np.zeros(len(sc.draws), np.uint8)
+        for frame in range(3):
+            oracle.raster_stats(reset=True)
+            rk, _, _, _, _ = oracle.frame_two_pass(rs, cam, W, H, flags, hist)
+            tris, frags = oracle.raster_stats(reset=True)
+            plain.render(cam, flags)
+            counted.render(cam, flags | N.STATS)
+            sp, sc_ = plain.stats(), counted.stats()
+            print(f"cfg{cfg} frame {frame}: oracle {tris} triangles {frags} fragments; device {sc_.raster_triangles} "
+                  f"triangles {sc_.fragments} fragments, {sc_.fragment_atomics} atomics, {sc_.fragment_stores} "
+                  f"tile stores", flush=True)
+            assert np.array_equal(plain.keys(), rk) and np.array_equal(counted.keys(), rk), frame
+            assert sp.raster_triangles == 0 and sp.fragments == 0, "uncounted frames count nothing"
+            assert sc_.overflow == 0
+            assert sc_.raster_triangles == tris, frame
+            assert sc_.fragments == frags, frame
+            assert sc_.fragment_atomics + sc_.fragment_stores <= sc_.fragments
+    finally:
+        plain.close()
+        counted.close()

@@ -333,3 +333,50 @@ def test_clipping_non_finite_and_degenerate_input_is_bounded():
         keys = oracle.raster_triangle(c0, c1, c2, 9, W, H)
         cov = keys != CLEAR
         assert np.all(oracle.keys_ids(keys[cov]) == 9)
+
+
+def test_raster_stats_count_fragments_and_front_faces():
+    """The oracle's raster counters (the reference for the device's
+    VKGV_STATS counts): a pixel-aligned rectangle of W x H pixels tessellated
+    into two triangles gives exactly W * H fragments (every centre once, by the
+    tie rule); an outward-wound sphere with back-face culling sets up exactly
+    the triangles whose float64 screen winding faces the camera, and its
+    fragments are at least the covered pixels (overdraw only adds)."""
+    Wv, Hv = 64, 48
+    x0, y0, x1, y1 = 5, 7, 41, 30
+    corners = [(x0, y0), (x1, y0), (x1, y1), (x0, y1)]
+    c = [px_to_clip(x, y, 0.5, Wv, Hv) for (x, y) in corners]
+    oracle.raster_stats(reset=True)
+    keys = np.full(Wv * Hv, CLEAR, np.uint64)
+    oracle.raster_triangle(c[0], c[1], c[2], 1, Wv, Hv, keys=keys)
+    oracle.raster_triangle(c[0], c[2], c[3], 2, Wv, Hv, keys=keys)
+    _, frags = oracle.raster_stats(reset=True)
+    assert frags == (x1 - x0) * (y1 - y0)
+    assert (keys != CLEAR).sum() == frags
+    # front faces of a closed sphere against float64 screen winding
+    pos, idx = icosphere(3)
+    sc = tiny_scene([(pos, idx)], [(0, translate_scale((0, 0, 0), 2.0), (1, 1, 1, 1), False)])
+    rs = oracle.Scene(sc)
+    W, H = 128, 96
+    cam = camera_from_view(view_desc(W, H, (0.5, 0.7, 8), (0, 0, 0)))
+    VP = np.asarray(cam.view_projection, np.float64).reshape(4, 4).T
+    M = np.asarray(translate_scale((0, 0, 0), 2.0), np.float64)
+    clip = (VP @ M @ np.c_[pos, np.ones(len(pos))].T).T
+    s = clip[:, :2] / clip[:, 3:4]
+    t = idx.reshape(-1, 3)
+    area = ((s[t[:, 1], 0] - s[t[:, 0], 0]) * (s[t[:, 2], 1] - s[t[:, 0], 1])
+            - (s[t[:, 1], 1] - s[t[:, 0], 1]) * (s[t[:, 2], 0] - s[t[:, 0], 0]))
+    assert np.abs(area).min() > 1e-9                    # no edge-on triangle to misjudge
+    lst = np.arange(len(sc.draws), dtype=np.uint32)
+    oracle.raster_stats(reset=True)
+    keys = oracle.raster(rs, cam.view_projection, lst, W, H)
+    tris, frags = oracle.raster_stats(reset=True)
+    # the visibility buffer's Y flip reverses the winding of front faces (reverse-Z
+    # projection with P[1][1] negated): they are the ones with negative area here
+    front = int((area < 0).sum())
+    assert tris == front, (tris, front, int((area > 0).sum()))
+    assert frags >= (keys != CLEAR).sum() > 1000
+    oracle.raster_stats(reset=True)
+    oracle.raster(rs, cam.view_projection, lst, W, H, flags=N.NO_BACKFACE)
+    tris_all, frags_all = oracle.raster_stats(reset=True)
+    assert tris_all == len(t) and frags_all > frags

@@ -217,7 +217,8 @@ HIP_SYMBOLS = [
     "vkgv_batch_create", "vkgv_batch_destroy", "vkgv_render_batch", "vkgv_batch_read_stats",
     "vkgv_view_aa_image", "vkgv_view_aa_backward", "vkgv_upload_textures",
     "vkgv_set_hierarchy",
-    "vkgv_update_transforms", "vkgv_read_transforms", "vkgv_view_read_motion",
+    "vkgv_update_transforms", "vkgv_read_transforms", "vkgv_view_read_motion", "vkgv_measure_atomic_rate",
+    "vkgv_upload_scene_device", "vkgv_set_transforms_device",
 ]
 
 
@@ -255,6 +256,9 @@ def hip():
               C.POINTER(Material), _u32)
         _bind(lib, "vkgv_set_draws", C.c_int, _p, _p, _u32)
         _bind(lib, "vkgv_set_transforms", C.c_int, _p, _p, _u32)
+        _bind(lib, "vkgv_set_transforms_device", C.c_int, _p, _p, _u32, _p)
+        _bind(lib, "vkgv_upload_scene_device", C.c_int, _p, C.POINTER(MeshDesc), _u32, C.POINTER(PrimitiveDesc), _u32,
+              C.POINTER(Material), _u32)
         _bind(lib, "vkgv_view_create", C.c_int, _p, _u32, _u32, C.POINTER(_p))
         _bind(lib, "vkgv_view_destroy", None, _p)
         _bind(lib, "vkgv_render", C.c_int, _p, _p, C.POINTER(FrameDesc))
@@ -269,6 +273,7 @@ def hip():
         _bind(lib, "vkgv_set_tuning", C.c_int, _p, C.POINTER(Tuning))
         _bind(lib, "vkgv_get_tuning", C.c_int, _p, C.POINTER(Tuning))
         _bind(lib, "vkgv_srgb_thresholds", None, _p)
+        _bind(lib, "vkgv_measure_atomic_rate", C.c_int, _p, _u32, C.POINTER(C.c_double))
         _bind(lib, "vkgv_batch_create", C.c_int, _p, _p, _u32, C.POINTER(_p))
         _bind(lib, "vkgv_batch_destroy", None, _p)
         _bind(lib, "vkgv_render_batch", C.c_int, _p, _p, _p, _u32)

@@ -19,7 +19,7 @@ CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "lib")
 INCLUDE = os.path.join(ROOT, "include")
 
-HIP_SOURCES = ["api.hip", "cull.hip", "raster.hip", "hiz.hip", "resolve.hip", "grad.hip", "anim.hip"]
+HIP_SOURCES = ["api.hip", "cull.hip", "raster.hip", "hiz.hip", "resolve.hip", "grad.hip", "anim.hip", "calib.hip"]
 HIP_HEADERS = ["internal.hpp", "device_math.hpp"]
 
 # -ffp-contract=off: the kernels fix their floating-point evaluation order.

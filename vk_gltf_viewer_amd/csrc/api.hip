@@ -527,6 +527,23 @@ void vkgv_srgb_thresholds(float out[255])
     if (out) srgb_thresholds(out);
 }
 
+int vkgv_measure_atomic_rate(vkgv_ctx* c, uint32_t scattered, double* atomics_per_second)
+{
+    if (!c || !atomics_per_second) return VKGV_ERROR_INVALID_ARGUMENT;
+    HIP_TRY(c, hipSetDevice(c->device));
+    HIP_TRY(c, hipDeviceSynchronize());
+    DevBuf keys;
+    int rc = dalloc(c, keys, sizeof(unsigned long long) * calib_key_count());
+    if (rc) return rc;
+    launch_clear_keys(static_cast<unsigned long long*>(keys.p), calib_key_count(), c->stream);
+    const int r = measure_atomic_rate(static_cast<unsigned long long*>(keys.p), (uint32_t)c->cus, scattered ? 1u : 0u,
+                                      c->stream, atomics_per_second);
+    const hipError_t e = hipStreamSynchronize(c->stream);
+    dfree(keys);
+    if (r != 0 || e != hipSuccess) return fail(c, VKGV_ERROR_HIP, "measure_atomic_rate: a timed launch failed");
+    return VKGV_OK;
+}
+
 int vkgv_set_tuning(vkgv_ctx* c, const vkgv_tuning* t)
 {
     if (!c || !t) return VKGV_ERROR_INVALID_ARGUMENT;
@@ -577,9 +594,14 @@ int vkgv_sync(vkgv_ctx* c)
     return VKGV_OK;
 }
 
-int vkgv_upload_scene(vkgv_ctx* c, const vkgv_mesh_desc* meshes, uint32_t mesh_count,
-                      const vkgv_primitive_desc* prims, uint32_t prim_count, const vkgv_material* materials,
-                      uint32_t material_count)
+} // extern "C"
+
+// The scene upload of vkgv_upload_scene (mesh arrays in host memory, checked
+// on the host) and vkgv_upload_scene_device (mesh arrays in device memory,
+// copied device to device and checked by a kernel on the packed copies).
+static int upload_scene_impl(vkgv_ctx* c, const vkgv_mesh_desc* meshes, uint32_t mesh_count,
+                             const vkgv_primitive_desc* prims, uint32_t prim_count, const vkgv_material* materials,
+                             uint32_t material_count, bool device_src)
 {
     if (!c) return VKGV_ERROR_INVALID_ARGUMENT;
     if ((mesh_count && !meshes) || (prim_count && !prims) || !materials || material_count == 0)
@@ -592,6 +614,7 @@ int vkgv_upload_scene(vkgv_ctx* c, const vkgv_mesh_desc* meshes, uint32_t mesh_c
         if ((m.meshlet_count && !m.meshlets) || (m.vertex_index_count && !m.vertex_indices)
             || (m.triangle_index_bytes && !m.triangle_indices) || (m.vertex_count && !m.vertices))
             return fail(c, VKGV_ERROR_INVALID_ARGUMENT, "upload_scene: mesh " + std::to_string(mi) + " has a null array");
+        if (device_src) continue;         // device arrays: checked on the device after the copy
         for (uint64_t k = 0; k < m.vertex_index_count; ++k)
             if (m.vertex_indices[k] >= m.vertex_count)
                 return fail(c, VKGV_ERROR_INVALID_ARGUMENT, "upload_scene: vertex index out of range in mesh " + std::to_string(mi));
@@ -667,19 +690,77 @@ int vkgv_upload_scene(vkgv_ctx* c, const vkgv_mesh_desc* meshes, uint32_t mesh_c
     std::vector<vkgv_meshlet_cone> all_cones;
     int rc;
     for (int k = 0; k < 4; ++k) {
+        if (device_src) {
+            // device to device, each mesh's part at its packed offset
+            if ((rc = dalloc(c, c->mesh_bufs[k], total[k]))) return rc;
+            for (uint32_t mi = 0; mi < mesh_count; ++mi)
+                if (part_bytes(meshes[mi], k))
+                    HIP_TRY(c, hipMemcpyAsync(static_cast<uint8_t*>(c->mesh_bufs[k].p) + off[4 * mi + k],
+                                              part_src(meshes[mi], k), part_bytes(meshes[mi], k), hipMemcpyDefault,
+                                              c->stream));
+            continue;
+        }
         std::vector<uint8_t> staging(total[k] ? total[k] : 1, 0);
         for (uint32_t mi = 0; mi < mesh_count; ++mi)
             if (part_bytes(meshes[mi], k)) std::memcpy(staging.data() + off[4 * mi + k], part_src(meshes[mi], k), part_bytes(meshes[mi], k));
         if ((rc = upload(c, c->mesh_bufs[k], staging.data(), total[k]))) return rc;
         HIP_TRY(c, hipStreamSynchronize(c->stream));   // staging is freed at the end of this iteration
     }
+    uint64_t cone_total = 0;
     for (uint32_t mi = 0; mi < mesh_count; ++mi) {
         const vkgv_mesh_desc& m = meshes[mi];
         c->mesh_meshlets[mi] = m.meshlet_count;
         if (m.cones) {
-            cone_first[mi] = (uint32_t)all_cones.size();
-            all_cones.insert(all_cones.end(), m.cones, m.cones + m.meshlet_count);
+            cone_first[mi] = (uint32_t)cone_total;
+            cone_total += m.meshlet_count;
+            if (!device_src) all_cones.insert(all_cones.end(), m.cones, m.cones + m.meshlet_count);
         }
+    }
+    if (cone_total >= 0xFFFFFFFFull) return fail(c, VKGV_ERROR_LIMIT, "upload_scene: more than 2^32 - 1 cones");
+    if (device_src) {
+        // every index a kernel follows, checked on the packed device copies
+        std::vector<MeshCheck> chk(std::max<uint32_t>(mesh_count, 1u));
+        uint64_t meshlets = 0;
+        for (uint32_t mi = 0; mi < mesh_count; ++mi) {
+            const vkgv_mesh_desc& m = meshes[mi];
+            MeshCheck& q = chk[mi];
+            q.vidx_off = off[4 * mi + 0] / 4u;
+            q.vidx_count = m.vertex_index_count;
+            q.tri_off = off[4 * mi + 1];
+            q.tri_bytes = m.triangle_index_bytes;
+            q.vtx_count = m.vertex_count;
+            q.ml_off = off[4 * mi + 3];
+            q.ml_count = m.meshlet_count;
+            q.ml_first = meshlets;
+            meshlets += m.meshlet_count;
+        }
+        DevBuf dchk, dbad;
+        if ((rc = upload(c, dchk, chk.data(), sizeof(MeshCheck) * chk.size()))) return rc;
+        if ((rc = dalloc(c, dbad, sizeof(unsigned long long)))) { dfree(dchk); return rc; }
+        const unsigned long long none = ~0ull;
+        unsigned long long bad = none;
+        hipError_t e = hipMemcpyAsync(dbad.p, &none, sizeof none, hipMemcpyHostToDevice, c->stream);
+        if (e == hipSuccess)
+            e = launch_check_meshes(static_cast<const MeshCheck*>(dchk.p), mesh_count, meshlets,
+                                    static_cast<const uint32_t*>(c->mesh_bufs[0].p),
+                                    static_cast<const uint8_t*>(c->mesh_bufs[1].p),
+                                    static_cast<const uint8_t*>(c->mesh_bufs[3].p),
+                                    static_cast<unsigned long long*>(dbad.p), c->stream);
+        if (e == hipSuccess) e = hipMemcpyAsync(&bad, dbad.p, sizeof bad, hipMemcpyDeviceToHost, c->stream);
+        if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+        dfree(dchk);
+        dfree(dbad);
+        if (e != hipSuccess) return fail(c, VKGV_ERROR_HIP, std::string("upload_scene_device: ") + hipGetErrorString(e));
+        if (bad != none)
+            return fail(c, VKGV_ERROR_INVALID_ARGUMENT,
+                        "upload_scene_device: meshlet " + std::to_string(bad & 0xFFFFFFFFull) + " of mesh "
+                            + std::to_string(bad >> 32) + " is malformed or indexes out of range");
+        if ((rc = dalloc(c, c->cones, sizeof(vkgv_meshlet_cone) * std::max<uint64_t>(cone_total, 1)))) return rc;
+        for (uint32_t mi = 0; mi < mesh_count; ++mi)
+            if (meshes[mi].cones && meshes[mi].meshlet_count)
+                HIP_TRY(c, hipMemcpyAsync(static_cast<vkgv_meshlet_cone*>(c->cones.p) + cone_first[mi], meshes[mi].cones,
+                                          sizeof(vkgv_meshlet_cone) * meshes[mi].meshlet_count, hipMemcpyDefault,
+                                          c->stream));
     }
     std::vector<vkgv_primitive> dp(prim_count);
     std::vector<uint32_t> cb(std::max<uint32_t>(prim_count, 1u), 0xFFFFFFFFu);
@@ -705,7 +786,8 @@ int vkgv_upload_scene(vkgv_ctx* c, const vkgv_mesh_desc* meshes, uint32_t mesh_c
     c->host_materials.assign(materials, materials + material_count);
     if ((rc = rebuild_mat_tex(c))) return rc;
     if ((rc = upload(c, c->cone_base, cb.data(), sizeof(uint32_t) * cb.size()))) return rc;
-    if ((rc = upload(c, c->cones, all_cones.data(), sizeof(vkgv_meshlet_cone) * all_cones.size()))) return rc;
+    if (!device_src && (rc = upload(c, c->cones, all_cones.data(), sizeof(vkgv_meshlet_cone) * all_cones.size())))
+        return rc;
     HIP_TRY(c, hipStreamSynchronize(c->stream));
     c->prim_meshlets.assign(prim_count, 0);
     for (uint32_t p = 0; p < prim_count; ++p) c->prim_meshlets[p] = meshes[prims[p].mesh_index].meshlet_count;
@@ -714,6 +796,22 @@ int vkgv_upload_scene(vkgv_ctx* c, const vkgv_mesh_desc* meshes, uint32_t mesh_c
     // a new scene invalidates the draw list (world.cpp:181-182), cleared above
     c->scene = true;
     return VKGV_OK;
+}
+
+extern "C" {
+
+int vkgv_upload_scene(vkgv_ctx* c, const vkgv_mesh_desc* meshes, uint32_t mesh_count,
+                      const vkgv_primitive_desc* prims, uint32_t prim_count, const vkgv_material* materials,
+                      uint32_t material_count)
+{
+    return upload_scene_impl(c, meshes, mesh_count, prims, prim_count, materials, material_count, false);
+}
+
+int vkgv_upload_scene_device(vkgv_ctx* c, const vkgv_mesh_desc* meshes, uint32_t mesh_count,
+                             const vkgv_primitive_desc* prims, uint32_t prim_count, const vkgv_material* materials,
+                             uint32_t material_count)
+{
+    return upload_scene_impl(c, meshes, mesh_count, prims, prim_count, materials, material_count, true);
 }
 
 int vkgv_upload_textures(vkgv_ctx* c, const vkgv_texture_desc* textures, uint32_t count)
@@ -800,9 +898,10 @@ int vkgv_set_draws(vkgv_ctx* c, const vkgv_meshlet_draw* draws, uint32_t count)
 namespace {
 
 // Capacity for `count` transforms in the current and previous buffers; when
-// the count stays, the current transforms become the previous ones (stream
-// ordered on the context's stream).  Reallocation invalidates captured frames.
-int reserve_transforms(vkgv_ctx* c, uint32_t count)
+// the count stays, the current transforms become the previous ones (ordered
+// on `st`).  Reallocation synchronizes the device and invalidates captured
+// frames.
+int reserve_transforms(vkgv_ctx* c, uint32_t count, hipStream_t st)
 {
     const size_t bytes = sizeof(float) * 16 * (size_t)count;
     if (c->transforms.bytes < bytes || !c->transforms.p || c->prev_transforms.bytes < bytes || !c->prev_transforms.p) {
@@ -817,7 +916,7 @@ int reserve_transforms(vkgv_ctx* c, uint32_t count)
         if ((rc = dalloc(c, c->prev_transforms, std::max(bytes, c->transforms.bytes)))) return rc;
     }
     if (count == c->transform_count && count == c->prev_count && bytes)
-        HIP_TRY(c, hipMemcpyAsync(c->prev_transforms.p, c->transforms.p, bytes, hipMemcpyDeviceToDevice, c->stream));
+        HIP_TRY(c, hipMemcpyAsync(c->prev_transforms.p, c->transforms.p, bytes, hipMemcpyDeviceToDevice, st));
     return VKGV_OK;
 }
 
@@ -841,13 +940,26 @@ int vkgv_set_transforms(vkgv_ctx* c, const float* mat4s, uint32_t count)
     if (!c) return VKGV_ERROR_INVALID_ARGUMENT;
     if (count && !mat4s) return fail(c, VKGV_ERROR_INVALID_ARGUMENT, "set_transforms: null array");
     HIP_TRY(c, hipSetDevice(c->device));
-    int rc = reserve_transforms(c, count);
+    int rc = reserve_transforms(c, count, c->stream);
     if (rc) return rc;
     const size_t bytes = sizeof(float) * 16 * (size_t)count;
     if (bytes) HIP_TRY(c, hipMemcpyAsync(c->transforms.p, mat4s, bytes, hipMemcpyHostToDevice, c->stream));
     if ((rc = finish_transforms(c, count, c->stream))) return rc;
     HIP_TRY(c, hipStreamSynchronize(c->stream));
     return VKGV_OK;
+}
+
+int vkgv_set_transforms_device(vkgv_ctx* c, const float* mat4s, uint32_t count, void* stream)
+{
+    if (!c) return VKGV_ERROR_INVALID_ARGUMENT;
+    if (count && !mat4s) return fail(c, VKGV_ERROR_INVALID_ARGUMENT, "set_transforms_device: null array");
+    HIP_TRY(c, hipSetDevice(c->device));
+    hipStream_t st = stream ? static_cast<hipStream_t>(stream) : c->stream;
+    int rc = reserve_transforms(c, count, st);
+    if (rc) return rc;
+    const size_t bytes = sizeof(float) * 16 * (size_t)count;
+    if (bytes) HIP_TRY(c, hipMemcpyAsync(c->transforms.p, mat4s, bytes, hipMemcpyDefault, st));
+    return finish_transforms(c, count, st);
 }
 
 int vkgv_set_hierarchy(vkgv_ctx* c, const vkgv_node* nodes, uint32_t node_count, const uint32_t* children,
@@ -953,7 +1065,7 @@ int vkgv_set_hierarchy(vkgv_ctx* c, const vkgv_node* nodes, uint32_t node_count,
     d.chain_first = static_cast<const uint32_t*>(c->an_chain_first.p);
     d.chain_nodes = static_cast<const uint32_t*>(c->an_chain.p);
     // the transform buffer holds the mesh nodes' matrices at time 0 (no previous frame yet)
-    if ((rc = reserve_transforms(c, mesh_count))) return rc;
+    if ((rc = reserve_transforms(c, mesh_count, c->stream))) return rc;
     c->prev_count = 0;
     launch_update_transforms(d, 0.0f, static_cast<float*>(c->transforms.p), c->stream);
     HIP_TRY(c, hipGetLastError());

@@ -6,9 +6,10 @@
 // meshlet AABBs to world space once, then for every view of the group runs
 // the frustum test, the optional normal-cone test and the Hi-Z probe, and
 // appends the unit's visible draws to the view's list with one atomicAdd per
-// (unit, view) (ballot + mbcnt prefix).  The list order across units is
-// unspecified; the visible SET is what the reference defines (its
-// per-task-workgroup payload is re-packed by the mesh-shader launch anyway).
+// (unit, view) (ballot + mbcnt prefix), issued for all the group's views
+// before any is waited on.  The list order across units is unspecified; the
+// visible SET is what the reference defines (its per-task-workgroup payload is
+// re-packed by the mesh-shader launch anyway).
 #include <algorithm>
 
 #include "internal.hpp"
@@ -36,13 +37,24 @@ __device__ __forceinline__ bool aabb_in_frustum(const float c[3], const float e[
     return true;
 }
 
-// projectAabb (culling.h.glsl:44-56) + the probe (visbuffer.task.glsl:56-64).
-__device__ __forceinline__ bool occlusion_visible(const float c[3], const float e[3], const float* vpm,
-                                                  const float* __restrict__ pyr, const PyramidDesc& pd)
+// A Hi-Z probe, ready to fetch: the 2x2 footprint of the AABB's screen
+// centre in the NEAREST mip of its extent, and the AABB's nearest depth.
+struct ProbeAt {
+    uint32_t off;       // first texel of the level
+    int lw;             // level width
+    int x0, x1, y0, y1; // footprint texels
+    float maxz;
+};
+
+// projectAabb (culling.h.glsl:44-56), the probe's level and coordinates
+// (visbuffer.task.glsl:56-59), and textureLod's mip selection through the MIN
+// sampler (application.cpp:437-458): lod clamped to [0, 16] (NaN -> 0),
+// NEAREST mip ceil(lod + 0.5) - 1, clamped to the last level.  The level
+// sizes and offsets are read from lanes: lane k holds level k's (pw, ph, po);
+// every lane of the wave calls this.
+__device__ __forceinline__ ProbeAt probe_setup(const float c[3], const float e[3], const Mat4& vp, float w0, float h0,
+                                               int pw, int ph, uint32_t po, uint32_t last)
 {
-    Mat4 vp;
-#pragma unroll
-    for (int i = 0; i < 16; ++i) vp.m[i] = vpm[i];
     float mnx = 1.0f, mny = 1.0f, mnz = 1.0f, mxx = -1.0f, mxy = -1.0f, mxz = -1.0f;
 #pragma unroll
     for (int i = 0; i < 8; ++i) {
@@ -57,11 +69,34 @@ __device__ __forceinline__ bool occlusion_visible(const float c[3], const float 
         mxx = gmax(mxx, u); mxy = gmax(mxy, v); mxz = gmax(mxz, z);
     }
     (void)mnz;
-    const float width = (mxx - mnx) * (float)pd.w[0];
-    const float height = (mxy - mny) * (float)pd.h[0];
+    const float width = (mxx - mnx) * w0;
+    const float height = (mxy - mny) * h0;
     const float level = floor_log2(gmax(width, height));
-    const float depth = pyramid_probe(pyr, pd, (mnx + mxx) * 0.5f, (mny + mxy) * 0.5f, level);
-    return depth < mxz;
+    const float u = (mnx + mxx) * 0.5f, v = (mny + mxy) * 0.5f;
+    float lam = level;
+    if (!(lam >= 0.0f)) lam = 0.0f;
+    if (lam > 16.0f) lam = 16.0f;
+    uint32_t d = (uint32_t)(ceilf(lam + 0.5f) - 1.0f);
+    if (d > last) d = last;
+    ProbeAt p;
+    p.lw = __shfl(pw, (int)d);
+    const int lh = __shfl(ph, (int)d);
+    p.off = (uint32_t)__shfl((int)po, (int)d);
+    axis_footprint(u * (float)p.lw - 0.5f, p.lw, p.x0, p.x1);
+    axis_footprint(v * (float)lh - 0.5f, lh, p.y0, p.y1);
+    p.maxz = mxz;
+    return p;
+}
+
+// The probe's depth: the minimum of its 2x2 footprint.
+__device__ __forceinline__ float probe_fetch(const float* __restrict__ pyr, const ProbeAt& p)
+{
+    const float* tex = pyr + p.off;
+    float r = tex[p.y0 * p.lw + p.x0];
+    r = gmin(r, tex[p.y0 * p.lw + p.x1]);
+    r = gmin(r, tex[p.y1 * p.lw + p.x0]);
+    r = gmin(r, tex[p.y1 * p.lw + p.x1]);
+    return r;
 }
 
 __device__ __forceinline__ float3 mat4_dir(const Mat4& m, float x, float y, float z)
@@ -141,7 +176,11 @@ __device__ __forceinline__ void draw_cull_setup(const SceneDev& s, uint32_t d, D
 {
     dc.valid = d < s.draw_count;
     dc.cone = false;
-    if (!dc.valid) return;
+    if (!dc.valid) {   // finite inputs for the probe every lane computes
+        dc.wc[0] = dc.wc[1] = dc.wc[2] = 0.0f;
+        dc.we[0] = dc.we[1] = dc.we[2] = 0.0f;
+        return;
+    }
     const float4* rp = reinterpret_cast<const float4*>(s.cull + d);
     const float4 r0 = rp[0], r1 = rp[1];
     const uint32_t tidx = __float_as_uint(r1.z);
@@ -161,80 +200,132 @@ __device__ __forceinline__ void draw_cull_setup(const SceneDev& s, uint32_t d, D
 
 // Persistent cull + compaction.  A work unit is 64 * P consecutive draws
 // (lane l takes draws base + 64 i + l, i < P, so each of the P record loads
-// of a wave is one contiguous 2 KiB span) and a group of views.  Waves stride
-// over the units.  Per view, the P ballots of the unit give its visible count,
-// reserved in the view's list with one atomicAdd per (unit, view), and each
-// visible lane's position (mbcnt); the lists come out in draw order within a
-// unit.  In the late pass the same ballots write the history words.
+// of a wave is one contiguous 2 KiB span) and a group of at most 64 / P
+// views.  Waves stride over the units.
+//  - Per view, the P draws' tests run together: the occlusion probes' texel
+//    loads are issued before any is compared, and the pyramid's level sizes
+//    and offsets come from lanes (lane k holds level k & 15) by a cross-lane
+//    read instead of a per-lane table load.
+//  - Phase 1 records each lane's visibility per (view, draw) in a bitset and
+//    reserves the unit's span of each view's list with one atomicAdd, issued
+//    by lane j for view j and not waited on until phase 2, which writes the
+//    raster records (draw order within the unit).
+//  - The late pass writes the history words from the same ballots.
 template <int P, bool CONE>
 __global__ __launch_bounds__(kBlock) void cull_kernel(CullArgs a)
 {
+    static_assert(64 % P == 0, "a unit's (view, draw) bits fill at most one 64-bit word");
     const uint32_t lane = threadIdx.x & 63u;
     const uint32_t nwaves = (gridDim.x * kBlock) >> 6;
+    const bool single = a.mode == CULL_SINGLE;
+    const bool probe = a.mode == CULL_LATE || (single && (a.flags & VKGV_CULL_OCCLUSION_PREV));
     for (uint32_t u = (blockIdx.x * kBlock + threadIdx.x) >> 6; u < a.units; u += nwaves) {
         const uint32_t chunk = u / a.groups, group = u - chunk * a.groups;
         const uint32_t base = chunk * 64u * (uint32_t)P;
         DrawCull dc[P];
 #pragma unroll
         for (int i = 0; i < P; ++i) draw_cull_setup<CONE>(a.s, base + 64u * i + lane, dc[i]);
-        const uint32_t v0 = group * a.group_views, v1 = min(v0 + a.group_views, a.b.nviews);
-        for (uint32_t vi = v0; vi < v1; ++vi) {
-            const ViewDev& view = batch_view(a.b, vi);
-            const CameraArg& cam = batch_camera(a.b, vi);
+        const uint32_t v0 = group * a.group_views, nv = min(a.group_views, a.b.nviews - v0);
+        unsigned long long bits = 0ull;   // bit j * P + i: draw i of this lane visible in view v0 + j
+        uint32_t reserved = 0;            // lane j: first list position of the unit in view v0 + j
+        // the unit's history words of every view of the group in one load: lane
+        // j * P + i holds view v0 + j's word of draws base + 64 i .. + 63 (read
+        // before the late pass below rewrites them)
+        unsigned long long hword = 0ull;
+        if (!single) {
+            const uint32_t jl = lane / (uint32_t)P, il = lane % (uint32_t)P;
+            if (jl < nv && base + 64u * il < a.s.draw_count)
+                hword = reinterpret_cast<const unsigned long long*>(batch_view(a.b, v0 + jl).history)[(base >> 6) + il];
+        }
+        // ---- phase 1: visibility of every (view, draw), one reservation per view
+        for (uint32_t j = 0; j < nv; ++j) {
+            const ViewDev& view = batch_view(a.b, v0 + j);
+            const CameraArg& cam = batch_camera(a.b, v0 + j);
             unsigned long long* history = reinterpret_cast<unsigned long long*>(view.history);
-            unsigned long long mask[P];
+            bool pass[P], vis[P];
+#pragma unroll
+            for (int i = 0; i < P; ++i) {
+                const uint32_t src = j * (uint32_t)P + (uint32_t)i;
+                const unsigned long long hw =
+                    ((unsigned long long)__builtin_amdgcn_readlane((uint32_t)(hword >> 32), src) << 32)
+                    | __builtin_amdgcn_readlane((uint32_t)hword, src);
+                const bool old = dc[i].valid && ((hw >> lane) & 1ull);
+                // the early pass tests only what the history lists
+                pass[i] = dc[i].valid && (a.mode != CULL_EARLY || old);
+                if (pass[i] && (a.flags & VKGV_CULL_FRUSTUM)) pass[i] = aabb_in_frustum(dc[i].wc, dc[i].we, cam.cam);
+                if (CONE && pass[i] && dc[i].cone)
+                    pass[i] = !cone_culled_from(dc[i].apex, dc[i].axis, dc[i].cutoff, cam.eye);
+                vis[i] = single ? pass[i] : (a.mode == CULL_EARLY ? pass[i] : old);   // late: old = drawn early
+            }
+            if (probe) {
+                // only the (i) with a lane past the frustum test probe (a uniform branch)
+                bool anyp[P], any = false;
+#pragma unroll
+                for (int i = 0; i < P; ++i) {
+                    anyp[i] = __ballot(pass[i]) != 0ull;
+                    any = any || anyp[i];
+                }
+                ProbeAt pr[P] = {};
+                float depth[P] = {};
+                if (any) {
+                    // pyramid level geometry in lanes: lane k holds level k & 15
+                    const uint32_t lk = lane & 15u;
+                    const int pw = (int)view.pyr.w[lk], ph = (int)view.pyr.h[lk];
+                    const uint32_t po = view.pyr.off[lk];
+                    const uint32_t last = view.pyr.levels - 1u;
+                    const float w0 = (float)view.pyr.w[0], h0 = (float)view.pyr.h[0];
+                    const float* vpm = single ? cam.cam.prev_occlusion_view_projection : cam.cam.occlusion_view_projection;
+                    Mat4 vp;
+#pragma unroll
+                    for (int q = 0; q < 16; ++q) vp.m[q] = vpm[q];
+#pragma unroll
+                    for (int i = 0; i < P; ++i)
+                        if (anyp[i]) pr[i] = probe_setup(dc[i].wc, dc[i].we, vp, w0, h0, pw, ph, po, last);
+#pragma unroll
+                    for (int i = 0; i < P; ++i)
+                        if (anyp[i]) depth[i] = probe_fetch(view.pyramid, pr[i]);
+                }
+#pragma unroll
+                for (int i = 0; i < P; ++i) {
+                    const bool occ = anyp[i] && pass[i] && depth[i] < pr[i].maxz;
+                    if (single) {
+                        vis[i] = occ;
+                    } else {
+                        const bool old = vis[i];
+                        vis[i] = occ && !(pass[i] && old);
+                        // the history of the next frame: visible against this frame's pyramid
+                        const unsigned long long hb = __ballot(occ);
+                        if (lane == 0 && base + 64u * i < a.s.draw_count) history[(base >> 6) + i] = hb;
+                    }
+                }
+            }
             uint32_t total = 0;
 #pragma unroll
             for (int i = 0; i < P; ++i) {
-                const uint32_t d = base + 64u * i + lane;
-                bool vis = false, newbit = false;
-                if (dc[i].valid) {
-                    bool pass = true;
-                    if (a.flags & VKGV_CULL_FRUSTUM) pass = aabb_in_frustum(dc[i].wc, dc[i].we, cam.cam);
-                    if (CONE && pass && dc[i].cone)
-                        pass = !cone_culled_from(dc[i].apex, dc[i].axis, dc[i].cutoff, cam.eye);
-                    if (a.mode == CULL_SINGLE) {
-                        vis = pass;
-                        if (vis && (a.flags & VKGV_CULL_OCCLUSION_PREV))
-                            vis = occlusion_visible(dc[i].wc, dc[i].we, cam.cam.prev_occlusion_view_projection,
-                                                    view.pyramid, view.pyr);
-                    } else {
-                        const bool old = (history[d >> 6] >> (d & 63u)) & 1ull;
-                        if (a.mode == CULL_EARLY) {
-                            vis = pass && old;
-                        } else {
-                            const bool occ = pass && occlusion_visible(dc[i].wc, dc[i].we,
-                                                                       cam.cam.occlusion_view_projection, view.pyramid,
-                                                                       view.pyr);
-                            newbit = occ;
-                            vis = occ && !(pass && old);
-                        }
-                    }
-                }
-                if (a.mode == CULL_LATE) {
-                    const unsigned long long bits = __ballot(newbit);
-                    if (lane == 0 && base + 64u * i < a.s.draw_count) history[(base >> 6) + i] = bits;
-                }
-                mask[i] = __ballot(vis);
-                total += (uint32_t)__popcll(mask[i]);
+                total += (uint32_t)__popcll(__ballot(vis[i]));
+                if (vis[i]) bits |= 1ull << (j * (uint32_t)P + (uint32_t)i);
             }
-            if (total == 0) continue;
-            uint32_t out = 0;
-            if (lane == 0) out = atomicAdd(&view.counters->vis_count[a.list], total);
-            out = __shfl(out, 0);
-            RasterRec* dst = view.visible[a.list];
+            if (total && lane == j) reserved = atomicAdd(&view.counters->vis_count[a.list], total);
+        }
+        if (__ballot(bits != 0ull) == 0ull) continue;
+        // ---- phase 2: the raster records of the visible draws
+        for (uint32_t j = 0; j < nv; ++j) {
+            uint32_t out = __builtin_amdgcn_readlane(reserved, j);
+            RasterRec* dst = batch_view(a.b, v0 + j).visible[a.list];
 #pragma unroll
             for (int i = 0; i < P; ++i) {
-                if ((mask[i] >> lane) & 1ull) {
+                const bool mine = (bits >> (j * (uint32_t)P + (uint32_t)i)) & 1ull;
+                const unsigned long long m = __ballot(mine);
+                if (mine) {
                     const uint32_t rank =
-                        __builtin_amdgcn_mbcnt_hi((uint32_t)(mask[i] >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)mask[i], 0u));
+                        __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
                     const uint4* src = reinterpret_cast<const uint4*>(a.s.rrec + base + 64u * i + lane);
                     uint4* o = reinterpret_cast<uint4*>(dst + out + rank);
                     const uint4 q0 = src[0], q1 = src[1];
                     o[0] = q0;
                     o[1] = q1;
                 }
-                out += (uint32_t)__popcll(mask[i]);
+                out += (uint32_t)__popcll(m);
             }
         }
     }
@@ -326,6 +417,36 @@ __global__ __launch_bounds__(kBlock) void build_cull_records_kernel(SceneDev s, 
     rrec[d] = q;
 }
 
+// One thread per meshlet of a scene uploaded from device memory: the same
+// conditions the host upload checks (counts within the limits, 4-byte
+// aligned triangle offset, ranges within the mesh's arrays, vertex indices
+// below the vertex count, triangle indices below the meshlet's vertex count),
+// read from the packed copies only after the ranges are known to hold.
+__global__ __launch_bounds__(kBlock) void check_meshes_kernel(const MeshCheck* meshes, uint32_t mesh_count,
+                                                              uint64_t meshlets, const uint32_t* vidx,
+                                                              const uint8_t* tri, const uint8_t* ml,
+                                                              unsigned long long* bad)
+{
+    const uint64_t g = (uint64_t)blockIdx.x * kBlock + threadIdx.x;
+    if (g >= meshlets) return;
+    uint32_t lo = 0, hi = mesh_count - 1;   // the last mesh whose first meshlet is <= g
+    while (lo < hi) {
+        const uint32_t mid = (lo + hi + 1u) >> 1;
+        if (meshes[mid].ml_first <= g) lo = mid; else hi = mid - 1u;
+    }
+    const MeshCheck m = meshes[lo];
+    const uint64_t k = g - m.ml_first;
+    const vkgv_meshlet* p = reinterpret_cast<const vkgv_meshlet*>(ml + m.ml_off) + k;
+    const uint32_t voff = p->vertex_offset, toff = p->triangle_offset;
+    const uint32_t vc = p->vertex_count, tc = p->triangle_count;
+    const uint64_t tbytes = ((uint64_t)tc * 3u + 3u) & ~3ull;
+    bool ok = vc >= 1u && vc <= VKGV_MAX_VERTICES && tc >= 1u && tc <= VKGV_MAX_PRIMITIVES && (toff & 3u) == 0u
+              && (uint64_t)voff + vc <= m.vidx_count && (uint64_t)toff + tbytes <= m.tri_bytes;
+    for (uint32_t v = 0; ok && v < vc; ++v) ok = vidx[m.vidx_off + voff + v] < m.vtx_count;
+    for (uint32_t t = 0; ok && t < 3u * tc; ++t) ok = tri[m.tri_off + toff + t] < vc;
+    if (!ok) atomicMin(bad, ((unsigned long long)lo << 32) | k);
+}
+
 __global__ __launch_bounds__(kBlock) void clear_keys_kernel(ulonglong2* keys, uint64_t n2, unsigned long long* tail, uint32_t odd)
 {
     const ulonglong2 v = make_ulonglong2(VKGV_KEY_CLEAR, VKGV_KEY_CLEAR);
@@ -345,6 +466,16 @@ void launch_build_cull_records(const SceneDev& s, CullRec* out, RasterRec* rrec,
 {
     if (!s.draw_count) return;
     build_cull_records_kernel<<<(s.draw_count + kBlock - 1) / kBlock, kBlock, 0, st>>>(s, out, rrec);
+}
+
+hipError_t launch_check_meshes(const MeshCheck* meshes, uint32_t mesh_count, uint64_t meshlets, const uint32_t* vidx,
+                               const uint8_t* tri, const uint8_t* ml, unsigned long long* bad, hipStream_t st)
+{
+    if (!meshlets || !mesh_count) return hipSuccess;
+    const uint64_t blocks = (meshlets + kBlock - 1) / kBlock;
+    if (blocks > 0x7FFFFFFFull) return hipErrorInvalidValue;
+    check_meshes_kernel<<<(uint32_t)blocks, kBlock, 0, st>>>(meshes, mesh_count, meshlets, vidx, tri, ml, bad);
+    return hipGetLastError();
 }
 
 void launch_clear_keys(unsigned long long* keys, uint64_t n, hipStream_t st)
@@ -368,6 +499,26 @@ void launch_prev_mvp(const SceneDev& s, const BatchDev& b, const BatchGrid& g, h
     prev_mvp_kernel<<<dim3((s.transform_count + kBlock - 1) / kBlock, g.nviews), kBlock, 0, st>>>(s, b);
 }
 
+namespace {
+
+// Persistent grid: as many blocks as stay resident (the kernel's occupancy,
+// queried once per instantiation), each wave striding over the units; fewer
+// when there are fewer units.
+template <int P, bool CONE>
+void launch_cull_kernel(const CullArgs& a, uint32_t cus, hipStream_t st)
+{
+    static const uint32_t per_cu = [] {
+        int n = 0;
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, cull_kernel<P, CONE>, kBlock, 0) != hipSuccess || n < 1)
+            n = 1;
+        return (uint32_t)n;
+    }();
+    const uint32_t blocks = std::min<uint32_t>((a.units + 3u) / 4u, cus * per_cu);
+    cull_kernel<P, CONE><<<blocks, kBlock, 0, st>>>(a);
+}
+
+} // namespace
+
 void launch_cull(const SceneDev& s, const BatchDev& b, const BatchGrid& g, uint32_t flags, CullMode mode,
                  uint32_t list, uint32_t cus, hipStream_t st)
 {
@@ -387,18 +538,17 @@ void launch_cull(const SceneDev& s, const BatchDev& b, const BatchGrid& g, uint3
     const uint32_t chunks = (s.draw_count + 64u * per_lane - 1) / (64u * per_lane);
     const uint32_t target = cus * 32u;
     uint32_t groups = std::min<uint32_t>(g.nviews, std::max<uint32_t>(1u, (target + chunks - 1) / chunks));
-    a.group_views = (g.nviews + groups - 1) / groups;
+    // a lane keeps one visibility bit per (view, draw) of its unit: at most 64
+    a.group_views = std::min<uint32_t>((g.nviews + groups - 1) / groups, 64u / per_lane);
     a.groups = (g.nviews + a.group_views - 1) / a.group_views;
     a.units = chunks * a.groups;
-    // persistent grid: at most 16 waves per CU, each striding over the units
-    const uint32_t blocks = std::min<uint32_t>((a.units + 3u) / 4u, cus * 4u);
     const bool cone = flags & VKGV_CULL_CONE;
     if (per_lane == 4) {
-        if (cone) cull_kernel<4, true><<<blocks, kBlock, 0, st>>>(a);
-        else cull_kernel<4, false><<<blocks, kBlock, 0, st>>>(a);
+        if (cone) launch_cull_kernel<4, true>(a, cus, st);
+        else launch_cull_kernel<4, false>(a, cus, st);
     } else {
-        if (cone) cull_kernel<1, true><<<blocks, kBlock, 0, st>>>(a);
-        else cull_kernel<1, false><<<blocks, kBlock, 0, st>>>(a);
+        if (cone) launch_cull_kernel<1, true>(a, cus, st);
+        else launch_cull_kernel<1, false>(a, cus, st);
     }
 }
 

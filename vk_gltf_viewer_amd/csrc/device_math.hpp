@@ -116,27 +116,4 @@ __device__ __forceinline__ float floor_log2(float x)
     return (float)(e - 127);
 }
 
-// textureLod through the MIN sampler (application.cpp:437-458): lod clamped
-// to [0, 16] (NaN -> 0), NEAREST mip ceil(lod + 0.5) - 1, clamped to the last
-// level, 2x2 linear footprint there.
-__device__ __forceinline__ float pyramid_probe(const float* __restrict__ pyr, const PyramidDesc& pd, float u, float v,
-                                               float level)
-{
-    float lam = level;
-    if (!(lam >= 0.0f)) lam = 0.0f;
-    if (lam > 16.0f) lam = 16.0f;
-    uint32_t d = (uint32_t)(ceilf(lam + 0.5f) - 1.0f);
-    if (d > pd.levels - 1) d = pd.levels - 1;
-    const int lw = (int)pd.w[d], lh = (int)pd.h[d];
-    const float* tex = pyr + pd.off[d];
-    int x0, x1, y0, y1;
-    axis_footprint(u * (float)lw - 0.5f, lw, x0, x1);
-    axis_footprint(v * (float)lh - 0.5f, lh, y0, y1);
-    float r = tex[y0 * lw + x0];
-    r = gmin(r, tex[y0 * lw + x1]);
-    r = gmin(r, tex[y1 * lw + x0]);
-    r = gmin(r, tex[y1 * lw + x1]);
-    return r;
-}
-
 } // namespace vkgv

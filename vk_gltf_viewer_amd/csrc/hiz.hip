@@ -89,6 +89,8 @@ __global__ __launch_bounds__(256) void hiz_tiles_kernel(BatchDev b)
 // instead of global memory (one global round trip for the whole tail when
 // its first level fits).
 constexpr uint32_t kTailA = kHizTailTexels, kTailB = kHizTailTexels / 2;   // floats of the two LDS level buffers
+constexpr uint32_t kTailPer = kTailA / 1024u;                             // texels per thread of a tail level
+static_assert(kTailA % 1024u == 0, "the tail's levels are whole rounds of its 1024 threads");
 
 __global__ __launch_bounds__(1024) void hiz_tail_kernel(BatchDev b)
 {
@@ -105,17 +107,25 @@ __global__ __launch_bounds__(1024) void hiz_tail_kernel(BatchDev b)
         float* dst = pyr + pd.off[k];
         float* ldst = nullptr;
         if (dw * dh <= (next_a ? kTailA : kTailB)) ldst = next_a ? s_a : s_b;
-        for (uint32_t t = threadIdx.x; t < dw * dh; t += 1024u) {
-            const uint32_t x = t % dw, y = t / dw;
-            int x0, x1, y0, y1;
-            footprint(x, dw, sw, x0, x1);
-            footprint(y, dh, sh, y0, y1);
-            float d = src[y0 * sw + x0];
-            d = gmin(d, src[y0 * sw + x1]);
-            d = gmin(d, src[y1 * sw + x0]);
-            d = gmin(d, src[y1 * sw + x1]);
-            dst[t] = d;
-            if (ldst) ldst[t] = d;
+        // a level holds at most kTailA texels: kTailPer per thread, all of a
+        // thread's loads issued before the first minimum (the first level is
+        // read from global memory, where one round trip each would add up)
+        const uint32_t n = dw * dh;
+#pragma unroll
+        for (uint32_t r = 0; r < kTailPer; ++r) {
+            const uint32_t t = threadIdx.x + r * 1024u;
+            if (t < n) {
+                const uint32_t x = t % dw, y = t / dw;
+                int x0, x1, y0, y1;
+                footprint(x, dw, sw, x0, x1);
+                footprint(y, dh, sh, y0, y1);
+                float d = src[y0 * sw + x0];
+                d = gmin(d, src[y0 * sw + x1]);
+                d = gmin(d, src[y1 * sw + x0]);
+                d = gmin(d, src[y1 * sw + x1]);
+                dst[t] = d;
+                if (ldst) ldst[t] = d;
+            }
         }
         __syncthreads();
         lsrc = ldst;

@@ -327,6 +327,29 @@ void launch_aa_image(const SceneDev& s, const unsigned long long* keys, uint32_t
                      const float* vp, float* image, hipStream_t st);
 void launch_aa_backward(const SceneDev& s, const unsigned long long* keys, uint32_t* mat, uint32_t W, uint32_t H,
                         const float* vp, const float* g_image, double* g_transforms, hipStream_t st);
+// One mesh of a scene uploaded from device memory, as its packed copies hold
+// it: what the device-side check of its meshlets needs.
+struct MeshCheck {
+    uint64_t vidx_off;              // first vertex index, elements into the packed vertex-index buffer
+    uint64_t vidx_count;
+    uint64_t tri_off;               // first triangle byte, into the packed triangle buffer
+    uint64_t tri_bytes;
+    uint64_t vtx_count;
+    uint64_t ml_off;                // first meshlet, bytes into the packed meshlet buffer
+    uint64_t ml_first;              // meshlets of the meshes before this one
+    uint32_t ml_count;
+    uint32_t pad;
+};
+// Checks every meshlet of the packed scene (counts, ranges, vertex indices,
+// triangle indices): *bad receives the smallest (mesh << 32 | meshlet) of a
+// malformed one, and keeps its value (~0) when all are well formed.
+hipError_t launch_check_meshes(const MeshCheck* meshes, uint32_t mesh_count, uint64_t meshlets, const uint32_t* vidx,
+                               const uint8_t* tri, const uint8_t* ml, unsigned long long* bad, hipStream_t st);
+
+// Calibration (calib.hip): 64-bit atomicMin throughput on a 3840 x 2160 key
+// buffer of calib_key_count() keys, coalesced or scattered; 0 on success.
+int measure_atomic_rate(unsigned long long* keys, uint32_t cus, uint32_t scattered, hipStream_t st, double* rate);
+uint64_t calib_key_count();
 uint32_t hiz_tile_levels(const PyramidDesc& pd);
 uint32_t hiz_tail_start(const PyramidDesc& pd, uint32_t tile_levels);
 

@@ -52,11 +52,12 @@ class DifferentiableFrame:
         t = transforms.detach()
         if t.dim() != 2 or t.shape[1] != 16:
             raise ValueError("transforms must be [T, 16]")
-        host = t.to("cpu", torch.float32).numpy()
-        # work still queued on this stream (an earlier step) may read the transforms
-        torch.cuda.current_stream().synchronize()
-        self.r.set_transforms(np.ascontiguousarray(host))
-        self._count = host.shape[0]
+        if t.is_cuda:
+            # device to device on torch's stream: no host copy, no synchronization
+            self.r.set_transforms_device(t.to(torch.float32).contiguous())
+        else:
+            self.r.set_transforms(np.ascontiguousarray(t.to(torch.float32).numpy()))
+        self._count = t.shape[0]
         self._device = transforms.device
         self._dtype = transforms.dtype
         s = torch.cuda.current_stream()

@@ -79,6 +79,56 @@ class Renderer:
         del keep
         self.upload_textures(sc.textures)
 
+    def upload_scene_device(self, sc: SceneArrays, corrupt=None):
+        """vkgv_upload_scene_device: each mesh's arrays are first placed in
+        device memory (torch tensors, standing in for a caller's own device
+        buffers) and handed over by address.  `corrupt(mesh_index, parts)` may
+        modify the device copies (tests of the device-side check)."""
+        import torch
+        arr = (N.MeshDesc * max(1, len(sc.meshes)))()
+        keep = []
+        for i, m in enumerate(sc.meshes):
+            host = [np.ascontiguousarray(m.vertex_indices, dtype=np.uint32), np.ascontiguousarray(m.triangle_indices,
+                                                                                                 dtype=np.uint8),
+                    np.ascontiguousarray(m.vertices), np.ascontiguousarray(m.meshlets)]
+            if m.cones is not None:
+                host.append(np.ascontiguousarray(m.cones))
+            dev = [torch.from_numpy(h.view(np.uint8).reshape(-1).copy()).to("cuda") for h in host]
+            if corrupt is not None:
+                corrupt(i, dev)
+            keep.extend(dev)
+            d = arr[i]
+            d.vertex_indices, d.vertex_index_count = dev[0].data_ptr(), host[0].size
+            d.triangle_indices, d.triangle_index_bytes = dev[1].data_ptr(), host[1].size
+            d.vertices, d.vertex_count = dev[2].data_ptr(), host[2].size
+            d.meshlets, d.meshlet_count = dev[3].data_ptr(), host[3].size
+            d.cones = dev[4].data_ptr() if m.cones is not None else None
+        torch.cuda.synchronize()          # the copies above ran on torch's stream
+        prims = np.ascontiguousarray(sc.primitives)
+        mats = np.ascontiguousarray(sc.materials)
+        self._check(self._lib.vkgv_upload_scene_device(
+            self._h, arr, len(sc.meshes), C.cast(prims.ctypes.data, C.POINTER(N.PrimitiveDesc)), len(prims),
+            C.cast(mats.ctypes.data, C.POINTER(N.Material)), len(mats)))
+        del keep
+        self.upload_textures(sc.textures)
+
+    def set_transforms_device(self, transforms, stream=None):
+        """vkgv_set_transforms_device from a contiguous float32 [T, 16] torch
+        device tensor, ordered on torch's current stream (fenced when that is
+        the default stream, which the library reads as its own)."""
+        import torch
+        t = transforms
+        if not (isinstance(t, torch.Tensor) and t.is_cuda and t.dtype == torch.float32 and t.is_contiguous()
+                and t.dim() == 2 and t.shape[1] == 16):
+            raise ValueError("transforms must be a contiguous float32 [T, 16] device tensor")
+        s = torch.cuda.current_stream()
+        if s.cuda_stream:
+            self._check(self._lib.vkgv_set_transforms_device(self._h, t.data_ptr(), t.shape[0], s.cuda_stream))
+            return
+        s.synchronize()
+        self._check(self._lib.vkgv_set_transforms_device(self._h, t.data_ptr(), t.shape[0], None))
+        self.sync()
+
     def upload_textures(self, textures):
         """Albedo textures (scene.Texture list) for VKGV_RESOLVE_TEXTURED."""
         descs, keep = texture_descs(textures)
@@ -133,6 +183,12 @@ class Renderer:
         t = N.Tuning()
         self._check(self._lib.vkgv_get_tuning(self._h, C.byref(t)))
         return t.small_max_extent, t.raster_blocks_per_cu, t.tile_item_tris, t.resolve_cache_slots
+
+    def measure_atomic_rate(self, scattered: bool = False) -> float:
+        """64-bit atomicMin per second on a 4K key buffer (vkgv_measure_atomic_rate)."""
+        r = C.c_double()
+        self._check(self._lib.vkgv_measure_atomic_rate(self._h, 1 if scattered else 0, C.byref(r)))
+        return r.value
 
     def device_identity(self) -> str:
         """Architecture and compute-unit count of the context's device (the
