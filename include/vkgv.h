@@ -479,6 +479,15 @@ int  vkgv_batch_read_stats(vkgv_ctx* ctx, vkgv_batch* batch, vkgv_frame_stats* s
  * pixels of different colour blended along the front triangle's crossed
  * edge (see csrc/grad.hip). */
 int vkgv_view_aa_image(vkgv_ctx* ctx, vkgv_view* view, const vkgv_camera* camera, float* image, void* stream);
+/* vkgv_view_aa_image without any device allocation: no counterpart in the
+ * reference; for callers that cannot allocate (code under graph capture,
+ * serving loops).  The same kernels and the same image, bit for bit, but the
+ * per-pixel scratch is a fixed array in the library (one per GPU, held until
+ * the process ends) instead of a buffer the view allocates on first use.
+ * Views of more than 3840 x 2160 pixels fail with VKGV_ERROR_LIMIT.  Every
+ * call on one GPU shares that array: calls that overlap on one GPU (other
+ * streams, other contexts of the device) race. */
+int vkgv_view_aa_image_noalloc(vkgv_ctx* ctx, vkgv_view* view, const vkgv_camera* camera, float* image, void* stream);
 /* g_transforms (double [transform_count][16], column-major like
  * vkgv_set_transforms) += d(sum g_image . image) / d transforms, for
  * g_image float [H][W][3].  transform_count must be at least the count last

@@ -215,7 +215,7 @@ HIP_SYMBOLS = [
     "vkgv_view_read_color", "vkgv_view_read_pyramid", "vkgv_view_write_pyramid", "vkgv_view_write_depth",
     "vkgv_view_read_visible", "vkgv_view_read_stats", "vkgv_set_tuning", "vkgv_get_tuning", "vkgv_srgb_thresholds",
     "vkgv_batch_create", "vkgv_batch_destroy", "vkgv_render_batch", "vkgv_batch_read_stats",
-    "vkgv_view_aa_image", "vkgv_view_aa_backward", "vkgv_upload_textures",
+    "vkgv_view_aa_image", "vkgv_view_aa_image_noalloc", "vkgv_view_aa_backward", "vkgv_upload_textures",
     "vkgv_set_hierarchy",
     "vkgv_update_transforms", "vkgv_read_transforms", "vkgv_view_read_motion", "vkgv_measure_atomic_rate",
     "vkgv_upload_scene_device", "vkgv_set_transforms_device",
@@ -279,6 +279,7 @@ def hip():
         _bind(lib, "vkgv_render_batch", C.c_int, _p, _p, _p, _u32)
         _bind(lib, "vkgv_batch_read_stats", C.c_int, _p, _p, C.POINTER(FrameStats))
         _bind(lib, "vkgv_view_aa_image", C.c_int, _p, _p, C.POINTER(Camera), _p, _p)
+        _bind(lib, "vkgv_view_aa_image_noalloc", C.c_int, _p, _p, C.POINTER(Camera), _p, _p)
         _bind(lib, "vkgv_view_aa_backward", C.c_int, _p, _p, C.POINTER(Camera), _p, _p, _u32, _p)
         _hip = lib
     return _hip

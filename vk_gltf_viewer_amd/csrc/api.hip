@@ -26,6 +26,16 @@ namespace vkgv {
 uint32_t hiz_tile_levels(const PyramidDesc& pd);
 }
 
+// Scratch of vkgv_view_aa_image_noalloc: the per-pixel material indices of
+// the image pass, for views of at most kNoAllocPixels pixels (4K UHD, 33 MB).
+// Each GPU the library runs on holds its own copy until the process ends.
+// The pass's first kernel writes all W*H entries it later reads, so nothing
+// needs clearing between calls.  Host code never uses the array's name as an
+// address: vkgv_create resolves the device address with hipGetSymbolAddress
+// (per context, with the context's device current).
+constexpr size_t kNoAllocPixels = 3840u * 2160u;
+static __device__ __align__(256) uint32_t scratch[kNoAllocPixels];
+
 namespace {
 
 struct DevBuf {
@@ -76,6 +86,8 @@ struct vkgv_ctx {
     // is set at vkgv_create, the context's first n device allocations succeed and
     // every later one fails with VKGV_ERROR_OUT_OF_MEMORY (-1: off).
     int64_t fault_alloc_after = -1;
+    // device address of `scratch` on this context's device
+    uint32_t* noalloc_scratch = nullptr;
 };
 
 struct vkgv_view {
@@ -492,6 +504,15 @@ int vkgv_create(const vkgv_create_info* info, vkgv_ctx** out)
     if (hipGetDeviceProperties(&prop, c->device) == hipSuccess && prop.multiProcessorCount > 0)
         c->cus = prop.multiProcessorCount;
     if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) { delete c; return VKGV_ERROR_HIP; }
+    {
+        // this device's copy of the no-alloc scratch (the device is current)
+        void* sp = nullptr;
+        if (hipGetSymbolAddress(&sp, HIP_SYMBOL(scratch)) != hipSuccess || !sp) {
+            vkgv_destroy(c);
+            return VKGV_ERROR_HIP;
+        }
+        c->noalloc_scratch = static_cast<uint32_t*>(sp);
+    }
     SrgbTables tab;
     std::memset(&tab, 0, sizeof tab);
     srgb_thresholds(tab.thr);
@@ -1357,6 +1378,24 @@ int vkgv_view_aa_image(vkgv_ctx* c, vkgv_view* v, const vkgv_camera* cam, float*
     if ((rc = ensure_aa_mat(c, v))) return rc;
     launch_aa_image(scene_dev(c), v->dev.keys, static_cast<uint32_t*>(v->aa_mat.p), v->dev.width, v->dev.height,
                     cam->view_projection, image, st);
+    HIP_TRY(c, hipGetLastError());
+    return VKGV_OK;
+}
+
+int vkgv_view_aa_image_noalloc(vkgv_ctx* c, vkgv_view* v, const vkgv_camera* cam, float* image, void* stream)
+{
+    int rc = check_grad(c, v, cam, image, image);
+    if (rc) return rc;
+    const uint64_t npx = (uint64_t)v->dev.width * v->dev.height;
+    if (npx > kNoAllocPixels)
+        return fail(c, VKGV_ERROR_LIMIT, "aa_image_noalloc: the view has " + std::to_string(npx)
+                                             + " pixels, the scratch holds at most " + std::to_string(kNoAllocPixels));
+    if (!c->noalloc_scratch) return fail(c, VKGV_ERROR_STATE, "aa_image_noalloc: no scratch address");
+    HIP_TRY(c, hipSetDevice(c->device));
+    hipStream_t st = stream ? static_cast<hipStream_t>(stream) : c->stream;
+    if (v->last_stream && v->last_stream != st) HIP_TRY(c, hipStreamSynchronize(v->last_stream));
+    launch_aa_image(scene_dev(c), v->dev.keys, c->noalloc_scratch, v->dev.width, v->dev.height, cam->view_projection,
+                    image, st);
     HIP_TRY(c, hipGetLastError());
     return VKGV_OK;
 }

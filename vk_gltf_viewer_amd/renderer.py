@@ -310,6 +310,15 @@ class View:
             self.r._lib.vkgv_view_aa_image(self.r._h, self._h, C.byref(camera), out.data_ptr(), st)))
         return out
 
+    def aa_image_noalloc(self, camera: N.Camera, out):
+        """aa_image through vkgv_view_aa_image_noalloc: the library allocates
+        nothing (its scratch is a fixed per-GPU array); `out` is required."""
+        import torch
+        self._tensor(out, (self.height, self.width, 3), torch.float32, "out")
+        self._torch_call(lambda st: self.r._check(
+            self.r._lib.vkgv_view_aa_image_noalloc(self.r._h, self._h, C.byref(camera), out.data_ptr(), st)))
+        return out
+
     def aa_backward(self, camera: N.Camera, g_image, g_transforms):
         """g_transforms (float64 [T, 16] device tensor) += d(sum g_image * image) / d transforms."""
         import torch
