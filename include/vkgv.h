@@ -465,6 +465,19 @@ int  vkgv_render_batch(vkgv_ctx* ctx, vkgv_batch* batch, const vkgv_frame_desc* 
  * frames since the previous call. */
 int  vkgv_batch_read_stats(vkgv_ctx* ctx, vkgv_batch* batch, vkgv_frame_stats* stats);
 
+/* Autotuner for C and C++ hosts (no counterpart in the reference, whose raster
+ * is fixed function): renders the batch frame `frames` (as vkgv_render_batch
+ * would, flags and stream from frames[0]) under candidate raster tunings —
+ * small-triangle extent 4/8/16/32, raster blocks per CU 2/3, tile work item
+ * 8/16/32 triangles, resolve cache 0/all slots — by a coordinate search (one
+ * knob at a time, the others at the best so far, up to three sweeps), each
+ * timed with HIP events over `frames_per_candidate` frames after one warm-up
+ * frame; tunings whose frames fill a raster queue are rejected.  The fastest
+ * is applied to the context and returned (with its ms per frame); the views
+ * hold the last candidate's frame.  Synchronous. */
+int  vkgv_autotune_batch(vkgv_ctx* ctx, vkgv_batch* batch, const vkgv_frame_desc* frames, uint32_t count,
+                         uint32_t frames_per_candidate, vkgv_tuning* best, float* best_ms);
+
 /* ------------------------------------------------------------------------ */
 /* Differentiable frame: gradients of an image loss with respect to the
  * instance transforms, through analytic edge antialiasing of the view's last

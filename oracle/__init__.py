@@ -13,7 +13,8 @@ import subprocess
 import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-LIB = os.path.join(HERE, "build", "libvkgv_oracle.so")
+# VKGV_ORACLE_LIB selects another build of the oracle (the sanitized one, make sanitize)
+LIB = os.environ.get("VKGV_ORACLE_LIB") or os.path.join(HERE, "build", "libvkgv_oracle.so")
 
 _lib = None
 

@@ -97,3 +97,35 @@ def test_tuning_limits(rend):
         rend.set_tuning(0, 3, 16, 0)
     rend.set_tuning(*DEFAULT)
     assert rend.get_tuning() == DEFAULT
+
+
+def test_native_autotuner_picks_a_valid_tuning(rend):
+    """vkgv_autotune_batch (the tuner C and C++ hosts call): it applies a
+    tuning from the candidate set whose frames fill no queue, no slower than
+    the default tuning timed the same way (within timing noise), and the frame
+    under it equals the default frame bit for bit."""
+    sc = Scene(3, 3).arrays()
+    rend.load(sc)
+    vds = [sc.views[i] for i in (0, 5, 9, 20, 41, 63)]
+    views = [rend.create_view(v.width, v.height) for v in vds]
+    batch = rend.create_batch(views)
+    cams = [camera_from_view(v) for v in vds]
+    flags = N.CULL_FRUSTUM | N.CULL_TWO_PASS | N.RESOLVE_BARY | N.GRAPH
+    try:
+        rend.set_tuning(*DEFAULT)
+        batch.render(cams, flags)
+        ref = [(v.keys(), v.color()) for v in views]
+        best, ms = batch.autotune(cams, flags, frames_per_candidate=3)
+        print(f"native autotuner: {best} at {ms:.4f} ms per batch frame", flush=True)
+        assert rend.get_tuning() == best
+        assert best[0] in (4, 8, 16, 32) and best[1] in (2, 3) and best[2] in (8, 16, 32)
+        assert 0.0 < ms < 1e29
+        batch.render(cams, flags)
+        assert batch.stats().overflow == 0
+        for i, v in enumerate(views):
+            assert np.array_equal(v.keys(), ref[i][0]) and np.array_equal(v.color(), ref[i][1]), i
+    finally:
+        rend.set_tuning(*DEFAULT)
+        batch.close()
+        for v in views:
+            v.close()

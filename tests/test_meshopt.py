@@ -224,3 +224,42 @@ def test_filter_quaternion_reconstructs_unit_quaternions():
     got = enc.astype(np.float64) / 32767.0
     assert np.abs(got - q).max() < 2.0 / rmax
     assert (enc[np.arange(len(q)), qc] > 0).all()
+
+
+def test_mutated_streams_are_refused_or_decoded_in_bounds():
+    """Random truncations, insertions and byte flips of valid streams of every
+    codec: each decode returns a status (never reads or writes out of bounds;
+    the sanitized run of this file checks that with AddressSanitizer), and a
+    stream that still decodes gives exactly `count` elements."""
+    rng = np.random.default_rng(99)
+    tris = rng.integers(0, 40, (30, 3))
+    tris = tris[(tris[:, 0] != tris[:, 1]) & (tris[:, 1] != tris[:, 2]) & (tris[:, 0] != tris[:, 2])]
+    cases = []
+    for version in (0, 1):
+        s, _ = encode_index_buffer(tris, version, rng)
+        cases.append((MO.decode_index_buffer, s, tris.size, 4))
+    v = rng.integers(0, 256, (37, 12), dtype=np.uint8)
+    cases.append((MO.decode_vertex_buffer, encode_vertex_buffer(v), 37, 12))
+    seq = np.cumsum(rng.integers(-5, 9, 50)) + 100
+    cases.append((MO.decode_index_sequence, encode_index_sequence(seq), seq.size, 4))
+    decoded = refused = 0
+    for fn, s, count, size in cases:
+        for _ in range(400):
+            b = bytearray(s)
+            kind = rng.integers(0, 3)
+            if kind == 0:
+                b = b[:rng.integers(0, len(b))]
+            elif kind == 1:
+                p = rng.integers(0, len(b) + 1)
+                b = b[:p] + bytes(rng.integers(0, 256, rng.integers(1, 5), dtype=np.uint8)) + b[p:]
+            else:
+                for _ in range(rng.integers(1, 4)):
+                    b[rng.integers(0, len(b))] = rng.integers(0, 256)
+            rc, out = fn(bytes(b), count, size)
+            if rc == 0:
+                decoded += 1
+                assert len(out) == count * size
+            else:
+                refused += 1
+                assert rc in (-1, -2, -3)
+    assert refused > 500 and decoded > 0

@@ -227,3 +227,62 @@ def test_lower_bound_is_a_bisection():
             i = int(np.searchsorted(t, time, side="left"))
             want = 0 if i == 0 else (k - 1 if i == k else i - 1)
             assert oracle.anim_sample(sp, t, v, 3, time)[0] == want
+
+
+def _ulp32(x):
+    return np.spacing(np.abs(np.float32(x))).astype(np.float64)
+
+
+def test_cubic_spline_and_slerp_ulp_bounds():
+    """The deliberate deviations from the reference's float forms, bounded.
+    The reference's cubic spline takes t^2 and t^3 with std::powf
+    (assets.hpp:88-89); here (oracle and kernel alike) they are t * t and
+    (t * t) * t, the rest of the expression in the reference's order.  Its
+    slerp is fastgltf's float acos / sin; here a double-precision series.
+    On random keys both stay within a few float32 ulps of the float64 closed
+    forms, and the spline within a few ulps of the reference's own float
+    expression (powf emulated as the correctly rounded t^3)."""
+    rng = np.random.default_rng(17)
+    times = np.array([0.0, 1.0], np.float32)
+    sp = sampler(2, N.INTERP_CUBICSPLINE)
+    worst64 = worst_ref = 0.0
+    for _ in range(400):
+        vals = rng.normal(0, 2, 6 * 3).astype(np.float32)          # (in, value, out) x 2 keys x 3 comps
+        t = np.float32(rng.uniform(0.02, 0.98))
+        got = oracle.anim_sample(sp, times, vals, 3, float(t)).astype(np.float64)
+        v0, b0, a1, v1 = vals[3:6], vals[6:9], vals[9:12], vals[12:15]
+        u = np.float64(t)
+        exact = ((2 * u ** 3 - 3 * u ** 2 + 1) * v0 + (u ** 3 - 2 * u ** 2 + u) * b0
+                 + (-2 * u ** 3 + 3 * u ** 2) * v1 + (u ** 3 - u ** 2) * a1)
+        # the reference's float32 expression with powf (t2 = t*t exactly rounded; t3 correctly rounded)
+        f = np.float32
+        t2, t3 = f(t * t), f(np.float64(t) ** 3)
+        h00, h10 = f(f(f(2) * t3 - f(3) * t2) + f(1)), f(f(t3 - f(2) * t2) + t)
+        h01, h11 = f(f(-2) * t3 + f(3) * t2), f(t3 - t2)
+        ref = (((v0 * h00 + (b0 * h10) * f(1)) + v1 * h01) + (a1 * h11) * f(1)).astype(np.float64)
+        scale = _ulp32(np.abs(vals).max())
+        worst64 = max(worst64, float(np.abs(got - exact).max() / scale))
+        worst_ref = max(worst_ref, float(np.abs(got - ref).max() / scale))
+    print(f"cubic spline: {worst64:.2f} ulp from float64, {worst_ref:.2f} ulp from the powf form")
+    assert worst64 <= 8.0 and worst_ref <= 8.0
+    # slerp: within a few ulps of the float64 rotation by the interpolated angle
+    sp = sampler(2, N.INTERP_LINEAR)
+    worst = 0.0
+    for _ in range(400):
+        axis = rng.normal(size=3)
+        a0, a1_ = rng.uniform(-3, 3, 2)
+        q0, q1 = axis_quat(axis, a0), axis_quat(axis, a1_)
+        vals = np.concatenate([q0, q1]).astype(np.float32)
+        t = float(np.float32(rng.uniform(0.01, 0.99)))
+        got = oracle.anim_sample(sp, times, vals, 4, t).astype(np.float64)
+        a, b = vals[:4].astype(np.float64), vals[4:].astype(np.float64)
+        c = float(a @ b)
+        if c < 0:
+            b, c = -b, -c
+        if c > 1 - 1.1920928955078125e-7:
+            continue                                                 # the lerp branch
+        ang = np.arccos(c)
+        want = (np.sin((1 - t) * ang) * a + np.sin(t * ang) * b) / np.sin(ang)
+        worst = max(worst, float(np.abs(got - want).max() / _ulp32(1.0)))
+    print(f"slerp: {worst:.2f} ulp of 1.0 from float64")
+    assert worst <= 4.0

@@ -15,8 +15,10 @@ import os
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_DIR = os.path.join(_HERE, "lib")
-HIP_LIB = os.path.join(LIB_DIR, "libvkgv_hip.so")
-HOST_LIB = os.path.join(LIB_DIR, "libvkgv_host.so")
+# VKGV_HIP_LIB selects another build of the HIP library
+HIP_LIB = os.environ.get("VKGV_HIP_LIB") or os.path.join(LIB_DIR, "libvkgv_hip.so")
+# VKGV_HOST_LIB selects another build of the host library (the sanitized one)
+HOST_LIB = os.environ.get("VKGV_HOST_LIB") or os.path.join(LIB_DIR, "libvkgv_host.so")
 
 # ---------------------------------------------------------------------------
 # Constants (include/vkgv.h)
@@ -218,7 +220,7 @@ HIP_SYMBOLS = [
     "vkgv_view_aa_image", "vkgv_view_aa_image_noalloc", "vkgv_view_aa_backward", "vkgv_upload_textures",
     "vkgv_set_hierarchy",
     "vkgv_update_transforms", "vkgv_read_transforms", "vkgv_view_read_motion", "vkgv_measure_atomic_rate",
-    "vkgv_upload_scene_device", "vkgv_set_transforms_device",
+    "vkgv_upload_scene_device", "vkgv_set_transforms_device", "vkgv_autotune_batch",
 ]
 
 
@@ -278,6 +280,7 @@ def hip():
         _bind(lib, "vkgv_batch_destroy", None, _p)
         _bind(lib, "vkgv_render_batch", C.c_int, _p, _p, _p, _u32)
         _bind(lib, "vkgv_batch_read_stats", C.c_int, _p, _p, C.POINTER(FrameStats))
+        _bind(lib, "vkgv_autotune_batch", C.c_int, _p, _p, _p, _u32, _u32, C.POINTER(Tuning), C.POINTER(C.c_float))
         _bind(lib, "vkgv_view_aa_image", C.c_int, _p, _p, C.POINTER(Camera), _p, _p)
         _bind(lib, "vkgv_view_aa_image_noalloc", C.c_int, _p, _p, C.POINTER(Camera), _p, _p)
         _bind(lib, "vkgv_view_aa_backward", C.c_int, _p, _p, C.POINTER(Camera), _p, _p, _u32, _p)

@@ -37,10 +37,19 @@ def _run(cmd):
     subprocess.run(cmd, check=True)
 
 
-def build_host(verbose=True):
-    os.makedirs(LIB, exist_ok=True)
-    out = os.path.join(LIB, "libvkgv_host.so")
-    _run(["g++", "-O2", "-std=c++17", "-fPIC", "-shared", "-fopenmp", "-ffp-contract=off",
+# AddressSanitizer + UndefinedBehaviorSanitizer build of the host library (CPU
+# code only), for the sanitized test run (tests/test_sanitizers.py); loaded in
+# place of the plain build through VKGV_HOST_LIB.
+SANITIZE_FLAGS = ["-O1", "-g", "-fno-omit-frame-pointer", "-fsanitize=address,undefined",
+                  "-fno-sanitize-recover=undefined"]
+
+
+def build_host(verbose=True, sanitize=False):
+    out_dir = os.path.join(LIB, "sanitized") if sanitize else LIB
+    os.makedirs(out_dir, exist_ok=True)
+    out = os.path.join(out_dir, "libvkgv_host.so")
+    opt = SANITIZE_FLAGS if sanitize else ["-O2"]
+    _run(["g++", *opt, "-std=c++17", "-fPIC", "-shared", "-fopenmp", "-ffp-contract=off",
           "-Wall", "-Wextra", "-Wno-unused-parameter", "-I", INCLUDE, "-o", out,
           os.path.join(CSRC, "host", "host.cpp"), os.path.join(CSRC, "host", "meshopt.cpp")])
     return out

@@ -1717,4 +1717,83 @@ int vkgv_batch_read_stats(vkgv_ctx* c, vkgv_batch* b, vkgv_frame_stats* s)
     return VKGV_OK;
 }
 
+int vkgv_autotune_batch(vkgv_ctx* c, vkgv_batch* b, const vkgv_frame_desc* frames, uint32_t count,
+                        uint32_t frames_per_candidate, vkgv_tuning* best_out, float* best_ms)
+{
+    if (!c) return VKGV_ERROR_INVALID_ARGUMENT;
+    if (!b || b->ctx != c || !frames || count != b->views.size() || !best_out)
+        return fail(c, VKGV_ERROR_INVALID_ARGUMENT, "autotune_batch: bad arguments");
+    const uint32_t nf = std::max<uint32_t>(frames_per_candidate, 1u);
+    HIP_TRY(c, hipSetDevice(c->device));
+    hipStream_t st = frames[0].stream ? static_cast<hipStream_t>(frames[0].stream) : c->stream;
+    hipEvent_t e0, e1;
+    HIP_TRY(c, hipEventCreate(&e0));
+    if (hipEventCreate(&e1) != hipSuccess) { (void)hipEventDestroy(e0); return fail(c, VKGV_ERROR_HIP, "hipEventCreate"); }
+    const RasterTuning before = c->tuning;
+    // the candidate values of each knob (the Python autotuner's default set)
+    const std::vector<std::vector<uint32_t>> axes = {
+        { 4u, 8u, 16u, 32u }, { 2u, 3u }, { 8u, 16u, 32u }, { 0u, c->cache_slots } };
+    auto knob = [](RasterTuning& t, int k) -> uint32_t& {
+        return k == 0 ? t.small_max_extent : k == 1 ? t.raster_blocks_per_cu : k == 2 ? t.tile_item_tris
+                                                                                  : t.resolve_cache_slots;
+    };
+    std::vector<std::pair<RasterTuning, float>> timed;
+    int rc = VKGV_OK;
+    // ms per batch frame under t, +inf when a frame had a queue full (the
+    // direct path: slower, and a sign that the tuning does not fit the scene)
+    auto measure = [&](const RasterTuning& t) -> float {
+        for (const auto& e : timed)
+            if (std::memcmp(&e.first, &t, sizeof t) == 0) return e.second;
+        c->tuning = t;
+        float ms = 1e30f;
+        bool ok = (rc = vkgv_render_batch(c, b, frames, count)) == VKGV_OK;   // warm-up (and graph capture)
+        ok = ok && hipEventRecord(e0, st) == hipSuccess;
+        for (uint32_t f = 0; ok && f < nf; ++f) ok = (rc = vkgv_render_batch(c, b, frames, count)) == VKGV_OK;
+        ok = ok && hipEventRecord(e1, st) == hipSuccess && hipEventSynchronize(e1) == hipSuccess;
+        float el = 0.0f;
+        ok = ok && hipEventElapsedTime(&el, e0, e1) == hipSuccess;
+        vkgv_frame_stats s;
+        ok = ok && (rc = vkgv_batch_read_stats(c, b, &s)) == VKGV_OK;
+        if (ok && s.overflow == 0) ms = el / (float)nf;
+        if (!ok && rc == VKGV_OK) rc = fail(c, VKGV_ERROR_HIP, "autotune_batch: timing failed");
+        timed.push_back({ t, ms });
+        return ms;
+    };
+    RasterTuning cur = before;
+    for (int k = 0; k < 4; ++k) {   // start from the current value where it is a candidate
+        const auto& ax = axes[k];
+        if (std::find(ax.begin(), ax.end(), knob(cur, k)) == ax.end()) knob(cur, k) = ax[0];
+    }
+    for (int sweep = 0; sweep < 3 && rc == VKGV_OK; ++sweep) {
+        bool changed = false;
+        for (int k = 0; k < 4 && rc == VKGV_OK; ++k) {
+            RasterTuning bt = cur;
+            float bm = measure(cur);
+            for (uint32_t v : axes[k]) {
+                RasterTuning t = cur;
+                knob(t, k) = v;
+                const float m = measure(t);
+                if (m < bm) { bm = m; bt = t; }
+            }
+            if (std::memcmp(&bt, &cur, sizeof bt) != 0) { cur = bt; changed = true; }
+        }
+        if (!changed) break;
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    if (rc != VKGV_OK) { c->tuning = before; return rc; }
+    const float m = measure(cur);
+    if (m >= 1e30f) {
+        c->tuning = before;
+        return fail(c, VKGV_ERROR_STATE, "autotune_batch: every candidate filled a raster queue");
+    }
+    c->tuning = cur;
+    best_out->small_max_extent = cur.small_max_extent;
+    best_out->raster_blocks_per_cu = cur.raster_blocks_per_cu;
+    best_out->tile_item_tris = cur.tile_item_tris;
+    best_out->resolve_cache_slots = cur.resolve_cache_slots;
+    if (best_ms) *best_ms = m;
+    return VKGV_OK;
+}
+
 } // extern "C"

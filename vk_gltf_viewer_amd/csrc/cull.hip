@@ -219,7 +219,10 @@ __global__ __launch_bounds__(kBlock) void cull_kernel(CullArgs a)
     const uint32_t nwaves = (gridDim.x * kBlock) >> 6;
     const bool single = a.mode == CULL_SINGLE;
     const bool probe = a.mode == CULL_LATE || (single && (a.flags & VKGV_CULL_OCCLUSION_PREV));
-    for (uint32_t u = (blockIdx.x * kBlock + threadIdx.x) >> 6; u < a.units; u += nwaves) {
+    // the wave's index, uniform in the compiler's eyes: unit, view group and
+    // view loop bounds then live in scalar registers and the loops branch on them
+    const uint32_t wave = __builtin_amdgcn_readfirstlane((blockIdx.x * kBlock + threadIdx.x) >> 6);
+    for (uint32_t u = wave; u < a.units; u += nwaves) {
         const uint32_t chunk = u / a.groups, group = u - chunk * a.groups;
         const uint32_t base = chunk * 64u * (uint32_t)P;
         DrawCull dc[P];
@@ -245,10 +248,12 @@ __global__ __launch_bounds__(kBlock) void cull_kernel(CullArgs a)
             bool pass[P], vis[P];
 #pragma unroll
             for (int i = 0; i < P; ++i) {
+                // (readlane returns a signed int: widen each half as uint32_t, or a
+                // set bit 31 of the low half would sign-extend into the high half)
                 const uint32_t src = j * (uint32_t)P + (uint32_t)i;
-                const unsigned long long hw =
-                    ((unsigned long long)__builtin_amdgcn_readlane((uint32_t)(hword >> 32), src) << 32)
-                    | __builtin_amdgcn_readlane((uint32_t)hword, src);
+                const uint32_t hi = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)(hword >> 32), src);
+                const uint32_t lo = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)hword, src);
+                const unsigned long long hw = ((unsigned long long)hi << 32) | (unsigned long long)lo;
                 const bool old = dc[i].valid && ((hw >> lane) & 1ull);
                 // the early pass tests only what the history lists
                 pass[i] = dc[i].valid && (a.mode != CULL_EARLY || old);
@@ -308,7 +313,22 @@ __global__ __launch_bounds__(kBlock) void cull_kernel(CullArgs a)
             if (total && lane == j) reserved = atomicAdd(&view.counters->vis_count[a.list], total);
         }
         if (__ballot(bits != 0ull) == 0ull) continue;
-        // ---- phase 2: the raster records of the visible draws
+        // ---- phase 2: the raster records of the visible draws.  A draw's
+        // record is the same for every view: each lane loads the records of its
+        // draws visible in any view once, all P loads in flight together, then
+        // stores them into every view's list that holds them.
+        uint4 rec[P][2];
+#pragma unroll
+        for (int i = 0; i < P; ++i) {
+            unsigned long long in_any = 0ull;
+            for (uint32_t j = 0; j < nv; ++j) in_any |= (bits >> (j * (uint32_t)P + (uint32_t)i)) & 1ull;
+            rec[i][0] = rec[i][1] = make_uint4(0u, 0u, 0u, 0u);
+            if (in_any) {
+                const uint4* src = reinterpret_cast<const uint4*>(a.s.rrec + base + 64u * i + lane);
+                rec[i][0] = src[0];
+                rec[i][1] = src[1];
+            }
+        }
         for (uint32_t j = 0; j < nv; ++j) {
             uint32_t out = __builtin_amdgcn_readlane(reserved, j);
             RasterRec* dst = batch_view(a.b, v0 + j).visible[a.list];
@@ -319,11 +339,9 @@ __global__ __launch_bounds__(kBlock) void cull_kernel(CullArgs a)
                 if (mine) {
                     const uint32_t rank =
                         __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
-                    const uint4* src = reinterpret_cast<const uint4*>(a.s.rrec + base + 64u * i + lane);
                     uint4* o = reinterpret_cast<uint4*>(dst + out + rank);
-                    const uint4 q0 = src[0], q1 = src[1];
-                    o[0] = q0;
-                    o[1] = q1;
+                    o[0] = rec[i][0];
+                    o[1] = rec[i][1];
                 }
                 out += (uint32_t)__popcll(m);
             }

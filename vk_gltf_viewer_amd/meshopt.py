@@ -4,6 +4,7 @@ does it (assets.cpp:111-171), but with the decoder's status checked."""
 from __future__ import annotations
 
 import ctypes as C
+import os
 
 import numpy as np
 
@@ -33,7 +34,27 @@ def decode(data: bytes, count: int, stride: int, mode: str | int = "ATTRIBUTES",
 
 
 def _call(name, data, count, size):
-    src = np.frombuffer(bytes(data), np.uint8)
+    data = bytes(data)
+    if os.environ.get("VKGV_MESHOPT_EXACT_BUFFERS"):
+        # the sanitized test run: source and destination in malloc'd blocks of
+        # their exact sizes, so that an access past either end is detected
+        libc = C.CDLL(None)
+        libc.malloc.restype = C.c_void_p
+        libc.malloc.argtypes = [C.c_size_t]
+        libc.free.argtypes = [C.c_void_p]
+        n_out = count * size
+        src = libc.malloc(max(1, len(data)))
+        dst = libc.malloc(max(1, n_out))
+        try:
+            C.memmove(src, data, len(data))
+            C.memset(dst, 0, max(1, n_out))
+            rc = getattr(N.host(), name)(dst, count, size, src if data else None, len(data))
+            out = np.frombuffer(C.string_at(dst, n_out), np.uint8).copy()
+        finally:
+            libc.free(src)
+            libc.free(dst)
+        return rc, out
+    src = np.frombuffer(data, np.uint8)
     out = np.zeros(max(1, count * size), np.uint8)
     rc = getattr(N.host(), name)(out.ctypes.data, count, size, src.ctypes.data if src.size else None, src.size)
     return rc, out[:count * size]

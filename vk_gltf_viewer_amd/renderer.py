@@ -245,6 +245,20 @@ class Batch:
         self.r._check(self.r._lib.vkgv_batch_read_stats(self.r._h, self._h, C.byref(s)))
         return s
 
+    def autotune(self, cameras, flags: int, frames_per_candidate: int = 3, eyes=None):
+        """The library's own autotuner (vkgv_autotune_batch) on this batch
+        frame: applies and returns the fastest tuning and its ms per frame."""
+        for i, cam in enumerate(cameras):
+            f = self._frames[i]
+            f.camera = cam
+            f.eye[:] = list(eyes[i]) if eyes is not None else [0.0, 0.0, 0.0]
+            f.flags = flags
+            f.stream = None
+        t, ms = N.Tuning(), C.c_float()
+        self.r._check(self.r._lib.vkgv_autotune_batch(self.r._h, self._h, self._frames, len(self.views),
+                                                      frames_per_candidate, C.byref(t), C.byref(ms)))
+        return (t.small_max_extent, t.raster_blocks_per_cu, t.tile_item_tris, t.resolve_cache_slots), ms.value
+
 
 class View:
     def __init__(self, r: Renderer, h, width, height):
