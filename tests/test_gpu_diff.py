@@ -215,8 +215,125 @@ def test_empty_and_degenerate_inputs():
     img.sum().backward()
     assert float(img.abs().sum()) == 0.0
     assert float(attr.grad.abs().sum()) == 0.0 and float(c.grad.abs().sum()) == 0.0
+    attr.grad = None
+    c.grad = None
+    img = e.shade(c, attr, tris, ids, depth)
+    img.sum().backward()
+    assert float(img.abs().sum()) == 0.0
+    assert float(attr.grad.abs().sum()) == 0.0 and float(c.grad.abs().sum()) == 0.0
     none = torch.zeros((0, 3), dtype=torch.int32, device=DEV)
     ids, depth = e.rasterize(clip, none, 8, 8, False)
     assert int(ids.abs().sum()) == 0
     with pytest.raises(RuntimeError):
         e.rasterize(clip, torch.tensor([[0, 1, 3]], dtype=torch.int32, device=DEV), 8, 8, False)
+
+
+# ---------------------------------------------------------------------------
+# The backward gathers per (camera, instance, vertex) over each triangle's
+# screen box; pixels outside their own triangle's box ("strays": ids that do
+# not come from the raster, triangles with a box over 64 px) go through the
+# per-pixel scatter.  Both against the FP64 reference, in float64.
+
+def _grid_case(size, shift=False):
+    """A 5x5-vertex grid (32 triangles) seen by 2 cameras as 2 instances: one
+    filling the view (boxes of hundreds of pixels), one shrunk to a few pixels
+    per triangle.  ids / depth from the raster, optionally with the rows of
+    the small instance shifted 3 pixels sideways (ids outside their
+    triangles' boxes)."""
+    from vk_gltf_viewer_amd.scene import view_desc
+    g = np.linspace(-1.0, 1.0, 5)
+    pos = np.array([[x, y, 0.0] for y in g for x in g], np.float64)
+    tris = []
+    for r in range(4):
+        for q in range(4):
+            a = r * 5 + q
+            tris += [[a, a + 1, a + 6], [a, a + 6, a + 5]]
+    tris = np.array(tris, np.int32)
+    xf = np.stack([np.eye(4), np.eye(4)])
+    xf[1, :3, :3] *= 0.06
+    xf[1, :3, 3] = [0.3, -0.2, 0.1]
+    vps = np.stack([D.camera_matrix(view_desc(size, size, (0.2, 0.1, 2.6), (0.0, 0.0, 0.0), fovy_deg=45.0)),
+                    D.camera_matrix(view_desc(size, size, (-0.4, 0.3, 2.4), (0.0, 0.0, 0.0), fovy_deg=45.0))])
+    ph = np.concatenate([pos, np.ones((len(pos), 1))], 1)
+    clip = np.einsum("ciad,vd->civa", np.einsum("cab,ibd->ciad", vps.astype(np.float64), xf), ph)
+    e = D.ext()
+    ids, depth = e.rasterize(_t(clip.astype(np.float32)), _t(tris), size, size, False)
+    ids, depth = ids.cpu().numpy(), depth.cpu().numpy()
+    if shift:
+        rows = np.nonzero(((ids - 1) // len(tris) == 1) & (ids > 0))[1]
+        r0, r1 = int(rows.min()), int(rows.max()) + 1
+        ids[:, r0:r1] = np.roll(ids[:, r0:r1], 3, axis=2)
+    rng = np.random.default_rng(11)
+    attr = rng.uniform(0.0, 1.0, (len(pos), 3))
+    target = rng.uniform(0.0, 1.0, ids.shape + (3,))
+    return clip, attr, tris, ids, depth, target
+
+
+def _ops_gradients(clip, attr, tris, ids, depth, target, fused):
+    e = D.ext()
+    c = _t(clip).requires_grad_()
+    a = _t(attr).requires_grad_()
+    tt, it, dt = _t(tris), _t(ids), _t(depth)
+    if fused:
+        img = e.shade(c, a, tt, it, dt)
+    else:
+        img = e.antialias(e.interpolate(c, a, tt, it), c, tt, it, dt)
+    ((img - _t(target)) ** 2).sum().backward()
+    return c.grad.cpu().numpy(), a.grad.cpu().numpy()
+
+
+@pytest.mark.parametrize("fused", [False, True])
+@pytest.mark.parametrize("shift", [False, True])
+def test_stray_pixels_match_fp64(shift, fused):
+    print(f"case: grid 2 instances 2 cameras 96x96 float64, shifted rows {shift}, fused {fused}", flush=True)
+    clip, attr, tris, ids, depth, target = _grid_case(96, shift)
+    big = (ids > 0) & ((ids - 1) // len(tris) == 0)
+    assert big.sum() > 1000 and (((ids - 1) // len(tris) == 1) & (ids > 0)).sum() > 10   # both instances on screen
+    g_clip, g_attr = _ops_gradients(clip, attr, tris, ids, depth, target, fused)
+    r_clip, r_attr = oracle.diff_backward(clip, attr, tris, ids, depth, target, True)
+    for name, g, r in (("clip", g_clip, r_clip), ("attr", g_attr, r_attr)):
+        err = np.linalg.norm(g - r) / np.linalg.norm(r)
+        print(f"strays {shift} fused {fused}: {name} |g - ref| / |ref| = {err:.3e}")
+        assert err < 1e-9, name
+
+
+def test_fused_shade_matches_separate_functions():
+    """shade() is antialias(interpolate()): the same image bit for bit, and
+    gradients equal to float64 rounding (one vertex pass sums both stages'
+    terms instead of autograd adding two clip gradients)."""
+    print("case: icosphere L6, 2 instances, 3 cameras, 128x128, float64, shade vs separate", flush=True)
+    s = D.c4_scene(instances=2, cameras=3, size=128)
+    clip = D.clip_positions(_t(s.pos).double(), _t(s.xforms).double(), _t(s.view_projs).double())
+    e = D.ext()
+    tris = _t(s.tris)
+    ids, depth = e.rasterize(clip.float().contiguous(), tris, 128, 128, False)
+    attr = _t(s.albedo).double()
+    target = torch.rand((3, 128, 128, 3), dtype=torch.float64, device=DEV, generator=torch.Generator(DEV).manual_seed(5))
+    out = []
+    for fused in (False, True):
+        c = clip.detach().clone().requires_grad_()
+        a = attr.clone().requires_grad_()
+        img = e.shade(c, a, tris, ids, depth) if fused else e.antialias(e.interpolate(c, a, tris, ids), c, tris, ids,
+                                                                        depth)
+        ((img - target) ** 2).sum().backward()
+        out.append((img.detach(), c.grad, a.grad))
+    assert torch.equal(out[0][0], out[1][0])
+    for k in (1, 2):
+        err = float((out[0][k] - out[1][k]).norm() / out[0][k].norm())
+        assert err < 1e-13, k
+
+
+def test_gathered_gradients_are_deterministic():
+    print("case: icosphere L6, 2 instances, 3 cameras, 128x128, mixed, backward twice", flush=True)
+    case = Case(2, 3, 128)
+    first = [g.copy() for g in case.grads]
+    case.pos.grad = None
+    case.alb.grad = None
+    case.xf.grad = None
+    img, _, _ = D.render(case.pos, case.alb, case.tris, case.xf, case.vp, 128, 128, precision="mixed")
+    ((img - case.target) ** 2).sum().backward()
+    again = (case.pos.grad.double().cpu().numpy(), case.alb.grad.double().cpu().numpy())
+    # positions and albedo: gathers and fixed-order sums only (the transform
+    # gradient sums its blocks with atomics)
+    for a, b in zip(first[:2], again):
+        assert np.array_equal(a, b)

@@ -2,6 +2,9 @@
 // interpolate and antialias as torch::autograd::Function's whose backward
 // runs the hand-written kernels of diff.hip.
 #include <torch/extension.h>
+#include <algorithm>
+#include <mutex>
+#include <vector>
 #include <c10/hip/HIPStream.h>
 
 #include "diff.hpp"
@@ -73,6 +76,77 @@ template <typename F> void dispatch_real(const Tensor& clip, const Tensor& attr,
     }
 }
 
+// The vertex adjacency of a mesh for the backward gathers, built on the
+// device and kept for the last few meshes: an entry holds a weak reference
+// to the triangle tensor's storage and its version counter, so a freed or
+// modified mesh is rebuilt.
+struct AdjEntry {
+    c10::weak_intrusive_ptr<c10::StorageImpl> storage;
+    int64_t offset, version, T, V;
+    c10::Device device;
+    Tensor off, adj;
+};
+// Never destroyed: its device tensors must not be freed after the runtime at exit.
+std::vector<AdjEntry>& adj_cache()
+{
+    static auto* c = new std::vector<AdjEntry>();
+    return *c;
+}
+
+std::pair<Tensor, Tensor> adjacency(const Tensor& tris, int V)
+{
+    static std::mutex mu;   // backwards of several devices run on their own threads
+    std::lock_guard<std::mutex> lock(mu);
+    auto& cache = adj_cache();
+    const int64_t T = tris.size(0);
+    for (size_t k = 0; k < cache.size(); ++k) {
+        AdjEntry& e = cache[k];
+        auto st = e.storage.lock();
+        if (st.get() == tris.storage().unsafeGetStorageImpl() && e.offset == tris.storage_offset() &&
+            e.version == tris._version() && e.T == T && e.V == V && e.device == tris.device()) {
+            std::pair<Tensor, Tensor> r{ e.off, e.adj };
+            if (k) std::rotate(cache.begin(), cache.begin() + k, cache.begin() + k + 1);
+            return r;
+        }
+    }
+    auto opt = tris.options().dtype(torch::kInt32);
+    Tensor off = torch::empty({ 2 * (int64_t)V + 1 }, opt), adj = torch::empty({ std::max<int64_t>(3 * T, 1) }, opt);
+    vkgv_diff::mesh_adjacency(tris.data_ptr<int>(), (int)T, V, off.data_ptr<int>(), adj.data_ptr<int>(), stream());
+    cache.insert(cache.begin(), AdjEntry{ tris.storage().getWeakStorageImpl(), tris.storage_offset(), tris._version(), T,
+                                          V, tris.device(), off, adj });
+    if (cache.size() > 4) cache.pop_back();
+    return { off, adj };
+}
+
+Tensor stray_flags(const Shape& s, const Tensor& like)
+{
+    return torch::empty({ (int64_t)vkgv_diff::stray_flag_count(s.C, s.H, s.W) }, like.options().dtype(torch::kInt32));
+}
+
+Tensor tri_boxes(const Tensor& clip, const Tensor& tris, const Shape& s)
+{
+    Tensor boxes = torch::empty({ std::max<int64_t>((int64_t)vkgv_diff::tri_boxes_words(s.C, s.I, s.T), 1) },
+                                clip.options().dtype(torch::kInt64));
+    auto* b = reinterpret_cast<unsigned long long*>(boxes.data_ptr<int64_t>());
+    if (clip.scalar_type() == torch::kFloat64)
+        vkgv_diff::tri_boxes(clip.data_ptr<double>(), tris.data_ptr<int>(), s.C, s.I, s.V, s.T, s.H, s.W, b, stream());
+    else
+        vkgv_diff::tri_boxes(clip.data_ptr<float>(), tris.data_ptr<int>(), s.C, s.I, s.V, s.T, s.H, s.W, b, stream());
+    return boxes;
+}
+
+unsigned long long* box_ptr(const Tensor& boxes)
+{
+    return reinterpret_cast<unsigned long long*>(boxes.data_ptr<int64_t>());
+}
+
+const unsigned* flag_ptr(const Tensor& flags) { return reinterpret_cast<const unsigned*>(flags.data_ptr<int>()); }
+
+void check_gather(const Shape& s)
+{
+    TORCH_CHECK(((int64_t)s.V + 255) / 256 * s.C * s.I < ((int64_t)1 << 31), "too many camera-instance-vertex triples");
+}
+
 struct InterpolateFn : public torch::autograd::Function<InterpolateFn> {
     static Tensor forward(AutogradContext* ctx, Tensor clip, Tensor attr, Tensor tris, Tensor ids)
     {
@@ -80,28 +154,37 @@ struct InterpolateFn : public torch::autograd::Function<InterpolateFn> {
         check_real(attr, "attr", clip);
         TORCH_CHECK(attr.dim() == 2 && attr.size(0) == clip.size(2), "attr must be [V, A]");
         Tensor out = torch::empty({ s.C, s.H, s.W, s.A }, attr.options());
+        Tensor flags = stray_flags(s, ids), boxes = tri_boxes(clip, tris, s);
         dispatch_real(clip, attr, [&](auto cc, auto r) {
             using C = decltype(cc);
             using R = decltype(r);
             vkgv_diff::interp_fwd(clip.data_ptr<C>(), attr.data_ptr<R>(), tris.data_ptr<int>(), ids.data_ptr<int>(), s.C,
-                                  s.I, s.V, s.T, s.H, s.W, s.A, out.data_ptr<R>(), stream());
+                                  s.I, s.V, s.T, s.H, s.W, s.A, out.data_ptr<R>(), box_ptr(boxes),
+                                  reinterpret_cast<unsigned*>(flags.data_ptr<int>()), stream());
         });
-        ctx->save_for_backward({ clip, attr, tris, ids });
+        ctx->save_for_backward({ clip, attr, tris, ids, flags, boxes });
         return out;
     }
     static variable_list backward(AutogradContext* ctx, variable_list grads)
     {
         const auto saved = ctx->get_saved_variables();
-        const Tensor &clip = saved[0], &attr = saved[1], &tris = saved[2], &ids = saved[3];
+        const Tensor &clip = saved[0], &attr = saved[1], &tris = saved[2], &ids = saved[3], &flags = saved[4];
+        const Tensor& boxes = saved[5];
         const Shape s = shape_of(clip, tris, ids, (int)attr.size(1));
+        check_gather(s);
         Tensor g_out = grads[0].to(attr.scalar_type()).contiguous();
-        Tensor g_clip = torch::zeros_like(clip), g_attr = torch::zeros_like(attr);
+        Tensor g_clip = torch::empty_like(clip), g_attr = torch::empty_like(attr);
+        Tensor part = torch::empty({ std::max<int64_t>((int64_t)s.C * s.I * s.V * s.A, 1) }, attr.options());
+        Tensor contrib = torch::empty({ std::max<int64_t>((int64_t)s.C * s.H * s.W * 3 * (3 + s.A), 1) }, attr.options());
+        const auto adj = adjacency(tris, s.V);
         dispatch_real(clip, attr, [&](auto cc, auto r) {
             using C = decltype(cc);
             using R = decltype(r);
             vkgv_diff::interp_bwd(clip.data_ptr<C>(), attr.data_ptr<R>(), tris.data_ptr<int>(), ids.data_ptr<int>(), s.C,
-                                  s.I, s.V, s.T, s.H, s.W, s.A, g_out.data_ptr<R>(), g_clip.data_ptr<C>(),
-                                  g_attr.data_ptr<R>(), stream());
+                                  s.I, s.V, s.T, s.H, s.W, s.A, box_ptr(boxes), flag_ptr(flags),
+                                  adj.first.data_ptr<int>(), adj.second.data_ptr<int>(), g_out.data_ptr<R>(),
+                                  contrib.data_ptr<R>(), part.data_ptr<R>(), g_clip.data_ptr<C>(), g_attr.data_ptr<R>(),
+                                  stream());
         });
         return { g_clip, g_attr, Tensor(), Tensor() };
     }
@@ -117,30 +200,93 @@ struct AntialiasFn : public torch::autograd::Function<AntialiasFn> {
                     "color must be [C, H, W, A]");
         TORCH_CHECK(depth.sizes() == ids.sizes(), "depth must match ids");
         Tensor out = torch::empty_like(color);
+        Tensor flags = stray_flags(s, ids), boxes = tri_boxes(clip, tris, s);
         dispatch_real(clip, color, [&](auto cc, auto r) {
             using C = decltype(cc);
             using R = decltype(r);
             vkgv_diff::aa_fwd(color.data_ptr<R>(), clip.data_ptr<C>(), tris.data_ptr<int>(), ids.data_ptr<int>(),
-                              depth.data_ptr<float>(), s.C, s.I, s.V, s.T, s.H, s.W, s.A, out.data_ptr<R>(), stream());
+                              depth.data_ptr<float>(), s.C, s.I, s.V, s.T, s.H, s.W, s.A, out.data_ptr<R>(),
+                              box_ptr(boxes), reinterpret_cast<unsigned*>(flags.data_ptr<int>()), stream());
         });
-        ctx->save_for_backward({ color, clip, tris, ids, depth });
+        ctx->save_for_backward({ color, clip, tris, ids, depth, flags, boxes });
         return out;
     }
     static variable_list backward(AutogradContext* ctx, variable_list grads)
     {
         const auto saved = ctx->get_saved_variables();
         const Tensor &color = saved[0], &clip = saved[1], &tris = saved[2], &ids = saved[3], &depth = saved[4];
+        const Tensor &flags = saved[5], &boxes = saved[6];
         const Shape s = shape_of(clip, tris, ids, (int)color.size(3));
+        check_gather(s);
         Tensor g_out = grads[0].to(color.scalar_type()).contiguous();
-        Tensor g_color = torch::empty_like(color), g_clip = torch::zeros_like(clip);
+        Tensor g_color = torch::empty_like(color), g_clip = torch::empty_like(clip);
+        Tensor contrib = torch::empty({ std::max<int64_t>((int64_t)s.C * s.H * s.W * 9, 1) }, color.options());
+        const auto adj = adjacency(tris, s.V);
         dispatch_real(clip, color, [&](auto cc, auto r) {
             using C = decltype(cc);
             using R = decltype(r);
             vkgv_diff::aa_bwd(color.data_ptr<R>(), clip.data_ptr<C>(), tris.data_ptr<int>(), ids.data_ptr<int>(),
-                              depth.data_ptr<float>(), s.C, s.I, s.V, s.T, s.H, s.W, s.A, g_out.data_ptr<R>(),
-                              g_color.data_ptr<R>(), g_clip.data_ptr<C>(), stream());
+                              depth.data_ptr<float>(), s.C, s.I, s.V, s.T, s.H, s.W, s.A, box_ptr(boxes),
+                              flag_ptr(flags), adj.first.data_ptr<int>(), adj.second.data_ptr<int>(),
+                              g_out.data_ptr<R>(), contrib.data_ptr<R>(), g_color.data_ptr<R>(), g_clip.data_ptr<C>(),
+                              stream());
         });
         return { g_color, g_clip, Tensor(), Tensor(), Tensor() };
+    }
+};
+
+// antialias(interpolate(clip, attr, tris, ids), clip, tris, ids, depth) as one
+// function: the same forward, and one backward whose vertex pass sums both
+// stages' terms into a single clip gradient.
+struct ShadeFn : public torch::autograd::Function<ShadeFn> {
+    static Tensor forward(AutogradContext* ctx, Tensor clip, Tensor attr, Tensor tris, Tensor ids, Tensor depth)
+    {
+        check_f32(depth, "depth");
+        const Shape s = shape_of(clip, tris, ids, attr.dim() == 2 ? (int)attr.size(1) : 0);
+        check_real(attr, "attr", clip);
+        TORCH_CHECK(attr.dim() == 2 && attr.size(0) == clip.size(2), "attr must be [V, A]");
+        TORCH_CHECK(depth.sizes() == ids.sizes(), "depth must match ids");
+        Tensor color = torch::empty({ s.C, s.H, s.W, s.A }, attr.options());
+        Tensor out = torch::empty_like(color);
+        Tensor flags = stray_flags(s, ids), boxes = tri_boxes(clip, tris, s);
+        dispatch_real(clip, attr, [&](auto cc, auto r) {
+            using C = decltype(cc);
+            using R = decltype(r);
+            vkgv_diff::interp_fwd(clip.data_ptr<C>(), attr.data_ptr<R>(), tris.data_ptr<int>(), ids.data_ptr<int>(), s.C,
+                                  s.I, s.V, s.T, s.H, s.W, s.A, color.data_ptr<R>(), box_ptr(boxes),
+                                  reinterpret_cast<unsigned*>(flags.data_ptr<int>()), stream());
+            vkgv_diff::aa_fwd(color.data_ptr<R>(), clip.data_ptr<C>(), tris.data_ptr<int>(), ids.data_ptr<int>(),
+                              depth.data_ptr<float>(), s.C, s.I, s.V, s.T, s.H, s.W, s.A, out.data_ptr<R>(),
+                              box_ptr(boxes), nullptr, stream());
+        });
+        ctx->save_for_backward({ clip, attr, tris, ids, depth, color, flags, boxes });
+        return out;
+    }
+    static variable_list backward(AutogradContext* ctx, variable_list grads)
+    {
+        const auto saved = ctx->get_saved_variables();
+        const Tensor &clip = saved[0], &attr = saved[1], &tris = saved[2], &ids = saved[3], &depth = saved[4];
+        const Tensor &color = saved[5], &flags = saved[6], &boxes = saved[7];
+        const Shape s = shape_of(clip, tris, ids, (int)attr.size(1));
+        check_gather(s);
+        Tensor g_out = grads[0].to(attr.scalar_type()).contiguous();
+        Tensor g_clip = torch::empty_like(clip), g_attr = torch::empty_like(attr), g_color = torch::empty_like(color);
+        const int64_t npx = (int64_t)s.C * s.H * s.W;
+        Tensor part = torch::empty({ std::max<int64_t>((int64_t)s.C * s.I * s.V * s.A, 1) }, attr.options());
+        Tensor contrib_i = torch::empty({ std::max<int64_t>(npx * 3 * (3 + s.A), 1) }, attr.options());
+        Tensor contrib_a = torch::empty({ std::max<int64_t>(npx * 9, 1) }, attr.options());
+        const auto adj = adjacency(tris, s.V);
+        dispatch_real(clip, attr, [&](auto cc, auto r) {
+            using C = decltype(cc);
+            using R = decltype(r);
+            vkgv_diff::shade_bwd(color.data_ptr<R>(), clip.data_ptr<C>(), attr.data_ptr<R>(), tris.data_ptr<int>(),
+                                 ids.data_ptr<int>(), depth.data_ptr<float>(), s.C, s.I, s.V, s.T, s.H, s.W, s.A,
+                                 box_ptr(boxes), flag_ptr(flags), adj.first.data_ptr<int>(), adj.second.data_ptr<int>(),
+                                 g_out.data_ptr<R>(), g_color.data_ptr<R>(), contrib_a.data_ptr<R>(),
+                                 contrib_i.data_ptr<R>(), part.data_ptr<R>(), g_clip.data_ptr<C>(),
+                                 g_attr.data_ptr<R>(), stream());
+        });
+        return { g_clip, g_attr, Tensor(), Tensor(), Tensor() };
     }
 };
 
@@ -183,6 +329,35 @@ struct TransformFn : public torch::autograd::Function<TransformFn> {
 
 Tensor transform(Tensor pos, Tensor A) { return TransformFn::apply(pos, A); }
 
+// Vertex indices of tris within 0..V-1 (else an error).  The check reads a
+// minimum and maximum back to the host, a synchronization, so a mesh that
+// passed is remembered (storage, offset, version and V, like the adjacency)
+// and not checked again.
+void check_vertex_range(const Tensor& tris, int64_t V)
+{
+    struct Checked {
+        c10::weak_intrusive_ptr<c10::StorageImpl> storage;
+        int64_t offset, version, numel, V;
+    };
+    static auto* seen = new std::vector<Checked>();   // never destroyed, like the adjacency cache
+    static std::mutex mu;
+    if (!tris.numel()) return;
+    std::lock_guard<std::mutex> lock(mu);
+    for (const Checked& e : *seen) {
+        auto st = e.storage.lock();
+        if (st.get() == tris.storage().unsafeGetStorageImpl() && e.offset == tris.storage_offset() &&
+            e.version == tris._version() && e.numel == tris.numel() && e.V == V)
+            return;
+    }
+    const auto mm = torch::aminmax(tris);
+    const Tensor both = torch::stack({ std::get<0>(mm), std::get<1>(mm) }).cpu();
+    const int mn = both[0].item<int>(), mx = both[1].item<int>();
+    TORCH_CHECK(mn >= 0 && mx < V, "triangle vertex index out of range");
+    seen->insert(seen->begin(), Checked{ tris.storage().getWeakStorageImpl(), tris.storage_offset(), tris._version(),
+                                         tris.numel(), V });
+    if (seen->size() > 8) seen->pop_back();
+}
+
 std::vector<Tensor> rasterize(Tensor clip, Tensor tris, int64_t H, int64_t W, bool cull_back)
 {
     check_f32(clip, "clip");
@@ -190,10 +365,7 @@ std::vector<Tensor> rasterize(Tensor clip, Tensor tris, int64_t H, int64_t W, bo
     TORCH_CHECK(clip.dim() == 4 && clip.size(3) == 4, "clip must be [C, I, V, 4]");
     TORCH_CHECK(H > 0 && W > 0 && H <= 16384 && W <= 16384, "resolution 1..16384");
     TORCH_CHECK(clip.size(1) * tris.size(0) < (int64_t)0x7FFFFFFF, "instances x triangles must fit the 31-bit id");
-    if (tris.numel()) {
-        const auto mx = tris.max().item<int>(), mn = tris.min().item<int>();
-        TORCH_CHECK(mn >= 0 && mx < clip.size(2), "triangle vertex index out of range");
-    }
+    check_vertex_range(tris, clip.size(2));
     const int C = (int)clip.size(0);
     auto opt = clip.options();
     Tensor keys = torch::full({ C, H, W }, -1, opt.dtype(torch::kInt64));
@@ -216,6 +388,11 @@ Tensor antialias(Tensor color, Tensor clip, Tensor tris, Tensor ids, Tensor dept
     return AntialiasFn::apply(color, clip, tris, ids, depth);
 }
 
+Tensor shade(Tensor clip, Tensor attr, Tensor tris, Tensor ids, Tensor depth)
+{
+    return ShadeFn::apply(clip, attr, tris, ids, depth);
+}
+
 } // namespace
 
 PYBIND11_MODULE(TORCH_EXTENSION_NAME, m)
@@ -225,4 +402,5 @@ PYBIND11_MODULE(TORCH_EXTENSION_NAME, m)
     m.def("rasterize", &rasterize, "triangle raster -> (ids, depth); no gradient");
     m.def("interpolate", &interpolate, "perspective-correct interpolation of per-vertex attributes");
     m.def("antialias", &antialias, "analytic edge antialiasing");
+    m.def("shade", &shade, "antialias(interpolate(clip, attr, tris, ids), clip, tris, ids, depth) with one backward");
 }

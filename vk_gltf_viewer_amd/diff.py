@@ -77,9 +77,11 @@ def render(pos, attr, tris, xforms, view_projs, height: int, width: int, antiali
     adt = torch.float64 if precision == "float64" else torch.float32
     clip = clip_positions(pos.to(cdt), xforms.to(cdt), view_projs.to(cdt))
     ids, depth = e.rasterize(clip.detach().float().contiguous(), tris, height, width, False)
-    img = e.interpolate(clip, attr.to(adt).contiguous(), tris, ids)
-    if antialias:
-        img = e.antialias(img, clip, tris, ids, depth)
+    attr = attr.to(adt).contiguous()
+    if antialias:   # interpolation and antialiasing as one function: one backward vertex pass
+        img = e.shade(clip, attr, tris, ids, depth)
+    else:
+        img = e.interpolate(clip, attr, tris, ids)
     return img, ids, depth
 
 
