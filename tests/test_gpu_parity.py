@@ -312,3 +312,41 @@ def test_raster_stats_match_oracle(rend, cfg):
     finally:
         plain.close()
         counted.close()
+
+
+def test_binned_path_both_scans(rend):
+    """The big-triangle lists are built by a scan folded into the fill pass for
+    views of at most 8192 screen tiles (32 px), and by a separate one-block
+    scan above that.  A 4096 x 2304 view (9216 tiles) takes the separate scan;
+    a 1280 x 720 view alone takes the folded one, and in a batch with the
+    large view the separate one: frames stay equal to the oracle's in every
+    order (the per-tile cursors of the folded scan stay zero in between)."""
+    sc = scene(2, 2)
+    load(rend, sc)
+    vd = sc.views[0]
+    cam = camera_from_view(vd)
+    rs = oracle.Scene(sc)
+    flags = N.CULL_FRUSTUM | N.CULL_TWO_PASS
+    small, large = rend.create_view(1280, 720), rend.create_view(4096, 2304)
+    batch = rend.create_batch([small, large])
+    try:
+        hist = {(1280, 720): np.zeros(len(sc.draws), np.uint8), (4096, 2304): np.zeros(len(sc.draws), np.uint8)}
+        for step, mode in enumerate(["alone", "batch", "alone", "batch"]):
+            print(f"case: binned path, step {step} {mode}", flush=True)
+            if mode == "alone":
+                small.render(cam, flags)
+                large.render(cam, flags)
+            else:
+                batch.render([cam, cam], flags)
+            for v, (W, H) in ((small, (1280, 720)), (large, (4096, 2304))):
+                rk, _, _, re, rl = oracle.frame_two_pass(rs, cam, W, H, flags, hist[(W, H)])
+                assert np.array_equal(v.visible(0), re) and np.array_equal(v.visible(1), rl), (step, W)
+                assert np.array_equal(v.keys(), rk), (step, W)
+                st = v.stats()
+                print(f"  {W}x{H}: big triangles {st.big_triangles}, tile pairs {st.big_tiles}", flush=True)
+                assert st.overflow == 0, (step, W)
+                assert step > 0 or W < 4096 or st.big_triangles > 0, "the large view bins big triangles"
+    finally:
+        batch.close()
+        small.close()
+        large.close()

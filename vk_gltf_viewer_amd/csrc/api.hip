@@ -392,6 +392,7 @@ int build_maps(vkgv_ctx* c, const std::vector<const ViewDev*>& views, DevBuf& ma
         for (uint64_t k = 0; k < nhiz; ++k) hiz.push_back(vi | (uint32_t)(k << 8));
         for (uint64_t k = 0; k < nres; ++k) resolve.push_back(vi | (uint32_t)(k << 8));
         if (v.hiz_tail_start < v.pyr.levels) g.max_tail_levels = std::max(g.max_tail_levels, v.pyr.levels);
+        g.max_bin_tiles = std::max(g.max_bin_tiles, v.tiles_x * v.tiles_y);
         for (uint32_t k = v.hiz_tile_levels; k < v.hiz_tail_start && k < kHizMaxLevels; ++k)
             g.hiz_level_blocks[k] = std::max(g.hiz_level_blocks[k], (uint32_t)(((uint64_t)v.pyr.w[k] * v.pyr.h[k] + 255) / 256));
     }
@@ -1210,7 +1211,8 @@ int vkgv_view_create(vkgv_ctx* c, uint32_t width, uint32_t height, vkgv_view** o
     if (hipMemsetAsync(v->pyramid.p, 0, v->pyramid.bytes, c->stream) != hipSuccess
         || hipMemsetAsync(v->counters.p, 0, v->counters.bytes, c->stream) != hipSuccess
         || hipMemsetAsync(v->color.p, 0, v->color.bytes, c->stream) != hipSuccess
-        || hipMemsetAsync(v->bin_count.p, 0, v->bin_count.bytes, c->stream) != hipSuccess) {
+        || hipMemsetAsync(v->bin_count.p, 0, v->bin_count.bytes, c->stream) != hipSuccess
+        || hipMemsetAsync(v->bin_start.p, 0, v->bin_start.bytes, c->stream) != hipSuccess) {
         vkgv_view_destroy(v);
         return fail(c, VKGV_ERROR_HIP, "view_create: clear failed");
     }

@@ -225,7 +225,7 @@ struct ViewDev {
     uint32_t clip_capacity;
     uint32_t tiles_x, tiles_y;      // kBinTile screen tiles
     uint32_t* bin_count;            // per tile; zero between passes
-    uint32_t* bin_start;            // per tile
+    uint32_t* bin_start;            // per tile: list start, or the fused fill's cursor; zero between passes
     uint32_t* bins;                 // big-triangle indices, grouped by tile
     uint32_t bin_capacity;          // also the capacity of `pairs` (64 per chunk)
     TileItem* tile_items;           // tiles * kTileSplit
@@ -293,6 +293,7 @@ struct BatchGrid {
     uint32_t clear_blocks, hiz_blocks, resolve_blocks;
     uint32_t max_tail_levels;       // any view needs the one-block Hi-Z tail
     uint32_t max_draws;
+    uint32_t max_bin_tiles;         // most binned-path screen tiles of one view
     uint32_t hiz_level_blocks[kHizMaxLevels];   // per level: 256-texel blocks of the largest view between tiles and tail
 };
 

@@ -902,7 +902,7 @@ __device__ __forceinline__ void bin_view(const BigArgs& a)
 // store is coalesced: writes the per-tile list starts, zeroes the counts for
 // the fill pass (which counts again) and writes the work items.
 constexpr uint32_t kScanThreads = 1024;
-constexpr uint32_t kScanChunk = 4096;
+constexpr uint32_t kScanChunk = 8192;   // a 4K view's 8160 tiles in one round (96 KB of LDS)
 constexpr uint32_t kScanPer = kScanChunk / kScanThreads;
 
 __device__ __forceinline__ void bin_scan_view(const BigArgs& a, uint32_t item_tris)
@@ -955,7 +955,7 @@ __device__ __forceinline__ void bin_scan_view(const BigArgs& a, uint32_t item_tr
         __syncthreads();
         for (uint32_t i = tid; i < m; i += kScanThreads) {
             const uint32_t n = s_cnt[i], st = s_start[i], sp = tile_split(n, item_tris);
-            a.bin_start[c0 + i] = st;
+            a.bin_start[c0 + i] = n ? st : 0u;   // zero where no pair lands: the fused fill's cursors
             a.bin_count[c0 + i] = 0u;   // the fill pass counts again; the tile pass leaves it zero
             for (uint32_t p = 0; p < sp; ++p) {
                 TileItem it;
@@ -998,6 +998,110 @@ __device__ __forceinline__ void bin_fill_view(const BigArgs& a)
     }
 }
 
+// Scan and fill in one pass, for views of at most kScanChunk tiles: every
+// block scans the tile counts itself (redundant work of a few microseconds
+// per block, in place of a launch and a one-block pass), writes its share of
+// the work items, and scatters its pairs with bin_start as per-tile cursors
+// (zero between passes: the tile pass clears the cursors of the tiles it ran,
+// the others never move).  Thread t holds the counts of tiles
+// [kFillPer t, kFillPer (t + 1)) in registers (16-byte loads); only the list
+// starts go to LDS, for the pairs' lookups.  The count array stays as binned.
+constexpr uint32_t kFillThreads = 256;
+constexpr uint32_t kFillPer = kScanChunk / kFillThreads;   // tiles per thread in the scan
+static_assert(kScanChunk % (4 * kFillThreads) == 0, "the fused scan loads whole uint4 runs per thread");
+
+// Tile i of the fused scan's LDS starts: one padding word per 32 keeps each
+// thread's run of kFillPer tiles on distinct banks.
+__device__ __forceinline__ uint32_t fill_lds(uint32_t i) { return i + (i >> 5); }
+constexpr uint32_t kFillLds = kScanChunk + kScanChunk / 32;
+
+__device__ __forceinline__ void bin_fill_scan_view(const BigArgs& a, uint32_t item_tris, uint32_t* s_start)
+{
+    __shared__ uint32_t s_wsum[kFillThreads / 64], s_wisum[kFillThreads / 64], s_wmax[kFillThreads / 64];
+    const uint32_t tid = threadIdx.x, lane = tid & 63u, w = tid >> 6;
+    uint32_t nbig, nchunks;
+    pass_extent(a, nbig, nchunks);
+    if (nchunks == 0) {   // no big triangle this pass (uniform over the grid)
+        if (blockIdx.x == 0 && tid == 0) a.ctr->tile_items = 0u;
+        return;
+    }
+    const uint32_t ntiles = a.tiles_x * a.tiles_y;   // at most kScanChunk (checked at launch)
+    const uint32_t b0 = tid * kFillPer;
+    uint32_t cnt[kFillPer];
+#pragma unroll
+    for (uint32_t q = 0; q < kFillPer; q += 4) {
+        if (b0 + q + 4 <= ntiles) {
+            const uint4 v4 = *reinterpret_cast<const uint4*>(a.bin_count + b0 + q);
+            cnt[q] = v4.x; cnt[q + 1] = v4.y; cnt[q + 2] = v4.z; cnt[q + 3] = v4.w;
+        } else {
+#pragma unroll
+            for (uint32_t r = 0; r < 4; ++r) cnt[q + r] = b0 + q + r < ntiles ? a.bin_count[b0 + q + r] : 0u;
+        }
+    }
+    uint32_t sum = 0, isum = 0, most = 0;
+#pragma unroll
+    for (uint32_t u = 0; u < kFillPer; ++u) {
+        sum += cnt[u];
+        isum += tile_split(cnt[u], item_tris);
+        most = max(most, cnt[u]);
+    }
+    uint32_t x = sum, ix = isum;
+#pragma unroll
+    for (int off = 1; off < 64; off <<= 1) {
+        const uint32_t y = __shfl_up(x, off), iy = __shfl_up(ix, off);
+        if (lane >= (uint32_t)off) { x += y; ix += iy; }
+    }
+#pragma unroll
+    for (int off = 32; off >= 1; off >>= 1) most = max(most, __shfl_xor(most, off));
+    if (lane == 63u) { s_wsum[w] = x; s_wisum[w] = ix; }
+    if (lane == 0u) s_wmax[w] = most;
+    __syncthreads();
+    uint32_t wbase = 0, iwbase = 0, total = 0, itotal = 0;
+#pragma unroll
+    for (uint32_t q = 0; q < kFillThreads / 64; ++q) {
+        if (q < w) { wbase += s_wsum[q]; iwbase += s_wisum[q]; }
+        total += s_wsum[q];
+        itotal += s_wisum[q];
+        most = max(most, s_wmax[q]);
+    }
+    // list starts to LDS; work items of this thread's tiles from one block each
+    const bool items = tid % gridDim.x == blockIdx.x;
+    uint32_t run = wbase + x - sum, irun = iwbase + ix - isum;
+#pragma unroll
+    for (uint32_t u = 0; u < kFillPer; ++u) {
+        const uint32_t n = cnt[u], sp = tile_split(n, item_tris);
+        s_start[fill_lds(b0 + u)] = run;
+        if (items)
+            for (uint32_t p = 0; p < sp; ++p) {
+                TileItem it;
+                it.code = (b0 + u) | (p << 24) | ((sp - 1u) << 28);
+                it.start = run;
+                it.n = n;
+                it.pad = 0u;
+                a.items[irun + p] = it;
+            }
+        run += n;
+        irun += sp;
+    }
+    if (blockIdx.x == 0 && tid == 0) {
+        atomicMax(&a.ctr->max_tile_tris, most);
+        a.ctr->bin_total = total;
+        a.ctr->tile_items = itotal;
+        a.ctr->big_total_tris += nbig;
+        a.ctr->big_total_pairs += total;
+    }
+    __syncthreads();
+    // the pairs, grid-stride
+    const unsigned long long np = (unsigned long long)nchunks * 64ull;   // at most the pair capacity
+    for (uint32_t i = blockIdx.x * kFillThreads + tid; i < (uint32_t)np; i += gridDim.x * kFillThreads) {
+        const BinPair p = a.pairs[i];
+        if (p.tile == 0xFFFFFFFFu) continue;
+        const uint32_t slot = s_start[fill_lds(p.tile)] + atomicAdd(&a.bin_start[p.tile], 1u);
+        a.bins[slot] = p.tri;
+    }
+    __syncthreads();   // the LDS starts are reused by the next view
+}
+
 // Tile pass: each wave runs a grid-stride loop over the halves (32x16
 // pixels) of the work items.  Item (tile, part, S) rasterizes triangles
 // [n*part/S, n*(part+1)/S) of the tile's list of n; a half keeps its keys in
@@ -1032,7 +1136,10 @@ __device__ __forceinline__ void tile_view(const BigArgs& a)
         const uint32_t start = __builtin_amdgcn_readfirstlane(item.start);
         const uint32_t tile = code & 0xFFFFFFu, part = (code >> 24) & 15u, split = (code >> 28) + 1u;
         const uint32_t b = start + n_all * part / split, e = start + n_all * (part + 1u) / split;
-        if (part == 0 && hv % kTileParts == 0 && lane == 0) a.bin_count[tile] = 0u;
+        if (part == 0 && hv % kTileParts == 0 && lane == 0) {
+            a.bin_count[tile] = 0u;
+            a.bin_start[tile] = 0u;   // the fused fill's cursor
+        }
         const int px = (int)((tile % a.tiles_x) * kBinTile) + lx;
         const int py0 = (int)((tile / a.tiles_x) * kBinTile + (hv % kTileParts) * (kBinTile / kTileParts)) + ly;
         const bool col_in = px < (int)a.W;
@@ -1350,6 +1457,21 @@ __global__ __launch_bounds__(256) void bin_fill_kernel(BatchDev b)
     }
 }
 
+__global__ __launch_bounds__(kFillThreads) void bin_fill_scan_kernel(BatchDev b, uint32_t item_tris)
+{
+    __shared__ uint32_t s_start[kFillLds];
+    for (uint32_t v0 = 0; v0 < b.nviews; v0 += 64u) {
+        const uint32_t n = lane_view_value(b, v0, big_count);
+        const uint32_t m = min(64u, b.nviews - v0);
+        for (uint32_t j = 0; j < m; ++j)
+            if (__builtin_amdgcn_readlane(n, j)) {
+                bin_fill_scan_view(big_args(batch_view(b, v0 + j)), item_tris, s_start);
+            } else if (blockIdx.x == 0 && threadIdx.x == 0) {
+                batch_view(b, v0 + j).counters->tile_items = 0u;   // no big triangle in this view
+            }
+    }
+}
+
 // Wave 0 also clears every view's queue counters for the next pass (their
 // last reader, the fill pass, has finished).
 template <bool STATS>
@@ -1400,8 +1522,12 @@ void launch_raster_big(const SceneDev& s, const BatchDev& b, const BatchGrid& g,
     if (stats) clip_kernel<true><<<std::max(64u, tile_grid / 4u), kClipThreads, 0, st>>>(b, ra);
     else clip_kernel<false><<<std::max(64u, tile_grid / 4u), kClipThreads, 0, st>>>(b, ra);
     bin_kernel<<<1024, 256, 0, st>>>(b);
-    bin_scan_kernel<<<g.nviews, kScanThreads, 0, st>>>(b, tile_item_tris);
-    bin_fill_kernel<<<256, 256, 0, st>>>(b);
+    if (g.max_bin_tiles <= kScanChunk) {
+        bin_fill_scan_kernel<<<256, kFillThreads, 0, st>>>(b, tile_item_tris);
+    } else {
+        bin_scan_kernel<<<g.nviews, kScanThreads, 0, st>>>(b, tile_item_tris);
+        bin_fill_kernel<<<256, 256, 0, st>>>(b);
+    }
     if (stats) tile_kernel<true><<<tile_grid, 256, 0, st>>>(b);
     else tile_kernel<false><<<tile_grid, 256, 0, st>>>(b);
 }
