@@ -14,8 +14,8 @@
   edge nearly parallel to a pixel pair, so rounding the clip positions to
   float32 alone moves the exact gradient by about 1e-4 in norm;
 - an optimisation of positions and albedo towards the target lowers the loss.
-The configuration is the subdivision-6 icosphere (40,962 vertices) as 8
-instances, 8 ring cameras at 512 x 512, and a target mesh perturbed by
+The configuration is the frequency-71 geodesic icosphere (50,412 vertices)
+as 8 instances, 8 ring cameras at 512 x 512, and a target mesh perturbed by
 N(0, 0.01) on positions and N(0, 0.05) on albedo (L2 loss).
 """
 import numpy as np
@@ -80,7 +80,7 @@ _cases = {}
 
 def c4_case(precision):
     if precision not in _cases:
-        print(f"case: C4 icosphere L6, 8 instances, 8 cameras, 512x512, {precision}", flush=True)
+        print(f"case: C4 geodesic icosphere, 8 instances, 8 cameras, 512x512, {precision}", flush=True)
         c = Case(8, 8, 512, precision)
         print(f"c4 {precision} forward+backward (first call) {c.ms:.2f} ms", flush=True)
         _cases[precision] = c
@@ -89,7 +89,7 @@ def c4_case(precision):
 
 @pytest.fixture(scope="module")
 def multi():
-    print("case: icosphere L6, 3 instances, 4 cameras, 256x256, mixed", flush=True)
+    print("case: geodesic icosphere, 3 instances, 4 cameras, 256x256, mixed", flush=True)
     return Case(3, 4, 256, "mixed")
 
 
@@ -301,7 +301,7 @@ def test_fused_shade_matches_separate_functions():
     """shade() is antialias(interpolate()): the same image bit for bit, and
     gradients equal to float64 rounding (one vertex pass sums both stages'
     terms instead of autograd adding two clip gradients)."""
-    print("case: icosphere L6, 2 instances, 3 cameras, 128x128, float64, shade vs separate", flush=True)
+    print("case: geodesic icosphere, 2 instances, 3 cameras, 128x128, float64, shade vs separate", flush=True)
     s = D.c4_scene(instances=2, cameras=3, size=128)
     clip = D.clip_positions(_t(s.pos).double(), _t(s.xforms).double(), _t(s.view_projs).double())
     e = D.ext()
@@ -324,7 +324,7 @@ def test_fused_shade_matches_separate_functions():
 
 
 def test_gathered_gradients_are_deterministic():
-    print("case: icosphere L6, 2 instances, 3 cameras, 128x128, mixed, backward twice", flush=True)
+    print("case: geodesic icosphere, 2 instances, 3 cameras, 128x128, mixed, backward twice", flush=True)
     case = Case(2, 3, 128)
     first = [g.copy() for g in case.grads]
     case.pos.grad = None

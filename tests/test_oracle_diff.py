@@ -20,7 +20,7 @@ from diffcheck import (Problem, analytic, central_differences, choose_params, cl
 @pytest.fixture(scope="module")
 def small():
     from vk_gltf_viewer_amd.diff import c4_scene, c4_target
-    s = c4_scene(instances=3, cameras=3, size=96)
+    s = c4_scene(instances=3, cameras=3, size=96, frequency=40)
     tp, ta = c4_target(s)
     cl = clip32(s.pos, s.xforms, s.view_projs)
     ids, dep = oracle.diff_raster(cl, s.tris, 96, 96)
@@ -151,3 +151,23 @@ def test_backward_matches_central_differences(small, aa):
     assert err.max() < 1e-6, (worst, pick(g, worst[0], worst[1]))
     # the parameters reach the loss: most of them have a clearly non-zero gradient
     assert reaching_fraction(fd, g, 1e-5) > 0.8
+
+
+@pytest.mark.parametrize("freq", [1, 2, 7, 71])
+def test_geodesic_sphere_is_a_closed_outward_icosphere(freq):
+    """The differentiable configuration's mesh: 10 f^2 + 2 unit vertices,
+    20 f^2 triangles, every edge shared by exactly two triangles in opposite
+    directions (closed, consistently wound), every face normal outward, and
+    at f = 1 the icosahedron (all edges of one length)."""
+    from vk_gltf_viewer_amd.diff import geodesic_sphere
+    pos, tris = geodesic_sphere(freq)
+    assert pos.shape == (10 * freq * freq + 2, 3) and tris.shape == (20 * freq * freq, 3)
+    assert np.allclose(np.linalg.norm(pos, axis=1), 1.0, atol=1e-12)
+    e = tris[:, [0, 1, 1, 2, 2, 0]].reshape(-1, 2)
+    directed = set(map(tuple, e))
+    assert len(directed) == len(e) and all((b, a) in directed for a, b in directed)
+    a, b, c = pos[tris[:, 0]], pos[tris[:, 1]], pos[tris[:, 2]]
+    assert (np.einsum("ij,ij->i", np.cross(b - a, c - a), a + b + c) > 0).all()
+    if freq == 1:
+        lengths = np.linalg.norm(pos[e[:, 0]] - pos[e[:, 1]], axis=1)
+        assert np.ptp(lengths) < 1e-12

@@ -13,6 +13,7 @@ buffer.  The definitions are in vk_gltf_viewer_amd/csrc/diff/diff.hip.
 from __future__ import annotations
 
 import ctypes
+import functools
 import importlib.machinery
 import importlib.util
 import math
@@ -103,23 +104,76 @@ def camera_matrix(view) -> np.ndarray:
     return np.asarray(cam.view_projection, np.float32).reshape(4, 4).T.copy()
 
 
-def c4_scene(seed: int = 4, instances: int = 8, cameras: int = 8, size: int = 512) -> DiffScene:
-    """The differentiable configuration: the subdivision-6 icosphere (40,962
-    vertices, 81,920 triangles, radius 1) with its per-vertex albedo, as
-    `instances` copies on a circle of radius 1.1 (scale 0.45, seeded rotation
-    about y; one instance has the identity transform), seen by `cameras`
-    cameras on a ring of radius 3 at size x size."""
-    from .scene import Scene, view_desc
-    sc = Scene(4, seed).arrays()
-    m = sc.meshes[0]
+@functools.lru_cache(maxsize=4)
+def _geodesic_sphere(n: int):
+    return _build_geodesic_sphere(n)
+
+
+def geodesic_sphere(frequency: int):
+    """Unit geodesic icosphere of class I (copies of a cached build): see
+    _build_geodesic_sphere."""
+    pos, tris = _geodesic_sphere(int(frequency))
+    return pos.copy(), tris.copy()
+
+
+def _build_geodesic_sphere(frequency: int):
+    """Unit geodesic icosphere of class I: each icosahedron face cut into
+    frequency^2 triangles on its barycentric grid, the grid points projected
+    to the sphere.  10 f^2 + 2 vertices, 20 f^2 triangles, outward
+    counter-clockwise winding (a face's grid triangles keep its orientation).
+    Returns (pos [V, 3] float64, tris [T, 3] int32)."""
+    n = int(frequency)
+    t = (1.0 + 5.0 ** 0.5) / 2.0
+    ico = np.array([[-1, t, 0], [1, t, 0], [-1, -t, 0], [1, -t, 0], [0, -1, t], [0, 1, t],
+                    [0, -1, -t], [0, 1, -t], [t, 0, -1], [t, 0, 1], [-t, 0, -1], [-t, 0, 1]], np.float64)
+    faces = [(0, 11, 5), (0, 5, 1), (0, 1, 7), (0, 7, 10), (0, 10, 11), (1, 5, 9), (5, 11, 4), (11, 10, 2),
+             (10, 7, 6), (7, 1, 8), (3, 9, 4), (3, 4, 2), (3, 2, 6), (3, 6, 8), (3, 8, 9), (4, 9, 5),
+             (2, 4, 11), (6, 2, 10), (8, 6, 7), (9, 8, 1)]
+    index = {}
+    pos = []
+
+    def vertex(weights):
+        # a grid point is sum w_k ico[k] / n with integer weights; shared
+        # edge and corner points get one key from their nonzero weights
+        key = tuple(sorted((k, w) for k, w in weights.items() if w))
+        v = index.get(key)
+        if v is None:
+            p = sum(w * ico[k] for k, w in key) / n
+            v = index[key] = len(pos)
+            pos.append(p / np.linalg.norm(p))
+        return v
+
     tris = []
-    for ml in m.meshlets:
-        vo, to, tc = int(ml["vertex_offset"]), int(ml["triangle_offset"]), int(ml["triangle_count"])
-        loc = m.triangle_indices[to:to + 3 * tc].reshape(-1, 3).astype(np.int64)
-        tris.append(m.vertex_indices[vo + loc])
-    tris = np.concatenate(tris).astype(np.int32)
-    pos = m.vertices["position"].astype(np.float32)
-    albedo = (m.vertices["color"][:, :3].astype(np.float32) / 255.0) ** 2.2
+    for a, b, c in faces:
+        grid = {}
+        for i in range(n + 1):
+            for j in range(n + 1 - i):
+                w = {}
+                for k, x in ((a, n - i - j), (b, i), (c, j)):
+                    w[k] = w.get(k, 0) + x
+                grid[i, j] = vertex(w)
+        for i in range(n):
+            for j in range(n - i):
+                tris.append((grid[i, j], grid[i + 1, j], grid[i, j + 1]))
+                if i + j < n - 1:
+                    tris.append((grid[i + 1, j], grid[i + 1, j + 1], grid[i, j + 1]))
+    return np.array(pos), np.array(tris, np.int32)
+
+
+def c4_scene(seed: int = 4, instances: int = 8, cameras: int = 8, size: int = 512,
+             frequency: int = 71) -> DiffScene:
+    """The differentiable configuration: a 50k-vertex icosphere-derived mesh,
+    the frequency-71 geodesic icosphere (50,412 vertices, 100,820 triangles,
+    radius 1), with a seeded smooth per-vertex linear RGB albedo in
+    [0.1, 0.9], as `instances` copies on a circle of radius 1.1 (scale 0.45,
+    seeded rotation about y; one instance has the identity transform), seen
+    by `cameras` cameras on a ring of radius 3 at size x size."""
+    from .scene import view_desc
+    p64, tris = geodesic_sphere(frequency)
+    pos = p64.astype(np.float32)
+    ph = np.random.default_rng(seed + 1000).uniform(0.0, 2.0 * math.pi, (3, 3))
+    albedo = np.stack([0.5 + 0.4 * np.sin(3.0 * p64[:, 0] + ph[c, 0]) * np.cos(2.0 * p64[:, 1] + ph[c, 1])
+                       * np.cos(2.5 * p64[:, 2] + ph[c, 2]) for c in range(3)], 1).astype(np.float32)
     rng = np.random.default_rng(seed)
     xf = []
     for k in range(instances):
