@@ -95,6 +95,7 @@ struct vkgv_view {
     ViewDev dev{};
     DevBuf keys, color, pyramid, visible0, visible1, history, counters, big, clipq, bin_count, bin_start, bins;
     DevBuf setups, pairs, tri_cache, slot_albedo, slot_mat, draw_slot, big_chunk, chunk_tri, tile_items, redo;
+    DevBuf frag_stage, frag_jobs;    // deferred fragment runs of heavy meshlets
     DevBuf aa_mat;                   // per-pixel material indices of the differentiable frame (allocated on first use)
     DevBuf mvp_tab;                  // per-frame VP M table, kMvpStride floats per transform
     DevBuf pmvp_tab, motion;         // prevVP * previous M table and W*H motion vectors (allocated on first use)
@@ -1174,7 +1175,13 @@ int vkgv_view_create(vkgv_ctx* c, uint32_t width, uint32_t height, vkgv_view** o
     if (!rc) rc = dalloc(c, v->slot_mat, sizeof(uint32_t) * (size_t)c->cache_slots);
     if (!rc) rc = dalloc(c, v->chunk_tri, sizeof(uint32_t) * (size_t)(bin_cap / 64u));
     if (!rc) rc = dalloc(c, v->tile_items, sizeof(TileItem) * (size_t)ntiles * kTileSplit);
+    if (!rc) rc = dalloc(c, v->frag_stage, sizeof(FragStage) * (size_t)kFragStageCapacity);
+    if (!rc) rc = dalloc(c, v->frag_jobs, sizeof(FragJob) * (size_t)kFragJobCapacity);
     if (rc) { vkgv_view_destroy(v); return rc; }
+    d.frag_stage = static_cast<FragStage*>(v->frag_stage.p);
+    d.frag_jobs = static_cast<FragJob*>(v->frag_jobs.p);
+    d.stage_capacity = kFragStageCapacity;
+    d.job_capacity = kFragJobCapacity;
     d.chunk_tri = static_cast<uint32_t*>(v->chunk_tri.p);
     d.tile_items = static_cast<TileItem*>(v->tile_items.p);
     d.setups = static_cast<BigSetup*>(v->setups.p);
@@ -1237,7 +1244,7 @@ void vkgv_view_destroy(vkgv_view* v)
     for (DevBuf* b : { &v->keys, &v->color, &v->pyramid, &v->visible0, &v->visible1, &v->history, &v->counters, &v->big,
                        &v->clipq, &v->bin_count, &v->bin_start, &v->bins, &v->setups, &v->pairs, &v->tri_cache,
                        &v->slot_albedo, &v->slot_mat, &v->draw_slot, &v->big_chunk, &v->chunk_tri, &v->tile_items, &v->maps,
-                       &v->aa_mat, &v->mvp_tab, &v->pmvp_tab, &v->motion, &v->redo })
+                       &v->aa_mat, &v->mvp_tab, &v->pmvp_tab, &v->motion, &v->redo, &v->frag_stage, &v->frag_jobs })
         dfree(*b);
     for (auto& set : v->ev)
         for (auto& e : set)

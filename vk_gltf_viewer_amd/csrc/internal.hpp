@@ -196,11 +196,34 @@ struct Counters {
     unsigned long long fragments;       // covered pixel centres with 0 <= z <= 1 (visibility-buffer updates)
     unsigned long long frag_atomics;    // global 64-bit atomicMin writes of those updates
     unsigned long long frag_stores;     // tile merges written with a plain read-min-store instead
-    unsigned long long pad2[4];
+    // deferred fragment runs of the current pass: staged meshlets << kJobCountBits | jobs reserved
+    unsigned long long frag_jobs;
+    unsigned long long pad2[3];
 };
 static_assert(sizeof(Counters) == 128, "Counters layout");
 
 constexpr uint32_t kTileSplit = 8;      // most work items sharing one screen tile's triangle list
+
+// Deferred fragment runs.  A meshlet whose small triangles cover more than
+// kJobKeep + kJobSlots / 2 pixel slots keeps the first kJobKeep for its own
+// raster wave and hands the rest, kJobSlots at a time, to the bin pass: its
+// triangle setups and slot prefix are staged in a FragStage record (an
+// opaque copy of the raster's LDS arrays, 124 x 52 B + 128 x 4 B + total),
+// and each job names the record and its first slot.
+constexpr uint32_t kJobKeep = 512;
+constexpr uint32_t kJobSlots = 256;
+constexpr uint32_t kFragStageWords = 1744;   // 16-byte multiple
+struct FragStage {
+    uint32_t w[kFragStageWords];
+};
+static_assert(sizeof(FragStage) % 16 == 0, "FragStage records are copied 16 bytes at a time");
+constexpr uint32_t kJobCountBits = 38;           // jobs field of Counters::frag_jobs (at most 2^25 meshlets x 140 jobs)
+constexpr uint32_t kFragStageCapacity = 1024;    // staged meshlets per view and pass (7.1 MB)
+constexpr uint32_t kFragJobCapacity = 1u << 15;  // their jobs per view and pass
+struct FragJob {
+    uint32_t stage;                 // FragStage record, ~0u: reserved but unused
+    uint32_t first;                 // first pixel slot of the run
+};
 // Resolve block: 64 columns x (4 * kResolveRows) rows; each thread resolves
 // kResolveRows pixels of its column, their loads issued together.
 constexpr uint32_t kResolveRows = 4;
@@ -231,6 +254,9 @@ struct ViewDev {
     TileItem* tile_items;           // tiles * kTileSplit
     BigSetup* setups;               // per big triangle
     BinPair* pairs;
+    FragStage* frag_stage;          // deferred fragment runs: staged meshlets
+    FragJob* frag_jobs;             // and their jobs
+    uint32_t stage_capacity, job_capacity;
     // resolve cache: triangle t of the meshlet in visible-list slot s is
     // tri_cache[s * 128 + t]; draw_slot[d] is the slot of draw d this frame
     TriCache* tri_cache;

@@ -126,6 +126,9 @@ struct RasterArgs {
     uint32_t* slot_mat;
     uint32_t* draw_slot;
     uint32_t cache_slots;
+    FragStage* stage;               // deferred fragment runs of heavy meshlets
+    FragJob* jobs;
+    uint32_t stage_cap, job_cap;
 };
 
 __device__ __forceinline__ void wave_sync()
@@ -179,9 +182,13 @@ __device__ __forceinline__ void depth_plane(const int* x, const int* y, const fl
 }
 
 // Global 64-bit minimum without a returned value (the wave does not wait).
+// Issued through a global (address space 1) pointer: a flat atomic would also
+// count in the LDS counter, so the raster's next LDS read would wait for
+// every atomic in flight; a global one waits for none of them.
+typedef __attribute__((address_space(1))) unsigned long long global_u64;
 __device__ __forceinline__ void key_min(unsigned long long* p, unsigned long long key)
 {
-    (void)__hip_atomic_fetch_min(p, key, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    (void)__hip_atomic_fetch_min((global_u64*)p, key, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
 // No early depth read: the wave does not wait on memory inside the pixel loop.
@@ -483,6 +490,71 @@ __device__ __forceinline__ uint32_t tri_byte(const uint32_t* words, uint32_t b)
     return (words[b >> 2] >> ((b & 3u) * 8u)) & 0xFFu;
 }
 
+// Words of a FragStage record: the staged triangles, then the 128-entry slot
+// prefix, then the slot total.
+constexpr uint32_t kStageTriWords = VKGV_MESHLET_TRIANGLES * (uint32_t)(sizeof(SmallTri) / 4);
+constexpr uint32_t kStageTri4 = kStageTriWords / 4;                 // the triangles as 16-byte words
+static_assert(kStageTriWords % 4 == 0, "the staged triangles are copied 16 bytes at a time");
+static_assert(kStageTriWords + 128u + 4u <= kFragStageWords, "a staged meshlet fits its record");
+
+// Pixel slots [k, kend) of a meshlet's small triangles (tri[] staged setups,
+// pre[] the 128-entry exclusive prefix of their bounding-box pixel counts,
+// `total` their sum), walked by one lane: one search finds the triangle of
+// the first slot, then the lane walks the run with integer edge steps and
+// moves to the next triangle when the run crosses a triangle's start (no
+// search per pixel).  Covered centres with 0 <= z <= 1 update the
+// visibility buffer with a 64-bit atomicMin.
+template <bool STATS>
+__device__ __forceinline__ void walk_small(const SmallTri* tri, const uint32_t* pre, uint32_t total, uint32_t k,
+                                           uint32_t kend, unsigned long long* keys, uint32_t W, RasterCount<STATS>& rc)
+{
+    if (k >= kend) return;
+    uint32_t lo = 0, hi = 127;   // last triangle whose first slot is <= k
+    while (lo < hi) {
+        const uint32_t mid = (lo + hi + 1u) >> 1;
+        if (pre[mid] <= k) lo = mid; else hi = mid - 1u;
+    }
+    uint32_t next = lo < 127u ? pre[lo + 1u] : total;
+    SmallTriR st = load_small(tri[lo]);
+    uint32_t bw = st.bw;
+    const uint32_t r = k - pre[lo];
+    int jj = (int)(r / bw), ii = (int)(r - (uint32_t)jj * bw);
+    int e0 = st.C[0] + st.A[0] * ii + st.B[0] * jj;
+    int e1 = st.C[1] + st.A[1] * ii + st.B[1] * jj;
+    int e2 = st.C[2] + st.A[2] * ii + st.B[2] * jj;
+    for (;;) {
+        if ((e0 | e1 | e2) >= 0) {
+            const float z = st.z0 + (st.za * (st.oxf + (float)ii) + st.zb * (st.oyf + (float)jj));
+            if (z >= 0.0f && z <= 1.0f) {
+                write_fragment(keys, W, (int)(st.org & 0xFFFFu) + ii, (int)(st.org >> 16) + jj, z, st.id);
+                rc.frag_atomic();
+            }
+        }
+        if (++k >= kend) break;
+        if (k == next) {   // first slot of a later triangle (empty ones skipped)
+            while (lo < 127u && pre[lo + 1u] <= k) ++lo;
+            next = lo < 127u ? pre[lo + 1u] : total;
+            st = load_small(tri[lo]);
+            bw = st.bw;
+            ii = 0;
+            jj = 0;
+            e0 = st.C[0];
+            e1 = st.C[1];
+            e2 = st.C[2];
+        } else if (++ii == (int)bw) {   // next row of the bounding box
+            ii = 0;
+            ++jj;
+            e0 = st.C[0] + st.B[0] * jj;
+            e1 = st.C[1] + st.B[1] * jj;
+            e2 = st.C[2] + st.B[2] * jj;
+        } else {
+            e0 += st.A[0];
+            e1 += st.A[1];
+            e2 += st.A[2];
+        }
+    }
+}
+
 // One view's share of the meshlet raster (wave w of the grid takes meshlets
 // w, w + waves, ... of the view's list).
 template <bool STATS>
@@ -492,8 +564,8 @@ __device__ __forceinline__ void raster_view(const RasterArgs& a)
     __shared__ float4 s_clip[kWaves][64];
     __shared__ SVtx s_sv[kWaves][64];
     __shared__ uint32_t s_tri[kWaves][96];
-    __shared__ SmallTri s_st[kWaves][VKGV_MESHLET_TRIANGLES];
-    __shared__ uint32_t s_pre[kWaves][128];
+    __shared__ __align__(16) SmallTri s_st[kWaves][VKGV_MESHLET_TRIANGLES];
+    __shared__ __align__(16) uint32_t s_pre[kWaves][128];
 
     const int w = threadIdx.x >> 6;
     const uint32_t lane = threadIdx.x & 63u;
@@ -664,58 +736,40 @@ __device__ __forceinline__ void raster_view(const RasterArgs& a)
         const uint32_t total = tot0 + tot1;
         wave_sync();
 
-        // Lane l takes the contiguous run of pixel slots [l * per, (l + 1) * per):
-        // one search finds the triangle of its first slot, then it walks the
-        // run with integer edge steps and moves to the next triangle when the
-        // run crosses a triangle's start (no search per pixel).
-        const uint32_t per = (total + 63u) >> 6;
-        uint32_t k = lane * per;
-        const uint32_t kend = min(k + per, total);
-        if (k < kend) {
-            uint32_t lo = 0, hi = 127;   // last triangle whose first slot is <= k
-            while (lo < hi) {
-                const uint32_t mid = (lo + hi + 1u) >> 1;
-                if (s_pre[w][mid] <= k) lo = mid; else hi = mid - 1u;
+        // A heavy meshlet keeps the first kJobKeep slots and defers the rest
+        // to the bin pass in runs of kJobSlots (staged record + jobs), so that
+        // its fragment atomics spread over many CUs instead of queueing in
+        // this wave's.  Without room for the record or its jobs the wave walks
+        // every slot itself.
+        uint32_t keep = total;
+        if (total > kJobKeep + kJobSlots / 2u) {   // wave-uniform
+            const uint32_t njobs = (total - kJobKeep + kJobSlots - 1u) / kJobSlots;
+            // one atomic reserves the record (high bits) and the jobs (low bits)
+            unsigned long long res = 0ull;
+            if (lane == 0) res = atomicAdd(&a.ctr->frag_jobs, (1ull << kJobCountBits) | (unsigned long long)njobs);
+            res = __shfl(res, 0);
+            const uint32_t sidx = (uint32_t)(res >> kJobCountBits);
+            const unsigned long long j0 = res & ((1ull << kJobCountBits) - 1ull);
+            const bool room = sidx < a.stage_cap && j0 + njobs <= (unsigned long long)a.job_cap;
+            // every reserved job inside the list is written, marked unused without room
+#pragma unroll 1
+            for (uint32_t q = lane; q < njobs; q += 64u)
+                if (j0 + q < (unsigned long long)a.job_cap)
+                    a.jobs[j0 + q] = room ? FragJob{ sidx, kJobKeep + q * kJobSlots } : FragJob{ 0xFFFFFFFFu, 0u };
+            if (room) {
+                uint4* dst = reinterpret_cast<uint4*>(a.stage[sidx].w);
+                const uint4* st4 = reinterpret_cast<const uint4*>(s_st[w]);
+#pragma unroll 2
+                for (uint32_t q = lane; q < kStageTri4; q += 64u) dst[q] = st4[q];
+                if (lane < 32u) dst[kStageTri4 + lane] = reinterpret_cast<const uint4*>(s_pre[w])[lane];
+                if (lane == 0) dst[kStageTri4 + 32u] = make_uint4(total, 0u, 0u, 0u);
+                keep = kJobKeep;
             }
-            uint32_t next = lo < 127u ? s_pre[w][lo + 1u] : total;
-            SmallTriR st = load_small(s_st[w][lo]);
-            uint32_t bw = st.bw;
-            const uint32_t r = k - s_pre[w][lo];
-            int jj = (int)(r / bw), ii = (int)(r - (uint32_t)jj * bw);
-            int e0 = st.C[0] + st.A[0] * ii + st.B[0] * jj;
-            int e1 = st.C[1] + st.A[1] * ii + st.B[1] * jj;
-            int e2 = st.C[2] + st.A[2] * ii + st.B[2] * jj;
-            for (;;) {
-                if ((e0 | e1 | e2) >= 0) {
-                    const float z = st.z0 + (st.za * (st.oxf + (float)ii) + st.zb * (st.oyf + (float)jj));
-                    if (z >= 0.0f && z <= 1.0f) {
-                        write_fragment(a.keys, a.W, (int)(st.org & 0xFFFFu) + ii, (int)(st.org >> 16) + jj, z, st.id);
-                        rc.frag_atomic();
-                    }
-                }
-                if (++k >= kend) break;
-                if (k == next) {   // first slot of a later triangle (empty ones skipped)
-                    while (lo < 127u && s_pre[w][lo + 1u] <= k) ++lo;
-                    next = lo < 127u ? s_pre[w][lo + 1u] : total;
-                    st = load_small(s_st[w][lo]);
-                    bw = st.bw;
-                    ii = 0;
-                    jj = 0;
-                    e0 = st.C[0];
-                    e1 = st.C[1];
-                    e2 = st.C[2];
-                } else if (++ii == (int)bw) {   // next row of the bounding box
-                    ii = 0;
-                    ++jj;
-                    e0 = st.C[0] + st.B[0] * jj;
-                    e1 = st.C[1] + st.B[1] * jj;
-                    e2 = st.C[2] + st.B[2] * jj;
-                } else {
-                    e0 += st.A[0];
-                    e1 += st.A[1];
-                    e2 += st.A[2];
-                }
-            }
+        }
+        {
+            const uint32_t per = (keep + 63u) >> 6;
+            const uint32_t k = lane * per;
+            walk_small(s_st[w], s_pre[w], total, k, min(k + per, keep), a.keys, a.W, rc);
         }
         wave_sync();
         // the meshlet is redone by the direct path after this kernel
@@ -751,6 +805,9 @@ struct BigArgs {
     BigSetup* setups;
     BinPair* pairs;
     TileItem* items;
+    const FragStage* stage;
+    const FragJob* jobs;
+    uint32_t job_cap;
 };
 
 // One lane per queued triangle, in its own LDS polygon buffers; the fan
@@ -812,20 +869,56 @@ __device__ __forceinline__ uint32_t tile_split(uint32_t n, uint32_t item_tris)
     return min(kTileSplit, (n + item_tris - 1u) / item_tris);
 }
 
+// A deferred fragment run (one wave): lane l walks its share of the job's
+// kJobSlots slots, reading the staged triangles and slot prefix straight from
+// the record in global memory.  (Staging them through LDS measured slower:
+// 30 KB of static LDS made every launch of this 1024-block kernel about
+// 10 us longer, empty ones included.)
+template <bool STATS>
+__device__ __forceinline__ void frag_job(const BigArgs& a, uint32_t j, RasterCount<STATS>& rc)
+{
+    const uint32_t lane = threadIdx.x & 63u;
+    const FragJob job = a.jobs[j];
+    const uint32_t stage = __builtin_amdgcn_readfirstlane(job.stage);
+    const uint32_t first = __builtin_amdgcn_readfirstlane(job.first);
+    if (stage == 0xFFFFFFFFu) return;   // reserved by a meshlet that found no room
+    const uint32_t* rec = a.stage[stage].w;
+    const SmallTri* tri = reinterpret_cast<const SmallTri*>(rec);
+    const uint32_t* pre = rec + kStageTriWords;
+    const uint32_t total = __builtin_amdgcn_readfirstlane(rec[kStageTriWords + 128u]);
+    const uint32_t end = min(first + kJobSlots, total);
+    const uint32_t per = (end - first + 63u) >> 6;
+    const uint32_t k = first + lane * per;
+    walk_small(tri, pre, total, k, min(k + per, end), a.keys, a.W, rc);
+}
+
+// The bin pass's work items: the 64-tile chunks of the big triangles, then
+// the deferred fragment runs of heavy meshlets.  Waves are numbered
+// block-minor, so consecutive items (chunks of one triangle, runs of one
+// meshlet) run on different CUs.
+//
 // One wave per 64-tile chunk of a big triangle's bounding-box tiles.  The
 // wave of chunk 0 stores the triangle's rasterization setup.  Lane l tests
 // tile 64*chunk + l: a tile is skipped when one edge is negative at all four
 // corner pixel centres of the tile's part of the bounding box (no centre there
 // can be covered); kept tiles write pair 64*chunk + l and count themselves.
 // bin_scan_kernel then scans the per-tile counts.
+template <bool STATS>
 __device__ __forceinline__ void bin_view(const BigArgs& a)
 {
+    RasterCount<STATS> rc;
     const uint32_t w = threadIdx.x >> 6, lane = threadIdx.x & 63u;
-    const uint32_t gw = blockIdx.x * 4u + w, nw = gridDim.x * 4u;
+    const uint32_t gw = w * gridDim.x + blockIdx.x, nw = gridDim.x * 4u;
     uint32_t nbig, nchunks;
     pass_extent(a, nbig, nchunks);
-    if (nchunks == 0) return;   // uniform over the grid: no big triangle this pass
-    for (uint32_t c = gw; c < nchunks; c += nw) {
+    const uint32_t njobs =
+        (uint32_t)min(a.ctr->frag_jobs & ((1ull << kJobCountBits) - 1ull), (unsigned long long)a.job_cap);
+    if (nchunks == 0 && njobs == 0) return;   // uniform over the grid
+    for (uint32_t c = gw; c < nchunks + njobs; c += nw) {
+        if (c >= nchunks) {
+            frag_job<STATS>(a, c - nchunks, rc);
+            continue;
+        }
         // the chunk is in use iff its entry names a slot stored in this pass
         // whose range holds it (chunks reserved by a triangle that found no
         // room keep an entry of an earlier pass, or none); uniform over the wave
@@ -895,6 +988,7 @@ __device__ __forceinline__ void bin_view(const BigArgs& a)
         }
         a.pairs[64u * c + lane] = bp;
     }
+    rc.flush(a.ctr);
 }
 
 // Exclusive scans (one block) of the tile counts and of the tile pass's work
@@ -1244,6 +1338,9 @@ __device__ __forceinline__ BigArgs big_args(const ViewDev& v)
     a.setups = v.setups;
     a.pairs = v.pairs;
     a.items = v.tile_items;
+    a.stage = v.frag_stage;
+    a.jobs = v.frag_jobs;
+    a.job_cap = v.job_capacity;
     return a;
 }
 
@@ -1262,12 +1359,17 @@ struct RasterBatchArgs {
 // as one concatenated list: view j's meshlet i goes to wave
 // (i + meshlets of the views before j) mod waves, so every wave gets within
 // one meshlet of the average however the meshlets spread over the views.
+// Waves are numbered block-minor (wave w of block b is wave w * blocks + b),
+// so consecutive meshlets of a list, which are neighbours on screen and
+// similar in cost, run in different blocks and thus on different CUs and
+// XCDs: the fragment atomics of a cluster of heavy meshlets near the camera
+// do not queue in one CU's memory pipeline.
 // Three waves per SIMD: the register budget (168) the kernel fits without spilling.
 template <bool STATS>
 __global__ __launch_bounds__(kBlock, 3) void raster_kernel(RasterBatchArgs ab)
 {
     const uint32_t lane = threadIdx.x & 63u;
-    const uint32_t gw = blockIdx.x * kWaves + (threadIdx.x >> 6);
+    const uint32_t gw = (threadIdx.x >> 6) * gridDim.x + blockIdx.x;
     const uint32_t nwaves = gridDim.x * kWaves;
     uint32_t carry = 0;   // meshlets of the views before v0, mod nwaves
     for (uint32_t v0 = 0; v0 < ab.b.nviews; v0 += 64u) {
@@ -1319,6 +1421,10 @@ __global__ __launch_bounds__(kBlock, 3) void raster_kernel(RasterBatchArgs ab)
             a.slot_mat = v.slot_mat;
             a.draw_slot = v.draw_slot;
             a.cache_slots = min(v.cache_slots, ab.cache_slots);
+            a.stage = v.frag_stage;
+            a.jobs = v.frag_jobs;
+            a.stage_cap = v.stage_capacity;
+            a.job_cap = v.job_capacity;
             raster_view<STATS>(a);
         }
         carry = (carry + __builtin_amdgcn_readlane(incl, 63)) % nwaves;
@@ -1431,13 +1537,16 @@ __global__ __launch_bounds__(kClipThreads) void clip_kernel(BatchDev b, RedoArgs
     }
 }
 
+template <bool STATS>
 __global__ __launch_bounds__(256) void bin_kernel(BatchDev b)
 {
     for (uint32_t v0 = 0; v0 < b.nviews; v0 += 64u) {
-        const uint32_t n = lane_view_value(b, v0, big_count);
+        const uint32_t n = lane_view_value(b, v0, [](const Counters& c) {
+            return c.big_count | (c.frag_jobs != 0ull ? 1u : 0u);
+        });
         const uint32_t m = min(64u, b.nviews - v0);
         for (uint32_t j = 0; j < m; ++j)
-            if (__builtin_amdgcn_readlane(n, j)) bin_view(big_args(batch_view(b, v0 + j)));
+            if (__builtin_amdgcn_readlane(n, j)) bin_view<STATS>(big_args(batch_view(b, v0 + j)));
     }
 }
 
@@ -1486,6 +1595,7 @@ __global__ __launch_bounds__(256) void tile_kernel(BatchDev b)
             c->big_count = 0u;
             c->redo_count = 0u;
             c->clip_count = 0u;
+            c->frag_jobs = 0ull;
         }
         const uint32_t m = min(64u, b.nviews - v0);
         for (uint32_t j = 0; j < m; ++j)
@@ -1521,7 +1631,8 @@ void launch_raster_big(const SceneDev& s, const BatchDev& b, const BatchGrid& g,
     const bool stats = flags & VKGV_STATS;
     if (stats) clip_kernel<true><<<std::max(64u, tile_grid / 4u), kClipThreads, 0, st>>>(b, ra);
     else clip_kernel<false><<<std::max(64u, tile_grid / 4u), kClipThreads, 0, st>>>(b, ra);
-    bin_kernel<<<1024, 256, 0, st>>>(b);
+    if (stats) bin_kernel<true><<<1024, 256, 0, st>>>(b);
+    else bin_kernel<false><<<1024, 256, 0, st>>>(b);
     if (g.max_bin_tiles <= kScanChunk) {
         bin_fill_scan_kernel<<<256, kFillThreads, 0, st>>>(b, tile_item_tris);
     } else {
