@@ -212,17 +212,21 @@ typedef struct vkgv_frame_stats {
     uint32_t visible_early;          /* draws rasterized in the first (or only) pass */
     uint32_t visible_late;           /* draws rasterized in the late pass (two-pass) */
     uint32_t big_triangles;          /* triangles sent to the tiled path */
-    uint32_t big_tiles;              /* (32x32 screen tile, triangle) pairs they were binned to */
+    uint32_t big_tiles;              /* (32x32 screen tile, triangle) pairs they were binned to (frames
+                                        with VKGV_STATS; 0 otherwise) */
     uint32_t clipped_triangles;      /* triangles that needed near/guard-band clipping */
     uint32_t overflow;               /* non-zero if a queue was full: bit 0 big-triangle queue, bit 1 clip
-                                        queue, bit 2 tile-pair list.  The frame is still complete: those
-                                        triangles took the slower direct path (one lane per triangle) */
+                                        queue, bit 2 big-triangle chunk list, bit 3 a screen tile's
+                                        triangle list.  The frame is still complete: those triangles took
+                                        the slower direct path (the wave sweeps the triangle, or its part
+                                        in that tile) */
     uint32_t timed_frames;           /* frames averaged in the ms_* fields */
     /* per-pass GPU milliseconds (HIP events), averaged over the frames rendered
      * with VKGV_TIMING since the previous vkgv_view_read_stats (at most 64) */
     float    ms_clear, ms_cull, ms_raster, ms_hiz, ms_cull_late, ms_raster_late,
              ms_resolve, ms_pyramid, ms_total;
-    uint32_t max_tile_tris;          /* most big triangles binned to one 32x32 tile */
+    uint32_t max_tile_tris;          /* most big triangles binned to one 32x32 tile (frames with
+                                        VKGV_STATS; 0 otherwise) */
     /* raster counts of the last frame rendered with VKGV_STATS (0 otherwise) */
     uint64_t raster_triangles;       /* triangles past the back-face test set up for rasterization */
     uint64_t fragments;              /* covered pixel centres with 0 <= z <= 1: visibility-buffer updates */
@@ -237,6 +241,9 @@ typedef struct vkgv_create_info {
     int32_t  device;                 /* HIP device ordinal */
     uint32_t big_triangle_capacity;  /* per view and raster pass; 0 = default (2^18), at most 2^23 */
     uint32_t clip_triangle_capacity; /* per view and raster pass; 0 = default (2^16), at most 2^22 */
+    uint32_t tile_triangle_capacity; /* big triangles one 32x32 screen tile's list holds per raster pass;
+                                        0 = default (min(1024, 2^24 / tiles), at least 64); tiles x capacity
+                                        at most 2^28 */
 } vkgv_create_info;
 
 typedef struct vkgv_ctx  vkgv_ctx;

@@ -1,14 +1,15 @@
 """Full queues.  A triangle that finds the big-triangle queue, the clip queue
-or the tile-pair list full is rasterized by the direct path (its lane walks
-its bounding box), so the frame does not change: keys, colours and visible
-lists are bit-identical to those of a frame with room to spare, and to the
-oracle, while the overflow bits say which queue was full.
+or the chunk list full is rasterized by the direct path (the wave sweeps its
+bounding box), and one that finds a screen tile's list full is swept over
+that tile, so the frame does not change: keys, colours and visible lists are
+bit-identical to those of a frame with room to spare, and to the oracle,
+while the overflow bits say which queue was full.
 
 The scene: a field of spheres over a tiled ground plane that runs under and
 behind the camera, with the near plane cutting the nearest spheres and the
 ground (clipping), large ground tiles (big triangles), and, with the
-small-triangle extent at one pixel, more big triangles than the tile-pair
-list has chunks (640x480: 2^15)."""
+small-triangle extent at one pixel, more big triangles than the chunk list
+has chunks (640x480: 2^15)."""
 import numpy as np
 import pytest
 
@@ -104,6 +105,8 @@ def test_roomy_frame_matches_oracle(crowded):
     (ONE_PIXEL, {}, 0b100),
     (ONE_PIXEL, dict(big_triangle_capacity=40000, clip_triangle_capacity=2), 0b110),
     (ONE_PIXEL, dict(big_triangle_capacity=8, clip_triangle_capacity=1), 0b011),
+    (DEFAULT_TUNING, dict(tile_triangle_capacity=1), 0b1000),
+    (ONE_PIXEL, dict(tile_triangle_capacity=3), 0b1000),
 ])
 def test_full_queues_give_the_same_frame(crowded, tuning, caps, bits):
     print(f"tuning {tuning}, capacities {caps}", flush=True)
@@ -135,7 +138,7 @@ def test_full_queues_in_a_batch(crowded):
             v.close()
     finally:
         r.close()
-    r = Renderer(big_triangle_capacity=8, clip_triangle_capacity=1)
+    r = Renderer(big_triangle_capacity=8, clip_triangle_capacity=1, tile_triangle_capacity=1)
     try:
         r.load(crowded)
         views = [r.create_view(w, h) for w, h, _ in specs]

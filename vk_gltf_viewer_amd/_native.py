@@ -132,7 +132,8 @@ class Tuning(C.Structure):
 
 
 class CreateInfo(C.Structure):
-    _fields_ = [("device", C.c_int32), ("big_triangle_capacity", C.c_uint32), ("clip_triangle_capacity", C.c_uint32)]
+    _fields_ = [("device", C.c_int32), ("big_triangle_capacity", C.c_uint32), ("clip_triangle_capacity", C.c_uint32),
+                ("tile_triangle_capacity", C.c_uint32)]
 
 
 class ViewDesc(C.Structure):

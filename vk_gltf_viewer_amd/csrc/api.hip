@@ -80,6 +80,7 @@ struct vkgv_ctx {
     int64_t max_albedo_index = -1;   // largest texture handle a material names (-1: none)
     uint32_t big_capacity = 1u << 18;
     uint32_t clip_capacity = 1u << 16;
+    uint32_t tile_capacity = 0;      // per-tile big-triangle list; 0: by view size
     uint32_t cache_slots = 8192;     // resolve-cache meshlet slots per view
     uint64_t generation = 0;         // bumped when view tables must be rebuilt (draw lists reallocated)
     // Allocation fault injection for error-path tests: when VKGV_FAULT_ALLOC_AFTER=n
@@ -93,8 +94,8 @@ struct vkgv_ctx {
 struct vkgv_view {
     vkgv_ctx* ctx = nullptr;
     ViewDev dev{};
-    DevBuf keys, color, pyramid, visible0, visible1, history, counters, big, clipq, bin_count, bin_start, bins;
-    DevBuf setups, pairs, tri_cache, slot_albedo, slot_mat, draw_slot, big_chunk, chunk_tri, tile_items, redo;
+    DevBuf keys, color, pyramid, visible0, visible1, history, counters, big, clipq, bin_count, bins;
+    DevBuf setups, tri_cache, slot_albedo, slot_mat, draw_slot, big_chunk, chunk_tri, redo;
     DevBuf frag_stage, frag_jobs;    // deferred fragment runs of heavy meshlets
     DevBuf aa_mat;                   // per-pixel material indices of the differentiable frame (allocated on first use)
     DevBuf mvp_tab;                  // per-frame VP M table, kMvpStride floats per transform
@@ -393,7 +394,6 @@ int build_maps(vkgv_ctx* c, const std::vector<const ViewDev*>& views, DevBuf& ma
         for (uint64_t k = 0; k < nhiz; ++k) hiz.push_back(vi | (uint32_t)(k << 8));
         for (uint64_t k = 0; k < nres; ++k) resolve.push_back(vi | (uint32_t)(k << 8));
         if (v.hiz_tail_start < v.pyr.levels) g.max_tail_levels = std::max(g.max_tail_levels, v.pyr.levels);
-        g.max_bin_tiles = std::max(g.max_bin_tiles, v.tiles_x * v.tiles_y);
         for (uint32_t k = v.hiz_tile_levels; k < v.hiz_tail_start && k < kHizMaxLevels; ++k)
             g.hiz_level_blocks[k] = std::max(g.hiz_level_blocks[k], (uint32_t)(((uint64_t)v.pyr.w[k] * v.pyr.h[k] + 255) / 256));
     }
@@ -495,7 +495,11 @@ int vkgv_create(const vkgv_create_info* info, vkgv_ctx** out)
     }
     if (info && info->big_triangle_capacity) c->big_capacity = info->big_triangle_capacity;
     if (info && info->clip_triangle_capacity) c->clip_capacity = info->clip_triangle_capacity;
-    if (c->big_capacity > (1u << 23) || c->clip_capacity > (1u << 22)) { delete c; return VKGV_ERROR_LIMIT; }
+    if (info && info->tile_triangle_capacity) c->tile_capacity = info->tile_triangle_capacity;
+    if (c->big_capacity > (1u << 23) || c->clip_capacity > (1u << 22) || c->tile_capacity > (1u << 28)) {
+        delete c;
+        return VKGV_ERROR_LIMIT;
+    }
     int count = 0;
     if (hipGetDeviceCount(&count) != hipSuccess || count <= c->device || c->device < 0) {
         delete c;
@@ -1161,20 +1165,24 @@ int vkgv_view_create(vkgv_ctx* c, uint32_t width, uint32_t height, vkgv_view** o
     if (!rc) rc = dalloc(c, v->big_chunk, sizeof(uint32_t) * (size_t)c->big_capacity);
     const uint32_t tiles_x = (width + kBinTile - 1) / kBinTile, tiles_y = (height + kBinTile - 1) / kBinTile;
     const uint32_t ntiles = tiles_x * tiles_y;
-    // tile-pair list: 1024 pairs (16 chunks) per screen tile, 2^21 .. 2^24
-    // (at most 2^18 chunks, which keeps a wave's chunk sum within 32 bits)
-    const uint32_t bin_cap = std::min<uint32_t>(std::max<uint32_t>(1u << 21, 1024u * ntiles), 1u << 24);
+    // big-triangle chunk list: 16 chunks (1024 tile slots) per screen tile,
+    // 2^15 .. 2^18 chunks (which keeps a wave's chunk sum within 32 bits);
+    // per-tile triangle lists: min(1024, 2^24 / tiles), at least 64, unless set
+    const uint32_t chunk_cap = std::min<uint32_t>(std::max<uint32_t>(1u << 15, 16u * ntiles), 1u << 18);
+    const uint32_t tile_cap = c->tile_capacity ? c->tile_capacity
+                                               : std::max<uint32_t>(64u, std::min<uint32_t>(1024u, (1u << 24) / ntiles));
+    if ((uint64_t)tile_cap * ntiles > (1ull << 28)) {
+        vkgv_view_destroy(v);
+        return fail(c, VKGV_ERROR_LIMIT, "view_create: tile_triangle_capacity x screen tiles beyond 2^28");
+    }
     if (!rc) rc = dalloc(c, v->clipq, sizeof(ClipTri) * (size_t)c->clip_capacity);
-    if (!rc) rc = dalloc(c, v->bin_count, sizeof(uint32_t) * ntiles);
-    if (!rc) rc = dalloc(c, v->bin_start, sizeof(uint32_t) * ntiles);
-    if (!rc) rc = dalloc(c, v->bins, sizeof(uint32_t) * (size_t)bin_cap);
+    if (!rc) rc = dalloc(c, v->bin_count, sizeof(uint32_t) * 2u * ((ntiles + 3u) & ~3u));
+    if (!rc) rc = dalloc(c, v->bins, sizeof(uint32_t) * (size_t)tile_cap * ntiles);
     if (!rc) rc = dalloc(c, v->setups, sizeof(BigSetup) * (size_t)c->big_capacity);
-    if (!rc) rc = dalloc(c, v->pairs, sizeof(BinPair) * (size_t)bin_cap);
     if (!rc) rc = dalloc(c, v->tri_cache, sizeof(TriCache) * 128u * (size_t)c->cache_slots);
     if (!rc) rc = dalloc(c, v->slot_albedo, sizeof(float4) * (size_t)c->cache_slots);
     if (!rc) rc = dalloc(c, v->slot_mat, sizeof(uint32_t) * (size_t)c->cache_slots);
-    if (!rc) rc = dalloc(c, v->chunk_tri, sizeof(uint32_t) * (size_t)(bin_cap / 64u));
-    if (!rc) rc = dalloc(c, v->tile_items, sizeof(TileItem) * (size_t)ntiles * kTileSplit);
+    if (!rc) rc = dalloc(c, v->chunk_tri, sizeof(uint32_t) * (size_t)chunk_cap);
     if (!rc) rc = dalloc(c, v->frag_stage, sizeof(FragStage) * (size_t)kFragStageCapacity);
     if (!rc) rc = dalloc(c, v->frag_jobs, sizeof(FragJob) * (size_t)kFragJobCapacity);
     if (rc) { vkgv_view_destroy(v); return rc; }
@@ -1183,9 +1191,7 @@ int vkgv_view_create(vkgv_ctx* c, uint32_t width, uint32_t height, vkgv_view** o
     d.stage_capacity = kFragStageCapacity;
     d.job_capacity = kFragJobCapacity;
     d.chunk_tri = static_cast<uint32_t*>(v->chunk_tri.p);
-    d.tile_items = static_cast<TileItem*>(v->tile_items.p);
     d.setups = static_cast<BigSetup*>(v->setups.p);
-    d.pairs = static_cast<BinPair*>(v->pairs.p);
     d.tri_cache = static_cast<TriCache*>(v->tri_cache.p);
     d.slot_albedo = static_cast<float4*>(v->slot_albedo.p);
     d.slot_mat = static_cast<uint32_t*>(v->slot_mat.p);
@@ -1202,9 +1208,9 @@ int vkgv_view_create(vkgv_ctx* c, uint32_t width, uint32_t height, vkgv_view** o
     d.tiles_x = tiles_x;
     d.tiles_y = tiles_y;
     d.bin_count = static_cast<uint32_t*>(v->bin_count.p);
-    d.bin_start = static_cast<uint32_t*>(v->bin_start.p);
     d.bins = static_cast<uint32_t*>(v->bins.p);
-    d.bin_capacity = bin_cap;
+    d.tile_capacity = tile_cap;
+    d.chunk_capacity = chunk_cap;
     d.hiz_tile_levels = hiz_tile_levels(d.pyr);
     d.hiz_tail_start = hiz_tail_start(d.pyr, d.hiz_tile_levels);
     d.hiz_tiles_x = (d.pyr.w[0] + 31) / 32;
@@ -1218,8 +1224,7 @@ int vkgv_view_create(vkgv_ctx* c, uint32_t width, uint32_t height, vkgv_view** o
     if (hipMemsetAsync(v->pyramid.p, 0, v->pyramid.bytes, c->stream) != hipSuccess
         || hipMemsetAsync(v->counters.p, 0, v->counters.bytes, c->stream) != hipSuccess
         || hipMemsetAsync(v->color.p, 0, v->color.bytes, c->stream) != hipSuccess
-        || hipMemsetAsync(v->bin_count.p, 0, v->bin_count.bytes, c->stream) != hipSuccess
-        || hipMemsetAsync(v->bin_start.p, 0, v->bin_start.bytes, c->stream) != hipSuccess) {
+        || hipMemsetAsync(v->bin_count.p, 0, v->bin_count.bytes, c->stream) != hipSuccess) {
         vkgv_view_destroy(v);
         return fail(c, VKGV_ERROR_HIP, "view_create: clear failed");
     }
@@ -1242,8 +1247,8 @@ void vkgv_view_destroy(vkgv_view* v)
         if (v->last_stream) (void)hipStreamSynchronize(v->last_stream);
     }
     for (DevBuf* b : { &v->keys, &v->color, &v->pyramid, &v->visible0, &v->visible1, &v->history, &v->counters, &v->big,
-                       &v->clipq, &v->bin_count, &v->bin_start, &v->bins, &v->setups, &v->pairs, &v->tri_cache,
-                       &v->slot_albedo, &v->slot_mat, &v->draw_slot, &v->big_chunk, &v->chunk_tri, &v->tile_items, &v->maps,
+                       &v->clipq, &v->bin_count, &v->bins, &v->setups, &v->tri_cache,
+                       &v->slot_albedo, &v->slot_mat, &v->draw_slot, &v->big_chunk, &v->chunk_tri, &v->maps,
                        &v->aa_mat, &v->mvp_tab, &v->pmvp_tab, &v->motion, &v->redo, &v->frag_stage, &v->frag_jobs })
         dfree(*b);
     for (auto& set : v->ev)

@@ -222,6 +222,18 @@ __global__ __launch_bounds__(kBlock) void cull_kernel(CullArgs a)
     // the wave's index, uniform in the compiler's eyes: unit, view group and
     // view loop bounds then live in scalar registers and the loops branch on them
     const uint32_t wave = __builtin_amdgcn_readfirstlane((blockIdx.x * kBlock + threadIdx.x) >> 6);
+    if (a.mode == CULL_LATE && wave == 0) {
+        // the first raster pass's queue counters, zeroed for the late pass
+        // (their last reader, that pass's tile kernel, has finished)
+        for (uint32_t v = lane; v < a.b.nviews; v += 64u) {
+            Counters* c = batch_view(a.b, v).counters;
+            c->big_chunks = 0ull;
+            c->big_count = 0u;
+            c->redo_count = 0u;
+            c->clip_count = 0u;
+            c->frag_jobs = 0ull;
+        }
+    }
     for (uint32_t u = wave; u < a.units; u += nwaves) {
         const uint32_t chunk = u / a.groups, group = u - chunk * a.groups;
         const uint32_t base = chunk * 64u * (uint32_t)P;
@@ -351,7 +363,8 @@ __global__ __launch_bounds__(kBlock) void cull_kernel(CullArgs a)
 
 // Frame start: each block clears kClearKeysPerBlock keys of its view
 // (kClearKeysPerBlock / kBlock = 32 per thread, two per 16-byte store); the
-// view's first block also zeroes its counters.
+// view's first block also zeroes its counters, and the view's blocks zero the
+// per-tile triangle counts of both raster passes.
 static_assert(kClearKeysPerBlock % (2 * kBlock) == 0, "the unrolled clear stores whole key pairs per thread");
 __global__ __launch_bounds__(kBlock) void frame_start_kernel(SceneDev s, BatchDev b)
 {
@@ -360,6 +373,11 @@ __global__ __launch_bounds__(kBlock) void frame_start_kernel(SceneDev s, BatchDe
     const ViewDev& v = batch_view(b, vi);
     if (local == 0 && threadIdx.x < sizeof(Counters) / 4) reinterpret_cast<uint32_t*>(v.counters)[threadIdx.x] = 0u;
     const uint64_t n = (uint64_t)v.width * v.height;
+    {
+        const uint32_t nblk = (uint32_t)((n + kClearKeysPerBlock - 1) / kClearKeysPerBlock);
+        const uint32_t ncount = 2u * tile_count_stride(v);
+        for (uint32_t i = local * kBlock + threadIdx.x; i < ncount; i += nblk * kBlock) v.bin_count[i] = 0u;
+    }
     // this frame's VP M and det(M) of every transform, computed once per view
     // and read by the raster and the resolve (the same arithmetic they used
     // per meshlet and per pixel); the view's clear blocks share the entries

@@ -142,20 +142,6 @@ struct BigSetup {
 };
 static_assert(sizeof(BigSetup) == 80, "BigSetup layout");
 
-// A (screen tile, big triangle) pair found by the binning pass; tile ~0u
-// marks a bounding-box tile that the triangle cannot cover.
-struct BinPair {
-    uint32_t tile, tri;
-};
-
-// Work item of the tile pass: part `part` of `split` of screen tile `tile`
-// rasterizes triangles part, part + split, ... of the tile's list
-// bins[start .. start + n).  code = tile | part << 24 | (split - 1) << 28.
-struct TileItem {
-    uint32_t code, start, n, pad;
-};
-static_assert(sizeof(TileItem) == 16, "TileItem layout");
-
 // Data of one rasterized triangle kept for the resolve: with v_k the clip
 // (x, y, w) of vertex k, r[k] = v_{k+1} x v_{k+2} (the barycentric of vertex
 // k is proportional to r[k] . (xn, yn, 1)), and the packed RGBA8 colours.
@@ -182,14 +168,14 @@ struct Counters {
     uint32_t big_count;             // big-triangle slots reserved in the current raster pass
     uint32_t redo_count;            // meshlets of the current raster pass in `redo`
     uint32_t clip_count;            // triangles queued for clipping in the current pass
-    uint32_t bin_total;             // (tile, triangle) pairs binned in the current pass
+    uint32_t reserved1;
     uint32_t clipped;               // statistics over the frame
-    uint32_t overflow;              // slow path taken: bit 0 big queue full, bit 1 clip queue full, bit 2 pair list full
+    uint32_t overflow;              // slow path taken: bit 0 big queue, bit 1 clip queue, bit 2 chunk list, bit 3 a tile list full
     uint32_t reserved2;
-    uint32_t big_total_tris;
-    uint32_t big_total_pairs;
-    uint32_t max_tile_tris;         // most big triangles binned to one tile this frame
-    uint32_t tile_items;            // work items of the tile pass (TileItem)
+    uint32_t big_total_tris;        // big triangles over the frame's passes
+    uint32_t big_total_pairs;       // their (tile, triangle) pairs (frames with VKGV_STATS)
+    uint32_t max_tile_tris;         // most big triangles binned to one tile (frames with VKGV_STATS)
+    uint32_t reserved3;
     uint32_t pad;
     // raster statistics over the frame, counted only by frames with VKGV_STATS
     unsigned long long raster_tris;     // triangles past the back-face test that were set up for raster
@@ -202,7 +188,7 @@ struct Counters {
 };
 static_assert(sizeof(Counters) == 128, "Counters layout");
 
-constexpr uint32_t kTileSplit = 8;      // most work items sharing one screen tile's triangle list
+constexpr uint32_t kTileSplit = 8;      // most tile-pass parts sharing one screen tile's triangle list
 
 // Deferred fragment runs.  A meshlet whose small triangles cover more than
 // kJobKeep + kJobSlots / 2 pixel slots keeps the first kJobKeep for its own
@@ -247,13 +233,11 @@ struct ViewDev {
     ClipTri* clipq;
     uint32_t clip_capacity;
     uint32_t tiles_x, tiles_y;      // kBinTile screen tiles
-    uint32_t* bin_count;            // per tile; zero between passes
-    uint32_t* bin_start;            // per tile: list start, or the fused fill's cursor; zero between passes
-    uint32_t* bins;                 // big-triangle indices, grouped by tile
-    uint32_t bin_capacity;          // also the capacity of `pairs` (64 per chunk)
-    TileItem* tile_items;           // tiles * kTileSplit
+    uint32_t* bin_count;            // per pass and tile (pass p at p * tile_count_stride): triangles binned; zeroed at frame start
+    uint32_t* bins;                 // per tile: a list of tile_capacity big-triangle indices
+    uint32_t tile_capacity;         // triangles a tile's list holds (the rest take the direct path)
+    uint32_t chunk_capacity;        // 64-tile chunks of big triangles per pass
     BigSetup* setups;               // per big triangle
-    BinPair* pairs;
     FragStage* frag_stage;          // deferred fragment runs: staged meshlets
     FragJob* frag_jobs;             // and their jobs
     uint32_t stage_capacity, job_capacity;
@@ -274,6 +258,13 @@ struct ViewDev {
     uint32_t hiz_tiles_x;           // 32x32 level-0 tiles per row
     uint32_t resolve_blocks_x;      // 64-pixel resolve blocks per row
 };
+
+// Stride of a view's per-pass tile counts: the tile count rounded up to 4, so
+// that each pass's array starts 16-byte aligned.
+__host__ __device__ __forceinline__ uint32_t tile_count_stride(const ViewDev& v)
+{
+    return (v.tiles_x * v.tiles_y + 3u) & ~3u;
+}
 
 // Camera as kernels take it (a by-value copy of vkgv_camera).
 struct CameraArg {
@@ -319,7 +310,6 @@ struct BatchGrid {
     uint32_t clear_blocks, hiz_blocks, resolve_blocks;
     uint32_t max_tail_levels;       // any view needs the one-block Hi-Z tail
     uint32_t max_draws;
-    uint32_t max_bin_tiles;         // most binned-path screen tiles of one view
     uint32_t hiz_level_blocks[kHizMaxLevels];   // per level: 256-texel blocks of the largest view between tiles and tail
 };
 

@@ -19,10 +19,12 @@
 //                 LDS, back-face culls its <= 124 triangles (2 per lane) and
 //                 rasterizes the small ones cooperatively: a wave prefix sum
 //                 over their bounding-box pixel counts gives each lane an equal
-//                 run of pixel slots, walked with integer edge steps.  Large
-//                 triangles go to the big queue (a slot, and the 64-tile chunks
-//                 of the triangle's 32x32-pixel bounding-box tiles in the pair
-//                 list), triangles needing clipping to the clip queue.  A
+//                 run of pixel slots, walked with integer edge steps (a heavy
+//                 meshlet hands most of its slots to the bin pass as deferred
+//                 runs).  Large triangles go to the big queue (a slot, and the
+//                 64-tile chunks of the triangle's 32x32-pixel bounding-box
+//                 tiles in the chunk list), triangles needing clipping to the
+//                 clip queue.  A
 //                 meshlet with a triangle that found either full is listed for
 //                 the direct path.
 //  clip_kernel    clips the queued triangles; their fans join the big queue
@@ -32,14 +34,14 @@
 //                 box at a time, so that no triangle is lost when a queue is
 //                 full (slower, same result).
 //  bin_kernel     one wave per chunk: stores the triangle's setup (chunk 0),
-//                 tests 64 tiles, writes a (tile, triangle) pair per tile that
-//                 can hold a covered pixel centre and counts it in its tile.
-//  bin_scan       one block: scans the per-tile counts and cuts each tile's
-//                 list into up to kTileSplit work items of a tunable size.
-//  bin_fill       scatters the pairs into per-tile triangle lists.
-//  tile_kernel    grid-stride loop over the work items, each rasterizing its
-//                 share of a tile's triangles into a 32x32 key tile in LDS (LDS
-//                 atomicMin), then merging it into the visibility buffer.
+//                 tests 64 tiles and appends the triangle to the fixed-size
+//                 list of every tile that can hold a covered pixel centre (a
+//                 tile whose list is full gets the triangle swept directly).
+//                 Then the deferred fragment runs of heavy meshlets.
+//  tile_kernel    static work items (tile, 32x16 band, part of the tile's
+//                 list), each rasterizing its share of the tile's triangles
+//                 into keys held in registers, then merging them into the
+//                 visibility buffer.
 #include "internal.hpp"
 
 namespace vkgv {
@@ -253,18 +255,47 @@ __device__ __forceinline__ long long edge_at(const Edge& e, long long sx, long l
     return e.ex * (sy - e.ay) - e.ey * (sx - e.ax);
 }
 
-// Triangles that found a queue full: every lane with `want` set hands its
-// triangle to the whole wave, one triangle at a time, and the wave sweeps its
-// bounding box in 8x8-pixel blocks with the tile pass's coverage rule and
-// depth expression, so it writes exactly the fragments the binned path would.
-// Called by all 64 lanes of a wave.  Slower than the binned path, but no
-// triangle is lost.
+// A whole wave sweeps the pixel rectangle [rx0, rx1] x [ry0, ry1] of one
+// triangle (wave-uniform arguments: edges, snapped vertices, depth plane) in
+// 8x8-pixel blocks with the tile pass's coverage rule and depth expression,
+// writing its fragments with atomicMin, so it writes exactly the fragments
+// the tile pass would.
 template <bool STATS>
-__device__ void wave_sweep(bool want, const TriSetup& t, uint32_t id, unsigned long long* keys, uint32_t W,
+__device__ void sweep_rect(const Edge* e, const int* x, const int* y, const float* z, float za, float zb, uint32_t id,
+                           int rx0, int ry0, int rx1, int ry1, unsigned long long* keys, uint32_t W,
                            RasterCount<STATS>& rc)
 {
     const uint32_t lane = threadIdx.x & 63u;
     const int lx = (int)(lane & 7u), ly = (int)(lane >> 3);
+    for (int by = ry0; by <= ry1; by += 8) {
+        const int py = by + ly;
+        const long long sy = (long long)py * 256 + 128;
+        const float dyf = (float)(py * 256 + 128 - y[0]) * 0.00390625f;
+        for (int bx = rx0; bx <= rx1; bx += 8) {
+            const int px = bx + lx;
+            if (px > rx1 || py > ry1) continue;
+            const long long sx = (long long)px * 256 + 128;
+            bool in = true;
+#pragma unroll
+            for (int q = 0; q < 3; ++q) in = in && edge_at(e[q], sx, sy) - (e[q].tl ? 0 : 1) >= 0;
+            if (!in) continue;
+            const float dxf = (float)(px * 256 + 128 - x[0]) * 0.00390625f;
+            const float zz = z[0] + (za * dxf + zb * dyf);
+            if (!(zz >= 0.0f && zz <= 1.0f)) continue;
+            write_fragment(keys, W, px, py, zz, id);
+            rc.frag_atomic();
+        }
+    }
+}
+
+// Triangles that found a queue full: every lane with `want` set hands its
+// triangle to the whole wave, one triangle at a time, and the wave sweeps its
+// bounding box (sweep_rect).  Called by all 64 lanes of a wave.  Slower than
+// the binned path, but no triangle is lost.
+template <bool STATS>
+__device__ void wave_sweep(bool want, const TriSetup& t, uint32_t id, unsigned long long* keys, uint32_t W,
+                           RasterCount<STATS>& rc)
+{
     unsigned long long pending = __ballot(want);
     while (pending != 0ull) {
         const int j = __ffsll((long long)pending) - 1;
@@ -285,29 +316,11 @@ __device__ void wave_sweep(bool want, const TriSetup& t, uint32_t id, unsigned l
         edges_of(x, y, e);
         float za, zb;
         depth_plane(x, y, z, area, za, zb);
-        for (int by = py0; by <= py1; by += 8) {
-            const int py = by + ly;
-            const long long sy = (long long)py * 256 + 128;
-            const float dyf = (float)(py * 256 + 128 - y[0]) * 0.00390625f;
-            for (int bx = px0; bx <= px1; bx += 8) {
-                const int px = bx + lx;
-                if (px > px1 || py > py1) continue;
-                const long long sx = (long long)px * 256 + 128;
-                bool in = true;
-#pragma unroll
-                for (int q = 0; q < 3; ++q) in = in && edge_at(e[q], sx, sy) - (e[q].tl ? 0 : 1) >= 0;
-                if (!in) continue;
-                const float dxf = (float)(px * 256 + 128 - x[0]) * 0.00390625f;
-                const float zz = z[0] + (za * dxf + zb * dyf);
-                if (!(zz >= 0.0f && zz <= 1.0f)) continue;
-                write_fragment(keys, W, px, py, zz, tid);
-                rc.frag_atomic();
-            }
-        }
+        sweep_rect(e, x, y, z, za, zb, tid, px0, py0, px1, py1, keys, W, rc);
     }
 }
 
-// 64-tile chunks a big triangle reserves in the pair list: none when it
+// 64-tile chunks a big triangle reserves in the chunk list: none when it
 // needs more than the whole list (it is rasterized directly).
 __device__ __forceinline__ uint32_t reserve_chunks(const TriSetup& t, uint32_t chunk_cap)
 {
@@ -331,9 +344,9 @@ struct BigQueue {
 
 // Stores a big triangle in `slot` with its chunks from c0 (chunk_tri marks
 // them as the triangle's) and returns true.  Without room in the queue or in
-// the pair list it returns false, marks its slot empty (big_chunk ~0u; its
+// the chunk list it returns false, marks its slot empty (big_chunk ~0u; its
 // reserved chunks keep stale chunk_tri entries, which the bin pass rejects)
-// and sets overflow bit 0 (queue) or 2 (pair list): the frame takes the
+// and sets overflow bit 0 (queue) or 2 (chunk list): the frame takes the
 // slower direct path for the triangle.
 __device__ __forceinline__ bool store_big(const TriSetup& t, uint32_t id, uint32_t slot, unsigned long long c0,
                                           uint32_t chunks, const BigQueue& q)
@@ -367,7 +380,7 @@ __device__ __forceinline__ bool push_big(const TriSetup& t, uint32_t id, const B
 
 // Wave-wide append (called by all 64 lanes): a prefix sum of the lanes'
 // (1, chunks) pairs and one atomic per counter and wave reserve consecutive
-// slots and chunks in lane order.  A lane's chunk count is at most the pair
+// slots and chunks in lane order.  A lane's chunk count is at most the chunk
 // list's chunk capacity (below 2^19), so the 64-lane sums fit their 32-bit
 // halves.  Returns true on a lane whose triangle found no room.
 __device__ __forceinline__ bool push_big_wave(bool want, const TriSetup& t, uint32_t id, uint32_t lane,
@@ -798,13 +811,10 @@ struct BigArgs {
     unsigned long long* keys;
     uint32_t W, H;
     uint32_t tiles_x, tiles_y;
-    uint32_t* bin_count;
-    uint32_t* bin_start;
+    uint32_t* bin_count;            // this pass's per-tile counts
     uint32_t* bins;
-    uint32_t bin_cap;
+    uint32_t tile_cap;
     BigSetup* setups;
-    BinPair* pairs;
-    TileItem* items;
     const FragStage* stage;
     const FragJob* jobs;
     uint32_t job_cap;
@@ -855,18 +865,12 @@ __device__ __forceinline__ uint32_t chunks_of(const BigTri& t)
     return (ntx * nty + 63u) >> 6;
 }
 
-// Big-triangle slots of this pass (stored or not) and the chunks of the pair
+// Big-triangle slots of this pass (stored or not) and the chunks of the chunk
 // list in use.
 __device__ __forceinline__ void pass_extent(const BigArgs& a, uint32_t& nbig, uint32_t& nchunks)
 {
     nbig = min(a.ctr->big_count, a.big_cap);
     nchunks = (uint32_t)min(a.ctr->big_chunks, (unsigned long long)a.chunk_cap);
-}
-
-// Work items of a tile whose list holds n triangles (0 for an empty tile).
-__device__ __forceinline__ uint32_t tile_split(uint32_t n, uint32_t item_tris)
-{
-    return min(kTileSplit, (n + item_tris - 1u) / item_tris);
 }
 
 // A deferred fragment run (one wave): lane l walks its share of the job's
@@ -901,16 +905,17 @@ __device__ __forceinline__ void frag_job(const BigArgs& a, uint32_t j, RasterCou
 // wave of chunk 0 stores the triangle's rasterization setup.  Lane l tests
 // tile 64*chunk + l: a tile is skipped when one edge is negative at all four
 // corner pixel centres of the tile's part of the bounding box (no centre there
-// can be covered); kept tiles write pair 64*chunk + l and count themselves.
-// bin_scan_kernel then scans the per-tile counts.
+// can be covered); a kept tile appends the triangle to its list.
 template <bool STATS>
 __device__ __forceinline__ void bin_view(const BigArgs& a)
 {
     RasterCount<STATS> rc;
+    uint32_t pairs = 0u, most = 0u;   // (tile, triangle) pairs binned, longest tile list (STATS)
     const uint32_t w = threadIdx.x >> 6, lane = threadIdx.x & 63u;
     const uint32_t gw = w * gridDim.x + blockIdx.x, nw = gridDim.x * 4u;
     uint32_t nbig, nchunks;
     pass_extent(a, nbig, nchunks);
+    if (gw == 0 && lane == 0 && nbig) atomicAdd(&a.ctr->big_total_tris, nbig);
     const uint32_t njobs =
         (uint32_t)min(a.ctr->frag_jobs & ((1ull << kJobCountBits) - 1ull), (unsigned long long)a.job_cap);
     if (nchunks == 0 && njobs == 0) return;   // uniform over the grid
@@ -931,10 +936,7 @@ __device__ __forceinline__ void bin_view(const BigArgs& a)
             t = a.big[ti];
         }
         const bool used = known && c0 <= c && c - c0 < chunks_of(t);
-        if (!used) {
-            a.pairs[64u * c + lane] = BinPair{ 0xFFFFFFFFu, 0u };
-            continue;
-        }
+        if (!used) continue;
         const uint32_t j = c - c0;
         Edge e[3];
         edges_of(t.x, t.y, e);
@@ -965,9 +967,8 @@ __device__ __forceinline__ void bin_view(const BigArgs& a)
         const int ntx = tx1 - tx0 + 1;
         const uint32_t nt = (uint32_t)(ntx * (ty1 - ty0 + 1));
         const uint32_t k = 64u * j + lane;
-        BinPair bp;
-        bp.tile = 0xFFFFFFFFu;
-        bp.tri = ti;
+        bool keep = false;
+        uint32_t tile = 0u;
         if (k < nt) {
             const int ty = ty0 + (int)k / ntx, tx = tx0 + (int)k % ntx;
             const long long cx0 = (long long)max(tx * (int)kBinTile, px0) * 256 + 128;
@@ -981,149 +982,229 @@ __device__ __forceinline__ void bin_view(const BigArgs& a)
                 const long long m1 = max(edge_at(e[q], cx0, cy1), edge_at(e[q], cx1, cy1));
                 out = out || max(m0, m1) < 0;
             }
-            if (!out) {
-                bp.tile = (uint32_t)ty * a.tiles_x + (uint32_t)tx;
-                atomicAdd(&a.bin_count[bp.tile], 1u);
+            keep = !out;
+            tile = (uint32_t)ty * a.tiles_x + (uint32_t)tx;
+        }
+        // the triangle joins its tile's list, or, when the list is full, is
+        // swept over that tile right here (the tile pass, a later launch,
+        // merges its own list over these keys)
+        uint32_t pos = 0u;
+        if (keep) {
+            pos = atomicAdd(&a.bin_count[tile], 1u);
+            if (pos < a.tile_cap) a.bins[(size_t)tile * a.tile_cap + pos] = ti;
+        }
+        const bool spill = keep && pos >= a.tile_cap;
+        if constexpr (STATS) {
+            pairs += keep ? 1u : 0u;
+            most = max(most, keep ? pos + 1u : 0u);
+        }
+        if (__ballot(spill) != 0ull) {   // wave-uniform
+            if (lane == 0) atomicOr(&a.ctr->overflow, 8u);
+            const long long area = ((long long)t.x[1] - t.x[0]) * ((long long)t.y[2] - t.y[0])
+                                   - ((long long)t.y[1] - t.y[0]) * ((long long)t.x[2] - t.x[0]);
+            float za, zb;
+            depth_plane(t.x, t.y, t.z, area, za, zb);
+            unsigned long long pending = __ballot(spill);
+            while (pending != 0ull) {
+                const int sl = __ffsll((long long)pending) - 1;
+                pending &= pending - 1ull;
+                const uint32_t st = __shfl(tile, sl);
+                const int rx = (int)(st % a.tiles_x) * (int)kBinTile, ry = (int)(st / a.tiles_x) * (int)kBinTile;
+                sweep_rect(e, t.x, t.y, t.z, za, zb, t.id, max(rx, px0), max(ry, py0),
+                           min(rx + (int)kBinTile - 1, px1), min(ry + (int)kBinTile - 1, py1), a.keys, a.W, rc);
             }
         }
-        a.pairs[64u * c + lane] = bp;
+    }
+    if constexpr (STATS) {
+#pragma unroll
+        for (int off = 32; off >= 1; off >>= 1) {
+            pairs += __shfl_xor(pairs, off);
+            most = max(most, __shfl_xor(most, off));
+        }
+        if (lane == 0 && pairs) {
+            atomicAdd(&a.ctr->big_total_pairs, pairs);
+            atomicMax(&a.ctr->max_tile_tris, most);
+        }
     }
     rc.flush(a.ctr);
 }
 
-// Exclusive scans (one block) of the tile counts and of the tile pass's work
-// items, kScanChunk tiles at a time through LDS so that every global load and
-// store is coalesced: writes the per-tile list starts, zeroes the counts for
-// the fill pass (which counts again) and writes the work items.
-constexpr uint32_t kScanThreads = 1024;
-constexpr uint32_t kScanChunk = 8192;   // a 4K view's 8160 tiles in one round (96 KB of LDS)
-constexpr uint32_t kScanPer = kScanChunk / kScanThreads;
+// Tile pass.  Work item (tile, band, part): band b (32x16 pixels) of a
+// screen tile, rasterizing part p of its list of n triangles
+// (triangles [n p / S, n (p + 1) / S), S = min(kTileSplit, ceil(n / item
+// tris)) parts).  A band keeps its keys in registers (lane l owns column
+// l & 31, rows (l >> 5) + 2r, r = 0..7 of the band), then merges them into
+// the visibility buffer: plain read-min-store when the tile has one part (the
+// meshlet raster and the bin pass of this pass have finished and bands are
+// disjoint; the old keys are read when the item starts), atomicMin
+// otherwise.  Setups are loaded 64 at a time, one per lane, and broadcast
+// with readlane.
+//
+// The items are enumerated statically, part-major: item i is part
+// i / (2 tiles) of band i % (2 tiles); wave w takes items w, w + waves, ...,
+// 64 at a time: lane l reads the count of item r + l's tile and the wave runs
+// the items whose part the count has.  The items of one tile are 2 tiles
+// apart, so they run in different waves.
+constexpr int kTileParts = 2;
+constexpr int kTileRows = (int)(kBinTile * kBinTile / 64u) / kTileParts;     // pixels per lane in a band
+constexpr int kSetupWords = (int)(sizeof(BigSetup) / 4);
+static_assert(kSetupWords % 4 == 0, "setups are loaded 16 bytes at a time");
 
-__device__ __forceinline__ void bin_scan_view(const BigArgs& a, uint32_t item_tris)
+// Parts of a tile whose list holds n triangles (0 for an empty tile).
+__device__ __forceinline__ uint32_t tile_split(uint32_t n, uint32_t item_tris)
 {
-    __shared__ uint32_t s_cnt[kScanChunk], s_start[kScanChunk], s_istart[kScanChunk];
-    __shared__ uint32_t s_wsum[kScanThreads / 64], s_wisum[kScanThreads / 64], s_wmax[kScanThreads / 64];
-    const uint32_t tid = threadIdx.x, w = tid >> 6, lane = tid & 63u;
-    uint32_t nbig, nchunks;
-    pass_extent(a, nbig, nchunks);
-    if (nchunks == 0) {   // no big triangle this pass: no work items
-        if (tid == 0) a.ctr->tile_items = 0u;
-        return;
+    return min(kTileSplit, (n + item_tris - 1u) / item_tris);
+}
+
+// One work item: band `band` of screen tile `tile`, triangles [b, e) of the
+// tile's list (wave-uniform arguments).
+template <bool STATS>
+__device__ __forceinline__ void tile_item(const BigArgs& a, uint32_t tile, uint32_t band, uint32_t b, uint32_t e,
+                                          uint32_t split, RasterCount<STATS>& rc)
+{
+    const uint32_t lane = threadIdx.x & 63u;
+    const int lx = (int)(lane & 31u), ly = (int)(lane >> 5);
+    const BigSetup* __restrict__ setups = a.setups;
+    const int px = (int)((tile % a.tiles_x) * kBinTile) + lx;
+    const int py0 = (int)((tile / a.tiles_x) * kBinTile + band * (kBinTile / kTileParts)) + ly;
+    const bool col_in = px < (int)a.W;
+    unsigned long long old[kTileRows];
+    if (split == 1) {
+#pragma unroll
+        for (int r = 0; r < kTileRows; ++r) {
+            const int py = py0 + 2 * r;
+            old[r] = col_in && py < (int)a.H ? a.keys[(size_t)py * a.W + px] : 0ull;
+        }
     }
-    const uint32_t ntiles = a.tiles_x * a.tiles_y;
-    uint32_t run_base = 0, irun_base = 0, most = 0;
-    for (uint32_t c0 = 0; c0 < ntiles; c0 += kScanChunk) {
-        const uint32_t m = min(kScanChunk, ntiles - c0);
-        for (uint32_t i = tid; i < m; i += kScanThreads) s_cnt[i] = a.bin_count[c0 + i];
-        __syncthreads();
-        const uint32_t b = tid * kScanPer;
-        uint32_t v[kScanPer], sum = 0, isum = 0;
+    unsigned long long key[kTileRows];
 #pragma unroll
-        for (uint32_t u = 0; u < kScanPer; ++u) {
-            v[u] = b + u < m ? s_cnt[b + u] : 0u;
-            sum += v[u];
-            isum += tile_split(v[u], item_tris);
-            most = max(most, v[u]);
-        }
-        uint32_t x = sum, ix = isum;
+    for (int r = 0; r < kTileRows; ++r) key[r] = VKGV_KEY_CLEAR;
+    const int sx = px * 256 + 128;
+    for (uint32_t t0 = b; t0 < e; t0 += 64u) {
+        // batch of up to 64 triangles: lane j holds the setup of triangle t0 + j
+        // (two loads for the batch), broadcast one at a time below
+        const uint32_t m = min(64u, e - t0);
+        uint32_t sw[kSetupWords];
+        if (lane < m) {
+            const uint32_t* src = reinterpret_cast<const uint32_t*>(setups + a.bins[t0 + lane]);
 #pragma unroll
-        for (int off = 1; off < 64; off <<= 1) {
-            const uint32_t y = __shfl_up(x, off), iy = __shfl_up(ix, off);
-            if (lane >= (uint32_t)off) { x += y; ix += iy; }
-        }
-        if (lane == 63u) { s_wsum[w] = x; s_wisum[w] = ix; }
-        __syncthreads();
-        uint32_t wbase = 0, iwbase = 0, total = 0, itotal = 0;
-        for (uint32_t q = 0; q < kScanThreads / 64; ++q) {
-            if (q < w) { wbase += s_wsum[q]; iwbase += s_wisum[q]; }
-            total += s_wsum[q];
-            itotal += s_wisum[q];
-        }
-        uint32_t run = run_base + wbase + x - sum, irun = irun_base + iwbase + ix - isum;
-#pragma unroll
-        for (uint32_t u = 0; u < kScanPer; ++u) {
-            if (b + u < m) { s_start[b + u] = run; s_istart[b + u] = irun; }
-            run += v[u];
-            irun += tile_split(v[u], item_tris);
-        }
-        __syncthreads();
-        for (uint32_t i = tid; i < m; i += kScanThreads) {
-            const uint32_t n = s_cnt[i], st = s_start[i], sp = tile_split(n, item_tris);
-            a.bin_start[c0 + i] = n ? st : 0u;   // zero where no pair lands: the fused fill's cursors
-            a.bin_count[c0 + i] = 0u;   // the fill pass counts again; the tile pass leaves it zero
-            for (uint32_t p = 0; p < sp; ++p) {
-                TileItem it;
-                it.code = (c0 + i) | (p << 24) | ((sp - 1u) << 28);
-                it.start = st;
-                it.n = n;
-                it.pad = 0u;
-                a.items[s_istart[i] + p] = it;
+            for (int q = 0; q < kSetupWords; q += 4) {
+                const uint4 v4 = *reinterpret_cast<const uint4*>(src + q);
+                sw[q] = v4.x; sw[q + 1] = v4.y; sw[q + 2] = v4.z; sw[q + 3] = v4.w;
             }
         }
-        run_base += total;
-        irun_base += itotal;
-        __syncthreads();   // LDS reused by the next chunk
-    }
+        for (uint32_t j = 0; j < m; ++j) {
+            BigSetup s;
+            uint32_t* dst = reinterpret_cast<uint32_t*>(&s);
 #pragma unroll
-    for (int off = 32; off >= 1; off >>= 1) most = max(most, __shfl_xor(most, off));
-    if (lane == 0) s_wmax[w] = most;
-    __syncthreads();
-    if (tid == 0) {
-        for (uint32_t q = 1; q < kScanThreads / 64; ++q) most = max(most, s_wmax[q]);
-        atomicMax(&a.ctr->max_tile_tris, most);
-        a.ctr->bin_total = run_base;
-        a.ctr->tile_items = irun_base;
-        a.ctr->big_total_tris += nbig;
-        a.ctr->big_total_pairs += run_base;
+            for (int q = 0; q < kSetupWords; ++q) dst[q] = __builtin_amdgcn_readlane(sw[q], j);
+            const int bx0 = (int)(s.bb0 & 0xFFFFu), by0 = (int)(s.bb0 >> 16);
+            const int bx1 = (int)(s.bb1 & 0xFFFFu), by1 = (int)(s.bb1 >> 16);
+            const bool col_ok = px >= bx0 && px <= bx1;
+            long long ev[3];
+#pragma unroll
+            for (int q = 0; q < 3; ++q)
+                ev[q] = s.e0[q] - (long long)(px - bx0) * (long long)s.ey[q] * 256
+                        + (long long)(py0 - by0) * (long long)s.ex[q] * 256;
+            const float dxf = (float)(sx - s.x0) * 0.00390625f;
+            if (!col_ok) continue;
+#pragma unroll
+            for (int r = 0; r < kTileRows; ++r) {
+                const int py = py0 + 2 * r;
+                if (r) {
+#pragma unroll
+                    for (int q = 0; q < 3; ++q) ev[q] += (long long)s.ex[q] * 512;   // +2 rows
+                }
+                if (py < by0 || py > by1) continue;
+                if ((ev[0] | ev[1] | ev[2]) < 0) continue;
+                const float dyf = (float)(py * 256 + 128 - s.y0) * 0.00390625f;
+                const float z = s.z0 + (s.za * dxf + s.zb * dyf);
+                if (!(z >= 0.0f && z <= 1.0f)) continue;
+                const unsigned long long k = make_key(z, s.id);
+                key[r] = k < key[r] ? k : key[r];
+                rc.frag();
+            }
+        }
+    }
+    if (!col_in) return;
+#pragma unroll
+    for (int r = 0; r < kTileRows; ++r) {
+        const int py = py0 + 2 * r;
+        if (key[r] == VKGV_KEY_CLEAR || py >= (int)a.H) continue;
+        unsigned long long* p = a.keys + (size_t)py * a.W + px;
+        if (split == 1) {
+            if (key[r] < old[r]) *p = key[r];
+            rc.store();
+        } else {
+            key_min(p, key[r]);
+            rc.atomic();
+        }
     }
 }
 
-// Scatter of the pairs into per-tile triangle lists (counts rebuilt).
-__device__ __forceinline__ void bin_fill_view(const BigArgs& a)
+// Views of more than kScanTiles tiles: the items are enumerated statically,
+// part-major: item i is part i / (2 tiles) of band i % (2 tiles); wave w takes
+// items w, w + waves, ..., 64 at a time: lane l reads the count of item r + l's
+// tile and the wave runs the items whose part the count has.
+template <bool STATS>
+__device__ __forceinline__ void tile_view_static(const BigArgs& a, uint32_t item_tris, RasterCount<STATS>& rc)
 {
-    uint32_t nbig, nchunks;
-    pass_extent(a, nbig, nchunks);
-    const unsigned long long np = (unsigned long long)nchunks * 64ull;   // at most the pair capacity
-    for (uint32_t i = blockIdx.x * 256u + threadIdx.x; i < (uint32_t)np; i += gridDim.x * 256u) {
-        const BinPair p = a.pairs[i];
-        if (p.tile == 0xFFFFFFFFu) continue;
-        const uint32_t slot = a.bin_start[p.tile] + atomicAdd(&a.bin_count[p.tile], 1u);
-        a.bins[slot] = p.tri;
+    const uint32_t lane = threadIdx.x & 63u;
+    const uint32_t wpb = blockDim.x >> 6;
+    const uint32_t wave = blockIdx.x * wpb + (threadIdx.x >> 6), nwaves = gridDim.x * wpb;
+    const uint32_t nband = a.tiles_x * a.tiles_y * (uint32_t)kTileParts;
+    const unsigned long long nitems = (unsigned long long)nband * kTileSplit;
+    for (unsigned long long r0 = 0; wave + r0 * nwaves < nitems; r0 += 64u) {
+        const unsigned long long it = wave + (r0 + lane) * nwaves;
+        uint32_t code = 0u, n = 0u;
+        bool live = false;
+        if (it < nitems) {
+            const uint32_t part = (uint32_t)(it / nband), hv = (uint32_t)(it - (unsigned long long)part * nband);
+            n = min(a.bin_count[hv / (uint32_t)kTileParts], a.tile_cap);
+            live = part < tile_split(n, item_tris);
+            code = hv | (part << 28);
+        }
+        unsigned long long pending = __ballot(live);
+        while (pending != 0ull) {
+            const int j = __ffsll((long long)pending) - 1;
+            pending &= pending - 1ull;
+            const uint32_t cj = __builtin_amdgcn_readlane(code, j), nj = __builtin_amdgcn_readlane(n, j);
+            const uint32_t hv = cj & 0x0FFFFFFFu, part = cj >> 28;
+            const uint32_t tile = hv / (uint32_t)kTileParts, split = tile_split(nj, item_tris);
+            const uint32_t start = tile * a.tile_cap;
+            tile_item<STATS>(a, tile, hv % (uint32_t)kTileParts, start + nj * part / split,
+                             start + nj * (part + 1u) / split, split, rc);
+        }
     }
 }
 
-// Scan and fill in one pass, for views of at most kScanChunk tiles: every
-// block scans the tile counts itself (redundant work of a few microseconds
-// per block, in place of a launch and a one-block pass), writes its share of
-// the work items, and scatters its pairs with bin_start as per-tile cursors
-// (zero between passes: the tile pass clears the cursors of the tiles it ran,
-// the others never move).  Thread t holds the counts of tiles
-// [kFillPer t, kFillPer (t + 1)) in registers (16-byte loads); only the list
-// starts go to LDS, for the pairs' lookups.  The count array stays as binned.
-constexpr uint32_t kFillThreads = 256;
-constexpr uint32_t kFillPer = kScanChunk / kFillThreads;   // tiles per thread in the scan
-static_assert(kScanChunk % (4 * kFillThreads) == 0, "the fused scan loads whole uint4 runs per thread");
+// Views of at most kScanTiles tiles (4K included): every block scans the
+// tile counts itself (kScanPer tiles per thread, in registers, 16-byte
+// loads), so the items form one compact list, and wave g of the grid takes
+// (item, band) pairs g, g + waves, ...  When the list has at most kScanTiles
+// items and no list is longer than kItemMaxN, the scan writes each item's
+// record (tile | part << 13 | (split - 1) << 16 | n << 19) to LDS, one read
+// per item; otherwise it writes the tiles' first items and a wave finds an
+// item's tile by binary search.  The redundant per-block scan costs a few
+// microseconds; it replaces a launch, and the compact list keeps every wave
+// within one item of the average.
+constexpr uint32_t kTileThreads = 1024;
+constexpr uint32_t kScanTiles = 8192;
+constexpr uint32_t kScanPer = kScanTiles / kTileThreads;   // tiles per thread in the scan
+constexpr uint32_t kItemMaxN = 2047;                       // longest list an LDS item record holds
+static_assert(kScanPer % 4 == 0, "the scan loads whole uint4 runs per thread");
+static_assert(kScanTiles <= (1u << 13) && kTileSplit <= 8, "item records: 13-bit tile, 3-bit part and split");
 
-// Tile i of the fused scan's LDS starts: one padding word per 32 keeps each
-// thread's run of kFillPer tiles on distinct banks.
-__device__ __forceinline__ uint32_t fill_lds(uint32_t i) { return i + (i >> 5); }
-constexpr uint32_t kFillLds = kScanChunk + kScanChunk / 32;
-
-__device__ __forceinline__ void bin_fill_scan_view(const BigArgs& a, uint32_t item_tris, uint32_t* s_start)
+template <bool STATS>
+__device__ __forceinline__ void tile_view_scan(const BigArgs& a, uint32_t item_tris, uint32_t* s_arr,
+                                               uint32_t* s_wsum, RasterCount<STATS>& rc)
 {
-    __shared__ uint32_t s_wsum[kFillThreads / 64], s_wisum[kFillThreads / 64], s_wmax[kFillThreads / 64];
     const uint32_t tid = threadIdx.x, lane = tid & 63u, w = tid >> 6;
-    uint32_t nbig, nchunks;
-    pass_extent(a, nbig, nchunks);
-    if (nchunks == 0) {   // no big triangle this pass (uniform over the grid)
-        if (blockIdx.x == 0 && tid == 0) a.ctr->tile_items = 0u;
-        return;
-    }
-    const uint32_t ntiles = a.tiles_x * a.tiles_y;   // at most kScanChunk (checked at launch)
-    const uint32_t b0 = tid * kFillPer;
-    uint32_t cnt[kFillPer];
+    const uint32_t ntiles = a.tiles_x * a.tiles_y;   // at most kScanTiles
+    const uint32_t b0 = tid * kScanPer;
+    uint32_t cnt[kScanPer];
 #pragma unroll
-    for (uint32_t q = 0; q < kFillPer; q += 4) {
+    for (uint32_t q = 0; q < kScanPer; q += 4) {
         if (b0 + q + 4 <= ntiles) {
             const uint4 v4 = *reinterpret_cast<const uint4*>(a.bin_count + b0 + q);
             cnt[q] = v4.x; cnt[q + 1] = v4.y; cnt[q + 2] = v4.z; cnt[q + 3] = v4.w;
@@ -1132,197 +1213,81 @@ __device__ __forceinline__ void bin_fill_scan_view(const BigArgs& a, uint32_t it
             for (uint32_t r = 0; r < 4; ++r) cnt[q + r] = b0 + q + r < ntiles ? a.bin_count[b0 + q + r] : 0u;
         }
     }
-    uint32_t sum = 0, isum = 0, most = 0;
+    uint32_t isum = 0;
 #pragma unroll
-    for (uint32_t u = 0; u < kFillPer; ++u) {
-        sum += cnt[u];
+    for (uint32_t u = 0; u < kScanPer; ++u) {
+        cnt[u] = min(cnt[u], a.tile_cap);
         isum += tile_split(cnt[u], item_tris);
-        most = max(most, cnt[u]);
     }
-    uint32_t x = sum, ix = isum;
+    uint32_t x = isum;
 #pragma unroll
     for (int off = 1; off < 64; off <<= 1) {
-        const uint32_t y = __shfl_up(x, off), iy = __shfl_up(ix, off);
-        if (lane >= (uint32_t)off) { x += y; ix += iy; }
+        const uint32_t y = __shfl_up(x, off);
+        if (lane >= (uint32_t)off) x += y;
     }
-#pragma unroll
-    for (int off = 32; off >= 1; off >>= 1) most = max(most, __shfl_xor(most, off));
-    if (lane == 63u) { s_wsum[w] = x; s_wisum[w] = ix; }
-    if (lane == 0u) s_wmax[w] = most;
+    if (lane == 63u) s_wsum[w] = x;
     __syncthreads();
-    uint32_t wbase = 0, iwbase = 0, total = 0, itotal = 0;
+    uint32_t wbase = 0, total = 0;
 #pragma unroll
-    for (uint32_t q = 0; q < kFillThreads / 64; ++q) {
-        if (q < w) { wbase += s_wsum[q]; iwbase += s_wisum[q]; }
+    for (uint32_t q = 0; q < kTileThreads / 64; ++q) {
+        if (q < w) wbase += s_wsum[q];
         total += s_wsum[q];
-        itotal += s_wisum[q];
-        most = max(most, s_wmax[q]);
     }
-    // list starts to LDS; work items of this thread's tiles from one block each
-    const bool items = tid % gridDim.x == blockIdx.x;
-    uint32_t run = wbase + x - sum, irun = iwbase + ix - isum;
+    // records per item, or first item per tile (block-uniform choice)
+    const bool records = total <= kScanTiles && a.tile_cap <= kItemMaxN;
+    uint32_t run = wbase + x - isum;
 #pragma unroll
-    for (uint32_t u = 0; u < kFillPer; ++u) {
-        const uint32_t n = cnt[u], sp = tile_split(n, item_tris);
-        s_start[fill_lds(b0 + u)] = run;
-        if (items)
-            for (uint32_t p = 0; p < sp; ++p) {
-                TileItem it;
-                it.code = (b0 + u) | (p << 24) | ((sp - 1u) << 28);
-                it.start = run;
-                it.n = n;
-                it.pad = 0u;
-                a.items[irun + p] = it;
-            }
-        run += n;
-        irun += sp;
-    }
-    if (blockIdx.x == 0 && tid == 0) {
-        atomicMax(&a.ctr->max_tile_tris, most);
-        a.ctr->bin_total = total;
-        a.ctr->tile_items = itotal;
-        a.ctr->big_total_tris += nbig;
-        a.ctr->big_total_pairs += total;
+    for (uint32_t u = 0; u < kScanPer; ++u) {
+        const uint32_t sp = tile_split(cnt[u], item_tris);
+        if (records) {
+            for (uint32_t p = 0; p < sp; ++p)
+                s_arr[run + p] = (b0 + u) | (p << 13) | ((sp - 1u) << 16) | (cnt[u] << 19);
+        } else {
+            s_arr[b0 + u] = run;
+        }
+        run += sp;
     }
     __syncthreads();
-    // the pairs, grid-stride
-    const unsigned long long np = (unsigned long long)nchunks * 64ull;   // at most the pair capacity
-    for (uint32_t i = blockIdx.x * kFillThreads + tid; i < (uint32_t)np; i += gridDim.x * kFillThreads) {
-        const BinPair p = a.pairs[i];
-        if (p.tile == 0xFFFFFFFFu) continue;
-        const uint32_t slot = s_start[fill_lds(p.tile)] + atomicAdd(&a.bin_start[p.tile], 1u);
-        a.bins[slot] = p.tri;
+    const uint32_t nwaves = gridDim.x * (kTileThreads / 64);
+    for (uint32_t hv = blockIdx.x + gridDim.x * w; hv < kTileParts * total; hv += nwaves) {
+        const uint32_t item = hv / (uint32_t)kTileParts;
+        uint32_t tile, part, split, n;
+        if (records) {
+            const uint32_t r = __builtin_amdgcn_readfirstlane(s_arr[item]);
+            tile = r & 0x1FFFu;
+            part = (r >> 13) & 7u;
+            split = ((r >> 16) & 7u) + 1u;
+            n = r >> 19;
+        } else {
+            uint32_t lo = 0, hi = ntiles - 1u;   // last tile whose first item is <= item
+            while (lo < hi) {
+                const uint32_t mid = (lo + hi + 1u) >> 1;
+                if (s_arr[mid] <= item) lo = mid; else hi = mid - 1u;
+            }
+            tile = __builtin_amdgcn_readfirstlane(lo);
+            part = item - s_arr[tile];
+            n = __builtin_amdgcn_readfirstlane(min(a.bin_count[tile], a.tile_cap));
+            split = tile_split(n, item_tris);
+        }
+        const uint32_t start = tile * a.tile_cap;
+        tile_item<STATS>(a, tile, hv % (uint32_t)kTileParts, start + n * part / split, start + n * (part + 1u) / split,
+                         split, rc);
     }
-    __syncthreads();   // the LDS starts are reused by the next view
-}
-
-// Tile pass: each wave runs a grid-stride loop over the halves (32x16
-// pixels) of the work items.  Item (tile, part, S) rasterizes triangles
-// [n*part/S, n*(part+1)/S) of the tile's list of n; a half keeps its keys in
-// registers (lane l owns column l & 31, rows (l >> 5) + 2r, r = 0..7 of the
-// half), then merges them into the visibility buffer:
-// plain read-min-store when the tile has one item (the meshlet raster of this
-// pass has finished and tiles are disjoint; the old keys are read when the
-// item starts), atomicMin otherwise.  Setups are loaded 64 at a time, one per
-// lane, and broadcast with readlane.  Items of one tile are adjacent, so
-// they run on different XCDs.  Part 0 of each tile zeroes the tile's count
-// for the next pass.
-// A tile-pass wave covers kTileParts horizontal bands of a screen tile's list:
-// 64 lanes = 32 columns x 2 rows, each lane kTileRows pixels of its column.
-constexpr int kTileParts = 2;
-constexpr int kTileRows = (int)(kBinTile * kBinTile / 64u) / kTileParts;     // pixels per lane in a band
-constexpr int kSetupWords = (int)(sizeof(BigSetup) / 4);
-static_assert(kSetupWords % 4 == 0, "setups are loaded 16 bytes at a time");
-
-template <bool STATS>
-__device__ __forceinline__ void tile_view(const BigArgs& a)
-{
-    RasterCount<STATS> rc;
-    const uint32_t lane = threadIdx.x & 63u;
-    const uint32_t wave = blockIdx.x * 4u + (threadIdx.x >> 6), nwaves = gridDim.x * 4u;
-    const uint32_t nitems = __builtin_amdgcn_readfirstlane(a.ctr->tile_items);
-    const int lx = (int)(lane & 31u), ly = (int)(lane >> 5);
-    const BigSetup* __restrict__ setups = a.setups;
-    for (uint32_t hv = wave; hv < kTileParts * nitems; hv += nwaves) {
-        const TileItem item = a.items[hv / kTileParts];
-        const uint32_t code = __builtin_amdgcn_readfirstlane(item.code);
-        const uint32_t n_all = __builtin_amdgcn_readfirstlane(item.n);
-        const uint32_t start = __builtin_amdgcn_readfirstlane(item.start);
-        const uint32_t tile = code & 0xFFFFFFu, part = (code >> 24) & 15u, split = (code >> 28) + 1u;
-        const uint32_t b = start + n_all * part / split, e = start + n_all * (part + 1u) / split;
-        if (part == 0 && hv % kTileParts == 0 && lane == 0) {
-            a.bin_count[tile] = 0u;
-            a.bin_start[tile] = 0u;   // the fused fill's cursor
-        }
-        const int px = (int)((tile % a.tiles_x) * kBinTile) + lx;
-        const int py0 = (int)((tile / a.tiles_x) * kBinTile + (hv % kTileParts) * (kBinTile / kTileParts)) + ly;
-        const bool col_in = px < (int)a.W;
-        unsigned long long old[kTileRows];
-        if (split == 1) {
-#pragma unroll
-            for (int r = 0; r < kTileRows; ++r) {
-                const int py = py0 + 2 * r;
-                old[r] = col_in && py < (int)a.H ? a.keys[(size_t)py * a.W + px] : 0ull;
-            }
-        }
-        unsigned long long key[kTileRows];
-#pragma unroll
-        for (int r = 0; r < kTileRows; ++r) key[r] = VKGV_KEY_CLEAR;
-        const int sx = px * 256 + 128;
-        for (uint32_t t0 = b; t0 < e; t0 += 64u) {
-            // batch of up to 64 triangles: lane j holds the setup of triangle t0 + j
-            // (two loads for the batch), broadcast one at a time below
-            const uint32_t m = min(64u, e - t0);
-            uint32_t sw[kSetupWords];
-            if (lane < m) {
-                const uint32_t* src = reinterpret_cast<const uint32_t*>(setups + a.bins[t0 + lane]);
-#pragma unroll
-                for (int q = 0; q < kSetupWords; q += 4) {
-                    const uint4 v4 = *reinterpret_cast<const uint4*>(src + q);
-                    sw[q] = v4.x; sw[q + 1] = v4.y; sw[q + 2] = v4.z; sw[q + 3] = v4.w;
-                }
-            }
-            for (uint32_t j = 0; j < m; ++j) {
-                BigSetup s;
-                uint32_t* dst = reinterpret_cast<uint32_t*>(&s);
-#pragma unroll
-                for (int q = 0; q < kSetupWords; ++q) dst[q] = __builtin_amdgcn_readlane(sw[q], j);
-                const int bx0 = (int)(s.bb0 & 0xFFFFu), by0 = (int)(s.bb0 >> 16);
-                const int bx1 = (int)(s.bb1 & 0xFFFFu), by1 = (int)(s.bb1 >> 16);
-                if (px < bx0 || px > bx1) continue;
-                long long ev[3];
-#pragma unroll
-                for (int q = 0; q < 3; ++q)
-                    ev[q] = s.e0[q] - (long long)(px - bx0) * (long long)s.ey[q] * 256
-                            + (long long)(py0 - by0) * (long long)s.ex[q] * 256;
-                const float dxf = (float)(sx - s.x0) * 0.00390625f;
-#pragma unroll
-                for (int r = 0; r < kTileRows; ++r) {
-                    const int py = py0 + 2 * r;
-                    if (r) {
-#pragma unroll
-                        for (int q = 0; q < 3; ++q) ev[q] += (long long)s.ex[q] * 512;   // +2 rows
-                    }
-                    if (py < by0 || py > by1) continue;
-                    if ((ev[0] | ev[1] | ev[2]) < 0) continue;
-                    const float dyf = (float)(py * 256 + 128 - s.y0) * 0.00390625f;
-                    const float z = s.z0 + (s.za * dxf + s.zb * dyf);
-                    if (!(z >= 0.0f && z <= 1.0f)) continue;
-                    const unsigned long long k = make_key(z, s.id);
-                    key[r] = k < key[r] ? k : key[r];
-                    rc.frag();
-                }
-            }
-        }
-        if (!col_in) continue;
-#pragma unroll
-        for (int r = 0; r < kTileRows; ++r) {
-            const int py = py0 + 2 * r;
-            if (key[r] == VKGV_KEY_CLEAR || py >= (int)a.H) continue;
-            unsigned long long* p = a.keys + (size_t)py * a.W + px;
-            if (split == 1) {
-                if (key[r] < old[r]) *p = key[r];
-                rc.store();
-            } else {
-                key_min(p, key[r]);
-                rc.atomic();
-            }
-        }
-    }
-    rc.flush(a.ctr);
+    __syncthreads();   // the LDS array is reused by the next view
 }
 
 // ---------------------------------------------------------------------------
 // Kernels over a batch of views: every view's work in one launch.
 
-__device__ __forceinline__ BigArgs big_args(const ViewDev& v)
+// The view's big-triangle path in raster pass `pass` (0: first, 1: late).
+__device__ __forceinline__ BigArgs big_args(const ViewDev& v, uint32_t pass)
 {
     BigArgs a;
     a.big = v.big;
     a.big_chunk = v.big_chunk;
     a.big_cap = v.big_capacity;
     a.chunk_tri = v.chunk_tri;
-    a.chunk_cap = v.bin_capacity / 64u;
+    a.chunk_cap = v.chunk_capacity;
     a.clipq = v.clipq;
     a.clip_cap = v.clip_capacity;
     a.ctr = v.counters;
@@ -1331,13 +1296,10 @@ __device__ __forceinline__ BigArgs big_args(const ViewDev& v)
     a.H = v.height;
     a.tiles_x = v.tiles_x;
     a.tiles_y = v.tiles_y;
-    a.bin_count = v.bin_count;
-    a.bin_start = v.bin_start;
+    a.bin_count = v.bin_count + (size_t)pass * tile_count_stride(v);
     a.bins = v.bins;
-    a.bin_cap = v.bin_capacity;
+    a.tile_cap = v.tile_capacity;
     a.setups = v.setups;
-    a.pairs = v.pairs;
-    a.items = v.tile_items;
     a.stage = v.frag_stage;
     a.jobs = v.frag_jobs;
     a.job_cap = v.job_capacity;
@@ -1408,7 +1370,7 @@ __global__ __launch_bounds__(kBlock, 3) void raster_kernel(RasterBatchArgs ab)
             a.big_cap = v.big_capacity;
             a.redo = v.redo;
             a.chunk_tri = v.chunk_tri;
-            a.chunk_cap = v.bin_capacity / 64u;
+            a.chunk_cap = v.chunk_capacity;
             a.clipq = v.clipq;
             a.clip_cap = v.clip_capacity;
             a.ctr = v.counters;
@@ -1530,15 +1492,16 @@ __global__ __launch_bounds__(kClipThreads) void clip_kernel(BatchDev b, RedoArgs
         const uint32_t nr = lane_view_value(b, v0, [](const Counters& c) { return c.redo_count; });
         const uint32_t m = min(64u, b.nviews - v0);
         for (uint32_t j = 0; j < m; ++j) {
-            if (__builtin_amdgcn_readlane(n, j)) clip_view<STATS>(big_args(batch_view(b, v0 + j)));
+            if (__builtin_amdgcn_readlane(n, j)) clip_view<STATS>(big_args(batch_view(b, v0 + j), ra.list));
             const uint32_t r = __builtin_amdgcn_readlane(nr, j);
             if (r) redo_view<STATS>(ra, batch_view(b, v0 + j), r);
         }
     }
 }
 
+// Per view: the big triangles' chunks, then the deferred fragment runs.
 template <bool STATS>
-__global__ __launch_bounds__(256) void bin_kernel(BatchDev b)
+__global__ __launch_bounds__(256) void bin_kernel(BatchDev b, uint32_t pass)
 {
     for (uint32_t v0 = 0; v0 < b.nviews; v0 += 64u) {
         const uint32_t n = lane_view_value(b, v0, [](const Counters& c) {
@@ -1546,60 +1509,28 @@ __global__ __launch_bounds__(256) void bin_kernel(BatchDev b)
         });
         const uint32_t m = min(64u, b.nviews - v0);
         for (uint32_t j = 0; j < m; ++j)
-            if (__builtin_amdgcn_readlane(n, j)) bin_view<STATS>(big_args(batch_view(b, v0 + j)));
+            if (__builtin_amdgcn_readlane(n, j)) bin_view<STATS>(big_args(batch_view(b, v0 + j), pass));
     }
 }
 
-// One block per view.
-__global__ __launch_bounds__(kScanThreads) void bin_scan_kernel(BatchDev b, uint32_t item_tris)
-{
-    bin_scan_view(big_args(batch_view(b, blockIdx.x)), item_tris);
-}
-
-__global__ __launch_bounds__(256) void bin_fill_kernel(BatchDev b)
-{
-    for (uint32_t v0 = 0; v0 < b.nviews; v0 += 64u) {
-        const uint32_t n = lane_view_value(b, v0, big_count);
-        const uint32_t m = min(64u, b.nviews - v0);
-        for (uint32_t j = 0; j < m; ++j)
-            if (__builtin_amdgcn_readlane(n, j)) bin_fill_view(big_args(batch_view(b, v0 + j)));
-    }
-}
-
-__global__ __launch_bounds__(kFillThreads) void bin_fill_scan_kernel(BatchDev b, uint32_t item_tris)
-{
-    __shared__ uint32_t s_start[kFillLds];
-    for (uint32_t v0 = 0; v0 < b.nviews; v0 += 64u) {
-        const uint32_t n = lane_view_value(b, v0, big_count);
-        const uint32_t m = min(64u, b.nviews - v0);
-        for (uint32_t j = 0; j < m; ++j)
-            if (__builtin_amdgcn_readlane(n, j)) {
-                bin_fill_scan_view(big_args(batch_view(b, v0 + j)), item_tris, s_start);
-            } else if (blockIdx.x == 0 && threadIdx.x == 0) {
-                batch_view(b, v0 + j).counters->tile_items = 0u;   // no big triangle in this view
-            }
-    }
-}
-
-// Wave 0 also clears every view's queue counters for the next pass (their
-// last reader, the fill pass, has finished).
+// Per view with big triangles in this pass: its tile items.
 template <bool STATS>
-__global__ __launch_bounds__(256) void tile_kernel(BatchDev b)
+__global__ __launch_bounds__(kTileThreads) void tile_kernel(BatchDev b, uint32_t pass, uint32_t item_tris)
 {
-    const uint32_t wave = blockIdx.x * 4u + (threadIdx.x >> 6);
+    __shared__ uint32_t s_arr[kScanTiles];
+    __shared__ uint32_t s_wsum[kTileThreads / 64];
+    RasterCount<STATS> rc;
     for (uint32_t v0 = 0; v0 < b.nviews; v0 += 64u) {
-        const uint32_t n = lane_view_value(b, v0, [](const Counters& c) { return c.tile_items; });
-        if (wave == 0 && v0 + (threadIdx.x & 63u) < b.nviews) {
-            Counters* c = batch_view(b, v0 + (threadIdx.x & 63u)).counters;
-            c->big_chunks = 0ull;
-            c->big_count = 0u;
-            c->redo_count = 0u;
-            c->clip_count = 0u;
-            c->frag_jobs = 0ull;
-        }
+        const uint32_t n = lane_view_value(b, v0, big_count);
         const uint32_t m = min(64u, b.nviews - v0);
-        for (uint32_t j = 0; j < m; ++j)
-            if (__builtin_amdgcn_readlane(n, j)) tile_view<STATS>(big_args(batch_view(b, v0 + j)));
+        for (uint32_t j = 0; j < m; ++j) {
+            if (!__builtin_amdgcn_readlane(n, j)) continue;
+            const ViewDev& v = batch_view(b, v0 + j);
+            const BigArgs a = big_args(v, pass);
+            if (v.tiles_x * v.tiles_y <= kScanTiles) tile_view_scan<STATS>(a, item_tris, s_arr, s_wsum, rc);
+            else tile_view_static<STATS>(a, item_tris, rc);
+            rc.flush(a.ctr);
+        }
     }
 }
 
@@ -1622,6 +1553,7 @@ void launch_raster(const SceneDev& s, const BatchDev& b, uint32_t flags, uint32_
 void launch_raster_big(const SceneDev& s, const BatchDev& b, const BatchGrid& g, uint32_t flags, uint32_t list,
                        uint32_t small_max_extent, uint32_t tile_grid, uint32_t tile_item_tris, hipStream_t st)
 {
+    (void)g;
     RedoArgs ra;
     ra.s = s;
     ra.flags = flags;
@@ -1631,16 +1563,12 @@ void launch_raster_big(const SceneDev& s, const BatchDev& b, const BatchGrid& g,
     const bool stats = flags & VKGV_STATS;
     if (stats) clip_kernel<true><<<std::max(64u, tile_grid / 4u), kClipThreads, 0, st>>>(b, ra);
     else clip_kernel<false><<<std::max(64u, tile_grid / 4u), kClipThreads, 0, st>>>(b, ra);
-    if (stats) bin_kernel<true><<<1024, 256, 0, st>>>(b);
-    else bin_kernel<false><<<1024, 256, 0, st>>>(b);
-    if (g.max_bin_tiles <= kScanChunk) {
-        bin_fill_scan_kernel<<<256, kFillThreads, 0, st>>>(b, tile_item_tris);
-    } else {
-        bin_scan_kernel<<<g.nviews, kScanThreads, 0, st>>>(b, tile_item_tris);
-        bin_fill_kernel<<<256, 256, 0, st>>>(b);
-    }
-    if (stats) tile_kernel<true><<<tile_grid, 256, 0, st>>>(b);
-    else tile_kernel<false><<<tile_grid, 256, 0, st>>>(b);
+    if (stats) bin_kernel<true><<<1024, 256, 0, st>>>(b, list);
+    else bin_kernel<false><<<1024, 256, 0, st>>>(b, list);
+    // one round of the tile kernel's 1024-thread blocks at its occupancy
+    const uint32_t tile_blocks = std::max(1u, tile_grid / 8u);
+    if (stats) tile_kernel<true><<<tile_blocks, kTileThreads, 0, st>>>(b, list, tile_item_tris);
+    else tile_kernel<false><<<tile_blocks, kTileThreads, 0, st>>>(b, list, tile_item_tris);
 }
 
 } // namespace vkgv

@@ -31,13 +31,15 @@ class VkgvError(RuntimeError):
 
 
 class Renderer:
-    def __init__(self, device: int = 0, big_triangle_capacity: int = 0, clip_triangle_capacity: int = 0):
+    def __init__(self, device: int = 0, big_triangle_capacity: int = 0, clip_triangle_capacity: int = 0,
+                 tile_triangle_capacity: int = 0):
         """Queue capacities are per view and raster pass (0 = the library's
-        defaults); a triangle that finds its queue full is rasterized by the
-        slower direct path, so they change speed, never the image."""
+        defaults; the tile capacity is per 32x32 screen tile); a triangle that
+        finds its queue full is rasterized by the slower direct path, so they
+        change speed, never the image."""
         self._lib = N.hip()
         self.device = device
-        info = N.CreateInfo(device, big_triangle_capacity, clip_triangle_capacity)
+        info = N.CreateInfo(device, big_triangle_capacity, clip_triangle_capacity, tile_triangle_capacity)
         h = C.c_void_p()
         rc = self._lib.vkgv_create(C.byref(info), C.byref(h))
         if rc != 0:
