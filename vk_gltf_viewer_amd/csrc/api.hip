@@ -95,6 +95,7 @@ struct vkgv_view {
     vkgv_ctx* ctx = nullptr;
     ViewDev dev{};
     DevBuf keys, color, pyramid, visible0, visible1, history, counters, big, clipq, bin_count, bins;
+    DevBuf keys_b, counters_b;       // the second key buffer and counters (frames alternate between them)
     DevBuf setups, tri_cache, slot_albedo, slot_mat, draw_slot, big_chunk, chunk_tri, redo;
     DevBuf frag_stage, frag_jobs;    // deferred fragment runs of heavy meshlets
     DevBuf aa_mat;                   // per-pixel material indices of the differentiable frame (allocated on first use)
@@ -121,21 +122,29 @@ struct vkgv_view {
 struct vkgv_batch {
     vkgv_ctx* ctx = nullptr;
     std::vector<vkgv_view*> views;
-    DevBuf table, maps, cams;
+    DevBuf maps, cams;
+    // Two device tables of the views' descriptors (and a captured frame for
+    // each): frames alternate between the views' two key buffers, so in a run
+    // of batch frames the tables alternate too.  A table is identified by the
+    // key buffer each view renders into and the context generation.
+    static constexpr uint32_t kTables = 2;
+    DevBuf table[kTables];
+    std::vector<const void*> table_keys[kTables];
+    uint64_t table_generation[kTables] = { ~0ull, ~0ull };
+    uint32_t table_next = 0;
     BatchDev batch{};
     BatchGrid grid{};
-    uint64_t generation = ~0ull;
     // pinned staging ring for the per-frame cameras
     static constexpr uint32_t kStage = 4;
     CameraArg* staging[kStage] = {};
     hipEvent_t staged[kStage] = {};
     bool stage_used[kStage] = {};
     uint32_t stage_next = 0;
-    // captured frame
-    hipGraphExec_t exec = nullptr;
-    uint32_t exec_flags = 0;
-    hipStream_t exec_stream = nullptr;
-    RasterTuning exec_tuning{ 0u, 0u, 0u, 0u };
+    // captured frame, per table
+    hipGraphExec_t exec[kTables] = {};
+    uint32_t exec_flags[kTables] = {};
+    hipStream_t exec_stream[kTables] = {};
+    RasterTuning exec_tuning[kTables] = {};
     // whole-frame timing ring
     static constexpr uint32_t kRing = 64;
     hipEvent_t ev0[kRing] = {}, ev1[kRing] = {};
@@ -415,6 +424,17 @@ int build_maps(vkgv_ctx* c, const std::vector<const ViewDev*>& views, DevBuf& ma
     return VKGV_OK;
 }
 
+// The view as the next frame uses it: frames alternate between two key
+// buffers and two counter sets; the frame renders into the one the previous
+// frame's resolve cleared and clears the other for the frame after.
+ViewDev next_frame_view(const ViewDev& d)
+{
+    ViewDev n = d;
+    std::swap(n.keys, n.keys_next);
+    std::swap(n.counters, n.counters_next);
+    return n;
+}
+
 // Device copy of the views' descriptors (synchronous; only when they change).
 int upload_table(vkgv_ctx* c, const std::vector<const ViewDev*>& views, DevBuf& table, BatchDev& bd)
 {
@@ -440,7 +460,6 @@ void frame_launches(vkgv_ctx* c, const BatchDev& b, const BatchGrid& g, uint32_t
     const uint32_t raster_grid = (uint32_t)c->cus * c->tuning.raster_blocks_per_cu;
     const uint32_t tile_grid = (uint32_t)c->cus * 8u;
     mark(EV_START);
-    launch_frame_start(s, b, g, st);
     if (flags & VKGV_MOTION_VECTORS) launch_prev_mvp(s, b, g, st);
     mark(EV_CLEAR);
     launch_cull(s, b, g, flags, two_pass ? CULL_EARLY : CULL_SINGLE, 0, (uint32_t)c->cus, st);
@@ -457,7 +476,10 @@ void frame_launches(vkgv_ctx* c, const BatchDev& b, const BatchGrid& g, uint32_t
         launch_raster_big(s, b, g, flags, 1, c->tuning.small_max_extent, tile_grid, c->tuning.tile_item_tris, st);
         mark(EV_RASTER_LATE);
     }
+    // the resolve also clears the next frame's key buffer and zeroes its
+    // counters; a frame without one does that here
     if (!(flags & VKGV_SKIP_RESOLVE)) launch_resolve(s, b, g, flags, st);
+    else launch_next_frame_clear(b, g, st);
     mark(EV_RESOLVE);
     if (!two_pass && (flags & VKGV_BUILD_PYRAMID)) launch_hiz(b, g, st);
     mark(EV_PYRAMID);
@@ -1160,7 +1182,9 @@ int vkgv_view_create(vkgv_ctx* c, uint32_t width, uint32_t height, vkgv_view** o
     if (!rc) rc = dalloc(c, v->keys, 8 * px);
     if (!rc) rc = dalloc(c, v->color, 4 * px);
     if (!rc) rc = dalloc(c, v->pyramid, 4 * (size_t)std::max(off, 1u));
+    if (!rc) rc = dalloc(c, v->keys_b, 8 * px);
     if (!rc) rc = dalloc(c, v->counters, sizeof(Counters));
+    if (!rc) rc = dalloc(c, v->counters_b, sizeof(Counters));
     if (!rc) rc = dalloc(c, v->big, sizeof(BigTri) * (size_t)c->big_capacity);
     if (!rc) rc = dalloc(c, v->big_chunk, sizeof(uint32_t) * (size_t)c->big_capacity);
     const uint32_t tiles_x = (width + kBinTile - 1) / kBinTile, tiles_y = (height + kBinTile - 1) / kBinTile;
@@ -1197,9 +1221,11 @@ int vkgv_view_create(vkgv_ctx* c, uint32_t width, uint32_t height, vkgv_view** o
     d.slot_mat = static_cast<uint32_t*>(v->slot_mat.p);
     d.cache_slots = c->cache_slots;
     d.keys = static_cast<unsigned long long*>(v->keys.p);
+    d.keys_next = static_cast<unsigned long long*>(v->keys_b.p);
     d.color = static_cast<uint32_t*>(v->color.p);
     d.pyramid = static_cast<float*>(v->pyramid.p);
     d.counters = static_cast<Counters*>(v->counters.p);
+    d.counters_next = static_cast<Counters*>(v->counters_b.p);
     d.big = static_cast<BigTri*>(v->big.p);
     d.big_chunk = static_cast<uint32_t*>(v->big_chunk.p);
     d.big_capacity = c->big_capacity;
@@ -1223,12 +1249,14 @@ int vkgv_view_create(vkgv_ctx* c, uint32_t width, uint32_t height, vkgv_view** o
     // (the reference leaves it undefined, application.cpp:743); empty visbuffer.
     if (hipMemsetAsync(v->pyramid.p, 0, v->pyramid.bytes, c->stream) != hipSuccess
         || hipMemsetAsync(v->counters.p, 0, v->counters.bytes, c->stream) != hipSuccess
+        || hipMemsetAsync(v->counters_b.p, 0, v->counters_b.bytes, c->stream) != hipSuccess
         || hipMemsetAsync(v->color.p, 0, v->color.bytes, c->stream) != hipSuccess
         || hipMemsetAsync(v->bin_count.p, 0, v->bin_count.bytes, c->stream) != hipSuccess) {
         vkgv_view_destroy(v);
         return fail(c, VKGV_ERROR_HIP, "view_create: clear failed");
     }
     launch_clear_keys(d.keys, px, c->stream);
+    launch_clear_keys(d.keys_next, px, c->stream);
     for (auto& set : v->ev)
         for (auto& e : set)
             if (hipEventCreate(&e) != hipSuccess) { vkgv_view_destroy(v); return fail(c, VKGV_ERROR_HIP, "hipEventCreate"); }
@@ -1246,7 +1274,8 @@ void vkgv_view_destroy(vkgv_view* v)
         (void)hipStreamSynchronize(c->stream);
         if (v->last_stream) (void)hipStreamSynchronize(v->last_stream);
     }
-    for (DevBuf* b : { &v->keys, &v->color, &v->pyramid, &v->visible0, &v->visible1, &v->history, &v->counters, &v->big,
+    for (DevBuf* b : { &v->keys, &v->keys_b, &v->counters_b, &v->color, &v->pyramid, &v->visible0, &v->visible1,
+                       &v->history, &v->counters, &v->big,
                        &v->clipq, &v->bin_count, &v->bins, &v->setups, &v->tri_cache,
                        &v->slot_albedo, &v->slot_mat, &v->draw_slot, &v->big_chunk, &v->chunk_tri, &v->maps,
                        &v->aa_mat, &v->mvp_tab, &v->pmvp_tab, &v->motion, &v->redo, &v->frag_stage, &v->frag_jobs })
@@ -1274,11 +1303,12 @@ int vkgv_render(vkgv_ctx* c, vkgv_view* v, const vkgv_frame_desc* f)
     const uint32_t slot = v->ring_next;
     BatchDev b = v->batch;
     b.views = nullptr;
-    b.view0 = v->dev;
+    b.view0 = next_frame_view(v->dev);
     b.cams = nullptr;
     b.cam0.cam = f->camera;
     b.cam0.eye[0] = f->eye[0]; b.cam0.eye[1] = f->eye[1]; b.cam0.eye[2] = f->eye[2];
     frame_launches(c, b, v->grid, flags, st, [&](Ev e) { if (timed) (void)hipEventRecord(v->ev[slot][e], st); });
+    v->dev = b.view0;   // the frame's buffers are now the view's current ones
     HIP_TRY(c, hipGetLastError());
     if (timed) {
         v->ring_flags[slot] = flags;
@@ -1347,7 +1377,7 @@ static int copy_out(vkgv_ctx* c, vkgv_view* v, void* dst, const void* src, size_
 int vkgv_view_read_visbuffer(vkgv_ctx* c, vkgv_view* v, uint64_t* keys)
 {
     if (check_view(c, v)) return check_view(c, v);
-    return copy_out(c, v, keys, v->keys.p, 8ull * v->dev.width * v->dev.height);
+    return copy_out(c, v, keys, v->dev.keys, 8ull * v->dev.width * v->dev.height);
 }
 
 int vkgv_view_read_ids_depth(vkgv_ctx* c, vkgv_view* v, uint32_t* ids, float* depth)
@@ -1356,7 +1386,7 @@ int vkgv_view_read_ids_depth(vkgv_ctx* c, vkgv_view* v, uint32_t* ids, float* de
     if (rc) return rc;
     const size_t n = (size_t)v->dev.width * v->dev.height;
     std::vector<uint64_t> k(n);
-    if ((rc = copy_out(c, v, k.data(), v->keys.p, 8 * n))) return rc;
+    if ((rc = copy_out(c, v, k.data(), v->dev.keys, 8 * n))) return rc;
     for (size_t i = 0; i < n; ++i) {
         if (ids) ids[i] = 0xFFFFFFFEu - (uint32_t)k[i];
         if (depth) {
@@ -1488,7 +1518,7 @@ int vkgv_view_read_visible(vkgv_ctx* c, vkgv_view* v, uint32_t pass, uint32_t* o
     if (rc) return rc;
     if (pass > 1 || !count) return fail(c, VKGV_ERROR_INVALID_ARGUMENT, "read_visible: bad pass or null count");
     Counters h{};
-    if ((rc = copy_out(c, v, &h, v->counters.p, sizeof h))) return rc;
+    if ((rc = copy_out(c, v, &h, v->dev.counters, sizeof h))) return rc;
     *count = h.vis_count[pass];
     const uint32_t n = std::min(h.vis_count[pass], capacity);
     if (n && out) {
@@ -1507,7 +1537,7 @@ int vkgv_view_read_stats(vkgv_ctx* c, vkgv_view* v, vkgv_frame_stats* s)
     if (rc) return rc;
     if (!s) return fail(c, VKGV_ERROR_INVALID_ARGUMENT, "read_stats: null");
     Counters h{};
-    if ((rc = copy_out(c, v, &h, v->counters.p, sizeof h))) return rc;
+    if ((rc = copy_out(c, v, &h, v->dev.counters, sizeof h))) return rc;
     std::memset(s, 0, sizeof *s);
     s->draws = c->draw_count;
     s->visible_early = h.vis_count[0];
@@ -1598,8 +1628,11 @@ void vkgv_batch_destroy(vkgv_batch* b)
         if (b->last_stream) (void)hipStreamSynchronize(b->last_stream);
         (void)hipStreamSynchronize(c->stream);
     }
-    if (b->exec) (void)hipGraphExecDestroy(b->exec);
-    for (DevBuf* d : { &b->table, &b->maps, &b->cams }) dfree(*d);
+    for (uint32_t t = 0; t < vkgv_batch::kTables; ++t) {
+        if (b->exec[t]) (void)hipGraphExecDestroy(b->exec[t]);
+        dfree(b->table[t]);
+    }
+    for (DevBuf* d : { &b->maps, &b->cams }) dfree(*d);
     for (uint32_t k = 0; k < vkgv_batch::kStage; ++k) {
         if (b->staging[k]) (void)hipHostFree(b->staging[k]);
         if (b->staged[k]) (void)hipEventDestroy(b->staged[k]);
@@ -1628,13 +1661,33 @@ int vkgv_render_batch(vkgv_ctx* c, vkgv_batch* b, const vkgv_frame_desc* frames,
         if ((rc = ensure_lists(c, v))) return rc;
         if ((flags & VKGV_MOTION_VECTORS) && (rc = ensure_motion(c, v))) return rc;
     }
-    if (b->generation != c->generation) {
-        std::vector<const ViewDev*> devs;
-        for (vkgv_view* v : b->views) devs.push_back(&v->dev);
-        if ((rc = upload_table(c, devs, b->table, b->batch))) return rc;
-        b->generation = c->generation;
-        if (b->exec) { (void)hipGraphExecDestroy(b->exec); b->exec = nullptr; }
+    // the views as this frame uses them, and a device table of them
+    std::vector<ViewDev> next;
+    std::vector<const void*> keys;
+    for (vkgv_view* v : b->views) {
+        next.push_back(next_frame_view(v->dev));
+        keys.push_back(next.back().keys);
     }
+    uint32_t tb = vkgv_batch::kTables;
+    for (uint32_t t = 0; t < vkgv_batch::kTables; ++t)
+        if (b->table_generation[t] == c->generation && b->table_keys[t] == keys) tb = t;
+    if (tb == vkgv_batch::kTables) {
+        // an earlier frame may still read the slot: wait for it (only when a
+        // table is rebuilt, not in a steady run of frames)
+        if (b->last_stream) HIP_TRY(c, hipStreamSynchronize(b->last_stream));
+        tb = b->table_next;
+        b->table_next = (tb + 1) % vkgv_batch::kTables;
+        b->table_generation[tb] = ~0ull;
+        if (b->exec[tb]) { (void)hipGraphExecDestroy(b->exec[tb]); b->exec[tb] = nullptr; }
+        std::vector<const ViewDev*> devs;
+        for (const ViewDev& d : next) devs.push_back(&d);
+        BatchDev tmp = b->batch;
+        if ((rc = upload_table(c, devs, b->table[tb], tmp))) return rc;
+        b->table_keys[tb] = keys;
+        b->table_generation[tb] = c->generation;
+    }
+    BatchDev bd = b->batch;
+    bd.views = static_cast<const ViewDev*>(b->table[tb].p);
     hipStream_t st = frames[0].stream ? static_cast<hipStream_t>(frames[0].stream) : c->stream;
     for (vkgv_view* v : b->views)
         if (v->last_stream && v->last_stream != st) HIP_TRY(c, hipStreamSynchronize(v->last_stream));
@@ -1655,33 +1708,35 @@ int vkgv_render_batch(vkgv_ctx* c, vkgv_batch* b, const vkgv_frame_desc* frames,
     const uint32_t slot = b->ring_next;
     if (timed) HIP_TRY(c, hipEventRecord(b->ev0[slot], st));
     if (flags & VKGV_GRAPH) {
-        const bool stale = !b->exec || b->exec_flags != fl || b->exec_stream != st
-                           || b->exec_tuning.small_max_extent != c->tuning.small_max_extent
-                           || b->exec_tuning.raster_blocks_per_cu != c->tuning.raster_blocks_per_cu
-                           || b->exec_tuning.tile_item_tris != c->tuning.tile_item_tris
-                           || b->exec_tuning.resolve_cache_slots != c->tuning.resolve_cache_slots;
+        const RasterTuning& et = b->exec_tuning[tb];
+        const bool stale = !b->exec[tb] || b->exec_flags[tb] != fl || b->exec_stream[tb] != st
+                           || et.small_max_extent != c->tuning.small_max_extent
+                           || et.raster_blocks_per_cu != c->tuning.raster_blocks_per_cu
+                           || et.tile_item_tris != c->tuning.tile_item_tris
+                           || et.resolve_cache_slots != c->tuning.resolve_cache_slots;
         if (stale) {
-            if (b->exec) { (void)hipGraphExecDestroy(b->exec); b->exec = nullptr; }
+            if (b->exec[tb]) { (void)hipGraphExecDestroy(b->exec[tb]); b->exec[tb] = nullptr; }
             HIP_TRY(c, hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
-            frame_launches(c, b->batch, b->grid, fl, st, [](Ev) {});
+            frame_launches(c, bd, b->grid, fl, st, [](Ev) {});
             hipGraph_t g = nullptr;
             const hipError_t ce = hipStreamEndCapture(st, &g);
             if (ce != hipSuccess || !g)
                 return fail(c, VKGV_ERROR_HIP, std::string("render_batch: graph capture: ") + hipGetErrorString(ce));
-            const hipError_t ie = hipGraphInstantiate(&b->exec, g, nullptr, nullptr, 0);
+            const hipError_t ie = hipGraphInstantiate(&b->exec[tb], g, nullptr, nullptr, 0);
             (void)hipGraphDestroy(g);
             if (ie != hipSuccess) {
-                b->exec = nullptr;
+                b->exec[tb] = nullptr;
                 return fail(c, VKGV_ERROR_HIP, std::string("render_batch: graph instantiate: ") + hipGetErrorString(ie));
             }
-            b->exec_flags = fl;
-            b->exec_stream = st;
-            b->exec_tuning = c->tuning;
+            b->exec_flags[tb] = fl;
+            b->exec_stream[tb] = st;
+            b->exec_tuning[tb] = c->tuning;
         }
-        HIP_TRY(c, hipGraphLaunch(b->exec, st));
+        HIP_TRY(c, hipGraphLaunch(b->exec[tb], st));
     } else {
-        frame_launches(c, b->batch, b->grid, fl, st, [](Ev) {});
+        frame_launches(c, bd, b->grid, fl, st, [](Ev) {});
     }
+    for (size_t i = 0; i < b->views.size(); ++i) b->views[i]->dev = next[i];   // the frame's buffers are current
     HIP_TRY(c, hipGetLastError());
     if (timed) {
         HIP_TRY(c, hipEventRecord(b->ev1[slot], st));
@@ -1705,7 +1760,7 @@ int vkgv_batch_read_stats(vkgv_ctx* c, vkgv_batch* b, vkgv_frame_stats* s)
     std::memset(s, 0, sizeof *s);
     for (vkgv_view* v : b->views) {
         Counters h{};
-        HIP_TRY(c, hipMemcpy(&h, v->counters.p, sizeof h, hipMemcpyDeviceToHost));
+        HIP_TRY(c, hipMemcpy(&h, v->dev.counters, sizeof h, hipMemcpyDeviceToHost));
         s->draws += c->draw_count;
         s->visible_early += h.vis_count[0];
         s->visible_late += h.vis_count[1];

@@ -222,6 +222,27 @@ __global__ __launch_bounds__(kBlock) void cull_kernel(CullArgs a)
     // the wave's index, uniform in the compiler's eyes: unit, view group and
     // view loop bounds then live in scalar registers and the loops branch on them
     const uint32_t wave = __builtin_amdgcn_readfirstlane((blockIdx.x * kBlock + threadIdx.x) >> 6);
+    if (a.mode != CULL_LATE) {
+        // this frame's VP M and det(M) of every transform, per view, read by
+        // the raster and the resolve (the same arithmetic they used per
+        // meshlet and per pixel); the grid's threads share the entries
+        const uint32_t gt = blockIdx.x * kBlock + threadIdx.x, nt = gridDim.x * kBlock;
+        for (uint32_t vi = 0; vi < a.b.nviews; ++vi) {
+            const ViewDev& v = batch_view(a.b, vi);
+            const uint32_t count = min(a.s.transform_count, v.mvp_capacity);
+            if (gt >= count) continue;
+            const Mat4 vp = load_mat4(batch_camera(a.b, vi).cam.view_projection);
+            for (uint32_t t = gt; t < count; t += nt) {
+                const Mat4 m = load_mat4(a.s.transforms + 16ull * t);
+                const Mat4 mvp = mat4_mul(vp, m);
+                float4* o = reinterpret_cast<float4*>(v.mvp + (size_t)t * kMvpStride);
+#pragma unroll
+                for (int c = 0; c < 4; ++c)
+                    o[c] = make_float4(mvp.m[4 * c], mvp.m[4 * c + 1], mvp.m[4 * c + 2], mvp.m[4 * c + 3]);
+                o[4] = make_float4(det4(m), 0.0f, 0.0f, 0.0f);
+            }
+        }
+    }
     if (a.mode == CULL_LATE && wave == 0) {
         // the first raster pass's queue counters, zeroed for the late pass
         // (their last reader, that pass's tile kernel, has finished)
@@ -361,50 +382,32 @@ __global__ __launch_bounds__(kBlock) void cull_kernel(CullArgs a)
     }
 }
 
-// Frame start: each block clears kClearKeysPerBlock keys of its view
+// Next-frame clear, for frames without a resolve (which does it otherwise):
+// each block clears kClearKeysPerBlock keys of its view's next key buffer
 // (kClearKeysPerBlock / kBlock = 32 per thread, two per 16-byte store); the
-// view's first block also zeroes its counters, and the view's blocks zero the
-// per-tile triangle counts of both raster passes.
+// view's first block also zeroes the next frame's counters, and the view's
+// blocks zero the per-tile triangle counts of both raster passes.
 static_assert(kClearKeysPerBlock % (2 * kBlock) == 0, "the unrolled clear stores whole key pairs per thread");
-__global__ __launch_bounds__(kBlock) void frame_start_kernel(SceneDev s, BatchDev b)
+__global__ __launch_bounds__(kBlock) void next_frame_clear_kernel(BatchDev b)
 {
     const uint32_t e = b.map_clear[blockIdx.x];
     const uint32_t vi = e & 0xFFu, local = e >> 8;
     const ViewDev& v = batch_view(b, vi);
-    if (local == 0 && threadIdx.x < sizeof(Counters) / 4) reinterpret_cast<uint32_t*>(v.counters)[threadIdx.x] = 0u;
+    if (local == 0 && threadIdx.x < sizeof(Counters) / 4) reinterpret_cast<uint32_t*>(v.counters_next)[threadIdx.x] = 0u;
     const uint64_t n = (uint64_t)v.width * v.height;
     {
         const uint32_t nblk = (uint32_t)((n + kClearKeysPerBlock - 1) / kClearKeysPerBlock);
         const uint32_t ncount = 2u * tile_count_stride(v);
         for (uint32_t i = local * kBlock + threadIdx.x; i < ncount; i += nblk * kBlock) v.bin_count[i] = 0u;
     }
-    // this frame's VP M and det(M) of every transform, computed once per view
-    // and read by the raster and the resolve (the same arithmetic they used
-    // per meshlet and per pixel); the view's clear blocks share the entries
-    {
-        const uint32_t nblk = (uint32_t)((n + kClearKeysPerBlock - 1) / kClearKeysPerBlock);
-        const uint32_t count = min(s.transform_count, v.mvp_capacity);
-        if ((uint64_t)local * kBlock < count) {
-            const Mat4 vp = load_mat4(batch_camera(b, vi).cam.view_projection);
-            for (uint32_t t = local * kBlock + threadIdx.x; t < count; t += nblk * kBlock) {
-                const Mat4 m = load_mat4(s.transforms + 16ull * t);
-                const Mat4 mvp = mat4_mul(vp, m);
-                float4* o = reinterpret_cast<float4*>(v.mvp + (size_t)t * kMvpStride);
-#pragma unroll
-                for (int c = 0; c < 4; ++c)
-                    o[c] = make_float4(mvp.m[4 * c], mvp.m[4 * c + 1], mvp.m[4 * c + 2], mvp.m[4 * c + 3]);
-                o[4] = make_float4(det4(m), 0.0f, 0.0f, 0.0f);
-            }
-        }
-    }
     const uint64_t first = (uint64_t)local * kClearKeysPerBlock;
     const ulonglong2 c2 = make_ulonglong2(VKGV_KEY_CLEAR, VKGV_KEY_CLEAR);
-    ulonglong2* k2 = reinterpret_cast<ulonglong2*>(v.keys + first);
+    ulonglong2* k2 = reinterpret_cast<ulonglong2*>(v.keys_next + first);
     if (first + kClearKeysPerBlock <= n) {
 #pragma unroll
         for (uint32_t q = 0; q < kClearKeysPerBlock / 2u / kBlock; ++q) k2[q * kBlock + threadIdx.x] = c2;
     } else {
-        for (uint64_t i = first + threadIdx.x; i < n; i += kBlock) v.keys[i] = VKGV_KEY_CLEAR;
+        for (uint64_t i = first + threadIdx.x; i < n; i += kBlock) v.keys_next[i] = VKGV_KEY_CLEAR;
     }
 }
 
@@ -524,9 +527,9 @@ void launch_clear_keys(unsigned long long* keys, uint64_t n, hipStream_t st)
                                                            (uint32_t)(n & 1));
 }
 
-void launch_frame_start(const SceneDev& s, const BatchDev& b, const BatchGrid& g, hipStream_t st)
+void launch_next_frame_clear(const BatchDev& b, const BatchGrid& g, hipStream_t st)
 {
-    frame_start_kernel<<<g.clear_blocks, kBlock, 0, st>>>(s, b);
+    next_frame_clear_kernel<<<g.clear_blocks, kBlock, 0, st>>>(b);
 }
 
 void launch_prev_mvp(const SceneDev& s, const BatchDev& b, const BatchGrid& g, hipStream_t st)

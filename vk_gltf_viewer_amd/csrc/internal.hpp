@@ -218,14 +218,16 @@ constexpr uint32_t kBinTile = 32;   // screen tile of the binned path, pixels
 
 struct ViewDev {
     uint32_t width, height;
-    unsigned long long* keys;       // W*H visibility-buffer keys
+    unsigned long long* keys;       // W*H visibility-buffer keys of this frame
+    unsigned long long* keys_next;  // the other key buffer: cleared by this frame's resolve for the next frame
     uint32_t* color;                // W*H RGBA8
     float* pyramid;
     PyramidDesc pyr;
     RasterRec* visible[2];          // compacted visible draws (first pass / late pass)
     uint32_t* redo;                 // list positions of the pass's meshlets that found a queue full
     uint32_t* history;              // two-pass visibility bits, ceil(draws/64)*2 words
-    Counters* counters;
+    Counters* counters;             // this frame's
+    Counters* counters_next;        // the next frame's, zeroed by this frame's resolve
     BigTri* big;                    // big triangles of the current raster pass
     uint32_t* big_chunk;            // first 64-tile chunk of each big triangle
     uint32_t* chunk_tri;            // big triangle of each 64-tile chunk
@@ -282,7 +284,7 @@ struct BatchDev {
     CameraArg cam0;
     ViewDev view0;                  // a single-view frame's view, by value (no table load)
     uint32_t nviews;
-    const uint32_t* map_clear;      // frame-start (counter reset + key clear) blocks
+    const uint32_t* map_clear;      // next-frame clear blocks (frames without a resolve)
     const uint32_t* map_hiz;        // Hi-Z tile blocks
     const uint32_t* map_resolve;    // resolve blocks
 };
@@ -322,7 +324,7 @@ void launch_build_cull_records(const SceneDev& s, CullRec* out, RasterRec* rrec,
 void launch_clear_keys(unsigned long long* keys, uint64_t n, hipStream_t st);
 // Frame start of every view of the batch: all counters zeroed, keys cleared,
 // the view's table of VP M and det(M) per transform computed.
-void launch_frame_start(const SceneDev& s, const BatchDev& b, const BatchGrid& g, hipStream_t st);
+void launch_next_frame_clear(const BatchDev& b, const BatchGrid& g, hipStream_t st);
 // Every view's prevViewProjection * previous transform table (motion vectors).
 void launch_prev_mvp(const SceneDev& s, const BatchDev& b, const BatchGrid& g, hipStream_t st);
 // Persistent cull + compaction of every view's draws; `cus` sizes the grid.

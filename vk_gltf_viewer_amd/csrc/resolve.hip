@@ -247,6 +247,15 @@ __global__ __launch_bounds__(256) void resolve_kernel(ResolveBatchArgs ab)
     float2* const view_motion = view.motion;
     const float* const view_pmvp = view.pmvp;
     const uint32_t blk_x = local % view.resolve_blocks_x, blk_y = local / view.resolve_blocks_x;
+    // the next frame's state: its counters (first block) and the per-tile
+    // triangle counts of both raster passes (spread over the view's blocks)
+    {
+        if (local == 0 && threadIdx.x < sizeof(Counters) / 4)
+            reinterpret_cast<uint32_t*>(view.counters_next)[threadIdx.x] = 0u;
+        const uint32_t nblk = view.resolve_blocks_x * ((view.height + 4u * kRows - 1u) / (4u * kRows));
+        const uint32_t ncount = 2u * tile_count_stride(view);
+        for (uint32_t i = local * 256u + threadIdx.x; i < ncount; i += nblk * 256u) view.bin_count[i] = 0u;
+    }
 
     // LDS tables: sRGB thresholds and buckets, exact quotients b / 255 of a byte
     __shared__ float s_thr[256];
@@ -276,6 +285,13 @@ __global__ __launch_bounds__(256) void resolve_kernel(ResolveBatchArgs ab)
     for (int k = 0; k < kRows; ++k) {
         const uint32_t py = py0 + 4u * k;
         key[k] = py < a.H ? keys[(size_t)py * a.W + px] : VKGV_KEY_CLEAR;
+    }
+    // the next frame's key buffer, cleared where this frame's keys are read
+    // (the frame-start clear, folded into a pass that is not bound by HBM)
+#pragma unroll
+    for (int k = 0; k < kRows; ++k) {
+        const uint32_t py = py0 + 4u * k;
+        if (py < a.H) view.keys_next[(size_t)py * a.W + px] = VKGV_KEY_CLEAR;
     }
     // draw of each pixel; ~0u: no shading (clear key or a key not written by this draw list)
     uint32_t drw[kRows], slot[kRows];
