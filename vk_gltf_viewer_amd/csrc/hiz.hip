@@ -87,7 +87,9 @@ __global__ __launch_bounds__(256) void hiz_tiles_kernel(BatchDev b)
 // The levels after the tile pass's, one block per view.  A level small
 // enough for LDS is also kept there, so that the following levels read LDS
 // instead of global memory (one global round trip for the whole tail when
-// its first level fits).
+// its first level fits).  A texel's footprint depends only on its column
+// (x0, x1) and row (y0, y1), so each level's column and row footprints are
+// computed once into LDS (dw + dh <= kTailA + 1 entries) instead of per texel.
 constexpr uint32_t kTailA = kHizTailTexels, kTailB = kHizTailTexels / 2;   // floats of the two LDS level buffers
 constexpr uint32_t kTailPer = kTailA / 1024u;                             // texels per thread of a tail level
 static_assert(kTailA % 1024u == 0, "the tail's levels are whole rounds of its 1024 threads");
@@ -96,6 +98,7 @@ __global__ __launch_bounds__(1024) void hiz_tail_kernel(BatchDev b)
 {
     __shared__ float s_a[kTailA];
     __shared__ float s_b[kTailB];
+    __shared__ uint32_t s_fp[kTailA + 1];   // footprints: columns [0, dw), then rows [dw, dw + dh): t0 | t1 << 16
     const ViewDev& v = batch_view(b, blockIdx.x);
     const PyramidDesc& pd = v.pyr;
     float* pyr = v.pyramid;
@@ -107,6 +110,13 @@ __global__ __launch_bounds__(1024) void hiz_tail_kernel(BatchDev b)
         float* dst = pyr + pd.off[k];
         float* ldst = nullptr;
         if (dw * dh <= (next_a ? kTailA : kTailB)) ldst = next_a ? s_a : s_b;
+        for (uint32_t i = threadIdx.x; i < dw + dh; i += 1024u) {
+            int t0, t1;
+            if (i < dw) footprint(i, dw, sw, t0, t1);
+            else footprint(i - dw, dh, sh, t0, t1);
+            s_fp[i] = (uint32_t)t0 | ((uint32_t)t1 << 16);
+        }
+        __syncthreads();
         // a level holds at most kTailA texels: kTailPer per thread, all of a
         // thread's loads issued before the first minimum (the first level is
         // read from global memory, where one round trip each would add up)
@@ -115,10 +125,9 @@ __global__ __launch_bounds__(1024) void hiz_tail_kernel(BatchDev b)
         for (uint32_t r = 0; r < kTailPer; ++r) {
             const uint32_t t = threadIdx.x + r * 1024u;
             if (t < n) {
-                const uint32_t x = t % dw, y = t / dw;
-                int x0, x1, y0, y1;
-                footprint(x, dw, sw, x0, x1);
-                footprint(y, dh, sh, y0, y1);
+                const uint32_t y = t / dw, x = t - y * dw;
+                const uint32_t fx = s_fp[x], fy = s_fp[dw + y];
+                const uint32_t x0 = fx & 0xFFFFu, x1 = fx >> 16, y0 = fy & 0xFFFFu, y1 = fy >> 16;
                 float d = src[y0 * sw + x0];
                 d = gmin(d, src[y0 * sw + x1]);
                 d = gmin(d, src[y1 * sw + x0]);

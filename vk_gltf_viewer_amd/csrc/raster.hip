@@ -27,8 +27,8 @@
 //                 clip queue.  A
 //                 meshlet with a triangle that found either full is listed for
 //                 the direct path.
-//  clip_kernel    clips the queued triangles; their fans join the big queue
-//                 (or, without room, the direct path).  Then redoes the listed
+//  clip_kernel    clips the queued triangles, one per wave; their fans join the
+//                 big queue (or, without room, the direct path).  Then redoes the listed
 //                 meshlets: each of their triangles that is not small takes the
 //                 direct path, where the wave sweeps one triangle's bounding
 //                 box at a time, so that no triangle is lost when a queue is
@@ -415,15 +415,9 @@ __device__ __forceinline__ bool push_big_wave(bool want, const TriSetup& t, uint
 // non-finite or degenerate input can produce more crossings: such a polygon
 // is dropped (returns 0) and nothing is written past kClipMax.
 constexpr int kClipMax = 16;
-constexpr uint32_t kClipThreads = 64;   // one wave per clip block; polygons live in LDS
+constexpr uint32_t kClipThreads = 64;   // one wave per clip block
 
-// A lane's polygon in LDS: vertex k at base[k * kClipThreads] (bank-conflict free).
-struct StridedPoly {
-    float4* base;
-    __device__ __forceinline__ float4& operator[](int k) const { return base[k * (int)kClipThreads]; }
-};
-
-// A lane's polygon in its private memory.
+// A polygon of one lane: vertex k at base[k] (private memory or LDS).
 struct LocalPoly {
     float4* base;
     __device__ __forceinline__ float4& operator[](int k) const { return base[k]; }
@@ -820,38 +814,39 @@ struct BigArgs {
     uint32_t job_cap;
 };
 
-// One lane per queued triangle, in its own LDS polygon buffers; the fan
-// joins the big triangles, or, without room, the wave's direct sweep.  The
-// block is one wave, and every lane runs the same iterations.
+// One queued triangle per wave at a time: lane 0 clips it in the wave's LDS
+// polygon buffers, then lane f - 1 sets up fan triangle (p0, pf, pf+1),
+// f = 1 .. m - 2, and the fans join the big triangles with one reservation
+// for the wave; a fan without room is swept by the wave.  The block is one
+// wave.
 template <bool STATS>
 __device__ __forceinline__ void clip_view(const BigArgs& a)
 {
     RasterCount<STATS> rc;
-    __shared__ float4 s_poly[2][kClipMax][kClipThreads];
-    const StridedPoly pa{ &s_poly[0][0][threadIdx.x] }, pb{ &s_poly[1][0][threadIdx.x] };
+    __shared__ float4 s_poly[2 * kClipMax];
+    const uint32_t lane = threadIdx.x & 63u;
+    const LocalPoly pa{ s_poly }, pb{ s_poly + kClipMax };
     const BigQueue q{ a.big, a.big_chunk, a.big_cap, a.chunk_tri, a.chunk_cap, a.ctr, a.keys, a.W };
     const uint32_t n = min(a.ctr->clip_count, a.clip_cap);
     const float hw = (float)a.W * 0.5f, hh = (float)a.H * 0.5f;
-    for (uint32_t i0 = blockIdx.x * kClipThreads; i0 < n; i0 += gridDim.x * kClipThreads) {
-        const uint32_t i = i0 + threadIdx.x;
-        ClipTri ct;
-        ct.id = 0u;
+    for (uint32_t i = blockIdx.x; i < n; i += gridDim.x) {
+        const ClipTri ct = a.clipq[i];
+        const uint32_t id = __builtin_amdgcn_readfirstlane(ct.id);
         int m = 0;
-        if (i < n) {
-            ct = a.clipq[i];
+        if (lane == 0) {
             m = clip_polygon(pa, pb, ct.c[0], ct.c[1], ct.c[2], a.W, a.H);
             if (m) {
                 atomicAdd(&a.ctr->clipped, 1u);
                 if (!polygon_projects(pb, m, hw, hh)) m = 0;
             }
         }
-        for (int f = 1; f + 1 < kClipMax; ++f) {
-            if (__ballot(f + 1 < m) == 0ull) break;   // wave-uniform
-            TriSetup t;
-            bool spill = false;
-            if (f + 1 < m && fan_setup(pb, f, hw, hh, (int)a.W, (int)a.H, t)) spill = !push_big(t, ct.id, q);
-            wave_sweep(spill, t, ct.id, a.keys, a.W, rc);
-        }
+        m = __shfl(m, 0);
+        wave_sync();   // lane 0's polygon is visible to the wave
+        TriSetup t;
+        const bool fan = (int)lane + 2 < m && fan_setup(pb, (int)lane + 1, hw, hh, (int)a.W, (int)a.H, t);
+        const bool spill = push_big_wave(fan, t, id, lane, q);
+        if (__ballot(spill) != 0ull) wave_sweep(spill, t, id, a.keys, a.W, rc);
+        wave_sync();   // the polygon buffers are reused by the wave's next triangle
     }
     rc.flush(a.ctr);
 }
@@ -1099,7 +1094,7 @@ __device__ __forceinline__ void tile_item(const BigArgs& a, uint32_t tile, uint3
 #pragma unroll
             for (int q = 0; q < kSetupWords; ++q) dst[q] = __builtin_amdgcn_readlane(sw[q], j);
             const int bx0 = (int)(s.bb0 & 0xFFFFu), by0 = (int)(s.bb0 >> 16);
-            const int bx1 = (int)(s.bb1 & 0xFFFFu), by1 = (int)(s.bb1 >> 16);
+            const int bx1 = (int)(s.bb1 & 0xFFFFu);
             const bool col_ok = px >= bx0 && px <= bx1;
             long long ev[3];
 #pragma unroll
@@ -1108,21 +1103,32 @@ __device__ __forceinline__ void tile_item(const BigArgs& a, uint32_t tile, uint3
                         + (long long)(py0 - by0) * (long long)s.ex[q] * 256;
             const float dxf = (float)(sx - s.x0) * 0.00390625f;
             if (!col_ok) continue;
+            // Rows without branches: a centre is covered iff its three edge
+            // functions are >= 0 (the sign of their OR's high word).  A covered
+            // centre lies in the triangle's bounding box, so no box test is
+            // needed; centres outside the view are never stored.  Depth is
+            // the same expression as everywhere, with za * dxf formed once.
+            const float zx = s.za * dxf;
+            const int ybase = py0 * 256 + 128 - s.y0;
+            const uint32_t idw = 0xFFFFFFFEu - s.id;
 #pragma unroll
             for (int r = 0; r < kTileRows; ++r) {
-                const int py = py0 + 2 * r;
                 if (r) {
 #pragma unroll
                     for (int q = 0; q < 3; ++q) ev[q] += (long long)s.ex[q] * 512;   // +2 rows
                 }
-                if (py < by0 || py > by1) continue;
-                if ((ev[0] | ev[1] | ev[2]) < 0) continue;
-                const float dyf = (float)(py * 256 + 128 - s.y0) * 0.00390625f;
-                const float z = s.z0 + (s.za * dxf + s.zb * dyf);
-                if (!(z >= 0.0f && z <= 1.0f)) continue;
-                const unsigned long long k = make_key(z, s.id);
+                const int hi = (int)(ev[0] >> 32) | (int)(ev[1] >> 32) | (int)(ev[2] >> 32);
+                const float dyf = (float)(ybase + 512 * r) * 0.00390625f;
+                const float z = s.z0 + (zx + s.zb * dyf);
+                const bool ok = (hi >= 0) & (z >= 0.0f) & (z <= 1.0f);
+                // an uncovered centre's key is the clear key, never below key[r]
+                const uint32_t khi = ok ? ~__float_as_uint(z + 0.0f) : 0xFFFFFFFFu;
+                const uint32_t klo = ok ? idw : 0xFFFFFFFFu;
+                const unsigned long long k = ((unsigned long long)khi << 32) | klo;
                 key[r] = k < key[r] ? k : key[r];
-                rc.frag();
+                if constexpr (STATS) {
+                    if (ok && py0 + 2 * r < (int)a.H) rc.frag();   // centres below the view are never stored
+                }
             }
         }
     }
