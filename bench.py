@@ -454,26 +454,27 @@ def timed_frames(r, view, cam, flags, steps, warmup):
 
 def pass_table(sc, view, st, counts, W, H):
     """Per-pass ms and algorithmic bytes (per unit x units):
-      clear       8 B per pixel written (64-bit key)
       cull_early  36 B per draw (12 B record + 24 B AABB) + 64 B per instance
-                  transform + 4 B per visible draw id written
+                  transform + 4 B per visible draw id written, + 80 B per
+                  transform of the frame's VP M table written
       cull_late   the same + 16 B per Hi-Z probe (2x2 fp32; at most one per draw)
       raster      12 B per vertex + 3 B per triangle of every rasterized meshlet
                   + 8 B per fragment (one 64-bit key update), both passes
       hiz         4 B per depth pixel read + 4 B per pyramid texel written
-      resolve     8 B key read + 4 B RGBA8 written per pixel + 39 B per visible
-                  triangle (3 x 12 B positions + 3 B indices)"""
+      resolve     8 B key read + 4 B RGBA8 written + 8 B clear key written
+                  into the next frame's visibility buffer per pixel + 39 B
+                  per visible triangle (3 x 12 B positions + 3 B indices)
+    (The frame has no clear pass: the resolve clears the next frame's keys.)"""
     npx = W * H
     vis = np.concatenate([view.visible(0), view.visible(1)])
     n_draw, n_inst = len(sc.draws), len(sc.transforms)
-    ms = {"clear": st.ms_clear, "cull_early": st.ms_cull, "cull_late": st.ms_cull_late,
+    ms = {"cull_early": st.ms_cull, "cull_late": st.ms_cull_late,
           "raster": st.ms_raster + st.ms_raster_late, "hiz": st.ms_hiz, "resolve": st.ms_resolve}
-    alg = {"clear": 8.0 * npx,
-           "cull_early": 36.0 * n_draw + 64.0 * n_inst + 4.0 * st.visible_early,
+    alg = {"cull_early": 36.0 * n_draw + 144.0 * n_inst + 4.0 * st.visible_early,
            "cull_late": 36.0 * n_draw + 64.0 * n_inst + 16.0 * n_draw + 4.0 * st.visible_late,
            "raster": float(meshlet_bytes(sc, vis)) + 8.0 * counts.fragments,
            "hiz": 4.0 * npx + 4.0 * pyramid_texels(W, H),
-           "resolve": 12.0 * npx + 39.0 * visible_triangles(sc, vis)}
+           "resolve": 20.0 * npx + 39.0 * visible_triangles(sc, vis)}
     return ms, alg, vis
 
 

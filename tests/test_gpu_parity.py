@@ -315,12 +315,12 @@ def test_raster_stats_match_oracle(rend, cfg):
 
 
 def test_binned_path_both_scans(rend):
-    """The big-triangle lists are built by a scan folded into the fill pass for
-    views of at most 8192 screen tiles (32 px), and by a separate one-block
-    scan above that.  A 4096 x 2304 view (9216 tiles) takes the separate scan;
-    a 1280 x 720 view alone takes the folded one, and in a batch with the
-    large view the separate one: frames stay equal to the oracle's in every
-    order (the per-tile cursors of the folded scan stay zero in between)."""
+    """The tile pass finds its work items by a scan of the per-tile counts in
+    every block for views of at most 8192 screen tiles (32 px), and by a
+    static enumeration above that.  A 4096 x 2304 view (9216 tiles) takes the
+    static path, a 1280 x 720 view the scan, alone and in one batch: frames
+    stay equal to the oracle's in every order (the per-tile counts of both
+    raster passes are zeroed for the next frame in between)."""
     sc = scene(2, 2)
     load(rend, sc)
     vd = sc.views[0]

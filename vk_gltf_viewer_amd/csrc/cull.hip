@@ -225,22 +225,24 @@ __global__ __launch_bounds__(kBlock) void cull_kernel(CullArgs a)
     if (a.mode != CULL_LATE) {
         // this frame's VP M and det(M) of every transform, per view, read by
         // the raster and the resolve (the same arithmetic they used per
-        // meshlet and per pixel); the grid's threads share the entries
+        // meshlet and per pixel); the grid's threads share the (view,
+        // transform) entries, one each, so that a batch's views run side by
+        // side rather than one after another on the same threads
         const uint32_t gt = blockIdx.x * kBlock + threadIdx.x, nt = gridDim.x * kBlock;
-        for (uint32_t vi = 0; vi < a.b.nviews; ++vi) {
+        const uint32_t tc = a.s.transform_count;
+        const unsigned long long entries = (unsigned long long)tc * a.b.nviews;
+        for (unsigned long long e = gt; e < entries; e += nt) {
+            const uint32_t vi = (uint32_t)(e / tc), t = (uint32_t)(e - (unsigned long long)vi * tc);
             const ViewDev& v = batch_view(a.b, vi);
-            const uint32_t count = min(a.s.transform_count, v.mvp_capacity);
-            if (gt >= count) continue;
+            if (t >= v.mvp_capacity) continue;
             const Mat4 vp = load_mat4(batch_camera(a.b, vi).cam.view_projection);
-            for (uint32_t t = gt; t < count; t += nt) {
-                const Mat4 m = load_mat4(a.s.transforms + 16ull * t);
-                const Mat4 mvp = mat4_mul(vp, m);
-                float4* o = reinterpret_cast<float4*>(v.mvp + (size_t)t * kMvpStride);
+            const Mat4 m = load_mat4(a.s.transforms + 16ull * t);
+            const Mat4 mvp = mat4_mul(vp, m);
+            float4* o = reinterpret_cast<float4*>(v.mvp + (size_t)t * kMvpStride);
 #pragma unroll
-                for (int c = 0; c < 4; ++c)
-                    o[c] = make_float4(mvp.m[4 * c], mvp.m[4 * c + 1], mvp.m[4 * c + 2], mvp.m[4 * c + 3]);
-                o[4] = make_float4(det4(m), 0.0f, 0.0f, 0.0f);
-            }
+            for (int c = 0; c < 4; ++c)
+                o[c] = make_float4(mvp.m[4 * c], mvp.m[4 * c + 1], mvp.m[4 * c + 2], mvp.m[4 * c + 3]);
+            o[4] = make_float4(det4(m), 0.0f, 0.0f, 0.0f);
         }
     }
     if (a.mode == CULL_LATE && wave == 0) {

@@ -901,20 +901,18 @@ __device__ __forceinline__ void frag_job(const BigArgs& a, uint32_t j, RasterCou
 // tile 64*chunk + l: a tile is skipped when one edge is negative at all four
 // corner pixel centres of the tile's part of the bounding box (no centre there
 // can be covered); a kept tile appends the triangle to its list.
+//
+// The wave takes items first, first + nw, ... of the view's nitems (chunks,
+// then runs).
 template <bool STATS>
-__device__ __forceinline__ void bin_view(const BigArgs& a)
+__device__ __forceinline__ void bin_view(const BigArgs& a, uint32_t first, uint32_t nw, uint32_t nitems)
 {
     RasterCount<STATS> rc;
     uint32_t pairs = 0u, most = 0u;   // (tile, triangle) pairs binned, longest tile list (STATS)
-    const uint32_t w = threadIdx.x >> 6, lane = threadIdx.x & 63u;
-    const uint32_t gw = w * gridDim.x + blockIdx.x, nw = gridDim.x * 4u;
+    const uint32_t lane = threadIdx.x & 63u;
     uint32_t nbig, nchunks;
     pass_extent(a, nbig, nchunks);
-    if (gw == 0 && lane == 0 && nbig) atomicAdd(&a.ctr->big_total_tris, nbig);
-    const uint32_t njobs =
-        (uint32_t)min(a.ctr->frag_jobs & ((1ull << kJobCountBits) - 1ull), (unsigned long long)a.job_cap);
-    if (nchunks == 0 && njobs == 0) return;   // uniform over the grid
-    for (uint32_t c = gw; c < nchunks + njobs; c += nw) {
+    for (uint32_t c = first; c < nitems; c += nw) {
         if (c >= nchunks) {
             frag_job<STATS>(a, c - nchunks, rc);
             continue;
@@ -1193,7 +1191,9 @@ __device__ __forceinline__ void tile_view_static(const BigArgs& a, uint32_t item
 // per item; otherwise it writes the tiles' first items and a wave finds an
 // item's tile by binary search.  The redundant per-block scan costs a few
 // microseconds; it replaces a launch, and the compact list keeps every wave
-// within one item of the average.
+// within one item of the average.  In a batch, a view's items continue the
+// previous scanned view's wave rotation, so views with few items each still
+// spread over the grid.
 constexpr uint32_t kTileThreads = 1024;
 constexpr uint32_t kScanTiles = 8192;
 constexpr uint32_t kScanPer = kScanTiles / kTileThreads;   // tiles per thread in the scan
@@ -1203,7 +1203,7 @@ static_assert(kScanTiles <= (1u << 13) && kTileSplit <= 8, "item records: 13-bit
 
 template <bool STATS>
 __device__ __forceinline__ void tile_view_scan(const BigArgs& a, uint32_t item_tris, uint32_t* s_arr,
-                                               uint32_t* s_wsum, RasterCount<STATS>& rc)
+                                               uint32_t* s_wsum, uint32_t& carry, RasterCount<STATS>& rc)
 {
     const uint32_t tid = threadIdx.x, lane = tid & 63u, w = tid >> 6;
     const uint32_t ntiles = a.tiles_x * a.tiles_y;   // at most kScanTiles
@@ -1255,7 +1255,10 @@ __device__ __forceinline__ void tile_view_scan(const BigArgs& a, uint32_t item_t
     }
     __syncthreads();
     const uint32_t nwaves = gridDim.x * (kTileThreads / 64);
-    for (uint32_t hv = blockIdx.x + gridDim.x * w; hv < kTileParts * total; hv += nwaves) {
+    // the view's items start where the previous view's ended (mod waves)
+    const uint32_t first = (blockIdx.x + gridDim.x * w + nwaves - carry) % nwaves;
+    carry = (carry + kTileParts * total) % nwaves;
+    for (uint32_t hv = first; hv < kTileParts * total; hv += nwaves) {
         const uint32_t item = hv / (uint32_t)kTileParts;
         uint32_t tile, part, split, n;
         if (records) {
@@ -1505,17 +1508,44 @@ __global__ __launch_bounds__(kClipThreads) void clip_kernel(BatchDev b, RedoArgs
     }
 }
 
-// Per view: the big triangles' chunks, then the deferred fragment runs.
+// Per view: the big triangles' chunks, then the deferred fragment runs.  The
+// views' items are dealt out as one concatenated list, as the raster kernel
+// deals its meshlets (view j's item i goes to wave (i + items of the views
+// before j) mod waves), so that a batch of views with few items each keeps
+// the whole grid busy instead of running the views one after another.
 template <bool STATS>
 __global__ __launch_bounds__(256) void bin_kernel(BatchDev b, uint32_t pass)
 {
+    const uint32_t lane = threadIdx.x & 63u;
+    const uint32_t gw = (threadIdx.x >> 6) * gridDim.x + blockIdx.x, nw = gridDim.x * 4u;
+    uint32_t carry = 0;   // items of the views before v0, mod nw
     for (uint32_t v0 = 0; v0 < b.nviews; v0 += 64u) {
-        const uint32_t n = lane_view_value(b, v0, [](const Counters& c) {
-            return c.big_count | (c.frag_jobs != 0ull ? 1u : 0u);
-        });
+        uint32_t cnt = 0;
+        if (v0 + lane < b.nviews) {
+            const ViewDev& v = batch_view(b, v0 + lane);
+            const Counters& c = *v.counters;
+            const uint32_t nbig = min(c.big_count, v.big_capacity);
+            const uint32_t nchunks = (uint32_t)min(c.big_chunks, (unsigned long long)v.chunk_capacity);
+            const uint32_t njobs =
+                (uint32_t)min(c.frag_jobs & ((1ull << kJobCountBits) - 1ull), (unsigned long long)v.job_capacity);
+            cnt = nchunks + njobs;
+            if (gw == 0 && nbig) atomicAdd(&v.counters->big_total_tris, nbig);
+        }
+        uint32_t incl = cnt % nw;
+#pragma unroll
+        for (uint32_t off = 1; off < 64u; off <<= 1) {
+            const uint32_t y = __shfl_up(incl, off);
+            if (lane >= off) incl = (incl + y) % nw;
+        }
+        const uint32_t excl = (incl + nw - cnt % nw) % nw;
         const uint32_t m = min(64u, b.nviews - v0);
-        for (uint32_t j = 0; j < m; ++j)
-            if (__builtin_amdgcn_readlane(n, j)) bin_view<STATS>(big_args(batch_view(b, v0 + j), pass));
+        for (uint32_t j = 0; j < m; ++j) {
+            const uint32_t count = __builtin_amdgcn_readlane(cnt, j);
+            const uint32_t rot = (carry + __builtin_amdgcn_readlane(excl, j)) % nw;
+            const uint32_t first = (gw + nw - rot) % nw;
+            if (first < count) bin_view<STATS>(big_args(batch_view(b, v0 + j), pass), first, nw, count);
+        }
+        carry = (carry + __builtin_amdgcn_readlane(incl, 63)) % nw;
     }
 }
 
@@ -1526,6 +1556,7 @@ __global__ __launch_bounds__(kTileThreads) void tile_kernel(BatchDev b, uint32_t
     __shared__ uint32_t s_arr[kScanTiles];
     __shared__ uint32_t s_wsum[kTileThreads / 64];
     RasterCount<STATS> rc;
+    uint32_t carry = 0;   // items of the scanned views before this one, mod waves
     for (uint32_t v0 = 0; v0 < b.nviews; v0 += 64u) {
         const uint32_t n = lane_view_value(b, v0, big_count);
         const uint32_t m = min(64u, b.nviews - v0);
@@ -1533,7 +1564,7 @@ __global__ __launch_bounds__(kTileThreads) void tile_kernel(BatchDev b, uint32_t
             if (!__builtin_amdgcn_readlane(n, j)) continue;
             const ViewDev& v = batch_view(b, v0 + j);
             const BigArgs a = big_args(v, pass);
-            if (v.tiles_x * v.tiles_y <= kScanTiles) tile_view_scan<STATS>(a, item_tris, s_arr, s_wsum, rc);
+            if (v.tiles_x * v.tiles_y <= kScanTiles) tile_view_scan<STATS>(a, item_tris, s_arr, s_wsum, carry, rc);
             else tile_view_static<STATS>(a, item_tris, rc);
             rc.flush(a.ctr);
         }
