@@ -140,6 +140,51 @@ __device__ __forceinline__ void wave_sync()
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
+// A wave-uniform value read by a vector load, moved to a scalar register.
+__device__ __forceinline__ uint32_t uni(uint32_t x) { return (uint32_t)__builtin_amdgcn_readfirstlane((int)x); }
+__device__ __forceinline__ int uni(int x) { return __builtin_amdgcn_readfirstlane(x); }
+template <typename T>
+__device__ __forceinline__ T* uni(T* p)
+{
+    const unsigned long long u = (unsigned long long)p;
+    const uint32_t lo = uni((uint32_t)u), hi = uni((uint32_t)(u >> 32));
+    return (T*)(((unsigned long long)hi << 32) | lo);
+}
+
+// Loads and stores through global (address space 1) pointers.  A pointer
+// read from memory is generic to the compiler, and a generic (flat) access
+// also counts in the LDS counter, so every later LDS wait of the wave would
+// wait for it; a global one is counted with vector memory only.
+template <typename T>
+__device__ __forceinline__ T gload(const T* p)
+{
+    return *(const __attribute__((address_space(1))) T*)p;
+}
+template <typename T>
+__device__ __forceinline__ void gstore(T* p, const T& v)
+{
+    *(__attribute__((address_space(1))) T*)p = v;
+}
+// 16- and 8-byte records through the compiler's own vector types (HIP's
+// vector classes have no assignment to an address-space-qualified object)
+typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
+typedef uint32_t u32x2 __attribute__((ext_vector_type(2)));
+__device__ __forceinline__ void gstore(uint4* p, const uint4& v)
+{
+    const u32x4 x = { v.x, v.y, v.z, v.w };
+    *(__attribute__((address_space(1))) u32x4*)p = x;
+}
+__device__ __forceinline__ void gstore(float4* p, const float4& v)
+{
+    gstore(reinterpret_cast<uint4*>(p),
+           make_uint4(__float_as_uint(v.x), __float_as_uint(v.y), __float_as_uint(v.z), __float_as_uint(v.w)));
+}
+__device__ __forceinline__ void gstore(FragJob* p, const FragJob& v)
+{
+    const u32x2 x = { v.stage, v.first };
+    *(__attribute__((address_space(1))) u32x2*)p = x;
+}
+
 __device__ __forceinline__ bool project_vtx(float4 c, float hw, float hh, SVtx& o)
 {
     if (!(c.w > 0.0f)) return false;
@@ -573,6 +618,7 @@ __device__ __forceinline__ void raster_view(const RasterArgs& a)
     __shared__ uint32_t s_tri[kWaves][96];
     __shared__ __align__(16) SmallTri s_st[kWaves][VKGV_MESHLET_TRIANGLES];
     __shared__ __align__(16) uint32_t s_pre[kWaves][128];
+    __shared__ uint32_t s_list[kWaves][128];   // the meshlet's triangles past the per-slot tests
 
     const int w = threadIdx.x >> 6;
     const uint32_t lane = threadIdx.x & 63u;
@@ -588,111 +634,203 @@ __device__ __forceinline__ void raster_view(const RasterArgs& a)
     __shared__ uint32_t s_spill[kWaves];
     if (lane == 0) s_spill[w] = 0u;
 
-    // The raster record of the wave's next meshlet is loaded while the
-    // current one is processed.
-    const uint4* list4 = reinterpret_cast<const uint4*>(a.list);
-    uint32_t idx = a.first;
-    uint4 nx0 = make_uint4(0u, 0u, 0u, 0u), nx1 = nx0;
-    if (idx < count) { nx0 = list4[2u * idx]; nx1 = list4[2u * idx + 1u]; }
+    // A software pipeline over the wave's meshlets i = first, first + nwaves,
+    // ...: each input of a meshlet is loaded while earlier meshlets are
+    // rasterized, so that a meshlet waits on memory once (the drain at its
+    // start, which also covers the previous meshlet's fragment atomics) instead
+    // of once per dependent load (record -> vertex indices -> vertices):
+    //   during meshlet i:  record of i + 3       (lane k < 8: word k)
+    //                      vertex indices of i + 2 (lane l < vertex count)
+    //                      vertices, VP M row (lane k < 17: float k) and
+    //                      triangle words (lanes l and l + 64) of i + 1
+    // Records and rows are held one word per lane and read with readlane.
+    const uint32_t* listw = reinterpret_cast<const uint32_t*>(a.list);
+    const uint32_t* vidx_base = a.s.vidx_base;
+    const uint8_t* vtx_bytes = reinterpret_cast<const uint8_t*>(a.s.vtx_base);
+    const bool tex = (a.flags & VKGV_RESOLVE_TEXTURED) != 0;   // the textured resolve reads the uvs too
+    uint32_t idx = __builtin_amdgcn_readfirstlane(a.first);
+    // record word `lane` of list entry j (0 past the list)
+    auto rec_word = [&](uint32_t j) -> uint32_t {
+        return (j < count && lane < 8u) ? gload(listw + (size_t)j * 8u + lane) : 0u;
+    };
+    auto field = [](uint32_t r, int k) -> uint32_t { return (uint32_t)__builtin_amdgcn_readlane((int)r, k); };
+    // vertex index of lane `lane` of the meshlet with record r
+    auto vidx_of = [&](uint32_t r, bool live) -> uint32_t {
+        const uint32_t vc = field(r, 6) & 0xFFu;
+        return (live && lane < vc) ? gload(vidx_base + field(r, 2) + lane) : 0u;
+    };
+    struct Fetch {
+        float x, y, z;      // vertex position
+        uint32_t col, uv;   // vertex colour and uv words
+        uint32_t row;       // VP M row word (lanes 0..16)
+        uint32_t t0, t1;    // triangle words lane, lane + 64
+        uint32_t mat;       // material albedo factor word (lanes 0..3)
+    };
+    // the inputs of the meshlet with record r and vertex indices vi
+    auto fetch = [&](uint32_t r, uint32_t vi, bool live, Fetch& f) {
+        f.x = f.y = f.z = 0.0f;
+        f.col = f.uv = f.row = f.t0 = f.t1 = f.mat = 0u;
+        if (!live) return;
+        const uint32_t vc = field(r, 6) & 0xFFu, tc = (field(r, 6) >> 8) & 0xFFu;
+        if (lane < vc) {
+            const float* pp = reinterpret_cast<const float*>(vtx_bytes + field(r, 4)) + (size_t)vi * 6;
+            f.x = gload(pp);
+            f.y = gload(pp + 1);
+            f.z = gload(pp + 2);
+            f.col = gload(reinterpret_cast<const uint32_t*>(pp) + 3);
+            if (tex) f.uv = gload(reinterpret_cast<const uint32_t*>(pp) + 5);
+        }
+        if (lane < 17u) f.row = __float_as_uint(gload(a.mvp + (size_t)field(r, 1) * kMvpStride + lane));
+        if (lane < 4u) f.mat = __float_as_uint(gload(&a.s.materials[field(r, 5) & 0x7FFFFFFFu].albedo_factor[lane]));
+        const uint32_t* src = reinterpret_cast<const uint32_t*>(a.s.tri_base + field(r, 3));
+        const uint32_t ndw = (tc * 3u + 3u) >> 2;
+        if (lane < ndw) f.t0 = gload(src + lane);
+        if (lane + 64u < ndw) f.t1 = gload(src + lane + 64u);
+    };
+    uint32_t r0 = rec_word(idx), r1 = rec_word(idx + nwaves), r2 = rec_word(idx + 2u * nwaves);
+    uint32_t x1 = vidx_of(r1, idx + nwaves < count);
+    Fetch f0;
+    fetch(r0, vidx_of(r0, idx < count), idx < count, f0);
     while (idx < count) {
-        const uint32_t d = __builtin_amdgcn_readfirstlane(nx0.x);
-        const uint32_t tr = __builtin_amdgcn_readfirstlane(nx0.y);
-        const uint32_t vo = __builtin_amdgcn_readfirstlane(nx0.z);
-        const uint32_t to = __builtin_amdgcn_readfirstlane(nx0.w);
-        const uint32_t vb = __builtin_amdgcn_readfirstlane(nx1.x);
-        const uint32_t matw = __builtin_amdgcn_readfirstlane(nx1.y);
-        const uint32_t counts = __builtin_amdgcn_readfirstlane(nx1.z);
+        const uint32_t d = field(r0, 0);
+        const uint32_t matw = field(r0, 5);
+        const uint32_t counts = field(r0, 6);
         const uint32_t nidx = idx + nwaves;
-        if (nidx < count) { nx0 = list4[2u * nidx]; nx1 = list4[2u * nidx + 1u]; }
-        // VP M and det(M) of the transform, computed once per frame at frame start
-        const float* mrow = a.mvp + (size_t)tr * kMvpStride;
-        const Mat4 mvp = load_mat4(mrow);
-        const float tdet = mrow[16];
+        // VP M and det(M) of the transform, computed once per frame by the first cull
+        Mat4 mvp;
+#pragma unroll
+        for (int k = 0; k < 16; ++k) mvp.m[k] = __uint_as_float(field(f0.row, k));
+        const float tdet = __uint_as_float(field(f0.row, 16));
         const bool cull_faces = !(matw >> 31) && !(a.flags & VKGV_NO_BACKFACE);
         // the flat resolve reads only the slot albedo; the interpolating ones the triangles too
         const bool bary = (a.flags & (VKGV_RESOLVE_BARY | VKGV_RESOLVE_TEXTURED | VKGV_MOTION_VECTORS)) != 0;
-        const bool tex = (a.flags & VKGV_RESOLVE_TEXTURED) != 0;   // the textured resolve reads the uvs too
         const uint32_t vc = counts & 0xFFu, tc = (counts >> 8) & 0xFFu;
         // resolve-cache slot of this meshlet (none when the cache is full)
         const uint32_t slot = slot0 + idx;
         const bool cached = slot < a.cache_slots;
-        if (lane == 0) {
-            a.draw_slot[d] = cached ? slot : 0xFFFFFFFFu;
-            if (cached) {
-                const vkgv_material& mat = a.s.materials[matw & 0x7FFFFFFFu];
-                a.slot_albedo[slot] = make_float4(mat.albedo_factor[0], mat.albedo_factor[1], mat.albedo_factor[2],
-                                                  mat.albedo_factor[3]);
-                a.slot_mat[slot] = matw & 0x7FFFFFFFu;
+        {
+            const float4 albedo = make_float4(__uint_as_float(field(f0.mat, 0)), __uint_as_float(field(f0.mat, 1)),
+                                              __uint_as_float(field(f0.mat, 2)), __uint_as_float(field(f0.mat, 3)));
+            if (lane == 0) {
+                gstore(a.draw_slot + d, cached ? slot : 0xFFFFFFFFu);
+                if (cached) {
+                    gstore(a.slot_albedo + slot, albedo);
+                    gstore(a.slot_mat + slot, matw & 0x7FFFFFFFu);
+                }
             }
         }
 
         // vertices: clip = (VP * M) * (pos, 1) (visbuffer.mesh.glsl:44-65)
         if (lane < vc) {
-            const uint32_t vi = a.s.vidx_base[vo + lane];
-            const float* pp = reinterpret_cast<const float*>(reinterpret_cast<const uint8_t*>(a.s.vtx_base) + vb)
-                              + (size_t)vi * 6;
-            const float4 c = mat4_point(mvp, pp[0], pp[1], pp[2]);
+            const float4 c = mat4_point(mvp, f0.x, f0.y, f0.z);
             s_clip[w][lane] = c;
-            s_col[w][lane] = reinterpret_cast<const uint32_t*>(pp)[3];
-            if (tex) s_uv[w][lane] = reinterpret_cast<const uint32_t*>(pp)[5];
+            s_col[w][lane] = f0.col;
+            if (tex) s_uv[w][lane] = f0.uv;
             SVtx sv;
             sv.x = 0; sv.y = 0; sv.z = 0.0f;
             sv.ok = project_vtx(c, hw, hh, sv) ? 1u : 0u;
             s_sv[w][lane] = sv;
         }
         {
-            const uint32_t* src = reinterpret_cast<const uint32_t*>(a.s.tri_base + to);
             const uint32_t ndw = (tc * 3u + 3u) >> 2;
-            for (uint32_t k = lane; k < ndw; k += 64u) s_tri[w][k] = src[k];
+            if (lane < ndw) s_tri[w][lane] = f0.t0;
+            if (lane + 64u < ndw) s_tri[w][lane + 64u] = f0.t1;
         }
+        // the pipeline's loads for the next meshlets
+        fetch(r1, x1, nidx < count, f0);
+        x1 = vidx_of(r2, nidx + nwaves < count);
+        r0 = r1;
+        r1 = r2;
+        r2 = rec_word(nidx + 2u * nwaves);
         wave_sync();
 
-        uint32_t npix[2] = { 0u, 0u };
+        // Stage 1, every triangle slot: the resolve cache, the back-face test
+        // and setup_tri's pixel-range test (a projected triangle whose
+        // bounding box holds no pixel centre rasterizes nothing).  The
+        // triangles left (and those that need clipping) are packed into
+        // s_list (t | i0 << 8 | i1 << 16 | i2 << 24), so that the setup below
+        // runs over them in as few 64-lane rounds as there are: one for a
+        // typical meshlet, whose back faces and pixel-less triangles are most
+        // of its slots.
+        uint32_t nlive = 0;
 #pragma unroll
         for (int sl = 0; sl < 2; ++sl) {
             const uint32_t t = lane + 64u * (uint32_t)sl;
-            bool want_big = false;
-            TriSetup ts;
-            uint32_t id = 0;
-            do {   // per-lane triangle work; `continue` leaves it
-                if (t >= tc) continue;
+            bool live = false;
+            uint32_t packed = 0u;
+            if (t < tc) {
                 const uint32_t i0 = tri_byte(s_tri[w], 3u * t), i1 = tri_byte(s_tri[w], 3u * t + 1u),
                                i2 = tri_byte(s_tri[w], 3u * t + 2u);
                 const float4 c0 = s_clip[w][i0], c1 = s_clip[w][i1], c2 = s_clip[w][i2];
                 if (cached && bary) {
-                    // r[k] = v_{k+1} x v_{k+2} over the clip (x, y, w) of the vertices
-                    TriCache tcache;
-                    const float4 cv[3] = { c0, c1, c2 };
-#pragma unroll
-                    for (int k = 0; k < 3; ++k) {
-                        const float4 p = cv[(k + 1) % 3], q = cv[(k + 2) % 3];
-                        tcache.r[k][0] = p.y * q.w - p.w * q.y;
-                        tcache.r[k][1] = p.w * q.x - p.x * q.w;
-                        tcache.r[k][2] = p.x * q.y - p.y * q.x;
-                    }
-                    tcache.col[0] = s_col[w][i0]; tcache.col[1] = s_col[w][i1]; tcache.col[2] = s_col[w][i2];
-                    // 48 bytes always, the uv words only for textured frames
+                    // TriCache: r[k] = v_{k+1} x v_{k+2} over the clip (x, y, w) of the
+                    // vertices, then the colours; 48 bytes always, the uv words only
+                    // for textured frames (written from registers: a local TriCache
+                    // would be placed in LDS by the compiler)
+                    const float r00 = c1.y * c2.w - c1.w * c2.y, r01 = c1.w * c2.x - c1.x * c2.w,
+                                r02 = c1.x * c2.y - c1.y * c2.x;
+                    const float r10 = c2.y * c0.w - c2.w * c0.y, r11 = c2.w * c0.x - c2.x * c0.w,
+                                r12 = c2.x * c0.y - c2.y * c0.x;
+                    const float r20 = c0.y * c1.w - c0.w * c1.y, r21 = c0.w * c1.x - c0.x * c1.w,
+                                r22 = c0.x * c1.y - c0.y * c1.x;
                     uint4* dst = reinterpret_cast<uint4*>(a.tri_cache + (size_t)slot * 128u + t);
-                    const uint4* srcw = reinterpret_cast<const uint4*>(&tcache);
-                    dst[0] = srcw[0]; dst[1] = srcw[1]; dst[2] = srcw[2];
-                    if (tex) dst[3] = make_uint4(s_uv[w][i0], s_uv[w][i1], s_uv[w][i2], 0u);
+                    gstore(dst, make_uint4(__float_as_uint(r00), __float_as_uint(r01), __float_as_uint(r02),
+                                           __float_as_uint(r10)));
+                    gstore(dst + 1, make_uint4(__float_as_uint(r11), __float_as_uint(r12), __float_as_uint(r20),
+                                               __float_as_uint(r21)));
+                    gstore(dst + 2, make_uint4(__float_as_uint(r22), s_col[w][i0], s_col[w][i1], s_col[w][i2]));
+                    if (tex) gstore(dst + 3, make_uint4(s_uv[w][i0], s_uv[w][i1], s_uv[w][i2], 0u));
                 }
+                static_assert(offsetof(TriCache, col) == 36 && offsetof(TriCache, uv) == 48, "TriCache words");
+                live = true;
                 if (cull_faces) {
                     const float det = det3_xyw(c0, c1, c2);
-                    const bool culled = tdet < 0.0f ? det < 0.0f : det > 0.0f;
-                    if (culled) continue;
+                    live = !(tdet < 0.0f ? det < 0.0f : det > 0.0f);
                 }
+                if (live) {
+                    rc.tri();
+                    const SVtx v0 = s_sv[w][i0], v1 = s_sv[w][i1], v2 = s_sv[w][i2];
+                    if (v0.ok & v1.ok & v2.ok) {
+                        const int minx = min(v0.x, min(v1.x, v2.x)), maxx = max(v0.x, max(v1.x, v2.x));
+                        const int miny = min(v0.y, min(v1.y, v2.y)), maxy = max(v0.y, max(v1.y, v2.y));
+                        live = max((minx - 128 + 255) >> 8, 0) <= min((maxx - 128) >> 8, W - 1)
+                               && max((miny - 128 + 255) >> 8, 0) <= min((maxy - 128) >> 8, H - 1);
+                    }
+                }
+                packed = t | (i0 << 8) | (i1 << 16) | (i2 << 24);
+            }
+            const unsigned long long m = __ballot(live);
+            const uint32_t pos = nlive + __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32),
+                                                                    __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
+            if (live) s_list[w][pos] = packed;
+            nlive += (uint32_t)__popcll(m);
+        }
+        wave_sync();
+
+        // Stage 2, the packed triangles: clip queue, setup, small or big.
+        uint32_t npix[2] = { 0u, 0u };
+#pragma unroll
+        for (int sl = 0; sl < 2; ++sl) {
+            if (64u * (uint32_t)sl >= nlive) break;   // wave-uniform
+            const uint32_t pos = lane + 64u * (uint32_t)sl;
+            bool want_big = false;
+            TriSetup ts;
+            uint32_t id = 0;
+            do {   // per-lane triangle work; `continue` leaves it
+                if (pos >= nlive) continue;
+                const uint32_t e = s_list[w][pos];
+                const uint32_t t = e & 0xFFu, i0 = (e >> 8) & 0xFFu, i1 = (e >> 16) & 0xFFu, i2 = e >> 24;
                 id = (d << VKGV_TRIANGLE_BITS) | t;
-                rc.tri();
                 const SVtx v0 = s_sv[w][i0], v1 = s_sv[w][i1], v2 = s_sv[w][i2];
                 if (!(v0.ok & v1.ok & v2.ok)) {
                     const uint32_t slot = atomicAdd(&a.ctr->clip_count, 1u);
                     if (slot < a.clip_cap) {
-                        ClipTri ct;
-                        ct.c[0] = c0; ct.c[1] = c1; ct.c[2] = c2;
-                        ct.id = id;
-                        ct.pad[0] = ct.pad[1] = ct.pad[2] = 0u;
-                        a.clipq[slot] = ct;
+                        // the record's four 16-byte words, from LDS and registers
+                        float4* q = reinterpret_cast<float4*>(a.clipq + slot);
+                        gstore(q, s_clip[w][i0]);
+                        gstore(q + 1, s_clip[w][i1]);
+                        gstore(q + 2, s_clip[w][i2]);
+                        gstore(reinterpret_cast<uint4*>(q) + 3, make_uint4(id, 0u, 0u, 0u));
                     } else {
                         atomicOr(&a.ctr->overflow, 2u);
                         s_spill[w] = 1u;
@@ -721,7 +859,7 @@ __device__ __forceinline__ void raster_view(const RasterArgs& a)
                     st.dim = (uint16_t)(bw | (bh << 8));
                     st.pad = 0;
                     st.id = id;
-                    s_st[w][t] = st;
+                    s_st[w][pos] = st;
                     npix[sl] = bw * bh;
                 } else {
                     want_big = true;
@@ -762,14 +900,14 @@ __device__ __forceinline__ void raster_view(const RasterArgs& a)
 #pragma unroll 1
             for (uint32_t q = lane; q < njobs; q += 64u)
                 if (j0 + q < (unsigned long long)a.job_cap)
-                    a.jobs[j0 + q] = room ? FragJob{ sidx, kJobKeep + q * kJobSlots } : FragJob{ 0xFFFFFFFFu, 0u };
+                    gstore(a.jobs + j0 + q, room ? FragJob{ sidx, kJobKeep + q * kJobSlots } : FragJob{ 0xFFFFFFFFu, 0u });
             if (room) {
                 uint4* dst = reinterpret_cast<uint4*>(a.stage[sidx].w);
                 const uint4* st4 = reinterpret_cast<const uint4*>(s_st[w]);
 #pragma unroll 2
-                for (uint32_t q = lane; q < kStageTri4; q += 64u) dst[q] = st4[q];
-                if (lane < 32u) dst[kStageTri4 + lane] = reinterpret_cast<const uint4*>(s_pre[w])[lane];
-                if (lane == 0) dst[kStageTri4 + 32u] = make_uint4(total, 0u, 0u, 0u);
+                for (uint32_t q = lane; q < kStageTri4; q += 64u) gstore(dst + q, st4[q]);
+                if (lane < 32u) gstore(dst + kStageTri4 + lane, reinterpret_cast<const uint4*>(s_pre[w])[lane]);
+                if (lane == 0) gstore(dst + kStageTri4 + 32u, make_uint4(total, 0u, 0u, 0u));
                 keep = kJobKeep;
             }
         }
@@ -1366,36 +1504,39 @@ __global__ __launch_bounds__(kBlock, 3) void raster_kernel(RasterBatchArgs ab)
             if (first >= count) continue;
             const uint32_t vi = v0 + j;
             const ViewDev& v = batch_view(ab.b, vi);
+            // the view's fields are the same for the whole wave: held in
+            // scalar registers (the compiler cannot tell from the vector
+            // loads that read them)
             RasterArgs a;
             a.s = ab.s;
-            a.list = v.visible[ab.list];
-            a.keys = v.keys;
-            a.W = v.width;
-            a.H = v.height;
+            a.list = uni(v.visible[ab.list]);
+            a.keys = uni(v.keys);
+            a.W = uni(v.width);
+            a.H = uni(v.height);
             a.flags = ab.flags;
             a.small_lim = ab.small_lim;
-            a.big = v.big;
-            a.big_chunk = v.big_chunk;
-            a.big_cap = v.big_capacity;
-            a.redo = v.redo;
-            a.chunk_tri = v.chunk_tri;
-            a.chunk_cap = v.chunk_capacity;
-            a.clipq = v.clipq;
-            a.clip_cap = v.clip_capacity;
-            a.ctr = v.counters;
-            a.mvp = v.mvp;
+            a.big = uni(v.big);
+            a.big_chunk = uni(v.big_chunk);
+            a.big_cap = uni(v.big_capacity);
+            a.redo = uni(v.redo);
+            a.chunk_tri = uni(v.chunk_tri);
+            a.chunk_cap = uni(v.chunk_capacity);
+            a.clipq = uni(v.clipq);
+            a.clip_cap = uni(v.clip_capacity);
+            a.ctr = uni(v.counters);
+            a.mvp = uni(v.mvp);
             a.count = count;
-            a.first = first;
+            a.first = uni(first);
             a.slot0 = __builtin_amdgcn_readlane(base, j);
-            a.tri_cache = v.tri_cache;
-            a.slot_albedo = v.slot_albedo;
-            a.slot_mat = v.slot_mat;
-            a.draw_slot = v.draw_slot;
-            a.cache_slots = min(v.cache_slots, ab.cache_slots);
-            a.stage = v.frag_stage;
-            a.jobs = v.frag_jobs;
-            a.stage_cap = v.stage_capacity;
-            a.job_cap = v.job_capacity;
+            a.tri_cache = uni(v.tri_cache);
+            a.slot_albedo = uni(v.slot_albedo);
+            a.slot_mat = uni(v.slot_mat);
+            a.draw_slot = uni(v.draw_slot);
+            a.cache_slots = uni(min(v.cache_slots, ab.cache_slots));
+            a.stage = uni(v.frag_stage);
+            a.jobs = uni(v.frag_jobs);
+            a.stage_cap = uni(v.stage_capacity);
+            a.job_cap = uni(v.job_capacity);
             raster_view<STATS>(a);
         }
         carry = (carry + __builtin_amdgcn_readlane(incl, 63)) % nwaves;
