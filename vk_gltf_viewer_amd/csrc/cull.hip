@@ -18,6 +18,7 @@ namespace vkgv {
 namespace {
 
 constexpr uint32_t kBlock = 256;
+constexpr uint32_t kReserveChunks = 4;   // chunks per list reservation
 
 // aabbPositions (culling.h.glsl:32-41).
 __constant__ float kCorner[8][3] = {
@@ -222,6 +223,12 @@ __global__ __launch_bounds__(kBlock) void cull_kernel(CullArgs a)
     // the wave's index, uniform in the compiler's eyes: unit, view group and
     // view loop bounds then live in scalar registers and the loops branch on them
     const uint32_t wave = __builtin_amdgcn_readfirstlane((blockIdx.x * kBlock + threadIdx.x) >> 6);
+    const uint32_t w = threadIdx.x >> 6;   // wave of the block
+    __shared__ float s_dc[kBlock / 64][6][64 * P];              // world AABB centre, extent of the unit's draws
+    __shared__ uint16_t s_pairs[kBlock / 64][64 * P];           // probe pairs: i << 6 | lane
+    __shared__ unsigned long long s_occ[kBlock / 64][P];        // probe results per slot i
+    __shared__ unsigned long long s_bits[kBlock / 64][kReserveChunks][64];   // a super-unit's chunks' bits
+    __shared__ uint32_t s_off[kBlock / 64][kReserveChunks][64];              // and per-view offsets (lane j)
     if (a.mode != CULL_LATE) {
         // this frame's VP M and det(M) of every transform, per view, read by
         // the raster and the resolve (the same arithmetic they used per
@@ -257,130 +264,193 @@ __global__ __launch_bounds__(kBlock) void cull_kernel(CullArgs a)
             c->frag_jobs = 0ull;
         }
     }
-    for (uint32_t u = wave; u < a.units; u += nwaves) {
-        const uint32_t chunk = u / a.groups, group = u - chunk * a.groups;
-        const uint32_t base = chunk * 64u * (uint32_t)P;
-        DrawCull dc[P];
-#pragma unroll
-        for (int i = 0; i < P; ++i) draw_cull_setup<CONE>(a.s, base + 64u * i + lane, dc[i]);
+    // A wave takes K consecutive chunks of one view group at a time (a
+    // super-unit) and reserves their span of each view's list with one
+    // atomicAdd: the views' counters see 1/K of the same-address atomics
+    // (which the memory side serializes) they would with one per chunk.  K
+    // is 1 while the units do not outnumber the waves (every wave keeps
+    // work), up to kReserveChunks when they do.  Phase 1 keeps each chunk's
+    // bits and per-view offsets in LDS.
+    const uint32_t chunks = a.units / a.groups;
+    const uint32_t K = max(1u, min(kReserveChunks, a.units / nwaves));
+    const uint32_t supers = (chunks + K - 1u) / K * a.groups;
+    for (uint32_t su = wave; su < supers; su += nwaves) {
+        const uint32_t sc = su / a.groups, group = su - sc * a.groups;
         const uint32_t v0 = group * a.group_views, nv = min(a.group_views, a.b.nviews - v0);
-        unsigned long long bits = 0ull;   // bit j * P + i: draw i of this lane visible in view v0 + j
-        uint32_t reserved = 0;            // lane j: first list position of the unit in view v0 + j
-        // the unit's history words of every view of the group in one load: lane
-        // j * P + i holds view v0 + j's word of draws base + 64 i .. + 63 (read
-        // before the late pass below rewrites them)
-        unsigned long long hword = 0ull;
-        if (!single) {
-            const uint32_t jl = lane / (uint32_t)P, il = lane % (uint32_t)P;
-            if (jl < nv && base + 64u * il < a.s.draw_count)
-                hword = reinterpret_cast<const unsigned long long*>(batch_view(a.b, v0 + jl).history)[(base >> 6) + il];
-        }
-        // ---- phase 1: visibility of every (view, draw), one reservation per view
-        for (uint32_t j = 0; j < nv; ++j) {
-            const ViewDev& view = batch_view(a.b, v0 + j);
-            const CameraArg& cam = batch_camera(a.b, v0 + j);
-            unsigned long long* history = reinterpret_cast<unsigned long long*>(view.history);
-            bool pass[P], vis[P];
+        const uint32_t kn = min(K, chunks - sc * K);
+        uint32_t run = 0;   // lane j: positions of the super-unit's chunks so far in view v0 + j
+        for (uint32_t kc = 0; kc < kn; ++kc) {
+            const uint32_t chunk = sc * K + kc;
+            const uint32_t base = chunk * 64u * (uint32_t)P;
+            DrawCull dc[P];
 #pragma unroll
-            for (int i = 0; i < P; ++i) {
-                // (readlane returns a signed int: widen each half as uint32_t, or a
-                // set bit 31 of the low half would sign-extend into the high half)
-                const uint32_t src = j * (uint32_t)P + (uint32_t)i;
-                const uint32_t hi = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)(hword >> 32), src);
-                const uint32_t lo = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)hword, src);
-                const unsigned long long hw = ((unsigned long long)hi << 32) | (unsigned long long)lo;
-                const bool old = dc[i].valid && ((hw >> lane) & 1ull);
-                // the early pass tests only what the history lists
-                pass[i] = dc[i].valid && (a.mode != CULL_EARLY || old);
-                if (pass[i] && (a.flags & VKGV_CULL_FRUSTUM)) pass[i] = aabb_in_frustum(dc[i].wc, dc[i].we, cam.cam);
-                if (CONE && pass[i] && dc[i].cone)
-                    pass[i] = !cone_culled_from(dc[i].apex, dc[i].axis, dc[i].cutoff, cam.eye);
-                vis[i] = single ? pass[i] : (a.mode == CULL_EARLY ? pass[i] : old);   // late: old = drawn early
-            }
-            if (probe) {
-                // only the (i) with a lane past the frustum test probe (a uniform branch)
-                bool anyp[P], any = false;
+            for (int i = 0; i < P; ++i) draw_cull_setup<CONE>(a.s, base + 64u * i + lane, dc[i]);
+            if (probe) {   // the world AABBs for the packed probes (the previous unit's reads are done)
 #pragma unroll
-                for (int i = 0; i < P; ++i) {
-                    anyp[i] = __ballot(pass[i]) != 0ull;
-                    any = any || anyp[i];
-                }
-                ProbeAt pr[P] = {};
-                float depth[P] = {};
-                if (any) {
-                    // pyramid level geometry in lanes: lane k holds level k & 15
-                    const uint32_t lk = lane & 15u;
-                    const int pw = (int)view.pyr.w[lk], ph = (int)view.pyr.h[lk];
-                    const uint32_t po = view.pyr.off[lk];
-                    const uint32_t last = view.pyr.levels - 1u;
-                    const float w0 = (float)view.pyr.w[0], h0 = (float)view.pyr.h[0];
-                    const float* vpm = single ? cam.cam.prev_occlusion_view_projection : cam.cam.occlusion_view_projection;
-                    Mat4 vp;
+                for (int i = 0; i < P; ++i)
 #pragma unroll
-                    for (int q = 0; q < 16; ++q) vp.m[q] = vpm[q];
-#pragma unroll
-                    for (int i = 0; i < P; ++i)
-                        if (anyp[i]) pr[i] = probe_setup(dc[i].wc, dc[i].we, vp, w0, h0, pw, ph, po, last);
-#pragma unroll
-                    for (int i = 0; i < P; ++i)
-                        if (anyp[i]) depth[i] = probe_fetch(view.pyramid, pr[i]);
-                }
-#pragma unroll
-                for (int i = 0; i < P; ++i) {
-                    const bool occ = anyp[i] && pass[i] && depth[i] < pr[i].maxz;
-                    if (single) {
-                        vis[i] = occ;
-                    } else {
-                        const bool old = vis[i];
-                        vis[i] = occ && !(pass[i] && old);
-                        // the history of the next frame: visible against this frame's pyramid
-                        const unsigned long long hb = __ballot(occ);
-                        if (lane == 0 && base + 64u * i < a.s.draw_count) history[(base >> 6) + i] = hb;
+                    for (int q = 0; q < 3; ++q) {
+                        s_dc[w][q][64 * i + lane] = dc[i].wc[q];
+                        s_dc[w][3 + q][64 * i + lane] = dc[i].we[q];
                     }
+            }
+            unsigned long long bits = 0ull;   // bit j * P + i: draw i of this lane visible in view v0 + j
+            uint32_t off = 0;                 // lane j: the chunk's first position in view v0 + j, from the reservation
+            // the unit's history words of every view of the group in one load: lane
+            // j * P + i holds view v0 + j's word of draws base + 64 i .. + 63 (read
+            // before the late pass below rewrites them)
+            unsigned long long hword = 0ull;
+            if (!single) {
+                const uint32_t jl = lane / (uint32_t)P, il = lane % (uint32_t)P;
+                if (jl < nv && base + 64u * il < a.s.draw_count)
+                    hword = reinterpret_cast<const unsigned long long*>(batch_view(a.b, v0 + jl).history)
+                        [(base >> 6) + il];
+            }
+            // ---- phase 1: visibility of every (view, draw), one reservation per view
+            for (uint32_t j = 0; j < nv; ++j) {
+                const ViewDev& view = batch_view(a.b, v0 + j);
+                const CameraArg& cam = batch_camera(a.b, v0 + j);
+                unsigned long long* history = reinterpret_cast<unsigned long long*>(view.history);
+                bool pass[P], vis[P];
+#pragma unroll
+                for (int i = 0; i < P; ++i) {
+                    // (readlane returns a signed int: widen each half as uint32_t, or a
+                    // set bit 31 of the low half would sign-extend into the high half)
+                    const uint32_t src = j * (uint32_t)P + (uint32_t)i;
+                    const uint32_t hi = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)(hword >> 32), src);
+                    const uint32_t lo = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)hword, src);
+                    const unsigned long long hw = ((unsigned long long)hi << 32) | (unsigned long long)lo;
+                    const bool old = dc[i].valid && ((hw >> lane) & 1ull);
+                    // the early pass tests only what the history lists
+                    pass[i] = dc[i].valid && (a.mode != CULL_EARLY || old);
+                    if (pass[i] && (a.flags & VKGV_CULL_FRUSTUM))
+                        pass[i] = aabb_in_frustum(dc[i].wc, dc[i].we, cam.cam);
+                    if (CONE && pass[i] && dc[i].cone)
+                        pass[i] = !cone_culled_from(dc[i].apex, dc[i].axis, dc[i].cutoff, cam.eye);
+                    vis[i] = single ? pass[i] : (a.mode == CULL_EARLY ? pass[i] : old);   // late: old = drawn early
+                }
+                if (probe) {
+                    // The probes of the (i, lane) pairs past the frustum test, packed
+                    // into rounds of 64, one pair per lane: in most views most of a
+                    // unit's draws fail the test, so the projected-AABB arithmetic
+                    // runs once per pair that needs it instead of once per lane of
+                    // every slot i with a pair.  The draws' world AABBs come from
+                    // LDS, the results go back through a bitmask per slot.
+                    uint32_t npass = 0;
+#pragma unroll
+                    for (int i = 0; i < P; ++i) {
+                        const unsigned long long m = __ballot(pass[i]);
+                        const uint32_t pos =
+                            npass + __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
+                        if (pass[i]) s_pairs[w][pos] = (uint16_t)(((uint32_t)i << 6) | lane);
+                        npass += (uint32_t)__popcll(m);
+                    }
+                    if (lane < (uint32_t)P) s_occ[w][lane] = 0ull;
+                    wave_sync();
+                    if (npass) {   // wave-uniform
+                        // pyramid level geometry in lanes: lane k holds level k & 15
+                        const uint32_t lk = lane & 15u;
+                        const int pw = (int)view.pyr.w[lk], ph = (int)view.pyr.h[lk];
+                        const uint32_t po = view.pyr.off[lk];
+                        const uint32_t last = view.pyr.levels - 1u;
+                        const float w0 = (float)view.pyr.w[0], h0 = (float)view.pyr.h[0];
+                        const float* vpm =
+                            single ? cam.cam.prev_occlusion_view_projection : cam.cam.occlusion_view_projection;
+                        Mat4 vp;
+#pragma unroll
+                        for (int q = 0; q < 16; ++q) vp.m[q] = vpm[q];
+                        for (uint32_t r = 0; r < npass; r += 64u) {
+                            const bool live = r + lane < npass;
+                            const uint32_t e = live ? (uint32_t)s_pairs[w][r + lane] : 0u;
+                            const uint32_t k = (e >> 6) * 64u + (e & 63u);   // the draw's slot in the unit
+                            // finite inputs for the lanes without a pair (every lane computes)
+                            float c[3] = { 0.0f, 0.0f, 0.0f }, ex[3] = { 0.0f, 0.0f, 0.0f };
+                            if (live) {
+#pragma unroll
+                                for (int q = 0; q < 3; ++q) {
+                                    c[q] = s_dc[w][q][k];
+                                    ex[q] = s_dc[w][3 + q][k];
+                                }
+                            }
+                            const ProbeAt pr = probe_setup(c, ex, vp, w0, h0, pw, ph, po, last);
+                            const float depth = probe_fetch(view.pyramid, pr);
+                            if (live && depth < pr.maxz) atomicOr(&s_occ[w][e >> 6], 1ull << (e & 63u));
+                        }
+                        wave_sync();
+                    }
+#pragma unroll
+                    for (int i = 0; i < P; ++i) {
+                        const bool occ = pass[i] && ((s_occ[w][i] >> lane) & 1ull);
+                        if (single) {
+                            vis[i] = occ;
+                        } else {
+                            const bool old = vis[i];
+                            vis[i] = occ && !(pass[i] && old);
+                            // the history of the next frame: visible against this frame's pyramid
+                            const unsigned long long hb = __ballot(occ);
+                            if (lane == 0 && base + 64u * i < a.s.draw_count) history[(base >> 6) + i] = hb;
+                        }
+                    }
+                    wave_sync();   // s_pairs and s_occ are rewritten for the next view
+                }
+                uint32_t total = 0;
+#pragma unroll
+                for (int i = 0; i < P; ++i) {
+                    total += (uint32_t)__popcll(__ballot(vis[i]));
+                    if (vis[i]) bits |= 1ull << (j * (uint32_t)P + (uint32_t)i);
+                }
+                if (lane == j) {
+                    off = run;
+                    run += total;
                 }
             }
-            uint32_t total = 0;
+            s_bits[w][kc][lane] = bits;
+            s_off[w][kc][lane] = off;
+        }
+        // lane j: the super-unit's first list position in view v0 + j
+        uint32_t reserved = 0;
+        if (lane < nv && run) reserved = atomicAdd(&batch_view(a.b, v0 + lane).counters->vis_count[a.list], run);
+        wave_sync();
+        for (uint32_t kc = 0; kc < kn; ++kc) {
+            const unsigned long long bits = s_bits[w][kc][lane];
+            if (__ballot(bits != 0ull) == 0ull) continue;
+            const uint32_t base = (sc * K + kc) * 64u * (uint32_t)P;
+            const uint32_t first = reserved + s_off[w][kc][lane];   // lane j: the chunk's position in view v0 + j
+            // ---- phase 2: the raster records of the visible draws.  A draw's
+            // record is the same for every view: each lane loads the records of its
+            // draws visible in any view once, all P loads in flight together, then
+            // stores them into every view's list that holds them.
+            uint4 rec[P][2];
 #pragma unroll
             for (int i = 0; i < P; ++i) {
-                total += (uint32_t)__popcll(__ballot(vis[i]));
-                if (vis[i]) bits |= 1ull << (j * (uint32_t)P + (uint32_t)i);
-            }
-            if (total && lane == j) reserved = atomicAdd(&view.counters->vis_count[a.list], total);
-        }
-        if (__ballot(bits != 0ull) == 0ull) continue;
-        // ---- phase 2: the raster records of the visible draws.  A draw's
-        // record is the same for every view: each lane loads the records of its
-        // draws visible in any view once, all P loads in flight together, then
-        // stores them into every view's list that holds them.
-        uint4 rec[P][2];
-#pragma unroll
-        for (int i = 0; i < P; ++i) {
-            unsigned long long in_any = 0ull;
-            for (uint32_t j = 0; j < nv; ++j) in_any |= (bits >> (j * (uint32_t)P + (uint32_t)i)) & 1ull;
-            rec[i][0] = rec[i][1] = make_uint4(0u, 0u, 0u, 0u);
-            if (in_any) {
-                const uint4* src = reinterpret_cast<const uint4*>(a.s.rrec + base + 64u * i + lane);
-                rec[i][0] = src[0];
-                rec[i][1] = src[1];
-            }
-        }
-        for (uint32_t j = 0; j < nv; ++j) {
-            uint32_t out = __builtin_amdgcn_readlane(reserved, j);
-            RasterRec* dst = batch_view(a.b, v0 + j).visible[a.list];
-#pragma unroll
-            for (int i = 0; i < P; ++i) {
-                const bool mine = (bits >> (j * (uint32_t)P + (uint32_t)i)) & 1ull;
-                const unsigned long long m = __ballot(mine);
-                if (mine) {
-                    const uint32_t rank =
-                        __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
-                    uint4* o = reinterpret_cast<uint4*>(dst + out + rank);
-                    o[0] = rec[i][0];
-                    o[1] = rec[i][1];
+                unsigned long long in_any = 0ull;
+                for (uint32_t j = 0; j < nv; ++j) in_any |= (bits >> (j * (uint32_t)P + (uint32_t)i)) & 1ull;
+                rec[i][0] = rec[i][1] = make_uint4(0u, 0u, 0u, 0u);
+                if (in_any) {
+                    const uint4* src = reinterpret_cast<const uint4*>(a.s.rrec + base + 64u * i + lane);
+                    rec[i][0] = src[0];
+                    rec[i][1] = src[1];
                 }
-                out += (uint32_t)__popcll(m);
+            }
+            for (uint32_t j = 0; j < nv; ++j) {
+                uint32_t out = __builtin_amdgcn_readlane(first, j);
+                RasterRec* dst = batch_view(a.b, v0 + j).visible[a.list];
+#pragma unroll
+                for (int i = 0; i < P; ++i) {
+                    const bool mine = (bits >> (j * (uint32_t)P + (uint32_t)i)) & 1ull;
+                    const unsigned long long m = __ballot(mine);
+                    if (mine) {
+                        const uint32_t rank =
+                            __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
+                        uint4* o = reinterpret_cast<uint4*>(dst + out + rank);
+                        o[0] = rec[i][0];
+                        o[1] = rec[i][1];
+                    }
+                    out += (uint32_t)__popcll(m);
+                }
             }
         }
+        wave_sync();   // s_bits and s_off are rewritten by the next super-unit
     }
 }
 

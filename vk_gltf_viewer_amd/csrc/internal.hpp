@@ -303,6 +303,14 @@ __device__ __forceinline__ const ViewDev& batch_view(const BatchDev& b, uint32_t
 }
 
 // Host-side sizes of a batch's launches.
+// Orders a wave's LDS writes before its later LDS reads by other lanes.
+__device__ __forceinline__ void wave_sync()
+{
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
 constexpr uint32_t kMvpStride = 20;          // floats per transform in a view's VP M table (80-byte rows)
 constexpr uint32_t kHizMaxLevels = 16;
 constexpr uint32_t kHizTailTexels = 8192;   // largest first tail level (the tail keeps it in LDS)
