@@ -583,6 +583,10 @@ def frame_workload(config, variant, steps, warmup, log, with_cpu, extra=True, tu
                "ms_frame": ms_frame, "events_ms_total": round(st.ms_total, 4),
                "passes_ms": {k: round(v, 4) for k, v in ms.items()}, "passes_alg_gbs": gbs,
                "passes_traffic": {k: measured_traffic(k, W, H) for k in ms},
+               # measured HBM bytes over algorithmic bytes: above 1, bytes moved that the
+               # pass's algorithm does not need (Hi-Z: 8-byte keys read for their 4-byte depth)
+               "passes_traffic_ratio": {k: (round(measured_traffic(k, W, H) / alg[k], 3)
+                                            if measured_traffic(k, W, H) and alg[k] else None) for k in ms},
                "roofline": roofline, "raster": raster_object(r, st, counts, ms_frame, log), "sc": sc, "cam": cam}
         if extra:
             out["cold"] = cold_frames(view, cam, flags, max(3, steps // 4))
@@ -681,6 +685,7 @@ def main():
                    "visible_early": scn["visible_early"], "visible_late": scn["visible_late"],
                    "parallelism": "single GPU"},
         "passes_ms": f["passes_ms"], "passes_alg_gbs": f["passes_alg_gbs"], "passes_traffic": f["passes_traffic"],
+        "passes_traffic_ratio": f["passes_traffic_ratio"],
         "gpu_frame_ms_events": f["events_ms_total"],
         "rasterized_triangles_per_s": f["raster"]["rasterized_triangles_per_s_frame"],
         "roofline": f["roofline"],
